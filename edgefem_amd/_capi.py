@@ -1,0 +1,337 @@
+"""ctypes binding of ``include/edgefem_amd.h`` plus torch plumbing for device memory.
+
+PyTorch only supplies device allocations and the current HIP stream; every computation
+runs in the HIP library.  Loading fails loudly when the library is missing: there is no
+CPU fallback anywhere in the package.
+"""
+
+from __future__ import annotations
+
+import ctypes as C
+import os
+from typing import List, Optional, Sequence
+
+import numpy as np
+
+_HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(_HERE, "_lib", "libedgefem_amd.so")
+
+EFM_C128 = 0
+EFM_C64 = 1
+EFM_METHOD_COCG = 0
+EFM_METHOD_BICGSTAB = 1
+
+# every symbol include/edgefem_amd.h declares
+EXPORTS = [
+    "efm_last_error", "efm_version",
+    "efm_topology_create", "efm_topology_destroy", "efm_topology_get_sizes", "efm_topology_export",
+    "efm_assemble_km", "efm_assemble_maxwell", "efm_port_surface_mass", "efm_combine",
+    "efm_solver_create", "efm_solver_destroy", "efm_solver_set_matrix", "efm_solver_solve",
+    "efm_solver_spmv", "efm_solver_profile", "efm_solver_profile_read",
+    "efm_port_overlap",
+]
+
+
+class EdgeFemError(RuntimeError):
+    pass
+
+
+class TopologySizes(C.Structure):
+    _fields_ = [("n_nodes", C.c_int64), ("n_tets", C.c_int64), ("n_tris", C.c_int64),
+                ("n_edges", C.c_int64), ("nnz", C.c_int64)]
+
+
+class SolveOpts(C.Structure):
+    _fields_ = [("method", C.c_int), ("tolerance", C.c_double), ("max_iterations", C.c_int),
+                ("check_every", C.c_int)]
+
+
+class SolveResultC(C.Structure):
+    _fields_ = [("iterations", C.c_int), ("converged", C.c_int), ("max_rel_residual", C.c_double)]
+
+
+_lib = None
+
+
+def load_library(path: str = LIB_PATH) -> C.CDLL:
+    """Load the HIP library; raises if it has not been built."""
+    global _lib
+    if _lib is not None:
+        return _lib
+    if not os.path.exists(path):
+        raise EdgeFemError(
+            f"HIP library not found at {path}; build it with `python -m edgefem_amd.build` "
+            "(there is no CPU fallback)")
+    lib = C.CDLL(path)
+    vp, i64, i32, dbl = C.c_void_p, C.c_int64, C.c_int, C.c_double
+    lib.efm_last_error.restype = C.c_char_p
+    lib.efm_version.restype = i32
+    lib.efm_topology_create.argtypes = [i64, vp, i64, vp, i64, vp, vp, C.POINTER(vp)]
+    lib.efm_topology_destroy.argtypes = [vp]
+    lib.efm_topology_destroy.restype = None
+    lib.efm_topology_get_sizes.argtypes = [vp, C.POINTER(TopologySizes)]
+    lib.efm_topology_export.argtypes = [vp] + [vp] * 7 + [vp]
+    lib.efm_assemble_km.argtypes = [vp, vp, vp, vp, vp, vp, vp, vp]
+    lib.efm_assemble_maxwell.argtypes = [vp, vp, vp, vp, vp, vp, dbl, vp, vp]
+    lib.efm_port_surface_mass.argtypes = [vp, vp, i64, vp, vp, vp, vp]
+    lib.efm_combine.argtypes = [vp, vp, vp, dbl, i32, vp, vp, i32, vp, vp]
+    lib.efm_solver_create.argtypes = [i64, i64, vp, vp, i32, i32, vp, C.POINTER(vp)]
+    lib.efm_solver_destroy.argtypes = [vp]
+    lib.efm_solver_destroy.restype = None
+    lib.efm_solver_set_matrix.argtypes = [vp, vp, vp]
+    lib.efm_solver_solve.argtypes = [vp, vp, vp, C.POINTER(SolveOpts), C.POINTER(SolveResultC),
+                                     vp, vp, vp]
+    lib.efm_solver_spmv.argtypes = [vp, vp, vp, vp]
+    lib.efm_solver_profile.argtypes = [vp, i32]
+    lib.efm_solver_profile_read.argtypes = [vp, C.POINTER(C.c_double), C.POINTER(C.c_int64)]
+    lib.efm_port_overlap.argtypes = [i64, i32, i64, vp, vp, vp, i32, vp, vp]
+    for name in EXPORTS:
+        getattr(lib, name)  # AttributeError if the library lacks a declared symbol
+    _lib = lib
+    return lib
+
+
+def _check(rc: int, what: str) -> None:
+    if rc != 0:
+        msg = _lib.efm_last_error().decode(errors="replace") if _lib is not None else ""
+        raise EdgeFemError(f"{what} failed (status {rc}): {msg}")
+
+
+# ---------------------------------------------------------------------------
+# torch plumbing
+# ---------------------------------------------------------------------------
+def _torch():
+    import torch
+    return torch
+
+
+def device() -> "torch.device":
+    torch = _torch()
+    if not torch.cuda.is_available():
+        raise EdgeFemError("no HIP device visible: the EdgeFEM MI355X path runs only on the GPU")
+    return torch.device("cuda", torch.cuda.current_device())
+
+
+def stream_ptr() -> int:
+    torch = _torch()
+    return torch.cuda.current_stream().cuda_stream
+
+
+def dptr(t) -> Optional[int]:
+    if t is None:
+        return None
+    if not t.is_cuda:
+        raise EdgeFemError("expected a device tensor")
+    if not t.is_contiguous():
+        raise EdgeFemError("expected a contiguous tensor")
+    return t.data_ptr()
+
+
+def to_dev(a, dtype=None):
+    """numpy / torch -> contiguous device tensor."""
+    torch = _torch()
+    if isinstance(a, torch.Tensor):
+        t = a
+    else:
+        t = torch.from_numpy(np.ascontiguousarray(a))
+    if dtype is not None:
+        t = t.to(dtype)
+    return t.to(device()).contiguous()
+
+
+# ---------------------------------------------------------------------------
+# Topology
+# ---------------------------------------------------------------------------
+class Topology:
+    """Device-resident edges, signs, CSR pattern and gather maps of one mesh."""
+
+    def __init__(self, node_ids, tet_nodes, tri_nodes):
+        torch = _torch()
+        lib = load_library()
+        self._lib = lib
+        ids = to_dev(node_ids, torch.int64)
+        tets = to_dev(np.asarray(tet_nodes).reshape(-1, 4) if not isinstance(tet_nodes, torch.Tensor) else tet_nodes, torch.int32)
+        tris = to_dev(np.asarray(tri_nodes).reshape(-1, 3) if not isinstance(tri_nodes, torch.Tensor) else tri_nodes, torch.int32)
+        h = C.c_void_p()
+        _check(lib.efm_topology_create(ids.numel(), dptr(ids), tets.numel() // 4, dptr(tets) if tets.numel() else None,
+                                       tris.numel() // 3, dptr(tris) if tris.numel() else None,
+                                       stream_ptr(), C.byref(h)), "efm_topology_create")
+        self._h = h
+        sz = TopologySizes()
+        _check(lib.efm_topology_get_sizes(h, C.byref(sz)), "efm_topology_get_sizes")
+        self.n_nodes, self.n_tets, self.n_tris = sz.n_nodes, sz.n_tets, sz.n_tris
+        self.n_edges, self.nnz = sz.n_edges, sz.nnz
+        dev = device()
+        self.edges = torch.empty((self.n_edges, 2), dtype=torch.int64, device=dev)
+        self.tet_edges = torch.empty((self.n_tets, 6), dtype=torch.int32, device=dev)
+        self.tet_sign = torch.empty((self.n_tets, 6), dtype=torch.int8, device=dev)
+        self.tri_edges = torch.empty((self.n_tris, 3), dtype=torch.int32, device=dev)
+        self.tri_sign = torch.empty((self.n_tris, 3), dtype=torch.int8, device=dev)
+        self.row_ptr = torch.empty(self.n_edges + 1, dtype=torch.int32, device=dev)
+        self.col_idx = torch.empty(self.nnz, dtype=torch.int32, device=dev)
+        _check(lib.efm_topology_export(h, dptr(self.edges), dptr(self.tet_edges), dptr(self.tet_sign),
+                                       dptr(self.tri_edges) if self.n_tris else None,
+                                       dptr(self.tri_sign) if self.n_tris else None,
+                                       dptr(self.row_ptr), dptr(self.col_idx), stream_ptr()),
+               "efm_topology_export")
+
+    @property
+    def handle(self):
+        return self._h
+
+    def close(self):
+        if getattr(self, "_h", None):
+            self._lib.efm_topology_destroy(self._h)
+            self._h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+
+def assemble_km(topo: Topology, xyz, eps_tet, inv_mu_tet=None, dirichlet=None):
+    torch = _torch()
+    lib = load_library()
+    x = to_dev(xyz, torch.float64)
+    eps = to_dev(eps_tet, torch.complex128)
+    imu = None if inv_mu_tet is None else to_dev(inv_mu_tet, torch.complex128)
+    dm = None if dirichlet is None else to_dev(dirichlet, torch.uint8)
+    K = torch.empty(topo.nnz, dtype=torch.complex128, device=x.device)
+    M = torch.empty(topo.nnz, dtype=torch.complex128, device=x.device)
+    _check(lib.efm_assemble_km(topo.handle, dptr(x), dptr(eps), dptr(imu), dptr(dm), dptr(K), dptr(M),
+                               stream_ptr()), "efm_assemble_km")
+    return K, M
+
+
+def assemble_maxwell(topo: Topology, xyz, eps_tet, inv_mu_tet, dirichlet, abc_mask, omega):
+    torch = _torch()
+    lib = load_library()
+    x = to_dev(xyz, torch.float64)
+    eps = to_dev(eps_tet, torch.complex128)
+    imu = None if inv_mu_tet is None else to_dev(inv_mu_tet, torch.complex128)
+    dm = None if dirichlet is None else to_dev(dirichlet, torch.uint8)
+    am = None if abc_mask is None else to_dev(abc_mask, torch.uint8)
+    A = torch.empty(topo.nnz, dtype=torch.complex128, device=x.device)
+    _check(lib.efm_assemble_maxwell(topo.handle, dptr(x), dptr(eps), dptr(imu), dptr(dm), dptr(am),
+                                    float(omega), dptr(A), stream_ptr()), "efm_assemble_maxwell")
+    return A
+
+
+def port_surface_mass(topo: Topology, xyz, tri_sel, dirichlet=None):
+    torch = _torch()
+    lib = load_library()
+    x = to_dev(xyz, torch.float64)
+    sel = to_dev(np.asarray(tri_sel, np.int32) if not isinstance(tri_sel, torch.Tensor) else tri_sel, torch.int32)
+    dm = None if dirichlet is None else to_dev(dirichlet, torch.uint8)
+    Ms = torch.empty(topo.nnz, dtype=torch.float64, device=x.device)
+    _check(lib.efm_port_surface_mass(topo.handle, dptr(x), sel.numel(), dptr(sel) if sel.numel() else None,
+                                     dptr(dm), dptr(Ms), stream_ptr()), "efm_port_surface_mass")
+    return Ms
+
+
+def combine(topo: Topology, K, M, k0sq: float, Ms_list: Sequence = (), coefs: Sequence[complex] = (),
+            out_dtype=None):
+    torch = _torch()
+    lib = load_library()
+    out_dtype = out_dtype or torch.complex128
+    out_type = EFM_C128 if out_dtype == torch.complex128 else EFM_C64
+    A = torch.empty(topo.nnz, dtype=out_dtype, device=K.device)
+    npo = len(Ms_list)
+    ptrs = (C.c_void_p * max(npo, 1))(*[dptr(m) for m in Ms_list])
+    co = np.zeros(2 * max(npo, 1), np.float64)
+    for i, c in enumerate(coefs):
+        co[2 * i] = complex(c).real
+        co[2 * i + 1] = complex(c).imag
+    _check(lib.efm_combine(topo.handle, dptr(K), dptr(M), float(k0sq), npo, ptrs,
+                           co.ctypes.data_as(C.c_void_p), out_type, dptr(A), stream_ptr()), "efm_combine")
+    return A
+
+
+# ---------------------------------------------------------------------------
+# Solver
+# ---------------------------------------------------------------------------
+class KrylovSolver:
+    """Jacobi-preconditioned COCG / BiCGSTAB on a fixed CSR pattern, nrhs systems."""
+
+    def __init__(self, row_ptr, col_idx, nrhs: int = 1, dtype=None):
+        torch = _torch()
+        lib = load_library()
+        self._lib = lib
+        self.row_ptr = to_dev(row_ptr, torch.int32)
+        self.col_idx = to_dev(col_idx, torch.int32)
+        self.n = self.row_ptr.numel() - 1
+        self.nnz = self.col_idx.numel()
+        self.nrhs = int(nrhs)
+        self.dtype = dtype or torch.complex128
+        vt = EFM_C128 if self.dtype == torch.complex128 else EFM_C64
+        h = C.c_void_p()
+        _check(lib.efm_solver_create(self.n, self.nnz, dptr(self.row_ptr), dptr(self.col_idx), self.nrhs, vt,
+                                     stream_ptr(), C.byref(h)), "efm_solver_create")
+        self._h = h
+
+    def set_matrix(self, values):
+        torch = _torch()
+        v = to_dev(values, self.dtype)
+        if v.numel() != self.nnz:
+            raise EdgeFemError(f"matrix has {v.numel()} values, pattern has {self.nnz}")
+        _check(self._lib.efm_solver_set_matrix(self._h, dptr(v), stream_ptr()), "efm_solver_set_matrix")
+
+    def solve(self, B, tol=1e-10, max_iterations=10000, method=EFM_METHOD_COCG, check_every=32, out=None):
+        """B: (n, nrhs) or (n,) complex128 device tensor.  Returns (X, info)."""
+        torch = _torch()
+        Bd = to_dev(B, torch.complex128)
+        if Bd.numel() != self.n * self.nrhs:
+            raise EdgeFemError("right-hand side has the wrong size")
+        X = out if out is not None else torch.empty_like(Bd)
+        opts = SolveOpts(method, float(tol), int(max_iterations), int(check_every))
+        res = SolveResultC()
+        iters = np.zeros(self.nrhs, np.int32)
+        resid = np.zeros(self.nrhs, np.float64)
+        _check(self._lib.efm_solver_solve(self._h, dptr(Bd), dptr(X), C.byref(opts), C.byref(res),
+                                          iters.ctypes.data_as(C.c_void_p), resid.ctypes.data_as(C.c_void_p),
+                                          stream_ptr()), "efm_solver_solve")
+        info = dict(iterations=res.iterations, converged=bool(res.converged),
+                    max_rel_residual=res.max_rel_residual, iters=iters, resid=resid)
+        return X, info
+
+    def spmv(self, x):
+        torch = _torch()
+        xd = to_dev(x, torch.complex128)
+        y = torch.empty_like(xd)
+        _check(self._lib.efm_solver_spmv(self._h, dptr(xd), dptr(y), stream_ptr()), "efm_solver_spmv")
+        return y
+
+    def profile(self, enable=True):
+        _check(self._lib.efm_solver_profile(self._h, 1 if enable else 0), "efm_solver_profile")
+
+    def profile_read(self):
+        ms = C.c_double()
+        n = C.c_int64()
+        _check(self._lib.efm_solver_profile_read(self._h, C.byref(ms), C.byref(n)), "efm_solver_profile_read")
+        return ms.value, n.value
+
+    def close(self):
+        if getattr(self, "_h", None):
+            self._lib.efm_solver_destroy(self._h)
+            self._h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+
+def port_overlap(gidx, gval, X, nrhs: int):
+    """out[j, a] = g_j^H x_a for j over ports (rows of gidx/gval)."""
+    torch = _torch()
+    lib = load_library()
+    gi = to_dev(gidx, torch.int32)
+    gv = to_dev(gval, torch.complex128)
+    P, L = gi.shape
+    out = np.zeros(P * nrhs, np.complex128)
+    _check(lib.efm_port_overlap(X.numel() // nrhs, P, L, dptr(gi), dptr(gv), dptr(X), int(nrhs),
+                                out.ctypes.data_as(C.c_void_p), stream_ptr()), "efm_port_overlap")
+    return out.reshape(P, nrhs)

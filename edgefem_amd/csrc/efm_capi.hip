@@ -1,0 +1,67 @@
+// C ABI housekeeping (error string, version) and the port-overlap S extraction.
+#include "efm_internal.h"
+
+#include <string>
+
+namespace efm {
+static thread_local std::string g_last_error;
+void set_error(const std::string &msg) { g_last_error = msg; }
+} // namespace efm
+
+using namespace efm;
+
+extern "C" const char *efm_last_error(void) { return efm::g_last_error.c_str(); }
+
+extern "C" int efm_version(void) { return 1; }
+
+namespace {
+
+constexpr int kBlock = 256;
+
+// out[j * nrhs + a] = sum_k conj(g[j,k]) * X[idx[j,k] * nrhs + a]; one block per (j, a),
+// fixed-order reduction.  Negative indices are padding.
+__global__ void __launch_bounds__(kBlock)
+    k_port_overlap(int64_t n, int64_t len, const int32_t *__restrict__ gidx,
+                   const double2 *__restrict__ gval, const double2 *__restrict__ X, int nrhs,
+                   double2 *__restrict__ out) {
+  const int j = blockIdx.x / nrhs, a = blockIdx.x % nrhs;
+  double2 acc = make_double2(0, 0);
+  for (int64_t k = threadIdx.x; k < len; k += kBlock) {
+    int32_t e = gidx[(int64_t)j * len + k];
+    if (e < 0 || (int64_t)e >= n)
+      continue;
+    double2 g = gval[(int64_t)j * len + k];
+    acc = cadd(acc, cmul(cconj(g), X[(int64_t)e * nrhs + a]));
+  }
+  __shared__ double2 lds[kBlock];
+  lds[threadIdx.x] = acc;
+  __syncthreads();
+  for (int off = kBlock / 2; off > 0; off >>= 1) {
+    if (threadIdx.x < off)
+      lds[threadIdx.x] = cadd(lds[threadIdx.x], lds[threadIdx.x + off]);
+    __syncthreads();
+  }
+  if (threadIdx.x == 0)
+    out[blockIdx.x] = lds[0];
+}
+
+} // namespace
+
+extern "C" int efm_port_overlap(int64_t n, int n_ports, int64_t len, const int32_t *gidx,
+                                const double *gval, const double *X, int nrhs, double *h_out,
+                                void *stream) {
+  EFM_REQUIRE(gidx && gval && X && h_out, "efm_port_overlap: NULL argument");
+  EFM_REQUIRE(n > 0 && n_ports > 0 && len > 0 && nrhs > 0, "efm_port_overlap: bad sizes");
+  hipStream_t st = (hipStream_t)stream;
+  DevBuf out;
+  int rc = out.alloc((size_t)n_ports * nrhs * sizeof(double2));
+  if (rc != EFM_OK)
+    return rc;
+  hipLaunchKernelGGL(k_port_overlap, dim3(n_ports * nrhs), dim3(kBlock), 0, st, n, len, gidx,
+                     (const double2 *)gval, (const double2 *)X, nrhs, out.as<double2>());
+  EFM_LAUNCH_CHECK();
+  EFM_HIP_CHECK(hipMemcpyAsync(h_out, out.p, (size_t)n_ports * nrhs * sizeof(double2),
+                               hipMemcpyDeviceToHost, st));
+  EFM_HIP_CHECK(hipStreamSynchronize(st));
+  return EFM_OK;
+}

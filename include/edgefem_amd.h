@@ -1,0 +1,165 @@
+/*
+ * edgefem_amd.h -- C ABI of the MI355X (gfx950) implementation of EdgeFEM's
+ * frequency-domain Nedelec hot path.
+ *
+ * Conventions
+ *   - Every pointer argument is a DEVICE pointer unless its name starts with h_.
+ *   - Complex values are interleaved (re, im): "c128" = 2 doubles, "c64" = 2 floats.
+ *   - Indices are 0-based int32; gmsh node IDs are int64.
+ *   - `stream` is a hipStream_t passed as void* (NULL = the null stream).  All work is
+ *     enqueued on it; functions that return host-visible results synchronise it.
+ *   - Every function returns an efm_status; on failure efm_last_error() gives a
+ *     message (thread-local).  No C++ exception crosses this boundary.
+ *   - Objects (efm_topology, efm_solver) are not thread-safe; one thread per object.
+ *
+ * Multi-RHS layout: a block of nrhs vectors of length n is stored system-minor,
+ * element (row r, system s) at [r * nrhs + s].
+ *
+ * Each entry point names the reference function it replaces (file:line in the
+ * EdgeFEM sources).
+ */
+#ifndef EDGEFEM_AMD_H
+#define EDGEFEM_AMD_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+typedef enum {
+  EFM_OK = 0,
+  EFM_ERR_ARG = 1,       /* invalid argument / shape */
+  EFM_ERR_HIP = 2,       /* HIP runtime error */
+  EFM_ERR_NOMEM = 3,     /* device allocation failed */
+  EFM_ERR_NOT_CONVERGED = 4,
+  EFM_ERR_MESH = 5       /* mesh does not satisfy an assumption (message says which) */
+} efm_status;
+
+enum { EFM_C128 = 0, EFM_C64 = 1 };                 /* matrix value storage */
+enum { EFM_METHOD_COCG = 0, EFM_METHOD_BICGSTAB = 1 };
+
+const char *efm_last_error(void);
+int efm_version(void);
+
+/* ------------------------------------------------------------------------
+ * Mesh topology: global edges in first-seen order, orientation signs, the CSR
+ * pattern of the assembled matrix and the element->nnz gather maps.
+ * Replaces build_edges() (src/mesh_gmsh.cpp:104-146, key rule include/edgefem/mesh.hpp:56-60)
+ * and the pattern that setFromTriplets builds (src/assemble_maxwell.cpp:205-207).
+ * tet_nodes / tri_nodes index node_ids (0-based positions in $Nodes order).
+ * ---------------------------------------------------------------------- */
+typedef struct efm_topology efm_topology;
+
+int efm_topology_create(int64_t n_nodes, const int64_t *node_ids, int64_t n_tets,
+                        const int32_t *tet_nodes, int64_t n_tris, const int32_t *tri_nodes,
+                        void *stream, efm_topology **out);
+void efm_topology_destroy(efm_topology *topo);
+
+typedef struct {
+  int64_t n_nodes, n_tets, n_tris, n_edges, nnz;
+} efm_topology_sizes;
+int efm_topology_get_sizes(const efm_topology *topo, efm_topology_sizes *h_out);
+
+/* Copy topology arrays into caller buffers (any may be NULL):
+ *   edges[n_edges*2] int64 (min ID, max ID); tet_edges[T*6] int32; tet_sign[T*6] int8;
+ *   tri_edges[F*3] int32; tri_sign[F*3] int8; row_ptr[n_edges+1], col_idx[nnz] int32. */
+int efm_topology_export(const efm_topology *topo, int64_t *edges, int32_t *tet_edges,
+                        int8_t *tet_sign, int32_t *tri_edges, int8_t *tri_sign,
+                        int32_t *row_ptr, int32_t *col_idx, void *stream);
+
+/* ------------------------------------------------------------------------
+ * Assembly
+ * ---------------------------------------------------------------------- */
+/* K = sum_e (K_e / mu_e) s_i s_j and M = sum_e (eps_e M_e) s_i s_j on the CSR pattern,
+ * Dirichlet rows/cols zeroed, K diagonal 1 and M diagonal 0 on Dirichlet edges.
+ * Replaces assemble_maxwell_km (src/sweep.cpp:90-172), element matrices of
+ * src/edge_basis.cpp:14-86.  xyz[n_nodes*3]; eps_tet[T] c128; inv_mu_tet[T] c128 or
+ * NULL (= 1); dirichlet[n_edges] uint8 or NULL; K_val[nnz], M_val[nnz] c128. */
+int efm_assemble_km(const efm_topology *topo, const double *xyz, const double *eps_tet,
+                    const double *inv_mu_tet, const uint8_t *dirichlet, double *K_val,
+                    double *M_val, void *stream);
+
+/* A = sum_e (K_e/mu_e - k0^2 eps_e M_e) s_i s_j, + j k0 on the diagonal of edges with
+ * abc_mask set, Dirichlet rows/cols zeroed with unit diagonal; k0 = omega / c0.
+ * Replaces the volume, ABC and Dirichlet parts of assemble_maxwell
+ * (src/assemble_maxwell.cpp:46-207, 244-265, 322-347).  abc_mask may be NULL. */
+int efm_assemble_maxwell(const efm_topology *topo, const double *xyz, const double *eps_tet,
+                         const double *inv_mu_tet, const uint8_t *dirichlet,
+                         const uint8_t *abc_mask, double omega, double *A_val, void *stream);
+
+/* Port surface mass M_s = sum over the selected triangles of s_i s_j M_tri(i,j),
+ * Dirichlet rows/cols skipped, written as real values on the volume pattern (zero off
+ * the port).  Replaces assemble_port_surface_mass (src/ports/wave_port.cpp:548-585) and
+ * triangle_whitney_mass_matrix (src/edge_basis.cpp:88-130).  tri_sel[n_sel] = indices
+ * of the port triangles in mesh order. */
+int efm_port_surface_mass(const efm_topology *topo, const double *xyz, int64_t n_sel,
+                          const int32_t *tri_sel, const uint8_t *dirichlet, double *Ms_val,
+                          void *stream);
+
+/* A_val = K_val - k0sq * M_val + sum_p coef[p] * Ms_val[p]  (c128 out, or c64 when
+ * out_type == EFM_C64).  h_coef[n_ports] c128 on the host; Ms_vals[p] device arrays.
+ * Replaces KMMatrices::combine (src/sweep.cpp:82-88) plus the jbeta M_s port operator
+ * of src/assemble_maxwell.cpp:734-746. */
+int efm_combine(const efm_topology *topo, const double *K_val, const double *M_val,
+                double k0sq, int n_ports, const double *const *Ms_vals, const double *h_coef,
+                int out_type, void *A_out, void *stream);
+
+/* ------------------------------------------------------------------------
+ * Krylov solver on a CSR matrix: COCG (complex-symmetric A) or BiCGSTAB, Jacobi
+ * preconditioned, x0 = 0, stop when ||r_s|| <= tol * ||b_s|| per system.  FP64 vectors
+ * and accumulation; the matrix may be stored c128 or c64.
+ * Replaces solve_linear (src/solver.cpp:35-193, defaults include/edgefem/solver.hpp:16-36).
+ * ---------------------------------------------------------------------- */
+typedef struct efm_solver efm_solver;
+
+int efm_solver_create(int64_t n, int64_t nnz, const int32_t *row_ptr, const int32_t *col_idx,
+                      int nrhs, int value_type, void *stream, efm_solver **out);
+void efm_solver_destroy(efm_solver *s);
+/* Install matrix values (c128 or c64 per value_type); also computes the Jacobi diagonal. */
+int efm_solver_set_matrix(efm_solver *s, const void *values, void *stream);
+
+typedef struct {
+  int method;            /* EFM_METHOD_COCG or EFM_METHOD_BICGSTAB */
+  double tolerance;      /* relative residual per system */
+  int max_iterations;
+  int check_every;       /* host convergence poll interval (iterations) */
+} efm_solve_opts;
+
+typedef struct {
+  int iterations;        /* max over systems */
+  int converged;         /* 1 if every system converged */
+  double max_rel_residual;
+} efm_solve_result;
+
+/* B[n*nrhs] c128 in, X[n*nrhs] c128 out.  h_iters / h_resid (nrhs each, may be NULL)
+ * receive per-system iteration counts and final recursive relative residuals. */
+int efm_solver_solve(efm_solver *s, const double *B, double *X, const efm_solve_opts *h_opts,
+                     efm_solve_result *h_result, int32_t *h_iters, double *h_resid,
+                     void *stream);
+
+/* y = A x (all nrhs systems), FP64 accumulation. */
+int efm_solver_spmv(efm_solver *s, const double *x, double *y, void *stream);
+
+/* Per-kernel timing of the fused SpMV launches inside efm_solver_solve, taken with HIP
+ * events on the solve's stream: enable, then read the accumulated totals. */
+int efm_solver_profile(efm_solver *s, int enable);
+int efm_solver_profile_read(efm_solver *s, double *h_spmv_ms_total, int64_t *h_spmv_launches);
+
+/* ------------------------------------------------------------------------
+ * Port overlaps for S-parameter extraction: h_out[j * nrhs + a] = g_j^H x_a with g_j
+ * given on a compact list of port edges (negative gidx entries are padding).  With
+ * g_j = M_s,j e_j this is the modal amplitude of src/assemble_maxwell.cpp:773-783; with
+ * g_j = w_j it is the port voltage V_j of 374-391 (the caller applies the -delta /
+ * sqrt(Z0) terms).  X[n*nrhs] system-minor; gidx[n_ports*len] int32; gval c128;
+ * h_out c128 [n_ports*nrhs]; synchronises the stream.
+ * ---------------------------------------------------------------------- */
+int efm_port_overlap(int64_t n, int n_ports, int64_t len, const int32_t *gidx, const double *gval,
+                     const double *X, int nrhs, double *h_out, void *stream);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* EDGEFEM_AMD_H */

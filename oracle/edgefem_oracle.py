@@ -1,0 +1,602 @@
+"""numpy/scipy restatement of the EdgeFEM hot path (test infrastructure only).
+
+Functions and the reference code each one follows:
+
+==========================  =========================================================
+load_gmsh_v2                src/mesh_gmsh.cpp:15-101
+build_edges                 src/mesh_gmsh.cpp:104-146, include/edgefem/mesh.hpp:56-60
+gradients_and_volume        src/edge_basis.cpp:14-26
+whitney_curl_curl_matrix    src/edge_basis.cpp:47-63
+whitney_mass_matrix         src/edge_basis.cpp:65-86 (lambda_int 28-30)
+triangle_whitney_mass       src/edge_basis.cpp:88-130
+build_edge_pec              src/bc.cpp:47-80
+assemble_maxwell            src/assemble_maxwell.cpp:46-350 (PML stretch s = 1 branch)
+assemble_maxwell_km         src/sweep.cpp:90-172
+port_surface_mass           src/ports/wave_port.cpp:548-585
+solve_port_mode_2d          src/ports/wave_port.cpp:410-514
+build_wave_port_2d          src/ports/wave_port.cpp:516-546
+solve_te10_mode             src/ports/port_eigensolve.cpp:45-63
+sparams_eigenmode           src/assemble_maxwell.cpp:637-787
+calculate_sparams           src/assemble_maxwell.cpp:352-394 (port loading 209-242)
+frequency_sweep_km          src/sweep.cpp:174-329 (K/M reuse branch)
+==========================  =========================================================
+
+Third-party routines replaced: Eigen ``setFromTriplets`` (duplicates summed) ->
+``scipy.sparse.coo_matrix(...).tocsr()``; Eigen ``GeneralizedSelfAdjointEigenSolver`` ->
+``scipy.linalg.eigh(K, M)`` (same LAPACK problem class; eigenvector sign is fixed by
+the physically anchored rule in ``orient_port_mode``, because the reference's sign is
+whatever Eigen returns and cannot be reproduced); Eigen ``SparseLU`` ->
+``scipy.sparse.linalg.splu`` (complex128 SuperLU).
+
+Parity pins (independent of the formulas restated here), all exercised in tests/:
+
+* topology: brute-force invariants of tests/test_edge_indexing.cpp:11-31 on the shipped
+  cube_cavity.msh fixture; exact edge counts of the structured meshes (closed form);
+  Euler characteristic V - E + F - T = 1 of a ball;
+* element matrices: rational closed forms on the unit tet; an independent 4-point
+  quadrature (exact for degree 2) of the mass integrand; K_e . G = 0 for the discrete
+  gradient; the 3 triangles of tests/test_triangle_mass_matrix.cpp:88-102 against a
+  7-point degree-5 quadrature at 1e-12;
+* assembly: a dense brute-force element loop on small meshes; A == K - k0^2 M; A is
+  Hermitian for real materials, non-Hermitian when lossy (tests/test_maxwell.cpp);
+* port mode: kc^2 against the analytic (pi/a)^2 as h -> 0, scipy eigh residual;
+* S-parameters: |S21| ~ 1, |S11| ~ 0, arg S21 ~ -beta L, reciprocity, and the
+  reference-measured values on examples/rect_waveguide.msh (docs/validation.md:20-27).
+"""
+
+from __future__ import annotations
+
+import dataclasses
+import math
+from typing import Dict, List, Optional, Sequence
+
+import numpy as np
+import scipy.linalg
+import scipy.sparse as sp
+import scipy.sparse.linalg as spla
+
+C0 = 299792458.0
+MU0 = 4.0 * math.pi * 1e-7
+EPS0 = 1.0 / (MU0 * C0 * C0)
+ETA0_ASM = MU0 * C0            # assemble_maxwell.cpp:43
+ETA0_PORT = 376.730313668      # port_eigensolve.cpp:15
+
+TET_PAIRS = np.array([(0, 1), (0, 2), (0, 3), (1, 2), (1, 3), (2, 3)])   # edge_basis.cpp:9-12
+TRI_PAIRS = np.array([(0, 1), (1, 2), (2, 0)])                           # mesh_gmsh.cpp:110-112
+
+
+# ---------------------------------------------------------------------------
+# Mesh
+# ---------------------------------------------------------------------------
+@dataclasses.dataclass
+class OMesh:
+    node_ids: np.ndarray      # (N,) int64 gmsh IDs in $Nodes order
+    xyz: np.ndarray           # (N,3)
+    tet_ids: np.ndarray       # (T,4) int64 gmsh IDs
+    tet_phys: np.ndarray
+    tri_ids: np.ndarray       # (F,3)
+    tri_phys: np.ndarray
+    edges: np.ndarray = None          # (n,2) int64 IDs (min,max)
+    tet_edges: np.ndarray = None      # (T,6)
+    tet_sign: np.ndarray = None       # (T,6) +-1
+    tri_edges: np.ndarray = None      # (F,3)
+    tri_sign: np.ndarray = None
+    node_index: Dict[int, int] = None
+
+    @property
+    def n_edges(self):
+        return self.edges.shape[0]
+
+    def tet_xyz(self):
+        idx = np.vectorize(self.node_index.__getitem__)(self.tet_ids) if self.tet_ids.size else self.tet_ids
+        return self.xyz[idx]
+
+    def tri_xyz(self):
+        idx = np.vectorize(self.node_index.__getitem__)(self.tri_ids) if self.tri_ids.size else self.tri_ids
+        return self.xyz[idx]
+
+
+def load_gmsh_v2(path: str) -> OMesh:
+    """Token-stream parse of gmsh v2.2 ASCII (mesh_gmsh.cpp:15-101): $Nodes gives
+    ``id x y z``; $Elements keeps types 2 (Tri3) and 4 (Tet4) with the first tag as
+    ``phys``; type 1 lines are read and dropped; other types skip the rest of the line."""
+    with open(path) as f:
+        lines = f.read().split("\n")
+    node_ids, xyz, tets, tetp, tris, trip = [], [], [], [], [], []
+    i = 0
+    while i < len(lines):
+        line = lines[i]
+        if line == "$Nodes":
+            n = int(lines[i + 1].split()[0])
+            block = np.array(" ".join(lines[i + 2:i + 2 + n]).split(), dtype=np.float64).reshape(n, 4)
+            node_ids = block[:, 0].astype(np.int64)
+            xyz = block[:, 1:4].copy()
+            i += 2 + n
+        elif line == "$Elements":
+            m = int(lines[i + 1].split()[0])
+            for r in range(m):
+                tok = lines[i + 2 + r].split()
+                typ = int(tok[1])
+                ntag = int(tok[2])
+                phys = int(tok[3]) if ntag > 0 else 0
+                rest = tok[3 + ntag:]
+                if typ == 2:
+                    tris.append([int(v) for v in rest[:3]]); trip.append(phys)
+                elif typ == 4:
+                    tets.append([int(v) for v in rest[:4]]); tetp.append(phys)
+            i += 2 + m
+        else:
+            i += 1
+    mesh = OMesh(node_ids=np.asarray(node_ids, np.int64), xyz=np.asarray(xyz, float),
+                 tet_ids=np.asarray(tets, np.int64).reshape(-1, 4), tet_phys=np.asarray(tetp, np.int32),
+                 tri_ids=np.asarray(tris, np.int64).reshape(-1, 3), tri_phys=np.asarray(trip, np.int32))
+    mesh.node_index = {int(v): k for k, v in enumerate(mesh.node_ids)}
+    build_edges(mesh)
+    return mesh
+
+
+def mesh_from_arrays(node_ids, xyz, tet_ids, tet_phys, tri_ids, tri_phys) -> OMesh:
+    mesh = OMesh(node_ids=np.asarray(node_ids, np.int64), xyz=np.asarray(xyz, float),
+                 tet_ids=np.asarray(tet_ids, np.int64), tet_phys=np.asarray(tet_phys, np.int32),
+                 tri_ids=np.asarray(tri_ids, np.int64), tri_phys=np.asarray(tri_phys, np.int32))
+    mesh.node_index = {int(v): k for k, v in enumerate(mesh.node_ids)}
+    build_edges(mesh)
+    return mesh
+
+
+def build_edges(mesh: OMesh) -> None:
+    """First-seen global edge numbering (mesh_gmsh.cpp:104-146): scan tets in file
+    order with local pairs (0,1),(0,2),(0,3),(1,2),(1,3),(2,3), then tris with pairs
+    (0,1),(1,2),(2,0).  Edge stored as (min ID, max ID); orientation +1 iff
+    conn[a] < conn[b] (gmsh IDs); key = (a<<32) ^ b of the sorted pair."""
+    index: Dict[int, int] = {}
+    edges: List = []
+    T = mesh.tet_ids.shape[0]
+    F = mesh.tri_ids.shape[0]
+    tet_edges = np.empty((T, 6), np.int32)
+    tet_sign = np.empty((T, 6), np.int8)
+    tri_edges = np.empty((F, 3), np.int32)
+    tri_sign = np.empty((F, 3), np.int8)
+    for conn, pairs, out_e, out_s in ((mesh.tet_ids, TET_PAIRS, tet_edges, tet_sign),
+                                      (mesh.tri_ids, TRI_PAIRS, tri_edges, tri_sign)):
+        cl = conn.tolist()
+        for t, c in enumerate(cl):
+            for e, (pa, pb) in enumerate(pairs):
+                a = c[pa]; b = c[pb]
+                lo, hi = (a, b) if a < b else (b, a)
+                key = ((lo << 32) ^ hi) & 0xFFFFFFFFFFFFFFFF
+                idx = index.get(key)
+                if idx is None:
+                    idx = len(edges)
+                    index[key] = idx
+                    edges.append((lo, hi))
+                out_e[t, e] = idx
+                out_s[t, e] = 1 if a < b else -1
+    mesh.edges = np.asarray(edges, np.int64).reshape(-1, 2)
+    mesh.tet_edges, mesh.tet_sign, mesh.tri_edges, mesh.tri_sign = tet_edges, tet_sign, tri_edges, tri_sign
+
+
+# ---------------------------------------------------------------------------
+# Element matrices (batched over tets)
+# ---------------------------------------------------------------------------
+def gradients_and_volume(X: np.ndarray):
+    """X (T,4,3).  B = [v0-v3, v1-v3, v2-v3] (columns); grad lambda_i = column i of
+    B^{-T} for i < 3, grad lambda_3 = -sum; V = |det B| / 6 (edge_basis.cpp:14-26)."""
+    B = np.stack([X[:, 0] - X[:, 3], X[:, 1] - X[:, 3], X[:, 2] - X[:, 3]], axis=2)  # columns
+    Binv = np.linalg.inv(B)
+    g = np.empty((X.shape[0], 4, 3))
+    g[:, 0:3, :] = Binv            # row i of B^{-1} == column i of B^{-T}
+    g[:, 3, :] = -(g[:, 0] + g[:, 1] + g[:, 2])
+    V = np.abs(np.linalg.det(B)) / 6.0
+    return g, V
+
+
+def whitney_curl_curl_matrix(X: np.ndarray) -> np.ndarray:
+    """K_e(i,j) = V c_i . c_j with c_i = 2 grad(l_a) x grad(l_b) (edge_basis.cpp:47-63)."""
+    g, V = gradients_and_volume(X)
+    c = 2.0 * np.cross(g[:, TET_PAIRS[:, 0]], g[:, TET_PAIRS[:, 1]])   # (T,6,3)
+    return V[:, None, None] * np.einsum("tik,tjk->tij", c, c)
+
+
+def whitney_mass_matrix(X: np.ndarray) -> np.ndarray:
+    """M_e(i,j) = gb.gd I(a,c) - gb.gc I(a,d) - ga.gd I(b,c) + ga.gc I(b,d) with
+    I(i,j) = V/10 (i==j) or V/20 (edge_basis.cpp:65-86)."""
+    g, V = gradients_and_volume(X)
+    G = np.einsum("tik,tjk->tij", g, g)     # (T,4,4) grad dot products
+    I = np.where(np.eye(4, dtype=bool)[None], V[:, None, None] / 10.0, V[:, None, None] / 20.0)
+    a = TET_PAIRS[:, 0][:, None]; b = TET_PAIRS[:, 1][:, None]
+    c = TET_PAIRS[:, 0][None, :]; d = TET_PAIRS[:, 1][None, :]
+    M = (G[:, b, d] * I[:, a, c] - G[:, b, c] * I[:, a, d]
+         - G[:, a, d] * I[:, b, c] + G[:, a, c] * I[:, b, d])
+    return M
+
+
+def triangle_whitney_mass(v: np.ndarray) -> np.ndarray:
+    """3x3 surface Whitney mass on triangles v (F,3,3), local edges (0,1),(1,2),(2,0);
+    in-plane gradients g_k = n x (opposite edge) / 2A; I = A/6 or A/12
+    (edge_basis.cpp:88-130).  Degenerate triangles (2A < 1e-30) give zeros."""
+    v = np.asarray(v, float).reshape(-1, 3, 3)
+    normal = np.cross(v[:, 1] - v[:, 0], v[:, 2] - v[:, 0])
+    area2 = np.linalg.norm(normal, axis=1)
+    ok = area2 >= 1e-30
+    safe = np.where(ok, area2, 1.0)
+    nh = normal / safe[:, None]
+    g = np.stack([np.cross(nh, v[:, 2] - v[:, 1]), np.cross(nh, v[:, 0] - v[:, 2]),
+                  np.cross(nh, v[:, 1] - v[:, 0])], axis=1) / safe[:, None, None]
+    area = safe / 2.0
+    G = np.einsum("tik,tjk->tij", g, g)
+    I = np.where(np.eye(3, dtype=bool)[None], area[:, None, None] / 6.0, area[:, None, None] / 12.0)
+    a = TRI_PAIRS[:, 0][:, None]; b = TRI_PAIRS[:, 1][:, None]
+    c = TRI_PAIRS[:, 0][None, :]; d = TRI_PAIRS[:, 1][None, :]
+    M = (G[:, b, d] * I[:, a, c] - G[:, b, c] * I[:, a, d]
+         - G[:, a, d] * I[:, b, c] + G[:, a, c] * I[:, b, d])
+    M[~ok] = 0.0
+    return M
+
+
+def triangle_surface_curls(v: np.ndarray):
+    """Per-triangle (curls[3], area) with curl_e = 2 (g_a x g_b) . n (wave_port.cpp:463-476)."""
+    v = np.asarray(v, float).reshape(-1, 3, 3)
+    normal = np.cross(v[:, 1] - v[:, 0], v[:, 2] - v[:, 0])
+    area2 = np.linalg.norm(normal, axis=1)
+    ok = area2 >= 1e-30
+    safe = np.where(ok, area2, 1.0)
+    nh = normal / safe[:, None]
+    g = np.stack([np.cross(nh, v[:, 2] - v[:, 1]), np.cross(nh, v[:, 0] - v[:, 2]),
+                  np.cross(nh, v[:, 1] - v[:, 0])], axis=1) / safe[:, None, None]
+    curls = 2.0 * np.einsum("tek,tk->te", np.cross(g[:, TRI_PAIRS[:, 0]], g[:, TRI_PAIRS[:, 1]]), nh)
+    return curls, safe / 2.0, ok, g
+
+
+# ---------------------------------------------------------------------------
+# Boundary conditions and materials
+# ---------------------------------------------------------------------------
+def build_edge_pec(mesh: OMesh, pec_tag: int) -> np.ndarray:
+    """Dirichlet edges = all edges of tris with the PEC tag (bc.cpp:47-80); returns a
+    boolean mask over global edges."""
+    mask = np.zeros(mesh.n_edges, bool)
+    sel = mesh.tri_phys == pec_tag
+    mask[mesh.tri_edges[sel].ravel()] = True
+    return mask
+
+
+@dataclasses.dataclass
+class Params:
+    """Subset of MaxwellParams (maxwell.hpp:46-143) the path uses.  Material precedence:
+    per-tet array (the build's extension) > region map > global value."""
+    omega: float = 0.0
+    eps_r: complex = 1.0
+    mu_r: complex = 1.0
+    eps_r_regions: Dict[int, complex] = dataclasses.field(default_factory=dict)
+    mu_r_regions: Dict[int, complex] = dataclasses.field(default_factory=dict)
+    eps_tet: Optional[np.ndarray] = None
+    mu_tet: Optional[np.ndarray] = None
+    use_abc: bool = False
+    abc_surface_tags: set = dataclasses.field(default_factory=set)
+    port_abc_scale: float = 1.0
+
+
+def tet_materials(mesh: OMesh, p: Params):
+    T = mesh.tet_ids.shape[0]
+    if p.eps_tet is not None:
+        eps = np.asarray(p.eps_tet, np.complex128)
+    else:
+        eps = np.array([p.eps_r_regions.get(int(t), p.eps_r) for t in mesh.tet_phys], np.complex128) if T else np.zeros(0, complex)
+    if p.mu_tet is not None:
+        mu = np.asarray(p.mu_tet, np.complex128)
+    else:
+        mu = np.array([p.mu_r_regions.get(int(t), p.mu_r) for t in mesh.tet_phys], np.complex128) if T else np.zeros(0, complex)
+    return eps, mu
+
+
+# ---------------------------------------------------------------------------
+# Assembly
+# ---------------------------------------------------------------------------
+def _triplets(mesh: OMesh):
+    T = mesh.tet_ids.shape[0]
+    rows = np.repeat(mesh.tet_edges, 6, axis=1).reshape(T, 6, 6)
+    cols = np.tile(mesh.tet_edges, (1, 6)).reshape(T, 6, 6)
+    sign = (mesh.tet_sign[:, :, None].astype(float) * mesh.tet_sign[:, None, :])
+    return rows, cols, sign
+
+
+def _dirichlet_zero(A: sp.csr_matrix, mask: np.ndarray, diag_value) -> sp.csr_matrix:
+    """Zero rows and columns of Dirichlet edges in place (explicit zeros kept), then set
+    the diagonal (assemble_maxwell.cpp:322-347)."""
+    A = A.tocsr(copy=True)
+    A.sort_indices()
+    rows = np.repeat(np.arange(A.shape[0]), np.diff(A.indptr))
+    kill = mask[rows] | mask[A.indices]
+    A.data[kill] = 0
+    if diag_value is not None:
+        d = np.nonzero(mask)[0]
+        pos = _find(A, d, d)
+        A.data[pos] = diag_value
+    return A
+
+
+def _find(A: sp.csr_matrix, r: np.ndarray, c: np.ndarray) -> np.ndarray:
+    """Positions of (r, c) in a sorted CSR matrix; raises if absent."""
+    out = np.empty(len(r), np.int64)
+    for k, (ri, ci) in enumerate(zip(r, c)):
+        lo, hi = A.indptr[ri], A.indptr[ri + 1]
+        j = lo + np.searchsorted(A.indices[lo:hi], ci)
+        if j >= hi or A.indices[j] != ci:
+            raise KeyError((ri, ci))
+        out[k] = j
+    return out
+
+
+def volume_pattern(mesh: OMesh) -> sp.csr_matrix:
+    rows, cols, _ = _triplets(mesh)
+    n = mesh.n_edges
+    P = sp.coo_matrix((np.ones(rows.size), (rows.ravel(), cols.ravel())), shape=(n, n)).tocsr()
+    P.sort_indices()
+    return P
+
+
+def assemble_maxwell_km(mesh: OMesh, p: Params, dirichlet: np.ndarray):
+    """K = sum (K_e / mu_r) s_i s_j, M = sum (eps_r M_e) s_i s_j, duplicates summed in
+    triplet order; Dirichlet rows/cols zeroed, K diagonal 1, M diagonal 0
+    (sweep.cpp:90-172).  Returns sorted CSR matrices with the full volume pattern."""
+    X = mesh.tet_xyz()
+    Ke = whitney_curl_curl_matrix(X)
+    Me = whitney_mass_matrix(X)
+    eps, mu = tet_materials(mesh, p)
+    rows, cols, sign = _triplets(mesh)
+    n = mesh.n_edges
+    kv = (Ke.astype(np.complex128) / mu[:, None, None]) * sign
+    mv = (eps[:, None, None] * Me) * sign
+    K = sp.coo_matrix((kv.ravel(), (rows.ravel(), cols.ravel())), shape=(n, n)).tocsr()
+    M = sp.coo_matrix((mv.ravel(), (rows.ravel(), cols.ravel())), shape=(n, n)).tocsr()
+    K.sort_indices(); M.sort_indices()
+    K = _dirichlet_zero(K, dirichlet, 1.0)
+    M = _dirichlet_zero(M, dirichlet, None)
+    return K, M
+
+
+def assemble_maxwell(mesh: OMesh, p: Params, dirichlet: np.ndarray, ports=(), active=-1):
+    """A = sum (K_e/mu - k0^2 eps M_e) s_i s_j (+ rank-1 port loading w w^H / Z0, + j k0
+    ABC on boundary edges), Dirichlet rows/cols zeroed with unit diagonal, b = 2w/sqrt(Z0)
+    on the active port (assemble_maxwell.cpp:46-350, PML stretch s = 1)."""
+    X = mesh.tet_xyz()
+    Ke = whitney_curl_curl_matrix(X)
+    Me = whitney_mass_matrix(X)
+    eps, mu = tet_materials(mesh, p)
+    k0 = p.omega / C0
+    rows, cols, sign = _triplets(mesh)
+    n = mesh.n_edges
+    val = (Ke / mu[:, None, None] - (k0 * k0) * eps[:, None, None] * Me) * sign
+    R = [rows.ravel()]; C = [cols.ravel()]; V = [val.ravel()]
+    b = np.zeros(n, np.complex128)
+    for i, port in enumerate(ports):
+        if port.Z0 == 0 or len(port.weights) != len(port.edges):
+            continue
+        e = np.asarray(port.edges)
+        w = np.asarray(port.weights, np.complex128)
+        keep = ~dirichlet[e]
+        e = e[keep]; w = w[keep]
+        R.append(np.repeat(e, len(e))); C.append(np.tile(e, len(e)))
+        V.append((w[:, None] * np.conj(w)[None, :] / port.Z0).ravel())
+        if i == active:
+            np.add.at(b, e, 2.0 / np.sqrt(complex(port.Z0)) * w)
+    A = sp.coo_matrix((np.concatenate(V), (np.concatenate(R), np.concatenate(C))), shape=(n, n)).tocsr()
+    A.sort_indices()
+    if p.use_abc:
+        sel = np.ones(mesh.tri_ids.shape[0], bool) if not p.abc_surface_tags else np.isin(mesh.tri_phys, list(p.abc_surface_tags))
+        be = np.unique(mesh.tri_edges[sel].ravel())
+        be = be[~dirichlet[be]]
+        A = A + sp.csr_matrix((np.full(be.size, 1j * k0), (be, be)), shape=(n, n))
+        A.sort_indices()
+    A = _dirichlet_zero(A, dirichlet, 1.0)
+    b[dirichlet] = 0
+    return A, b
+
+
+def port_surface_mass(mesh: OMesh, surface_tag: int, dirichlet: np.ndarray) -> sp.csr_matrix:
+    """M_s = sum over port-tag tris of s_i s_j M_tri(i,j), Dirichlet rows/cols skipped
+    (wave_port.cpp:548-585)."""
+    sel = np.nonzero(mesh.tri_phys == surface_tag)[0]
+    n = mesh.n_edges
+    if sel.size == 0:
+        return sp.csr_matrix((n, n))
+    idx = np.vectorize(mesh.node_index.__getitem__)(mesh.tri_ids[sel])
+    Mt = triangle_whitney_mass(mesh.xyz[idx])
+    e = mesh.tri_edges[sel]; s = mesh.tri_sign[sel].astype(float)
+    val = Mt * s[:, :, None] * s[:, None, :]
+    r = np.repeat(e, 3, axis=1).reshape(-1, 3, 3)
+    c = np.tile(e, (1, 3)).reshape(-1, 3, 3)
+    keep = ~(dirichlet[r] | dirichlet[c])
+    Ms = sp.coo_matrix((val[keep], (r[keep], c[keep])), shape=(n, n)).tocsr()
+    Ms.sort_indices()
+    return Ms
+
+
+# ---------------------------------------------------------------------------
+# Ports
+# ---------------------------------------------------------------------------
+@dataclasses.dataclass
+class OPort:
+    surface_tag: int
+    kc: float
+    Z0: complex
+    beta: complex
+    edges: np.ndarray
+    weights: np.ndarray
+    kc_sq_discrete: float = 0.0
+
+
+def solve_te10_mode(a: float, freq: float):
+    """kc = pi/a, beta = sqrt(max(0, k^2 - kc^2)), Z0 = eta0 k / beta with the literal
+    eta0 = 376.730313668 (port_eigensolve.cpp:45-63)."""
+    k = 2 * math.pi * freq / C0
+    kc = math.pi / a
+    beta = math.sqrt(max(0.0, k * k - kc * kc))
+    Z0 = ETA0_PORT * k / beta if beta > 0 else 0.0
+    return kc, beta, Z0
+
+
+def orient_port_mode(mesh: OMesh, surface_tag: int, vfull: np.ndarray) -> np.ndarray:
+    """Fix the eigenvector sign: p = sum_e w_e int_S N_e . y dS > 0, with the closed form
+    int_tri N_e dS = (A/3)(grad l_b - grad l_a) (lumped_port.cpp:87-90); falls back to x
+    when |p| is negligible.  The reference leaves the sign to Eigen; this rule is the
+    build's documented convention."""
+    sel = np.nonzero(mesh.tri_phys == surface_tag)[0]
+    idx = np.vectorize(mesh.node_index.__getitem__)(mesh.tri_ids[sel])
+    _, area, ok, g = triangle_surface_curls(mesh.xyz[idx])
+    integ = (area / 3.0)[:, None, None] * (g[:, TRI_PAIRS[:, 1]] - g[:, TRI_PAIRS[:, 0]])  # (F,3,3)
+    w = vfull[mesh.tri_edges[sel]] * mesh.tri_sign[sel]
+    pvec = np.einsum("fe,fek->k", w, integ)
+    scale = np.abs(vfull).max() * np.sqrt(area.sum()) + 1e-300
+    comp = pvec[1] if abs(pvec[1]) > 1e-9 * scale else pvec[0]
+    return -vfull if comp < 0 else vfull
+
+
+def solve_port_mode_2d(mesh: OMesh, surface_tag: int, dirichlet: np.ndarray, target_kc_sq: float):
+    """Dense generalised eigenproblem K_s e = kc^2 M_s e on the free port edges, with
+    K_s(i,j) = s_i s_j c_i c_j A and M_s from triangle_whitney_mass; picks the eigenvalue
+    closest to the target among those >= 1e-6 max(target, 1) (wave_port.cpp:410-514).
+    Free edges are indexed in first-seen order over the port tris."""
+    sel = np.nonzero(mesh.tri_phys == surface_tag)[0]
+    local = {}
+    order = []
+    for t in sel:
+        for le in range(3):
+            ge = int(mesh.tri_edges[t, le])
+            if dirichlet[ge] or ge in local:
+                continue
+            local[ge] = len(order)
+            order.append(ge)
+    n = len(order)
+    m = mesh.n_edges
+    if n == 0:
+        return np.zeros(m), 0.0
+    idx = np.vectorize(mesh.node_index.__getitem__)(mesh.tri_ids[sel])
+    v = mesh.xyz[idx]
+    curls, area, ok, _ = triangle_surface_curls(v)
+    Mt = triangle_whitney_mass(v)
+    K = np.zeros((n, n)); M = np.zeros((n, n))
+    for k, t in enumerate(sel):
+        if not ok[k]:
+            continue
+        for i in range(3):
+            li = local.get(int(mesh.tri_edges[t, i]))
+            if li is None:
+                continue
+            for j in range(3):
+                lj = local.get(int(mesh.tri_edges[t, j]))
+                if lj is None:
+                    continue
+                s = float(mesh.tri_sign[t, i]) * float(mesh.tri_sign[t, j])
+                K[li, lj] += s * curls[k, i] * curls[k, j] * area[k]
+                M[li, lj] += s * Mt[k, i, j]
+    evals, evecs = scipy.linalg.eigh(K, M)
+    kc_min = 1e-6 * max(target_kc_sq, 1.0)
+    best = -1
+    best_d = np.inf
+    for i, ev in enumerate(evals):
+        if ev < kc_min:
+            continue
+        d = abs(ev - target_kc_sq)
+        if d < best_d:
+            best_d = d; best = i
+    if best < 0:
+        return np.zeros(m), 0.0
+    vfull = np.zeros(m)
+    vfull[np.asarray(order)] = evecs[:, best]
+    return orient_port_mode(mesh, surface_tag, vfull), float(evals[best])
+
+
+def build_wave_port_2d(mesh: OMesh, surface_tag: int, kc: float, Z0: complex, beta: complex,
+                       dirichlet: np.ndarray, target_kc_sq: float) -> OPort:
+    """Port edges = sorted set of all port-tri edges (PEC included, weight 0); weights =
+    the 2D mode on them; mode.kc <- sqrt(discrete kc^2) (wave_port.cpp:516-546)."""
+    v, kc2 = solve_port_mode_2d(mesh, surface_tag, dirichlet, target_kc_sq)
+    if kc2 > 0:
+        kc = math.sqrt(kc2)
+    edges = np.unique(mesh.tri_edges[mesh.tri_phys == surface_tag].ravel())
+    return OPort(surface_tag=surface_tag, kc=kc, Z0=Z0, beta=beta, edges=edges,
+                 weights=v[edges].astype(np.complex128), kc_sq_discrete=kc2)
+
+
+def port_eps_mu(mesh: OMesh, p: Params, surface_tag: int) -> complex:
+    """eps*mu of the first tet (file order) owning a face on the port surface, faces
+    scanned {0,1,2},{0,1,3},{0,2,3},{1,2,3} (assemble_maxwell.cpp:652-686)."""
+    faces = {tuple(sorted(t)) for t in mesh.tri_ids[mesh.tri_phys == surface_tag].tolist()}
+    eps, mu = tet_materials(mesh, p)
+    for t, c in enumerate(mesh.tet_ids.tolist()):
+        for f in ((0, 1, 2), (0, 1, 3), (0, 2, 3), (1, 2, 3)):
+            if tuple(sorted((c[f[0]], c[f[1]], c[f[2]]))) in faces:
+                return complex(eps[t] * mu[t])
+    return 1.0 + 0j
+
+
+def eigenmode_setup(mesh: OMesh, p: Params, dirichlet: np.ndarray, ports: Sequence[OPort]):
+    """betas, M_s per port and M_s-normalised port vectors (assemble_maxwell.cpp:644-728).
+    Returns None for betas when a port is below cutoff."""
+    k0 = p.omega / C0
+    betas = []
+    for port in ports:
+        em = port_eps_mu(mesh, p, port.surface_tag)
+        b2 = em * k0 * k0 - port.kc * port.kc
+        if b2.real <= 0:
+            return None, None, None
+        betas.append(np.sqrt(complex(b2)))
+    Ms = [port_surface_mass(mesh, port.surface_tag, dirichlet) for port in ports]
+    vecs = []
+    n = mesh.n_edges
+    for port, M in zip(ports, Ms):
+        e = np.zeros(n, np.complex128)
+        keep = ~dirichlet[port.edges]
+        e[port.edges[keep]] = port.weights[keep]
+        nsq = np.real(np.vdot(e, M @ e))
+        if nsq > 1e-30:
+            e = e / np.sqrt(nsq)
+        vecs.append(e)
+    return np.array(betas), Ms, vecs
+
+
+def sparams_eigenmode(mesh: OMesh, p: Params, dirichlet: np.ndarray, ports: Sequence[OPort],
+                      solver="splu", return_systems=False):
+    """A = assemble_maxwell(no ports) + sum_i j beta_i scale M_s,i; b_a = 2 j beta_a scale
+    M_s,a e_a; S(j,a) = e_j^H M_s,j x - delta_ja (assemble_maxwell.cpp:637-787).  The two
+    excitations share A, so one LU serves both.  Below cutoff the result is NaN (the
+    reference returns an uninitialised matrix there)."""
+    P = len(ports)
+    betas, Ms, vecs = eigenmode_setup(mesh, p, dirichlet, ports)
+    if betas is None:
+        return np.full((P, P), np.nan + 1j * np.nan)
+    A, _ = assemble_maxwell(mesh, p, dirichlet)
+    for i in range(P):
+        A = A + (1j * p.port_abc_scale * betas[i]) * Ms[i]
+    A = A.tocsc()
+    B = np.stack([2.0 * (1j * p.port_abc_scale * betas[a]) * (Ms[a] @ vecs[a]) for a in range(P)], axis=1)
+    if solver == "splu":
+        lu = spla.splu(A, permc_spec="MMD_AT_PLUS_A")
+        X = lu.solve(B)
+    else:
+        X = solver(A, B)
+    S = np.empty((P, P), np.complex128)
+    for a in range(P):
+        for j in range(P):
+            S[j, a] = np.vdot(vecs[j], Ms[j] @ X[:, a]) - (1.0 if j == a else 0.0)
+    if return_systems:
+        return S, dict(A=A, B=B, X=X, betas=betas, Ms=Ms, vecs=vecs)
+    return S
+
+
+def calculate_sparams(mesh: OMesh, p: Params, dirichlet: np.ndarray, ports: Sequence[OPort]):
+    """Rank-1 port loading path with FP64 LU in place of the Krylov solve
+    (assemble_maxwell.cpp:352-394): V_j = sum conj(w_k) x_k over free edges,
+    S_ii = (V_i - sqrt Z0_i)/sqrt Z0_i, S_ji = V_j / sqrt Z0_i."""
+    P = len(ports)
+    S = np.empty((P, P), np.complex128)
+    for i in range(P):
+        A, b = assemble_maxwell(mesh, p, dirichlet, ports, i)
+        x = spla.splu(A.tocsc(), permc_spec="MMD_AT_PLUS_A").solve(b)
+        vinc = np.sqrt(complex(ports[i].Z0))
+        for j in range(P):
+            keep = ~dirichlet[ports[j].edges]
+            V = np.sum(np.conj(ports[j].weights[keep]) * x[ports[j].edges[keep]])
+            S[j, i] = (V - vinc) / vinc if i == j else V / vinc
+    return S

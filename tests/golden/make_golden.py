@@ -1,0 +1,87 @@
+"""Write the golden fixtures of tests/golden/ with the CPU oracle (FP64, scipy SuperLU).
+
+    python tests/golden/make_golden.py [c1] [sweep]
+
+c1     c1_sparams_10ghz.npz: S of the 16x8x32 WR-90 guide (seed 1) at 10 GHz, the discrete
+       port cutoffs and SHA-256 digests of the topology / CSR arrays.
+sweep  c3_sweep.npz: S at the 128 frequencies linspace(8.2e9, 12.4e9, 128) on the same
+       mesh (ports built once; only beta changes with frequency).
+"""
+import hashlib
+import math
+import multiprocessing as mp
+import os
+import sys
+from concurrent.futures import ProcessPoolExecutor
+
+import numpy as np
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+ROOT = os.path.dirname(os.path.dirname(HERE))
+sys.path.insert(0, ROOT)
+
+from oracle import edgefem_oracle as O  # noqa: E402
+from edgefem_amd import meshgen as mg  # noqa: E402
+
+
+def sha(a):
+    return hashlib.sha256(np.ascontiguousarray(a).tobytes()).hexdigest()
+
+
+def c1_setup():
+    g = mg.wr90_guide()
+    m = O.mesh_from_arrays(g.node_ids, g.xyz, g.tet_ids, g.tet_phys, g.tri_ids, g.tri_phys)
+    pec = O.build_edge_pec(m, 1)
+    a = mg.WR90_A
+    kc, beta, Z0 = O.solve_te10_mode(a, 10e9)
+    ports = [O.build_wave_port_2d(m, t, kc, Z0, beta, pec, (math.pi / a) ** 2) for t in (2, 3)]
+    return m, pec, ports
+
+
+def topo_digests(m):
+    P = O.volume_pattern(m)
+    return dict(edges=sha(m.edges.astype(np.int64)), tet_edges=sha(m.tet_edges.astype(np.int32)),
+                tet_sign=sha(m.tet_sign.astype(np.int8)), tri_edges=sha(m.tri_edges.astype(np.int32)),
+                tri_sign=sha(m.tri_sign.astype(np.int8)), row_ptr=sha(P.indptr.astype(np.int32)),
+                col_idx=sha(P.indices.astype(np.int32)))
+
+
+def make_c1():
+    m, pec, ports = c1_setup()
+    S = O.sparams_eigenmode(m, O.Params(omega=2 * math.pi * 10e9), pec, ports)
+    d = topo_digests(m)
+    np.savez(os.path.join(HERE, "c1_sparams_10ghz.npz"), S=S,
+             kc2=np.array([p.kc_sq_discrete for p in ports]), n_edges=m.n_edges,
+             nnz=O.volume_pattern(m).nnz, **{"sha_" + k: v for k, v in d.items()})
+    print("c1", S)
+
+
+_STATE = {}
+
+
+def _sweep_point(f):
+    if "m" not in _STATE:
+        _STATE["m"] = c1_setup()
+    m, pec, ports = _STATE["m"]
+    return O.sparams_eigenmode(m, O.Params(omega=2 * math.pi * f), pec, ports)
+
+
+def make_sweep():
+    freqs = np.linspace(8.2e9, 12.4e9, 128)
+    # fresh single-threaded BLAS in each worker (forking a process whose BLAS pool is
+    # already running can leave the children spinning)
+    os.environ["OPENBLAS_NUM_THREADS"] = "1"
+    os.environ["OMP_NUM_THREADS"] = "1"
+    os.environ["MKL_NUM_THREADS"] = "1"
+    with ProcessPoolExecutor(max_workers=7, mp_context=mp.get_context("spawn")) as ex:
+        S = np.stack(list(ex.map(_sweep_point, freqs, chunksize=4)))
+    np.savez(os.path.join(HERE, "c3_sweep.npz"), freqs=freqs, S=S)
+    print("sweep", S.shape, np.abs(S[:, 1, 0]).min())
+
+
+if __name__ == "__main__":
+    what = sys.argv[1:] or ["c1", "sweep"]
+    if "c1" in what:
+        make_c1()
+    if "sweep" in what:
+        make_sweep()

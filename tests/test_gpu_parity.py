@@ -1,0 +1,337 @@
+"""GPU parity of the HIP path against the CPU oracle and the golden fixtures.
+
+Every test here runs the product path (the HIP library through its C ABI) and checks it
+with the oracle on the same inputs:
+  * integer topology (edges, signs, CSR pattern): bit-exact;
+  * element / matrix values: |delta| <= 1e-12 * (sum of |contributions| of the entry);
+  * S-parameters: |delta S| <= 1e-6 against FP64 SuperLU.
+"""
+
+import math
+import sys
+
+import numpy as np
+import pytest
+
+from conftest import golden
+
+pytestmark = pytest.mark.gpu
+
+
+def _p(msg):
+    print(msg, flush=True)
+    sys.stdout.flush()
+
+
+@pytest.fixture(scope="module")
+def torch_dev():
+    import torch
+    assert torch.cuda.is_available(), "GPU tests need a HIP device"
+    from edgefem_amd import _capi
+    _capi.load_library()
+    return torch
+
+
+def _oracle_mesh_from_file(name):
+    from oracle import edgefem_oracle as O
+    return O.load_gmsh_v2(golden(name))
+
+
+def _oracle_mesh_from_gen(g):
+    from oracle import edgefem_oracle as O
+    return O.mesh_from_arrays(g.node_ids, g.xyz, g.tet_ids, g.tet_phys, g.tri_ids, g.tri_phys)
+
+
+def _check_topology(pm, om):
+    from oracle import edgefem_oracle as O
+    np.testing.assert_array_equal(pm.edge_nodes, om.edges)
+    np.testing.assert_array_equal(pm.tet_edges, om.tet_edges)
+    np.testing.assert_array_equal(pm.tet_sign, om.tet_sign)
+    np.testing.assert_array_equal(pm.tri_edges, om.tri_edges)
+    np.testing.assert_array_equal(pm.tri_sign, om.tri_sign)
+    P = O.volume_pattern(om)
+    np.testing.assert_array_equal(pm.topology.row_ptr.cpu().numpy(), P.indptr)
+    np.testing.assert_array_equal(pm.topology.col_idx.cpu().numpy(), P.indices)
+
+
+@pytest.mark.parametrize("name", ["cube_cavity.msh", "mixed_cavity.msh", "rect_waveguide.msh"])
+def test_topology_bit_exact_on_fixtures(torch_dev, name):
+    from edgefem_amd import pyedgefem as em
+    _p(f"[gpu] topology {name}")
+    _check_topology(em.load_gmsh(golden(name)), _oracle_mesh_from_file(name))
+
+
+def test_topology_bit_exact_shuffled_ids(torch_dev):
+    """Non-contiguous, permuted gmsh IDs and shuffled element order."""
+    from edgefem_amd import meshgen as mg
+    from edgefem_amd import pyedgefem as em
+    from oracle import edgefem_oracle as O
+    _p("[gpu] topology shuffled ids")
+    g = mg.box_mesh(0.01, 0.008, 0.012, 5, 4, 6, seed=7)
+    rng = np.random.default_rng(11)
+    ids = rng.choice(np.arange(10, 10 * g.node_ids.size + 10), size=g.node_ids.size, replace=False).astype(np.int64)
+    tperm = rng.permutation(g.tets.shape[0])
+    tets = g.tets[tperm]
+    # rotate vertex order of some tets (orientation does not matter for topology)
+    tets[::3] = tets[::3][:, [1, 2, 0, 3]]
+    pm = em.mesh_from_arrays(ids, g.xyz, tets, g.tet_phys[tperm], g.tris, g.tri_phys)
+    om = O.mesh_from_arrays(ids, g.xyz, ids[tets], g.tet_phys[tperm], ids[g.tris], g.tri_phys)
+    _check_topology(pm, om)
+
+
+def test_topology_c1_digest(torch_dev):
+    import hashlib
+    from edgefem_amd import meshgen as mg
+    from edgefem_amd import pyedgefem as em
+    _p("[gpu] topology c1 digest")
+    g = mg.wr90_guide()
+    pm = em.mesh_from_arrays(g.node_ids, g.xyz, g.tets, g.tet_phys, g.tris, g.tri_phys)
+    ref = np.load(golden("c1_sparams_10ghz.npz"))
+    sha = lambda a: hashlib.sha256(np.ascontiguousarray(a).tobytes()).hexdigest()
+    assert pm.num_edges() == int(ref["n_edges"]) == 31416
+    assert pm.topology.nnz == int(ref["nnz"]) == 484536
+    assert sha(pm.edge_nodes.astype(np.int64)) == str(ref["sha_edges"])
+    assert sha(pm.tet_edges.astype(np.int32)) == str(ref["sha_tet_edges"])
+    assert sha(pm.tet_sign.astype(np.int8)) == str(ref["sha_tet_sign"])
+    assert sha(pm.tri_edges.astype(np.int32)) == str(ref["sha_tri_edges"])
+    assert sha(pm.tri_sign.astype(np.int8)) == str(ref["sha_tri_sign"])
+    assert sha(pm.topology.row_ptr.cpu().numpy()) == str(ref["sha_row_ptr"])
+    assert sha(pm.topology.col_idx.cpu().numpy()) == str(ref["sha_col_idx"])
+
+
+def _entry_scale(om, p, dirichlet, kind):
+    """Scale of the 1e-12 criterion per nnz (r, c): sqrt(S_rr S_cc) with S the assembly
+    of |element entries|.  K and M are sums of positive semidefinite element blocks, so
+    |K_rc| <= sqrt(K_rr K_cc); entries that cancel to ~0 are judged against that bound."""
+    import scipy.sparse as sp
+    from oracle import edgefem_oracle as O
+    X = om.tet_xyz()
+    Ke = np.abs(O.whitney_curl_curl_matrix(X))
+    Me = np.abs(O.whitney_mass_matrix(X))
+    eps, mu = O.tet_materials(om, p)
+    T = X.shape[0]
+    rows = np.repeat(om.tet_edges, 6, axis=1).reshape(T, 6, 6)
+    cols = np.tile(om.tet_edges, (1, 6)).reshape(T, 6, 6)
+    n = om.n_edges
+
+    def scale(v):
+        S = sp.coo_matrix((v.ravel(), (rows.ravel(), cols.ravel())), shape=(n, n)).tocsr()
+        S.sort_indices()
+        d = S.diagonal() + 1.0 * dirichlet
+        r = np.repeat(np.arange(n), np.diff(S.indptr))
+        return np.sqrt(d[r] * d[S.indices])
+
+    sk = scale(Ke / np.abs(mu)[:, None, None])
+    sm = scale(Me * np.abs(eps)[:, None, None])
+    if kind == "K":
+        return sk
+    if kind == "M":
+        return sm
+    k0 = p.omega / O.C0
+    return sk + k0 * k0 * sm
+
+
+def _c1_lossy():
+    from edgefem_amd import meshgen as mg
+    g = mg.wr90_guide()
+    eps = mg.slab_eps(g, z0=15e-3, z1=35e-3)
+    return g, eps
+
+
+def test_km_and_direct_assembly_values(torch_dev):
+    from edgefem_amd import pyedgefem as em
+    from oracle import edgefem_oracle as O
+    _p("[gpu] assembly values c1 lossy slab")
+    g, eps = _c1_lossy()
+    om = _oracle_mesh_from_gen(g)
+    pm = em.mesh_from_arrays(g.node_ids, g.xyz, g.tets, g.tet_phys, g.tris, g.tri_phys)
+    pec = O.build_edge_pec(om, 1)
+    p = O.Params(omega=2 * math.pi * 10e9, eps_tet=eps)
+    Ko, Mo = O.assemble_maxwell_km(om, p, pec)
+    params = em.MaxwellParams()
+    params.omega = p.omega
+    params.eps_r_tet = eps
+    bc = em.build_edge_pec(pm, 1)
+    km = em.assemble_maxwell_km(pm, params, bc)
+    Kg = km.K.values.cpu().numpy()
+    Mg = km.M.values.cpu().numpy()
+    sK = _entry_scale(om, p, pec, "K")
+    sM = _entry_scale(om, p, pec, "M")
+    assert np.max(np.abs(Kg - Ko.data) / sK) <= 1e-12
+    assert np.max(np.abs(Mg - Mo.data) / sM) <= 1e-12
+    # direct form with ABC on the port faces
+    p.use_abc = True
+    p.abc_surface_tags = {2, 3}
+    Ao, _ = O.assemble_maxwell(om, p, pec)
+    params.use_abc = True
+    params.abc_surface_tags = {2, 3}
+    Ag = em.assemble_maxwell(pm, params, bc).A.values.cpu().numpy()
+    sA = _entry_scale(om, p, pec, "A") + (p.omega / O.C0)
+    assert np.max(np.abs(Ag - Ao.data) / sA) <= 1e-12
+
+
+def test_port_surface_mass_values(torch_dev):
+    from edgefem_amd import meshgen as mg
+    from edgefem_amd import pyedgefem as em
+    from oracle import edgefem_oracle as O
+    _p("[gpu] port surface mass")
+    om = _oracle_mesh_from_file("rect_waveguide.msh")
+    pm = em.load_gmsh(golden("rect_waveguide.msh"))
+    pec = O.build_edge_pec(om, 1)
+    bc = em.build_edge_pec(pm, 1)
+    for tag in (2, 3):
+        Mo = O.port_surface_mass(om, tag, pec)
+        Mg = em.port_surface_mass(pm, tag, bc.dirichlet_edges).cpu().numpy()
+        rp = pm.topology.row_ptr.cpu().numpy()
+        ci = pm.topology.col_idx.cpu().numpy()
+        import scipy.sparse as sp
+        G = sp.csr_matrix((Mg, ci, rp), shape=Mo.shape)
+        D = (G - Mo).toarray()
+        assert np.abs(D).max() <= 1e-12 * np.abs(Mo.toarray()).max()
+
+
+def _c1_em_setup(matrix_freq=10e9):
+    from edgefem_amd import meshgen as mg
+    from edgefem_amd import pyedgefem as em
+    g = mg.wr90_guide()
+    pm = em.mesh_from_arrays(g.node_ids, g.xyz, g.tets, g.tet_phys, g.tris, g.tri_phys)
+    bc = em.build_edge_pec(pm, 1)
+    dims = em.RectWaveguidePort(mg.WR90_A, mg.WR90_B)
+    mode = em.solve_te10_mode(dims, matrix_freq)
+    kc2 = (math.pi / mg.WR90_A) ** 2
+    ports = [em.build_wave_port_2d(pm, t, mode, bc.dirichlet_edges, kc2) for t in (2, 3)]
+    return pm, bc, ports
+
+
+def test_c1_eigenmode_sparams_fp64(torch_dev):
+    from edgefem_amd import pyedgefem as em
+    _p("[gpu] c1 eigenmode S fp64")
+    pm, bc, ports = _c1_em_setup()
+    ref = np.load(golden("c1_sparams_10ghz.npz"))
+    np.testing.assert_allclose([p.mode.kc ** 2 for p in ports], ref["kc2"], rtol=1e-10)
+    params = em.MaxwellParams()
+    params.omega = 2 * math.pi * 10e9
+    st = {}
+    S = em.calculate_sparams_eigenmode(pm, params, bc, ports, _state=st)
+    _p(f"    iterations {st['info']['iters']} resid {st['info']['resid']}")
+    assert np.abs(S - ref["S"]).max() <= 1e-6
+    assert st["info"]["converged"]
+
+
+def test_c1_eigenmode_sparams_fp32_matrix(torch_dev):
+    import torch
+    from edgefem_amd import pyedgefem as em
+    _p("[gpu] c1 eigenmode S fp32 matrix")
+    pm, bc, ports = _c1_em_setup()
+    ref = np.load(golden("c1_sparams_10ghz.npz"))
+    params = em.MaxwellParams()
+    params.omega = 2 * math.pi * 10e9
+    S = em.calculate_sparams_eigenmode(pm, params, bc, ports, matrix_dtype=torch.complex64)
+    assert np.abs(S - ref["S"]).max() <= 1e-5
+
+
+@pytest.mark.parametrize("f_ghz,s11,s21,ph", [(8.0, 0.003, 0.9997, 80.8), (10.0, 0.013, 0.9992, -100.7),
+                                              (12.0, 0.053, 0.9973, 103.9)])
+def test_wr90_reference_measurements_on_gpu(torch_dev, f_ghz, s11, s21, ph):
+    """docs/validation.md:20-27 through the GPU path, to the printed digits."""
+    from edgefem_amd import pyedgefem as em
+    _p(f"[gpu] rect_waveguide {f_ghz} GHz")
+    pm = em.load_gmsh(golden("rect_waveguide.msh"))
+    bc = em.build_edge_pec(pm, 1)
+    dims = em.RectWaveguidePort(0.02286, 0.01016)
+    mode = em.solve_te10_mode(dims, f_ghz * 1e9)
+    ports = [em.build_wave_port_2d(pm, t, mode, bc.dirichlet_edges, (math.pi / 0.02286) ** 2) for t in (2, 3)]
+    params = em.MaxwellParams()
+    params.omega = 2 * math.pi * f_ghz * 1e9
+    S = em.calculate_sparams_eigenmode(pm, params, bc, ports)
+    assert abs(abs(S[0, 0]) - s11) <= 0.0005 + 1e-9
+    assert abs(abs(S[1, 0]) - s21) <= 0.00005 + 1e-9
+    dph = (np.degrees(np.angle(S[1, 0])) - ph + 180) % 360 - 180
+    assert abs(dph) <= 0.05 + 1e-9
+
+
+def test_solve_linear_bicgstab_nonsymmetric(torch_dev):
+    """BiCGSTAB on a non-symmetric matrix (lossy A plus a skew perturbation on the
+    pattern) against SuperLU."""
+    import scipy.sparse.linalg as spla
+    import torch
+    from edgefem_amd import pyedgefem as em
+    from edgefem_amd import _capi
+    _p("[gpu] bicgstab nonsymmetric")
+    pm = em.load_gmsh(golden("rect_waveguide.msh"))
+    bc = em.build_edge_pec(pm, 1)
+    params = em.MaxwellParams()
+    params.omega = 2 * math.pi * 10e9
+    params.use_abc = True
+    A = em.assemble_maxwell(pm, params, bc).A
+    vals = A.values.clone()
+    rp = A.row_ptr.cpu().numpy()
+    ci = A.col_idx.cpu().numpy()
+    rows = np.repeat(np.arange(len(rp) - 1), np.diff(rp))
+    pert = np.where(rows < ci, 0.05j, -0.02) * np.abs(vals.cpu().numpy())
+    vals = vals + torch.from_numpy(pert).to(vals.device)
+    An = em.SpMatC(A.row_ptr, A.col_idx, vals, symmetric=False)
+    rng = np.random.default_rng(5)
+    b = rng.standard_normal(pm.num_edges()) + 1j * rng.standard_normal(pm.num_edges())
+    res = em.solve_linear(An, em.VecC(b))
+    assert res.converged and res.method.startswith("BiCGSTAB")
+    x_ref = spla.splu(An.to_scipy().tocsc()).solve(b)
+    assert np.linalg.norm(res.x.to_numpy() - x_ref) <= 1e-8 * np.linalg.norm(x_ref)
+
+
+def test_solver_masks_converged_systems(torch_dev):
+    """Two right-hand sides of very different difficulty: the easy one stops early and
+    its solution no longer changes; a zero right-hand side returns x = 0 at once."""
+    import torch
+    from edgefem_amd import _capi
+    from edgefem_amd import pyedgefem as em
+    _p("[gpu] masking")
+    pm, bc, ports = _c1_em_setup()
+    params = em.MaxwellParams()
+    params.omega = 2 * math.pi * 10e9
+    st = {}
+    em.calculate_sparams_eigenmode(pm, params, bc, ports, _state=st)
+    A = st["A"]
+    B = st["B"].clone()
+    n = pm.num_edges()
+    B2 = torch.zeros((n, 4), dtype=torch.complex128, device=B.device)
+    B2[:, 0] = B[:, 0]
+    B2[:, 2] = B[:, 1]
+    solver = _capi.KrylovSolver(pm.topology.row_ptr, pm.topology.col_idx, nrhs=4)
+    solver.set_matrix(A)
+    X, info = solver.solve(B2.reshape(-1), tol=1e-10, max_iterations=20000)
+    X = X.reshape(n, 4)
+    assert info["iters"][1] == 0 and info["iters"][3] == 0
+    assert torch.count_nonzero(X[:, 1]) == 0
+    # the same systems solved alone give the same iterates
+    X1, info1 = solver.solve(B2.reshape(-1), tol=1e-10, max_iterations=20000)
+    assert torch.equal(X1.reshape(n, 4), X)                    # deterministic
+    assert info["iters"][0] > 100
+
+
+def test_full_size_c2_topology_properties(torch_dev):
+    """Config-2 mesh at full size (1,327,104 tets): exact counts, symmetric pattern,
+    valid signs, every tet edge in its own row."""
+    from edgefem_amd import meshgen as mg
+    from edgefem_amd import pyedgefem as em
+    _p("[gpu] c2 topology full size")
+    g = mg.box_mesh(mg.WR90_A, mg.WR90_B, 100e-3, 48, 24, 192, seed=1)
+    pm = em.mesh_from_arrays(g.node_ids, g.xyz, g.tets, g.tet_phys, g.tris, g.tri_phys)
+    assert pm.num_tets() == 1327104
+    assert pm.num_edges() == 1593480
+    assert pm.topology.nnz == 25661064
+    rp = pm.topology.row_ptr.cpu().numpy().astype(np.int64)
+    ci = pm.topology.col_idx.cpu().numpy().astype(np.int64)
+    rows = np.repeat(np.arange(rp.size - 1), np.diff(rp))
+    key = rows * rp.size + ci
+    keyT = ci * rp.size + rows
+    assert np.array_equal(np.sort(key), np.sort(keyT))        # structurally symmetric
+    assert np.all(np.diff(ci)[np.diff(rows) == 0] > 0)          # sorted, distinct
+    assert set(np.unique(pm.tet_sign)) <= {-1, 1}
+    e = pm.edge_nodes
+    assert np.all(e[:, 0] < e[:, 1])
+    # first-seen: the first occurrence of edge k in the tet scan precedes that of edge k+1
+    te = pm.tet_edges.ravel()
+    uniq, first = np.unique(te, return_index=True)
+    assert uniq.size == pm.num_edges()
+    assert np.all(np.diff(first) > 0)

@@ -1,0 +1,336 @@
+"""Pins of the CPU oracle against the reference's own tests, fixtures and measured
+numbers, and against independent mathematics (closed forms, quadrature, invariants,
+brute force).  CPU only."""
+
+import math
+
+import numpy as np
+import pytest
+import scipy.sparse as sp
+
+from conftest import golden
+from oracle import edgefem_oracle as O
+from edgefem_amd import meshgen as mg
+
+
+# ---------------------------------------------------------------------------
+# topology
+# ---------------------------------------------------------------------------
+@pytest.mark.parametrize("name", ["cube_cavity.msh", "mixed_cavity.msh", "rect_waveguide.msh"])
+def test_edge_indexing_invariants(name):
+    """tests/test_edge_indexing.cpp:11-31 on the shipped fixtures."""
+    m = O.load_gmsh_v2(golden(name))
+    assert m.n_edges > 0
+    assert np.all(m.edges[:, 0] < m.edges[:, 1])
+    keys = {((int(a) << 32) ^ int(b)): i for i, (a, b) in enumerate(m.edges.tolist())}
+    assert all(keys[(int(a) << 32) ^ int(b)] == i for i, (a, b) in enumerate(m.edges.tolist()))
+    for t in range(m.tet_ids.shape[0]):
+        for k, (pa, pb) in enumerate(O.TET_PAIRS):
+            a, b = m.tet_ids[t, pa], m.tet_ids[t, pb]
+            e = m.edges[m.tet_edges[t, k]]
+            assert e[0] == min(a, b) and e[1] == max(a, b)
+            assert m.tet_sign[t, k] == (1 if a < b else -1)
+
+
+def _first_seen_by_unique(tet_ids, tri_ids):
+    """Independent formulation: all slots in scan order, np.unique(return_index) gives
+    the first occurrence of every sorted pair; ranking by it is the first-seen order."""
+    pairs = [tet_ids[:, O.TET_PAIRS[:, 0]], tet_ids[:, O.TET_PAIRS[:, 1]]]
+    a = np.concatenate([pairs[0].ravel(), tri_ids[:, O.TRI_PAIRS[:, 0]].ravel()])
+    b = np.concatenate([pairs[1].ravel(), tri_ids[:, O.TRI_PAIRS[:, 1]].ravel()])
+    lo, hi = np.minimum(a, b), np.maximum(a, b)
+    key = lo * (int(hi.max()) + 1) + hi
+    uniq, first, inv = np.unique(key, return_index=True, return_inverse=True)
+    rank = np.empty(uniq.size, np.int64)
+    rank[np.argsort(first, kind="stable")] = np.arange(uniq.size)
+    slot_edge = rank[inv]
+    order = np.argsort(first, kind="stable")
+    edges = np.stack([lo[first[order]], hi[first[order]]], axis=1)
+    return edges, slot_edge
+
+
+@pytest.mark.parametrize("name", ["cube_cavity.msh", "rect_waveguide.msh"])
+def test_first_seen_numbering_brute_force(name):
+    m = O.load_gmsh_v2(golden(name))
+    edges, slot_edge = _first_seen_by_unique(m.tet_ids, m.tri_ids)
+    T = m.tet_ids.shape[0]
+    np.testing.assert_array_equal(edges, m.edges)
+    np.testing.assert_array_equal(slot_edge[:6 * T].reshape(T, 6), m.tet_edges)
+    np.testing.assert_array_equal(slot_edge[6 * T:].reshape(-1, 3), m.tri_edges)
+
+
+def test_structured_counts_and_euler():
+    """Closed-form counts of the 16x8x32 Kuhn mesh and V - E + F - T = 1 for a ball."""
+    g = mg.wr90_guide()
+    m = O.mesh_from_arrays(g.node_ids, g.xyz, g.tet_ids, g.tet_phys, g.tri_ids, g.tri_phys)
+    nx, ny, nz = 16, 8, 32
+    V = (nx + 1) * (ny + 1) * (nz + 1)
+    E_axis = nx * (ny + 1) * (nz + 1) + (nx + 1) * ny * (nz + 1) + (nx + 1) * (ny + 1) * nz
+    E_diag = nx * ny * (nz + 1) + nx * (ny + 1) * nz + (nx + 1) * ny * nz
+    E = E_axis + E_diag + nx * ny * nz
+    assert m.xyz.shape[0] == V == 5049
+    assert m.n_edges == E == 31416
+    faces = np.sort(np.concatenate([m.tet_ids[:, f] for f in ([0, 1, 2], [0, 1, 3], [0, 2, 3], [1, 2, 3])]), axis=1)
+    F = np.unique(faces, axis=0).shape[0]
+    T = m.tet_ids.shape[0]
+    assert V - E + F - T == 1
+    P = O.volume_pattern(m)
+    assert P.nnz == 484536
+    # boundary triangles are exactly the faces used once
+    fu, cnt = np.unique(faces, axis=0, return_counts=True)
+    bnd = fu[cnt == 1]
+    tri = np.unique(np.sort(m.tri_ids, axis=1), axis=0)
+    np.testing.assert_array_equal(bnd, tri)
+
+
+# ---------------------------------------------------------------------------
+# element matrices
+# ---------------------------------------------------------------------------
+def _whitney(v, lam_fn, x):
+    """All six Whitney functions at point x of the tet v (independent evaluation)."""
+    g = lam_fn["grad"]
+    lam = lam_fn["lam"](x)
+    return np.array([lam[a] * g[b] - lam[b] * g[a] for a, b in O.TET_PAIRS])
+
+
+def _bary(v):
+    B = np.stack([v[0] - v[3], v[1] - v[3], v[2] - v[3]], axis=1)
+    Binv = np.linalg.inv(B)
+
+    def lam(x):
+        l3 = Binv @ (x - v[3])
+        return np.array([l3[0], l3[1], l3[2], 1 - l3.sum()])
+    grads = np.array([Binv[0], Binv[1], Binv[2], -Binv.sum(axis=0)])
+    return {"lam": lam, "grad": grads}
+
+
+def _quad_tet(v, f):
+    """4-point rule exact for degree 2 (Keast / Stroud)."""
+    a, b = 0.5854101966249685, 0.1381966011250105
+    vol = abs(np.linalg.det(np.stack([v[1] - v[0], v[2] - v[0], v[3] - v[0]]))) / 6
+    pts = [a * v[i] + b * sum(v[j] for j in range(4) if j != i) for i in range(4)]
+    return sum(f(p) for p in pts) * vol / 4
+
+
+def _curl_fd(fn, x, h=1e-3):
+    J = np.zeros((6, 3, 3))
+    for d in range(3):
+        e = np.zeros(3); e[d] = h
+        J[:, :, d] = (fn(x + e) - fn(x - e)) / (2 * h)
+    return np.stack([J[:, 2, 1] - J[:, 1, 2], J[:, 0, 2] - J[:, 2, 0], J[:, 1, 0] - J[:, 0, 1]], axis=1)
+
+
+TETS = [np.array([[0, 0, 0], [1, 0, 0], [0, 1, 0], [0, 0, 1]], float),
+        np.array([[0.1, -0.2, 0.3], [1.3, 0.1, 0.2], [0.2, 1.1, -0.1], [0.3, 0.2, 0.9]], float),
+        np.array([[0, 0, 0], [2e-3, 0, 0], [0, 1e-3, 0], [5e-4, 4e-4, 1.5e-3]], float)]
+
+
+@pytest.mark.parametrize("v", TETS)
+def test_element_matrices_vs_quadrature(v):
+    K = O.whitney_curl_curl_matrix(v[None])[0]
+    M = O.whitney_mass_matrix(v[None])[0]
+    bf = _bary(v)
+    Mq = _quad_tet(v, lambda x: (lambda N: N @ N.T)(_whitney(v, bf, x)))
+    cent = v.mean(axis=0)
+    h = 1e-3 * np.abs(v).max()
+    c = _curl_fd(lambda x: _whitney(v, bf, x), cent, h)
+    vol = abs(np.linalg.det(np.stack([v[1] - v[0], v[2] - v[0], v[3] - v[0]]))) / 6
+    Kq = vol * c @ c.T
+    assert np.abs(M - Mq).max() <= 1e-12 * np.abs(Mq).max()
+    assert np.abs(K - Kq).max() <= 1e-8 * np.abs(Kq).max()   # finite-difference curl
+    # discrete gradient in the kernel, PSD rank 3, SPD mass
+    G = np.zeros((6, 4))
+    for e, (a, b) in enumerate(O.TET_PAIRS):
+        G[e, a], G[e, b] = -1.0, 1.0
+    assert np.abs(K @ G).max() <= 1e-12 * np.abs(K).max()
+    ev = np.linalg.eigvalsh(K)
+    assert np.sum(ev > 1e-10 * ev.max()) == 3 and ev.min() > -1e-10 * ev.max()
+    assert np.linalg.eigvalsh(M).min() > 0
+
+
+def test_unit_tet_rational_entries():
+    """Hand-derived entries on the unit tet: V = 1/6, curl of edge (0,1) = (0,-2,2)."""
+    K = O.whitney_curl_curl_matrix(TETS[0][None])[0]
+    M = O.whitney_mass_matrix(TETS[0][None])[0]
+    assert K[0, 0] == pytest.approx(8 / 6, rel=1e-15)       # |(0,-2,2)|^2 / 6
+    assert K[3, 3] == pytest.approx(4 / 6, rel=1e-15)       # edge (1,2): 2 e_x x e_y = (0,0,2)
+    # M(0,0) = |g1|^2 I(0,0) - 2 g0.g1 I(0,1) + |g0|^2 I(1,1) = 1/60 + 2/120 + 3/60 = 1/12
+    assert M[0, 0] == pytest.approx(1 / 12, rel=1e-14)
+
+
+def test_scaling_laws():
+    v = TETS[1]
+    K1, M1 = O.whitney_curl_curl_matrix(v[None])[0], O.whitney_mass_matrix(v[None])[0]
+    K2, M2 = O.whitney_curl_curl_matrix((2.5 * v)[None])[0], O.whitney_mass_matrix((2.5 * v)[None])[0]
+    np.testing.assert_allclose(K2, K1 / 2.5, rtol=0, atol=1e-13 * np.abs(K1).max())
+    np.testing.assert_allclose(M2, M1 * 2.5, rtol=0, atol=1e-13 * np.abs(M1).max())
+
+
+def _tri_quad_mass(v):
+    """7-point degree-5 rule, as tests/test_triangle_mass_matrix.cpp:40-64 describes."""
+    w0 = 9.0 / 40.0
+    a1, b1, w1 = 0.059715871789770, 0.470142064105115, 0.132394152788506
+    a2, b2, w2 = 0.797426985353087, 0.101286507323456, 0.125939180544827
+    pts = [(1 / 3, 1 / 3, 1 / 3, w0), (a1, b1, b1, w1), (b1, a1, b1, w1), (b1, b1, a1, w1),
+           (a2, b2, b2, w2), (b2, a2, b2, w2), (b2, b2, a2, w2)]
+    normal = np.cross(v[1] - v[0], v[2] - v[0])
+    area2 = np.linalg.norm(normal)
+    nh = normal / area2
+    g = [np.cross(nh, v[2] - v[1]) / area2, np.cross(nh, v[0] - v[2]) / area2, np.cross(nh, v[1] - v[0]) / area2]
+    M = np.zeros((3, 3))
+    for l0, l1, l2, w in pts:
+        lam = [l0, l1, l2]
+        N = [lam[a] * g[b] - lam[b] * g[a] for a, b in O.TRI_PAIRS]
+        for i in range(3):
+            for j in range(3):
+                M[i, j] += w * (area2 / 2) * N[i] @ N[j]
+    return M
+
+
+@pytest.mark.parametrize("v", [
+    np.array([[0, 0, 0], [1, 0, 0], [0, 1, 0]], float),
+    np.array([[0.2, -0.1, 0.0], [1.7, 0.3, 0.0], [0.4, 2.1, 0.0]], float),
+    np.array([[0.1, 0.2, 0.3], [1.1, 0.4, 0.9], [0.3, 1.5, 0.7]], float)])
+def test_triangle_mass_matches_reference_test(v):
+    """The three triangles of tests/test_triangle_mass_matrix.cpp:88-102 at 1e-12."""
+    M = O.triangle_whitney_mass(v[None])[0]
+    Mq = _tri_quad_mass(v)
+    assert np.linalg.norm(M - Mq) / np.linalg.norm(Mq) < 1e-12
+    assert np.linalg.norm(M - M.T) < 1e-14 * np.linalg.norm(M)
+    assert np.linalg.eigvalsh(M).min() > 0
+
+
+# ---------------------------------------------------------------------------
+# assembly
+# ---------------------------------------------------------------------------
+def _brute_dense(m, p, dirichlet):
+    """Dense element loop written out scalar by scalar (independent of the COO path)."""
+    n = m.n_edges
+    A = np.zeros((n, n), complex)
+    X = m.tet_xyz()
+    eps, mu = O.tet_materials(m, p)
+    k0 = p.omega / O.C0
+    for t in range(X.shape[0]):
+        Ke = O.whitney_curl_curl_matrix(X[t][None])[0]
+        Me = O.whitney_mass_matrix(X[t][None])[0]
+        for i in range(6):
+            for j in range(6):
+                s = m.tet_sign[t, i] * m.tet_sign[t, j]
+                A[m.tet_edges[t, i], m.tet_edges[t, j]] += s * (Ke[i, j] / mu[t] - k0 * k0 * eps[t] * Me[i, j])
+    d = np.nonzero(dirichlet)[0]
+    A[d, :] = 0
+    A[:, d] = 0
+    A[d, d] = 1
+    return A
+
+
+def test_assembly_brute_force_and_km_identity():
+    m = O.load_gmsh_v2(golden("cube_cavity.msh"))
+    pec = O.build_edge_pec(m, 1)
+    rng = np.random.default_rng(3)
+    p = O.Params(omega=2 * math.pi * 3e8, eps_tet=1 + rng.random(m.tet_ids.shape[0]) - 0.05j)
+    A, b = O.assemble_maxwell(m, p, pec)
+    Ad = _brute_dense(m, p, pec)
+    np.testing.assert_allclose(A.toarray(), Ad, rtol=0, atol=1e-12 * np.abs(Ad).max())
+    K, M = O.assemble_maxwell_km(m, p, pec)
+    k0 = p.omega / O.C0
+    np.testing.assert_allclose((K - k0 * k0 * M).toarray(), A.toarray(), rtol=0, atol=1e-12 * np.abs(Ad).max())
+    # Dirichlet rows are unit vectors
+    for e in np.nonzero(pec)[0][:20]:
+        row = A.getrow(e).toarray().ravel()
+        assert row[e] == 1 and np.count_nonzero(row) == 1
+
+
+def test_hermitian_iff_lossless():
+    """tests/test_maxwell.cpp:8-17 and 94-105, with Eigen's isApprox criterion
+    ||A - A^H|| <= 1e-9 ||A||.  The reference's lossy case uses omega = 1 rad/s, where the
+    loss term (k0^2 ~ 1e-17) is far below that resolution, so it would not detect the
+    asymmetry; a 1 GHz omega is used here instead."""
+    m = O.load_gmsh_v2(golden("cube_cavity.msh"))
+    pec = O.build_edge_pec(m, 1)
+    omega = 2 * math.pi * 1e9
+
+    def herm_err(D):
+        return np.linalg.norm(D - D.conj().T) / np.linalg.norm(D)
+
+    D = O.assemble_maxwell(m, O.Params(omega=omega), pec)[0].toarray()
+    assert herm_err(D) <= 1e-9
+    D2 = O.assemble_maxwell(m, O.Params(omega=omega, eps_r=1.0 - 0.1j), pec)[0].toarray()
+    assert herm_err(D2) > 1e-3
+    assert np.linalg.norm(D2 - D2.T) <= 1e-14 * np.linalg.norm(D2)   # complex symmetric
+
+
+def test_abc_adds_j_k0():
+    """ABC adds + j k0 (not j omega: tests/test_abc.cpp is stale, the code adds j k0)."""
+    m = O.load_gmsh_v2(golden("cube_cavity.msh"))
+    pec = np.zeros(m.n_edges, bool)
+    p0 = O.Params(omega=2 * math.pi * 1e9)
+    p1 = O.Params(omega=2 * math.pi * 1e9, use_abc=True)
+    A0, _ = O.assemble_maxwell(m, p0, pec)
+    A1, _ = O.assemble_maxwell(m, p1, pec)
+    D = (A1 - A0).toarray()
+    bnd = np.unique(m.tri_edges.ravel())
+    k0 = p0.omega / O.C0
+    np.testing.assert_allclose(np.diag(D)[bnd], 1j * k0, rtol=1e-14)
+    D[bnd, bnd] = 0
+    assert np.abs(D).max() == 0
+
+
+# ---------------------------------------------------------------------------
+# ports and S-parameters
+# ---------------------------------------------------------------------------
+# docs/validation.md:20-27 (reference-measured on examples/rect_waveguide.msh)
+VALIDATION = [(7.0, 0.019, 0.9996, -150.6), (8.0, 0.003, 0.9997, 80.8), (9.0, 0.010, 0.9994, -15.6),
+              (10.0, 0.013, 0.9992, -100.7), (11.0, 0.041, 0.9982, 179.9), (12.0, 0.053, 0.9973, 103.9)]
+
+
+@pytest.fixture(scope="module")
+def wr90():
+    m = O.load_gmsh_v2(golden("rect_waveguide.msh"))
+    pec = O.build_edge_pec(m, 1)
+    return m, pec
+
+
+@pytest.mark.parametrize("f_ghz,s11,s21,ph", VALIDATION)
+def test_wr90_matches_reference_measurements(wr90, f_ghz, s11, s21, ph):
+    m, pec = wr90
+    a = 0.02286
+    kc, beta, Z0 = O.solve_te10_mode(a, f_ghz * 1e9)
+    ports = [O.build_wave_port_2d(m, tag, kc, Z0, beta, pec, (math.pi / a) ** 2) for tag in (2, 3)]
+    S = O.sparams_eigenmode(m, O.Params(omega=2 * math.pi * f_ghz * 1e9), pec, ports)
+    # printed digits of the reference table
+    assert abs(abs(S[0, 0]) - s11) <= 0.0005 + 1e-9
+    assert abs(abs(S[1, 0]) - s21) <= 0.00005 + 1e-9
+    dph = (np.degrees(np.angle(S[1, 0])) - ph + 180) % 360 - 180
+    assert abs(dph) <= 0.05 + 1e-9
+    assert abs(S[0, 1] - S[1, 0]) < 1e-9                  # reciprocity
+    assert np.linalg.svd(S, compute_uv=False).max() <= 1.0 + 1e-9
+
+
+def test_port_cutoff_converges_to_analytic():
+    a = mg.WR90_A
+    errs = []
+    for nx, ny in ((8, 4), (16, 8), (32, 16)):
+        g = mg.box_mesh(a, mg.WR90_B, 10e-3, nx, ny, 2, seed=0, jitter=0.0)
+        m = O.mesh_from_arrays(g.node_ids, g.xyz, g.tet_ids, g.tet_phys, g.tri_ids, g.tri_phys)
+        pec = O.build_edge_pec(m, 1)
+        _, kc2 = O.solve_port_mode_2d(m, 2, pec, (math.pi / a) ** 2)
+        errs.append(abs(kc2 - (math.pi / a) ** 2) / (math.pi / a) ** 2)
+    assert errs[2] < errs[1] < errs[0] and errs[2] < 2e-3
+    assert errs[1] / errs[2] > 3.0                        # O(h^2)
+
+
+def test_straight_guide_physics():
+    """|S21| ~ 1, |S11| ~ 0, arg S21 ~ -beta L on the generated 16x8x32 guide
+    (thresholds of tests/test_eigenmode_sparams.cpp:116-117)."""
+    g = mg.wr90_guide()
+    m = O.mesh_from_arrays(g.node_ids, g.xyz, g.tet_ids, g.tet_phys, g.tri_ids, g.tri_phys)
+    pec = O.build_edge_pec(m, 1)
+    f = 10e9
+    kc, beta, Z0 = O.solve_te10_mode(mg.WR90_A, f)
+    ports = [O.build_wave_port_2d(m, t, kc, Z0, beta, pec, (math.pi / mg.WR90_A) ** 2) for t in (2, 3)]
+    S = np.load(golden("c1_sparams_10ghz.npz"))["S"]
+    assert abs(S[0, 0]) < 0.1 and abs(S[1, 0]) > 0.95
+    dph = (np.degrees(np.angle(S[1, 0]) + beta * 0.05) + 180) % 360 - 180
+    assert abs(dph) < 10
+    assert abs(S[0, 1] - S[1, 0]) < 1e-9
