@@ -25,6 +25,7 @@ EFM_METHOD_BICGSTAB = 1
 EXPORTS = [
     "efm_last_error", "efm_version",
     "efm_topology_create", "efm_topology_destroy", "efm_topology_get_sizes", "efm_topology_export",
+    "efm_topology_export_nodes", "efm_solver_set_gradient",
     "efm_assemble_km", "efm_assemble_maxwell", "efm_port_surface_mass", "efm_combine",
     "efm_solver_create", "efm_solver_destroy", "efm_solver_set_matrix", "efm_solver_solve",
     "efm_solver_spmv", "efm_solver_profile", "efm_solver_profile_read",
@@ -71,6 +72,8 @@ def load_library(path: str = LIB_PATH) -> C.CDLL:
     lib.efm_topology_destroy.restype = None
     lib.efm_topology_get_sizes.argtypes = [vp, C.POINTER(TopologySizes)]
     lib.efm_topology_export.argtypes = [vp] + [vp] * 7 + [vp]
+    lib.efm_topology_export_nodes.argtypes = [vp, vp, vp, vp, vp]
+    lib.efm_solver_set_gradient.argtypes = [vp, i64, vp, vp, vp, vp, vp]
     lib.efm_assemble_km.argtypes = [vp, vp, vp, vp, vp, vp, vp, vp]
     lib.efm_assemble_maxwell.argtypes = [vp, vp, vp, vp, vp, vp, dbl, vp, vp]
     lib.efm_port_surface_mass.argtypes = [vp, vp, i64, vp, vp, vp, vp]
@@ -174,6 +177,11 @@ class Topology:
                                        dptr(self.tri_sign) if self.n_tris else None,
                                        dptr(self.row_ptr), dptr(self.col_idx), stream_ptr()),
                "efm_topology_export")
+        self.edge_nodes = torch.empty((self.n_edges, 2), dtype=torch.int32, device=dev)
+        self.n2e_ptr = torch.empty(self.n_nodes + 1, dtype=torch.int32, device=dev)
+        self.n2e = torch.empty(2 * self.n_edges, dtype=torch.int32, device=dev)
+        _check(lib.efm_topology_export_nodes(h, dptr(self.edge_nodes), dptr(self.n2e_ptr), dptr(self.n2e),
+                                             stream_ptr()), "efm_topology_export_nodes")
 
     @property
     def handle(self):
@@ -270,6 +278,15 @@ class KrylovSolver:
         _check(lib.efm_solver_create(self.n, self.nnz, dptr(self.row_ptr), dptr(self.col_idx), self.nrhs, vt,
                                      stream_ptr(), C.byref(h)), "efm_solver_create")
         self._h = h
+
+    def set_gradient(self, topo: "Topology", dirichlet=None):
+        """Enable the auxiliary gradient preconditioner for a mesh-backed system (call
+        before set_matrix)."""
+        torch = _torch()
+        self._aux_keep = (topo, None if dirichlet is None else to_dev(dirichlet, torch.uint8))
+        dm = self._aux_keep[1]
+        _check(self._lib.efm_solver_set_gradient(self._h, topo.n_nodes, dptr(topo.edge_nodes), dptr(topo.n2e_ptr),
+                                                 dptr(topo.n2e), dptr(dm), stream_ptr()), "efm_solver_set_gradient")
 
     def set_matrix(self, values):
         torch = _torch()

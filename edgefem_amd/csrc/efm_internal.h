@@ -60,4 +60,7 @@ struct efm_topology {
   efm::DevBuf row_idx;    // int32 [nnz]
   efm::DevBuf cptr;       // int32 [nnz+1] nnz -> contributor codes
   efm::DevBuf contrib;    // int32 [36T]   codes t*36 + i*6 + j, ascending t per nnz
+  efm::DevBuf edge_nidx;  // int32 [2*n_edges] node positions (lower-ID end, higher-ID end)
+  efm::DevBuf n2e_ptr;    // int32 [N+1]   node -> incident edges, ascending
+  efm::DevBuf n2e;        // int32 [2*n_edges]
 };

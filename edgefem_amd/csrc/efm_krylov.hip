@@ -1,21 +1,23 @@
 // Jacobi-preconditioned Krylov solvers on a CSR matrix for a block of independent
 // right-hand sides (replaces solve_linear, src/solver.cpp:35-193).
 //
-//   COCG      complex-symmetric A (A^T = A): the eigenmode/K-M/rank-1 systems.
+//   COCG      complex-symmetric A (A^T = A): the eigenmode/K-M systems.
 //   BiCGSTAB  general A (Bloch-periodic systems lose symmetry).
 //
 // Vectors and all reductions are FP64; matrix values are stored complex FP64 or
-// complex FP32 (promoted on load).  Each iteration of COCG is two fused kernels:
+// complex FP32 (promoted on load).  One COCG iteration is four launches:
 //
-//   spmv_p  : p_new = z + beta p_old (on the fly for every gathered column),
-//             q = A p_new, partial dots p_new^T q -> alpha (last block reduces)
-//   update  : x += alpha p, r -= alpha q, z = D^{-1} r, partial dots r^T z, r^H r
-//             -> beta, convergence flag (last block reduces)
+//   cocg_spmv   p_new = z + beta p_old (on the fly for every gathered column),
+//               q = A p_new; per-block partial sums of p_new^T q
+//   finalize    one block: alpha = rho / sum(partials)
+//   cocg_update x += alpha p, r -= alpha q, z = D^{-1} r; partial sums of r^T z, r^H r
+//   finalize    one block: convergence test, beta, active-system count
 //
-// A fixed grid with grid-stride loops keeps the number of partial sums small; the
-// last block to finish (atomic ticket) reduces them in a fixed order, so results are
-// deterministic.  Converged systems are masked: their vectors stop changing and the
-// host polls a single counter every `check_every` iterations.
+// Partial sums are combined by a separate one-block kernel in a fixed order, so the
+// results are deterministic and no kernel needs a device-scope fence (on gfx950 such
+// a fence writes back the XCD's L2 from every wave that issues it).  Converged systems
+// are masked: their vectors stop changing, and the host polls one counter every
+// `check_every` iterations.
 #include "efm_internal.h"
 
 #include <type_traits>
@@ -31,7 +33,7 @@ constexpr int kRowsPerBlock = kBlock / kLanesPerRow;
 constexpr int kMaxRhs = 8;
 
 struct SysState {
-  double2 rho;      // r^T z (COCG) / r0^T r (BiCGSTAB)
+  double2 rho;      // r^T z (COCG) / r0^H r (BiCGSTAB)
   double2 alpha;
   double2 beta;
   double2 omega;    // BiCGSTAB
@@ -44,20 +46,30 @@ struct SysState {
 };
 
 struct Control {
-  unsigned int ticket;   // last-block election
-  int n_active;          // systems still iterating
+  int n_active;     // systems still iterating
+  int pad;
+};
+
+enum Stage : int {
+  kFinInitCocg = 0,   // partials: [rz(R), bb(R)]
+  kFinInitBs = 1,     // partials: [bb(R)]
+  kFinAlpha = 2,      // partials: [pq(R)]
+  kFinBeta = 3,       // partials: [rz(R), rr(R)]
+  kFinBsRho0 = 4,     // partials: [r0^H r (R)]
+  kFinBsAlpha = 5,    // partials: [r0^H v (R)]
+  kFinBsOmega = 6,    // partials: [t^H s (R), t^H t (R)]
+  kFinBsX = 7,        // partials: [r0^H r (R), r^H r (R)]
 };
 
 __device__ inline double2 bil(double2 a, double2 b) {  // a*b (no conjugate)
   return make_double2(a.x * b.x - a.y * b.y, a.x * b.y + a.y * b.x);
 }
 
-// Block-wide sum of up to kMaxRhs complex values per thread into lds, result in lds[0..R).
-template <int R>
-__device__ inline void block_sum(double2 (&v)[R], double2 *lds) {
-  // wavefront reduction with shuffles, then across the 4 waves through LDS
+// Block-wide sum of V complex values per thread; result in lds[0..V) (all threads).
+template <int V>
+__device__ inline void block_sum(double2 (&v)[V], double2 *lds) {
 #pragma unroll
-  for (int s = 0; s < R; ++s) {
+  for (int s = 0; s < V; ++s) {
 #pragma unroll
     for (int off = 32; off > 0; off >>= 1) {
       v[s].x += __shfl_xor(v[s].x, off, 64);
@@ -67,76 +79,156 @@ __device__ inline void block_sum(double2 (&v)[R], double2 *lds) {
   const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
   if (lane == 0) {
 #pragma unroll
-    for (int s = 0; s < R; ++s)
-      lds[wave * R + s] = v[s];
+    for (int s = 0; s < V; ++s)
+      lds[wave * V + s] = v[s];
   }
   __syncthreads();
   if (threadIdx.x == 0) {
 #pragma unroll
-    for (int s = 0; s < R; ++s) {
+    for (int s = 0; s < V; ++s) {
       double2 a = lds[s];
       for (int w = 1; w < kBlock / 64; ++w)
-        a = cadd(a, lds[w * R + s]);
+        a = cadd(a, lds[w * V + s]);
       lds[s] = a;
     }
   }
   __syncthreads();
 }
 
-// Last-block election; returns true in every thread of the final block.
-__device__ inline bool last_block(unsigned int *ticket) {
-  __shared__ bool am_last;
-  __threadfence();
-  __syncthreads();
-  if (threadIdx.x == 0) {
-    unsigned int prev = atomicAdd(ticket, 1u);
-    am_last = (prev == gridDim.x - 1);
-  }
-  __syncthreads();
-  if (am_last)
-    __threadfence();
-  return am_last;
+template <int V>
+__device__ inline void write_partials(double2 (&v)[V], double2 *lds, double2 *partial) {
+  block_sum<V>(v, lds);
+  if (threadIdx.x < V)
+    partial[(int64_t)blockIdx.x * V + threadIdx.x] = lds[threadIdx.x];
 }
 
-// Deterministic reduction of partial[b * stride + s] over blocks b, by the last block.
-template <int R>
-__device__ inline void reduce_partials(const double2 *partial, int nb, int stride, double2 *lds,
-                                       double2 (&out)[R]) {
-  double2 v[R];
+// ---------------------------------------------------------------------------
+// Finalisation: one block reduces the per-block partials in a fixed order and
+// advances the per-system scalars.
+// ---------------------------------------------------------------------------
+template <int R, int V>
+__device__ inline void reduce_all(const double2 *__restrict__ partial, int nb, double2 *lds,
+                                  double2 (&tot)[V]) {
+  double2 v[V];
 #pragma unroll
-  for (int s = 0; s < R; ++s)
+  for (int s = 0; s < V; ++s)
     v[s] = make_double2(0.0, 0.0);
-  const volatile double *pv = reinterpret_cast<const volatile double *>(partial);
   for (int b = threadIdx.x; b < nb; b += kBlock) {
 #pragma unroll
-    for (int s = 0; s < R; ++s) {
-      int64_t at = ((int64_t)b * stride + s) * 2;
-      v[s].x += pv[at];
-      v[s].y += pv[at + 1];
-    }
+    for (int s = 0; s < V; ++s)
+      v[s] = cadd(v[s], partial[(int64_t)b * V + s]);
   }
-  block_sum<R>(v, lds);
+  block_sum<V>(v, lds);
 #pragma unroll
-  for (int s = 0; s < R; ++s)
-    out[s] = lds[s];
+  for (int s = 0; s < V; ++s)
+    tot[s] = lds[s];
+}
+
+template <int R, int Stage_>
+__global__ void __launch_bounds__(kBlock)
+    k_finalize(const double2 *__restrict__ partial, int nb, SysState *__restrict__ st,
+               Control *__restrict__ ctl, double tol2, int max_it) {
+  constexpr int V = (Stage_ == kFinInitBs || Stage_ == kFinAlpha || Stage_ == kFinBsRho0 ||
+                     Stage_ == kFinBsAlpha)
+                        ? R
+                        : 2 * R;
+  __shared__ double2 lds[(kBlock / 64) * V];
+  double2 tot[V];
+  reduce_all<R, V>(partial, nb, lds, tot);
+  if (threadIdx.x != 0)
+    return;
+  int na = 0;
+  for (int s = 0; s < R; ++s) {
+    SysState &S = st[s];
+    if constexpr (Stage_ == kFinInitCocg || Stage_ == kFinInitBs) {
+      const double bb = (Stage_ == kFinInitCocg) ? tot[R + s].x : tot[s].x;
+      S.bnorm2 = bb;
+      S.rnorm2 = bb;
+      S.rho = (Stage_ == kFinInitCocg) ? tot[s] : make_double2(1, 0);
+      S.alpha = (Stage_ == kFinInitCocg) ? make_double2(0, 0) : make_double2(1, 0);
+      S.beta = make_double2(0, 0);
+      S.omega = make_double2(1, 0);
+      S.iters = 0;
+      S.breakdown = 0;
+      S.active = (bb > 0.0) ? 1 : 0;  // zero right-hand side: x = 0 is exact
+      na += S.active;
+      continue;
+    }
+    if (!S.active)
+      continue;
+    if constexpr (Stage_ == kFinAlpha) {
+      if (tot[s].x == 0.0 && tot[s].y == 0.0) {
+        S.breakdown = 1;
+        S.alpha = make_double2(0, 0);
+      } else {
+        S.alpha = cdiv(S.rho, tot[s]);
+      }
+    } else if constexpr (Stage_ == kFinBeta) {
+      S.iters += 1;
+      S.rnorm2 = tot[R + s].x;
+      const double2 rho_new = tot[s];
+      if (S.rnorm2 <= tol2 * S.bnorm2) {
+        S.active = 0;
+      } else if (S.breakdown || (S.rho.x == 0.0 && S.rho.y == 0.0) || S.iters >= max_it) {
+        S.active = 0;
+      } else {
+        S.beta = cdiv(rho_new, S.rho);
+        S.rho = rho_new;
+      }
+    } else if constexpr (Stage_ == kFinBsRho0) {
+      const double2 rho_i = tot[s];
+      if ((rho_i.x == 0.0 && rho_i.y == 0.0)) {
+        S.breakdown = 1;
+        S.active = 0;
+      } else {
+        S.beta = cmul(cdiv(rho_i, S.rho), cdiv(S.alpha, S.omega));
+        S.rho = rho_i;
+      }
+    } else if constexpr (Stage_ == kFinBsAlpha) {
+      if (tot[s].x == 0.0 && tot[s].y == 0.0) {
+        S.breakdown = 1;
+        S.active = 0;
+      } else {
+        S.alpha = cdiv(S.rho, tot[s]);
+      }
+    } else if constexpr (Stage_ == kFinBsOmega) {
+      const double tt = tot[R + s].x;
+      S.omega = (tt > 0.0) ? cscale(tot[s], 1.0 / tt) : make_double2(0, 0);
+    } else if constexpr (Stage_ == kFinBsX) {
+      S.iters += 1;
+      S.rnorm2 = tot[R + s].x;
+      const double2 rho_i = tot[s];
+      if (S.rnorm2 <= tol2 * S.bnorm2) {
+        S.active = 0;
+      } else if (S.iters >= max_it || (S.omega.x == 0.0 && S.omega.y == 0.0)) {
+        S.active = 0;
+      } else if (rho_i.x == 0.0 && rho_i.y == 0.0) {
+        S.breakdown = 1;
+        S.active = 0;
+      } else {
+        S.beta = cmul(cdiv(rho_i, S.rho), cdiv(S.alpha, S.omega));
+        S.rho = rho_i;
+      }
+    }
+    na += S.active;
+  }
+  ctl->n_active = na;
 }
 
 // ---------------------------------------------------------------------------
-// Initialisation: x = 0, r = b, z = D^{-1} b, p = 0; rho = b^T z, ||b||^2.
+// COCG
 // ---------------------------------------------------------------------------
+// x = 0, r = b, z = D^{-1} b, p = 0; partials rho = b^T z, ||b||^2
 template <int R>
 __global__ void __launch_bounds__(kBlock)
-    k_init(int64_t n, const double2 *__restrict__ b, const double2 *__restrict__ dinv,
-           double2 *__restrict__ x, double2 *__restrict__ r, double2 *__restrict__ z,
-           double2 *__restrict__ p, double2 *__restrict__ partial, SysState *__restrict__ st,
-           Control *__restrict__ ctl, int method, double tol2) {
+    k_cocg_init(int64_t n, const double2 *__restrict__ b, const double2 *__restrict__ dinv,
+                double2 *__restrict__ x, double2 *__restrict__ r, double2 *__restrict__ z,
+                double2 *__restrict__ p, double2 *__restrict__ partial) {
   __shared__ double2 lds[(kBlock / 64) * 2 * R];
-  double2 rz[R], bb[R];
+  double2 v[2 * R];
 #pragma unroll
-  for (int s = 0; s < R; ++s) {
-    rz[s] = make_double2(0, 0);
-    bb[s] = make_double2(0, 0);
-  }
+  for (int s = 0; s < 2 * R; ++s)
+    v[s] = make_double2(0, 0);
   for (int64_t i = (int64_t)blockIdx.x * kBlock + threadIdx.x; i < n; i += (int64_t)gridDim.x * kBlock) {
     double2 di = dinv[i];
 #pragma unroll
@@ -148,57 +240,21 @@ __global__ void __launch_bounds__(kBlock)
       r[k] = bv;
       z[k] = zv;
       p[k] = make_double2(0, 0);
-      rz[s] = cadd(rz[s], bil(bv, zv));
-      bb[s].x += cabs2(bv);
+      v[s] = cadd(v[s], bil(bv, zv));
+      v[R + s].x += cabs2(bv);
     }
   }
-  double2 v[2 * R];
-#pragma unroll
-  for (int s = 0; s < R; ++s) {
-    v[s] = rz[s];
-    v[R + s] = bb[s];
-  }
-  block_sum<2 * R>(v, lds);
-  if (threadIdx.x == 0) {
-#pragma unroll
-    for (int s = 0; s < 2 * R; ++s)
-      partial[(int64_t)blockIdx.x * 2 * R + s] = lds[s];
-  }
-  if (!last_block(&ctl->ticket))
-    return;
-  double2 tot[2 * R];
-  reduce_partials<2 * R>(partial, gridDim.x, 2 * R, lds, tot);
-  if (threadIdx.x == 0) {
-    int na = 0;
-    for (int s = 0; s < R; ++s) {
-      SysState &S = st[s];
-      S.bnorm2 = tot[R + s].x;
-      S.rnorm2 = tot[R + s].x;
-      S.rho = tot[s];
-      S.alpha = make_double2(0, 0);
-      S.beta = make_double2(0, 0);
-      S.omega = make_double2(1, 0);
-      S.iters = 0;
-      S.breakdown = 0;
-      // zero right-hand side: x = 0 is the exact solution
-      S.active = (S.bnorm2 > 0.0) ? 1 : 0;
-      na += S.active;
-    }
-    ctl->n_active = na;
-    ctl->ticket = 0;
-  }
+  write_partials<2 * R>(v, lds, partial);
 }
 
-// ---------------------------------------------------------------------------
-// COCG step 1: p_new = z + beta p_old, q = A p_new, alpha = rho / (p_new^T q)
-// ---------------------------------------------------------------------------
+// p_new = z + beta p_old; q = A p_new; partials p_new^T q
 template <typename VT, int R>
 __global__ void __launch_bounds__(kBlock)
     k_cocg_spmv(int64_t n, const int32_t *__restrict__ row_ptr, const int32_t *__restrict__ col,
                 const VT *__restrict__ val, const double2 *__restrict__ z,
                 const double2 *__restrict__ p_old, double2 *__restrict__ p_new,
-                double2 *__restrict__ q, double2 *__restrict__ partial, SysState *__restrict__ st,
-                Control *__restrict__ ctl) {
+                double2 *__restrict__ q, double2 *__restrict__ partial,
+                const SysState *__restrict__ st) {
   __shared__ double2 lds[(kBlock / 64) * R];
   bool act[R];
   double2 beta[R];
@@ -207,7 +263,7 @@ __global__ void __launch_bounds__(kBlock)
     act[s] = st[s].active != 0;
     beta[s] = st[s].beta;
   }
-  const int g = threadIdx.x / kLanesPerRow;       // row slot in block
+  const int g = threadIdx.x / kLanesPerRow;
   const int lane = threadIdx.x % kLanesPerRow;
   double2 pq[R];
 #pragma unroll
@@ -215,19 +271,19 @@ __global__ void __launch_bounds__(kBlock)
     pq[s] = make_double2(0, 0);
   for (int64_t row0 = (int64_t)blockIdx.x * kRowsPerBlock; row0 < n;
        row0 += (int64_t)gridDim.x * kRowsPerBlock) {
-    int64_t row = row0 + g;
+    const int64_t row = row0 + g;
     double2 acc[R];
 #pragma unroll
     for (int s = 0; s < R; ++s)
       acc[s] = make_double2(0, 0);
     if (row < n) {
-      int32_t k0 = row_ptr[row], k1 = row_ptr[row + 1];
+      const int32_t k0 = row_ptr[row], k1 = row_ptr[row + 1];
       for (int32_t k = k0 + lane; k < k1; k += kLanesPerRow) {
-        double2 a = to_d2(val[k]);
-        int64_t c = col[k];
+        const double2 a = to_d2(val[k]);
+        const int64_t c = col[k];
 #pragma unroll
         for (int s = 0; s < R; ++s) {
-          double2 pc = cfma(beta[s], p_old[c * R + s], z[c * R + s]);
+          const double2 pc = cfma(beta[s], p_old[c * R + s], z[c * R + s]);
           acc[s] = cfma(a, pc, acc[s]);
         }
       }
@@ -245,49 +301,24 @@ __global__ void __launch_bounds__(kBlock)
       for (int s = 0; s < R; ++s) {
         if (!act[s])
           continue;
-        int64_t k = row * R + s;
-        double2 pn = cfma(beta[s], p_old[k], z[k]);
+        const int64_t k = row * R + s;
+        const double2 pn = cfma(beta[s], p_old[k], z[k]);
         p_new[k] = pn;
         q[k] = acc[s];
         pq[s] = cadd(pq[s], bil(pn, acc[s]));
       }
     }
   }
-  block_sum<R>(pq, lds);
-  if (threadIdx.x == 0) {
-#pragma unroll
-    for (int s = 0; s < R; ++s)
-      partial[(int64_t)blockIdx.x * R + s] = lds[s];
-  }
-  if (!last_block(&ctl->ticket))
-    return;
-  double2 tot[R];
-  reduce_partials<R>(partial, gridDim.x, R, lds, tot);
-  if (threadIdx.x == 0) {
-    for (int s = 0; s < R; ++s) {
-      SysState &S = st[s];
-      if (!S.active)
-        continue;
-      if (tot[s].x == 0.0 && tot[s].y == 0.0) {
-        S.breakdown = 1;
-        S.alpha = make_double2(0, 0);
-      } else {
-        S.alpha = cdiv(S.rho, tot[s]);
-      }
-    }
-    ctl->ticket = 0;
-  }
+  write_partials<R>(pq, lds, partial);
 }
 
-// ---------------------------------------------------------------------------
-// COCG step 2: x += alpha p, r -= alpha q, z = D^{-1} r; beta = (r^T z)_new / rho
-// ---------------------------------------------------------------------------
+// x += alpha p, r -= alpha q, z = D^{-1} r; partials r^T z, r^H r
 template <int R>
 __global__ void __launch_bounds__(kBlock)
     k_cocg_update(int64_t n, const double2 *__restrict__ dinv, const double2 *__restrict__ p,
                   const double2 *__restrict__ q, double2 *__restrict__ x, double2 *__restrict__ r,
-                  double2 *__restrict__ z, double2 *__restrict__ partial, SysState *__restrict__ st,
-                  Control *__restrict__ ctl, double tol2, int max_it) {
+                  double2 *__restrict__ z, double2 *__restrict__ partial,
+                  const SysState *__restrict__ st) {
   __shared__ double2 lds[(kBlock / 64) * 2 * R];
   bool act[R];
   double2 alpha[R];
@@ -296,68 +327,28 @@ __global__ void __launch_bounds__(kBlock)
     act[s] = st[s].active != 0;
     alpha[s] = st[s].alpha;
   }
-  double2 rz[R], rr[R];
+  double2 v[2 * R];
 #pragma unroll
-  for (int s = 0; s < R; ++s) {
-    rz[s] = make_double2(0, 0);
-    rr[s] = make_double2(0, 0);
-  }
+  for (int s = 0; s < 2 * R; ++s)
+    v[s] = make_double2(0, 0);
   for (int64_t i = (int64_t)blockIdx.x * kBlock + threadIdx.x; i < n; i += (int64_t)gridDim.x * kBlock) {
-    double2 di = dinv[i];
+    const double2 di = dinv[i];
 #pragma unroll
     for (int s = 0; s < R; ++s) {
       if (!act[s])
         continue;
-      int64_t k = i * R + s;
-      double2 xv = cfma(alpha[s], p[k], x[k]);
-      double2 aq = bil(alpha[s], q[k]);
-      double2 rv = csub(r[k], aq);
-      double2 zv = bil(di, rv);
+      const int64_t k = i * R + s;
+      const double2 xv = cfma(alpha[s], p[k], x[k]);
+      const double2 rv = csub(r[k], bil(alpha[s], q[k]));
+      const double2 zv = bil(di, rv);
       x[k] = xv;
       r[k] = rv;
       z[k] = zv;
-      rz[s] = cadd(rz[s], bil(rv, zv));
-      rr[s].x += cabs2(rv);
+      v[s] = cadd(v[s], bil(rv, zv));
+      v[R + s].x += cabs2(rv);
     }
   }
-  double2 v[2 * R];
-#pragma unroll
-  for (int s = 0; s < R; ++s) {
-    v[s] = rz[s];
-    v[R + s] = rr[s];
-  }
-  block_sum<2 * R>(v, lds);
-  if (threadIdx.x == 0) {
-#pragma unroll
-    for (int s = 0; s < 2 * R; ++s)
-      partial[(int64_t)blockIdx.x * 2 * R + s] = lds[s];
-  }
-  if (!last_block(&ctl->ticket))
-    return;
-  double2 tot[2 * R];
-  reduce_partials<2 * R>(partial, gridDim.x, 2 * R, lds, tot);
-  if (threadIdx.x == 0) {
-    int na = 0;
-    for (int s = 0; s < R; ++s) {
-      SysState &S = st[s];
-      if (!S.active)
-        continue;
-      S.iters += 1;
-      S.rnorm2 = tot[R + s].x;
-      double2 rho_new = tot[s];
-      if (S.rnorm2 <= tol2 * S.bnorm2) {
-        S.active = 0;
-      } else if (S.breakdown || (S.rho.x == 0.0 && S.rho.y == 0.0) || S.iters >= max_it) {
-        S.active = 0;
-      } else {
-        S.beta = cdiv(rho_new, S.rho);
-        S.rho = rho_new;
-        ++na;
-      }
-    }
-    ctl->n_active = na;
-    ctl->ticket = 0;
-  }
+  write_partials<2 * R>(v, lds, partial);
 }
 
 // ---------------------------------------------------------------------------
@@ -365,15 +356,12 @@ __global__ void __launch_bounds__(kBlock)
 //   rho_i = r0^H r; beta = (rho_i/rho_{i-1})(alpha/omega); p = r + beta (p - omega v)
 //   y = D^{-1} p; v = A y; alpha = rho_i / (r0^H v); s = r - alpha v
 //   zz = D^{-1} s; t = A zz; omega = (t^H s)/(t^H t); x += alpha y + omega zz; r = s - omega t
-// Kernels: bs_p (p update, y), spmv_v (v = A y, r0^T v), bs_s (s, zz), spmv_t (t = A zz,
-// t^H s, t^H t), bs_x (x, r, r0^T r, r^H r).  Same masking and reductions as COCG.
 // ---------------------------------------------------------------------------
 template <int R>
 __global__ void __launch_bounds__(kBlock)
     k_bs_init(int64_t n, const double2 *__restrict__ b, double2 *__restrict__ x,
               double2 *__restrict__ r, double2 *__restrict__ r0, double2 *__restrict__ p,
-              double2 *__restrict__ v, double2 *__restrict__ partial, SysState *__restrict__ st,
-              Control *__restrict__ ctl) {
+              double2 *__restrict__ v, double2 *__restrict__ partial) {
   __shared__ double2 lds[(kBlock / 64) * R];
   double2 bb[R];
 #pragma unroll
@@ -392,42 +380,13 @@ __global__ void __launch_bounds__(kBlock)
       bb[s].x += cabs2(bv);
     }
   }
-  block_sum<R>(bb, lds);
-  if (threadIdx.x == 0) {
-#pragma unroll
-    for (int s = 0; s < R; ++s)
-      partial[(int64_t)blockIdx.x * R + s] = lds[s];
-  }
-  if (!last_block(&ctl->ticket))
-    return;
-  double2 tot[R];
-  reduce_partials<R>(partial, gridDim.x, R, lds, tot);
-  if (threadIdx.x == 0) {
-    int na = 0;
-    for (int s = 0; s < R; ++s) {
-      SysState &S = st[s];
-      S.bnorm2 = tot[s].x;
-      S.rnorm2 = tot[s].x;
-      // r0 = b, so r0^T r = b^T b (bilinear)
-      S.rho = make_double2(1, 0);
-      S.alpha = make_double2(1, 0);
-      S.omega = make_double2(1, 0);
-      S.beta = make_double2(0, 0);
-      S.iters = 0;
-      S.breakdown = 0;
-      S.active = (S.bnorm2 > 0.0) ? 1 : 0;
-      na += S.active;
-    }
-    ctl->n_active = na;
-    ctl->ticket = 0;
-  }
+  write_partials<R>(bb, lds, partial);
 }
 
-// rho_1 = r0^H r for the first BiCGSTAB step; later steps get rho_i from bs_x.
 template <int R>
 __global__ void __launch_bounds__(kBlock)
     k_bs_dot_r0r(int64_t n, const double2 *__restrict__ r0, const double2 *__restrict__ r,
-                 double2 *__restrict__ partial, SysState *__restrict__ st, Control *__restrict__ ctl) {
+                 double2 *__restrict__ partial) {
   __shared__ double2 lds[(kBlock / 64) * R];
   double2 d[R];
 #pragma unroll
@@ -440,33 +399,7 @@ __global__ void __launch_bounds__(kBlock)
       d[s] = cadd(d[s], bil(cconj(r0[k]), r[k]));
     }
   }
-  block_sum<R>(d, lds);
-  if (threadIdx.x == 0) {
-#pragma unroll
-    for (int s = 0; s < R; ++s)
-      partial[(int64_t)blockIdx.x * R + s] = lds[s];
-  }
-  if (!last_block(&ctl->ticket))
-    return;
-  double2 tot[R];
-  reduce_partials<R>(partial, gridDim.x, R, lds, tot);
-  if (threadIdx.x == 0) {
-    for (int s = 0; s < R; ++s) {
-      SysState &S = st[s];
-      if (!S.active)
-        continue;
-      // stash rho_i in beta's slot for bs_p: beta = (rho_i/rho)(alpha/omega)
-      double2 rho_i = tot[s];
-      if ((rho_i.x == 0.0 && rho_i.y == 0.0) || (S.omega.x == 0.0 && S.omega.y == 0.0)) {
-        S.breakdown = 1;
-        S.active = 0;
-        continue;
-      }
-      S.beta = cmul(cdiv(rho_i, S.rho), cdiv(S.alpha, S.omega));
-      S.rho = rho_i;
-    }
-    ctl->ticket = 0;
-  }
+  write_partials<R>(d, lds, partial);
 }
 
 // p = r + beta (p - omega v); y = D^{-1} p
@@ -497,38 +430,37 @@ __global__ void __launch_bounds__(kBlock)
   }
 }
 
-// out = A in, with partial dots: mode 0 -> (r0^H out); mode 1 -> (out^H s, out^H out)
-template <typename VT, int R>
+// out = A in, partial dots: mode 0 -> (r0^H out); mode 1 -> (out^H s, out^H out)
+template <typename VT, int R, int MODE>
 __global__ void __launch_bounds__(kBlock)
     k_bs_spmv(int64_t n, const int32_t *__restrict__ row_ptr, const int32_t *__restrict__ col,
               const VT *__restrict__ val, const double2 *__restrict__ in, double2 *__restrict__ out,
-              const double2 *__restrict__ other, int mode, double2 *__restrict__ partial,
-              SysState *__restrict__ st, Control *__restrict__ ctl) {
-  __shared__ double2 lds[(kBlock / 64) * 2 * R];
+              const double2 *__restrict__ other, double2 *__restrict__ partial,
+              const SysState *__restrict__ st) {
+  constexpr int V = MODE == 0 ? R : 2 * R;
+  __shared__ double2 lds[(kBlock / 64) * V];
   bool act[R];
 #pragma unroll
   for (int s = 0; s < R; ++s)
     act[s] = st[s].active != 0;
   const int g = threadIdx.x / kLanesPerRow;
   const int lane = threadIdx.x % kLanesPerRow;
-  double2 d1[R], d2[R];
+  double2 d[V];
 #pragma unroll
-  for (int s = 0; s < R; ++s) {
-    d1[s] = make_double2(0, 0);
-    d2[s] = make_double2(0, 0);
-  }
+  for (int s = 0; s < V; ++s)
+    d[s] = make_double2(0, 0);
   for (int64_t row0 = (int64_t)blockIdx.x * kRowsPerBlock; row0 < n;
        row0 += (int64_t)gridDim.x * kRowsPerBlock) {
-    int64_t row = row0 + g;
+    const int64_t row = row0 + g;
     double2 acc[R];
 #pragma unroll
     for (int s = 0; s < R; ++s)
       acc[s] = make_double2(0, 0);
     if (row < n) {
-      int32_t k0 = row_ptr[row], k1 = row_ptr[row + 1];
+      const int32_t k0 = row_ptr[row], k1 = row_ptr[row + 1];
       for (int32_t k = k0 + lane; k < k1; k += kLanesPerRow) {
-        double2 a = to_d2(val[k]);
-        int64_t c = col[k];
+        const double2 a = to_d2(val[k]);
+        const int64_t c = col[k];
 #pragma unroll
         for (int s = 0; s < R; ++s)
           acc[s] = cfma(a, in[c * R + s], acc[s]);
@@ -547,53 +479,19 @@ __global__ void __launch_bounds__(kBlock)
       for (int s = 0; s < R; ++s) {
         if (!act[s])
           continue;
-        int64_t k = row * R + s;
+        const int64_t k = row * R + s;
         out[k] = acc[s];
-        double2 o = other[k];
-        if (mode == 0) {
-          d1[s] = cadd(d1[s], bil(cconj(o), acc[s]));
+        const double2 o = other[k];
+        if constexpr (MODE == 0) {
+          d[s] = cadd(d[s], bil(cconj(o), acc[s]));
         } else {
-          d1[s] = cadd(d1[s], bil(cconj(acc[s]), o));
-          d2[s].x += cabs2(acc[s]);
+          d[s] = cadd(d[s], bil(cconj(acc[s]), o));
+          d[R + s].x += cabs2(acc[s]);
         }
       }
     }
   }
-  double2 v[2 * R];
-#pragma unroll
-  for (int s = 0; s < R; ++s) {
-    v[s] = d1[s];
-    v[R + s] = d2[s];
-  }
-  block_sum<2 * R>(v, lds);
-  if (threadIdx.x == 0) {
-#pragma unroll
-    for (int s = 0; s < 2 * R; ++s)
-      partial[(int64_t)blockIdx.x * 2 * R + s] = lds[s];
-  }
-  if (!last_block(&ctl->ticket))
-    return;
-  double2 tot[2 * R];
-  reduce_partials<2 * R>(partial, gridDim.x, 2 * R, lds, tot);
-  if (threadIdx.x == 0) {
-    for (int s = 0; s < R; ++s) {
-      SysState &S = st[s];
-      if (!S.active)
-        continue;
-      if (mode == 0) {
-        if (tot[s].x == 0.0 && tot[s].y == 0.0) {
-          S.breakdown = 1;
-          S.active = 0;
-          continue;
-        }
-        S.alpha = cdiv(S.rho, tot[s]);
-      } else {
-        double tt = tot[R + s].x;
-        S.omega = (tt > 0.0) ? cscale(tot[s], 1.0 / tt) : make_double2(0, 0);
-      }
-    }
-    ctl->ticket = 0;
-  }
+  write_partials<V>(d, lds, partial);
 }
 
 // s = r - alpha v (stored in r); zz = D^{-1} s
@@ -622,13 +520,12 @@ __global__ void __launch_bounds__(kBlock)
   }
 }
 
-// x += alpha y + omega zz; r = s - omega t; rho_new = r0^H r; ||r||^2
+// x += alpha y + omega zz; r = s - omega t; partials r0^H r, ||r||^2
 template <int R>
 __global__ void __launch_bounds__(kBlock)
     k_bs_x(int64_t n, const double2 *__restrict__ y, const double2 *__restrict__ zz,
            const double2 *__restrict__ t, const double2 *__restrict__ r0, double2 *__restrict__ x,
-           double2 *__restrict__ r, double2 *__restrict__ partial, SysState *__restrict__ st,
-           Control *__restrict__ ctl, double tol2, int max_it) {
+           double2 *__restrict__ r, double2 *__restrict__ partial, const SysState *__restrict__ st) {
   __shared__ double2 lds[(kBlock / 64) * 2 * R];
   bool act[R];
   double2 alpha[R], omega[R];
@@ -638,12 +535,10 @@ __global__ void __launch_bounds__(kBlock)
     alpha[s] = st[s].alpha;
     omega[s] = st[s].omega;
   }
-  double2 d1[R], d2[R];
+  double2 d[2 * R];
 #pragma unroll
-  for (int s = 0; s < R; ++s) {
-    d1[s] = make_double2(0, 0);
-    d2[s] = make_double2(0, 0);
-  }
+  for (int s = 0; s < 2 * R; ++s)
+    d[s] = make_double2(0, 0);
   for (int64_t i = (int64_t)blockIdx.x * kBlock + threadIdx.x; i < n; i += (int64_t)gridDim.x * kBlock) {
 #pragma unroll
     for (int s = 0; s < R; ++s) {
@@ -654,55 +549,11 @@ __global__ void __launch_bounds__(kBlock)
       double2 rv = csub(r[k], bil(omega[s], t[k]));
       x[k] = xv;
       r[k] = rv;
-      d1[s] = cadd(d1[s], bil(cconj(r0[k]), rv));
-      d2[s].x += cabs2(rv);
+      d[s] = cadd(d[s], bil(cconj(r0[k]), rv));
+      d[R + s].x += cabs2(rv);
     }
   }
-  double2 v[2 * R];
-#pragma unroll
-  for (int s = 0; s < R; ++s) {
-    v[s] = d1[s];
-    v[R + s] = d2[s];
-  }
-  block_sum<2 * R>(v, lds);
-  if (threadIdx.x == 0) {
-#pragma unroll
-    for (int s = 0; s < 2 * R; ++s)
-      partial[(int64_t)blockIdx.x * 2 * R + s] = lds[s];
-  }
-  if (!last_block(&ctl->ticket))
-    return;
-  double2 tot[2 * R];
-  reduce_partials<2 * R>(partial, gridDim.x, 2 * R, lds, tot);
-  if (threadIdx.x == 0) {
-    int na = 0;
-    for (int s = 0; s < R; ++s) {
-      SysState &S = st[s];
-      if (!S.active)
-        continue;
-      S.iters += 1;
-      S.rnorm2 = tot[R + s].x;
-      if (S.rnorm2 <= tol2 * S.bnorm2) {
-        S.active = 0;
-        continue;
-      }
-      if (S.iters >= max_it || (S.omega.x == 0.0 && S.omega.y == 0.0)) {
-        S.active = 0;
-        continue;
-      }
-      double2 rho_i = tot[s];
-      if (rho_i.x == 0.0 && rho_i.y == 0.0) {
-        S.breakdown = 1;
-        S.active = 0;
-        continue;
-      }
-      S.beta = cmul(cdiv(rho_i, S.rho), cdiv(S.alpha, S.omega));
-      S.rho = rho_i;
-      ++na;
-    }
-    ctl->n_active = na;
-    ctl->ticket = 0;
-  }
+  write_partials<2 * R>(d, lds, partial);
 }
 
 // Jacobi inverse diagonal (1/a_rr; 1 where the diagonal is missing or zero).
@@ -727,16 +578,16 @@ __global__ void __launch_bounds__(kBlock)
   const int lane = threadIdx.x % kLanesPerRow;
   for (int64_t row0 = (int64_t)blockIdx.x * kRowsPerBlock; row0 < n;
        row0 += (int64_t)gridDim.x * kRowsPerBlock) {
-    int64_t row = row0 + g;
+    const int64_t row = row0 + g;
     double2 acc[R];
 #pragma unroll
     for (int s = 0; s < R; ++s)
       acc[s] = make_double2(0, 0);
     if (row < n) {
-      int32_t k0 = row_ptr[row], k1 = row_ptr[row + 1];
+      const int32_t k0 = row_ptr[row], k1 = row_ptr[row + 1];
       for (int32_t k = k0 + lane; k < k1; k += kLanesPerRow) {
-        double2 a = to_d2(val[k]);
-        int64_t c = col[k];
+        const double2 a = to_d2(val[k]);
+        const int64_t c = col[k];
 #pragma unroll
         for (int s = 0; s < R; ++s)
           acc[s] = cfma(a, x[c * R + s], acc[s]);
@@ -758,6 +609,203 @@ __global__ void __launch_bounds__(kBlock)
   }
 }
 
+
+// ---------------------------------------------------------------------------
+// Auxiliary-space (discrete gradient) preconditioner
+//   P r = D^{-1} r + G Dn^{-1} G^T r,   Dn = diag(G^T A G)
+// G is the edge-node incidence of the discrete gradient (-1 at the lower-ID end,
+// +1 at the higher-ID end), with Dirichlet edges removed.  P is complex symmetric,
+// so COCG stays applicable.  The curl-free error components, on which the edge
+// Jacobi step acts with the wrong scale (A ~ -k0^2 M there, diag(A) ~ K), are
+// corrected on the nodes.
+// ---------------------------------------------------------------------------
+__device__ inline int grad_sign(const int32_t *__restrict__ en, int32_t e, int32_t node) {
+  const int32_t a = en[2 * (int64_t)e], b = en[2 * (int64_t)e + 1];
+  if (a == b)
+    return 0;
+  return (node == b) ? 1 : ((node == a) ? -1 : 0);
+}
+
+template <typename VT>
+__global__ void k_aux_diag(int64_t N, const int32_t *__restrict__ n2e_ptr,
+                           const int32_t *__restrict__ n2e, const int32_t *__restrict__ en,
+                           const uint8_t *__restrict__ dir, const int32_t *__restrict__ row_ptr,
+                           const int32_t *__restrict__ col, const VT *__restrict__ val,
+                           double2 *__restrict__ dninv) {
+  int64_t nd = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (nd >= N)
+    return;
+  double2 d = make_double2(0, 0);
+  for (int32_t k = n2e_ptr[nd]; k < n2e_ptr[nd + 1]; ++k) {
+    const int32_t e = n2e[k];
+    if (dir && dir[e])
+      continue;
+    const int ge = grad_sign(en, e, (int32_t)nd);
+    if (!ge)
+      continue;
+    for (int32_t q = row_ptr[e]; q < row_ptr[e + 1]; ++q) {
+      const int32_t f = col[q];
+      if (dir && dir[f])
+        continue;
+      const int gf = grad_sign(en, f, (int32_t)nd);
+      if (!gf)
+        continue;
+      const double2 a = to_d2(val[q]);
+      const double sg = (double)(ge * gf);
+      d.x += sg * a.x;
+      d.y += sg * a.y;
+    }
+  }
+  dninv[nd] = (d.x == 0.0 && d.y == 0.0) ? make_double2(0, 0) : cdiv(make_double2(1, 0), d);
+}
+
+// y[n] = Dn^{-1} sum_e G[e,n] (r[e] - alpha q[e])   (alpha, q only when FUSED)
+template <int R, bool FUSED>
+__global__ void __launch_bounds__(kBlock)
+    k_aux_node(int64_t N, const int32_t *__restrict__ n2e_ptr, const int32_t *__restrict__ n2e,
+               const int32_t *__restrict__ en, const uint8_t *__restrict__ dir,
+               const double2 *__restrict__ dninv, const double2 *__restrict__ r,
+               const double2 *__restrict__ q, double2 *__restrict__ y,
+               const SysState *__restrict__ st) {
+  bool act[R];
+  double2 alpha[R];
+#pragma unroll
+  for (int s = 0; s < R; ++s) {
+    act[s] = st == nullptr || st[s].active != 0;
+    alpha[s] = FUSED ? st[s].alpha : make_double2(0, 0);
+  }
+  for (int64_t nd = (int64_t)blockIdx.x * kBlock + threadIdx.x; nd < N;
+       nd += (int64_t)gridDim.x * kBlock) {
+    double2 acc[R];
+#pragma unroll
+    for (int s = 0; s < R; ++s)
+      acc[s] = make_double2(0, 0);
+    for (int32_t k = n2e_ptr[nd]; k < n2e_ptr[nd + 1]; ++k) {
+      const int32_t e = n2e[k];
+      if (dir && dir[e])
+        continue;
+      const int g = grad_sign(en, e, (int32_t)nd);
+      if (!g)
+        continue;
+      const double sg = (double)g;
+#pragma unroll
+      for (int s = 0; s < R; ++s) {
+        const int64_t i = (int64_t)e * R + s;
+        double2 rv = r[i];
+        if (FUSED)
+          rv = csub(rv, bil(alpha[s], q[i]));
+        acc[s].x += sg * rv.x;
+        acc[s].y += sg * rv.y;
+      }
+    }
+    const double2 di = dninv[nd];
+#pragma unroll
+    for (int s = 0; s < R; ++s)
+      if (act[s])
+        y[nd * R + s] = bil(di, acc[s]);
+  }
+}
+
+// COCG update with the auxiliary preconditioner:
+//   r <- r - alpha q; x <- x + alpha p; z = D^{-1} r + (y[head] - y[tail]);
+//   partials r^T z, r^H r
+template <int R>
+__global__ void __launch_bounds__(kBlock)
+    k_aux_cocg_update(int64_t n, const int32_t *__restrict__ en, const uint8_t *__restrict__ dir,
+                      const double2 *__restrict__ dinv, const double2 *__restrict__ y,
+                      const double2 *__restrict__ p, const double2 *__restrict__ q,
+                      double2 *__restrict__ x, double2 *__restrict__ r, double2 *__restrict__ z,
+                      double2 *__restrict__ partial, const SysState *__restrict__ st) {
+  __shared__ double2 lds[(kBlock / 64) * 2 * R];
+  bool act[R];
+  double2 alpha[R];
+#pragma unroll
+  for (int s = 0; s < R; ++s) {
+    act[s] = st[s].active != 0;
+    alpha[s] = st[s].alpha;
+  }
+  double2 v[2 * R];
+#pragma unroll
+  for (int s = 0; s < 2 * R; ++s)
+    v[s] = make_double2(0, 0);
+  for (int64_t e = (int64_t)blockIdx.x * kBlock + threadIdx.x; e < n; e += (int64_t)gridDim.x * kBlock) {
+    const double2 di = dinv[e];
+    const bool dd = dir && dir[e];
+    const int32_t a = en[2 * e], b = en[2 * e + 1];
+#pragma unroll
+    for (int s = 0; s < R; ++s) {
+      if (!act[s])
+        continue;
+      const int64_t k = e * R + s;
+      const double2 xv = cfma(alpha[s], p[k], x[k]);
+      const double2 rv = csub(r[k], bil(alpha[s], q[k]));
+      double2 zv = bil(di, rv);
+      if (!dd && a != b)
+        zv = cadd(zv, csub(y[(int64_t)b * R + s], y[(int64_t)a * R + s]));
+      x[k] = xv;
+      r[k] = rv;
+      z[k] = zv;
+      v[s] = cadd(v[s], bil(rv, zv));
+      v[R + s].x += cabs2(rv);
+    }
+  }
+  write_partials<2 * R>(v, lds, partial);
+}
+
+// out = D^{-1} in + G y (plain application, for the initial residual and BiCGSTAB);
+// with PARTIALS also the partials in^T out and in^H in.
+template <int R, bool PARTIALS>
+__global__ void __launch_bounds__(kBlock)
+    k_aux_apply_edge(int64_t n, const int32_t *__restrict__ en, const uint8_t *__restrict__ dir,
+                     const double2 *__restrict__ dinv, const double2 *__restrict__ y,
+                     const double2 *__restrict__ in, double2 *__restrict__ out,
+                     double2 *__restrict__ partial, const SysState *__restrict__ st) {
+  __shared__ double2 lds[(kBlock / 64) * 2 * R];
+  bool act[R];
+#pragma unroll
+  for (int s = 0; s < R; ++s)
+    act[s] = st == nullptr || st[s].active != 0;
+  double2 v[2 * R];
+#pragma unroll
+  for (int s = 0; s < 2 * R; ++s)
+    v[s] = make_double2(0, 0);
+  for (int64_t e = (int64_t)blockIdx.x * kBlock + threadIdx.x; e < n; e += (int64_t)gridDim.x * kBlock) {
+    const double2 di = dinv[e];
+    const bool dd = dir && dir[e];
+    const int32_t a = en[2 * e], b = en[2 * e + 1];
+#pragma unroll
+    for (int s = 0; s < R; ++s) {
+      if (!act[s])
+        continue;
+      const int64_t k = e * R + s;
+      const double2 iv = in[k];
+      double2 o = bil(di, iv);
+      if (!dd && a != b)
+        o = cadd(o, csub(y[(int64_t)b * R + s], y[(int64_t)a * R + s]));
+      out[k] = o;
+      if (PARTIALS) {
+        v[s] = cadd(v[s], bil(iv, o));
+        v[R + s].x += cabs2(iv);
+      }
+    }
+  }
+  if (PARTIALS)
+    write_partials<2 * R>(v, lds, partial);
+}
+
+// x = 0, r = b, p = 0 (the preconditioned residual comes from the aux kernels)
+template <int R>
+__global__ void __launch_bounds__(kBlock)
+    k_cocg_init_aux(int64_t n, const double2 *__restrict__ b, double2 *__restrict__ x,
+                    double2 *__restrict__ r, double2 *__restrict__ p) {
+  for (int64_t i = (int64_t)blockIdx.x * kBlock + threadIdx.x; i < n * R;
+       i += (int64_t)gridDim.x * kBlock) {
+    x[i] = make_double2(0, 0);
+    r[i] = b[i];
+    p[i] = make_double2(0, 0);
+  }
+}
+
 } // namespace
 
 struct efm_solver {
@@ -766,8 +814,16 @@ struct efm_solver {
   const int32_t *col = nullptr;
   int R = 1;
   int vtype = EFM_C128;
-  int grid = 1024;
+  int grid = 1024;       // streaming kernels (grid-stride)
+  int grid_spmv = 1024;  // row-group kernels (grid-stride)
   efm::DevBuf val, dinv, x, r, z, p0, p1, q, r0, v, y, zz, t, partial, state, control;
+  // auxiliary gradient preconditioner (non-owning mesh arrays)
+  bool aux = false;
+  int64_t n_nodes = 0;
+  int grid_nodes = 1;
+  const int32_t *edge_nodes = nullptr, *n2e_ptr = nullptr, *n2e = nullptr;
+  const uint8_t *dir = nullptr;
+  efm::DevBuf dninv, ynode;
   bool profile = false;
   double spmv_ms = 0.0;
   int64_t spmv_launches = 0;
@@ -781,19 +837,51 @@ struct efm_solver {
 namespace {
 
 template <int R> struct Dispatch {
-  static int init(efm_solver *s, const double2 *b, hipStream_t st, int method, double tol2) {
-    if (method == EFM_METHOD_COCG)
-      hipLaunchKernelGGL(k_init<R>, dim3(s->grid), dim3(kBlock), 0, st, s->n, b, s->dinv.as<double2>(),
-                         s->x.as<double2>(), s->r.as<double2>(), s->z.as<double2>(),
-                         s->p0.as<double2>(), s->partial.as<double2>(), s->state.as<SysState>(),
-                         s->control.as<Control>(), method, tol2);
-    else
-      hipLaunchKernelGGL(k_bs_init<R>, dim3(s->grid), dim3(kBlock), 0, st, s->n, b,
-                         s->x.as<double2>(), s->r.as<double2>(), s->r0.as<double2>(),
-                         s->p0.as<double2>(), s->v.as<double2>(), s->partial.as<double2>(),
-                         s->state.as<SysState>(), s->control.as<Control>());
+  template <int Stage_>
+  static int fin(efm_solver *s, int nb, hipStream_t st, double tol2, int max_it) {
+    hipLaunchKernelGGL((k_finalize<R, Stage_>), dim3(1), dim3(kBlock), 0, st, s->partial.as<double2>(),
+                       nb, s->state.as<SysState>(), s->control.as<Control>(), tol2, max_it);
     EFM_LAUNCH_CHECK();
     return EFM_OK;
+  }
+
+  // y = Dn^{-1} G^T in (nodes), then out = D^{-1} in + G y; partials when asked
+  template <bool PARTIALS>
+  static int aux_apply(efm_solver *s, const double2 *in, double2 *out, hipStream_t st,
+                       const SysState *S) {
+    hipLaunchKernelGGL((k_aux_node<R, false>), dim3(s->grid_nodes), dim3(kBlock), 0, st, s->n_nodes,
+                       s->n2e_ptr, s->n2e, s->edge_nodes, s->dir, s->dninv.as<double2>(), in,
+                       (const double2 *)nullptr, s->ynode.as<double2>(), S);
+    EFM_LAUNCH_CHECK();
+    hipLaunchKernelGGL((k_aux_apply_edge<R, PARTIALS>), dim3(s->grid), dim3(kBlock), 0, st, s->n,
+                       s->edge_nodes, s->dir, s->dinv.as<double2>(), s->ynode.as<double2>(), in, out,
+                       s->partial.as<double2>(), S);
+    EFM_LAUNCH_CHECK();
+    return EFM_OK;
+  }
+
+  static int init(efm_solver *s, const double2 *b, hipStream_t st, int method, double tol2) {
+    if (method == EFM_METHOD_COCG && s->aux) {
+      hipLaunchKernelGGL(k_cocg_init_aux<R>, dim3(s->grid), dim3(kBlock), 0, st, s->n, b,
+                         s->x.as<double2>(), s->r.as<double2>(), s->p0.as<double2>());
+      EFM_LAUNCH_CHECK();
+      int rc = aux_apply<true>(s, s->r.as<double2>(), s->z.as<double2>(), st, nullptr);
+      if (rc != EFM_OK)
+        return rc;
+      return fin<kFinInitCocg>(s, s->grid, st, tol2, 0);
+    }
+    if (method == EFM_METHOD_COCG) {
+      hipLaunchKernelGGL(k_cocg_init<R>, dim3(s->grid), dim3(kBlock), 0, st, s->n, b,
+                         s->dinv.as<double2>(), s->x.as<double2>(), s->r.as<double2>(),
+                         s->z.as<double2>(), s->p0.as<double2>(), s->partial.as<double2>());
+      EFM_LAUNCH_CHECK();
+      return fin<kFinInitCocg>(s, s->grid, st, tol2, 0);
+    }
+    hipLaunchKernelGGL(k_bs_init<R>, dim3(s->grid), dim3(kBlock), 0, st, s->n, b, s->x.as<double2>(),
+                       s->r.as<double2>(), s->r0.as<double2>(), s->p0.as<double2>(),
+                       s->v.as<double2>(), s->partial.as<double2>());
+    EFM_LAUNCH_CHECK();
+    return fin<kFinInitBs>(s, s->grid, st, tol2, 0);
   }
 
   template <typename VT>
@@ -801,62 +889,87 @@ template <int R> struct Dispatch {
                        hipEvent_t e0, hipEvent_t e1) {
     double2 *p_old = (it & 1) ? s->p1.as<double2>() : s->p0.as<double2>();
     double2 *p_new = (it & 1) ? s->p0.as<double2>() : s->p1.as<double2>();
+    SysState *S = s->state.as<SysState>();
     if (e0)
       EFM_HIP_CHECK(hipEventRecord(e0, st));
-    hipLaunchKernelGGL((k_cocg_spmv<VT, R>), dim3(s->grid), dim3(kBlock), 0, st, s->n, s->row_ptr,
-                       s->col, s->val.as<VT>(), s->z.as<double2>(), p_old, p_new, s->q.as<double2>(),
-                       s->partial.as<double2>(), s->state.as<SysState>(), s->control.as<Control>());
+    hipLaunchKernelGGL((k_cocg_spmv<VT, R>), dim3(s->grid_spmv), dim3(kBlock), 0, st, s->n,
+                       s->row_ptr, s->col, s->val.as<VT>(), s->z.as<double2>(), p_old, p_new,
+                       s->q.as<double2>(), s->partial.as<double2>(), S);
     EFM_LAUNCH_CHECK();
     if (e1)
       EFM_HIP_CHECK(hipEventRecord(e1, st));
+    int rc = fin<kFinAlpha>(s, s->grid_spmv, st, tol2, max_it);
+    if (rc != EFM_OK)
+      return rc;
+    if (s->aux) {
+      hipLaunchKernelGGL((k_aux_node<R, true>), dim3(s->grid_nodes), dim3(kBlock), 0, st, s->n_nodes,
+                         s->n2e_ptr, s->n2e, s->edge_nodes, s->dir, s->dninv.as<double2>(),
+                         s->r.as<double2>(), s->q.as<double2>(), s->ynode.as<double2>(), S);
+      EFM_LAUNCH_CHECK();
+      hipLaunchKernelGGL(k_aux_cocg_update<R>, dim3(s->grid), dim3(kBlock), 0, st, s->n,
+                         s->edge_nodes, s->dir, s->dinv.as<double2>(), s->ynode.as<double2>(), p_new,
+                         s->q.as<double2>(), s->x.as<double2>(), s->r.as<double2>(),
+                         s->z.as<double2>(), s->partial.as<double2>(), S);
+      EFM_LAUNCH_CHECK();
+      return fin<kFinBeta>(s, s->grid, st, tol2, max_it);
+    }
     hipLaunchKernelGGL(k_cocg_update<R>, dim3(s->grid), dim3(kBlock), 0, st, s->n,
                        s->dinv.as<double2>(), p_new, s->q.as<double2>(), s->x.as<double2>(),
-                       s->r.as<double2>(), s->z.as<double2>(), s->partial.as<double2>(),
-                       s->state.as<SysState>(), s->control.as<Control>(), tol2, max_it);
+                       s->r.as<double2>(), s->z.as<double2>(), s->partial.as<double2>(), S);
     EFM_LAUNCH_CHECK();
-    return EFM_OK;
+    return fin<kFinBeta>(s, s->grid, st, tol2, max_it);
   }
 
   template <typename VT>
   static int bicgstab_iter(efm_solver *s, int it, hipStream_t st, double tol2, int max_it,
                            hipEvent_t e0, hipEvent_t e1) {
     SysState *S = s->state.as<SysState>();
-    Control *C = s->control.as<Control>();
     double2 *part = s->partial.as<double2>();
+    int rc;
     if (it == 0) {
       hipLaunchKernelGGL(k_bs_dot_r0r<R>, dim3(s->grid), dim3(kBlock), 0, st, s->n,
-                         s->r0.as<double2>(), s->r.as<double2>(), part, S, C);
+                         s->r0.as<double2>(), s->r.as<double2>(), part);
       EFM_LAUNCH_CHECK();
+      if ((rc = fin<kFinBsRho0>(s, s->grid, st, tol2, max_it)) != EFM_OK)
+        return rc;
     }
     hipLaunchKernelGGL(k_bs_p<R>, dim3(s->grid), dim3(kBlock), 0, st, s->n, s->dinv.as<double2>(),
                        s->r.as<double2>(), s->v.as<double2>(), s->p0.as<double2>(),
                        s->y.as<double2>(), S);
     EFM_LAUNCH_CHECK();
+    if (s->aux && (rc = aux_apply<false>(s, s->p0.as<double2>(), s->y.as<double2>(), st, S)) != EFM_OK)
+      return rc;
     if (e0)
       EFM_HIP_CHECK(hipEventRecord(e0, st));
-    hipLaunchKernelGGL((k_bs_spmv<VT, R>), dim3(s->grid), dim3(kBlock), 0, st, s->n, s->row_ptr,
-                       s->col, s->val.as<VT>(), s->y.as<double2>(), s->v.as<double2>(),
-                       s->r0.as<double2>(), 0, part, S, C);
+    hipLaunchKernelGGL((k_bs_spmv<VT, R, 0>), dim3(s->grid_spmv), dim3(kBlock), 0, st, s->n,
+                       s->row_ptr, s->col, s->val.as<VT>(), s->y.as<double2>(), s->v.as<double2>(),
+                       s->r0.as<double2>(), part, S);
     EFM_LAUNCH_CHECK();
     if (e1)
       EFM_HIP_CHECK(hipEventRecord(e1, st));
+    if ((rc = fin<kFinBsAlpha>(s, s->grid_spmv, st, tol2, max_it)) != EFM_OK)
+      return rc;
     hipLaunchKernelGGL(k_bs_s<R>, dim3(s->grid), dim3(kBlock), 0, st, s->n, s->dinv.as<double2>(),
                        s->v.as<double2>(), s->r.as<double2>(), s->zz.as<double2>(), S);
     EFM_LAUNCH_CHECK();
-    hipLaunchKernelGGL((k_bs_spmv<VT, R>), dim3(s->grid), dim3(kBlock), 0, st, s->n, s->row_ptr,
-                       s->col, s->val.as<VT>(), s->zz.as<double2>(), s->t.as<double2>(),
-                       s->r.as<double2>(), 1, part, S, C);
+    if (s->aux && (rc = aux_apply<false>(s, s->r.as<double2>(), s->zz.as<double2>(), st, S)) != EFM_OK)
+      return rc;
+    hipLaunchKernelGGL((k_bs_spmv<VT, R, 1>), dim3(s->grid_spmv), dim3(kBlock), 0, st, s->n,
+                       s->row_ptr, s->col, s->val.as<VT>(), s->zz.as<double2>(), s->t.as<double2>(),
+                       s->r.as<double2>(), part, S);
     EFM_LAUNCH_CHECK();
+    if ((rc = fin<kFinBsOmega>(s, s->grid_spmv, st, tol2, max_it)) != EFM_OK)
+      return rc;
     hipLaunchKernelGGL(k_bs_x<R>, dim3(s->grid), dim3(kBlock), 0, st, s->n, s->y.as<double2>(),
                        s->zz.as<double2>(), s->t.as<double2>(), s->r0.as<double2>(),
-                       s->x.as<double2>(), s->r.as<double2>(), part, S, C, tol2, max_it);
+                       s->x.as<double2>(), s->r.as<double2>(), part, S);
     EFM_LAUNCH_CHECK();
-    return EFM_OK;
+    return fin<kFinBsX>(s, s->grid, st, tol2, max_it);
   }
 
   template <typename VT> static int spmv(efm_solver *s, const double2 *x, double2 *y, hipStream_t st) {
-    hipLaunchKernelGGL((k_spmv<VT, R>), dim3(s->grid), dim3(kBlock), 0, st, s->n, s->row_ptr, s->col,
-                       s->val.as<VT>(), x, y);
+    hipLaunchKernelGGL((k_spmv<VT, R>), dim3(s->grid_spmv), dim3(kBlock), 0, st, s->n, s->row_ptr,
+                       s->col, s->val.as<VT>(), x, y);
     EFM_LAUNCH_CHECK();
     return EFM_OK;
   }
@@ -900,16 +1013,19 @@ extern "C" int efm_solver_create(int64_t n, int64_t nnz, const int32_t *row_ptr,
     if (hipGetDeviceProperties(&prop, dev) == hipSuccess && prop.multiProcessorCount > 0)
       ncu = prop.multiProcessorCount;
   }
-  int64_t need = (n + kRowsPerBlock - 1) / kRowsPerBlock;
-  int64_t g = (int64_t)ncu * 8;
-  s->grid = (int)(need < g ? (need < 1 ? 1 : need) : g);
+  const int64_t cap = (int64_t)ncu * 8;
+  int64_t need_spmv = (n + kRowsPerBlock - 1) / kRowsPerBlock;
+  int64_t need_vec = (n + kBlock - 1) / kBlock;
+  s->grid_spmv = (int)std::max<int64_t>(1, std::min(need_spmv, cap));
+  s->grid = (int)std::max<int64_t>(1, std::min(need_vec, cap));
   const size_t vb = (size_t)n * nrhs * sizeof(double2);
   int rc;
   const size_t vbytes = (size_t)nnz * (value_type == EFM_C128 ? 16 : 8);
+  const int gmax = std::max(s->grid, s->grid_spmv);
   if ((rc = s->val.alloc(vbytes)) || (rc = s->dinv.alloc(n * sizeof(double2))) ||
       (rc = s->x.alloc(vb)) || (rc = s->r.alloc(vb)) || (rc = s->z.alloc(vb)) ||
       (rc = s->p0.alloc(vb)) || (rc = s->p1.alloc(vb)) || (rc = s->q.alloc(vb)) ||
-      (rc = s->partial.alloc((size_t)s->grid * 2 * kMaxRhs * sizeof(double2))) ||
+      (rc = s->partial.alloc((size_t)gmax * 2 * kMaxRhs * sizeof(double2))) ||
       (rc = s->state.alloc(kMaxRhs * sizeof(SysState))) || (rc = s->control.alloc(sizeof(Control))))
     return rc;
   EFM_HIP_CHECK(hipMemset(s->control.p, 0, sizeof(Control)));
@@ -931,6 +1047,47 @@ extern "C" int efm_solver_set_matrix(efm_solver *s, const void *values, void *st
     hipLaunchKernelGGL(k_dinv<float2>, dim3(grid_for(s->n, 256)), dim3(256), 0, st, s->n, s->row_ptr,
                        s->col, s->val.as<float2>(), s->dinv.as<double2>());
   EFM_LAUNCH_CHECK();
+  if (s->aux) {
+    if (s->vtype == EFM_C128)
+      hipLaunchKernelGGL(k_aux_diag<double2>, dim3(grid_for(s->n_nodes, 256)), dim3(256), 0, st,
+                         s->n_nodes, s->n2e_ptr, s->n2e, s->edge_nodes, s->dir, s->row_ptr, s->col,
+                         s->val.as<double2>(), s->dninv.as<double2>());
+    else
+      hipLaunchKernelGGL(k_aux_diag<float2>, dim3(grid_for(s->n_nodes, 256)), dim3(256), 0, st,
+                         s->n_nodes, s->n2e_ptr, s->n2e, s->edge_nodes, s->dir, s->row_ptr, s->col,
+                         s->val.as<float2>(), s->dninv.as<double2>());
+    EFM_LAUNCH_CHECK();
+  }
+  return EFM_OK;
+}
+
+extern "C" int efm_solver_set_gradient(efm_solver *s, int64_t n_nodes, const int32_t *edge_nodes,
+                                       const int32_t *n2e_ptr, const int32_t *n2e,
+                                       const uint8_t *dirichlet, void *stream) {
+  EFM_REQUIRE(s, "efm_solver_set_gradient: NULL solver");
+  (void)stream;
+  if (n_nodes <= 0) {  // switch the auxiliary correction off
+    s->aux = false;
+    return EFM_OK;
+  }
+  EFM_REQUIRE(edge_nodes && n2e_ptr && n2e, "efm_solver_set_gradient: NULL mesh array");
+  int rc;
+  if ((rc = s->dninv.alloc((size_t)n_nodes * sizeof(double2))) ||
+      (rc = s->ynode.alloc((size_t)n_nodes * s->R * sizeof(double2))))
+    return rc;
+  s->n_nodes = n_nodes;
+  s->edge_nodes = edge_nodes;
+  s->n2e_ptr = n2e_ptr;
+  s->n2e = n2e;
+  s->dir = dirichlet;
+  int dev = 0, ncu = 256;
+  if (hipGetDevice(&dev) == hipSuccess) {
+    hipDeviceProp_t prop;
+    if (hipGetDeviceProperties(&prop, dev) == hipSuccess && prop.multiProcessorCount > 0)
+      ncu = prop.multiProcessorCount;
+  }
+  s->grid_nodes = (int)std::max<int64_t>(1, std::min<int64_t>((n_nodes + kBlock - 1) / kBlock, (int64_t)ncu * 8));
+  s->aux = true;
   return EFM_OK;
 }
 
@@ -1028,7 +1185,6 @@ extern "C" int efm_solver_solve(efm_solver *s, const double *B, double *X,
       }
     }
   }
-  // the solution of iteration-masked COCG lives in x for every system
   EFM_HIP_CHECK(hipMemcpyAsync(X, s->x.p, (size_t)s->n * s->R * sizeof(double2),
                                hipMemcpyDeviceToDevice, st));
   SysState hs[kMaxRhs];

@@ -121,9 +121,12 @@ __global__ void k_bucket_sort_mark(int64_t N, const int32_t *__restrict__ bptr,
 __global__ void k_bucket_assign(int64_t N, int64_t T, const int32_t *__restrict__ bptr,
                                 const int32_t *__restrict__ bslot,
                                 const int64_t *__restrict__ bkey, const int32_t *__restrict__ eid,
-                                const int64_t *__restrict__ node_ids, int32_t *__restrict__ tet_edges,
+                                const int64_t *__restrict__ node_ids,
+                                const int32_t *__restrict__ tet_nodes,
+                                const int32_t *__restrict__ tri_nodes, int32_t *__restrict__ tet_edges,
                                 int32_t *__restrict__ tri_edges, int64_t *__restrict__ edges,
-                                int32_t *__restrict__ ntet, int32_t *__restrict__ ntri) {
+                                int32_t *__restrict__ edge_nidx, int32_t *__restrict__ ntet,
+                                int32_t *__restrict__ ntri) {
   int64_t u = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
   if (u >= N)
     return;
@@ -149,6 +152,10 @@ __global__ void k_bucket_assign(int64_t N, int64_t T, const int32_t *__restrict_
     int64_t hid = bkey[g0];
     edges[2 * (int64_t)e] = lid < hid ? lid : hid;
     edges[2 * (int64_t)e + 1] = lid < hid ? hid : lid;
+    int32_t su, sv;
+    slot_nodes(bslot[g0], T, tet_nodes, tri_nodes, su, sv);
+    edge_nidx[2 * (int64_t)e] = (int32_t)u;
+    edge_nidx[2 * (int64_t)e + 1] = (su == (int32_t)u) ? sv : su;
     ntet[e] = nt;
     ntri[e] = nf;
     g0 = g1;
@@ -181,6 +188,47 @@ __global__ void k_bucket_fill_incidence(int64_t N, int64_t T, const int32_t *__r
         e2f[kf++] = (int32_t)(s - 6 * T);
     }
     g0 = g1;
+  }
+}
+
+// Node -> incident edges (each edge listed at both ends; degenerate edges once).
+__global__ void k_n2e_count(int64_t n, const int32_t *__restrict__ edge_nidx,
+                            int32_t *__restrict__ cnt) {
+  int64_t e = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (e >= n)
+    return;
+  int32_t a = edge_nidx[2 * e], b = edge_nidx[2 * e + 1];
+  atomicAdd(&cnt[a], 1);
+  if (b != a)
+    atomicAdd(&cnt[b], 1);
+}
+
+__global__ void k_n2e_fill(int64_t n, const int32_t *__restrict__ edge_nidx,
+                           const int32_t *__restrict__ ptr, int32_t *__restrict__ cur,
+                           int32_t *__restrict__ n2e) {
+  int64_t e = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (e >= n)
+    return;
+  int32_t a = edge_nidx[2 * e], b = edge_nidx[2 * e + 1];
+  n2e[ptr[a] + atomicAdd(&cur[a], 1)] = (int32_t)e;
+  if (b != a)
+    n2e[ptr[b] + atomicAdd(&cur[b], 1)] = (int32_t)e;
+}
+
+// Sort every node's list ascending (removes the fill-order nondeterminism).
+__global__ void k_n2e_sort(int64_t N, const int32_t *__restrict__ ptr, int32_t *__restrict__ n2e) {
+  int64_t u = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (u >= N)
+    return;
+  int32_t b0 = ptr[u], b1 = ptr[u + 1];
+  for (int32_t i = b0 + 1; i < b1; ++i) {
+    int32_t v = n2e[i];
+    int32_t j = i - 1;
+    while (j >= b0 && n2e[j] > v) {
+      n2e[j + 1] = n2e[j];
+      --j;
+    }
+    n2e[j + 1] = v;
   }
 }
 
@@ -389,6 +437,7 @@ extern "C" int efm_topology_create(int64_t n_nodes, const int64_t *node_ids, int
   const int64_t n = n_edges;
 
   ALLOC(topo->edges, n * 16);
+  ALLOC(topo->edge_nidx, n * 8);
   DevBuf ntet, ntri;
   ALLOC(ntet, n * 4);
   ALLOC(ntri, n * 4);
@@ -398,8 +447,9 @@ extern "C" int efm_topology_create(int64_t n_nodes, const int64_t *node_ids, int
   ALLOC(topo->e2f, F * 12 + 4);
   hipLaunchKernelGGL(k_bucket_assign, dim3(grid_for(N, kBlock)), dim3(kBlock), 0, st, N, T,
                      bptr.as<int32_t>(), bslot.as<int32_t>(), bkey.as<int64_t>(), eid.as<int32_t>(),
-                     ids, topo->tet_edges.as<int32_t>(), topo->tri_edges.as<int32_t>(),
-                     topo->edges.as<int64_t>(), ntet.as<int32_t>(), ntri.as<int32_t>());
+                     ids, tn, fn, topo->tet_edges.as<int32_t>(), topo->tri_edges.as<int32_t>(),
+                     topo->edges.as<int64_t>(), topo->edge_nidx.as<int32_t>(), ntet.as<int32_t>(),
+                     ntri.as<int32_t>());
   EFM_LAUNCH_CHECK();
   if ((rc = exclusive_scan_i32(ntet.as<int32_t>(), n, topo->e2t_ptr.as<int32_t>(), work.p, st, nullptr)) != EFM_OK)
     return rc;
@@ -417,6 +467,29 @@ extern "C" int efm_topology_create(int64_t n_nodes, const int64_t *node_ids, int
   bkey.alloc(0);
   is_first.alloc(0);
   eid.alloc(0);
+
+  // node -> edge incidence
+  {
+    DevBuf ncnt;
+    ALLOC(ncnt, N * 4);
+    ALLOC(topo->n2e_ptr, (N + 1) * 4);
+    ALLOC(topo->n2e, n * 8 + 4);
+    EFM_HIP_CHECK(hipMemsetAsync(ncnt.p, 0, N * 4, st));
+    hipLaunchKernelGGL(k_n2e_count, dim3(grid_for(n, kBlock)), dim3(kBlock), 0, st, n,
+                       topo->edge_nidx.as<int32_t>(), ncnt.as<int32_t>());
+    EFM_LAUNCH_CHECK();
+    if ((rc = exclusive_scan_i32(ncnt.as<int32_t>(), N, topo->n2e_ptr.as<int32_t>(), work.p, st, nullptr)) != EFM_OK)
+      return rc;
+    EFM_HIP_CHECK(hipMemsetAsync(ncnt.p, 0, N * 4, st));
+    hipLaunchKernelGGL(k_n2e_fill, dim3(grid_for(n, kBlock)), dim3(kBlock), 0, st, n,
+                       topo->edge_nidx.as<int32_t>(), topo->n2e_ptr.as<int32_t>(), ncnt.as<int32_t>(),
+                       topo->n2e.as<int32_t>());
+    EFM_LAUNCH_CHECK();
+    hipLaunchKernelGGL(k_n2e_sort, dim3(grid_for(N, kBlock)), dim3(kBlock), 0, st, N,
+                       topo->n2e_ptr.as<int32_t>(), topo->n2e.as<int32_t>());
+    EFM_LAUNCH_CHECK();
+    EFM_HIP_CHECK(hipStreamSynchronize(st));
+  }
 
   // CSR pattern
   DevBuf rowlen, err;
@@ -479,6 +552,21 @@ extern "C" int efm_topology_create(int64_t n_nodes, const int64_t *node_ids, int
 }
 
 extern "C" void efm_topology_destroy(efm_topology *topo) { delete topo; }
+
+extern "C" int efm_topology_export_nodes(const efm_topology *topo, int32_t *edge_nodes,
+                                         int32_t *n2e_ptr, int32_t *n2e, void *stream) {
+  EFM_REQUIRE(topo, "efm_topology_export_nodes: NULL topology");
+  hipStream_t st = (hipStream_t)stream;
+  if (edge_nodes)
+    EFM_HIP_CHECK(hipMemcpyAsync(edge_nodes, topo->edge_nidx.p, topo->n_edges * 8,
+                                 hipMemcpyDeviceToDevice, st));
+  if (n2e_ptr)
+    EFM_HIP_CHECK(hipMemcpyAsync(n2e_ptr, topo->n2e_ptr.p, (topo->N + 1) * 4,
+                                 hipMemcpyDeviceToDevice, st));
+  if (n2e)
+    EFM_HIP_CHECK(hipMemcpyAsync(n2e, topo->n2e.p, topo->n_edges * 8, hipMemcpyDeviceToDevice, st));
+  return EFM_OK;
+}
 
 extern "C" int efm_topology_get_sizes(const efm_topology *topo, efm_topology_sizes *h_out) {
   EFM_REQUIRE(topo && h_out, "efm_topology_get_sizes: NULL argument");

@@ -864,13 +864,15 @@ def _eigenmode_betas(mesh: Mesh, params: MaxwellParams, ports: Sequence[WavePort
 
 def calculate_sparams_eigenmode(mesh: Mesh, params: MaxwellParams, bc: BC, ports: Sequence[WavePort],
                                 *, solver_options: Optional[SolveOptions] = None,
-                                matrix_dtype=None, _state: Optional[dict] = None) -> np.ndarray:
+                                matrix_dtype=None, preconditioner: str = "aux",
+                                _state: Optional[dict] = None) -> np.ndarray:
     """Eigenmode S-parameters (assemble_maxwell.cpp:637-787) on the GPU:
     A = K/mu - k0^2 eps M + sum_i j beta_i scale M_s,i;  b_a = 2 j beta_a scale M_s,a e_a;
     S(j, a) = e_j^H M_s,j x_a - delta_ja.  The excitations share A, so all of them are
     solved as one block of right-hand sides (the reference re-assembles and re-solves per
     excitation).  ``matrix_dtype`` = torch.complex64 stores A in complex FP32 (FP64
-    vectors and reductions)."""
+    vectors and reductions).  ``preconditioner`` is "aux" (edge Jacobi plus the nodal
+    gradient correction, the default) or "jacobi"."""
     import torch
     P = len(ports)
     S = np.full((P, P), np.nan + 1j * np.nan, np.complex128)
@@ -890,6 +892,8 @@ def calculate_sparams_eigenmode(mesh: Mesh, params: MaxwellParams, bc: BC, ports
     A = _capi.combine(topo, K, M, k0 * k0, ops.Ms, coefs, out_dtype=matrix_dtype or torch.complex128)
     B = ops.rhs(betas, params.port_abc_scale, list(range(P)))
     solver = _capi.KrylovSolver(topo.row_ptr, topo.col_idx, nrhs=P, dtype=A.dtype)
+    if preconditioner == "aux":
+        solver.set_gradient(topo, dmask)
     solver.set_matrix(A)
     tol = 1e-14 if opts.use_direct else float(opts.tolerance)
     X, info = solver.solve(B, tol=tol, max_iterations=int(opts.max_iterations), method=_capi.EFM_METHOD_COCG)
@@ -898,7 +902,7 @@ def calculate_sparams_eigenmode(mesh: Mesh, params: MaxwellParams, bc: BC, ports
     for a in range(P):
         if not (info["resid"][a] <= tol or info["iters"][a] == 0):
             sys.stderr.write(f"WARNING: Solver did not converge in calculate_sparams_eigenmode (active port {a}) "
-                             f"(method=COCG+Jacobi, iters={info['iters'][a]}, residual={info['resid'][a]}). "
+                             f"(method=COCG+{preconditioner}, iters={info['iters'][a]}, residual={info['resid'][a]}). "
                              "S-parameters may be unreliable.\n")
             S[:, a] = np.nan + 1j * np.nan
     if _state is not None:

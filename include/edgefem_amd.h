@@ -69,6 +69,13 @@ int efm_topology_export(const efm_topology *topo, int64_t *edges, int32_t *tet_e
                         int8_t *tet_sign, int32_t *tri_edges, int8_t *tri_sign,
                         int32_t *row_ptr, int32_t *col_idx, void *stream);
 
+/* Node incidence of the edges, for the auxiliary gradient preconditioner:
+ *   edge_nodes[2*n_edges] int32 node positions (lower-ID end, higher-ID end);
+ *   n2e_ptr[n_nodes+1], n2e[2*n_edges] int32: incident edges of every node, ascending.
+ * No reference counterpart (the reference's ILUT needs no mesh data). */
+int efm_topology_export_nodes(const efm_topology *topo, int32_t *edge_nodes, int32_t *n2e_ptr,
+                              int32_t *n2e, void *stream);
+
 /* ------------------------------------------------------------------------
  * Assembly
  * ---------------------------------------------------------------------- */
@@ -107,8 +114,8 @@ int efm_combine(const efm_topology *topo, const double *K_val, const double *M_v
                 int out_type, void *A_out, void *stream);
 
 /* ------------------------------------------------------------------------
- * Krylov solver on a CSR matrix: COCG (complex-symmetric A) or BiCGSTAB, Jacobi
- * preconditioned, x0 = 0, stop when ||r_s|| <= tol * ||b_s|| per system.  FP64 vectors
+ * Krylov solver on a CSR matrix: COCG (complex-symmetric A) or BiCGSTAB, Jacobi or
+ * auxiliary-gradient preconditioned, x0 = 0, stop when ||r_s|| <= tol * ||b_s|| per system.  FP64 vectors
  * and accumulation; the matrix may be stored c128 or c64.
  * Replaces solve_linear (src/solver.cpp:35-193, defaults include/edgefem/solver.hpp:16-36).
  * ---------------------------------------------------------------------- */
@@ -117,8 +124,18 @@ typedef struct efm_solver efm_solver;
 int efm_solver_create(int64_t n, int64_t nnz, const int32_t *row_ptr, const int32_t *col_idx,
                       int nrhs, int value_type, void *stream, efm_solver **out);
 void efm_solver_destroy(efm_solver *s);
-/* Install matrix values (c128 or c64 per value_type); also computes the Jacobi diagonal. */
+/* Install matrix values (c128 or c64 per value_type); also computes the Jacobi diagonal
+ * and, when a gradient is set, the nodal diagonal diag(G^T A G). */
 int efm_solver_set_matrix(efm_solver *s, const void *values, void *stream);
+
+/* Enable the auxiliary-space preconditioner P = D^{-1} + G diag(G^T A G)^{-1} G^T, with G
+ * the discrete gradient of the mesh (arrays from efm_topology_export_nodes; they must
+ * outlive the solver) and Dirichlet edges removed from G.  n_nodes <= 0 switches it off.
+ * Call before efm_solver_set_matrix.  Replaces the preconditioner choice of
+ * src/solver.cpp:53-62 (IncompleteLUT) for mesh-backed systems. */
+int efm_solver_set_gradient(efm_solver *s, int64_t n_nodes, const int32_t *edge_nodes,
+                            const int32_t *n2e_ptr, const int32_t *n2e, const uint8_t *dirichlet,
+                            void *stream);
 
 typedef struct {
   int method;            /* EFM_METHOD_COCG or EFM_METHOD_BICGSTAB */

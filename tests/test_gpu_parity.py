@@ -250,6 +250,50 @@ def test_wr90_reference_measurements_on_gpu(torch_dev, f_ghz, s11, s21, ph):
     assert abs(dph) <= 0.05 + 1e-9
 
 
+def test_discrete_gradient_in_kernel_of_K(torch_dev):
+    """The node incidence used by the auxiliary preconditioner is the discrete
+    gradient of the assembled curl-curl matrix: K G = 0 (no Dirichlet edges)."""
+    import scipy.sparse as sp
+    from edgefem_amd import meshgen as mg
+    from edgefem_amd import pyedgefem as em
+    _p("[gpu] K G = 0")
+    g = mg.box_mesh(0.01, 0.008, 0.012, 5, 4, 6, seed=3)
+    pm = em.mesh_from_arrays(g.node_ids, g.xyz, g.tets, g.tet_phys, g.tris, g.tri_phys)
+    params = em.MaxwellParams()
+    km = em.assemble_maxwell_km(pm, params, em.BC())
+    K = km.K.to_scipy()
+    en = pm.topology.edge_nodes.cpu().numpy()
+    n, N = pm.num_edges(), pm.num_nodes()
+    G = sp.csr_matrix((np.tile([-1.0, 1.0], n), (np.repeat(np.arange(n), 2), en.ravel())), shape=(n, N))
+    # endpoints are ordered by gmsh ID: the lower-ID node first
+    assert np.all(pm.node_ids[en[:, 0]] < pm.node_ids[en[:, 1]])
+    KG = (K @ G).toarray()
+    assert np.abs(KG).max() <= 1e-12 * abs(K).max()
+    # node -> edge incidence lists every edge at both ends, ascending
+    ptr = pm.topology.n2e_ptr.cpu().numpy()
+    n2e = pm.topology.n2e.cpu().numpy()
+    for v in range(N):
+        lst = n2e[ptr[v]:ptr[v + 1]]
+        assert np.all(np.diff(lst) > 0)
+        assert set(lst.tolist()) == set(np.nonzero((en[:, 0] == v) | (en[:, 1] == v))[0].tolist())
+
+
+def test_jacobi_and_aux_agree(torch_dev):
+    """Both preconditioners reach the same S; the auxiliary one in far fewer iterations."""
+    from edgefem_amd import pyedgefem as em
+    _p("[gpu] jacobi vs aux")
+    pm, bc, ports = _c1_em_setup()
+    ref = np.load(golden("c1_sparams_10ghz.npz"))
+    params = em.MaxwellParams()
+    params.omega = 2 * math.pi * 10e9
+    s1, s2 = {}, {}
+    S1 = em.calculate_sparams_eigenmode(pm, params, bc, ports, preconditioner="jacobi", _state=s1)
+    S2 = em.calculate_sparams_eigenmode(pm, params, bc, ports, preconditioner="aux", _state=s2)
+    assert np.abs(S1 - ref["S"]).max() <= 1e-6 and np.abs(S2 - ref["S"]).max() <= 1e-6
+    _p(f"    iterations jacobi {s1['info']['iters']} aux {s2['info']['iters']}")
+    assert max(s2["info"]["iters"]) * 3 < max(s1["info"]["iters"])
+
+
 def test_solve_linear_bicgstab_nonsymmetric(torch_dev):
     """BiCGSTAB on a non-symmetric matrix (lossy A plus a skew perturbation on the
     pattern) against SuperLU."""
