@@ -193,6 +193,10 @@ def main():
     ap.add_argument("--no-check", action="store_true")
     ap.add_argument("--cpu-seconds", type=float, default=20.0)
     ap.add_argument("--small", action="store_true", help="16x8x32 guide (quick check)")
+    ap.add_argument("--spmv-variant", type=int, default=-1,
+                    help="fixed COCG SpMV variant (-1: autotune at the first solve)")
+    ap.add_argument("--calibrate-copy", type=int, default=0, metavar="MB",
+                    help="first run a plain device-to-device copy of this many MB (counter calibration)")
     args = ap.parse_args()
     if args.gpus != 1:
         print(json.dumps({"error": "this build runs on one GPU; --gpus must be 1 "
@@ -212,6 +216,15 @@ def main():
     log(f"setup: mesh + topology + ports {time.time() - t0:.1f}s; n={mesh.num_edges()} nnz={mesh.topology.nnz}")
     dtype = torch.complex128 if args.mode == "fp64" else torch.complex64
     pipe = Pipeline(mesh, bc, ports, params, dtype, args.tol, args.max_it, args.prec)
+    if args.spmv_variant >= 0:
+        pipe.solver.set_spmv_variant(args.spmv_variant)
+    if args.calibrate_copy > 0:
+        src = torch.ones(args.calibrate_copy * (1 << 20) // 8, dtype=torch.float64, device="cuda")
+        dst = torch.empty_like(src)
+        dst.copy_(src)
+        torch.cuda.synchronize()
+        log(f"calibration copy: {src.numel() * 8} bytes")
+        del src, dst
     for w in range(args.warmup):
         S = pipe.step()
         log(f"warmup {w}: iters {pipe.last_info['iters']} S21 {S[1, 0]:.6f}")
@@ -225,6 +238,7 @@ def main():
     torch.cuda.synchronize()
     te = time.perf_counter()
     spmv_ms, spmv_n = pipe.solver.profile_read()
+    variant, tune_ms = pipe.solver.spmv_variant()
     pipe.solver.profile(False)
     elapsed = te - ts
     ms_per_step = 1e3 * elapsed / args.steps
@@ -238,6 +252,14 @@ def main():
     achieved = spmv_bytes / (avg_ms * 1e-3)
     iters = int(max(pipe.last_info["iters"]))
 
+    traffic = None
+    tpath = os.path.join(ROOT, "profiles", "traffic_c2.json")
+    if os.path.exists(tpath) and not args.small:
+        tj = json.load(open(tpath))
+        if tj.get("variant") == variant and args.mode == "fp64":
+            for k, v in tj["kernels"].items():
+                if k.startswith("k_cocg_spmv"):
+                    traffic = v["hbm_bytes"] / 1e9
     dS = None
     ref_rel = None
     if not args.no_check:
@@ -260,8 +282,10 @@ def main():
                    "tol": args.tol,
                    "mode": args.mode},
         "roofline": {"bound": "hbm", "achieved": achieved / 1e9, "peak": HBM_PEAK / 1e9, "unit": "GB/s",
-                     "frac": achieved / HBM_PEAK, "traffic": None, "kernel": "k_cocg_spmv",
-                     "bytes_per_launch": spmv_bytes, "avg_launch_ms": avg_ms, "launches": spmv_n},
+                     "frac": achieved / HBM_PEAK, "traffic": traffic, "traffic_unit": "GB per launch",
+                     "kernel": "k_cocg_spmv",
+                     "bytes_per_launch": spmv_bytes, "avg_launch_ms": avg_ms, "launches": spmv_n,
+                     "variant": variant, "autotune_ms": [round(float(t), 4) for t in tune_ms]},
         "cpu_baseline": cpu,
         "iterations": iters,
         "delta_S_vs_fp64_reference": dS,

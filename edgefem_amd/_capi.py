@@ -29,7 +29,7 @@ EXPORTS = [
     "efm_assemble_km", "efm_assemble_maxwell", "efm_port_surface_mass", "efm_combine",
     "efm_solver_create", "efm_solver_destroy", "efm_solver_set_matrix", "efm_solver_solve",
     "efm_solver_spmv", "efm_solver_profile", "efm_solver_profile_read",
-    "efm_port_overlap",
+    "efm_port_overlap", "efm_solver_set_spmv_variant", "efm_solver_get_spmv_variant",
 ]
 
 
@@ -87,6 +87,8 @@ def load_library(path: str = LIB_PATH) -> C.CDLL:
     lib.efm_solver_spmv.argtypes = [vp, vp, vp, vp]
     lib.efm_solver_profile.argtypes = [vp, i32]
     lib.efm_solver_profile_read.argtypes = [vp, C.POINTER(C.c_double), C.POINTER(C.c_int64)]
+    lib.efm_solver_set_spmv_variant.argtypes = [vp, i32]
+    lib.efm_solver_get_spmv_variant.argtypes = [vp, C.POINTER(C.c_int), vp]
     lib.efm_port_overlap.argtypes = [i64, i32, i64, vp, vp, vp, i32, vp, vp]
     for name in EXPORTS:
         getattr(lib, name)  # AttributeError if the library lacks a declared symbol
@@ -319,6 +321,16 @@ class KrylovSolver:
         y = torch.empty_like(xd)
         _check(self._lib.efm_solver_spmv(self._h, dptr(xd), dptr(y), stream_ptr()), "efm_solver_spmv")
         return y
+
+    def set_spmv_variant(self, variant: int):
+        _check(self._lib.efm_solver_set_spmv_variant(self._h, int(variant)), "efm_solver_set_spmv_variant")
+
+    def spmv_variant(self):
+        v = C.c_int()
+        ms = np.zeros(6, np.float64)
+        _check(self._lib.efm_solver_get_spmv_variant(self._h, C.byref(v), ms.ctypes.data_as(C.c_void_p)),
+               "efm_solver_get_spmv_variant")
+        return v.value, ms
 
     def profile(self, enable=True):
         _check(self._lib.efm_solver_profile(self._h, 1 if enable else 0), "efm_solver_profile")

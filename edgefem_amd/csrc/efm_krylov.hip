@@ -216,14 +216,18 @@ __global__ void __launch_bounds__(kBlock)
 }
 
 // ---------------------------------------------------------------------------
-// COCG
+// COCG.  The search direction and the preconditioned residual of every row live in
+// one record W[row] = {z_0..z_{R-1}, p_0..p_{R-1}} (2R complex values), so the SpMV
+// gathers both with one cache line per column.  Two records alternate between
+// iterations: the SpMV reads W_a and writes p_new into W_b; the update writes the new
+// z into W_b; the next SpMV reads W_b.
 // ---------------------------------------------------------------------------
-// x = 0, r = b, z = D^{-1} b, p = 0; partials rho = b^T z, ||b||^2
+// x = 0, r = b, W0.z = D^{-1} b, W0.p = 0; partials rho = b^T z, ||b||^2
 template <int R>
 __global__ void __launch_bounds__(kBlock)
     k_cocg_init(int64_t n, const double2 *__restrict__ b, const double2 *__restrict__ dinv,
-                double2 *__restrict__ x, double2 *__restrict__ r, double2 *__restrict__ z,
-                double2 *__restrict__ p, double2 *__restrict__ partial) {
+                double2 *__restrict__ x, double2 *__restrict__ r, double2 *__restrict__ W,
+                double2 *__restrict__ partial) {
   __shared__ double2 lds[(kBlock / 64) * 2 * R];
   double2 v[2 * R];
 #pragma unroll
@@ -238,8 +242,8 @@ __global__ void __launch_bounds__(kBlock)
       double2 zv = bil(di, bv);
       x[k] = make_double2(0, 0);
       r[k] = bv;
-      z[k] = zv;
-      p[k] = make_double2(0, 0);
+      W[i * 2 * R + s] = zv;
+      W[i * 2 * R + R + s] = make_double2(0, 0);
       v[s] = cadd(v[s], bil(bv, zv));
       v[R + s].x += cabs2(bv);
     }
@@ -247,14 +251,103 @@ __global__ void __launch_bounds__(kBlock)
   write_partials<2 * R>(v, lds, partial);
 }
 
-// p_new = z + beta p_old; q = A p_new; partials p_new^T q
-template <typename VT, int R>
+// Fused COCG SpMV kernels.  All compute, for every row and active system s,
+//   p_new = z + beta p_old  (at every gathered column, from the record W_a),
+//   q = A p_new,  W_b.p = p_new,  partial sums of p_new^T q,
+// and differ only in how rows map onto lanes.  The autotuner times each variant on the
+// installed matrix and keeps the fastest (efm_solver_autotune).
+//
+// Row-group variants: LPR lanes share a row and stride over its entries; rows are
+// taken in chunks of 256/LPR.  Blocks are persistent and XCD-aware: block b runs on XCD
+// b % 8 and walks chunks of that XCD's contiguous share of the rows, so the column
+// gathers of one XCD stay within its own L2.
+template <typename VT, int R, int LPR>
 __global__ void __launch_bounds__(kBlock)
-    k_cocg_spmv(int64_t n, const int32_t *__restrict__ row_ptr, const int32_t *__restrict__ col,
-                const VT *__restrict__ val, const double2 *__restrict__ z,
-                const double2 *__restrict__ p_old, double2 *__restrict__ p_new,
-                double2 *__restrict__ q, double2 *__restrict__ partial,
-                const SysState *__restrict__ st) {
+    k_cocg_spmv_rows(int64_t n, const int32_t *__restrict__ row_ptr, const int32_t *__restrict__ col,
+                     const VT *__restrict__ val, const double2 *__restrict__ Wa,
+                     double2 *__restrict__ Wb, double2 *__restrict__ q, double2 *__restrict__ partial,
+                     const SysState *__restrict__ st) {
+  __shared__ double2 lds[(kBlock / 64) * R];
+  constexpr int kRows = kBlock / LPR;
+  bool act[R];
+  double2 beta[R];
+#pragma unroll
+  for (int s = 0; s < R; ++s) {
+    act[s] = st[s].active != 0;
+    beta[s] = st[s].beta;
+  }
+  const int g = threadIdx.x / LPR;
+  const int lane = threadIdx.x % LPR;
+  double2 pq[R];
+#pragma unroll
+  for (int s = 0; s < R; ++s)
+    pq[s] = make_double2(0, 0);
+  const int64_t nchunks = (n + kRows - 1) / kRows;
+  const int64_t span = (nchunks + 7) / 8;
+  const int xcd = blockIdx.x & 7;
+  const int per_xcd = gridDim.x >> 3;
+  const int64_t c_begin = xcd * span;
+  const int64_t c_end = min(nchunks, c_begin + span);
+  for (int64_t chunk = c_begin + (blockIdx.x >> 3); chunk < c_end; chunk += per_xcd) {
+    const int64_t row = chunk * kRows + g;
+    double2 acc[R];
+#pragma unroll
+    for (int s = 0; s < R; ++s)
+      acc[s] = make_double2(0, 0);
+    if (row < n) {
+      const int32_t k0 = row_ptr[row], k1 = row_ptr[row + 1];
+      for (int32_t k = k0 + lane; k < k1; k += LPR) {
+        const double2 a = to_d2(val[k]);
+        const double2 *w = Wa + (int64_t)col[k] * 2 * R;
+#pragma unroll
+        for (int s = 0; s < R; ++s)
+          acc[s] = cfma(a, cfma(beta[s], w[R + s], w[s]), acc[s]);
+      }
+    }
+#pragma unroll
+    for (int s = 0; s < R; ++s) {
+#pragma unroll
+      for (int off = LPR / 2; off > 0; off >>= 1) {
+        acc[s].x += __shfl_xor(acc[s].x, off, LPR);
+        acc[s].y += __shfl_xor(acc[s].y, off, LPR);
+      }
+    }
+    if (row < n && lane == 0) {
+      const double2 *w = Wa + row * 2 * R;
+      double2 *wb = Wb + row * 2 * R;
+#pragma unroll
+      for (int s = 0; s < R; ++s) {
+        if (!act[s])
+          continue;
+        const double2 pn = cfma(beta[s], w[R + s], w[s]);
+        wb[R + s] = pn;
+        q[row * R + s] = acc[s];
+        pq[s] = cadd(pq[s], bil(pn, acc[s]));
+      }
+    }
+  }
+  write_partials<R>(pq, lds, partial);
+}
+
+// Tiled variants.  A tile is a run of whole rows with at most TNNZ nonzeros and
+// kBlock/RL rows.  Phase 1: thread t takes entries k = kb + t + 256 j, loads (value,
+// column) coalesced, gathers the column's record and stores a * (z + beta p) in LDS;
+// all of these loads are independent.  Phase 2: RL lanes per row sum the row's
+// products from LDS; lane 0 writes q and p_new.  Persistent, XCD-aware tile walk.
+constexpr int kTileRows = 64;
+// nonzeros per tile: the LDS product buffer (tile_nnz * R complex doubles) stays <= 32 KB
+__host__ __device__ constexpr int tile_nnz(int R) { return R <= 2 ? 1024 : (R <= 4 ? 512 : 256); }
+// the smaller tiles of the second tiled variant (at least one entry per thread)
+__host__ __device__ constexpr int tile_nnz2(int R) { return tile_nnz(R) / 2 < 256 ? 256 : tile_nnz(R) / 2; }
+
+template <typename VT, int R, int TNNZ, int RL>
+__global__ void __launch_bounds__(kBlock)
+    k_cocg_spmv_tiled(int ntiles, int xcd_span, const int32_t *__restrict__ tile_row,
+                      const int32_t *__restrict__ row_ptr, const int32_t *__restrict__ col,
+                      const VT *__restrict__ val, const double2 *__restrict__ Wa,
+                      double2 *__restrict__ Wb, double2 *__restrict__ q,
+                      double2 *__restrict__ partial, const SysState *__restrict__ st) {
+  __shared__ double2 prod[TNNZ * R];
   __shared__ double2 lds[(kBlock / 64) * R];
   bool act[R];
   double2 beta[R];
@@ -263,62 +356,218 @@ __global__ void __launch_bounds__(kBlock)
     act[s] = st[s].active != 0;
     beta[s] = st[s].beta;
   }
-  const int g = threadIdx.x / kLanesPerRow;
-  const int lane = threadIdx.x % kLanesPerRow;
   double2 pq[R];
 #pragma unroll
   for (int s = 0; s < R; ++s)
     pq[s] = make_double2(0, 0);
-  for (int64_t row0 = (int64_t)blockIdx.x * kRowsPerBlock; row0 < n;
-       row0 += (int64_t)gridDim.x * kRowsPerBlock) {
-    const int64_t row = row0 + g;
+  const int xcd = blockIdx.x & 7;
+  const int per_xcd = gridDim.x >> 3;  // grid is a multiple of 8
+  const int lb = blockIdx.x >> 3;
+  const int t_begin = xcd * xcd_span;
+  const int t_end = min(ntiles, t_begin + xcd_span);
+  for (int tile = t_begin + lb; tile < t_end; tile += per_xcd) {
+    const int32_t r0 = tile_row[tile], r1 = tile_row[tile + 1];
+    const int32_t kb = row_ptr[r0], ke = row_ptr[r1];
+    constexpr int J = TNNZ / kBlock;
+    double2 a[J];
+    int32_t c[J];
+#pragma unroll
+    for (int j = 0; j < J; ++j) {
+      const int32_t k = kb + threadIdx.x + j * kBlock;
+      if (k < ke) {
+        a[j] = to_d2(val[k]);
+        c[j] = col[k];
+      } else {
+        a[j] = make_double2(0, 0);
+        c[j] = -1;
+      }
+    }
+#pragma unroll
+    for (int j = 0; j < J; ++j) {
+      const int32_t lk = threadIdx.x + j * kBlock;
+      if (c[j] >= 0) {
+        const double2 *w = Wa + (int64_t)c[j] * 2 * R;
+#pragma unroll
+        for (int s = 0; s < R; ++s) {
+          const double2 pc = cfma(beta[s], w[R + s], w[s]);
+          prod[lk * R + s] = bil(a[j], pc);
+        }
+      }
+    }
+    __syncthreads();
+    const int g = threadIdx.x / RL, l = threadIdx.x % RL;
+    const int32_t row = r0 + g;
     double2 acc[R];
 #pragma unroll
     for (int s = 0; s < R; ++s)
       acc[s] = make_double2(0, 0);
-    if (row < n) {
-      const int32_t k0 = row_ptr[row], k1 = row_ptr[row + 1];
-      for (int32_t k = k0 + lane; k < k1; k += kLanesPerRow) {
-        const double2 a = to_d2(val[k]);
-        const int64_t c = col[k];
+    if (row < r1) {
+      const int32_t k0 = row_ptr[row] - kb, k1 = row_ptr[row + 1] - kb;
+      for (int32_t k = k0 + l; k < k1; k += RL) {
 #pragma unroll
-        for (int s = 0; s < R; ++s) {
-          const double2 pc = cfma(beta[s], p_old[c * R + s], z[c * R + s]);
-          acc[s] = cfma(a, pc, acc[s]);
-        }
+        for (int s = 0; s < R; ++s)
+          acc[s] = cadd(acc[s], prod[k * R + s]);
       }
     }
 #pragma unroll
     for (int s = 0; s < R; ++s) {
 #pragma unroll
-      for (int off = kLanesPerRow / 2; off > 0; off >>= 1) {
-        acc[s].x += __shfl_xor(acc[s].x, off, kLanesPerRow);
-        acc[s].y += __shfl_xor(acc[s].y, off, kLanesPerRow);
+      for (int off = RL / 2; off > 0; off >>= 1) {
+        acc[s].x += __shfl_xor(acc[s].x, off, RL);
+        acc[s].y += __shfl_xor(acc[s].y, off, RL);
       }
     }
-    if (row < n && lane == 0) {
+    if (row < r1 && l == 0) {
+      const double2 *w = Wa + (int64_t)row * 2 * R;
+      double2 *wb = Wb + (int64_t)row * 2 * R;
 #pragma unroll
       for (int s = 0; s < R; ++s) {
         if (!act[s])
           continue;
-        const int64_t k = row * R + s;
-        const double2 pn = cfma(beta[s], p_old[k], z[k]);
-        p_new[k] = pn;
-        q[k] = acc[s];
+        const double2 pn = cfma(beta[s], w[R + s], w[s]);
+        wb[R + s] = pn;
+        q[(int64_t)row * R + s] = acc[s];
         pq[s] = cadd(pq[s], bil(pn, acc[s]));
       }
     }
+    __syncthreads();  // prod is reused by the next tile
   }
   write_partials<R>(pq, lds, partial);
 }
 
-// x += alpha p, r -= alpha q, z = D^{-1} r; partials r^T z, r^H r
+// Tiled variant with a cooperative record gather (R even).  The L1 of a CU looks up
+// about one cache line per clock, so a gather of four separate 16-byte pieces per
+// nonzero costs four lookups.  Here: (1a) the tile's columns are staged in LDS with
+// coalesced loads; (1b) four lanes read one column's 64-byte record run together (one
+// lookup) and form z + beta p; (1c) the values are loaded coalesced and multiplied in
+// LDS; (2) RL lanes per row reduce as in the plain tiled variant.
+template <typename VT, int R, int TNNZ, int RL>
+__global__ void __launch_bounds__(kBlock)
+    k_cocg_spmv_coop(int ntiles, int xcd_span, const int32_t *__restrict__ tile_row,
+                     const int32_t *__restrict__ row_ptr, const int32_t *__restrict__ col,
+                     const VT *__restrict__ val, const double2 *__restrict__ Wa,
+                     double2 *__restrict__ Wb, double2 *__restrict__ q,
+                     double2 *__restrict__ partial, const SysState *__restrict__ st) {
+  static_assert(R % 2 == 0, "cooperative gather needs an even number of systems");
+  __shared__ double2 prod[TNNZ * R];
+  __shared__ int32_t scol[TNNZ];
+  __shared__ double2 lds[(kBlock / 64) * R];
+  bool act[R];
+  double2 beta[R];
+#pragma unroll
+  for (int s = 0; s < R; ++s) {
+    act[s] = st[s].active != 0;
+    beta[s] = st[s].beta;
+  }
+  double2 pq[R];
+#pragma unroll
+  for (int s = 0; s < R; ++s)
+    pq[s] = make_double2(0, 0);
+  const int xcd = blockIdx.x & 7;
+  const int per_xcd = gridDim.x >> 3;
+  const int lb = blockIdx.x >> 3;
+  const int t_begin = xcd * xcd_span;
+  const int t_end = min(ntiles, t_begin + xcd_span);
+  constexpr int J = TNNZ / kBlock;
+  constexpr int E = 2 * R / 4;             // record entries per lane
+  constexpr int M = TNNZ / (kBlock / 4);  // nonzeros per lane group
+  const int l4 = threadIdx.x & 3, g4 = threadIdx.x >> 2;
+  for (int tile = t_begin + lb; tile < t_end; tile += per_xcd) {
+    const int32_t r0 = tile_row[tile], r1 = tile_row[tile + 1];
+    const int32_t kb = row_ptr[r0], ke = row_ptr[r1];
+    // (1a) columns -> LDS
+#pragma unroll
+    for (int j = 0; j < J; ++j) {
+      const int32_t lk = threadIdx.x + j * kBlock;
+      scol[lk] = (kb + lk < ke) ? col[kb + lk] : -1;
+    }
+    __syncthreads();
+    // (1b) cooperative record gather
+#pragma unroll 4
+    for (int m = 0; m < M; ++m) {
+      const int32_t lk = g4 + m * (kBlock / 4);
+      const int32_t c = scol[lk];
+      if (c >= 0) {
+        const double2 *w = Wa + (int64_t)c * 2 * R;
+        double2 ent[E];
+#pragma unroll
+        for (int j = 0; j < E; ++j)
+          ent[j] = w[l4 + 4 * j];
+        if constexpr (R == 2) {
+          double2 pv;
+          pv.x = __shfl_down(ent[0].x, 2, 4);
+          pv.y = __shfl_down(ent[0].y, 2, 4);
+          if (l4 < 2)
+            prod[lk * R + l4] = cfma(beta[l4], pv, ent[0]);
+        } else {
+#pragma unroll
+          for (int j = 0; j < E / 2; ++j) {
+            const int sidx = l4 + 4 * j;
+            prod[lk * R + sidx] = cfma(beta[sidx], ent[j + E / 2], ent[j]);
+          }
+        }
+      }
+    }
+    __syncthreads();
+    // (1c) values, coalesced
+#pragma unroll
+    for (int j = 0; j < J; ++j) {
+      const int32_t lk = threadIdx.x + j * kBlock;
+      if (kb + lk < ke) {
+        const double2 a = to_d2(val[kb + lk]);
+#pragma unroll
+        for (int s = 0; s < R; ++s)
+          prod[lk * R + s] = bil(a, prod[lk * R + s]);
+      }
+    }
+    __syncthreads();
+    // (2) row sums
+    const int g = threadIdx.x / RL, l = threadIdx.x % RL;
+    const int32_t row = r0 + g;
+    double2 acc[R];
+#pragma unroll
+    for (int s = 0; s < R; ++s)
+      acc[s] = make_double2(0, 0);
+    if (row < r1) {
+      const int32_t k0 = row_ptr[row] - kb, k1 = row_ptr[row + 1] - kb;
+      for (int32_t k = k0 + l; k < k1; k += RL) {
+#pragma unroll
+        for (int s = 0; s < R; ++s)
+          acc[s] = cadd(acc[s], prod[k * R + s]);
+      }
+    }
+#pragma unroll
+    for (int s = 0; s < R; ++s) {
+#pragma unroll
+      for (int off = RL / 2; off > 0; off >>= 1) {
+        acc[s].x += __shfl_xor(acc[s].x, off, RL);
+        acc[s].y += __shfl_xor(acc[s].y, off, RL);
+      }
+    }
+    if (row < r1 && l == 0) {
+      const double2 *w = Wa + (int64_t)row * 2 * R;
+      double2 *wb = Wb + (int64_t)row * 2 * R;
+#pragma unroll
+      for (int s = 0; s < R; ++s) {
+        if (!act[s])
+          continue;
+        const double2 pn = cfma(beta[s], w[R + s], w[s]);
+        wb[R + s] = pn;
+        q[(int64_t)row * R + s] = acc[s];
+        pq[s] = cadd(pq[s], bil(pn, acc[s]));
+      }
+    }
+    __syncthreads();
+  }
+  write_partials<R>(pq, lds, partial);
+}
+
+// x += alpha p, r -= alpha q, W_b.z = D^{-1} r; partials r^T z, r^H r
 template <int R>
 __global__ void __launch_bounds__(kBlock)
-    k_cocg_update(int64_t n, const double2 *__restrict__ dinv, const double2 *__restrict__ p,
+    k_cocg_update(int64_t n, const double2 *__restrict__ dinv, double2 *__restrict__ Wb,
                   const double2 *__restrict__ q, double2 *__restrict__ x, double2 *__restrict__ r,
-                  double2 *__restrict__ z, double2 *__restrict__ partial,
-                  const SysState *__restrict__ st) {
+                  double2 *__restrict__ partial, const SysState *__restrict__ st) {
   __shared__ double2 lds[(kBlock / 64) * 2 * R];
   bool act[R];
   double2 alpha[R];
@@ -333,17 +582,18 @@ __global__ void __launch_bounds__(kBlock)
     v[s] = make_double2(0, 0);
   for (int64_t i = (int64_t)blockIdx.x * kBlock + threadIdx.x; i < n; i += (int64_t)gridDim.x * kBlock) {
     const double2 di = dinv[i];
+    double2 *w = Wb + i * 2 * R;
 #pragma unroll
     for (int s = 0; s < R; ++s) {
       if (!act[s])
         continue;
       const int64_t k = i * R + s;
-      const double2 xv = cfma(alpha[s], p[k], x[k]);
+      const double2 xv = cfma(alpha[s], w[R + s], x[k]);
       const double2 rv = csub(r[k], bil(alpha[s], q[k]));
       const double2 zv = bil(di, rv);
       x[k] = xv;
       r[k] = rv;
-      z[k] = zv;
+      w[s] = zv;
       v[s] = cadd(v[s], bil(rv, zv));
       v[R + s].x += cabs2(rv);
     }
@@ -659,14 +909,39 @@ __global__ void k_aux_diag(int64_t N, const int32_t *__restrict__ n2e_ptr,
   dninv[nd] = (d.x == 0.0 && d.y == 0.0) ? make_double2(0, 0) : cdiv(make_double2(1, 0), d);
 }
 
-// y[n] = Dn^{-1} sum_e G[e,n] (r[e] - alpha q[e])   (alpha, q only when FUSED)
+// Per-incidence gradient entries G[e, n] in {-1, 0, +1} (0 for Dirichlet or degenerate
+// edges) and per-edge endpoints with the same masking folded in (-1, -1).
+__global__ void k_aux_prep_nodes(int64_t N, const int32_t *__restrict__ n2e_ptr,
+                                 const int32_t *__restrict__ n2e, const int32_t *__restrict__ en,
+                                 const uint8_t *__restrict__ dir, int8_t *__restrict__ gsign) {
+  int64_t nd = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (nd >= N)
+    return;
+  for (int32_t k = n2e_ptr[nd]; k < n2e_ptr[nd + 1]; ++k) {
+    const int32_t e = n2e[k];
+    gsign[k] = (dir && dir[e]) ? 0 : (int8_t)grad_sign(en, e, (int32_t)nd);
+  }
+}
+
+__global__ void k_aux_prep_edges(int64_t n, const int32_t *__restrict__ en,
+                                 const uint8_t *__restrict__ dir, int2 *__restrict__ en_eff) {
+  int64_t e = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (e >= n)
+    return;
+  const int32_t a = en[2 * e], b = en[2 * e + 1];
+  const bool off = (dir && dir[e]) || a == b;
+  en_eff[e] = off ? make_int2(-1, -1) : make_int2(a, b);
+}
+
+// y[n] = Dn^{-1} sum_e G[e,n] (r[e] - alpha q[e])   (alpha, q only when FUSED).
+// Eight lanes per node stride over its incidence list; fixed-order shuffle reduction.
+constexpr int kLanesPerNode = 8;
 template <int R, bool FUSED>
 __global__ void __launch_bounds__(kBlock)
     k_aux_node(int64_t N, const int32_t *__restrict__ n2e_ptr, const int32_t *__restrict__ n2e,
-               const int32_t *__restrict__ en, const uint8_t *__restrict__ dir,
-               const double2 *__restrict__ dninv, const double2 *__restrict__ r,
-               const double2 *__restrict__ q, double2 *__restrict__ y,
-               const SysState *__restrict__ st) {
+               const int8_t *__restrict__ gsign, const double2 *__restrict__ dninv,
+               const double2 *__restrict__ r, const double2 *__restrict__ q,
+               double2 *__restrict__ y, const SysState *__restrict__ st) {
   bool act[R];
   double2 alpha[R];
 #pragma unroll
@@ -674,47 +949,60 @@ __global__ void __launch_bounds__(kBlock)
     act[s] = st == nullptr || st[s].active != 0;
     alpha[s] = FUSED ? st[s].alpha : make_double2(0, 0);
   }
-  for (int64_t nd = (int64_t)blockIdx.x * kBlock + threadIdx.x; nd < N;
-       nd += (int64_t)gridDim.x * kBlock) {
+  const int g = threadIdx.x / kLanesPerNode, lane = threadIdx.x % kLanesPerNode;
+  constexpr int kNodesPerBlock = kBlock / kLanesPerNode;
+  for (int64_t nd0 = (int64_t)blockIdx.x * kNodesPerBlock; nd0 < N;
+       nd0 += (int64_t)gridDim.x * kNodesPerBlock) {
+    const int64_t nd = nd0 + g;
     double2 acc[R];
 #pragma unroll
     for (int s = 0; s < R; ++s)
       acc[s] = make_double2(0, 0);
-    for (int32_t k = n2e_ptr[nd]; k < n2e_ptr[nd + 1]; ++k) {
-      const int32_t e = n2e[k];
-      if (dir && dir[e])
-        continue;
-      const int g = grad_sign(en, e, (int32_t)nd);
-      if (!g)
-        continue;
-      const double sg = (double)g;
+    if (nd < N) {
+      for (int32_t k = n2e_ptr[nd] + lane; k < n2e_ptr[nd + 1]; k += kLanesPerNode) {
+        const int32_t e = n2e[k];
+        const int gs = gsign[k];
+        if (!gs)
+          continue;
+        const double sg = (double)gs;
 #pragma unroll
-      for (int s = 0; s < R; ++s) {
-        const int64_t i = (int64_t)e * R + s;
-        double2 rv = r[i];
-        if (FUSED)
-          rv = csub(rv, bil(alpha[s], q[i]));
-        acc[s].x += sg * rv.x;
-        acc[s].y += sg * rv.y;
+        for (int s = 0; s < R; ++s) {
+          const int64_t i = (int64_t)e * R + s;
+          double2 rv = r[i];
+          if (FUSED)
+            rv = csub(rv, bil(alpha[s], q[i]));
+          acc[s].x += sg * rv.x;
+          acc[s].y += sg * rv.y;
+        }
       }
     }
-    const double2 di = dninv[nd];
 #pragma unroll
-    for (int s = 0; s < R; ++s)
-      if (act[s])
-        y[nd * R + s] = bil(di, acc[s]);
+    for (int s = 0; s < R; ++s) {
+#pragma unroll
+      for (int off = kLanesPerNode / 2; off > 0; off >>= 1) {
+        acc[s].x += __shfl_xor(acc[s].x, off, kLanesPerNode);
+        acc[s].y += __shfl_xor(acc[s].y, off, kLanesPerNode);
+      }
+    }
+    if (nd < N && lane == 0) {
+      const double2 di = dninv[nd];
+#pragma unroll
+      for (int s = 0; s < R; ++s)
+        if (act[s])
+          y[nd * R + s] = bil(di, acc[s]);
+    }
   }
 }
 
 // COCG update with the auxiliary preconditioner:
-//   r <- r - alpha q; x <- x + alpha p; z = D^{-1} r + (y[head] - y[tail]);
+//   r <- r - alpha q; x <- x + alpha p; W_b.z = D^{-1} r + (y[head] - y[tail]);
 //   partials r^T z, r^H r
 template <int R>
 __global__ void __launch_bounds__(kBlock)
-    k_aux_cocg_update(int64_t n, const int32_t *__restrict__ en, const uint8_t *__restrict__ dir,
+    k_aux_cocg_update(int64_t n, const int2 *__restrict__ en_eff,
                       const double2 *__restrict__ dinv, const double2 *__restrict__ y,
-                      const double2 *__restrict__ p, const double2 *__restrict__ q,
-                      double2 *__restrict__ x, double2 *__restrict__ r, double2 *__restrict__ z,
+                      double2 *__restrict__ Wb, const double2 *__restrict__ q,
+                      double2 *__restrict__ x, double2 *__restrict__ r,
                       double2 *__restrict__ partial, const SysState *__restrict__ st) {
   __shared__ double2 lds[(kBlock / 64) * 2 * R];
   bool act[R];
@@ -730,21 +1018,21 @@ __global__ void __launch_bounds__(kBlock)
     v[s] = make_double2(0, 0);
   for (int64_t e = (int64_t)blockIdx.x * kBlock + threadIdx.x; e < n; e += (int64_t)gridDim.x * kBlock) {
     const double2 di = dinv[e];
-    const bool dd = dir && dir[e];
-    const int32_t a = en[2 * e], b = en[2 * e + 1];
+    const int2 ab = en_eff[e];
+    double2 *w = Wb + e * 2 * R;
 #pragma unroll
     for (int s = 0; s < R; ++s) {
       if (!act[s])
         continue;
       const int64_t k = e * R + s;
-      const double2 xv = cfma(alpha[s], p[k], x[k]);
+      const double2 xv = cfma(alpha[s], w[R + s], x[k]);
       const double2 rv = csub(r[k], bil(alpha[s], q[k]));
       double2 zv = bil(di, rv);
-      if (!dd && a != b)
-        zv = cadd(zv, csub(y[(int64_t)b * R + s], y[(int64_t)a * R + s]));
+      if (ab.x >= 0)
+        zv = cadd(zv, csub(y[(int64_t)ab.y * R + s], y[(int64_t)ab.x * R + s]));
       x[k] = xv;
       r[k] = rv;
-      z[k] = zv;
+      w[s] = zv;
       v[s] = cadd(v[s], bil(rv, zv));
       v[R + s].x += cabs2(rv);
     }
@@ -756,9 +1044,9 @@ __global__ void __launch_bounds__(kBlock)
 // with PARTIALS also the partials in^T out and in^H in.
 template <int R, bool PARTIALS>
 __global__ void __launch_bounds__(kBlock)
-    k_aux_apply_edge(int64_t n, const int32_t *__restrict__ en, const uint8_t *__restrict__ dir,
+    k_aux_apply_edge(int64_t n, const int2 *__restrict__ en_eff,
                      const double2 *__restrict__ dinv, const double2 *__restrict__ y,
-                     const double2 *__restrict__ in, double2 *__restrict__ out,
+                     const double2 *__restrict__ in, double2 *__restrict__ out, int ostride,
                      double2 *__restrict__ partial, const SysState *__restrict__ st) {
   __shared__ double2 lds[(kBlock / 64) * 2 * R];
   bool act[R];
@@ -771,8 +1059,7 @@ __global__ void __launch_bounds__(kBlock)
     v[s] = make_double2(0, 0);
   for (int64_t e = (int64_t)blockIdx.x * kBlock + threadIdx.x; e < n; e += (int64_t)gridDim.x * kBlock) {
     const double2 di = dinv[e];
-    const bool dd = dir && dir[e];
-    const int32_t a = en[2 * e], b = en[2 * e + 1];
+    const int2 ab = en_eff[e];
 #pragma unroll
     for (int s = 0; s < R; ++s) {
       if (!act[s])
@@ -780,9 +1067,9 @@ __global__ void __launch_bounds__(kBlock)
       const int64_t k = e * R + s;
       const double2 iv = in[k];
       double2 o = bil(di, iv);
-      if (!dd && a != b)
-        o = cadd(o, csub(y[(int64_t)b * R + s], y[(int64_t)a * R + s]));
-      out[k] = o;
+      if (ab.x >= 0)
+        o = cadd(o, csub(y[(int64_t)ab.y * R + s], y[(int64_t)ab.x * R + s]));
+      out[e * ostride + s] = o;
       if (PARTIALS) {
         v[s] = cadd(v[s], bil(iv, o));
         v[R + s].x += cabs2(iv);
@@ -793,20 +1080,23 @@ __global__ void __launch_bounds__(kBlock)
     write_partials<2 * R>(v, lds, partial);
 }
 
-// x = 0, r = b, p = 0 (the preconditioned residual comes from the aux kernels)
+// x = 0, r = b, W0.p = 0 (W0.z comes from the aux kernels)
 template <int R>
 __global__ void __launch_bounds__(kBlock)
     k_cocg_init_aux(int64_t n, const double2 *__restrict__ b, double2 *__restrict__ x,
-                    double2 *__restrict__ r, double2 *__restrict__ p) {
+                    double2 *__restrict__ r, double2 *__restrict__ W) {
   for (int64_t i = (int64_t)blockIdx.x * kBlock + threadIdx.x; i < n * R;
        i += (int64_t)gridDim.x * kBlock) {
     x[i] = make_double2(0, 0);
     r[i] = b[i];
-    p[i] = make_double2(0, 0);
+    const int64_t row = i / R, s = i - row * R;
+    W[row * 2 * R + R + s] = make_double2(0, 0);
   }
 }
 
 } // namespace
+
+constexpr int kNumSpmvVariants = 6;  // rows<4>, rows<8>, rows<16>, tiled<T,4>, tiled<T/2,8>, coop<T,4>
 
 struct efm_solver {
   int64_t n = 0, nnz = 0;
@@ -816,14 +1106,21 @@ struct efm_solver {
   int vtype = EFM_C128;
   int grid = 1024;       // streaming kernels (grid-stride)
   int grid_spmv = 1024;  // row-group kernels (grid-stride)
-  efm::DevBuf val, dinv, x, r, z, p0, p1, q, r0, v, y, zz, t, partial, state, control;
+  efm::DevBuf val, dinv, x, r, q, W0, W1, p0, r0, v, y, zz, t, partial, state, control;
+  // row tiles of the tiled SpMV variants (ntiles = 0: a row is longer than a tile)
+  efm::DevBuf tile_row[2];
+  int ntiles[2] = {0, 0}, xcd_span[2] = {0, 0}, grid_tiles[2] = {8, 8};
+  int grid_rows[3] = {8, 8, 8};
+  int spmv_variant = -1;  // -1: choose by timing at the first solve
+  double tune_ms[kNumSpmvVariants] = {0};
   // auxiliary gradient preconditioner (non-owning mesh arrays)
   bool aux = false;
   int64_t n_nodes = 0;
   int grid_nodes = 1;
   const int32_t *edge_nodes = nullptr, *n2e_ptr = nullptr, *n2e = nullptr;
   const uint8_t *dir = nullptr;
-  efm::DevBuf dninv, ynode;
+  efm::DevBuf dninv, ynode, gsign, en_eff;
+  int grid_node_groups = 1;
   bool profile = false;
   double spmv_ms = 0.0;
   int64_t spmv_launches = 0;
@@ -847,15 +1144,15 @@ template <int R> struct Dispatch {
 
   // y = Dn^{-1} G^T in (nodes), then out = D^{-1} in + G y; partials when asked
   template <bool PARTIALS>
-  static int aux_apply(efm_solver *s, const double2 *in, double2 *out, hipStream_t st,
+  static int aux_apply(efm_solver *s, const double2 *in, double2 *out, int ostride, hipStream_t st,
                        const SysState *S) {
-    hipLaunchKernelGGL((k_aux_node<R, false>), dim3(s->grid_nodes), dim3(kBlock), 0, st, s->n_nodes,
-                       s->n2e_ptr, s->n2e, s->edge_nodes, s->dir, s->dninv.as<double2>(), in,
-                       (const double2 *)nullptr, s->ynode.as<double2>(), S);
+    hipLaunchKernelGGL((k_aux_node<R, false>), dim3(s->grid_node_groups), dim3(kBlock), 0, st,
+                       s->n_nodes, s->n2e_ptr, s->n2e, s->gsign.as<int8_t>(), s->dninv.as<double2>(),
+                       in, (const double2 *)nullptr, s->ynode.as<double2>(), S);
     EFM_LAUNCH_CHECK();
     hipLaunchKernelGGL((k_aux_apply_edge<R, PARTIALS>), dim3(s->grid), dim3(kBlock), 0, st, s->n,
-                       s->edge_nodes, s->dir, s->dinv.as<double2>(), s->ynode.as<double2>(), in, out,
-                       s->partial.as<double2>(), S);
+                       s->en_eff.as<int2>(), s->dinv.as<double2>(), s->ynode.as<double2>(), in, out,
+                       ostride, s->partial.as<double2>(), S);
     EFM_LAUNCH_CHECK();
     return EFM_OK;
   }
@@ -863,9 +1160,9 @@ template <int R> struct Dispatch {
   static int init(efm_solver *s, const double2 *b, hipStream_t st, int method, double tol2) {
     if (method == EFM_METHOD_COCG && s->aux) {
       hipLaunchKernelGGL(k_cocg_init_aux<R>, dim3(s->grid), dim3(kBlock), 0, st, s->n, b,
-                         s->x.as<double2>(), s->r.as<double2>(), s->p0.as<double2>());
+                         s->x.as<double2>(), s->r.as<double2>(), s->W0.as<double2>());
       EFM_LAUNCH_CHECK();
-      int rc = aux_apply<true>(s, s->r.as<double2>(), s->z.as<double2>(), st, nullptr);
+      int rc = aux_apply<true>(s, s->r.as<double2>(), s->W0.as<double2>(), 2 * R, st, nullptr);
       if (rc != EFM_OK)
         return rc;
       return fin<kFinInitCocg>(s, s->grid, st, tol2, 0);
@@ -873,7 +1170,7 @@ template <int R> struct Dispatch {
     if (method == EFM_METHOD_COCG) {
       hipLaunchKernelGGL(k_cocg_init<R>, dim3(s->grid), dim3(kBlock), 0, st, s->n, b,
                          s->dinv.as<double2>(), s->x.as<double2>(), s->r.as<double2>(),
-                         s->z.as<double2>(), s->p0.as<double2>(), s->partial.as<double2>());
+                         s->W0.as<double2>(), s->partial.as<double2>());
       EFM_LAUNCH_CHECK();
       return fin<kFinInitCocg>(s, s->grid, st, tol2, 0);
     }
@@ -884,38 +1181,130 @@ template <int R> struct Dispatch {
     return fin<kFinInitBs>(s, s->grid, st, tol2, 0);
   }
 
+  // One fused SpMV of the given variant; *nb = number of partial sums written.
+  template <typename VT>
+  static int launch_spmv(efm_solver *s, int variant, const double2 *Wa, double2 *Wb, hipStream_t st,
+                         int *nb) {
+    SysState *S = s->state.as<SysState>();
+    constexpr int TN = tile_nnz(R);
+    switch (variant) {
+    case 0:
+      *nb = s->grid_rows[0];
+      hipLaunchKernelGGL((k_cocg_spmv_rows<VT, R, 4>), dim3(*nb), dim3(kBlock), 0, st, s->n, s->row_ptr,
+                         s->col, s->val.as<VT>(), Wa, Wb, s->q.as<double2>(), s->partial.as<double2>(), S);
+      break;
+    case 1:
+      *nb = s->grid_rows[1];
+      hipLaunchKernelGGL((k_cocg_spmv_rows<VT, R, 8>), dim3(*nb), dim3(kBlock), 0, st, s->n, s->row_ptr,
+                         s->col, s->val.as<VT>(), Wa, Wb, s->q.as<double2>(), s->partial.as<double2>(), S);
+      break;
+    case 2:
+      *nb = s->grid_rows[2];
+      hipLaunchKernelGGL((k_cocg_spmv_rows<VT, R, 16>), dim3(*nb), dim3(kBlock), 0, st, s->n, s->row_ptr,
+                         s->col, s->val.as<VT>(), Wa, Wb, s->q.as<double2>(), s->partial.as<double2>(), S);
+      break;
+    case 3:
+      *nb = s->grid_tiles[0];
+      hipLaunchKernelGGL((k_cocg_spmv_tiled<VT, R, TN, 4>), dim3(*nb), dim3(kBlock), 0, st, s->ntiles[0],
+                         s->xcd_span[0], s->tile_row[0].as<int32_t>(), s->row_ptr, s->col, s->val.as<VT>(),
+                         Wa, Wb, s->q.as<double2>(), s->partial.as<double2>(), S);
+      break;
+    case 4:
+      *nb = s->grid_tiles[1];
+      hipLaunchKernelGGL((k_cocg_spmv_tiled<VT, R, tile_nnz2(R), 8>), dim3(*nb), dim3(kBlock), 0, st,
+                         s->ntiles[1], s->xcd_span[1], s->tile_row[1].as<int32_t>(), s->row_ptr, s->col,
+                         s->val.as<VT>(), Wa, Wb, s->q.as<double2>(), s->partial.as<double2>(), S);
+      break;
+    case 5:
+      if constexpr (R % 2 == 0) {
+        *nb = s->grid_tiles[0];
+        hipLaunchKernelGGL((k_cocg_spmv_coop<VT, R, TN, 4>), dim3(*nb), dim3(kBlock), 0, st, s->ntiles[0],
+                           s->xcd_span[0], s->tile_row[0].as<int32_t>(), s->row_ptr, s->col,
+                           s->val.as<VT>(), Wa, Wb, s->q.as<double2>(), s->partial.as<double2>(), S);
+        break;
+      }
+      set_error("SpMV variant 5 needs an even number of right-hand sides");
+      return EFM_ERR_ARG;
+    default:
+      set_error("unknown SpMV variant");
+      return EFM_ERR_ARG;
+    }
+    EFM_LAUNCH_CHECK();
+    return EFM_OK;
+  }
+
+  // Time every available variant on the installed matrix (state and W0 must be valid:
+  // called right after init; the timed launches only write W1.p, q and the partials,
+  // which the first real iteration overwrites) and keep the fastest.
+  template <typename VT> static int autotune(efm_solver *s, hipStream_t st) {
+    hipEvent_t e0, e1;
+    EFM_HIP_CHECK(hipEventCreate(&e0));
+    EFM_HIP_CHECK(hipEventCreate(&e1));
+    int best = 1;
+    double best_ms = 1e30;
+    for (int v = 0; v < kNumSpmvVariants; ++v) {
+      s->tune_ms[v] = -1.0;
+      if (v >= 3 && v <= 4 && s->ntiles[v - 3] == 0)
+        continue;
+      if (v == 5 && (s->ntiles[0] == 0 || (R % 2) != 0))
+        continue;
+      int nb;
+      int rc = launch_spmv<VT>(s, v, s->W0.as<double2>(), s->W1.as<double2>(), st, &nb);  // warm-up
+      if (rc != EFM_OK)
+        return rc;
+      EFM_HIP_CHECK(hipEventRecord(e0, st));
+      const int reps = 5;
+      for (int k = 0; k < reps; ++k)
+        if ((rc = launch_spmv<VT>(s, v, s->W0.as<double2>(), s->W1.as<double2>(), st, &nb)) != EFM_OK)
+          return rc;
+      EFM_HIP_CHECK(hipEventRecord(e1, st));
+      EFM_HIP_CHECK(hipEventSynchronize(e1));
+      float ms = 0.f;
+      EFM_HIP_CHECK(hipEventElapsedTime(&ms, e0, e1));
+      s->tune_ms[v] = ms / reps;
+      if (ms / reps < best_ms) {
+        best_ms = ms / reps;
+        best = v;
+      }
+    }
+    (void)hipEventDestroy(e0);
+    (void)hipEventDestroy(e1);
+    s->spmv_variant = best;
+    return EFM_OK;
+  }
+
   template <typename VT>
   static int cocg_iter(efm_solver *s, int it, hipStream_t st, double tol2, int max_it,
                        hipEvent_t e0, hipEvent_t e1) {
-    double2 *p_old = (it & 1) ? s->p1.as<double2>() : s->p0.as<double2>();
-    double2 *p_new = (it & 1) ? s->p0.as<double2>() : s->p1.as<double2>();
+    double2 *Wa = (it & 1) ? s->W1.as<double2>() : s->W0.as<double2>();
+    double2 *Wb = (it & 1) ? s->W0.as<double2>() : s->W1.as<double2>();
     SysState *S = s->state.as<SysState>();
     if (e0)
       EFM_HIP_CHECK(hipEventRecord(e0, st));
-    hipLaunchKernelGGL((k_cocg_spmv<VT, R>), dim3(s->grid_spmv), dim3(kBlock), 0, st, s->n,
-                       s->row_ptr, s->col, s->val.as<VT>(), s->z.as<double2>(), p_old, p_new,
-                       s->q.as<double2>(), s->partial.as<double2>(), S);
-    EFM_LAUNCH_CHECK();
+    int nb_spmv = 0;
+    int rc = launch_spmv<VT>(s, s->spmv_variant, Wa, Wb, st, &nb_spmv);
+    if (rc != EFM_OK)
+      return rc;
     if (e1)
       EFM_HIP_CHECK(hipEventRecord(e1, st));
-    int rc = fin<kFinAlpha>(s, s->grid_spmv, st, tol2, max_it);
+    rc = fin<kFinAlpha>(s, nb_spmv, st, tol2, max_it);
     if (rc != EFM_OK)
       return rc;
     if (s->aux) {
-      hipLaunchKernelGGL((k_aux_node<R, true>), dim3(s->grid_nodes), dim3(kBlock), 0, st, s->n_nodes,
-                         s->n2e_ptr, s->n2e, s->edge_nodes, s->dir, s->dninv.as<double2>(),
+      hipLaunchKernelGGL((k_aux_node<R, true>), dim3(s->grid_node_groups), dim3(kBlock), 0, st,
+                         s->n_nodes, s->n2e_ptr, s->n2e, s->gsign.as<int8_t>(), s->dninv.as<double2>(),
                          s->r.as<double2>(), s->q.as<double2>(), s->ynode.as<double2>(), S);
       EFM_LAUNCH_CHECK();
       hipLaunchKernelGGL(k_aux_cocg_update<R>, dim3(s->grid), dim3(kBlock), 0, st, s->n,
-                         s->edge_nodes, s->dir, s->dinv.as<double2>(), s->ynode.as<double2>(), p_new,
+                         s->en_eff.as<int2>(), s->dinv.as<double2>(), s->ynode.as<double2>(), Wb,
                          s->q.as<double2>(), s->x.as<double2>(), s->r.as<double2>(),
-                         s->z.as<double2>(), s->partial.as<double2>(), S);
+                         s->partial.as<double2>(), S);
       EFM_LAUNCH_CHECK();
       return fin<kFinBeta>(s, s->grid, st, tol2, max_it);
     }
     hipLaunchKernelGGL(k_cocg_update<R>, dim3(s->grid), dim3(kBlock), 0, st, s->n,
-                       s->dinv.as<double2>(), p_new, s->q.as<double2>(), s->x.as<double2>(),
-                       s->r.as<double2>(), s->z.as<double2>(), s->partial.as<double2>(), S);
+                       s->dinv.as<double2>(), Wb, s->q.as<double2>(), s->x.as<double2>(),
+                       s->r.as<double2>(), s->partial.as<double2>(), S);
     EFM_LAUNCH_CHECK();
     return fin<kFinBeta>(s, s->grid, st, tol2, max_it);
   }
@@ -937,7 +1326,7 @@ template <int R> struct Dispatch {
                        s->r.as<double2>(), s->v.as<double2>(), s->p0.as<double2>(),
                        s->y.as<double2>(), S);
     EFM_LAUNCH_CHECK();
-    if (s->aux && (rc = aux_apply<false>(s, s->p0.as<double2>(), s->y.as<double2>(), st, S)) != EFM_OK)
+    if (s->aux && (rc = aux_apply<false>(s, s->p0.as<double2>(), s->y.as<double2>(), R, st, S)) != EFM_OK)
       return rc;
     if (e0)
       EFM_HIP_CHECK(hipEventRecord(e0, st));
@@ -952,7 +1341,7 @@ template <int R> struct Dispatch {
     hipLaunchKernelGGL(k_bs_s<R>, dim3(s->grid), dim3(kBlock), 0, st, s->n, s->dinv.as<double2>(),
                        s->v.as<double2>(), s->r.as<double2>(), s->zz.as<double2>(), S);
     EFM_LAUNCH_CHECK();
-    if (s->aux && (rc = aux_apply<false>(s, s->r.as<double2>(), s->zz.as<double2>(), st, S)) != EFM_OK)
+    if (s->aux && (rc = aux_apply<false>(s, s->r.as<double2>(), s->zz.as<double2>(), R, st, S)) != EFM_OK)
       return rc;
     hipLaunchKernelGGL((k_bs_spmv<VT, R, 1>), dim3(s->grid_spmv), dim3(kBlock), 0, st, s->n,
                        s->row_ptr, s->col, s->val.as<VT>(), s->zz.as<double2>(), s->t.as<double2>(),
@@ -1021,10 +1410,53 @@ extern "C" int efm_solver_create(int64_t n, int64_t nnz, const int32_t *row_ptr,
   const size_t vb = (size_t)n * nrhs * sizeof(double2);
   int rc;
   const size_t vbytes = (size_t)nnz * (value_type == EFM_C128 ? 16 : 8);
-  const int gmax = std::max(s->grid, s->grid_spmv);
+  // row tiles for the tiled SpMV variants: greedy runs of whole rows, <= TNNZ nonzeros
+  // and <= 256 / RL rows per tile; blocks = 8 XCDs x min(tiles per XCD, ncu)
+  std::vector<int32_t> h_rp(n + 1);
+  EFM_HIP_CHECK(hipMemcpy(h_rp.data(), row_ptr, (n + 1) * sizeof(int32_t), hipMemcpyDeviceToHost));
+  for (int tv = 0; tv < 2; ++tv) {
+    const int tcap = tv == 0 ? tile_nnz(nrhs) : tile_nnz2(nrhs);
+    const int rcap = tv == 0 ? kBlock / 4 : kBlock / 8;
+    std::vector<int32_t> tiles;
+    tiles.push_back(0);
+    bool tileable = true;
+    for (int64_t r = 0; r < n;) {
+      int64_t r1 = r;
+      while (r1 < n && (r1 - r) < rcap && (h_rp[r1 + 1] - h_rp[r]) <= tcap)
+        ++r1;
+      if (r1 == r) {  // a single row longer than a tile
+        tileable = false;
+        break;
+      }
+      tiles.push_back((int32_t)r1);
+      r = r1;
+    }
+    if (!tileable)
+      continue;
+    s->ntiles[tv] = (int)tiles.size() - 1;
+    s->xcd_span[tv] = (s->ntiles[tv] + 7) / 8;
+    const int64_t per_xcd = std::min<int64_t>(s->xcd_span[tv], (int64_t)ncu);
+    s->grid_tiles[tv] = (int)(8 * std::max<int64_t>(1, per_xcd));
+    if ((rc = s->tile_row[tv].alloc(tiles.size() * sizeof(int32_t))))
+      return rc;
+    EFM_HIP_CHECK(hipMemcpy(s->tile_row[tv].p, tiles.data(), tiles.size() * sizeof(int32_t),
+                            hipMemcpyHostToDevice));
+  }
+  const int lprs[3] = {4, 8, 16};
+  for (int v = 0; v < 3; ++v) {
+    const int64_t rows = kBlock / lprs[v];
+    const int64_t nchunks = (n + rows - 1) / rows;
+    const int64_t span = (nchunks + 7) / 8;
+    s->grid_rows[v] = (int)(8 * std::max<int64_t>(1, std::min<int64_t>(span, (int64_t)ncu)));
+  }
+  int gmax = std::max(s->grid, s->grid_spmv);
+  for (int v = 0; v < 3; ++v)
+    gmax = std::max(gmax, s->grid_rows[v]);
+  for (int v = 0; v < 2; ++v)
+    gmax = std::max(gmax, s->grid_tiles[v]);
   if ((rc = s->val.alloc(vbytes)) || (rc = s->dinv.alloc(n * sizeof(double2))) ||
-      (rc = s->x.alloc(vb)) || (rc = s->r.alloc(vb)) || (rc = s->z.alloc(vb)) ||
-      (rc = s->p0.alloc(vb)) || (rc = s->p1.alloc(vb)) || (rc = s->q.alloc(vb)) ||
+      (rc = s->x.alloc(vb)) || (rc = s->r.alloc(vb)) || (rc = s->q.alloc(vb)) ||
+      (rc = s->W0.alloc(2 * vb)) || (rc = s->W1.alloc(2 * vb)) ||
       (rc = s->partial.alloc((size_t)gmax * 2 * kMaxRhs * sizeof(double2))) ||
       (rc = s->state.alloc(kMaxRhs * sizeof(SysState))) || (rc = s->control.alloc(sizeof(Control))))
     return rc;
@@ -1065,7 +1497,6 @@ extern "C" int efm_solver_set_gradient(efm_solver *s, int64_t n_nodes, const int
                                        const int32_t *n2e_ptr, const int32_t *n2e,
                                        const uint8_t *dirichlet, void *stream) {
   EFM_REQUIRE(s, "efm_solver_set_gradient: NULL solver");
-  (void)stream;
   if (n_nodes <= 0) {  // switch the auxiliary correction off
     s->aux = false;
     return EFM_OK;
@@ -1087,6 +1518,17 @@ extern "C" int efm_solver_set_gradient(efm_solver *s, int64_t n_nodes, const int
       ncu = prop.multiProcessorCount;
   }
   s->grid_nodes = (int)std::max<int64_t>(1, std::min<int64_t>((n_nodes + kBlock - 1) / kBlock, (int64_t)ncu * 8));
+  const int64_t node_groups = (n_nodes + (kBlock / kLanesPerNode) - 1) / (kBlock / kLanesPerNode);
+  s->grid_node_groups = (int)std::max<int64_t>(1, std::min<int64_t>(node_groups, (int64_t)ncu * 8));
+  if ((rc = s->gsign.alloc((size_t)2 * s->n + 16)) || (rc = s->en_eff.alloc((size_t)s->n * sizeof(int2))))
+    return rc;
+  hipStream_t st = (hipStream_t)stream;
+  hipLaunchKernelGGL(k_aux_prep_nodes, dim3(grid_for(n_nodes, 256)), dim3(256), 0, st, n_nodes, n2e_ptr,
+                     n2e, edge_nodes, dirichlet, s->gsign.as<int8_t>());
+  EFM_LAUNCH_CHECK();
+  hipLaunchKernelGGL(k_aux_prep_edges, dim3(grid_for(s->n, 256)), dim3(256), 0, st, s->n, edge_nodes,
+                     dirichlet, s->en_eff.as<int2>());
+  EFM_LAUNCH_CHECK();
   s->aux = true;
   return EFM_OK;
 }
@@ -1100,6 +1542,26 @@ extern "C" int efm_solver_spmv(efm_solver *s, const double *x, double *y, void *
       return Dispatch<R>::template spmv<double2>(s, (const double2 *)x, (double2 *)y, st);
     return Dispatch<R>::template spmv<float2>(s, (const double2 *)x, (double2 *)y, st);
   });
+}
+
+extern "C" int efm_solver_set_spmv_variant(efm_solver *s, int variant) {
+  EFM_REQUIRE(s, "efm_solver_set_spmv_variant: NULL solver");
+  EFM_REQUIRE(variant >= -1 && variant < kNumSpmvVariants, "efm_solver_set_spmv_variant: bad variant");
+  EFM_REQUIRE(variant < 3 || s->ntiles[variant == 5 ? 0 : variant - 3] > 0,
+              "efm_solver_set_spmv_variant: matrix has a row longer than a tile");
+  EFM_REQUIRE(variant != 5 || s->R % 2 == 0,
+              "efm_solver_set_spmv_variant: variant 5 needs an even number of right-hand sides");
+  s->spmv_variant = variant;
+  return EFM_OK;
+}
+
+extern "C" int efm_solver_get_spmv_variant(const efm_solver *s, int *h_variant, double *h_tune_ms) {
+  EFM_REQUIRE(s && h_variant, "efm_solver_get_spmv_variant: NULL argument");
+  *h_variant = s->spmv_variant;
+  if (h_tune_ms)
+    for (int v = 0; v < kNumSpmvVariants; ++v)
+      h_tune_ms[v] = s->tune_ms[v];
+  return EFM_OK;
 }
 
 extern "C" int efm_solver_profile(efm_solver *s, int enable) {
@@ -1135,7 +1597,7 @@ extern "C" int efm_solver_solve(efm_solver *s, const double *B, double *X,
   if (!cocg) {
     const size_t vb = (size_t)s->n * s->R * sizeof(double2);
     if (!s->r0.p && ((rc = s->r0.alloc(vb)) || (rc = s->v.alloc(vb)) || (rc = s->y.alloc(vb)) ||
-                     (rc = s->zz.alloc(vb)) || (rc = s->t.alloc(vb))))
+                     (rc = s->zz.alloc(vb)) || (rc = s->t.alloc(vb)) || (rc = s->p0.alloc(vb))))
       return rc;
   }
   if (s->profile && s->evs.size() < (size_t)2 * check) {
@@ -1151,6 +1613,16 @@ extern "C" int efm_solver_solve(efm_solver *s, const double *B, double *X,
   });
   if (rc != EFM_OK)
     return rc;
+  if (cocg && s->spmv_variant < 0) {
+    rc = with_rhs(s->R, [&](auto RC) -> int {
+      constexpr int R = decltype(RC)::value;
+      if (s->vtype == EFM_C128)
+        return Dispatch<R>::template autotune<double2>(s, st);
+      return Dispatch<R>::template autotune<float2>(s, st);
+    });
+    if (rc != EFM_OK)
+      return rc;
+  }
   Control hc{};
   EFM_HIP_CHECK(hipMemcpyAsync(&hc, s->control.p, sizeof(Control), hipMemcpyDeviceToHost, st));
   EFM_HIP_CHECK(hipStreamSynchronize(st));

@@ -156,6 +156,15 @@ int efm_solver_solve(efm_solver *s, const double *B, double *X, const efm_solve_
                      efm_solve_result *h_result, int32_t *h_iters, double *h_resid,
                      void *stream);
 
+/* The COCG SpMV has several row-to-lane mappings (0-2: 4, 8, 16 lanes per row; 3-4:
+ * LDS-tiled with 4 or 8 lanes per row; 5: LDS-tiled with a cooperative gather of each
+ * column's record, even nrhs only).  By default (-1) the first solve times every
+ * variant on the installed matrix and keeps the fastest; a fixed variant makes results
+ * reproducible bit for bit across runs.  h_tune_ms (6 entries, may be NULL) receives the
+ * measured average ms per launch (-1 = not available).  No reference counterpart. */
+int efm_solver_set_spmv_variant(efm_solver *s, int variant);
+int efm_solver_get_spmv_variant(const efm_solver *s, int *h_variant, double *h_tune_ms);
+
 /* y = A x (all nrhs systems), FP64 accumulation. */
 int efm_solver_spmv(efm_solver *s, const double *x, double *y, void *stream);
 
