@@ -30,6 +30,7 @@ EXPORTS = [
     "efm_solver_create", "efm_solver_destroy", "efm_solver_set_matrix", "efm_solver_solve",
     "efm_solver_spmv", "efm_solver_profile", "efm_solver_profile_read",
     "efm_port_overlap", "efm_solver_set_spmv_variant", "efm_solver_get_spmv_variant",
+    "efm_sweep_create", "efm_sweep_destroy", "efm_sweep_solve",
 ]
 
 
@@ -89,6 +90,10 @@ def load_library(path: str = LIB_PATH) -> C.CDLL:
     lib.efm_solver_profile_read.argtypes = [vp, C.POINTER(C.c_double), C.POINTER(C.c_int64)]
     lib.efm_solver_set_spmv_variant.argtypes = [vp, i32]
     lib.efm_solver_get_spmv_variant.argtypes = [vp, C.POINTER(C.c_int), vp]
+    lib.efm_sweep_create.argtypes = [vp, vp, vp, i32, vp, vp, i32, vp, C.POINTER(vp)]
+    lib.efm_sweep_destroy.argtypes = [vp]
+    lib.efm_sweep_destroy.restype = None
+    lib.efm_sweep_solve.argtypes = [vp, i32, vp, vp, vp, vp, C.POINTER(SolveOpts), vp, vp, vp]
     lib.efm_port_overlap.argtypes = [i64, i32, i64, vp, vp, vp, i32, vp, vp]
     for name in EXPORTS:
         getattr(lib, name)  # AttributeError if the library lacks a declared symbol
@@ -364,3 +369,52 @@ def port_overlap(gidx, gval, X, nrhs: int):
     _check(lib.efm_port_overlap(X.numel() // nrhs, P, L, dptr(gi), dptr(gv), dptr(X), int(nrhs),
                                 out.ctypes.data_as(C.c_void_p), stream_ptr()), "efm_port_overlap")
     return out.reshape(P, nrhs)
+
+
+class SweepSolver:
+    """Batched COCG over many systems K - k0sq[s] M + sum_p coef[s,p] M_s,p sharing one
+    pattern (vectors [n_edges][nsys])."""
+
+    def __init__(self, topo: Topology, K, M, Ms_list: Sequence, dirichlet, max_systems: int):
+        torch = _torch()
+        lib = load_library()
+        self._lib = lib
+        self._keep = (topo, K, M, list(Ms_list), None if dirichlet is None else to_dev(dirichlet, torch.uint8))
+        P = len(Ms_list)
+        ptrs = (C.c_void_p * max(P, 1))(*[dptr(m) for m in Ms_list])
+        h = C.c_void_p()
+        _check(lib.efm_sweep_create(topo.handle, dptr(K), dptr(M), P, ptrs, dptr(self._keep[4]), int(max_systems),
+                                    stream_ptr(), C.byref(h)), "efm_sweep_create")
+        self._h = h
+        self.n = topo.n_edges
+        self.P = P
+        self.max_systems = int(max_systems)
+
+    def solve(self, k0sq, coef, B, tol=1e-10, max_iterations=10000, check_every=16):
+        """k0sq (nsys,), coef (nsys, P) complex, B (n, nsys) complex128 device tensor."""
+        torch = _torch()
+        k2 = np.ascontiguousarray(np.asarray(k0sq, np.float64))
+        nsys = k2.size
+        cf = np.ascontiguousarray(np.asarray(coef, np.complex128).reshape(nsys, self.P))
+        Bd = to_dev(B, torch.complex128)
+        if Bd.numel() != self.n * nsys:
+            raise EdgeFemError("right-hand side block has the wrong size")
+        X = torch.empty_like(Bd)
+        iters = np.zeros(nsys, np.int32)
+        resid = np.zeros(nsys, np.float64)
+        opts = SolveOpts(EFM_METHOD_COCG, float(tol), int(max_iterations), int(check_every))
+        _check(self._lib.efm_sweep_solve(self._h, nsys, k2.ctypes.data_as(C.c_void_p), cf.ctypes.data_as(C.c_void_p),
+                                         dptr(Bd), dptr(X), C.byref(opts), iters.ctypes.data_as(C.c_void_p),
+                                         resid.ctypes.data_as(C.c_void_p), stream_ptr()), "efm_sweep_solve")
+        return X, dict(iters=iters, resid=resid)
+
+    def close(self):
+        if getattr(self, "_h", None):
+            self._lib.efm_sweep_destroy(self._h)
+            self._h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass

@@ -1,0 +1,684 @@
+// Batched frequency sweep: many (frequency, excitation) systems that share one sparsity
+// pattern, solved together by preconditioned COCG with per-system convergence masks.
+//
+//   A_s = K - k0_s^2 M + sum_p c_{s,p} M_s,p        (c_{s,p} = j beta_p(f_s) scale)
+//
+// Replaces the per-frequency loop of RectWaveguideDesign.frequency_sweep
+// (python/edgefem/designs/waveguide.py:394-434), i.e. calculate_sparams_eigenmode
+// (src/assemble_maxwell.cpp:637-787) per frequency, and the K/M reuse idea of
+// frequency_sweep (src/sweep.cpp:192-329).
+//
+// Layout: every vector is [row][system]; the 64 lanes of a wavefront are 64 systems,
+// so a column gather of a wave is one contiguous 1 KB read, and the shared matrix
+// entries of a row are uniform across the wave.  The iteration is the single-reduction
+// (Chronopoulos-Gear) form of COCG, so the SpMV gathers only one vector.
+// The preconditioner is the auxiliary-gradient one (edge Jacobi + nodal correction);
+// its per-system diagonals are formed on the fly from frequency-independent parts:
+//   D_s(e)  = K_ee - k0_s^2 M_ee + c_{s,port(e)} Ms_ee
+//   Dn_s(n) = dK(n) - k0_s^2 dM(n) + sum_p c_{s,p} dMs_p(n),   d*(n) = diag(G^T * G)
+// Partial sums are reduced per system (one block per system) in a fixed order, so the
+// result of a system does not depend on which other systems share the launch.
+#include "efm_internal.h"
+
+#include <vector>
+
+using namespace efm;
+
+namespace {
+
+constexpr int kBlock = 256;
+constexpr int kSys = 64;           // systems per wavefront (one lane each)
+constexpr int kWaves = kBlock / 64;
+constexpr int kMaxPorts = 4;
+
+struct SwState {
+  double2 rho, alpha, beta;
+  double bnorm2, rnorm2;
+  int active, iters, breakdown, pad;
+};
+
+__device__ inline double2 bil(double2 a, double2 b) {
+  return make_double2(a.x * b.x - a.y * b.y, a.x * b.y + a.y * b.x);
+}
+
+__device__ inline double2 port_coef(const double2 *__restrict__ coef, int s, int P, int port) {
+  return (port >= 0) ? coef[s * P + port] : make_double2(0, 0);
+}
+
+// Per-block partial sums of V values per system: the kWaves waves of a block handle
+// the same 64 systems on different rows; combine them in LDS, fixed order.
+template <int V>
+__device__ inline void write_sys_partials(double2 (&v)[V], double2 *lds, double2 *partial,
+                                          int block_linear) {
+  const int w = threadIdx.x >> 6, lane = threadIdx.x & 63;
+#pragma unroll
+  for (int i = 0; i < V; ++i)
+    lds[(w * V + i) * 64 + lane] = v[i];
+  __syncthreads();
+  if (w == 0) {
+#pragma unroll
+    for (int i = 0; i < V; ++i) {
+      double2 a = lds[i * 64 + lane];
+      for (int ww = 1; ww < kWaves; ++ww)
+        a = cadd(a, lds[(ww * V + i) * 64 + lane]);
+      partial[((int64_t)block_linear * V + i) * 64 + lane] = a;
+    }
+  }
+}
+
+// Persistent XCD-aware walk over row chunks of kWaves rows: block bx runs on XCD bx % 8.
+struct ChunkWalk {
+  int64_t begin, end, stride;
+  __device__ ChunkWalk(int64_t nchunks) {
+    const int64_t span = (nchunks + 7) / 8;
+    const int xcd = blockIdx.x & 7;
+    begin = xcd * span + (blockIdx.x >> 3);
+    end = min(nchunks, (int64_t)(xcd + 1) * span);
+    stride = gridDim.x >> 3;
+  }
+};
+
+// ---------------------------------------------------------------------------
+// Chronopoulos-Gear COCG (one reduction point per iteration; the SpMV reads only u):
+//   r0 = b, u0 = P r0, w0 = A u0, g0 = u0^T r0, d0 = u0^T w0, a0 = g0/d0, b0 = 0
+//   p_i = u_i + b_i p_{i-1};  s_i = w_i + b_i s_{i-1}     (s = A p)
+//   x_{i+1} = x_i + a_i p_i;   r_{i+1} = r_i - a_i s_i
+//   u_{i+1} = P r_{i+1};  w_{i+1} = A u_{i+1};  g = u^T r;  d = u^T w
+//   b_{i+1} = g_{i+1}/g_i;  a_{i+1} = g_{i+1} / (d_{i+1} - b_{i+1} g_{i+1} / a_i)
+// (bilinear forms: COCG for complex-symmetric A).  Kernels per iteration: aux_node
+// (nodal part of P on r_{i+1}, formed on the fly), update (p, s, x, r, u and partials
+// of g, |r|^2), spmv (w and partials of d), finalize (scalars, convergence).
+// ---------------------------------------------------------------------------
+__global__ void __launch_bounds__(kBlock)
+    k_sw_init(int64_t n, int B, const double2 *__restrict__ b, double2 *__restrict__ x,
+              double2 *__restrict__ r, double2 *__restrict__ p, double2 *__restrict__ sv) {
+  for (int64_t i = (int64_t)blockIdx.x * kBlock + threadIdx.x; i < n * B; i += (int64_t)gridDim.x * kBlock) {
+    x[i] = make_double2(0, 0);
+    r[i] = b[i];
+    p[i] = make_double2(0, 0);
+    sv[i] = make_double2(0, 0);
+  }
+}
+
+// y[n][s] = Dn_s(n)^{-1} sum_k G (r - alpha (w + beta s))[e][s]   (FUSED), or G^T r
+template <bool FUSED>
+__global__ void __launch_bounds__(kBlock)
+    k_sw_aux_node(int64_t N, int B, int P, const int32_t *__restrict__ n2e_ptr,
+                  const int32_t *__restrict__ n2e, const int8_t *__restrict__ gsign,
+                  const double2 *__restrict__ dnK, const double2 *__restrict__ dnM,
+                  const double *__restrict__ dnMs, const double *__restrict__ k0sq,
+                  const double2 *__restrict__ coef, const double2 *__restrict__ r,
+                  const double2 *__restrict__ w, const double2 *__restrict__ sv,
+                  double2 *__restrict__ y, const SwState *__restrict__ st) {
+  const int s = blockIdx.y * kSys + (threadIdx.x & 63);
+  const int wv = threadIdx.x >> 6;
+  if (s >= B || !st[s].active)
+    return;
+  const double2 alpha = FUSED ? st[s].alpha : make_double2(0, 0);
+  const double2 beta = FUSED ? st[s].beta : make_double2(0, 0);
+  const double k2 = k0sq[s];
+  for (int64_t nd = (int64_t)blockIdx.x * kWaves + wv; nd < N; nd += (int64_t)gridDim.x * kWaves) {
+    double2 acc = make_double2(0, 0);
+    for (int32_t k = n2e_ptr[nd]; k < n2e_ptr[nd + 1]; ++k) {
+      const int gs = gsign[k];
+      if (!gs)
+        continue;
+      const int64_t i = (int64_t)n2e[k] * B + s;
+      double2 rv = r[i];
+      if (FUSED)
+        rv = csub(rv, bil(alpha, cfma(beta, sv[i], w[i])));
+      acc.x += gs * rv.x;
+      acc.y += gs * rv.y;
+    }
+    double2 d = make_double2(dnK[nd].x - k2 * dnM[nd].x, dnK[nd].y - k2 * dnM[nd].y);
+    for (int p = 0; p < P; ++p) {
+      const double m = dnMs[(int64_t)p * N + nd];
+      const double2 c = coef[s * P + p];
+      d.x += c.x * m;
+      d.y += c.y * m;
+    }
+    y[nd * B + s] = (d.x == 0.0 && d.y == 0.0) ? make_double2(0, 0) : cdiv(acc, d);
+  }
+}
+
+__device__ inline double2 edge_diag_inv(int64_t e, int s, int P, double k2, const double2 *dK,
+                                        const double2 *dM, const double *dMs,
+                                        const int8_t *rport, const double2 *coef) {
+  const double2 c = port_coef(coef, s, P, rport[e]);
+  const double m = dMs[e];
+  double2 d = make_double2(dK[e].x - k2 * dM[e].x + c.x * m, dK[e].y - k2 * dM[e].y + c.y * m);
+  return (d.x == 0.0 && d.y == 0.0) ? make_double2(1, 0) : cdiv(make_double2(1, 0), d);
+}
+
+// INIT: u = P r; partials g = u^T r, |r|^2.
+// else: p = u + beta p; s = w + beta s; x += alpha p; r -= alpha s; u = P r; partials.
+template <bool INIT>
+__global__ void __launch_bounds__(kBlock)
+    k_sw_update(int64_t n, int B, int P, const int2 *__restrict__ en_eff,
+                const double2 *__restrict__ dK, const double2 *__restrict__ dM,
+                const double *__restrict__ dMs, const int8_t *__restrict__ rport,
+                const double *__restrict__ k0sq, const double2 *__restrict__ coef,
+                const double2 *__restrict__ y, const double2 *__restrict__ w,
+                double2 *__restrict__ u, double2 *__restrict__ p, double2 *__restrict__ sv,
+                double2 *__restrict__ x, double2 *__restrict__ r, double2 *__restrict__ partial,
+                const SwState *__restrict__ st) {
+  __shared__ double2 lds[kWaves * 2 * 64];
+  const int s = blockIdx.y * kSys + (threadIdx.x & 63);
+  const int wv = threadIdx.x >> 6;
+  double2 v[2] = {make_double2(0, 0), make_double2(0, 0)};
+  const bool act = s < B && st[s].active != 0;
+  if (act) {
+    const double k2 = k0sq[s];
+    const double2 alpha = st[s].alpha, beta = st[s].beta;
+    for (int64_t e = (int64_t)blockIdx.x * kWaves + wv; e < n; e += (int64_t)gridDim.x * kWaves) {
+      const int64_t i = e * B + s;
+      double2 rv = r[i];
+      if (!INIT) {
+        const double2 pv = cfma(beta, p[i], u[i]);
+        const double2 svv = cfma(beta, sv[i], w[i]);
+        x[i] = cfma(alpha, pv, x[i]);
+        rv = csub(rv, bil(alpha, svv));
+        p[i] = pv;
+        sv[i] = svv;
+        r[i] = rv;
+      }
+      const double2 di = edge_diag_inv(e, s, P, k2, dK, dM, dMs, rport, coef);
+      double2 z = bil(di, rv);
+      const int2 ab = en_eff[e];
+      if (ab.x >= 0)
+        z = cadd(z, csub(y[(int64_t)ab.y * B + s], y[(int64_t)ab.x * B + s]));
+      u[i] = z;
+      v[0] = cadd(v[0], bil(rv, z));
+      v[1].x += cabs2(rv);
+    }
+  }
+  write_sys_partials<2>(v, lds, partial, blockIdx.y * gridDim.x + blockIdx.x);
+}
+
+// w = A_s u; partials d = u^T w.  Persistent, XCD-aware: the blocks of one XCD walk
+// the XCD's contiguous share of the rows, so the gathered columns of u (64 systems x
+// 16 B per column) stay in that XCD's L2; blockIdx.y strides over the system groups.
+__global__ void __launch_bounds__(kBlock)
+    k_sw_spmv(int64_t n, int B, int P, int groups, const int32_t *__restrict__ row_ptr,
+              const int32_t *__restrict__ col, const double2 *__restrict__ K,
+              const double2 *__restrict__ M, const double *__restrict__ Ms,
+              const int8_t *__restrict__ rport, const double *__restrict__ k0sq,
+              const double2 *__restrict__ coef, const double2 *__restrict__ u,
+              double2 *__restrict__ w, double2 *__restrict__ partial,
+              const SwState *__restrict__ st) {
+  __shared__ double2 lds[kWaves * 64];
+  const int lane = threadIdx.x & 63;
+  const int wv = threadIdx.x >> 6;
+  const int64_t nchunks = (n + kWaves - 1) / kWaves;
+  ChunkWalk cw(nchunks);
+  for (int g = blockIdx.y; g < groups; g += gridDim.y) {
+    const int s = g * kSys + lane;
+    double2 v[1] = {make_double2(0, 0)};
+    const bool act = s < B && st[s].active != 0;
+    if (act) {
+      const double k2 = k0sq[s];
+      for (int64_t chunk = cw.begin; chunk < cw.end; chunk += cw.stride) {
+        const int64_t row = chunk * kWaves + wv;
+        if (row >= n)
+          continue;
+        const double2 c = port_coef(coef, s, P, rport[row]);
+        double2 acc = make_double2(0, 0);
+        for (int32_t k = row_ptr[row]; k < row_ptr[row + 1]; ++k) {
+          const double2 kv = K[k], mv = M[k];
+          const double ms = Ms[k];
+          const double2 a = make_double2(kv.x - k2 * mv.x + c.x * ms, kv.y - k2 * mv.y + c.y * ms);
+          acc = cfma(a, u[(int64_t)col[k] * B + s], acc);
+        }
+        w[row * B + s] = acc;
+        v[0] = cadd(v[0], bil(u[row * B + s], acc));
+      }
+    }
+    write_sys_partials<1>(v, lds, partial, g * gridDim.x + blockIdx.x);
+    __syncthreads();  // lds reuse by the next group
+  }
+}
+
+// One block per system: reduce its partials of g, |r|^2 (update grid, gxu blocks) and
+// d (spmv grid, gxs blocks) in a fixed order and advance the scalars.
+template <bool INIT>
+__global__ void __launch_bounds__(kBlock)
+    k_sw_finalize(int B, int gxu, int gxs, const double2 *__restrict__ pu,
+                  const double2 *__restrict__ ps, SwState *__restrict__ st,
+                  int *__restrict__ active_flag, double tol2, int max_it) {
+  const int s = blockIdx.x;
+  if (s >= B)
+    return;
+  const int gy = s / kSys, lane = s % kSys;
+  double2 g = make_double2(0, 0), rr = make_double2(0, 0), d = make_double2(0, 0);
+  for (int bx = threadIdx.x; bx < gxu; bx += kBlock) {
+    const int64_t bl = (int64_t)gy * gxu + bx;
+    g = cadd(g, pu[(bl * 2 + 0) * 64 + lane]);
+    rr = cadd(rr, pu[(bl * 2 + 1) * 64 + lane]);
+  }
+  for (int bx = threadIdx.x; bx < gxs; bx += kBlock) {
+    const int64_t bl = (int64_t)gy * gxs + bx;
+    d = cadd(d, ps[bl * 64 + lane]);
+  }
+  __shared__ double2 sh[3][kBlock];
+  sh[0][threadIdx.x] = g;
+  sh[1][threadIdx.x] = rr;
+  sh[2][threadIdx.x] = d;
+  __syncthreads();
+  for (int off = kBlock / 2; off > 0; off >>= 1) {
+    if (threadIdx.x < off)
+      for (int i = 0; i < 3; ++i)
+        sh[i][threadIdx.x] = cadd(sh[i][threadIdx.x], sh[i][threadIdx.x + off]);
+    __syncthreads();
+  }
+  if (threadIdx.x != 0)
+    return;
+  g = sh[0][0];
+  const double rn2 = sh[1][0].x;
+  d = sh[2][0];
+  SwState &S = st[s];
+  if (INIT) {
+    S.bnorm2 = rn2;
+    S.rnorm2 = rn2;
+    S.iters = 0;
+    S.breakdown = 0;
+    S.beta = make_double2(0, 0);
+    S.rho = g;
+    S.active = rn2 > 0.0 ? 1 : 0;
+    if (S.active) {
+      if (d.x == 0.0 && d.y == 0.0) {
+        S.breakdown = 1;
+        S.active = 0;
+      } else {
+        S.alpha = cdiv(g, d);
+      }
+    }
+  } else if (S.active) {
+    S.iters += 1;
+    S.rnorm2 = rn2;
+    if (rn2 <= tol2 * S.bnorm2) {
+      S.active = 0;
+    } else if (S.iters >= max_it || (S.rho.x == 0.0 && S.rho.y == 0.0)) {
+      S.active = 0;
+    } else {
+      const double2 beta = cdiv(g, S.rho);
+      const double2 den = csub(d, cdiv(cmul(beta, g), S.alpha));
+      if (den.x == 0.0 && den.y == 0.0) {
+        S.breakdown = 1;
+        S.active = 0;
+      } else {
+        S.beta = beta;
+        S.alpha = cdiv(g, den);
+        S.rho = g;
+      }
+    }
+  }
+  active_flag[s] = S.active;
+}
+
+// Frequency-independent diagonals: edge diagonals of K, M, Ms; nodal diag(G^T X G).
+__global__ void k_sw_edge_diag(int64_t n, const int32_t *__restrict__ row_ptr,
+                               const int32_t *__restrict__ col, const double2 *__restrict__ K,
+                               const double2 *__restrict__ M, const double *__restrict__ Ms,
+                               double2 *__restrict__ dK, double2 *__restrict__ dM,
+                               double *__restrict__ dMs) {
+  int64_t e = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (e >= n)
+    return;
+  double2 a = make_double2(0, 0), b = make_double2(0, 0);
+  double c = 0.0;
+  for (int32_t k = row_ptr[e]; k < row_ptr[e + 1]; ++k)
+    if (col[k] == e) {
+      a = K[k];
+      b = M[k];
+      c = Ms[k];
+    }
+  dK[e] = a;
+  dM[e] = b;
+  dMs[e] = c;
+}
+
+__device__ inline int gsign_of(const int32_t *__restrict__ en, int32_t e, int32_t node) {
+  const int32_t a = en[2 * (int64_t)e], b = en[2 * (int64_t)e + 1];
+  if (a == b)
+    return 0;
+  return (node == b) ? 1 : ((node == a) ? -1 : 0);
+}
+
+// one thread per node: diag(G^T X G) for X in {K, M, Ms_p} (Dirichlet edges dropped)
+__global__ void k_sw_node_diag(int64_t N, int P, const int32_t *__restrict__ n2e_ptr,
+                               const int32_t *__restrict__ n2e, const int32_t *__restrict__ en,
+                               const uint8_t *__restrict__ dir, const int32_t *__restrict__ row_ptr,
+                               const int32_t *__restrict__ col, const double2 *__restrict__ K,
+                               const double2 *__restrict__ M, const double *const *__restrict__ Msp,
+                               double2 *__restrict__ dnK, double2 *__restrict__ dnM,
+                               double *__restrict__ dnMs) {
+  int64_t nd = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (nd >= N)
+    return;
+  double2 sk = make_double2(0, 0), sm = make_double2(0, 0);
+  double sp[kMaxPorts] = {0, 0, 0, 0};
+  for (int32_t k = n2e_ptr[nd]; k < n2e_ptr[nd + 1]; ++k) {
+    const int32_t e = n2e[k];
+    if (dir && dir[e])
+      continue;
+    const int ge = gsign_of(en, e, (int32_t)nd);
+    if (!ge)
+      continue;
+    for (int32_t q = row_ptr[e]; q < row_ptr[e + 1]; ++q) {
+      const int32_t f = col[q];
+      if (dir && dir[f])
+        continue;
+      const int gf = gsign_of(en, f, (int32_t)nd);
+      if (!gf)
+        continue;
+      const double sg = (double)(ge * gf);
+      sk.x += sg * K[q].x;
+      sk.y += sg * K[q].y;
+      sm.x += sg * M[q].x;
+      sm.y += sg * M[q].y;
+      for (int p = 0; p < P; ++p)
+        sp[p] += sg * Msp[p][q];
+    }
+  }
+  dnK[nd] = sk;
+  dnM[nd] = sm;
+  for (int p = 0; p < P; ++p)
+    dnMs[(int64_t)p * N + nd] = sp[p];
+}
+
+__global__ void k_sw_gsign(int64_t N, const int32_t *__restrict__ n2e_ptr, const int32_t *__restrict__ n2e,
+                           const int32_t *__restrict__ en, const uint8_t *__restrict__ dir,
+                           int8_t *__restrict__ gsign) {
+  int64_t nd = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (nd >= N)
+    return;
+  for (int32_t k = n2e_ptr[nd]; k < n2e_ptr[nd + 1]; ++k) {
+    const int32_t e = n2e[k];
+    gsign[k] = (dir && dir[e]) ? 0 : (int8_t)gsign_of(en, e, (int32_t)nd);
+  }
+}
+
+__global__ void k_sw_edge_prep(int64_t n, const int32_t *__restrict__ en, const uint8_t *__restrict__ dir,
+                               int2 *__restrict__ en_eff) {
+  int64_t e = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (e >= n)
+    return;
+  const int32_t a = en[2 * e], b = en[2 * e + 1];
+  en_eff[e] = ((dir && dir[e]) || a == b) ? make_int2(-1, -1) : make_int2(a, b);
+}
+
+// port index of every row (-1 none), and the combined port mass; error flag if two
+// ports share a row
+__global__ void k_sw_ports(int64_t n, int P, const int32_t *__restrict__ row_ptr,
+                           const double *const *__restrict__ Msp, double *__restrict__ Ms,
+                           int8_t *__restrict__ rport, int *__restrict__ err) {
+  int64_t r = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (r >= n)
+    return;
+  int port = -1;
+  for (int32_t k = row_ptr[r]; k < row_ptr[r + 1]; ++k) {
+    double m = 0.0;
+    for (int p = 0; p < P; ++p) {
+      const double v = Msp[p][k];
+      if (v != 0.0) {
+        if (port >= 0 && port != p)
+          atomicOr(err, 1);
+        port = p;
+        m = v;
+      }
+    }
+    Ms[k] = m;
+  }
+  rport[r] = (int8_t)port;
+}
+
+} // namespace
+
+struct efm_sweep {
+  int64_t n = 0, nnz = 0, N = 0;
+  int P = 0, Bmax = 0;
+  const int32_t *row_ptr = nullptr, *col = nullptr;
+  const int32_t *n2e_ptr = nullptr, *n2e = nullptr;
+  int gx_vec = 8, gx_nodes = 8, gx_spmv = 8, gy_spmv = 1;
+  efm::DevBuf K, M, Ms, rport, dK, dM, dMs, dnK, dnM, dnMs, gsign, en_eff;
+  efm::DevBuf x, r, p, sv, u, w, y, part_u, part_s, state, flags, k0sq, coef;
+};
+
+extern "C" int efm_sweep_create(const efm_topology *topo, const double *K_val, const double *M_val,
+                                int n_ports, const double *const *h_Ms_vals,
+                                const uint8_t *dirichlet, int max_systems, void *stream,
+                                efm_sweep **out) {
+  EFM_REQUIRE(out, "efm_sweep_create: out is NULL");
+  *out = nullptr;
+  EFM_REQUIRE(topo && K_val && M_val, "efm_sweep_create: NULL argument");
+  EFM_REQUIRE(n_ports >= 0 && n_ports <= kMaxPorts, "efm_sweep_create: 0..4 ports");
+  EFM_REQUIRE(n_ports == 0 || h_Ms_vals, "efm_sweep_create: port masses missing");
+  EFM_REQUIRE(max_systems >= 1 && max_systems <= 1024, "efm_sweep_create: 1..1024 systems");
+  hipStream_t st = (hipStream_t)stream;
+  auto s = std::make_unique<efm_sweep>();
+  const int64_t n = topo->n_edges, nnz = topo->nnz, N = topo->N;
+  s->n = n;
+  s->nnz = nnz;
+  s->N = N;
+  s->P = n_ports;
+  s->Bmax = ((max_systems + kSys - 1) / kSys) * kSys;
+  s->row_ptr = topo->row_ptr.as<int32_t>();
+  s->col = topo->col_idx.as<int32_t>();
+  s->n2e_ptr = topo->n2e_ptr.as<int32_t>();
+  s->n2e = topo->n2e.as<int32_t>();
+  int rc;
+  const size_t vb = (size_t)n * s->Bmax * sizeof(double2);
+  int dev = 0, ncu = 256;
+  if (hipGetDevice(&dev) == hipSuccess) {
+    hipDeviceProp_t prop;
+    if (hipGetDeviceProperties(&prop, dev) == hipSuccess && prop.multiProcessorCount > 0)
+      ncu = prop.multiProcessorCount;
+  }
+  const int groups = s->Bmax / kSys;
+  // blocks per system group: enough waves to fill the chip, a multiple of 8 (XCD walk)
+  auto gx_for = [&](int64_t items) {
+    int64_t want = std::max<int64_t>(8, (int64_t)ncu * 8 / groups);
+    int64_t need = (items + kWaves - 1) / kWaves;
+    int64_t g = std::min(want, std::max<int64_t>(need, 1));
+    return (int)(((g + 7) / 8) * 8);
+  };
+  s->gx_vec = gx_for(n);
+  s->gx_nodes = gx_for(N);
+  // SpMV grid: gx_spmv blocks per group slice, gy_spmv slices striding over the groups
+  {
+    const int64_t need = (n + kWaves - 1) / kWaves;
+    const int64_t want = (int64_t)ncu * 2;
+    s->gy_spmv = groups;
+    const int64_t g = std::min<int64_t>(want, std::max<int64_t>(need, 8));
+    s->gx_spmv = (int)(((g + 7) / 8) * 8);
+  }
+  if ((rc = s->K.alloc(nnz * 16)) || (rc = s->M.alloc(nnz * 16)) || (rc = s->Ms.alloc(nnz * 8)) ||
+      (rc = s->rport.alloc(n + 16)) || (rc = s->dK.alloc(n * 16)) || (rc = s->dM.alloc(n * 16)) ||
+      (rc = s->dMs.alloc(n * 8)) || (rc = s->dnK.alloc(N * 16)) || (rc = s->dnM.alloc(N * 16)) ||
+      (rc = s->dnMs.alloc((size_t)N * 8 * std::max(1, n_ports))) || (rc = s->gsign.alloc(2 * n + 16)) ||
+      (rc = s->en_eff.alloc(n * sizeof(int2))) || (rc = s->x.alloc(vb)) || (rc = s->r.alloc(vb)) ||
+      (rc = s->p.alloc(vb)) || (rc = s->sv.alloc(vb)) || (rc = s->u.alloc(vb)) || (rc = s->w.alloc(vb)) ||
+      (rc = s->y.alloc((size_t)N * s->Bmax * sizeof(double2))) ||
+      (rc = s->part_u.alloc((size_t)groups * s->gx_vec * 2 * 64 * sizeof(double2))) ||
+      (rc = s->part_s.alloc((size_t)groups * s->gx_spmv * 64 * sizeof(double2))) ||
+      (rc = s->state.alloc(s->Bmax * sizeof(SwState))) || (rc = s->flags.alloc(s->Bmax * sizeof(int))) ||
+      (rc = s->k0sq.alloc(s->Bmax * sizeof(double))) ||
+      (rc = s->coef.alloc((size_t)s->Bmax * std::max(1, n_ports) * sizeof(double2))))
+    return rc;
+  EFM_HIP_CHECK(hipMemcpyAsync(s->K.p, K_val, nnz * 16, hipMemcpyDeviceToDevice, st));
+  EFM_HIP_CHECK(hipMemcpyAsync(s->M.p, M_val, nnz * 16, hipMemcpyDeviceToDevice, st));
+  // port arrays: device pointer table
+  DevBuf ptab, err;
+  if ((rc = ptab.alloc(sizeof(double *) * kMaxPorts)) || (rc = err.alloc(sizeof(int))))
+    return rc;
+  const double *hp[kMaxPorts] = {nullptr, nullptr, nullptr, nullptr};
+  for (int p = 0; p < n_ports; ++p)
+    hp[p] = h_Ms_vals[p];
+  EFM_HIP_CHECK(hipMemcpyAsync(ptab.p, hp, sizeof(hp), hipMemcpyHostToDevice, st));
+  EFM_HIP_CHECK(hipMemsetAsync(err.p, 0, sizeof(int), st));
+  hipLaunchKernelGGL(k_sw_ports, dim3(grid_for(n, 256)), dim3(256), 0, st, n, n_ports, s->row_ptr,
+                     (const double *const *)ptab.p, s->Ms.as<double>(), s->rport.as<int8_t>(), err.as<int>());
+  EFM_LAUNCH_CHECK();
+  hipLaunchKernelGGL(k_sw_edge_diag, dim3(grid_for(n, 256)), dim3(256), 0, st, n, s->row_ptr, s->col,
+                     s->K.as<double2>(), s->M.as<double2>(), s->Ms.as<double>(), s->dK.as<double2>(),
+                     s->dM.as<double2>(), s->dMs.as<double>());
+  EFM_LAUNCH_CHECK();
+  const int32_t *en = topo->edge_nidx.as<int32_t>();
+  hipLaunchKernelGGL(k_sw_node_diag, dim3(grid_for(N, 256)), dim3(256), 0, st, N, n_ports, s->n2e_ptr,
+                     s->n2e, en, dirichlet, s->row_ptr, s->col, s->K.as<double2>(), s->M.as<double2>(),
+                     (const double *const *)ptab.p, s->dnK.as<double2>(), s->dnM.as<double2>(),
+                     s->dnMs.as<double>());
+  EFM_LAUNCH_CHECK();
+  hipLaunchKernelGGL(k_sw_gsign, dim3(grid_for(N, 256)), dim3(256), 0, st, N, s->n2e_ptr, s->n2e, en,
+                     dirichlet, s->gsign.as<int8_t>());
+  EFM_LAUNCH_CHECK();
+  hipLaunchKernelGGL(k_sw_edge_prep, dim3(grid_for(n, 256)), dim3(256), 0, st, n, en, dirichlet,
+                     s->en_eff.as<int2>());
+  EFM_LAUNCH_CHECK();
+  int h_err = 0;
+  EFM_HIP_CHECK(hipMemcpyAsync(&h_err, err.p, sizeof(int), hipMemcpyDeviceToHost, st));
+  EFM_HIP_CHECK(hipStreamSynchronize(st));
+  EFM_REQUIRE(h_err == 0, "efm_sweep_create: two ports share an edge row (not supported)");
+  *out = s.release();
+  return EFM_OK;
+}
+
+extern "C" void efm_sweep_destroy(efm_sweep *s) { delete s; }
+
+extern "C" int efm_sweep_solve(efm_sweep *s, int nsys, const double *h_k0sq, const double *h_coef,
+                               const double *B, double *X, const efm_solve_opts *h_opts,
+                               int32_t *h_iters, double *h_resid, void *stream) {
+  EFM_REQUIRE(s && h_k0sq && B && X && h_opts, "efm_sweep_solve: NULL argument");
+  EFM_REQUIRE(nsys >= 1 && nsys <= s->Bmax, "efm_sweep_solve: nsys out of range");
+  EFM_REQUIRE(s->P == 0 || h_coef, "efm_sweep_solve: port coefficients missing");
+  EFM_REQUIRE(h_opts->method == EFM_METHOD_COCG, "efm_sweep_solve: the sweep uses COCG");
+  hipStream_t st = (hipStream_t)stream;
+  const int Bp = ((nsys + kSys - 1) / kSys) * kSys;  // padded system count this call
+  const int groups = Bp / kSys;
+  const int P = s->P;
+  const double tol2 = h_opts->tolerance * h_opts->tolerance;
+  const int max_it = h_opts->max_iterations;
+  const int check = h_opts->check_every > 0 ? h_opts->check_every : 16;
+  // per-system scalars (padding systems get zero right-hand sides and stay inactive)
+  std::vector<double> k2(Bp, 0.0);
+  std::vector<double2> cf((size_t)Bp * std::max(1, P), make_double2(0, 0));
+  for (int i = 0; i < nsys; ++i) {
+    k2[i] = h_k0sq[i];
+    for (int p = 0; p < P; ++p)
+      cf[(size_t)i * P + p] = make_double2(h_coef[2 * ((size_t)i * P + p)], h_coef[2 * ((size_t)i * P + p) + 1]);
+  }
+  EFM_HIP_CHECK(hipMemcpyAsync(s->k0sq.p, k2.data(), Bp * sizeof(double), hipMemcpyHostToDevice, st));
+  EFM_HIP_CHECK(hipMemcpyAsync(s->coef.p, cf.data(), cf.size() * sizeof(double2), hipMemcpyHostToDevice, st));
+  // right-hand side with padding columns set to zero: rows of B are nsys wide
+  DevBuf Bpad;
+  const double2 *Bd = (const double2 *)B;
+  if (Bp != nsys) {
+    int rc = Bpad.alloc((size_t)s->n * Bp * sizeof(double2));
+    if (rc != EFM_OK)
+      return rc;
+    EFM_HIP_CHECK(hipMemsetAsync(Bpad.p, 0, (size_t)s->n * Bp * sizeof(double2), st));
+    EFM_HIP_CHECK(hipMemcpy2DAsync(Bpad.p, Bp * sizeof(double2), B, nsys * sizeof(double2),
+                                   nsys * sizeof(double2), s->n, hipMemcpyDeviceToDevice, st));
+    Bd = Bpad.as<double2>();
+  }
+  SwState *S = s->state.as<SwState>();
+  int *flags = s->flags.as<int>();
+  double2 *pu = s->part_u.as<double2>(), *ps = s->part_s.as<double2>();
+  const int2 *en = s->en_eff.as<int2>();
+  double2 *x = s->x.as<double2>(), *r = s->r.as<double2>(), *p = s->p.as<double2>();
+  double2 *sv = s->sv.as<double2>(), *u = s->u.as<double2>(), *w = s->w.as<double2>();
+  double2 *y = s->y.as<double2>();
+  hipLaunchKernelGGL(k_sw_init, dim3(grid_for(s->n * Bp, kBlock)), dim3(kBlock), 0, st, s->n, Bp, Bd, x, r,
+                     p, sv);
+  EFM_LAUNCH_CHECK();
+  // every system takes part in the first preconditioner application and SpMV
+  {
+    std::vector<SwState> init(Bp);
+    for (auto &v : init) {
+      v = SwState{};
+      v.active = 1;
+    }
+    EFM_HIP_CHECK(hipMemcpyAsync(S, init.data(), Bp * sizeof(SwState), hipMemcpyHostToDevice, st));
+  }
+  auto aux_node = [&](bool fused) -> int {
+    if (fused)
+      hipLaunchKernelGGL((k_sw_aux_node<true>), dim3(s->gx_nodes, groups), dim3(kBlock), 0, st, s->N, Bp, P,
+                         s->n2e_ptr, s->n2e, s->gsign.as<int8_t>(), s->dnK.as<double2>(), s->dnM.as<double2>(),
+                         s->dnMs.as<double>(), s->k0sq.as<double>(), s->coef.as<double2>(), r, w, sv, y, S);
+    else
+      hipLaunchKernelGGL((k_sw_aux_node<false>), dim3(s->gx_nodes, groups), dim3(kBlock), 0, st, s->N, Bp, P,
+                         s->n2e_ptr, s->n2e, s->gsign.as<int8_t>(), s->dnK.as<double2>(), s->dnM.as<double2>(),
+                         s->dnMs.as<double>(), s->k0sq.as<double>(), s->coef.as<double2>(), r,
+                         (const double2 *)nullptr, (const double2 *)nullptr, y, S);
+    EFM_LAUNCH_CHECK();
+    return EFM_OK;
+  };
+  auto update = [&](bool first) -> int {
+    if (first)
+      hipLaunchKernelGGL((k_sw_update<true>), dim3(s->gx_vec, groups), dim3(kBlock), 0, st, s->n, Bp, P, en,
+                         s->dK.as<double2>(), s->dM.as<double2>(), s->dMs.as<double>(), s->rport.as<int8_t>(),
+                         s->k0sq.as<double>(), s->coef.as<double2>(), y, w, u, p, sv, x, r, pu, S);
+    else
+      hipLaunchKernelGGL((k_sw_update<false>), dim3(s->gx_vec, groups), dim3(kBlock), 0, st, s->n, Bp, P, en,
+                         s->dK.as<double2>(), s->dM.as<double2>(), s->dMs.as<double>(), s->rport.as<int8_t>(),
+                         s->k0sq.as<double>(), s->coef.as<double2>(), y, w, u, p, sv, x, r, pu, S);
+    EFM_LAUNCH_CHECK();
+    return EFM_OK;
+  };
+  auto spmv = [&]() -> int {
+    hipLaunchKernelGGL(k_sw_spmv, dim3(s->gx_spmv, s->gy_spmv), dim3(kBlock), 0, st, s->n, Bp, P, groups, s->row_ptr,
+                       s->col, s->K.as<double2>(), s->M.as<double2>(), s->Ms.as<double>(), s->rport.as<int8_t>(),
+                       s->k0sq.as<double>(), s->coef.as<double2>(), u, w, ps, S);
+    EFM_LAUNCH_CHECK();
+    return EFM_OK;
+  };
+  auto finalize = [&](bool first) -> int {
+    if (first)
+      hipLaunchKernelGGL((k_sw_finalize<true>), dim3(Bp), dim3(kBlock), 0, st, Bp, s->gx_vec, s->gx_spmv, pu, ps,
+                         S, flags, tol2, max_it);
+    else
+      hipLaunchKernelGGL((k_sw_finalize<false>), dim3(Bp), dim3(kBlock), 0, st, Bp, s->gx_vec, s->gx_spmv, pu,
+                         ps, S, flags, tol2, max_it);
+    EFM_LAUNCH_CHECK();
+    return EFM_OK;
+  };
+  std::vector<int> hflags(Bp);
+  auto n_active = [&](int &count) -> int {
+    EFM_HIP_CHECK(hipMemcpyAsync(hflags.data(), flags, Bp * sizeof(int), hipMemcpyDeviceToHost, st));
+    EFM_HIP_CHECK(hipStreamSynchronize(st));
+    count = 0;
+    for (int v : hflags)
+      count += v != 0;
+    return EFM_OK;
+  };
+  int rc, active = 0;
+  if ((rc = aux_node(false)) || (rc = update(true)) || (rc = spmv()) || (rc = finalize(true)) ||
+      (rc = n_active(active)))
+    return rc;
+  int it = 0;
+  while (active > 0 && it < max_it) {
+    const int chunk = std::min(check, max_it - it);
+    for (int c = 0; c < chunk; ++c, ++it)
+      if ((rc = aux_node(true)) || (rc = update(false)) || (rc = spmv()) || (rc = finalize(false)))
+        return rc;
+    if ((rc = n_active(active)))
+      return rc;
+  }
+  // solution: the first nsys columns of x
+  if (Bp == nsys) {
+    EFM_HIP_CHECK(hipMemcpyAsync(X, s->x.p, (size_t)s->n * Bp * sizeof(double2), hipMemcpyDeviceToDevice, st));
+  } else {
+    EFM_HIP_CHECK(hipMemcpy2DAsync(X, nsys * sizeof(double2), s->x.p, Bp * sizeof(double2),
+                                   nsys * sizeof(double2), s->n, hipMemcpyDeviceToDevice, st));
+  }
+  std::vector<SwState> hs(Bp);
+  EFM_HIP_CHECK(hipMemcpyAsync(hs.data(), S, Bp * sizeof(SwState), hipMemcpyDeviceToHost, st));
+  EFM_HIP_CHECK(hipStreamSynchronize(st));
+  for (int i = 0; i < nsys; ++i) {
+    if (h_iters)
+      h_iters[i] = hs[i].iters;
+    if (h_resid)
+      h_resid[i] = hs[i].bnorm2 > 0 ? sqrt(hs[i].rnorm2 / hs[i].bnorm2) : 0.0;
+  }
+  return EFM_OK;
+}

@@ -98,7 +98,7 @@ class _Seq:
 class Mesh:
     """Tetrahedral mesh with its device-resident topology.
 
-    Array attributes (the build's additions): ``node_ids`` (N,), ``xyz`` (N,3),
+    Array attributes (beyond the reference's Mesh): ``node_ids`` (N,), ``xyz`` (N,3),
     ``tet_nodes``/``tri_nodes`` (0-based node positions), ``tet_phys``/``tri_phys``,
     ``edge_nodes`` (n,2 gmsh IDs), ``tet_edges``/``tet_sign``, ``tri_edges``/``tri_sign``
     (host copies) and ``topology`` (the device object)."""
@@ -764,27 +764,37 @@ def assemble_port_surface_mass(mesh: Mesh, surface_tag: int, dirichlet_edges) ->
     return SpMatC(topo.row_ptr, topo.col_idx, Ms.to(torch.complex128), True)
 
 
-def _port_face_eps_mu(mesh: Mesh, eps: np.ndarray, mu: np.ndarray, surface_tag: int) -> complex:
-    """eps*mu of the first tet (mesh order) with a face on the port; faces scanned
-    {0,1,2},{0,1,3},{0,2,3},{1,2,3} (assemble_maxwell.cpp:652-686)."""
+def _port_face_tet(mesh: Mesh, surface_tag: int) -> int:
+    """Index of the first tet (mesh order) with a face on the port surface, or -1; faces
+    scanned {0,1,2},{0,1,3},{0,2,3},{1,2,3} (assemble_maxwell.cpp:652-686).  Depends on
+    the mesh only, so it is cached on the mesh."""
+    cache = mesh.__dict__.setdefault("_port_face_tet", {})
+    if surface_tag in cache:
+        return cache[surface_tag]
+    t = -1
     tri = np.sort(mesh.node_ids[mesh.tri_nodes[mesh.tri_phys == surface_tag]], axis=1)
-    if tri.size == 0:
-        return 1.0 + 0j
-    ids = mesh.node_ids[mesh.tet_nodes]
-    faces = np.stack([ids[:, [0, 1, 2]], ids[:, [0, 1, 3]], ids[:, [0, 2, 3]], ids[:, [1, 2, 3]]], axis=1)
-    faces = np.sort(faces, axis=2).reshape(-1, 3)
-    big = np.int64(1) << 21
-    if ids.max() < big:
-        key = lambda a: (a[:, 0] << 42) | (a[:, 1] << 21) | a[:, 2]
-        hit = np.isin(key(faces), key(tri))
-    else:
-        s = set(map(tuple, tri.tolist()))
-        hit = np.array([tuple(f) in s for f in faces.tolist()])
-    w = np.nonzero(hit)[0]
-    if w.size == 0:
-        return 1.0 + 0j
-    t = int(w[0] // 4)
-    return complex(eps[t] * mu[t])
+    if tri.size:
+        ids = mesh.node_ids[mesh.tet_nodes]
+        faces = np.stack([ids[:, [0, 1, 2]], ids[:, [0, 1, 3]], ids[:, [0, 2, 3]], ids[:, [1, 2, 3]]], axis=1)
+        faces = np.sort(faces, axis=2).reshape(-1, 3)
+        big = np.int64(1) << 21
+        if ids.min() >= 0 and ids.max() < big:
+            key = lambda a: (a[:, 0] << 42) | (a[:, 1] << 21) | a[:, 2]
+            hit = np.isin(key(faces), key(tri))
+        else:
+            s = set(map(tuple, tri.tolist()))
+            hit = np.array([tuple(f) in s for f in faces.tolist()])
+        w = np.nonzero(hit)[0]
+        if w.size:
+            t = int(w[0] // 4)
+    cache[surface_tag] = t
+    return t
+
+
+def _port_face_eps_mu(mesh: Mesh, eps: np.ndarray, mu: np.ndarray, surface_tag: int) -> complex:
+    """eps*mu of the first tet with a face on the port (1 if there is none)."""
+    t = _port_face_tet(mesh, surface_tag)
+    return 1.0 + 0j if t < 0 else complex(eps[t] * mu[t])
 
 
 class _PortOps:
@@ -916,3 +926,71 @@ def check_passivity(S) -> bool:
     if not ok:
         sys.stderr.write(f"WARNING: S-matrix is not passive (max singular value {s.max():.6f})\n")
     return ok
+
+
+def frequency_sweep_eigenmode(mesh: Mesh, params: MaxwellParams, bc: BC, ports: Sequence[WavePort],
+                              frequencies, *, solver_options: Optional[SolveOptions] = None,
+                              max_batch: int = 256, _state: Optional[dict] = None) -> np.ndarray:
+    """S-parameters of the eigenmode formulation (assemble_maxwell.cpp:637-787) at many
+    frequencies, as RectWaveguideDesign.frequency_sweep computes them one frequency at a
+    time (waveguide.py:394-434).  K, M and the port masses are assembled once; every
+    (frequency, excitation) pair is one system of a batched COCG solve (up to
+    ``max_batch`` systems per launch sequence, per-system convergence).  The port modes
+    do not depend on frequency (the 2D eigenproblem has no frequency in it); beta is
+    recomputed per frequency.  Returns an array (nfreq, P, P); frequencies below a port
+    cutoff give NaN."""
+    import torch
+    freqs = np.asarray(frequencies, np.float64).ravel()
+    P = len(ports)
+    out = np.full((freqs.size, P, P), np.nan + 1j * np.nan, np.complex128)
+    if P == 0 or freqs.size == 0:
+        return out
+    opts = solver_options or SolveOptions()
+    eps, mu = tet_materials(mesh, params)
+    topo = mesh.topology
+    dmask = _dirichlet_mask(mesh, bc)
+    ops = _PortOps(mesh, bc, ports)
+    K, M = _capi.assemble_km(topo, mesh.dev_xyz, eps, 1.0 / mu, dmask)
+    # per frequency: k0^2 and the port betas (skip frequencies below any cutoff)
+    sys_k2, sys_coef, sys_f, sys_a = [], [], [], []
+    for i, f in enumerate(freqs):
+        p = MaxwellParams()
+        p.__dict__.update(params.__dict__)
+        p.omega = 2 * math.pi * f
+        betas = _eigenmode_betas(mesh, p, ports, eps, mu)
+        if betas is None:
+            continue
+        k0 = p.omega / C0
+        coefs = [1j * params.port_abc_scale * b for b in betas]
+        for a in range(P):
+            sys_k2.append(k0 * k0)
+            sys_coef.append(coefs)
+            sys_f.append(i)
+            sys_a.append(a)
+    if not sys_k2:
+        return out
+    nsys_all = len(sys_k2)
+    batch = int(min(max_batch, nsys_all, 1024))
+    sweep = _capi.SweepSolver(topo, K, M, ops.Ms, dmask, batch)
+    n = mesh.num_edges()
+    tol = 1e-14 if opts.use_direct else float(opts.tolerance)
+    infos = []
+    for b0 in range(0, nsys_all, batch):
+        b1 = min(nsys_all, b0 + batch)
+        idx = range(b0, b1)
+        Bh = np.zeros((n, b1 - b0), np.complex128)
+        for c, si in enumerate(idx):
+            a = sys_a[si]
+            Bh[ops.g_idx[a], c] = 2.0 * sys_coef[si][a] * ops.g_val[a]
+        Bd = _capi.to_dev(Bh, torch.complex128)
+        X, info = sweep.solve([sys_k2[si] for si in idx], [sys_coef[si] for si in idx], Bd, tol=tol,
+                              max_iterations=int(opts.max_iterations))
+        infos.append(info)
+        ov = _capi.port_overlap(ops.gi_dev, ops.gv_dev, X, b1 - b0)     # (P, nb)
+        for c, si in enumerate(idx):
+            i, a = sys_f[si], sys_a[si]
+            conv = info["resid"][c] <= tol or info["iters"][c] == 0
+            out[i, :, a] = ov[:, c] - (np.arange(P) == a) if conv else np.nan + 1j * np.nan
+    if _state is not None:
+        _state.update(dict(infos=infos, sweep=sweep, ops=ops))
+    return out

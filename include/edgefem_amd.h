@@ -184,6 +184,28 @@ int efm_solver_profile_read(efm_solver *s, double *h_spmv_ms_total, int64_t *h_s
 int efm_port_overlap(int64_t n, int n_ports, int64_t len, const int32_t *gidx, const double *gval,
                      const double *X, int nrhs, double *h_out, void *stream);
 
+/* ------------------------------------------------------------------------
+ * Batched frequency sweep: up to 1024 systems A_s = K - k0sq[s] M + sum_p coef[s,p] M_s,p
+ * on one pattern, solved together by COCG with the auxiliary-gradient preconditioner and
+ * per-system convergence.  Replaces the per-frequency loop of
+ * RectWaveguideDesign.frequency_sweep (python/edgefem/designs/waveguide.py:394-434) over
+ * calculate_sparams_eigenmode (src/assemble_maxwell.cpp:637-787), and the K/M reuse of
+ * frequency_sweep (src/sweep.cpp:192-329).
+ * K_val, M_val: c128 [nnz] from efm_assemble_km; h_Ms_vals[p]: device real [nnz] from
+ * efm_port_surface_mass (ports must not share edges); dirichlet[n_edges] or NULL.
+ * ---------------------------------------------------------------------- */
+typedef struct efm_sweep efm_sweep;
+
+int efm_sweep_create(const efm_topology *topo, const double *K_val, const double *M_val,
+                     int n_ports, const double *const *h_Ms_vals, const uint8_t *dirichlet,
+                     int max_systems, void *stream, efm_sweep **out);
+void efm_sweep_destroy(efm_sweep *sw);
+/* h_k0sq[nsys]; h_coef[nsys*n_ports] c128 (host); B, X: [n_edges][nsys] c128 (device);
+ * h_iters / h_resid (nsys, may be NULL).  h_opts->method must be EFM_METHOD_COCG. */
+int efm_sweep_solve(efm_sweep *sw, int nsys, const double *h_k0sq, const double *h_coef,
+                    const double *B, double *X, const efm_solve_opts *h_opts, int32_t *h_iters,
+                    double *h_resid, void *stream);
+
 #ifdef __cplusplus
 }
 #endif
