@@ -48,6 +48,11 @@ class SolveOpts(C.Structure):
                 ("check_every", C.c_int)]
 
 
+class RefineOpts(C.Structure):
+    _fields_ = [("inner_type", C.c_int), ("inner_tolerance", C.c_double), ("max_inner", C.c_int),
+                ("max_outer", C.c_int)]
+
+
 class SolveResultC(C.Structure):
     _fields_ = [("iterations", C.c_int), ("converged", C.c_int), ("max_rel_residual", C.c_double)]
 
@@ -93,7 +98,8 @@ def load_library(path: str = LIB_PATH) -> C.CDLL:
     lib.efm_sweep_create.argtypes = [vp, vp, vp, i32, vp, vp, i32, vp, C.POINTER(vp)]
     lib.efm_sweep_destroy.argtypes = [vp]
     lib.efm_sweep_destroy.restype = None
-    lib.efm_sweep_solve.argtypes = [vp, i32, vp, vp, vp, vp, C.POINTER(SolveOpts), vp, vp, vp]
+    lib.efm_sweep_solve.argtypes = [vp, i32, vp, vp, vp, vp, C.POINTER(SolveOpts), C.POINTER(RefineOpts), vp, vp,
+                                    vp]
     lib.efm_port_overlap.argtypes = [i64, i32, i64, vp, vp, vp, i32, vp, vp]
     for name in EXPORTS:
         getattr(lib, name)  # AttributeError if the library lacks a declared symbol
@@ -390,8 +396,12 @@ class SweepSolver:
         self.P = P
         self.max_systems = int(max_systems)
 
-    def solve(self, k0sq, coef, B, tol=1e-10, max_iterations=10000, check_every=16):
-        """k0sq (nsys,), coef (nsys, P) complex, B (n, nsys) complex128 device tensor."""
+    def solve(self, k0sq, coef, B, tol=1e-10, max_iterations=10000, check_every=16, precision="fp64",
+              inner_tol=1e-3, max_inner=1500, max_outer=12):
+        """k0sq (nsys,), coef (nsys, P) complex, B (n, nsys) complex128 device tensor.
+        precision "fp64": one FP64 COCG solve; "fp32": complex-FP32 inner COCG inside FP64
+        iterative refinement (true residual <= tol); "fp64-refined": the same loop with
+        FP64 inner vectors."""
         torch = _torch()
         k2 = np.ascontiguousarray(np.asarray(k0sq, np.float64))
         nsys = k2.size
@@ -403,8 +413,15 @@ class SweepSolver:
         iters = np.zeros(nsys, np.int32)
         resid = np.zeros(nsys, np.float64)
         opts = SolveOpts(EFM_METHOD_COCG, float(tol), int(max_iterations), int(check_every))
+        if precision == "fp64":
+            ref = None
+        elif precision in ("fp32", "fp64-refined"):
+            ref = C.byref(RefineOpts(EFM_C64 if precision == "fp32" else EFM_C128, float(inner_tol), int(max_inner),
+                                     int(max_outer)))
+        else:
+            raise EdgeFemError(f"unknown precision {precision!r}")
         _check(self._lib.efm_sweep_solve(self._h, nsys, k2.ctypes.data_as(C.c_void_p), cf.ctypes.data_as(C.c_void_p),
-                                         dptr(Bd), dptr(X), C.byref(opts), iters.ctypes.data_as(C.c_void_p),
+                                         dptr(Bd), dptr(X), C.byref(opts), ref, iters.ctypes.data_as(C.c_void_p),
                                          resid.ctypes.data_as(C.c_void_p), stream_ptr()), "efm_sweep_solve")
         return X, dict(iters=iters, resid=resid)
 

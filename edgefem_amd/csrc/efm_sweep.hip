@@ -33,8 +33,10 @@ constexpr int kMaxPorts = 4;
 
 struct SwState {
   double2 rho, alpha, beta;
-  double bnorm2, rnorm2;
-  int active, iters, breakdown, pad;
+  double bnorm2, rnorm2;  // inner solve: right-hand side and recursive residual norms^2
+  double bn_out, rn_out;  // refinement: |b|^2 and the last true residual |b - A x|^2
+  int active, iters, breakdown, total_iters;
+  int out_active, pad[3];
 };
 
 __device__ inline double2 bil(double2 a, double2 b) {
@@ -78,6 +80,25 @@ struct ChunkWalk {
   }
 };
 
+// Vector storage: complex FP64 (double2) or complex FP32 (float2); arithmetic and
+// reductions are FP64 either way.
+__device__ inline double2 ldv(const double2 *p) { return *p; }
+__device__ inline double2 ldv(const float2 *p) {
+  const float2 v = *p;
+  return make_double2((double)v.x, (double)v.y);
+}
+__device__ inline void stv(double2 *p, double2 v) { *p = v; }
+__device__ inline void stv(float2 *p, double2 v) { *p = make_float2((float)v.x, (float)v.y); }
+
+// Broadcast of lane l's value to the whole wavefront (l wave-uniform).
+__device__ inline int bcast(int v, int l) { return __builtin_amdgcn_readlane(v, l); }
+__device__ inline double bcast(double v, int l) {
+  const long long b = __double_as_longlong(v);
+  const int lo = __builtin_amdgcn_readlane((int)(b & 0xffffffffll), l);
+  const int hi = __builtin_amdgcn_readlane((int)(b >> 32), l);
+  return __longlong_as_double(((long long)hi << 32) | (unsigned int)lo);
+}
+
 // ---------------------------------------------------------------------------
 // Chronopoulos-Gear COCG (one reduction point per iteration; the SpMV reads only u):
 //   r0 = b, u0 = P r0, w0 = A u0, g0 = u0^T r0, d0 = u0^T w0, a0 = g0/d0, b0 = 0
@@ -89,55 +110,82 @@ struct ChunkWalk {
 // (nodal part of P on r_{i+1}, formed on the fly), update (p, s, x, r, u and partials
 // of g, |r|^2), spmv (w and partials of d), finalize (scalars, convergence).
 // ---------------------------------------------------------------------------
+// x = p = s = 0; r = b (b == nullptr: r already holds the right-hand side)
+template <typename VT>
 __global__ void __launch_bounds__(kBlock)
-    k_sw_init(int64_t n, int B, const double2 *__restrict__ b, double2 *__restrict__ x,
-              double2 *__restrict__ r, double2 *__restrict__ p, double2 *__restrict__ sv) {
+    k_sw_init(int64_t n, int B, const double2 *__restrict__ b, VT *__restrict__ x, VT *__restrict__ r,
+              VT *__restrict__ p, VT *__restrict__ sv) {
+  const double2 z = make_double2(0, 0);
   for (int64_t i = (int64_t)blockIdx.x * kBlock + threadIdx.x; i < n * B; i += (int64_t)gridDim.x * kBlock) {
-    x[i] = make_double2(0, 0);
-    r[i] = b[i];
-    p[i] = make_double2(0, 0);
-    sv[i] = make_double2(0, 0);
+    stv(x + i, z);
+    if (b)
+      stv(r + i, b[i]);
+    stv(p + i, z);
+    stv(sv + i, z);
   }
 }
 
-// y[n][s] = Dn_s(n)^{-1} sum_k G (r - alpha (w + beta s))[e][s]   (FUSED), or G^T r
-template <bool FUSED>
+// y[n][s] = Dn_s(n)^{-1} sum_k G (r - alpha (w + beta s))[e][s]   (FUSED), or G^T r.
+// One node per wavefront; the node's edge list and gradient signs are loaded across the
+// lanes and broadcast, so the gathers of its edges are independent of each other.
+template <typename VT, bool FUSED>
 __global__ void __launch_bounds__(kBlock)
     k_sw_aux_node(int64_t N, int B, int P, const int32_t *__restrict__ n2e_ptr,
                   const int32_t *__restrict__ n2e, const int8_t *__restrict__ gsign,
                   const double2 *__restrict__ dnK, const double2 *__restrict__ dnM,
                   const double *__restrict__ dnMs, const double *__restrict__ k0sq,
-                  const double2 *__restrict__ coef, const double2 *__restrict__ r,
-                  const double2 *__restrict__ w, const double2 *__restrict__ sv,
-                  double2 *__restrict__ y, const SwState *__restrict__ st) {
-  const int s = blockIdx.y * kSys + (threadIdx.x & 63);
+                  const double2 *__restrict__ coef, const VT *__restrict__ r,
+                  const VT *__restrict__ w, const VT *__restrict__ sv,
+                  VT *__restrict__ y, const SwState *__restrict__ st) {
+  constexpr int U = 4;
+  const int lane = threadIdx.x & 63;
+  const int s = blockIdx.y * kSys + lane;
   const int wv = threadIdx.x >> 6;
-  if (s >= B || !st[s].active)
+  const bool act = s < B && st[s].active != 0;
+  if (__ballot(act) == 0)
     return;
   const double2 alpha = FUSED ? st[s].alpha : make_double2(0, 0);
   const double2 beta = FUSED ? st[s].beta : make_double2(0, 0);
   const double k2 = k0sq[s];
   for (int64_t nd = (int64_t)blockIdx.x * kWaves + wv; nd < N; nd += (int64_t)gridDim.x * kWaves) {
     double2 acc = make_double2(0, 0);
-    for (int32_t k = n2e_ptr[nd]; k < n2e_ptr[nd + 1]; ++k) {
-      const int gs = gsign[k];
-      if (!gs)
-        continue;
-      const int64_t i = (int64_t)n2e[k] * B + s;
-      double2 rv = r[i];
-      if (FUSED)
-        rv = csub(rv, bil(alpha, cfma(beta, sv[i], w[i])));
-      acc.x += gs * rv.x;
-      acc.y += gs * rv.y;
+    const int32_t beg = n2e_ptr[nd], end = n2e_ptr[nd + 1];
+    for (int32_t base = beg; base < end; base += 64) {
+      const int len = min(64, end - base);
+      int ej = 0, gj = 0;
+      if (lane < len) {
+        ej = n2e[base + lane];
+        gj = gsign[base + lane];
+      }
+      for (int j = 0; j < len; j += U) {
+        double2 rv[U];
+#pragma unroll
+        for (int t = 0; t < U; ++t)
+          if (j + t < len) {
+            const int64_t i = (int64_t)bcast(ej, j + t) * B + s;
+            rv[t] = ldv(r + i);
+            if (FUSED)
+              rv[t] = csub(rv[t], bil(alpha, cfma(beta, ldv(sv + i), ldv(w + i))));
+          }
+#pragma unroll
+        for (int t = 0; t < U; ++t)
+          if (j + t < len) {
+            const double gs = (double)bcast(gj, j + t);
+            acc.x += gs * rv[t].x;
+            acc.y += gs * rv[t].y;
+          }
+      }
     }
-    double2 d = make_double2(dnK[nd].x - k2 * dnM[nd].x, dnK[nd].y - k2 * dnM[nd].y);
-    for (int p = 0; p < P; ++p) {
-      const double m = dnMs[(int64_t)p * N + nd];
-      const double2 c = coef[s * P + p];
-      d.x += c.x * m;
-      d.y += c.y * m;
+    if (act) {
+      double2 d = make_double2(dnK[nd].x - k2 * dnM[nd].x, dnK[nd].y - k2 * dnM[nd].y);
+      for (int p = 0; p < P; ++p) {
+        const double m = dnMs[(int64_t)p * N + nd];
+        const double2 c = coef[s * P + p];
+        d.x += c.x * m;
+        d.y += c.y * m;
+      }
+      stv(y + nd * B + s, (d.x == 0.0 && d.y == 0.0) ? make_double2(0, 0) : cdiv(acc, d));
     }
-    y[nd * B + s] = (d.x == 0.0 && d.y == 0.0) ? make_double2(0, 0) : cdiv(acc, d);
   }
 }
 
@@ -152,15 +200,15 @@ __device__ inline double2 edge_diag_inv(int64_t e, int s, int P, double k2, cons
 
 // INIT: u = P r; partials g = u^T r, |r|^2.
 // else: p = u + beta p; s = w + beta s; x += alpha p; r -= alpha s; u = P r; partials.
-template <bool INIT>
+template <typename VT, bool INIT>
 __global__ void __launch_bounds__(kBlock)
     k_sw_update(int64_t n, int B, int P, const int2 *__restrict__ en_eff,
                 const double2 *__restrict__ dK, const double2 *__restrict__ dM,
                 const double *__restrict__ dMs, const int8_t *__restrict__ rport,
                 const double *__restrict__ k0sq, const double2 *__restrict__ coef,
-                const double2 *__restrict__ y, const double2 *__restrict__ w,
-                double2 *__restrict__ u, double2 *__restrict__ p, double2 *__restrict__ sv,
-                double2 *__restrict__ x, double2 *__restrict__ r, double2 *__restrict__ partial,
+                const VT *__restrict__ y, const VT *__restrict__ w,
+                VT *__restrict__ u, VT *__restrict__ p, VT *__restrict__ sv,
+                VT *__restrict__ x, VT *__restrict__ r, double2 *__restrict__ partial,
                 const SwState *__restrict__ st) {
   __shared__ double2 lds[kWaves * 2 * 64];
   const int s = blockIdx.y * kSys + (threadIdx.x & 63);
@@ -172,22 +220,22 @@ __global__ void __launch_bounds__(kBlock)
     const double2 alpha = st[s].alpha, beta = st[s].beta;
     for (int64_t e = (int64_t)blockIdx.x * kWaves + wv; e < n; e += (int64_t)gridDim.x * kWaves) {
       const int64_t i = e * B + s;
-      double2 rv = r[i];
+      double2 rv = ldv(r + i);
       if (!INIT) {
-        const double2 pv = cfma(beta, p[i], u[i]);
-        const double2 svv = cfma(beta, sv[i], w[i]);
-        x[i] = cfma(alpha, pv, x[i]);
+        const double2 pv = cfma(beta, ldv(p + i), ldv(u + i));
+        const double2 svv = cfma(beta, ldv(sv + i), ldv(w + i));
+        stv(x + i, cfma(alpha, pv, ldv(x + i)));
         rv = csub(rv, bil(alpha, svv));
-        p[i] = pv;
-        sv[i] = svv;
-        r[i] = rv;
+        stv(p + i, pv);
+        stv(sv + i, svv);
+        stv(r + i, rv);
       }
       const double2 di = edge_diag_inv(e, s, P, k2, dK, dM, dMs, rport, coef);
       double2 z = bil(di, rv);
       const int2 ab = en_eff[e];
       if (ab.x >= 0)
-        z = cadd(z, csub(y[(int64_t)ab.y * B + s], y[(int64_t)ab.x * B + s]));
-      u[i] = z;
+        z = cadd(z, csub(ldv(y + (int64_t)ab.y * B + s), ldv(y + (int64_t)ab.x * B + s)));
+      stv(u + i, z);
       v[0] = cadd(v[0], bil(rv, z));
       v[1].x += cabs2(rv);
     }
@@ -195,16 +243,66 @@ __global__ void __launch_bounds__(kBlock)
   write_sys_partials<2>(v, lds, partial, blockIdx.y * gridDim.x + blockIdx.x);
 }
 
+// One row of A_s applied to a vector of the layout [row][system], called by a whole
+// wavefront (row wave-uniform, one system per lane).  The row's column indices and
+// shared K, M, M_s values are loaded once across the lanes (coalesced) and broadcast
+// with readlane, so the gathers of v carry no dependent index load and are issued
+// several at a time.
+template <typename VT>
+__device__ inline double2 sw_row_apply(int64_t row, int s, int B, int P, double k2,
+                                       const int32_t *__restrict__ row_ptr, const int32_t *__restrict__ col,
+                                       const double2 *__restrict__ K, const double2 *__restrict__ M,
+                                       const double *__restrict__ Ms, const int8_t *__restrict__ rport,
+                                       const double2 *__restrict__ coef, const VT *__restrict__ v) {
+  constexpr int U = 8;
+  const int lane = threadIdx.x & 63;
+  const double2 c = port_coef(coef, s, P, rport[row]);
+  const int32_t beg = row_ptr[row], end = row_ptr[row + 1];
+  double2 acc = make_double2(0, 0);
+  for (int32_t base = beg; base < end; base += 64) {
+    const int len = min(64, end - base);
+    int cj = 0;
+    double kx = 0, ky = 0, mx = 0, my = 0, ms = 0;
+    if (lane < len) {
+      cj = col[base + lane];
+      const double2 kv = K[base + lane], mv = M[base + lane];
+      kx = kv.x;
+      ky = kv.y;
+      mx = mv.x;
+      my = mv.y;
+      ms = Ms[base + lane];
+    }
+    for (int j = 0; j < len; j += U) {
+      double2 xv[U];
+#pragma unroll
+      for (int t = 0; t < U; ++t)
+        if (j + t < len)
+          xv[t] = ldv(v + (int64_t)bcast(cj, j + t) * B + s);
+#pragma unroll
+      for (int t = 0; t < U; ++t)
+        if (j + t < len) {
+          const int l = j + t;
+          const double m = bcast(ms, l);
+          const double2 a = make_double2(bcast(kx, l) - k2 * bcast(mx, l) + c.x * m,
+                                         bcast(ky, l) - k2 * bcast(my, l) + c.y * m);
+          acc = cfma(a, xv[t], acc);
+        }
+    }
+  }
+  return acc;
+}
+
 // w = A_s u; partials d = u^T w.  Persistent, XCD-aware: the blocks of one XCD walk
 // the XCD's contiguous share of the rows, so the gathered columns of u (64 systems x
 // 16 B per column) stay in that XCD's L2; blockIdx.y strides over the system groups.
+template <typename VT>
 __global__ void __launch_bounds__(kBlock)
     k_sw_spmv(int64_t n, int B, int P, int groups, const int32_t *__restrict__ row_ptr,
               const int32_t *__restrict__ col, const double2 *__restrict__ K,
               const double2 *__restrict__ M, const double *__restrict__ Ms,
               const int8_t *__restrict__ rport, const double *__restrict__ k0sq,
-              const double2 *__restrict__ coef, const double2 *__restrict__ u,
-              double2 *__restrict__ w, double2 *__restrict__ partial,
+              const double2 *__restrict__ coef, const VT *__restrict__ u,
+              VT *__restrict__ w, double2 *__restrict__ partial,
               const SwState *__restrict__ st) {
   __shared__ double2 lds[kWaves * 64];
   const int lane = threadIdx.x & 63;
@@ -215,27 +313,131 @@ __global__ void __launch_bounds__(kBlock)
     const int s = g * kSys + lane;
     double2 v[1] = {make_double2(0, 0)};
     const bool act = s < B && st[s].active != 0;
-    if (act) {
+    // the wave's rows are walked only if one of its 64 systems is still active; the
+    // gathers of inactive lanes read valid memory and are discarded
+    if (__ballot(act) != 0) {
       const double k2 = k0sq[s];
       for (int64_t chunk = cw.begin; chunk < cw.end; chunk += cw.stride) {
         const int64_t row = chunk * kWaves + wv;
         if (row >= n)
           continue;
-        const double2 c = port_coef(coef, s, P, rport[row]);
-        double2 acc = make_double2(0, 0);
-        for (int32_t k = row_ptr[row]; k < row_ptr[row + 1]; ++k) {
-          const double2 kv = K[k], mv = M[k];
-          const double ms = Ms[k];
-          const double2 a = make_double2(kv.x - k2 * mv.x + c.x * ms, kv.y - k2 * mv.y + c.y * ms);
-          acc = cfma(a, u[(int64_t)col[k] * B + s], acc);
+        const double2 acc = sw_row_apply(row, s, B, P, k2, row_ptr, col, K, M, Ms, rport, coef, u);
+        if (act) {
+          stv(w + row * B + s, acc);
+          v[0] = cadd(v[0], bil(ldv(u + row * B + s), acc));
         }
-        w[row * B + s] = acc;
-        v[0] = cadd(v[0], bil(u[row * B + s], acc));
       }
     }
     write_sys_partials<1>(v, lds, partial, g * gridDim.x + blockIdx.x);
     __syncthreads();  // lds reuse by the next group
   }
+}
+
+// Iterative refinement, outer step: r = b - A_s x in FP64 (x == nullptr: r = b), stored
+// in the inner solve's vector type; partials of |r|^2 (before rounding).  Systems whose
+// outer iteration has finished are skipped.
+template <typename VT>
+__global__ void __launch_bounds__(kBlock)
+    k_sw_residual(int64_t n, int B, int P, int groups, const int32_t *__restrict__ row_ptr,
+                  const int32_t *__restrict__ col, const double2 *__restrict__ K,
+                  const double2 *__restrict__ M, const double *__restrict__ Ms,
+                  const int8_t *__restrict__ rport, const double *__restrict__ k0sq,
+                  const double2 *__restrict__ coef, const double2 *__restrict__ b,
+                  const double2 *__restrict__ x, VT *__restrict__ r, double2 *__restrict__ partial,
+                  const SwState *__restrict__ st) {
+  __shared__ double2 lds[kWaves * 64];
+  const int lane = threadIdx.x & 63;
+  const int wv = threadIdx.x >> 6;
+  const int64_t nchunks = (n + kWaves - 1) / kWaves;
+  ChunkWalk cw(nchunks);
+  for (int g = blockIdx.y; g < groups; g += gridDim.y) {
+    const int s = g * kSys + lane;
+    double2 v[1] = {make_double2(0, 0)};
+    const bool act = s < B && st[s].out_active != 0;
+    if (__ballot(act) != 0) {
+      const double k2 = k0sq[s];
+      for (int64_t chunk = cw.begin; chunk < cw.end; chunk += cw.stride) {
+        const int64_t row = chunk * kWaves + wv;
+        if (row >= n)
+          continue;
+        double2 rv = b[row * B + s];
+        if (x)
+          rv = csub(rv, sw_row_apply(row, s, B, P, k2, row_ptr, col, K, M, Ms, rport, coef, x));
+        if (act) {
+          stv(r + row * B + s, rv);
+          v[0].x += cabs2(rv);
+        }
+      }
+    }
+    write_sys_partials<1>(v, lds, partial, g * gridDim.x + blockIdx.x);
+    __syncthreads();
+  }
+}
+
+// Outer step: x64 += d for the systems whose inner solve just ran.
+template <typename VT>
+__global__ void __launch_bounds__(kBlock)
+    k_sw_accumulate(int64_t n, int B, const VT *__restrict__ d, double2 *__restrict__ x,
+                    const SwState *__restrict__ st) {
+  const int s = blockIdx.y * kSys + (threadIdx.x & 63);
+  const int wv = threadIdx.x >> 6;
+  if (s >= B || !st[s].out_active)
+    return;
+  for (int64_t e = (int64_t)blockIdx.x * kWaves + wv; e < n; e += (int64_t)gridDim.x * kWaves) {
+    const int64_t i = e * B + s;
+    x[i] = cadd(x[i], ldv(d + i));
+  }
+}
+
+// Block-wide fixed-order sum of per-thread values.
+template <int V>
+__device__ inline void block_sum(double2 (&v)[V], double2 (*sh)[kBlock]) {
+#pragma unroll
+  for (int i = 0; i < V; ++i)
+    sh[i][threadIdx.x] = v[i];
+  __syncthreads();
+  for (int off = kBlock / 2; off > 0; off >>= 1) {
+    if (threadIdx.x < off)
+#pragma unroll
+      for (int i = 0; i < V; ++i)
+        sh[i][threadIdx.x] = cadd(sh[i][threadIdx.x], sh[i][threadIdx.x + off]);
+    __syncthreads();
+  }
+#pragma unroll
+  for (int i = 0; i < V; ++i)
+    v[i] = sh[i][0];
+}
+
+// Outer finalize, one block per system: true residual norm, outer convergence, and the
+// inner solve's activity (inner active = outer active).
+__global__ void __launch_bounds__(kBlock)
+    k_sw_outer_finalize(int B, int gxs, const double2 *__restrict__ ps, SwState *__restrict__ st,
+                        int *__restrict__ active_flag, double tol2, int max_it, int first) {
+  const int s = blockIdx.x;
+  if (s >= B)
+    return;
+  const int gy = s / kSys, lane = s % kSys;
+  __shared__ double2 sh[1][kBlock];
+  double2 v[1] = {make_double2(0, 0)};
+  for (int bx = threadIdx.x; bx < gxs; bx += kBlock)
+    v[0] = cadd(v[0], ps[((int64_t)gy * gxs + bx) * 64 + lane]);
+  block_sum<1>(v, sh);
+  if (threadIdx.x != 0)
+    return;
+  const double rn2 = v[0].x;
+  SwState &S = st[s];
+  if (first) {
+    S.bn_out = rn2;
+    S.rn_out = rn2;
+    S.total_iters = 0;
+    S.out_active = rn2 > 0.0 ? 1 : 0;
+  } else if (S.out_active) {
+    S.rn_out = rn2;
+    if (rn2 <= tol2 * S.bn_out || S.total_iters >= max_it || S.breakdown)
+      S.out_active = 0;
+  }
+  S.active = S.out_active;
+  active_flag[s] = S.out_active;
 }
 
 // One block per system: reduce its partials of g, |r|^2 (update grid, gxu blocks) and
@@ -244,37 +446,25 @@ template <bool INIT>
 __global__ void __launch_bounds__(kBlock)
     k_sw_finalize(int B, int gxu, int gxs, const double2 *__restrict__ pu,
                   const double2 *__restrict__ ps, SwState *__restrict__ st,
-                  int *__restrict__ active_flag, double tol2, int max_it) {
+                  int *__restrict__ active_flag, double tol2, int max_it, int max_inner) {
   const int s = blockIdx.x;
   if (s >= B)
     return;
   const int gy = s / kSys, lane = s % kSys;
-  double2 g = make_double2(0, 0), rr = make_double2(0, 0), d = make_double2(0, 0);
+  double2 v[3] = {make_double2(0, 0), make_double2(0, 0), make_double2(0, 0)};
   for (int bx = threadIdx.x; bx < gxu; bx += kBlock) {
     const int64_t bl = (int64_t)gy * gxu + bx;
-    g = cadd(g, pu[(bl * 2 + 0) * 64 + lane]);
-    rr = cadd(rr, pu[(bl * 2 + 1) * 64 + lane]);
+    v[0] = cadd(v[0], pu[(bl * 2 + 0) * 64 + lane]);
+    v[1] = cadd(v[1], pu[(bl * 2 + 1) * 64 + lane]);
   }
-  for (int bx = threadIdx.x; bx < gxs; bx += kBlock) {
-    const int64_t bl = (int64_t)gy * gxs + bx;
-    d = cadd(d, ps[bl * 64 + lane]);
-  }
+  for (int bx = threadIdx.x; bx < gxs; bx += kBlock)
+    v[2] = cadd(v[2], ps[((int64_t)gy * gxs + bx) * 64 + lane]);
   __shared__ double2 sh[3][kBlock];
-  sh[0][threadIdx.x] = g;
-  sh[1][threadIdx.x] = rr;
-  sh[2][threadIdx.x] = d;
-  __syncthreads();
-  for (int off = kBlock / 2; off > 0; off >>= 1) {
-    if (threadIdx.x < off)
-      for (int i = 0; i < 3; ++i)
-        sh[i][threadIdx.x] = cadd(sh[i][threadIdx.x], sh[i][threadIdx.x + off]);
-    __syncthreads();
-  }
+  block_sum<3>(v, sh);
   if (threadIdx.x != 0)
     return;
-  g = sh[0][0];
-  const double rn2 = sh[1][0].x;
-  d = sh[2][0];
+  const double2 g = v[0], d = v[2];
+  const double rn2 = v[1].x;
   SwState &S = st[s];
   if (INIT) {
     S.bnorm2 = rn2;
@@ -283,7 +473,7 @@ __global__ void __launch_bounds__(kBlock)
     S.breakdown = 0;
     S.beta = make_double2(0, 0);
     S.rho = g;
-    S.active = rn2 > 0.0 ? 1 : 0;
+    S.active = (S.active && rn2 > 0.0) ? 1 : 0;
     if (S.active) {
       if (d.x == 0.0 && d.y == 0.0) {
         S.breakdown = 1;
@@ -294,10 +484,11 @@ __global__ void __launch_bounds__(kBlock)
     }
   } else if (S.active) {
     S.iters += 1;
+    S.total_iters += 1;
     S.rnorm2 = rn2;
     if (rn2 <= tol2 * S.bnorm2) {
       S.active = 0;
-    } else if (S.iters >= max_it || (S.rho.x == 0.0 && S.rho.y == 0.0)) {
+    } else if (S.total_iters >= max_it || S.iters >= max_inner || (S.rho.x == 0.0 && S.rho.y == 0.0)) {
       S.active = 0;
     } else {
       const double2 beta = cdiv(g, S.rho);
@@ -442,6 +633,7 @@ struct efm_sweep {
   int gx_vec = 8, gx_nodes = 8, gx_spmv = 8, gy_spmv = 1;
   efm::DevBuf K, M, Ms, rport, dK, dM, dMs, dnK, dnM, dnMs, gsign, en_eff;
   efm::DevBuf x, r, p, sv, u, w, y, part_u, part_s, state, flags, k0sq, coef;
+  efm::DevBuf x64;  // refinement: FP64 solution (allocated on first use)
 };
 
 extern "C" int efm_sweep_create(const efm_topology *topo, const double *K_val, const double *M_val,
@@ -545,20 +737,150 @@ extern "C" int efm_sweep_create(const efm_topology *topo, const double *K_val, c
 
 extern "C" void efm_sweep_destroy(efm_sweep *s) { delete s; }
 
+namespace {
+
+// One launch sequence over the padded system block [Bp]: everything a solve needs.
+struct SwRun {
+  efm_sweep *s;
+  hipStream_t st;
+  int Bp, groups, P, check;
+  double tol2;
+  int max_it;
+  SwState *S;
+  int *flags;
+  double2 *pu, *ps;
+  std::vector<int> hflags;
+
+  int n_active(int &count) {
+    EFM_HIP_CHECK(hipMemcpyAsync(hflags.data(), flags, Bp * sizeof(int), hipMemcpyDeviceToHost, st));
+    EFM_HIP_CHECK(hipStreamSynchronize(st));
+    count = 0;
+    for (int v : hflags)
+      count += v != 0;
+    return EFM_OK;
+  }
+
+  // Preconditioned COCG on the systems whose state is active: x = A^{-1} r0, where r
+  // holds r0 on entry (b == nullptr) or is set from b.  Vectors of type VT.
+  template <typename VT> int cocg(const double2 *b, double inner_tol2, int max_inner) {
+    const int64_t n = s->n;
+    VT *x = s->x.as<VT>(), *r = s->r.as<VT>(), *p = s->p.as<VT>();
+    VT *sv = s->sv.as<VT>(), *u = s->u.as<VT>(), *w = s->w.as<VT>(), *y = s->y.as<VT>();
+    const int2 *en = s->en_eff.as<int2>();
+    hipLaunchKernelGGL((k_sw_init<VT>), dim3(grid_for(n * Bp, kBlock)), dim3(kBlock), 0, st, n, Bp, b, x, r, p,
+                       sv);
+    EFM_LAUNCH_CHECK();
+    auto aux_node = [&](bool fused) -> int {
+      if (fused)
+        hipLaunchKernelGGL((k_sw_aux_node<VT, true>), dim3(s->gx_nodes, groups), dim3(kBlock), 0, st, s->N, Bp,
+                           P, s->n2e_ptr, s->n2e, s->gsign.as<int8_t>(), s->dnK.as<double2>(),
+                           s->dnM.as<double2>(), s->dnMs.as<double>(), s->k0sq.as<double>(),
+                           s->coef.as<double2>(), r, w, sv, y, S);
+      else
+        hipLaunchKernelGGL((k_sw_aux_node<VT, false>), dim3(s->gx_nodes, groups), dim3(kBlock), 0, st, s->N,
+                           Bp, P, s->n2e_ptr, s->n2e, s->gsign.as<int8_t>(), s->dnK.as<double2>(),
+                           s->dnM.as<double2>(), s->dnMs.as<double>(), s->k0sq.as<double>(),
+                           s->coef.as<double2>(), r, (const VT *)nullptr, (const VT *)nullptr, y, S);
+      EFM_LAUNCH_CHECK();
+      return EFM_OK;
+    };
+    auto update = [&](bool first) -> int {
+      if (first)
+        hipLaunchKernelGGL((k_sw_update<VT, true>), dim3(s->gx_vec, groups), dim3(kBlock), 0, st, n, Bp, P, en,
+                           s->dK.as<double2>(), s->dM.as<double2>(), s->dMs.as<double>(), s->rport.as<int8_t>(),
+                           s->k0sq.as<double>(), s->coef.as<double2>(), y, w, u, p, sv, x, r, pu, S);
+      else
+        hipLaunchKernelGGL((k_sw_update<VT, false>), dim3(s->gx_vec, groups), dim3(kBlock), 0, st, n, Bp, P, en,
+                           s->dK.as<double2>(), s->dM.as<double2>(), s->dMs.as<double>(), s->rport.as<int8_t>(),
+                           s->k0sq.as<double>(), s->coef.as<double2>(), y, w, u, p, sv, x, r, pu, S);
+      EFM_LAUNCH_CHECK();
+      return EFM_OK;
+    };
+    auto spmv = [&]() -> int {
+      hipLaunchKernelGGL((k_sw_spmv<VT>), dim3(s->gx_spmv, s->gy_spmv), dim3(kBlock), 0, st, n, Bp, P, groups,
+                         s->row_ptr, s->col, s->K.as<double2>(), s->M.as<double2>(), s->Ms.as<double>(),
+                         s->rport.as<int8_t>(), s->k0sq.as<double>(), s->coef.as<double2>(), u, w, ps, S);
+      EFM_LAUNCH_CHECK();
+      return EFM_OK;
+    };
+    auto finalize = [&](bool first) -> int {
+      if (first)
+        hipLaunchKernelGGL((k_sw_finalize<true>), dim3(Bp), dim3(kBlock), 0, st, Bp, s->gx_vec, s->gx_spmv, pu,
+                           ps, S, flags, inner_tol2, max_it, max_inner);
+      else
+        hipLaunchKernelGGL((k_sw_finalize<false>), dim3(Bp), dim3(kBlock), 0, st, Bp, s->gx_vec, s->gx_spmv,
+                           pu, ps, S, flags, inner_tol2, max_it, max_inner);
+      EFM_LAUNCH_CHECK();
+      return EFM_OK;
+    };
+    int rc, active = 0;
+    if ((rc = aux_node(false)) || (rc = update(true)) || (rc = spmv()) || (rc = finalize(true)) ||
+        (rc = n_active(active)))
+      return rc;
+    int it = 0;
+    const int cap = std::min(max_it, max_inner);
+    while (active > 0 && it < cap) {
+      const int chunk = std::min(check, cap - it);
+      for (int c = 0; c < chunk; ++c, ++it)
+        if ((rc = aux_node(true)) || (rc = update(false)) || (rc = spmv()) || (rc = finalize(false)))
+          return rc;
+      if ((rc = n_active(active)))
+        return rc;
+    }
+    return EFM_OK;
+  }
+
+  // Iterative refinement: r = b - A x64 in FP64, inner COCG on r with VT vectors to a
+  // relative reduction inner_tol, x64 += d; until |b - A x64| <= tol |b| per system.
+  template <typename VT> int refine(const double2 *b, double inner_tol2, int max_inner, int max_outer) {
+    const int64_t n = s->n;
+    double2 *x64 = s->x64.as<double2>();
+    VT *r = s->r.as<VT>();
+    EFM_HIP_CHECK(hipMemsetAsync(x64, 0, (size_t)n * Bp * sizeof(double2), st));
+    auto residual = [&](bool first) -> int {
+      hipLaunchKernelGGL((k_sw_residual<VT>), dim3(s->gx_spmv, s->gy_spmv), dim3(kBlock), 0, st, n, Bp, P,
+                         groups, s->row_ptr, s->col, s->K.as<double2>(), s->M.as<double2>(), s->Ms.as<double>(),
+                         s->rport.as<int8_t>(), s->k0sq.as<double>(), s->coef.as<double2>(), b,
+                         first ? (const double2 *)nullptr : (const double2 *)x64, r, ps, S);
+      EFM_LAUNCH_CHECK();
+      hipLaunchKernelGGL(k_sw_outer_finalize, dim3(Bp), dim3(kBlock), 0, st, Bp, s->gx_spmv, ps, S, flags, tol2,
+                         max_it, first ? 1 : 0);
+      EFM_LAUNCH_CHECK();
+      return EFM_OK;
+    };
+    int rc, active = 0;
+    if ((rc = residual(true)) || (rc = n_active(active)))
+      return rc;
+    for (int k = 0; k < max_outer && active > 0; ++k) {
+      if ((rc = cocg<VT>(nullptr, inner_tol2, max_inner)))
+        return rc;
+      hipLaunchKernelGGL((k_sw_accumulate<VT>), dim3(s->gx_vec, groups), dim3(kBlock), 0, st, n, Bp,
+                         s->x.as<VT>(), x64, S);
+      EFM_LAUNCH_CHECK();
+      if ((rc = residual(false)) || (rc = n_active(active)))
+        return rc;
+    }
+    return EFM_OK;
+  }
+};
+
+}  // namespace
+
 extern "C" int efm_sweep_solve(efm_sweep *s, int nsys, const double *h_k0sq, const double *h_coef,
                                const double *B, double *X, const efm_solve_opts *h_opts,
-                               int32_t *h_iters, double *h_resid, void *stream) {
+                               const efm_refine_opts *h_refine, int32_t *h_iters, double *h_resid,
+                               void *stream) {
   EFM_REQUIRE(s && h_k0sq && B && X && h_opts, "efm_sweep_solve: NULL argument");
   EFM_REQUIRE(nsys >= 1 && nsys <= s->Bmax, "efm_sweep_solve: nsys out of range");
   EFM_REQUIRE(s->P == 0 || h_coef, "efm_sweep_solve: port coefficients missing");
   EFM_REQUIRE(h_opts->method == EFM_METHOD_COCG, "efm_sweep_solve: the sweep uses COCG");
+  EFM_REQUIRE(!h_refine || h_refine->inner_type == EFM_C128 || h_refine->inner_type == EFM_C64,
+              "efm_sweep_solve: inner_type must be EFM_C128 or EFM_C64");
+  EFM_REQUIRE(!h_refine || (h_refine->inner_tolerance > 0 && h_refine->inner_tolerance < 1),
+              "efm_sweep_solve: inner_tolerance must lie in (0, 1)");
   hipStream_t st = (hipStream_t)stream;
   const int Bp = ((nsys + kSys - 1) / kSys) * kSys;  // padded system count this call
-  const int groups = Bp / kSys;
   const int P = s->P;
-  const double tol2 = h_opts->tolerance * h_opts->tolerance;
-  const int max_it = h_opts->max_iterations;
-  const int check = h_opts->check_every > 0 ? h_opts->check_every : 16;
   // per-system scalars (padding systems get zero right-hand sides and stay inactive)
   std::vector<double> k2(Bp, 0.0);
   std::vector<double2> cf((size_t)Bp * std::max(1, P), make_double2(0, 0));
@@ -581,104 +903,67 @@ extern "C" int efm_sweep_solve(efm_sweep *s, int nsys, const double *h_k0sq, con
                                    nsys * sizeof(double2), s->n, hipMemcpyDeviceToDevice, st));
     Bd = Bpad.as<double2>();
   }
-  SwState *S = s->state.as<SwState>();
-  int *flags = s->flags.as<int>();
-  double2 *pu = s->part_u.as<double2>(), *ps = s->part_s.as<double2>();
-  const int2 *en = s->en_eff.as<int2>();
-  double2 *x = s->x.as<double2>(), *r = s->r.as<double2>(), *p = s->p.as<double2>();
-  double2 *sv = s->sv.as<double2>(), *u = s->u.as<double2>(), *w = s->w.as<double2>();
-  double2 *y = s->y.as<double2>();
-  hipLaunchKernelGGL(k_sw_init, dim3(grid_for(s->n * Bp, kBlock)), dim3(kBlock), 0, st, s->n, Bp, Bd, x, r,
-                     p, sv);
-  EFM_LAUNCH_CHECK();
-  // every system takes part in the first preconditioner application and SpMV
+  SwRun run;
+  run.s = s;
+  run.st = st;
+  run.Bp = Bp;
+  run.groups = Bp / kSys;
+  run.P = P;
+  run.check = h_opts->check_every > 0 ? h_opts->check_every : 16;
+  run.tol2 = h_opts->tolerance * h_opts->tolerance;
+  run.max_it = h_opts->max_iterations;
+  run.S = s->state.as<SwState>();
+  run.flags = s->flags.as<int>();
+  run.pu = s->part_u.as<double2>();
+  run.ps = s->part_s.as<double2>();
+  run.hflags.resize(Bp);
+  // every system starts active; the initial residual norms deactivate zero right-hand sides
   {
     std::vector<SwState> init(Bp);
     for (auto &v : init) {
       v = SwState{};
       v.active = 1;
+      v.out_active = 1;
     }
-    EFM_HIP_CHECK(hipMemcpyAsync(S, init.data(), Bp * sizeof(SwState), hipMemcpyHostToDevice, st));
+    EFM_HIP_CHECK(hipMemcpyAsync(run.S, init.data(), Bp * sizeof(SwState), hipMemcpyHostToDevice, st));
   }
-  auto aux_node = [&](bool fused) -> int {
-    if (fused)
-      hipLaunchKernelGGL((k_sw_aux_node<true>), dim3(s->gx_nodes, groups), dim3(kBlock), 0, st, s->N, Bp, P,
-                         s->n2e_ptr, s->n2e, s->gsign.as<int8_t>(), s->dnK.as<double2>(), s->dnM.as<double2>(),
-                         s->dnMs.as<double>(), s->k0sq.as<double>(), s->coef.as<double2>(), r, w, sv, y, S);
-    else
-      hipLaunchKernelGGL((k_sw_aux_node<false>), dim3(s->gx_nodes, groups), dim3(kBlock), 0, st, s->N, Bp, P,
-                         s->n2e_ptr, s->n2e, s->gsign.as<int8_t>(), s->dnK.as<double2>(), s->dnM.as<double2>(),
-                         s->dnMs.as<double>(), s->k0sq.as<double>(), s->coef.as<double2>(), r,
-                         (const double2 *)nullptr, (const double2 *)nullptr, y, S);
-    EFM_LAUNCH_CHECK();
-    return EFM_OK;
-  };
-  auto update = [&](bool first) -> int {
-    if (first)
-      hipLaunchKernelGGL((k_sw_update<true>), dim3(s->gx_vec, groups), dim3(kBlock), 0, st, s->n, Bp, P, en,
-                         s->dK.as<double2>(), s->dM.as<double2>(), s->dMs.as<double>(), s->rport.as<int8_t>(),
-                         s->k0sq.as<double>(), s->coef.as<double2>(), y, w, u, p, sv, x, r, pu, S);
-    else
-      hipLaunchKernelGGL((k_sw_update<false>), dim3(s->gx_vec, groups), dim3(kBlock), 0, st, s->n, Bp, P, en,
-                         s->dK.as<double2>(), s->dM.as<double2>(), s->dMs.as<double>(), s->rport.as<int8_t>(),
-                         s->k0sq.as<double>(), s->coef.as<double2>(), y, w, u, p, sv, x, r, pu, S);
-    EFM_LAUNCH_CHECK();
-    return EFM_OK;
-  };
-  auto spmv = [&]() -> int {
-    hipLaunchKernelGGL(k_sw_spmv, dim3(s->gx_spmv, s->gy_spmv), dim3(kBlock), 0, st, s->n, Bp, P, groups, s->row_ptr,
-                       s->col, s->K.as<double2>(), s->M.as<double2>(), s->Ms.as<double>(), s->rport.as<int8_t>(),
-                       s->k0sq.as<double>(), s->coef.as<double2>(), u, w, ps, S);
-    EFM_LAUNCH_CHECK();
-    return EFM_OK;
-  };
-  auto finalize = [&](bool first) -> int {
-    if (first)
-      hipLaunchKernelGGL((k_sw_finalize<true>), dim3(Bp), dim3(kBlock), 0, st, Bp, s->gx_vec, s->gx_spmv, pu, ps,
-                         S, flags, tol2, max_it);
-    else
-      hipLaunchKernelGGL((k_sw_finalize<false>), dim3(Bp), dim3(kBlock), 0, st, Bp, s->gx_vec, s->gx_spmv, pu,
-                         ps, S, flags, tol2, max_it);
-    EFM_LAUNCH_CHECK();
-    return EFM_OK;
-  };
-  std::vector<int> hflags(Bp);
-  auto n_active = [&](int &count) -> int {
-    EFM_HIP_CHECK(hipMemcpyAsync(hflags.data(), flags, Bp * sizeof(int), hipMemcpyDeviceToHost, st));
-    EFM_HIP_CHECK(hipStreamSynchronize(st));
-    count = 0;
-    for (int v : hflags)
-      count += v != 0;
-    return EFM_OK;
-  };
-  int rc, active = 0;
-  if ((rc = aux_node(false)) || (rc = update(true)) || (rc = spmv()) || (rc = finalize(true)) ||
-      (rc = n_active(active)))
-    return rc;
-  int it = 0;
-  while (active > 0 && it < max_it) {
-    const int chunk = std::min(check, max_it - it);
-    for (int c = 0; c < chunk; ++c, ++it)
-      if ((rc = aux_node(true)) || (rc = update(false)) || (rc = spmv()) || (rc = finalize(false)))
-        return rc;
-    if ((rc = n_active(active)))
+  int rc;
+  const double2 *xsrc;
+  if (!h_refine) {
+    if ((rc = run.cocg<double2>(Bd, run.tol2, run.max_it)))
       return rc;
-  }
-  // solution: the first nsys columns of x
-  if (Bp == nsys) {
-    EFM_HIP_CHECK(hipMemcpyAsync(X, s->x.p, (size_t)s->n * Bp * sizeof(double2), hipMemcpyDeviceToDevice, st));
+    xsrc = s->x.as<double2>();
   } else {
-    EFM_HIP_CHECK(hipMemcpy2DAsync(X, nsys * sizeof(double2), s->x.p, Bp * sizeof(double2),
+    if (!s->x64.p && (rc = s->x64.alloc((size_t)s->n * s->Bmax * sizeof(double2))))
+      return rc;
+    const double it2 = h_refine->inner_tolerance * h_refine->inner_tolerance;
+    const int mo = h_refine->max_outer > 0 ? h_refine->max_outer : 8;
+    const int mi = h_refine->max_inner > 0 ? h_refine->max_inner : run.max_it;
+    rc = (h_refine->inner_type == EFM_C64) ? run.refine<float2>(Bd, it2, mi, mo)
+                                           : run.refine<double2>(Bd, it2, mi, mo);
+    if (rc)
+      return rc;
+    xsrc = s->x64.as<double2>();
+  }
+  // solution: the first nsys columns
+  if (Bp == nsys) {
+    EFM_HIP_CHECK(hipMemcpyAsync(X, xsrc, (size_t)s->n * Bp * sizeof(double2), hipMemcpyDeviceToDevice, st));
+  } else {
+    EFM_HIP_CHECK(hipMemcpy2DAsync(X, nsys * sizeof(double2), xsrc, Bp * sizeof(double2),
                                    nsys * sizeof(double2), s->n, hipMemcpyDeviceToDevice, st));
   }
   std::vector<SwState> hs(Bp);
-  EFM_HIP_CHECK(hipMemcpyAsync(hs.data(), S, Bp * sizeof(SwState), hipMemcpyDeviceToHost, st));
+  EFM_HIP_CHECK(hipMemcpyAsync(hs.data(), run.S, Bp * sizeof(SwState), hipMemcpyDeviceToHost, st));
   EFM_HIP_CHECK(hipStreamSynchronize(st));
   for (int i = 0; i < nsys; ++i) {
     if (h_iters)
-      h_iters[i] = hs[i].iters;
-    if (h_resid)
-      h_resid[i] = hs[i].bnorm2 > 0 ? sqrt(hs[i].rnorm2 / hs[i].bnorm2) : 0.0;
+      h_iters[i] = hs[i].total_iters;
+    if (h_resid) {
+      if (h_refine)
+        h_resid[i] = hs[i].bn_out > 0 ? sqrt(hs[i].rn_out / hs[i].bn_out) : 0.0;
+      else
+        h_resid[i] = hs[i].bnorm2 > 0 ? sqrt(hs[i].rnorm2 / hs[i].bnorm2) : 0.0;
+    }
   }
   return EFM_OK;
 }

@@ -930,14 +930,20 @@ def check_passivity(S) -> bool:
 
 def frequency_sweep_eigenmode(mesh: Mesh, params: MaxwellParams, bc: BC, ports: Sequence[WavePort],
                               frequencies, *, solver_options: Optional[SolveOptions] = None,
-                              max_batch: int = 256, _state: Optional[dict] = None) -> np.ndarray:
+                              max_batch: int = 256, precision: str = "fp64",
+                              _state: Optional[dict] = None) -> np.ndarray:
     """S-parameters of the eigenmode formulation (assemble_maxwell.cpp:637-787) at many
     frequencies, as RectWaveguideDesign.frequency_sweep computes them one frequency at a
     time (waveguide.py:394-434).  K, M and the port masses are assembled once; every
     (frequency, excitation) pair is one system of a batched COCG solve (up to
     ``max_batch`` systems per launch sequence, per-system convergence).  The port modes
     do not depend on frequency (the 2D eigenproblem has no frequency in it); beta is
-    recomputed per frequency.  Returns an array (nfreq, P, P); frequencies below a port
+    recomputed per frequency.  ``precision`` "fp64" (default) runs FP64 COCG; "fp32" runs
+    the Krylov iterations on complex-FP32 vectors inside FP64 iterative refinement (FP64
+    residual b - A x, FP64 solution, FP64 reductions), so the tolerance holds for the true
+    FP64 residual.  On these indefinite Maxwell systems FP32 Krylov vectors stall near a
+    1e-4 relative residual, so the refined solve needs more iterations than the FP64 one
+    and is slower overall; it is kept for systems where that does not hold.  Returns an array (nfreq, P, P); frequencies below a port
     cutoff give NaN."""
     import torch
     freqs = np.asarray(frequencies, np.float64).ravel()
@@ -984,7 +990,7 @@ def frequency_sweep_eigenmode(mesh: Mesh, params: MaxwellParams, bc: BC, ports: 
             Bh[ops.g_idx[a], c] = 2.0 * sys_coef[si][a] * ops.g_val[a]
         Bd = _capi.to_dev(Bh, torch.complex128)
         X, info = sweep.solve([sys_k2[si] for si in idx], [sys_coef[si] for si in idx], Bd, tol=tol,
-                              max_iterations=int(opts.max_iterations))
+                              max_iterations=int(opts.max_iterations), precision=precision)
         infos.append(info)
         ov = _capi.port_overlap(ops.gi_dev, ops.gv_dev, X, b1 - b0)     # (P, nb)
         for c, si in enumerate(idx):

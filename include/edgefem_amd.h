@@ -200,11 +200,27 @@ int efm_sweep_create(const efm_topology *topo, const double *K_val, const double
                      int n_ports, const double *const *h_Ms_vals, const uint8_t *dirichlet,
                      int max_systems, void *stream, efm_sweep **out);
 void efm_sweep_destroy(efm_sweep *sw);
+/* Mixed precision by iterative refinement: the outer loop forms r = b - A x and updates
+ * x in FP64; each inner solve runs COCG with vectors stored as inner_type (EFM_C64:
+ * complex FP32; arithmetic and reductions stay FP64) until its residual drops by
+ * inner_tolerance.  No reference counterpart (the reference solves in FP64 only). */
+typedef struct {
+  int inner_type;          /* EFM_C128 or EFM_C64 */
+  double inner_tolerance;  /* relative residual reduction per inner solve, e.g. 1e-3 */
+  int max_inner;           /* iteration cap of one inner solve; <= 0: no cap of its own */
+  int max_outer;           /* <= 0: 8 */
+} efm_refine_opts;
+
 /* h_k0sq[nsys]; h_coef[nsys*n_ports] c128 (host); B, X: [n_edges][nsys] c128 (device);
- * h_iters / h_resid (nsys, may be NULL).  h_opts->method must be EFM_METHOD_COCG. */
+ * h_refine NULL: one FP64 COCG solve per system to h_opts->tolerance (recursive
+ * residual); otherwise refinement to h_opts->tolerance in the true FP64 residual, with
+ * h_opts->max_iterations bounding the inner iterations summed over outer steps.
+ * h_iters / h_resid (nsys, may be NULL): total inner iterations and final relative
+ * residual per system.  h_opts->method must be EFM_METHOD_COCG. */
 int efm_sweep_solve(efm_sweep *sw, int nsys, const double *h_k0sq, const double *h_coef,
-                    const double *B, double *X, const efm_solve_opts *h_opts, int32_t *h_iters,
-                    double *h_resid, void *stream);
+                    const double *B, double *X, const efm_solve_opts *h_opts,
+                    const efm_refine_opts *h_refine, int32_t *h_iters, double *h_resid,
+                    void *stream);
 
 #ifdef __cplusplus
 }

@@ -381,17 +381,28 @@ def test_full_size_c2_topology_properties(torch_dev):
     assert np.all(np.diff(first) > 0)
 
 
-def test_batched_sweep_matches_lu_golden(torch_dev):
-    """128 frequencies x 2 excitations = 256 systems in one batched solve on the 16x8x32
-    guide, against the FP64 SuperLU golden of every point."""
+@pytest.mark.parametrize("precision,max_batch", [("fp64", 256), ("fp64", 100), ("fp64-refined", 256)])
+def test_batched_sweep_matches_lu_golden(torch_dev, precision, max_batch):
+    """128 frequencies x 2 excitations = 256 systems on the 16x8x32 guide, batched, against
+    the FP64 SuperLU golden of every point: plain FP64 COCG, also with batches that are
+    not a multiple of the 64 systems of a wavefront (100, 100, 56), and the iterative
+    refinement loop (true FP64 residual b - A x per outer step) around FP64 inner solves.
+    Complex-FP32 inner vectors are not tested here: on these indefinite systems their
+    COCG iterations stall near a 1e-4..1e-3 relative residual for some frequencies."""
     from edgefem_amd import pyedgefem as em
-    _p("[gpu] batched sweep 256 systems")
+    _p(f"[gpu] batched sweep 256 systems precision={precision} max_batch={max_batch}")
     pm, bc, ports = _c1_em_setup()
     gold = np.load(golden("c3_sweep.npz"))
     params = em.MaxwellParams()
     st = {}
-    S = em.frequency_sweep_eigenmode(pm, params, bc, ports, gold["freqs"], _state=st)
+    opts = em.SolveOptions()
+    opts.tolerance = tol = 1e-10
+    S = em.frequency_sweep_eigenmode(pm, params, bc, ports, gold["freqs"], precision=precision,
+                                     max_batch=max_batch, solver_options=opts, _state=st)
     err = np.abs(S - gold["S"]).max()
-    its = st["infos"][0]["iters"]
-    _p(f"    max |dS| {err:.2e}; iterations min {its.min()} max {its.max()}")
+    its = np.concatenate([i["iters"] for i in st["infos"]])
+    res = np.concatenate([i["resid"] for i in st["infos"]])
+    _p(f"    max |dS| {err:.2e}; iterations min {its.min()} max {its.max()}; max resid {res.max():.1e}")
+    assert np.all(np.isfinite(S))
+    assert res.max() <= tol
     assert err <= 1e-6
