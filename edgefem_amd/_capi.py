@@ -30,7 +30,7 @@ EXPORTS = [
     "efm_solver_create", "efm_solver_destroy", "efm_solver_set_matrix", "efm_solver_solve",
     "efm_solver_spmv", "efm_solver_profile", "efm_solver_profile_read",
     "efm_port_overlap", "efm_solver_set_spmv_variant", "efm_solver_get_spmv_variant",
-    "efm_sweep_create", "efm_sweep_destroy", "efm_sweep_solve",
+    "efm_sweep_create", "efm_sweep_destroy", "efm_sweep_solve", "efm_eps_gradient",
 ]
 
 
@@ -96,6 +96,7 @@ def load_library(path: str = LIB_PATH) -> C.CDLL:
     lib.efm_solver_set_spmv_variant.argtypes = [vp, i32]
     lib.efm_solver_get_spmv_variant.argtypes = [vp, C.POINTER(C.c_int), vp]
     lib.efm_sweep_create.argtypes = [vp, vp, vp, i32, vp, vp, i32, vp, C.POINTER(vp)]
+    lib.efm_eps_gradient.argtypes = [vp, vp, C.c_int64, vp, i32, vp, i32, vp, i32, vp, vp, vp, vp, vp]
     lib.efm_sweep_destroy.argtypes = [vp]
     lib.efm_sweep_destroy.restype = None
     lib.efm_sweep_solve.argtypes = [vp, i32, vp, vp, vp, vp, C.POINTER(SolveOpts), C.POINTER(RefineOpts), vp, vp,
@@ -435,3 +436,34 @@ class SweepSolver:
             self.close()
         except Exception:
             pass
+
+
+def eps_gradient(topo: Topology, xyz, design, Lam, X, terms_l, terms_r, weights):
+    """grad[d] = sum_t Re(w_t * lam_{l_t}^T M_e x_{r_t}) over design tets (efm_eps_gradient).
+    design: int array of tet indices (host or device) or None for all tets; Lam, X: device
+    complex128 (n, nl) / (n, nx) system-minor; returns a float64 device tensor."""
+    torch = _torch()
+    lib = load_library()
+    if design is None:
+        dd = None
+        D = topo.n_tets
+    else:
+        dh = np.asarray(design.cpu().numpy() if hasattr(design, "cpu") else design, np.int64)
+        if dh.size and (dh.min() < 0 or dh.max() >= topo.n_tets):
+            raise EdgeFemError("design tet index out of range")
+        dd = to_dev(dh.astype(np.int32), torch.int32)
+        D = dh.size
+    Ld = to_dev(Lam, torch.complex128)
+    Xd = to_dev(X, torch.complex128)
+    nl = Ld.shape[1] if Ld.dim() == 2 else 1
+    nx = Xd.shape[1] if Xd.dim() == 2 else 1
+    tl = np.ascontiguousarray(np.asarray(terms_l, np.int32))
+    tr = np.ascontiguousarray(np.asarray(terms_r, np.int32))
+    w = np.ascontiguousarray(np.asarray(weights, np.complex128))
+    if not (tl.size == tr.size == w.size):
+        raise EdgeFemError("term arrays differ in length")
+    out = torch.empty(D, dtype=torch.float64, device=Xd.device)
+    _check(lib.efm_eps_gradient(topo.handle, dptr(xyz), int(D), dptr(dd), int(nl), dptr(Ld), int(nx), dptr(Xd),
+                                int(tl.size), tl.ctypes.data_as(C.c_void_p), tr.ctypes.data_as(C.c_void_p),
+                                w.ctypes.data_as(C.c_void_p), dptr(out), stream_ptr()), "efm_eps_gradient")
+    return out

@@ -292,9 +292,41 @@ __device__ inline double2 sw_row_apply(int64_t row, int s, int B, int P, double 
   return acc;
 }
 
+// Lane-staged metadata of one row (up to 64 entries; longer rows take the general path).
+struct RowStage {
+  int32_t beg, len;  // wave-uniform
+  int cj;
+  double kx, ky, mx, my, ms;
+  double2 c;  // port coefficient of this lane's system on this row
+};
+
+__device__ inline void stage_row(RowStage &q, int64_t row, int s, int P, const int32_t *__restrict__ row_ptr,
+                                 const int32_t *__restrict__ col, const double2 *__restrict__ K,
+                                 const double2 *__restrict__ M, const double *__restrict__ Ms,
+                                 const int8_t *__restrict__ rport, const double2 *__restrict__ coef) {
+  const int lane = threadIdx.x & 63;
+  q.beg = row_ptr[row];
+  q.len = row_ptr[row + 1] - q.beg;
+  q.c = port_coef(coef, s, P, rport[row]);
+  q.cj = 0;
+  q.kx = q.ky = q.mx = q.my = q.ms = 0.0;
+  if (lane < q.len && lane < 64) {
+    const int32_t k = q.beg + lane;
+    q.cj = col[k];
+    const double2 kv = K[k], mv = M[k];
+    q.kx = kv.x;
+    q.ky = kv.y;
+    q.mx = mv.x;
+    q.my = mv.y;
+    q.ms = Ms[k];
+  }
+}
+
 // w = A_s u; partials d = u^T w.  Persistent, XCD-aware: the blocks of one XCD walk
 // the XCD's contiguous share of the rows, so the gathered columns of u (64 systems x
 // 16 B per column) stay in that XCD's L2; blockIdx.y strides over the system groups.
+// Each wave keeps one row staged ahead: the next row's indices and values are loaded
+// while the current row's gathers are in flight.
 template <typename VT>
 __global__ void __launch_bounds__(kBlock)
     k_sw_spmv(int64_t n, int B, int P, int groups, const int32_t *__restrict__ row_ptr,
@@ -304,6 +336,7 @@ __global__ void __launch_bounds__(kBlock)
               const double2 *__restrict__ coef, const VT *__restrict__ u,
               VT *__restrict__ w, double2 *__restrict__ partial,
               const SwState *__restrict__ st) {
+  constexpr int U = 16;
   __shared__ double2 lds[kWaves * 64];
   const int lane = threadIdx.x & 63;
   const int wv = threadIdx.x >> 6;
@@ -317,15 +350,59 @@ __global__ void __launch_bounds__(kBlock)
     // gathers of inactive lanes read valid memory and are discarded
     if (__ballot(act) != 0) {
       const double k2 = k0sq[s];
-      for (int64_t chunk = cw.begin; chunk < cw.end; chunk += cw.stride) {
-        const int64_t row = chunk * kWaves + wv;
-        if (row >= n)
-          continue;
-        const double2 acc = sw_row_apply(row, s, B, P, k2, row_ptr, col, K, M, Ms, rport, coef, u);
+      int64_t chunk = cw.begin;
+      int64_t row = chunk * kWaves + wv;
+      RowStage cur;
+      if (chunk < cw.end && row < n)
+        stage_row(cur, row, s, P, row_ptr, col, K, M, Ms, rport, coef);
+      while (chunk < cw.end && row < n) {
+        const int64_t nchunk = chunk + cw.stride;
+        const int64_t nrow = nchunk * kWaves + wv;
+        double2 acc = make_double2(0, 0);
+        if (cur.len <= 64) {
+          double2 xv[U];
+#pragma unroll
+          for (int t = 0; t < U; ++t)
+            if (t < cur.len)
+              xv[t] = ldv(u + (int64_t)bcast(cur.cj, t) * B + s);
+          RowStage nxt;
+          if (nchunk < cw.end && nrow < n)
+            stage_row(nxt, nrow, s, P, row_ptr, col, K, M, Ms, rport, coef);
+#pragma unroll
+          for (int t = 0; t < U; ++t)
+            if (t < cur.len) {
+              const double m = bcast(cur.ms, t);
+              const double2 a = make_double2(bcast(cur.kx, t) - k2 * bcast(cur.mx, t) + cur.c.x * m,
+                                             bcast(cur.ky, t) - k2 * bcast(cur.my, t) + cur.c.y * m);
+              acc = cfma(a, xv[t], acc);
+            }
+          for (int j = U; j < cur.len; j += 8) {
+#pragma unroll
+            for (int t = 0; t < 8; ++t)
+              if (j + t < cur.len)
+                xv[t] = ldv(u + (int64_t)bcast(cur.cj, j + t) * B + s);
+#pragma unroll
+            for (int t = 0; t < 8; ++t)
+              if (j + t < cur.len) {
+                const int l = j + t;
+                const double m = bcast(cur.ms, l);
+                const double2 a = make_double2(bcast(cur.kx, l) - k2 * bcast(cur.mx, l) + cur.c.x * m,
+                                               bcast(cur.ky, l) - k2 * bcast(cur.my, l) + cur.c.y * m);
+                acc = cfma(a, xv[t], acc);
+              }
+          }
+          cur = nxt;
+        } else {
+          acc = sw_row_apply(row, s, B, P, k2, row_ptr, col, K, M, Ms, rport, coef, u);
+          if (nchunk < cw.end && nrow < n)
+            stage_row(cur, nrow, s, P, row_ptr, col, K, M, Ms, rport, coef);
+        }
         if (act) {
           stv(w + row * B + s, acc);
           v[0] = cadd(v[0], bil(ldv(u + row * B + s), acc));
         }
+        chunk = nchunk;
+        row = nrow;
       }
     }
     write_sys_partials<1>(v, lds, partial, g * gridDim.x + blockIdx.x);

@@ -943,8 +943,8 @@ def frequency_sweep_eigenmode(mesh: Mesh, params: MaxwellParams, bc: BC, ports: 
     residual b - A x, FP64 solution, FP64 reductions), so the tolerance holds for the true
     FP64 residual.  On these indefinite Maxwell systems FP32 Krylov vectors stall near a
     1e-4 relative residual, so the refined solve needs more iterations than the FP64 one
-    and is slower overall; it is kept for systems where that does not hold.  Returns an array (nfreq, P, P); frequencies below a port
-    cutoff give NaN."""
+    and is slower overall; it is kept for systems where that does not hold.  Returns an
+    array (nfreq, P, P); frequencies below a port cutoff give NaN."""
     import torch
     freqs = np.asarray(frequencies, np.float64).ravel()
     P = len(ports)
@@ -997,6 +997,10 @@ def frequency_sweep_eigenmode(mesh: Mesh, params: MaxwellParams, bc: BC, ports: 
             i, a = sys_f[si], sys_a[si]
             conv = info["resid"][c] <= tol or info["iters"][c] == 0
             out[i, :, a] = ov[:, c] - (np.arange(P) == a) if conv else np.nan + 1j * np.nan
+        if _state is not None and _state.get("keep_solutions"):
+            _state.setdefault("solutions", []).append(
+                dict(X=X, freq_index=[sys_f[si] for si in idx], excitation=[sys_a[si] for si in idx],
+                     k0sq=[sys_k2[si] for si in idx], coef=[sys_coef[si] for si in idx]))
     if _state is not None:
         _state.update(dict(infos=infos, sweep=sweep, ops=ops))
     return out

@@ -222,6 +222,23 @@ int efm_sweep_solve(efm_sweep *sw, int nsys, const double *h_k0sq, const double 
                     const efm_refine_opts *h_refine, int32_t *h_iters, double *h_resid,
                     void *stream);
 
+/* ------------------------------------------------------------------------
+ * Adjoint gradient with respect to the per-tet permittivity (no reference counterpart;
+ * it differentiates calculate_sparams_eigenmode, src/assemble_maxwell.cpp:637-787).
+ * For every design tet e = design[d] (design NULL: all tets, n_design == n_tets):
+ *   grad[d] = sum_t Re( h_w[t] * (s.L[:, l_t])_e^T M_e (s.X[:, r_t])_e )
+ * with M_e the Whitney mass block recomputed from xyz and s the edge signs.  L[n][nl],
+ * X[n][nx] c128 device (system-minor, e.g. the columns of one batched solve; L may equal
+ * X); h_term_l/h_term_r/h_w (host, n_terms <= 2048): column pairs and complex weights.
+ * grad: device double [n_design], overwritten; an invalid design index yields NaN.
+ * With A complex-symmetric, lam_j = A^{-1} conj(g_j); the caller folds k0^2, the loss
+ * weights and the lam/x scaling into h_w.  Synchronises the stream.
+ * ---------------------------------------------------------------------- */
+int efm_eps_gradient(const efm_topology *topo, const double *xyz, int64_t n_design,
+                     const int32_t *design, int nl, const double *Lam, int nx, const double *X,
+                     int n_terms, const int32_t *h_term_l, const int32_t *h_term_r,
+                     const double *h_w, double *grad, void *stream);
+
 #ifdef __cplusplus
 }
 #endif

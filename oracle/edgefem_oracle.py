@@ -18,7 +18,8 @@ build_wave_port_2d          src/ports/wave_port.cpp:516-546
 solve_te10_mode             src/ports/port_eigensolve.cpp:45-63
 sparams_eigenmode           src/assemble_maxwell.cpp:637-787
 calculate_sparams           src/assemble_maxwell.cpp:352-394 (port loading 209-242)
-frequency_sweep_km          src/sweep.cpp:174-329 (K/M reuse branch)
+eps_gradient_eigenmode      adjoint of sparams_eigenmode w.r.t. per-tet eps_r (no
+                            reference counterpart; pinned by central differences)
 ==========================  =========================================================
 
 Third-party routines replaced: Eigen ``setFromTriplets`` (duplicates summed) ->
@@ -263,7 +264,7 @@ def build_edge_pec(mesh: OMesh, pec_tag: int) -> np.ndarray:
 @dataclasses.dataclass
 class Params:
     """Subset of MaxwellParams (maxwell.hpp:46-143) the path uses.  Material precedence:
-    per-tet array (the build's extension) > region map > global value."""
+    per-tet array (an extension beyond the reference) > region map > global value."""
     omega: float = 0.0
     eps_r: complex = 1.0
     mu_r: complex = 1.0
@@ -600,3 +601,40 @@ def calculate_sparams(mesh: OMesh, p: Params, dirichlet: np.ndarray, ports: Sequ
             V = np.sum(np.conj(ports[j].weights[keep]) * x[ports[j].edges[keep]])
             S[j, i] = (V - vinc) / vinc if i == j else V / vinc
     return S
+
+
+def eps_gradient_eigenmode(mesh: OMesh, p: Params, dirichlet: np.ndarray, ports: Sequence[OPort],
+                           G: np.ndarray, design: Optional[np.ndarray] = None):
+    """Adjoint gradient of a real loss L(S) of the eigenmode S matrix at one frequency with
+    respect to the real part of the per-tet permittivity, given G = dL/d conj(S) (PyTorch's
+    convention: L = |S21|^2 gives G[1,0] = 2 S21).  With A x_a = b_a, S_ja = g_j^H x_a -
+    delta_ja and dA/d eps_e = -k0^2 M_e (the element mass scattered with edge signs):
+        dS_ja/d eps_e = k0^2 lam_j^T M_e x_a,   lam_j = A^{-T} conj(g_j),
+        dL/d eps_e = sum_ja Re(conj(G_ja) dS_ja/d eps_e).
+    beta_i is held fixed, so tets with a face on a port must not be design tets.
+    Returns (S, grad[design])."""
+    P = len(ports)
+    betas, Ms, vecs = eigenmode_setup(mesh, p, dirichlet, ports)
+    if betas is None:
+        raise ValueError("frequency below a port cutoff")
+    A, _ = assemble_maxwell(mesh, p, dirichlet)
+    for i in range(P):
+        A = A + (1j * p.port_abc_scale * betas[i]) * Ms[i]
+    A = A.tocsc()
+    lu = spla.splu(A, permc_spec="MMD_AT_PLUS_A")
+    B = np.stack([2.0 * (1j * p.port_abc_scale * betas[a]) * (Ms[a] @ vecs[a]) for a in range(P)], axis=1)
+    X = lu.solve(B)
+    g = np.stack([Ms[j] @ vecs[j] for j in range(P)], axis=1)           # g_j^H x = vdot(e_j, M_s x)
+    S = np.conj(g).T @ X - np.eye(P)
+    Lam = lu.solve(np.conj(g), trans="T")
+    k0 = p.omega / C0
+    T = mesh.tet_ids.shape[0]
+    design = np.arange(T) if design is None else np.asarray(design)
+    Me = whitney_mass_matrix(mesh.tet_xyz()[design])                   # (D,6,6)
+    sg = mesh.tet_sign[design].astype(float)
+    te = mesh.tet_edges[design]
+    xe = X[te] * sg[:, :, None]                                         # (D,6,P)
+    le = Lam[te] * sg[:, :, None]
+    dS = k0 * k0 * np.einsum("dkj,dkl,dla->dja", le, Me, xe)            # (D,P,P)
+    grad = np.einsum("ja,dja->d", np.conj(G), dS).real
+    return S, grad

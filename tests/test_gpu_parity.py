@@ -406,3 +406,74 @@ def test_batched_sweep_matches_lu_golden(torch_dev, precision, max_batch):
     assert np.all(np.isfinite(S))
     assert res.max() <= tol
     assert err <= 1e-6
+
+
+def _design_problem(freqs, seed=5):
+    """Small WR-90 section (6x3x12, 30 mm) with a per-tet permittivity design region in
+    its middle half, on the GPU and in the oracle."""
+    from edgefem_amd import adjoint as adj
+    from edgefem_amd import meshgen as mg
+    from edgefem_amd import pyedgefem as em
+    from oracle import edgefem_oracle as O
+    L = 30e-3
+    g = mg.box_mesh(mg.WR90_A, mg.WR90_B, L, 6, 3, 12, seed=seed)
+    pm = em.mesh_from_arrays(g.node_ids, g.xyz, g.tets, g.tet_phys, g.tris, g.tri_phys)
+    bc = em.build_edge_pec(pm, 1)
+    mode = em.solve_te10_mode(em.RectWaveguidePort(mg.WR90_A, mg.WR90_B), 10e9)
+    kc2 = (math.pi / mg.WR90_A) ** 2
+    ports = [em.build_wave_port_2d(pm, t, mode, bc.dirichlet_edges, kc2) for t in (2, 3)]
+    zc = g.centroids()[:, 2]
+    design = np.nonzero((zc > 0.25 * L) & (zc < 0.75 * L))[0]
+    eps0 = 1.0 + 5.0 * mg.uniform01(seed + 1, design.astype(np.int64))
+    opts = em.SolveOptions()
+    opts.tolerance = 1e-12
+    prob = adj.EpsDesignProblem(pm, em.MaxwellParams(), bc, ports, freqs, design, solver_options=opts)
+    om = O.mesh_from_arrays(g.node_ids, g.xyz, g.tet_ids, g.tet_phys, g.tri_ids, g.tri_phys)
+    opec = O.build_edge_pec(om, 1)
+    kc, beta, Z0 = O.solve_te10_mode(mg.WR90_A, 10e9)
+    oports = [O.build_wave_port_2d(om, t, kc, Z0, beta, opec, kc2) for t in (2, 3)]
+    return prob, design, eps0, (om, opec, oports)
+
+
+def test_adjoint_gradient_torch_autograd(torch_dev):
+    """J = sum_f |S21(f)|^2 through torch.autograd on the GPU: the gradient over every
+    design tet against the oracle's adjoint (SuperLU), and against central differences of
+    the GPU forward (h = 1e-4 eps_r) on seeded tets."""
+    import torch
+    from edgefem_amd import adjoint as adj
+    from oracle import edgefem_oracle as O
+    _p("[gpu] adjoint gradient vs oracle and central differences")
+    freqs = np.array([9e9, 10.5e9])
+    prob, design, eps0, (om, opec, oports) = _design_problem(freqs)
+    e = torch.tensor(eps0, dtype=torch.float64, device="cuda", requires_grad=True)
+    S = adj.SParamsOfPermittivity.apply(e, prob)
+    J = (S[:, 1, 0].abs() ** 2).sum()
+    J.backward()
+    grad = e.grad.cpu().numpy()
+    # oracle adjoint on the same mesh and permittivity
+    eps_full = prob.permittivity(eps0)
+    ref = np.zeros(design.size)
+    Jref = 0.0
+    for f in freqs:
+        p = O.Params(omega=2 * math.pi * f, eps_tet=eps_full)
+        So = O.sparams_eigenmode(om, p, opec, oports)
+        G = np.zeros((2, 2), complex)
+        G[1, 0] = 2 * So[1, 0]
+        Jref += abs(So[1, 0]) ** 2
+        ref += O.eps_gradient_eigenmode(om, p, opec, oports, G, design=design)[1]
+    Jv = float(J.detach())
+    _p(f"    J {Jv:.12f} (oracle {Jref:.12f}); max |grad| {np.abs(ref).max():.3e}")
+    assert abs(Jv - Jref) <= 1e-9
+    np.testing.assert_allclose(grad, ref, rtol=1e-6, atol=1e-8 * np.abs(ref).max())
+    # central differences of the GPU forward on a few tets
+    rng = np.random.default_rng(11)
+    for k in rng.choice(design.size, 3, replace=False):
+        h = 1e-4 * eps0[k]
+        ep, em_ = eps0.copy(), eps0.copy()
+        ep[k] += h
+        em_[k] -= h
+        Jp = float((np.abs(prob.solve(ep)[0][:, 1, 0]) ** 2).sum())
+        Jm = float((np.abs(prob.solve(em_)[0][:, 1, 0]) ** 2).sum())
+        fd = (Jp - Jm) / (2 * h)
+        _p(f"    tet {design[k]}: adjoint {grad[k]:.9e} central difference {fd:.9e}")
+        assert abs(grad[k] - fd) <= 1e-5 * max(abs(fd), 1e-6 * np.abs(ref).max())

@@ -334,3 +334,54 @@ def test_straight_guide_physics():
     dph = (np.degrees(np.angle(S[1, 0]) + beta * 0.05) + 180) % 360 - 180
     assert abs(dph) < 10
     assert abs(S[0, 1] - S[1, 0]) < 1e-9
+
+
+def _design_guide(nx=6, ny=3, nz=12, L=30e-3, seed=5):
+    """Small WR-90 section with a per-tet permittivity design region in the middle."""
+    from edgefem_amd import meshgen as mg
+    g = mg.box_mesh(mg.WR90_A, mg.WR90_B, L, nx, ny, nz, seed=seed)
+    m = O.mesh_from_arrays(g.node_ids, g.xyz, g.tet_ids, g.tet_phys, g.tri_ids, g.tri_phys)
+    pec = O.build_edge_pec(m, 1)
+    zc = g.centroids()[:, 2]
+    design = np.nonzero((zc > 0.25 * L) & (zc < 0.75 * L))[0]
+    eps = np.ones(g.tet_ids.shape[0], np.complex128)
+    eps[design] = 1.0 + 5.0 * mg.uniform01(seed + 1, design.astype(np.int64))
+    kc, beta, Z0 = O.solve_te10_mode(mg.WR90_A, 10e9)
+    ports = [O.build_wave_port_2d(m, t, kc, Z0, beta, pec, (math.pi / mg.WR90_A) ** 2) for t in (2, 3)]
+    return m, pec, ports, eps, design
+
+
+def test_adjoint_gradient_matches_central_differences():
+    """dJ/d eps_r for J = sum_f |S21(f)|^2 from the adjoint formula against central
+    differences (h = 1e-4 eps_r, two SuperLU solves per tet and frequency) on seeded
+    design tets; an error in the sign, the k0^2 factor, the transpose of lam or the edge
+    signs fails this."""
+    m, pec, ports, eps, design = _design_guide()
+    freqs = [9e9, 10.5e9]
+    rng = np.random.default_rng(3)
+    pick = rng.choice(design, 6, replace=False)
+
+    def J(e):
+        tot = 0.0
+        for f in freqs:
+            S = O.sparams_eigenmode(m, O.Params(omega=2 * math.pi * f, eps_tet=e), pec, ports)
+            tot += abs(S[1, 0]) ** 2
+        return tot
+
+    grad = np.zeros(pick.size)
+    for f in freqs:
+        p = O.Params(omega=2 * math.pi * f, eps_tet=eps)
+        S = O.sparams_eigenmode(m, p, pec, ports)
+        G = np.zeros((2, 2), complex)
+        G[1, 0] = 2 * S[1, 0]
+        _, gr = O.eps_gradient_eigenmode(m, p, pec, ports, G, design=pick)
+        grad += gr
+    fd = np.zeros(pick.size)
+    for k, t in enumerate(pick):
+        h = 1e-4 * eps[t].real
+        ep, em = eps.copy(), eps.copy()
+        ep[t] += h
+        em[t] -= h
+        fd[k] = (J(ep) - J(em)) / (2 * h)
+    assert np.all(np.abs(fd) > 1e-8)
+    np.testing.assert_allclose(grad, fd, rtol=1e-5, atol=1e-9 * np.abs(fd).max())
