@@ -72,6 +72,7 @@ class EpsDesignProblem:
         st = {"keep_solutions": True}
         S = em.frequency_sweep_eigenmode(self.mesh, p, self.bc, self.ports, self.freqs,
                                          solver_options=self.solver_options, max_batch=self.max_batch, _state=st)
+        self.last_infos = st.get("infos", [])
         return S, st.get("solutions", [])
 
     def gradient(self, solutions, G: np.ndarray):
