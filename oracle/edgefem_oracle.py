@@ -18,6 +18,9 @@ build_wave_port_2d          src/ports/wave_port.cpp:516-546
 solve_te10_mode             src/ports/port_eigensolve.cpp:45-63
 sparams_eigenmode           src/assemble_maxwell.cpp:637-787
 calculate_sparams           src/assemble_maxwell.cpp:352-394 (port loading 209-242)
+port_abc_coefficient        src/assemble_maxwell.cpp:274-320
+build_lumped_port           src/ports/lumped_port.cpp:21-160
+normalize_port_weights      src/assemble_maxwell.cpp:396-494
 eps_gradient_eigenmode      adjoint of sparams_eigenmode w.r.t. per-tet eps_r (no
                             reference counterpart; pinned by central differences)
 ==========================  =========================================================
@@ -275,6 +278,8 @@ class Params:
     use_abc: bool = False
     abc_surface_tags: set = dataclasses.field(default_factory=set)
     port_abc_scale: float = 1.0
+    use_port_abc: bool = False
+    port_abc_type: str = "beta"     # none | beta | beta_norm | impedance_match | modal_admittance
 
 
 def tet_materials(mesh: OMesh, p: Params):
@@ -389,9 +394,41 @@ def assemble_maxwell(mesh: OMesh, p: Params, dirichlet: np.ndarray, ports=(), ac
         be = be[~dirichlet[be]]
         A = A + sp.csr_matrix((np.full(be.size, 1j * k0), (be, be)), shape=(n, n))
         A.sort_indices()
+    if p.use_port_abc and p.port_abc_type != "none":
+        for port in ports:
+            coeff = port_abc_coefficient(p, port)
+            if coeff is None:
+                continue
+            e = np.asarray(port.edges)
+            e = e[~dirichlet[e]]
+            A = A + sp.csr_matrix((np.full(e.size, coeff), (e, e)), shape=(n, n))
+            A.sort_indices()
     A = _dirichlet_zero(A, dirichlet, 1.0)
     b[dirichlet] = 0
     return A, b
+
+
+def port_abc_coefficient(p: Params, port) -> Optional[complex]:
+    """Diagonal port ABC term of assemble_maxwell.cpp:274-320: only when k0^2 > kc^2;
+    j beta, j beta/k0, j beta sqrt(Re Z0/eta0) or j omega eps0/Re Z0, with eta0 = mu0 c0."""
+    if port.Z0 == 0:
+        return None
+    k0 = p.omega / C0
+    bsq = k0 * k0 - port.kc * port.kc
+    if bsq <= 0:
+        return None
+    beta = math.sqrt(bsq)
+    z0r = complex(port.Z0).real
+    t = p.port_abc_type
+    if t == "beta":
+        return 1j * beta
+    if t == "beta_norm":
+        return 1j * beta / k0
+    if t == "impedance_match":
+        return 1j * beta * math.sqrt(z0r / ETA0_ASM)
+    if t == "modal_admittance":
+        return 1j * p.omega * EPS0 / z0r
+    raise ValueError(f"unknown port ABC type {t!r}")
 
 
 def port_surface_mass(mesh: OMesh, surface_tag: int, dirichlet: np.ndarray) -> sp.csr_matrix:
@@ -601,6 +638,70 @@ def calculate_sparams(mesh: OMesh, p: Params, dirichlet: np.ndarray, ports: Sequ
             V = np.sum(np.conj(ports[j].weights[keep]) * x[ports[j].edges[keep]])
             S[j, i] = (V - vinc) / vinc if i == j else V / vinc
     return S
+
+
+def build_lumped_port(mesh: OMesh, surface_tag: int, z0: float = 50.0, e_dir=(0.0, 0.0, 1.0),
+                      mode: str = "surface_integral") -> OPort:
+    """Lumped port on the tris of ``surface_tag`` (lumped_port.cpp:21-160).  Weights: the
+    surface integral w_e = sum_tri s (A/3)(grad l_j - grad l_i).e (in-plane gradients,
+    local edges (0,1),(1,2),(2,0)), or the projection of the edge vector (min ID -> max
+    ID) on e; then scaled to ||w|| = sqrt(z0).  The reference keeps the edges in hash-set
+    order; here they are sorted (S does not depend on the order)."""
+    sel = np.nonzero(mesh.tri_phys == surface_tag)[0]
+    if sel.size == 0:
+        raise ValueError(f"no triangles with surface tag {surface_tag}")
+    edges = np.unique(mesh.tri_edges[sel].ravel())
+    pos = {int(e): k for k, e in enumerate(edges)}
+    ed = np.asarray(e_dir, float)
+    ed = ed / np.linalg.norm(ed)
+    w = np.zeros(edges.size, np.complex128)
+    if mode == "surface_integral":
+        for f in sel:
+            v = mesh.xyz[[mesh.node_index[int(i)] for i in mesh.tri_ids[f]]]
+            nrm = np.cross(v[1] - v[0], v[2] - v[0])
+            a2 = np.linalg.norm(nrm)
+            if a2 < 1e-30:
+                continue
+            nh = nrm / a2
+            g = [np.cross(nh, v[2] - v[1]) / a2, np.cross(nh, v[0] - v[2]) / a2, np.cross(nh, v[1] - v[0]) / a2]
+            for le, (li, lj) in enumerate(TRI_PAIRS):
+                ge = int(mesh.tri_edges[f, le])
+                w[pos[ge]] += mesh.tri_sign[f, le] * (a2 / 2.0 / 3.0) * np.dot(g[lj] - g[li], ed)
+    elif mode == "projection":
+        for k, e in enumerate(edges):
+            p0 = mesh.xyz[mesh.node_index[int(mesh.edges[e, 0])]]
+            p1 = mesh.xyz[mesh.node_index[int(mesh.edges[e, 1])]]
+            w[k] = np.dot(p1 - p0, ed)
+    else:
+        raise ValueError(f"unknown lumped port weight mode {mode!r}")
+    nw = np.linalg.norm(w)
+    if nw > 1e-15:
+        w = w * (math.sqrt(z0) / nw)
+    return OPort(surface_tag=surface_tag, kc=0.0, Z0=complex(z0), beta=0.0, edges=edges, weights=w)
+
+
+def normalize_port_weights(mesh: OMesh, p: Params, dirichlet: np.ndarray, ports: Sequence[OPort]):
+    """Scale each port's weights so that |w^H A^{-1} w| = Re Z0 with A assembled without
+    ports (assemble_maxwell.cpp:396-494); FP64 LU.  Returns new ports."""
+    A, _ = assemble_maxwell(mesh, p, dirichlet)
+    lu = spla.splu(A.tocsc(), permc_spec="MMD_AT_PLUS_A")
+    out = []
+    for port in ports:
+        q = dataclasses.replace(port, weights=np.array(port.weights, np.complex128))
+        if port.Z0 == 0 or len(port.weights) != len(port.edges):
+            out.append(q)
+            continue
+        e = np.asarray(port.edges)
+        keep = ~dirichlet[e]
+        rhs = np.zeros(A.shape[0], np.complex128)
+        rhs[e[keep]] = q.weights[keep]
+        y = lu.solve(rhs)
+        c = abs(np.sum(np.conj(q.weights[keep]) * y[e[keep]]))
+        target = complex(port.Z0).real
+        if np.linalg.norm(q.weights) >= 1e-15 and c > 1e-15 and target > 1e-15:
+            q.weights = q.weights * math.sqrt(target / c)
+        out.append(q)
+    return out
 
 
 def eps_gradient_eigenmode(mesh: OMesh, p: Params, dirichlet: np.ndarray, ports: Sequence[OPort],

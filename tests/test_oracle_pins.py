@@ -7,6 +7,7 @@ import math
 import numpy as np
 import pytest
 import scipy.sparse as sp
+import scipy.sparse.linalg as spla
 
 from conftest import golden
 from oracle import edgefem_oracle as O
@@ -385,3 +386,139 @@ def test_adjoint_gradient_matches_central_differences():
         fd[k] = (J(ep) - J(em)) / (2 * h)
     assert np.all(np.abs(fd) > 1e-8)
     np.testing.assert_allclose(grad, fd, rtol=1e-5, atol=1e-9 * np.abs(fd).max())
+
+
+def _cube():
+    m = O.load_gmsh_v2(golden("cube_cavity.msh"))
+    return m, O.build_edge_pec(m, 1)
+
+
+def _point_ports(edges=(10, 20), z0=50.0):
+    return [O.OPort(surface_tag=-1, kc=0.0, Z0=complex(z0), beta=0.0, edges=np.array([e]),
+                    weights=np.ones(1, np.complex128)) for e in edges]
+
+
+def test_wave_port_source_and_admittance():
+    """tests/test_maxwell.cpp:18-51: one-edge ports with unit weight and Z0 = 50 on the cube
+    cavity: b at the active port edge is 2/sqrt(Z0), and the port adds w conj(w)/Z0 = 1/Z0
+    to its diagonal (the reference test's stated intent)."""
+    m, pec = _cube()
+    assert not pec[10] and not pec[20]
+    p = O.Params(omega=1.0)
+    ports = _point_ports()
+    A, b = O.assemble_maxwell(m, p, pec, ports, 0)
+    A0, _ = O.assemble_maxwell(m, p, pec)
+    assert abs(b[10] - 2.0 / math.sqrt(50.0)) < 1e-12 and b[20] == 0
+    D = (A - A0).tocoo()
+    keep = np.abs(D.data) > 0
+    assert sorted(zip(D.row[keep], D.col[keep])) == [(10, 10), (20, 20)]
+    assert np.allclose(D.data[keep], 1.0 / 50.0, rtol=0, atol=1e-15)
+
+
+def test_rank1_sparams_woodbury():
+    """The rank-1 loading of calculate_sparams (assemble_maxwell.cpp:209-242, 352-394) on the
+    setup of tests/test_maxwell.cpp:53-79, against its closed form: with Y = A0^{-1} W (no
+    ports) and C = W^H Y, V = C (I + Z^{-1} C)^{-1} diag(2/sqrt Z0), S = V / sqrt(Z0_i) - I.
+    A wrong sign, a missing conjugate or a wrong 1/Z0 scale in the loading fails this.
+    The reference test's passivity assertion does not hold for this formulation here:
+    the lossless cavity gives c = (A0^{-1})_{10,10} = -0.0325 < 0, so S11 = (c - Z0)/(c + Z0)
+    = -1.0013."""
+    m, pec = _cube()
+    p = O.Params(omega=2 * math.pi * 1e9)
+    ports = _point_ports()
+    S = O.calculate_sparams(m, p, pec, ports)
+    A0, _ = O.assemble_maxwell(m, p, pec)
+    n = A0.shape[0]
+    W = np.zeros((n, 2), complex)
+    W[10, 0] = W[20, 1] = 1.0
+    Y = spla.spsolve(A0.tocsc(), W)
+    C = W.conj().T @ Y
+    Z = np.array([50.0, 50.0])
+    V = C @ np.linalg.solve(np.eye(2) + np.diag(1 / Z) @ C, np.diag(2 / np.sqrt(Z)))
+    S_ref = V / np.sqrt(Z)[None, :] - np.eye(2)
+    np.testing.assert_allclose(S, S_ref, rtol=1e-9, atol=1e-12)
+    c = C[0, 0]
+    assert abs(S[0, 0] - (c - 50.0) / (c + 50.0)) < 1e-6
+    # reciprocity for real weights (complex-symmetric A)
+    assert abs(S[0, 1] - S[1, 0]) < 1e-10
+
+
+def test_lumped_port_weights_vs_independent_gradients():
+    """Surface-integral weights of build_lumped_port (lumped_port.cpp:21-100) against an
+    independent evaluation: in-plane barycentric gradients from the pseudo-inverse of the
+    triangle's edge matrix, Whitney basis integrated by the 3-point edge-midpoint rule
+    (exact for degree 2)."""
+    m = O.load_gmsh_v2(golden("rect_waveguide.msh"))
+    port = O.build_lumped_port(m, 2, z0=50.0, e_dir=(0, 1, 0))
+    assert abs(np.linalg.norm(port.weights) - math.sqrt(50.0)) < 1e-12
+    sel = np.nonzero(m.tri_phys == 2)[0]
+    w = {int(e): 0.0 for e in port.edges}
+    ey = np.array([0.0, 1.0, 0.0])
+    for f in sel:
+        v = m.xyz[[m.node_index[int(i)] for i in m.tri_ids[f]]]
+        E = np.stack([v[1] - v[0], v[2] - v[0]], axis=1)          # 3x2
+        Gp = np.linalg.pinv(E)                                     # rows: grad l1, grad l2
+        g = [-(Gp[0] + Gp[1]), Gp[0], Gp[1]]
+        area = 0.5 * np.linalg.norm(np.cross(v[1] - v[0], v[2] - v[0]))
+        mids = [(0.5, 0.5, 0.0), (0.0, 0.5, 0.5), (0.5, 0.0, 0.5)]
+        for le, (li, lj) in enumerate(O.TRI_PAIRS):
+            val = 0.0
+            for lam in mids:
+                N = lam[li] * g[lj] - lam[lj] * g[li]
+                val += area / 3.0 * N.dot(ey)
+            w[int(m.tri_edges[f, le])] += m.tri_sign[f, le] * val
+    raw = np.array([w[int(e)] for e in port.edges])
+    raw *= math.sqrt(50.0) / np.linalg.norm(raw)
+    np.testing.assert_allclose(port.weights.real, raw, rtol=1e-10, atol=1e-13)
+
+
+def test_normalize_port_weights_sets_input_impedance():
+    """normalize_port_weights (assemble_maxwell.cpp:396-494) scales w so that
+    |w^H A0^{-1} w| = Re Z0, checked with an independent solve; a lumped port on a matched
+    guide then reflects (c - Z0)/(c + Z0) exactly (single-port Sherman-Morrison)."""
+    m = O.load_gmsh_v2(golden("rect_waveguide.msh"))
+    pec = O.build_edge_pec(m, 1)
+    p = O.Params(omega=2 * math.pi * 10e9)
+    ports = O.normalize_port_weights(m, p, pec, [O.build_lumped_port(m, 2, 50.0, (0, 1, 0))])
+    A0, _ = O.assemble_maxwell(m, p, pec)
+    e = ports[0].edges[~pec[ports[0].edges]]
+    w = ports[0].weights[~pec[ports[0].edges]]
+    rhs = np.zeros(A0.shape[0], complex)
+    rhs[e] = w
+    c = np.sum(np.conj(w) * spla.spsolve(A0.tocsc(), rhs)[e])
+    assert abs(abs(c) - 50.0) < 1e-8
+    S = O.calculate_sparams(m, p, pec, ports)
+    assert abs(S[0, 0] - (c - 50.0) / (c + 50.0)) < 1e-9
+
+
+@pytest.mark.parametrize("kind", ["beta", "beta_norm", "impedance_match", "modal_admittance"])
+def test_port_abc_diagonal_terms(kind):
+    """use_port_abc (assemble_maxwell.cpp:274-320): a diagonal term on the free edges of
+    every port, only above cutoff; the four coefficients against their definitions with
+    beta = sqrt(k0^2 - kc^2) of the TE10 mode."""
+    m = O.load_gmsh_v2(golden("rect_waveguide.msh"))
+    pec = O.build_edge_pec(m, 1)
+    a = 0.02286
+    kc, beta, Z0 = O.solve_te10_mode(a, 10e9)
+    ports = [O.build_wave_port_2d(m, t, kc, Z0, beta, pec, (math.pi / a) ** 2) for t in (2, 3)]
+    for port in ports:
+        port.kc = math.pi / a
+    p = O.Params(omega=2 * math.pi * 10e9, use_port_abc=True, port_abc_type=kind)
+    A1, _ = O.assemble_maxwell(m, p, pec, ports, 0)
+    p0 = O.Params(omega=2 * math.pi * 10e9)
+    A0, _ = O.assemble_maxwell(m, p0, pec, ports, 0)
+    D = (A1 - A0).tocoo()
+    k0 = 2 * math.pi * 10e9 / O.C0
+    bt = math.sqrt(k0 ** 2 - (math.pi / a) ** 2)
+    expect = {"beta": 1j * bt, "beta_norm": 1j * bt / k0,
+              "impedance_match": 1j * bt * math.sqrt(complex(Z0).real / (4e-7 * math.pi * O.C0)),
+              "modal_admittance": 1j * 2 * math.pi * 10e9 * O.EPS0 / complex(Z0).real}[kind]
+    keep = np.abs(D.data) > 0
+    free = set()
+    for port in ports:
+        free |= {int(e) for e in port.edges if not pec[e]}
+    assert np.all(D.row[keep] == D.col[keep]) and set(D.row[keep].tolist()) == free
+    np.testing.assert_allclose(D.data[keep], expect, rtol=1e-12)
+    # below cutoff: no term
+    p.omega = p0.omega = 2 * math.pi * 5e9
+    assert abs(O.assemble_maxwell(m, p, pec, ports, 0)[0] - O.assemble_maxwell(m, p0, pec, ports, 0)[0]).max() == 0
