@@ -31,6 +31,7 @@ EXPORTS = [
     "efm_solver_spmv", "efm_solver_profile", "efm_solver_profile_read",
     "efm_port_overlap", "efm_solver_set_spmv_variant", "efm_solver_get_spmv_variant",
     "efm_sweep_create", "efm_sweep_destroy", "efm_sweep_solve", "efm_eps_gradient",
+    "efm_solver_set_lowrank", "efm_add_diagonal",
 ]
 
 
@@ -96,6 +97,8 @@ def load_library(path: str = LIB_PATH) -> C.CDLL:
     lib.efm_solver_set_spmv_variant.argtypes = [vp, i32]
     lib.efm_solver_get_spmv_variant.argtypes = [vp, C.POINTER(C.c_int), vp]
     lib.efm_sweep_create.argtypes = [vp, vp, vp, i32, vp, vp, i32, vp, C.POINTER(vp)]
+    lib.efm_solver_set_lowrank.argtypes = [vp, i32, C.c_int64, vp, vp, vp, vp]
+    lib.efm_add_diagonal.argtypes = [vp, C.c_int64, vp, vp, vp, vp]
     lib.efm_eps_gradient.argtypes = [vp, vp, C.c_int64, vp, i32, vp, i32, vp, i32, vp, vp, vp, vp, vp]
     lib.efm_sweep_destroy.argtypes = [vp]
     lib.efm_sweep_destroy.restype = None
@@ -302,6 +305,19 @@ class KrylovSolver:
         _check(self._lib.efm_solver_set_gradient(self._h, topo.n_nodes, dptr(topo.edge_nodes), dptr(topo.n2e_ptr),
                                                  dptr(topo.n2e), dptr(dm), stream_ptr()), "efm_solver_set_gradient")
 
+    def set_lowrank(self, gidx, gval, coefs):
+        """Operator term A_eff = A + sum_p coefs[p] w_p w_p^H (call before set_matrix).
+        gidx: int32 device (P*len, -1 padding), gval: complex128 device (P*len)."""
+        torch = _torch()
+        c = np.ascontiguousarray(np.asarray(coefs, np.complex128))
+        gi = to_dev(gidx, torch.int32)
+        gv = to_dev(gval, torch.complex128)
+        P = c.size
+        L = gi.numel() // max(P, 1)
+        self._lr_keep = (gi, gv)
+        _check(self._lib.efm_solver_set_lowrank(self._h, P, L, dptr(gi), dptr(gv), c.ctypes.data_as(C.c_void_p),
+                                                stream_ptr()), "efm_solver_set_lowrank")
+
     def set_matrix(self, values):
         torch = _torch()
         v = to_dev(values, self.dtype)
@@ -467,3 +483,19 @@ def eps_gradient(topo: Topology, xyz, design, Lam, X, terms_l, terms_r, weights)
                                 int(tl.size), tl.ctypes.data_as(C.c_void_p), tr.ctypes.data_as(C.c_void_p),
                                 w.ctypes.data_as(C.c_void_p), dptr(out), stream_ptr()), "efm_eps_gradient")
     return out
+
+
+def add_diagonal(topo: Topology, rows, vals, A):
+    """A[r, r] += vals[i] for the distinct rows[i] (in place on the device values A)."""
+    torch = _torch()
+    lib = load_library()
+    r = np.ascontiguousarray(np.asarray(rows, np.int32))
+    if r.size == 0:
+        return A
+    if np.unique(r).size != r.size or r.min() < 0 or r.max() >= topo.n_edges:
+        raise EdgeFemError("add_diagonal: rows must be distinct edge indices")
+    rd = to_dev(r, torch.int32)
+    vd = to_dev(np.asarray(vals, np.complex128).reshape(-1), torch.complex128)
+    _check(lib.efm_add_diagonal(topo.handle, int(r.size), dptr(rd), dptr(vd), dptr(A), stream_ptr()),
+           "efm_add_diagonal")
+    return A

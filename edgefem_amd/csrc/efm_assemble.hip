@@ -10,6 +10,8 @@
 #include "efm_element.h"
 #include "efm_internal.h"
 
+#include <vector>
+
 using namespace efm;
 
 namespace {
@@ -350,6 +352,42 @@ extern "C" int efm_combine(const efm_topology *topo, const double *K_val, const 
     hipLaunchKernelGGL((k_combine<float2>), dim3(grid_for(topo->nnz, kBlock)), dim3(kBlock), 0, st,
                        topo->nnz, (const double2 *)K_val, (const double2 *)M_val, k0sq, n_ports,
                        pc, (float2 *)A_out);
+  EFM_LAUNCH_CHECK();
+  return EFM_OK;
+}
+
+namespace {
+__global__ void k_add_diagonal(int64_t m, const int32_t *__restrict__ rows, const double2 *__restrict__ vals,
+                               const int32_t *__restrict__ row_ptr, const int32_t *__restrict__ col,
+                               double2 *__restrict__ A) {
+  const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= m)
+    return;
+  const int32_t r = rows[i];
+  for (int32_t k = row_ptr[r]; k < row_ptr[r + 1]; ++k)
+    if (col[k] == r)
+      A[k] = cadd(A[k], vals[i]);
+}
+} // namespace
+
+extern "C" int efm_add_diagonal(const efm_topology *topo, int64_t m, const int32_t *rows, const double *vals,
+                                double *A_val, void *stream) {
+  EFM_REQUIRE(topo && A_val, "efm_add_diagonal: NULL argument");
+  EFM_REQUIRE(m >= 0 && (m == 0 || (rows && vals)), "efm_add_diagonal: rows/vals missing");
+  if (m == 0)
+    return EFM_OK;
+  hipStream_t st = (hipStream_t)stream;
+  std::vector<int32_t> h(m);
+  EFM_HIP_CHECK(hipMemcpyAsync(h.data(), rows, m * sizeof(int32_t), hipMemcpyDeviceToHost, st));
+  EFM_HIP_CHECK(hipStreamSynchronize(st));
+  std::vector<uint8_t> seen(topo->n_edges, 0);
+  for (int64_t i = 0; i < m; ++i) {
+    EFM_REQUIRE(h[i] >= 0 && h[i] < topo->n_edges, "efm_add_diagonal: row out of range");
+    EFM_REQUIRE(!seen[h[i]], "efm_add_diagonal: rows must be distinct");
+    seen[h[i]] = 1;
+  }
+  hipLaunchKernelGGL(k_add_diagonal, dim3(grid_for(m, 256)), dim3(256), 0, st, m, rows, (const double2 *)vals,
+                     topo->row_ptr.as<int32_t>(), topo->col_idx.as<int32_t>(), (double2 *)A_val);
   EFM_LAUNCH_CHECK();
   return EFM_OK;
 }

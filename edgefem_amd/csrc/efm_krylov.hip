@@ -861,6 +861,109 @@ __global__ void __launch_bounds__(kBlock)
 
 
 // ---------------------------------------------------------------------------
+// Low-rank term of the operator: A_eff = A + sum_p c_p w_p w_p^H, with w_p given on a
+// compact list of edges (the rank-1 port loading of src/assemble_maxwell.cpp:209-242,
+// applied as an operator instead of dense blocks in the sparsity pattern).  The ports
+// are disjoint (checked on the host), so every list entry owns its row.  One block:
+//   t_{p,s} = w_p^H in_s;  out_s += c_p w_p t_{p,s} on the port rows,
+// and the change of the dot products the preceding SpMV wrote into its partials:
+//   MODE 0 (COCG):          sum in_s . dout_s           (bilinear)       -> R values
+//   MODE 1 (BiCGSTAB v):    sum conj(other_s) dout_s                     -> R values
+//   MODE 2 (BiCGSTAB t):    sum conj(dout_s) other_s, |out_s|^2 change   -> 2R values
+//   MODE 3 (plain SpMV):    none
+// written as one extra partial block, so the finalisation sums one block more.
+// ---------------------------------------------------------------------------
+constexpr int kMaxPorts = 8;
+
+template <int R, int MODE>
+__global__ void __launch_bounds__(kBlock)
+    k_lowrank(int P, int64_t len, const int32_t *__restrict__ idx, const double2 *__restrict__ wv,
+              const double2 *__restrict__ coef, const double2 *__restrict__ in, int in_stride, int in_off,
+              double2 *__restrict__ out, const double2 *__restrict__ other, double2 *__restrict__ slot,
+              const SysState *__restrict__ st) {
+  constexpr int V = MODE == 2 ? 2 * R : R;
+  __shared__ double2 lds[(kBlock / 64) * 2 * kMaxRhs];
+  __shared__ double2 tsh[kMaxPorts * kMaxRhs];
+  bool act[R];
+#pragma unroll
+  for (int s = 0; s < R; ++s)
+    act[s] = (MODE == 3) || st[s].active != 0;
+  for (int p = 0; p < P; ++p) {
+    double2 v[R];
+#pragma unroll
+    for (int s = 0; s < R; ++s)
+      v[s] = make_double2(0, 0);
+    for (int64_t i = threadIdx.x; i < len; i += kBlock) {
+      const int32_t r = idx[p * len + i];
+      if (r < 0)
+        continue;
+      const double2 w = cconj(wv[p * len + i]);
+#pragma unroll
+      for (int s = 0; s < R; ++s)
+        v[s] = cfma(w, in[(int64_t)r * in_stride + in_off + s], v[s]);
+    }
+    block_sum<R>(v, lds);
+    if (threadIdx.x < R)
+      tsh[p * R + threadIdx.x] = lds[threadIdx.x];
+    __syncthreads();
+  }
+  double2 c[V];
+#pragma unroll
+  for (int s = 0; s < V; ++s)
+    c[s] = make_double2(0, 0);
+  for (int64_t i = threadIdx.x; i < (int64_t)P * len; i += kBlock) {
+    const int32_t r = idx[i];
+    if (r < 0)
+      continue;
+    const int p = (int)(i / len);
+    const double2 cw = cmul(coef[p], wv[i]);
+#pragma unroll
+    for (int s = 0; s < R; ++s) {
+      if (!act[s])
+        continue;
+      const int64_t k = (int64_t)r * R + s;
+      const double2 d = cmul(cw, tsh[p * R + s]);
+      const double2 o = out[k];
+      const double2 nw = cadd(o, d);
+      out[k] = nw;
+      if constexpr (MODE == 0) {
+        c[s] = cfma(in[(int64_t)r * in_stride + in_off + s], d, c[s]);
+      } else if constexpr (MODE == 1) {
+        c[s] = cfma(cconj(other[k]), d, c[s]);
+      } else if constexpr (MODE == 2) {
+        c[s] = cfma(cconj(d), other[k], c[s]);
+        c[R + s].x += cabs2(nw) - cabs2(o);
+      }
+    }
+  }
+  if constexpr (MODE != 3) {
+    block_sum<V>(c, lds);
+    if (threadIdx.x < V)
+      slot[threadIdx.x] = lds[threadIdx.x];
+  }
+}
+
+// Jacobi diagonal of A_eff on the port rows: a_rr + c_p |w_{p,r}|^2.
+template <typename VT>
+__global__ void k_lowrank_dinv(int64_t m, int64_t len, const int32_t *__restrict__ idx,
+                               const double2 *__restrict__ wv, const double2 *__restrict__ coef,
+                               const int32_t *__restrict__ row_ptr, const int32_t *__restrict__ col,
+                               const VT *__restrict__ val, double2 *__restrict__ dinv) {
+  const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= m)
+    return;
+  const int32_t r = idx[i];
+  if (r < 0)
+    return;
+  double2 d = make_double2(0, 0);
+  for (int32_t k = row_ptr[r]; k < row_ptr[r + 1]; ++k)
+    if (col[k] == r)
+      d = to_d2(val[k]);
+  d = cadd(d, cscale(coef[i / len], cabs2(wv[i])));
+  dinv[r] = (d.x == 0.0 && d.y == 0.0) ? make_double2(1.0, 0.0) : cdiv(make_double2(1.0, 0.0), d);
+}
+
+// ---------------------------------------------------------------------------
 // Auxiliary-space (discrete gradient) preconditioner
 //   P r = D^{-1} r + G Dn^{-1} G^T r,   Dn = diag(G^T A G)
 // G is the edge-node incidence of the discrete gradient (-1 at the lower-ID end,
@@ -1121,6 +1224,10 @@ struct efm_solver {
   const uint8_t *dir = nullptr;
   efm::DevBuf dninv, ynode, gsign, en_eff;
   int grid_node_groups = 1;
+  // low-rank term (ports): P lists of len entries
+  int lr_P = 0;
+  int64_t lr_len = 0;
+  efm::DevBuf lr_idx, lr_val, lr_coef;
   bool profile = false;
   double spmv_ms = 0.0;
   int64_t spmv_launches = 0;
@@ -1154,6 +1261,23 @@ template <int R> struct Dispatch {
                        s->en_eff.as<int2>(), s->dinv.as<double2>(), s->ynode.as<double2>(), in, out,
                        ostride, s->partial.as<double2>(), S);
     EFM_LAUNCH_CHECK();
+    return EFM_OK;
+  }
+
+  // Low-rank term after an SpMV that wrote `out` (and nb partial blocks of V values);
+  // returns the partial block count including the correction block.
+  template <int MODE>
+  static int lowrank(efm_solver *s, const double2 *in, int in_stride, int in_off, double2 *out,
+                     const double2 *other, int nb, hipStream_t st, int *nb_out) {
+    constexpr int V = MODE == 2 ? 2 * R : R;
+    *nb_out = nb;
+    if (s->lr_P == 0)
+      return EFM_OK;
+    hipLaunchKernelGGL((k_lowrank<R, MODE>), dim3(1), dim3(kBlock), 0, st, s->lr_P, s->lr_len,
+                       s->lr_idx.as<int32_t>(), s->lr_val.as<double2>(), s->lr_coef.as<double2>(), in, in_stride,
+                       in_off, out, other, s->partial.as<double2>() + (int64_t)nb * V, s->state.as<SysState>());
+    EFM_LAUNCH_CHECK();
+    *nb_out = MODE == 3 ? nb : nb + 1;
     return EFM_OK;
   }
 
@@ -1287,6 +1411,8 @@ template <int R> struct Dispatch {
       return rc;
     if (e1)
       EFM_HIP_CHECK(hipEventRecord(e1, st));
+    if ((rc = lowrank<0>(s, Wb, 2 * R, R, s->q.as<double2>(), nullptr, nb_spmv, st, &nb_spmv)) != EFM_OK)
+      return rc;
     rc = fin<kFinAlpha>(s, nb_spmv, st, tol2, max_it);
     if (rc != EFM_OK)
       return rc;
@@ -1336,7 +1462,11 @@ template <int R> struct Dispatch {
     EFM_LAUNCH_CHECK();
     if (e1)
       EFM_HIP_CHECK(hipEventRecord(e1, st));
-    if ((rc = fin<kFinBsAlpha>(s, s->grid_spmv, st, tol2, max_it)) != EFM_OK)
+    int nbv = 0;
+    if ((rc = lowrank<1>(s, s->y.as<double2>(), R, 0, s->v.as<double2>(), s->r0.as<double2>(), s->grid_spmv, st,
+                         &nbv)) != EFM_OK)
+      return rc;
+    if ((rc = fin<kFinBsAlpha>(s, nbv, st, tol2, max_it)) != EFM_OK)
       return rc;
     hipLaunchKernelGGL(k_bs_s<R>, dim3(s->grid), dim3(kBlock), 0, st, s->n, s->dinv.as<double2>(),
                        s->v.as<double2>(), s->r.as<double2>(), s->zz.as<double2>(), S);
@@ -1347,7 +1477,11 @@ template <int R> struct Dispatch {
                        s->row_ptr, s->col, s->val.as<VT>(), s->zz.as<double2>(), s->t.as<double2>(),
                        s->r.as<double2>(), part, S);
     EFM_LAUNCH_CHECK();
-    if ((rc = fin<kFinBsOmega>(s, s->grid_spmv, st, tol2, max_it)) != EFM_OK)
+    int nbt = 0;
+    if ((rc = lowrank<2>(s, s->zz.as<double2>(), R, 0, s->t.as<double2>(), s->r.as<double2>(), s->grid_spmv, st,
+                         &nbt)) != EFM_OK)
+      return rc;
+    if ((rc = fin<kFinBsOmega>(s, nbt, st, tol2, max_it)) != EFM_OK)
       return rc;
     hipLaunchKernelGGL(k_bs_x<R>, dim3(s->grid), dim3(kBlock), 0, st, s->n, s->y.as<double2>(),
                        s->zz.as<double2>(), s->t.as<double2>(), s->r0.as<double2>(),
@@ -1360,7 +1494,8 @@ template <int R> struct Dispatch {
     hipLaunchKernelGGL((k_spmv<VT, R>), dim3(s->grid_spmv), dim3(kBlock), 0, st, s->n, s->row_ptr,
                        s->col, s->val.as<VT>(), x, y);
     EFM_LAUNCH_CHECK();
-    return EFM_OK;
+    int nb = 0;
+    return lowrank<3>(s, x, R, 0, y, nullptr, 0, st, &nb);
   }
 };
 
@@ -1457,7 +1592,7 @@ extern "C" int efm_solver_create(int64_t n, int64_t nnz, const int32_t *row_ptr,
   if ((rc = s->val.alloc(vbytes)) || (rc = s->dinv.alloc(n * sizeof(double2))) ||
       (rc = s->x.alloc(vb)) || (rc = s->r.alloc(vb)) || (rc = s->q.alloc(vb)) ||
       (rc = s->W0.alloc(2 * vb)) || (rc = s->W1.alloc(2 * vb)) ||
-      (rc = s->partial.alloc((size_t)gmax * 2 * kMaxRhs * sizeof(double2))) ||
+      (rc = s->partial.alloc((size_t)(gmax + 1) * 2 * kMaxRhs * sizeof(double2))) ||
       (rc = s->state.alloc(kMaxRhs * sizeof(SysState))) || (rc = s->control.alloc(sizeof(Control))))
     return rc;
   EFM_HIP_CHECK(hipMemset(s->control.p, 0, sizeof(Control)));
@@ -1479,6 +1614,18 @@ extern "C" int efm_solver_set_matrix(efm_solver *s, const void *values, void *st
     hipLaunchKernelGGL(k_dinv<float2>, dim3(grid_for(s->n, 256)), dim3(256), 0, st, s->n, s->row_ptr,
                        s->col, s->val.as<float2>(), s->dinv.as<double2>());
   EFM_LAUNCH_CHECK();
+  if (s->lr_P > 0) {
+    const int64_t m = s->lr_P * s->lr_len;
+    if (s->vtype == EFM_C128)
+      hipLaunchKernelGGL(k_lowrank_dinv<double2>, dim3(grid_for(m, 256)), dim3(256), 0, st, m, s->lr_len,
+                         s->lr_idx.as<int32_t>(), s->lr_val.as<double2>(), s->lr_coef.as<double2>(), s->row_ptr,
+                         s->col, s->val.as<double2>(), s->dinv.as<double2>());
+    else
+      hipLaunchKernelGGL(k_lowrank_dinv<float2>, dim3(grid_for(m, 256)), dim3(256), 0, st, m, s->lr_len,
+                         s->lr_idx.as<int32_t>(), s->lr_val.as<double2>(), s->lr_coef.as<double2>(), s->row_ptr,
+                         s->col, s->val.as<float2>(), s->dinv.as<double2>());
+    EFM_LAUNCH_CHECK();
+  }
   if (s->aux) {
     if (s->vtype == EFM_C128)
       hipLaunchKernelGGL(k_aux_diag<double2>, dim3(grid_for(s->n_nodes, 256)), dim3(256), 0, st,
@@ -1490,6 +1637,42 @@ extern "C" int efm_solver_set_matrix(efm_solver *s, const void *values, void *st
                          s->val.as<float2>(), s->dninv.as<double2>());
     EFM_LAUNCH_CHECK();
   }
+  return EFM_OK;
+}
+
+extern "C" int efm_solver_set_lowrank(efm_solver *s, int n_terms, int64_t len, const int32_t *gidx,
+                                      const double *gval, const double *h_coef, void *stream) {
+  EFM_REQUIRE(s, "efm_solver_set_lowrank: NULL solver");
+  EFM_REQUIRE(n_terms >= 0 && n_terms <= kMaxPorts, "efm_solver_set_lowrank: 0..8 terms");
+  if (n_terms == 0 || len <= 0) {
+    s->lr_P = 0;
+    s->lr_len = 0;
+    return EFM_OK;
+  }
+  EFM_REQUIRE(gidx && gval && h_coef, "efm_solver_set_lowrank: NULL argument");
+  hipStream_t st = (hipStream_t)stream;
+  const int64_t m = (int64_t)n_terms * len;
+  std::vector<int32_t> h(m);
+  EFM_HIP_CHECK(hipMemcpyAsync(h.data(), gidx, m * sizeof(int32_t), hipMemcpyDeviceToHost, st));
+  EFM_HIP_CHECK(hipStreamSynchronize(st));
+  std::vector<uint8_t> seen(s->n, 0);
+  for (int64_t i = 0; i < m; ++i) {
+    if (h[i] < 0)
+      continue;
+    EFM_REQUIRE(h[i] < s->n, "efm_solver_set_lowrank: edge index out of range");
+    EFM_REQUIRE(!seen[h[i]], "efm_solver_set_lowrank: an edge appears twice (ports must be disjoint)");
+    seen[h[i]] = 1;
+  }
+  int rc;
+  if ((rc = s->lr_idx.alloc(m * sizeof(int32_t))) || (rc = s->lr_val.alloc(m * sizeof(double2))) ||
+      (rc = s->lr_coef.alloc(kMaxPorts * sizeof(double2))))
+    return rc;
+  EFM_HIP_CHECK(hipMemcpyAsync(s->lr_idx.p, gidx, m * sizeof(int32_t), hipMemcpyDeviceToDevice, st));
+  EFM_HIP_CHECK(hipMemcpyAsync(s->lr_val.p, gval, m * sizeof(double2), hipMemcpyDeviceToDevice, st));
+  EFM_HIP_CHECK(hipMemcpyAsync(s->lr_coef.p, h_coef, n_terms * sizeof(double2), hipMemcpyHostToDevice, st));
+  EFM_HIP_CHECK(hipStreamSynchronize(st));
+  s->lr_P = n_terms;
+  s->lr_len = len;
   return EFM_OK;
 }
 

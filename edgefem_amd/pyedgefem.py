@@ -342,8 +342,8 @@ class MaxwellParams:
         self.port_weight_scale = 1.0
         self.port_abc_scale = 1.0
         self.use_eigenmode_excitation = False
-        # build extension: per-tet complex permittivity / permeability arrays, which take
-        # precedence over the region maps (one value per tet in mesh order)
+        # per-tet complex permittivity / permeability arrays (an addition to the reference's
+        # fields), which take precedence over the region maps (one value per tet in mesh order)
         self.eps_r_tet: Optional[np.ndarray] = None
         self.mu_r_tet: Optional[np.ndarray] = None
 
@@ -361,9 +361,9 @@ class MaxwellParams:
 
     def _check_supported(self):
         if self.eps_models or self.mu_models:
-            raise NotImplementedError("dispersive material models are outside this build's path")
+            raise NotImplementedError("dispersive material models are not supported by this GPU path")
         if self.pml_regions or self.pml_tensor_regions:
-            raise NotImplementedError("PML regions are outside this build's path")
+            raise NotImplementedError("PML regions are not supported by this GPU path")
 
 
 def tet_materials(mesh: Mesh, p: MaxwellParams):
@@ -568,11 +568,15 @@ class SpMatC:
     structurally symmetric matrices of this path the arrays equal the reference's CSC
     arrays.  ``symmetric`` records whether A^T == A (selects COCG in solve_linear)."""
 
-    def __init__(self, row_ptr, col_idx, values, symmetric: bool = False):
+    def __init__(self, row_ptr, col_idx, values, symmetric: bool = False, lowrank=None):
         self.row_ptr = row_ptr
         self.col_idx = col_idx
         self.values = values
         self.symmetric = bool(symmetric)
+        # rank-1 port loading kept as an operator term: list of (edges, weights, coef),
+        # A_eff = A + sum coef w w^H (assemble_maxwell.cpp:209-242 inserts these blocks
+        # into the pattern instead); coeff / to_dense / nnz report the full matrix
+        self.lowrank = list(lowrank or [])
         self._host = None
 
     @property
@@ -581,7 +585,9 @@ class SpMatC:
         return (n, n)
 
     def nnz(self):
-        return int(self.col_idx.numel())
+        if not self.lowrank:
+            return int(self.col_idx.numel())
+        return int(self.to_scipy().nnz)
 
     def _h(self):
         if self._host is None:
@@ -592,15 +598,25 @@ class SpMatC:
         rp, ci, v = self._h()
         lo, hi = rp[row], rp[row + 1]
         j = lo + np.searchsorted(ci[lo:hi], col)
-        if j < hi and ci[j] == col:
-            return complex(v[j])
-        return 0j
+        out = complex(v[j]) if (j < hi and ci[j] == col) else 0j
+        for e, w, c in self.lowrank:
+            a = np.nonzero(e == row)[0]
+            b = np.nonzero(e == col)[0]
+            if a.size and b.size:
+                out += complex(c) * w[a[0]] * np.conj(w[b[0]])
+        return out
 
     def to_scipy(self):
         import scipy.sparse as sp
         rp, ci, v = self._h()
         n = self.shape[0]
-        return sp.csr_matrix((v, ci, rp), shape=(n, n))
+        A = sp.csr_matrix((v, ci, rp), shape=(n, n))
+        for e, w, c in self.lowrank:
+            A = A + sp.csr_matrix((np.outer(w, np.conj(w)).ravel() * complex(c),
+                                   (np.repeat(e, e.size), np.tile(e, e.size))), shape=(n, n))
+        if self.lowrank:
+            A.sort_indices()
+        return A
 
     def to_dense(self):
         return self.to_scipy().toarray()
@@ -664,10 +680,26 @@ def _solve_device(A: SpMatC, B, opts: SolveOptions, nrhs: int):
     tol = 1e-14 if opts.use_direct else float(opts.tolerance)
     method = _capi.EFM_METHOD_COCG if A.symmetric else _capi.EFM_METHOD_BICGSTAB
     solver = _capi.KrylovSolver(A.row_ptr, A.col_idx, nrhs=nrhs)
+    if A.lowrank:
+        solver.set_lowrank(*_pack_lowrank(A.lowrank))
     solver.set_matrix(A.values)
     X, info = solver.solve(B, tol=tol, max_iterations=int(opts.max_iterations), method=method)
     name = ("COCG" if A.symmetric else "BiCGSTAB") + "+Jacobi"
     return X, info, name
+
+
+def _pack_lowrank(lowrank):
+    """(edges, weights, coef) terms -> padded device index/value lists and coefficients."""
+    import torch
+    L = max([len(e) for e, _, _ in lowrank] + [1])
+    P = len(lowrank)
+    gi = np.full((P, L), -1, np.int32)
+    gv = np.zeros((P, L), np.complex128)
+    for k, (e, w, _) in enumerate(lowrank):
+        gi[k, :len(e)] = e
+        gv[k, :len(e)] = w
+    return (_capi.to_dev(gi.ravel(), torch.int32), _capi.to_dev(gv.ravel(), torch.complex128),
+            [complex(c) for _, _, c in lowrank])
 
 
 def solve_linear(A: SpMatC, b, options: Optional[SolveOptions] = None) -> SolveResult:
@@ -705,15 +737,44 @@ def _dirichlet_mask(mesh: Mesh, bc: BC) -> np.ndarray:
     return bc.mask(mesh.num_edges())
 
 
-def assemble_maxwell(mesh: Mesh, params: MaxwellParams, bc: BC, ports: Sequence[WavePort] = (),
-                     active_port_idx: int = -1) -> MaxwellAssembly:
-    """Volume + ABC + Dirichlet on the GPU (assemble_maxwell.cpp:46-350).  Rank-1 port
-    loading (209-242) grows the pattern with dense port blocks and is not formed here:
-    ``calculate_sparams`` applies it through the Woodbury identity instead."""
-    import torch
-    if ports:
-        raise NotImplementedError("assemble_maxwell with rank-1 port loading forms a dense port block; "
-                                  "use calculate_sparams / calculate_sparams_eigenmode")
+def _valid_port(port: WavePort) -> bool:
+    return len(port.weights) == len(port.edges) and complex(port.mode.Z0) != 0
+
+
+def _free_port(port: WavePort, dmask: np.ndarray):
+    """(edges, weights) of a port without its PEC edges."""
+    e = np.asarray(port.edges, np.int64)
+    w = np.asarray(port.weights, np.complex128)
+    keep = dmask[e] == 0 if e.size else np.zeros(0, bool)
+    return e[keep], w[keep]
+
+
+def _port_abc_coefficient(params: MaxwellParams, port: WavePort) -> Optional[complex]:
+    """Diagonal port ABC term (assemble_maxwell.cpp:274-320), only above cutoff, with
+    eta0 = mu0 c0 and eps0 = 1/(mu0 c0^2)."""
+    if complex(port.mode.Z0) == 0:
+        return None
+    k0 = _k0(params)
+    bsq = k0 * k0 - port.mode.kc * port.mode.kc
+    if bsq <= 0:
+        return None
+    beta = math.sqrt(bsq)
+    z0r = complex(port.mode.Z0).real
+    t = PortABCType(int(params.port_abc_type))
+    if t == PortABCType.Beta:
+        return 1j * beta
+    if t == PortABCType.BetaNorm:
+        return 1j * beta / k0
+    if t == PortABCType.ImpedanceMatch:
+        return 1j * beta * math.sqrt(z0r / (MU0 * C0))
+    if t == PortABCType.ModalAdmittance:
+        return 1j * params.omega * (1.0 / (MU0 * C0 * C0)) / z0r
+    return None
+
+
+def _assemble_with_ports(mesh: Mesh, params: MaxwellParams, bc: BC, ports: Sequence[WavePort]):
+    """Volume + ABC (+ port ABC diagonals) on the GPU, and the rank-1 loading terms of every
+    valid port as (edges, weights, 1/Z0) for the operator (assemble_maxwell.cpp:46-350)."""
     eps, mu = tet_materials(mesh, params)
     dmask = _dirichlet_mask(mesh, bc)
     abc = None
@@ -725,9 +786,223 @@ def assemble_maxwell(mesh: Mesh, params: MaxwellParams, bc: BC, ports: Sequence[
         abc[dmask.astype(bool)] = 0
     topo = mesh.topology
     A = _capi.assemble_maxwell(topo, mesh.dev_xyz, eps, 1.0 / mu, dmask, abc, params.omega)
-    sym = True
-    b = VecC(np.zeros(mesh.num_edges(), np.complex128))
-    return MaxwellAssembly(SpMatC(topo.row_ptr, topo.col_idx, A, symmetric=sym), b)
+    if params.use_port_abc and PortABCType(int(params.port_abc_type)) != PortABCType.None_:
+        rows, vals = {}, {}
+        for port in ports:
+            c = _port_abc_coefficient(params, port)
+            if c is None:
+                continue
+            for e in np.asarray(port.edges, np.int64):
+                if not dmask[e]:
+                    vals[int(e)] = vals.get(int(e), 0j) + c
+        if vals:
+            r = np.array(sorted(vals), np.int32)
+            _capi.add_diagonal(topo, r, [vals[int(k)] for k in r], A)
+    lowrank = []
+    for port in ports:
+        if not _valid_port(port):
+            continue
+        e, w = _free_port(port, dmask)
+        if e.size:
+            lowrank.append((e, w, 1.0 / complex(port.mode.Z0)))
+    return A, lowrank, dmask
+
+
+def assemble_maxwell(mesh: Mesh, params: MaxwellParams, bc: BC, ports: Sequence[WavePort] = (),
+                     active_port_idx: int = -1) -> MaxwellAssembly:
+    """A and b on the GPU (assemble_maxwell.cpp:46-350): volume terms, Dirichlet rows and
+    columns, the j k0 ABC and the port ABC diagonals.  The rank-1 port loading w w^H / Z0
+    (209-242) is kept as an operator term of the returned SpMatC (``lowrank``) instead of
+    dense blocks in the sparsity pattern; ``coeff``, ``to_dense`` and ``nnz`` report the
+    full matrix and ``solve_linear`` applies it.  b = 2 w / sqrt(Z0) on the free edges of
+    the active port."""
+    params._check_supported()
+    A, lowrank, dmask = _assemble_with_ports(mesh, params, bc, ports)
+    topo = mesh.topology
+    sym = all(not np.iscomplexobj(w) or np.abs(np.asarray(w).imag).max(initial=0.0) == 0 for _, w, _ in lowrank)
+    b = np.zeros(mesh.num_edges(), np.complex128)
+    if 0 <= active_port_idx < len(ports) and _valid_port(ports[active_port_idx]):
+        port = ports[active_port_idx]
+        e, w = _free_port(port, dmask)
+        np.add.at(b, e, 2.0 / cmath.sqrt(complex(port.mode.Z0)) * w)
+    return MaxwellAssembly(SpMatC(topo.row_ptr, topo.col_idx, A, symmetric=sym, lowrank=lowrank), VecC(b))
+
+
+def _solver_nrhs(P: int) -> int:
+    for r in (1, 2, 3, 4, 8):
+        if P <= r:
+            return r
+    raise EdgeFemError("at most 8 ports per solve")
+
+
+def calculate_sparams(mesh: Mesh, params: MaxwellParams, bc: BC, ports: Sequence[WavePort],
+                      solver_options: Optional[SolveOptions] = None, *, preconditioner: str = "aux",
+                      _state: Optional[dict] = None) -> np.ndarray:
+    """Rank-1 port S-parameters (assemble_maxwell.cpp:352-394) on the GPU.  A = volume +
+    ABC + port ABC + sum_p w_p w_p^H / Z0_p is shared by every excitation, so the P
+    right-hand sides b_i = 2 w_i / sqrt(Z0_i) are solved as one block (the reference
+    re-assembles and re-solves per active port).  V_j = sum conj(w_j) x on the free port
+    edges; S_ii = (V_i - sqrt Z0_i)/sqrt Z0_i, S_ji = V_j / sqrt Z0_i; a column whose solve
+    did not converge is NaN, with the reference's warning."""
+    import torch
+    params._check_supported()
+    P = len(ports)
+    if P == 0:
+        return np.zeros((0, 0), np.complex128)
+    opts = solver_options or SolveOptions()
+    asm = assemble_maxwell(mesh, params, bc, ports, -1)
+    Am = asm.A
+    dmask = _dirichlet_mask(mesh, bc)
+    topo = mesh.topology
+    n = mesh.num_edges()
+    R = _solver_nrhs(P)
+    Bh = np.zeros((n, R), np.complex128)
+    frees = [_free_port(p, dmask) for p in ports]
+    for i, port in enumerate(ports):
+        if _valid_port(port):
+            e, w = frees[i]
+            Bh[e, i] = 2.0 / cmath.sqrt(complex(port.mode.Z0)) * w
+    B = _capi.to_dev(Bh, torch.complex128)
+    solver = _capi.KrylovSolver(topo.row_ptr, topo.col_idx, nrhs=R)
+    if preconditioner == "aux":
+        solver.set_gradient(topo, dmask)
+    if Am.lowrank:
+        solver.set_lowrank(*_pack_lowrank(Am.lowrank))
+    solver.set_matrix(Am.values)
+    tol = 1e-14 if opts.use_direct else float(opts.tolerance)
+    method = _capi.EFM_METHOD_COCG if Am.symmetric else _capi.EFM_METHOD_BICGSTAB
+    X, info = solver.solve(B, tol=tol, max_iterations=int(opts.max_iterations), method=method)
+    L = max([f[0].size for f in frees] + [1])
+    gi = np.full((P, L), -1, np.int32)
+    gv = np.zeros((P, L), np.complex128)
+    for j, (e, w) in enumerate(frees):
+        gi[j, :e.size] = e
+        gv[j, :e.size] = w
+    V = _capi.port_overlap(_capi.to_dev(gi, torch.int32), _capi.to_dev(gv, torch.complex128),
+                           X, R)[:, :P]                                # V[j, i] = w_j^H x_i
+    S = np.full((P, P), np.nan + 1j * np.nan, np.complex128)
+    name = ("COCG" if Am.symmetric else "BiCGSTAB") + "+" + preconditioner
+    for i, port in enumerate(ports):
+        if not (info["resid"][i] <= tol or info["iters"][i] == 0):
+            sys.stderr.write(f"WARNING: Solver did not converge in calculate_sparams (active port {i}) "
+                             f"(method={name}, iters={info['iters'][i]}, residual={info['resid'][i]}). "
+                             "S-parameters may be unreliable.\n")
+            continue
+        vinc = cmath.sqrt(complex(port.mode.Z0))
+        for j in range(P):
+            S[j, i] = (V[j, i] - vinc) / vinc if i == j else V[j, i] / vinc
+    if _state is not None:
+        _state.update(dict(A=Am, B=B, X=X, info=info))
+    return S
+
+
+def normalize_port_weights(mesh: Mesh, params: MaxwellParams, bc: BC, ports: List[WavePort],
+                           solver_options: Optional[SolveOptions] = None) -> List[WavePort]:
+    """Scale each port's weights so that |w^H A^{-1} w| = Re Z0, A assembled without ports
+    (assemble_maxwell.cpp:396-494); all ports are solved as one block of right-hand sides
+    on the GPU.  Modifies the ports in place and returns them, as the Python binding of
+    the reference does."""
+    import torch
+    params._check_supported()
+    P = len(ports)
+    if P == 0:
+        return list(ports)
+    opts = solver_options or SolveOptions()
+    asm = assemble_maxwell(mesh, params, bc, [], -1)
+    dmask = _dirichlet_mask(mesh, bc)
+    topo = mesh.topology
+    n = mesh.num_edges()
+    R = _solver_nrhs(P)
+    Bh = np.zeros((n, R), np.complex128)
+    frees = [_free_port(p, dmask) for p in ports]
+    use = [_valid_port(p) for p in ports]
+    for i in range(P):
+        if use[i]:
+            e, w = frees[i]
+            Bh[e, i] = w
+    solver = _capi.KrylovSolver(topo.row_ptr, topo.col_idx, nrhs=R)
+    solver.set_gradient(topo, dmask)
+    solver.set_matrix(asm.A.values)
+    tol = 1e-14 if opts.use_direct else float(opts.tolerance)
+    X, info = solver.solve(_capi.to_dev(Bh, torch.complex128), tol=tol, max_iterations=int(opts.max_iterations),
+                           method=_capi.EFM_METHOD_COCG)
+    Xh = X.reshape(n, R).cpu().numpy()
+    for i, port in enumerate(ports):
+        if not use[i]:
+            continue
+        if not (info["resid"][i] <= tol or info["iters"][i] == 0):
+            sys.stderr.write("WARNING: Solver did not converge in normalize_port_weights\n"
+                             "  Skipping normalization for this port due to solver failure.\n")
+            continue
+        w_all = np.asarray(port.weights, np.complex128)
+        if np.linalg.norm(w_all) < 1e-15:
+            sys.stderr.write("WARNING: Port weight vector norm is near-zero.\n")
+            continue
+        e, w = frees[i]
+        c = abs(np.sum(np.conj(w) * Xh[e, i]))
+        target = complex(port.mode.Z0).real
+        if c > 1e-15 and target > 1e-15:
+            port.weights = w_all * math.sqrt(target / c)
+    return list(ports)
+
+
+class LumpedPortWeightMode(enum.IntEnum):
+    Projection = 0
+    SurfaceIntegral = 1
+
+
+class LumpedPortConfig:
+    def __init__(self):
+        self.surface_tag = 0
+        self.z0 = 50.0
+        self.e_direction = np.array([0.0, 0.0, 1.0])
+        self.weight_mode = LumpedPortWeightMode.SurfaceIntegral
+
+
+def build_lumped_port(mesh: Mesh, config: LumpedPortConfig) -> WavePort:
+    """Lumped port on the tris of ``config.surface_tag`` (lumped_port.cpp:21-160):
+    surface-integral weights w_e = sum_tri s (A/3)(grad l_j - grad l_i).e (in-plane
+    gradients), or the projection of the edge vector on e; scaled to ||w|| = sqrt(z0).
+    Edges are sorted (the reference keeps hash-set order; S does not depend on it)."""
+    sel = np.nonzero(mesh.tri_phys == config.surface_tag)[0]
+    if sel.size == 0:
+        raise RuntimeError(f"build_lumped_port: no triangles found with surface_tag = {config.surface_tag}")
+    edges = np.unique(mesh.tri_edges[sel].ravel())
+    ed = np.asarray(config.e_direction, np.float64)
+    ed = ed / np.linalg.norm(ed)
+    w = np.zeros(edges.size, np.complex128)
+    if int(config.weight_mode) == LumpedPortWeightMode.SurfaceIntegral:
+        v = mesh.xyz[mesh.tri_nodes[sel]]
+        nrm = np.cross(v[:, 1] - v[:, 0], v[:, 2] - v[:, 0])
+        a2 = np.linalg.norm(nrm, axis=1)
+        ok = a2 >= 1e-30
+        a2s = np.where(ok, a2, 1.0)
+        nh = nrm / a2s[:, None]
+        g = np.stack([np.cross(nh, v[:, 2] - v[:, 1]), np.cross(nh, v[:, 0] - v[:, 2]),
+                      np.cross(nh, v[:, 1] - v[:, 0])], axis=1) / a2s[:, None, None]
+        gd = g @ ed                                                   # (F, 3)
+        pairs = np.array([(0, 1), (1, 2), (2, 0)])
+        val = (a2s / 6.0)[:, None] * (gd[:, pairs[:, 1]] - gd[:, pairs[:, 0]]) * mesh.tri_sign[sel]
+        val[~ok] = 0.0
+        pos = np.searchsorted(edges, mesh.tri_edges[sel])
+        np.add.at(w, pos.ravel(), val.ravel())
+    else:
+        nid = {int(v): k for k, v in enumerate(mesh.node_ids)}
+        p0 = mesh.xyz[[nid[int(x)] for x in mesh.edge_nodes[edges, 0]]]
+        p1 = mesh.xyz[[nid[int(x)] for x in mesh.edge_nodes[edges, 1]]]
+        w = ((p1 - p0) @ ed).astype(np.complex128)
+    nw = np.linalg.norm(w)
+    if nw > 1e-15:
+        w = w * (math.sqrt(config.z0) / nw)
+    port = WavePort()
+    port.surface_tag = config.surface_tag
+    port.edges = [int(e) for e in edges]
+    port.weights = w
+    port.mode.Z0 = complex(config.z0)
+    port.mode.beta = 0j
+    port.mode.kc = 0.0
+    port.mode.fc = 0.0
+    return port
 
 
 def assemble_maxwell_km(mesh: Mesh, params: MaxwellParams, bc: BC):

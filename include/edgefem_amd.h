@@ -105,6 +105,11 @@ int efm_port_surface_mass(const efm_topology *topo, const double *xyz, int64_t n
                           const int32_t *tri_sel, const uint8_t *dirichlet, double *Ms_val,
                           void *stream);
 
+/* A[r, r] += vals[i] for distinct rows[i] (device int32 [m], c128 [m]) on the volume
+ * pattern of topo: the diagonal port ABC terms of src/assemble_maxwell.cpp:274-320. */
+int efm_add_diagonal(const efm_topology *topo, int64_t m, const int32_t *rows, const double *vals,
+                     double *A_val, void *stream);
+
 /* A_val = K_val - k0sq * M_val + sum_p coef[p] * Ms_val[p]  (c128 out, or c64 when
  * out_type == EFM_C64).  h_coef[n_ports] c128 on the host; Ms_vals[p] device arrays.
  * Replaces KMMatrices::combine (src/sweep.cpp:82-88) plus the jbeta M_s port operator
@@ -136,6 +141,16 @@ int efm_solver_set_matrix(efm_solver *s, const void *values, void *stream);
 int efm_solver_set_gradient(efm_solver *s, int64_t n_nodes, const int32_t *edge_nodes,
                             const int32_t *n2e_ptr, const int32_t *n2e, const uint8_t *dirichlet,
                             void *stream);
+
+/* Low-rank operator term: the solver iterates on A_eff = A + sum_p h_coef[p] w_p w_p^H, with
+ * w_p given on a compact edge list (gidx[p*len + i] >= 0, -1 = padding; gval the values;
+ * device arrays, copied).  This is the rank-1 port loading w w^H / Z0 of
+ * src/assemble_maxwell.cpp:209-242, applied inside the operator instead of as dense
+ * blocks added to the sparsity pattern; the Jacobi diagonal includes it.  Ports must be
+ * disjoint (an edge in two lists is rejected).  n_terms = 0 removes the term.  Call before
+ * efm_solver_set_matrix.  At most 8 terms. */
+int efm_solver_set_lowrank(efm_solver *s, int n_terms, int64_t len, const int32_t *gidx,
+                           const double *gval, const double *h_coef, void *stream);
 
 typedef struct {
   int method;            /* EFM_METHOD_COCG or EFM_METHOD_BICGSTAB */

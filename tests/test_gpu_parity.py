@@ -477,3 +477,116 @@ def test_adjoint_gradient_torch_autograd(torch_dev):
         fd = (Jp - Jm) / (2 * h)
         _p(f"    tet {design[k]}: adjoint {grad[k]:.9e} central difference {fd:.9e}")
         assert abs(grad[k] - fd) <= 1e-5 * max(abs(fd), 1e-6 * np.abs(ref).max())
+
+
+def _lumped_pair(pm, om, pec_o, bc, z0=50.0):
+    from edgefem_amd import pyedgefem as em
+    from oracle import edgefem_oracle as O
+    gports, oports = [], []
+    for tag in (2, 3):
+        cfg = em.LumpedPortConfig()
+        cfg.surface_tag = tag
+        cfg.z0 = z0
+        cfg.e_direction = np.array([0.0, 1.0, 0.0])
+        gports.append(em.build_lumped_port(pm, cfg))
+        oports.append(O.build_lumped_port(om, tag, z0, (0.0, 1.0, 0.0)))
+    return gports, oports
+
+
+def test_rank1_lumped_ports_normalized_vs_oracle(torch_dev):
+    """Lumped ports on both faces of examples/rect_waveguide.msh at 10 GHz: weights of
+    build_lumped_port, normalize_port_weights (block solve of the port-less system) and
+    calculate_sparams with the rank-1 loading applied inside the Krylov operator, against
+    the oracle (dense port blocks, SuperLU)."""
+    from edgefem_amd import pyedgefem as em
+    from oracle import edgefem_oracle as O
+    _p("[gpu] rank-1 lumped ports, normalized")
+    pm = em.load_gmsh(golden("rect_waveguide.msh"))
+    om = _oracle_mesh_from_file("rect_waveguide.msh")
+    bc = em.build_edge_pec(pm, 1)
+    pec = O.build_edge_pec(om, 1)
+    gports, oports = _lumped_pair(pm, om, pec, bc)
+    for g, o in zip(gports, oports):
+        np.testing.assert_array_equal(np.asarray(g.edges), o.edges)
+        np.testing.assert_allclose(g.weights, o.weights, rtol=1e-12, atol=1e-15)
+    params = em.MaxwellParams()
+    params.omega = 2 * math.pi * 10e9
+    op = O.Params(omega=params.omega)
+    opts = em.SolveOptions()
+    opts.tolerance = 1e-12
+    em.normalize_port_weights(pm, params, bc, gports, opts)
+    oports = O.normalize_port_weights(om, op, pec, oports)
+    for g, o in zip(gports, oports):
+        np.testing.assert_allclose(g.weights, o.weights, rtol=1e-8, atol=1e-12)
+    st = {}
+    S = em.calculate_sparams(pm, params, bc, gports, _state=st)
+    So = O.calculate_sparams(om, op, pec, oports)
+    _p(f"    S21 {S[1, 0]:.6f} (oracle {So[1, 0]:.6f}); iterations {st['info']['iters']}")
+    assert np.abs(S - So).max() <= 1e-6
+
+
+def test_rank1_wave_ports_port_abc_and_bicgstab(torch_dev):
+    """2D-eigenmode wave ports with the rank-1 loading plus the port ABC diagonal (Beta) on
+    examples/rect_waveguide.msh at 10 GHz, against the oracle; then the same ports with
+    weights multiplied by exp(j 0.7), which leaves S unchanged but makes the weights
+    complex, so the solve takes the BiCGSTAB path with the low-rank term."""
+    import copy
+    from edgefem_amd import pyedgefem as em
+    from oracle import edgefem_oracle as O
+    _p("[gpu] rank-1 wave ports + port ABC; BiCGSTAB")
+    pm = em.load_gmsh(golden("rect_waveguide.msh"))
+    om = _oracle_mesh_from_file("rect_waveguide.msh")
+    bc = em.build_edge_pec(pm, 1)
+    pec = O.build_edge_pec(om, 1)
+    a = 0.02286
+    mode = em.solve_te10_mode(em.RectWaveguidePort(a, 0.01016), 10e9)
+    kc2 = (math.pi / a) ** 2
+    gports = [em.build_wave_port_2d(pm, t, mode, bc.dirichlet_edges, kc2) for t in (2, 3)]
+    kc, beta, Z0 = O.solve_te10_mode(a, 10e9)
+    oports = [O.build_wave_port_2d(om, t, kc, Z0, beta, pec, kc2) for t in (2, 3)]
+    for g, o in zip(gports, oports):
+        o.kc = g.mode.kc
+        np.testing.assert_allclose(g.weights, o.weights, rtol=1e-9, atol=1e-12)
+    params = em.MaxwellParams()
+    params.omega = 2 * math.pi * 10e9
+    params.use_port_abc = True
+    params.port_abc_type = em.PortABCType.Beta
+    op = O.Params(omega=params.omega, use_port_abc=True, port_abc_type="beta")
+    S = em.calculate_sparams(pm, params, bc, gports)
+    So = O.calculate_sparams(om, op, pec, oports)
+    _p(f"    S {S.ravel()} oracle {So.ravel()}")
+    assert np.abs(S - So).max() <= 1e-6
+    cports = [copy.deepcopy(p) for p in gports]
+    for p in cports:
+        p.weights = np.asarray(p.weights, np.complex128) * np.exp(0.7j)
+    st = {}
+    Sc = em.calculate_sparams(pm, params, bc, cports, _state=st)
+    assert not st["A"].symmetric
+    _p(f"    complex weights: iterations {st['info']['iters']}")
+    assert np.abs(Sc - So).max() <= 1e-6
+
+
+def test_assemble_maxwell_with_ports_entries(torch_dev):
+    """assemble_maxwell with one-edge ports (tests/test_maxwell.cpp:18-51 setup) and lumped
+    ports: every entry of the returned matrix (volume + low-rank loading) and b against the
+    oracle's assembled matrix with dense port blocks, to 1e-12 of the row/column scale."""
+    from edgefem_amd import pyedgefem as em
+    from oracle import edgefem_oracle as O
+    _p("[gpu] assemble_maxwell with ports")
+    pm = em.load_gmsh(golden("rect_waveguide.msh"))
+    om = _oracle_mesh_from_file("rect_waveguide.msh")
+    bc = em.build_edge_pec(pm, 1)
+    pec = O.build_edge_pec(om, 1)
+    gports, oports = _lumped_pair(pm, om, pec, bc)
+    params = em.MaxwellParams()
+    params.omega = 2 * math.pi * 9e9
+    asm = em.assemble_maxwell(pm, params, bc, gports, 1)
+    Ao, bo = O.assemble_maxwell(om, O.Params(omega=params.omega), pec, oports, 1)
+    Ag = asm.A.to_scipy().tocsr()
+    D = (Ag - Ao).tocoo()
+    d = np.abs(Ao.diagonal())
+    scale = np.sqrt(d[D.row] * d[D.col]) + 1e-300
+    assert (np.abs(D.data) / scale).max() <= 1e-12
+    np.testing.assert_allclose(asm.b.to_numpy(), bo, rtol=0, atol=1e-14)
+    e = int(oports[1].edges[~pec[oports[1].edges]][0])
+    assert abs(asm.A.coeff(e, e) - Ao[e, e]) <= 1e-12 * abs(Ao[e, e])
