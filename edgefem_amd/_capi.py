@@ -31,7 +31,7 @@ EXPORTS = [
     "efm_solver_spmv", "efm_solver_profile", "efm_solver_profile_read",
     "efm_port_overlap", "efm_solver_set_spmv_variant", "efm_solver_get_spmv_variant",
     "efm_sweep_create", "efm_sweep_destroy", "efm_sweep_solve", "efm_eps_gradient",
-    "efm_solver_set_lowrank", "efm_add_diagonal",
+    "efm_solver_set_lowrank", "efm_add_diagonal", "efm_periodic_reduce",
 ]
 
 
@@ -99,6 +99,8 @@ def load_library(path: str = LIB_PATH) -> C.CDLL:
     lib.efm_sweep_create.argtypes = [vp, vp, vp, i32, vp, vp, i32, vp, C.POINTER(vp)]
     lib.efm_solver_set_lowrank.argtypes = [vp, i32, C.c_int64, vp, vp, vp, vp]
     lib.efm_add_diagonal.argtypes = [vp, C.c_int64, vp, vp, vp, vp]
+    lib.efm_periodic_reduce.argtypes = [C.c_int64, vp, vp, vp, C.c_int64, vp, vp, vp, vp, vp, vp, vp, vp,
+                                        C.c_int64, C.POINTER(C.c_int64), vp]
     lib.efm_eps_gradient.argtypes = [vp, vp, C.c_int64, vp, i32, vp, i32, vp, i32, vp, vp, vp, vp, vp]
     lib.efm_sweep_destroy.argtypes = [vp]
     lib.efm_sweep_destroy.restype = None
@@ -499,3 +501,30 @@ def add_diagonal(topo: Topology, rows, vals, A):
     _check(lib.efm_add_diagonal(topo.handle, int(r.size), dptr(rd), dptr(vd), dptr(A), stream_ptr()),
            "efm_add_diagonal")
     return A
+
+
+def periodic_reduce(row_ptr, col_idx, values, master, slave, coef_row, coef_col, dirichlet=None):
+    """Sparse periodic elimination (efm_periodic_reduce): returns the reduced CSR
+    (row_ptr, col_idx, values) as device tensors."""
+    torch = _torch()
+    lib = load_library()
+    rp = to_dev(row_ptr, torch.int32)
+    ci = to_dev(col_idx, torch.int32)
+    v = to_dev(values, torch.complex128)
+    n = rp.numel() - 1
+    m = np.ascontiguousarray(np.asarray(master, np.int32))
+    s_ = np.ascontiguousarray(np.asarray(slave, np.int32))
+    cr = np.ascontiguousarray(np.asarray(coef_row, np.complex128))
+    cc = np.ascontiguousarray(np.asarray(coef_col, np.complex128))
+    dh = None if dirichlet is None else np.ascontiguousarray(np.asarray(dirichlet, np.uint8))
+    rp_out = torch.empty(n + 1, dtype=torch.int32, device=rp.device)
+    nnz = C.c_int64(0)
+    args = (m.ctypes.data_as(C.c_void_p), s_.ctypes.data_as(C.c_void_p), cr.ctypes.data_as(C.c_void_p),
+            cc.ctypes.data_as(C.c_void_p), None if dh is None else dh.ctypes.data_as(C.c_void_p))
+    _check(lib.efm_periodic_reduce(n, dptr(rp), dptr(ci), dptr(v), int(m.size), *args, dptr(rp_out), None, None, 0,
+                                   C.byref(nnz), stream_ptr()), "efm_periodic_reduce")
+    col_out = torch.empty(max(nnz.value, 1), dtype=torch.int32, device=rp.device)
+    val_out = torch.empty(max(nnz.value, 1), dtype=torch.complex128, device=rp.device)
+    _check(lib.efm_periodic_reduce(n, dptr(rp), dptr(ci), dptr(v), int(m.size), *args, dptr(rp_out), dptr(col_out),
+                                   dptr(val_out), int(nnz.value), C.byref(nnz), stream_ptr()), "efm_periodic_reduce")
+    return rp_out, col_out[:nnz.value], val_out[:nnz.value]

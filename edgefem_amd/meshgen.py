@@ -105,13 +105,17 @@ class BoxMesh:
 
 def box_mesh(a: float, b: float, L: float, nx: int, ny: int, nz: int, *, seed: int = 0,
              jitter: float = 0.15, zplanes: Optional[np.ndarray] = None,
-             tags=(1, 2, 3, 100), side_tags=None) -> BoxMesh:
+             tags=(1, 2, 3, 100), side_tags=None, zflat=(), periodic_faces: bool = False) -> BoxMesh:
     """Generate the Kuhn-split box mesh.
 
     ``zplanes`` optionally gives the nz+1 z coordinates of the grid planes (for layered
     cells); otherwise they are uniform.  ``tags`` = (pec, port_z0, port_zL, volume).
     ``side_tags`` optionally overrides the wall tags per side as a dict with keys
-    ``x0, x1, y0, y1`` (used for periodic unit cells).
+    ``x0, x1, y0, y1`` (used for periodic unit cells).  Nodes on the z planes listed in
+    ``zflat`` are not jittered along z (a flat interior interface).  With
+    ``periodic_faces`` the x = a and y = b triangles are translated copies of the x = 0 and
+    y = 0 ones (same local node order), as a periodic mesher writes a slave surface;
+    otherwise every boundary triangle is wound with its normal pointing outward.
     """
     pec, p1, p2, vol = tags
     side = {"x0": pec, "x1": pec, "y0": pec, "y1": pec}
@@ -135,10 +139,12 @@ def box_mesh(a: float, b: float, L: float, nx: int, ny: int, nz: int, *, seed: i
         dz_hi = zs[np.minimum(k + 1, nz)] - zs[k]
         hz = np.minimum(dz_lo, dz_hi)
         ctr = node_idx * 3
+        flat = np.isin(k, np.asarray(list(zflat), dtype=np.int64))
         for ax, h in ((0, hx), (1, hy), (2, hz)):
             u = uniform01(seed, ctr + ax) * 2.0 - 1.0
             d = u * jitter * (h if np.ndim(h) == 0 else h)
-            xyz[:, ax] = np.where(interior, xyz[:, ax] + d, xyz[:, ax])
+            mov = interior & ~flat if ax == 2 else interior
+            xyz[:, ax] = np.where(mov, xyz[:, ax] + d, xyz[:, ax])
 
     def nid(ii, jj, kk):
         return ii + (nx + 1) * (jj + (ny + 1) * kk)
@@ -177,12 +183,12 @@ def box_mesh(a: float, b: float, L: float, nx: int, ny: int, nz: int, *, seed: i
     # y = 0 and y = b: u = x, v = z
     fk, fi = np.meshgrid(np.arange(nz), np.arange(nx), indexing="ij")
     fi = fi.ravel(); fk = fk.ravel()
-    for jj, tag, flip in ((0, side["y0"], False), (ny, side["y1"], True)):
+    for jj, tag, flip in ((0, side["y0"], False), (ny, side["y1"], not periodic_faces)):
         add_face(nid(fi, jj, fk), nid(fi + 1, jj, fk), nid(fi + 1, jj, fk + 1), nid(fi, jj, fk + 1), tag, flip)
     # x = 0 and x = a: u = y, v = z
     fk, fj = np.meshgrid(np.arange(nz), np.arange(ny), indexing="ij")
     fj = fj.ravel(); fk = fk.ravel()
-    for ii, tag, flip in ((0, side["x0"], True), (nx, side["x1"], False)):
+    for ii, tag, flip in ((0, side["x0"], True), (nx, side["x1"], periodic_faces)):
         add_face(nid(ii, fj, fk), nid(ii, fj + 1, fk), nid(ii, fj + 1, fk + 1), nid(ii, fj, fk + 1), tag, flip)
     tris = np.concatenate(tris).astype(np.int32)
     tri_phys = np.concatenate(phys)
@@ -239,3 +245,41 @@ def slab_eps(mesh: BoxMesh, z0=30e-3, z1=70e-3, lo=1.0, hi=4.0, tan_d=0.002, see
     eps = np.ones(T, dtype=np.complex128)
     eps[inside] = epsr[inside] * (1.0 - 1j * tan_d)
     return eps
+
+
+# Tags of the generated unit cells: periodic faces x0/x1/y0/y1, ports z=0 (2) and z=top
+# (3), PEC patch 1, substrate 101, air 100.
+CELL_TAGS = dict(x0=10, x1=11, y0=12, y1=13, port_bottom=2, port_top=3, patch=1, substrate=101, air=100)
+
+
+def unit_cell(period: float = 5e-3, n: int = 10, nz: int = 12, h_sub: float = 0.5e-3, h_air: float = 10e-3,
+              patch=None, seed: int = 0, jitter: float = 0.15) -> BoxMesh:
+    """Periodic unit cell: n x n x nz hexes over period x period x (h_sub + h_air); the
+    substrate takes n_sub = max(2, round(nz h_sub / (h_sub + h_air))) hex layers; the
+    optional PEC patch (w, l), centred, is snapped to grid lines on the substrate top
+    z = h_sub (a flat plane) and written as tagged triangles (tag 1)."""
+    t = CELL_TAGS
+    n_sub = max(2, int(round(nz * h_sub / (h_sub + h_air))))
+    zp = np.concatenate([np.linspace(0.0, h_sub, n_sub + 1), np.linspace(h_sub, h_sub + h_air, nz - n_sub + 1)[1:]])
+    g = box_mesh(period, period, h_sub + h_air, n, n, nz, seed=seed, jitter=jitter, zplanes=zp,
+                 tags=(t["patch"], t["port_bottom"], t["port_top"], t["air"]),
+                 side_tags={"x0": t["x0"], "x1": t["x1"], "y0": t["y0"], "y1": t["y1"]}, zflat=(n_sub,),
+                 periodic_faces=True)
+    hexk = (np.arange(g.tets.shape[0]) // 6) // (n * n)
+    g.tet_phys = np.where(hexk < n_sub, t["substrate"], t["air"]).astype(np.int32)
+    if patch is not None:
+        w, l = patch
+        h = period / n
+        i0 = int(round((period - w) / 2 / h))
+        j0 = int(round((period - l) / 2 / h))
+        i1 = i0 + max(1, int(round(w / h)))
+        j1 = j0 + max(1, int(round(l / h)))
+        fj, fi = np.meshgrid(np.arange(j0, min(j1, n)), np.arange(i0, min(i1, n)), indexing="ij")
+        fi = fi.ravel(); fj = fj.ravel()
+        nid = lambda ii, jj, kk: ii + (n + 1) * (jj + (n + 1) * kk)
+        c00, c10 = nid(fi, fj, n_sub), nid(fi + 1, fj, n_sub)
+        c11, c01 = nid(fi + 1, fj + 1, n_sub), nid(fi, fj + 1, n_sub)
+        pt = np.stack([np.stack([c00, c10, c11], -1), np.stack([c00, c11, c01], -1)], 1).reshape(-1, 3)
+        g.tris = np.concatenate([g.tris, pt.astype(np.int32)])
+        g.tri_phys = np.concatenate([g.tri_phys, np.full(pt.shape[0], t["patch"], np.int32)])
+    return g

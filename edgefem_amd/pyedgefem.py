@@ -126,6 +126,14 @@ class Mesh:
         self._dev_xyz = None
 
     # reference-style accessors ------------------------------------------------
+
+    def _node_pos(self):
+        """Map from gmsh node IDs to positions in the node arrays."""
+        if self.__dict__.get("_npos") is None:
+            order = np.argsort(self.node_ids, kind="stable")
+            self._npos = (order, self.node_ids[order])
+        order, sid = self._npos
+        return lambda ids: order[np.searchsorted(sid, np.asarray(ids, np.int64))]
     def num_nodes(self):
         return int(self.xyz.shape[0])
 
@@ -946,6 +954,319 @@ def normalize_port_weights(mesh: Mesh, params: MaxwellParams, bc: BC, ports: Lis
     return list(ports)
 
 
+class SweepResult:
+    def __init__(self):
+        self.frequencies: List[float] = []
+        self.S_matrices: List[np.ndarray] = []
+
+
+def frequency_sweep(mesh: Mesh, params: MaxwellParams, bc: BC, ports: Sequence[WavePort], frequencies,
+                    solver_options: Optional[SolveOptions] = None) -> SweepResult:
+    """frequency_sweep (sweep.cpp:174-329, K/M branch) on the GPU: K and M assembled once;
+    per frequency A = K - k0^2 M (efm_combine) + j k0 on the ABC edges, with the port
+    loading Y = sum w w^H / Z0 applied as the solver's low-rank term (set once); all
+    excitations b_i = 2 w_i / sqrt(Z0_i) solved as one block; S as in calculate_sparams,
+    NaN columns where a solve did not converge.  As in the reference, this branch has no
+    port ABC diagonals."""
+    import torch
+    params._check_supported()
+    res = SweepResult()
+    freqs = [float(f) for f in np.asarray(frequencies, np.float64).ravel()]
+    res.frequencies = freqs
+    P = len(ports)
+    if P == 0 or not freqs:
+        return res
+    opts = solver_options or SolveOptions()
+    eps, mu = tet_materials(mesh, params)
+    dmask = _dirichlet_mask(mesh, bc)
+    topo = mesh.topology
+    n = mesh.num_edges()
+    K, M = _capi.assemble_km(topo, mesh.dev_xyz, eps, 1.0 / mu, dmask)
+    frees = [_free_port(p, dmask) for p in ports]
+    lowrank = [(frees[i][0], frees[i][1], 1.0 / complex(p.mode.Z0)) for i, p in enumerate(ports)
+               if _valid_port(p) and frees[i][0].size]
+    sym = all(np.abs(np.asarray(w).imag).max(initial=0.0) == 0 for _, w, _ in lowrank)
+    abc_rows = np.zeros(0, np.int32)
+    if params.use_abc:
+        sel = np.ones(mesh.num_tris(), bool) if not params.abc_surface_tags else np.isin(
+            mesh.tri_phys, list(params.abc_surface_tags))
+        abc_rows = np.unique(mesh.tri_edges[sel].ravel()).astype(np.int32)
+        abc_rows = abc_rows[dmask[abc_rows] == 0]
+    R = _solver_nrhs(P)
+    Bh = np.zeros((n, R), np.complex128)
+    for i, port in enumerate(ports):
+        if _valid_port(port):
+            e, w = frees[i]
+            Bh[e, i] = 2.0 / cmath.sqrt(complex(port.mode.Z0)) * w
+    B = _capi.to_dev(Bh, torch.complex128)
+    L = max([f[0].size for f in frees] + [1])
+    gi = np.full((P, L), -1, np.int32)
+    gv = np.zeros((P, L), np.complex128)
+    for j, (e, w) in enumerate(frees):
+        gi[j, :e.size] = e
+        gv[j, :e.size] = w
+    gi_d, gv_d = _capi.to_dev(gi, torch.int32), _capi.to_dev(gv, torch.complex128)
+    solver = _capi.KrylovSolver(topo.row_ptr, topo.col_idx, nrhs=R)
+    solver.set_gradient(topo, dmask)
+    if lowrank:
+        solver.set_lowrank(*_pack_lowrank(lowrank))
+    tol = 1e-14 if opts.use_direct else float(opts.tolerance)
+    method = _capi.EFM_METHOD_COCG if sym else _capi.EFM_METHOD_BICGSTAB
+    for f in freqs:
+        k0 = 2 * math.pi * f / C0
+        A = _capi.combine(topo, K, M, k0 * k0)
+        if abc_rows.size:
+            _capi.add_diagonal(topo, abc_rows, np.full(abc_rows.size, 1j * k0), A)
+        solver.set_matrix(A)
+        X, info = solver.solve(B, tol=tol, max_iterations=int(opts.max_iterations), method=method)
+        V = _capi.port_overlap(gi_d, gv_d, X, R)[:, :P]
+        S = np.full((P, P), np.nan + 1j * np.nan, np.complex128)
+        for i, port in enumerate(ports):
+            if not (info["resid"][i] <= tol or info["iters"][i] == 0):
+                continue
+            vinc = cmath.sqrt(complex(port.mode.Z0))
+            for j in range(P):
+                S[j, i] = (V[j, i] - vinc) / vinc if i == j else V[j, i] / vinc
+        res.S_matrices.append(S)
+    return res
+
+
+class PeriodicPair:
+    def __init__(self, master_edge=0, slave_edge=0, master_orient=1, slave_orient=1, translation=(0.0, 0.0, 0.0)):
+        self.master_edge = int(master_edge)
+        self.slave_edge = int(slave_edge)
+        self.master_orient = int(master_orient)
+        self.slave_orient = int(slave_orient)
+        self.translation = np.asarray(translation, np.float64)
+
+
+class PeriodicBC:
+    def __init__(self):
+        self.pairs: List[PeriodicPair] = []
+        self.period_vector = np.zeros(3)
+        self.phase_shift = 1.0 + 0j
+
+
+def _edge_centroids(mesh: Mesh, edges: np.ndarray) -> np.ndarray:
+    nid = mesh._node_pos()
+    e = mesh.edge_nodes[edges]
+    return 0.5 * (mesh.xyz[nid(e[:, 0])] + mesh.xyz[nid(e[:, 1])])
+
+
+def build_periodic_pairs(mesh: Mesh, master_tag: int, slave_tag: int, period_vector,
+                         tolerance: float = 1e-9) -> PeriodicBC:
+    """Master/slave edge pairs whose centroids differ by the period (periodic.cpp:48-164):
+    lookup by rounded centroid, brute force when that misses; an unmatched master or a
+    slave matched twice raises.  The orientation of an edge on a face is its local sign
+    in the last triangle of that face (mesh order) containing it, +1 if none
+    (periodic.cpp:135-146).  Pairs are listed by ascending master edge (the reference
+    uses hash-set order; the elimination does not depend on it)."""
+    per = np.asarray(period_vector, np.float64)
+    me = np.unique(mesh.tri_edges[mesh.tri_phys == master_tag].ravel())
+    se = np.unique(mesh.tri_edges[mesh.tri_phys == slave_tag].ravel())
+    if me.size == 0:
+        raise RuntimeError(f"No edges found on master surface with tag {master_tag}")
+    if se.size == 0:
+        raise RuntimeError(f"No edges found on slave surface with tag {slave_tag}")
+    cm = _edge_centroids(mesh, me) + per
+    cs = _edge_centroids(mesh, se)
+    table = {tuple(k): i for i, k in enumerate(np.round(cs / tolerance).astype(np.int64))}
+    used = np.zeros(se.size, bool)
+    match = np.empty(me.size, np.int64)
+    for i, c in enumerate(cm):
+        k = table.get(tuple(np.round(c / tolerance).astype(np.int64)), -1)
+        if k < 0 or np.linalg.norm(cs[k] - c) >= tolerance:
+            hit = np.nonzero(np.linalg.norm(cs - c, axis=1) < tolerance)[0]
+            k = int(hit[0]) if hit.size else -1
+        if k < 0:
+            raise RuntimeError(f"Could not find matching slave edge for master edge {me[i]}")
+        if used[k]:
+            raise RuntimeError(f"Slave edge {se[k]} matched to multiple master edges")
+        used[k] = True
+        match[i] = k
+    slave = se[match]
+
+    def last_orient(tag, edges):
+        o = np.ones(edges.size, np.int64)
+        sel = np.nonzero(mesh.tri_phys == tag)[0]
+        te = mesh.tri_edges[sel].ravel()
+        ts = np.asarray(mesh.tri_sign[sel]).ravel()
+        pos = np.searchsorted(edges, te)
+        ok = (pos < edges.size) & (edges[np.minimum(pos, edges.size - 1)] == te)
+        o[pos[ok]] = ts[ok]          # later triangles overwrite earlier ones
+        return o
+    mo = last_orient(master_tag, me)
+    so_all = last_orient(slave_tag, se)
+    so = so_all[match]
+    pbc = PeriodicBC()
+    pbc.period_vector = per
+    pbc.pairs = [PeriodicPair(a, b, x, y, per) for a, b, x, y in zip(me, slave, mo, so)]
+    return pbc
+
+
+def set_floquet_phase(pbc: PeriodicBC, k_transverse) -> None:
+    """phase = exp(j (kx Lx + ky Ly)) (periodic.cpp:204-211)."""
+    k = np.asarray(k_transverse, np.float64)
+    arg = k[0] * pbc.period_vector[0] + k[1] * pbc.period_vector[1]
+    pbc.phase_shift = complex(math.cos(arg), math.sin(arg))
+
+
+def floquet_phase_from_angle(period_vector, theta: float, phi: float, k0: float) -> complex:
+    """exp(j k_inc . L), k_inc = k0 (sin t cos p, sin t sin p, -cos t) (periodic.cpp:213-225)."""
+    k = k0 * np.array([math.sin(theta) * math.cos(phi), math.sin(theta) * math.sin(phi), -math.cos(theta)])
+    arg = float(k @ np.asarray(period_vector, np.float64))
+    return complex(math.cos(arg), math.sin(arg))
+
+
+def validate_periodic_bc(mesh: Mesh, pbc: PeriodicBC) -> bool:
+    """periodic.cpp:166-202."""
+    if not pbc.pairs:
+        return False
+    n = mesh.num_edges()
+    ms, ss = set(), set()
+    for p in pbc.pairs:
+        if not (0 <= p.master_edge < n and 0 <= p.slave_edge < n):
+            return False
+        if p.master_orient not in (1, -1) or p.slave_orient not in (1, -1):
+            return False
+        if p.master_edge in ms or p.slave_edge in ss:
+            return False
+        ms.add(p.master_edge)
+        ss.add(p.slave_edge)
+    return True
+
+
+def count_surface_edges(mesh: Mesh, surface_tag: int) -> int:
+    return int(np.unique(mesh.tri_edges[mesh.tri_phys == surface_tag].ravel()).size)
+
+
+def _periodic_coefs(pbc: PeriodicBC, mode: str):
+    """(row, column, recovery) coefficients per pair.  "reference" is
+    assemble_maxwell.cpp:536-546 and 606-613 (rows phi o, columns conj(phi) o, x_s = phi o
+    x_m; the column operation implies x_s = conj(phi) o x_m, so for a complex phase the
+    recovered field does not satisfy the eliminated equations).  "consistent" uses rows
+    conj(phi) o and columns phi o, so the column operation matches the recovery."""
+    o = np.array([p.master_orient * p.slave_orient for p in pbc.pairs], np.float64)
+    ph = complex(pbc.phase_shift)
+    if mode == "reference":
+        return ph * o, np.conj(ph) * o, ph * o
+    if mode == "consistent":
+        return np.conj(ph) * o, ph * o, ph * o
+    raise ValueError(f"unknown periodic mode {mode!r}")
+
+
+def assemble_maxwell_periodic(mesh: Mesh, params: MaxwellParams, bc: BC, pbc: PeriodicBC,
+                              ports: Sequence[WavePort] = (), active_port_idx: int = -1, *,
+                              mode: str = "reference") -> MaxwellAssembly:
+    """assemble_maxwell_periodic (assemble_maxwell.cpp:496-574) on the GPU: the assembled
+    matrix is reduced by the sparse elimination of efm_periodic_reduce (the reference
+    copies it to a dense matrix); the port loading stays an operator term with its
+    vectors transformed the same way (u = Z E w, which equals Z F^H w in both
+    conventions); b' = Z E b."""
+    params._check_supported()
+    A, lowrank, dmask = _assemble_with_ports(mesh, params, bc, ports)
+    topo = mesh.topology
+    n = mesh.num_edges()
+    b = np.zeros(n, np.complex128)
+    if 0 <= active_port_idx < len(ports) and _valid_port(ports[active_port_idx]):
+        port = ports[active_port_idx]
+        e, w = _free_port(port, dmask)
+        np.add.at(b, e, 2.0 / cmath.sqrt(complex(port.mode.Z0)) * w)
+    if not pbc.pairs:
+        sym = all(np.abs(np.asarray(w).imag).max(initial=0.0) == 0 for _, w, _ in lowrank)
+        return MaxwellAssembly(SpMatC(topo.row_ptr, topo.col_idx, A, symmetric=sym, lowrank=lowrank), VecC(b))
+    a, c, _ = _periodic_coefs(pbc, mode)
+    m = np.array([p.master_edge for p in pbc.pairs], np.int64)
+    s_ = np.array([p.slave_edge for p in pbc.pairs], np.int64)
+    rp, ci, v = _capi.periodic_reduce(topo.row_ptr, topo.col_idx, A, m, s_, a, c, dmask)
+    act = (dmask[m] == 0) & (dmask[s_] == 0)
+    zero = np.zeros(n, bool)
+    zero[s_[dmask[s_] == 0]] = True
+
+    def transform(vec, coef):
+        out = vec.copy()
+        np.add.at(out, m[act], coef[act] * vec[s_[act]])
+        out[zero] = 0
+        return out
+    new_lr = []
+    for e, w, cf in lowrank:
+        full = np.zeros(n, np.complex128)
+        full[e] = w
+        u = transform(full, a)
+        vv = transform(full, np.conj(c))
+        if np.abs(u - vv).max() > 1e-14 * max(np.abs(u).max(), 1e-300):
+            raise NotImplementedError("left and right port vectors differ after the periodic elimination")
+        idx = np.nonzero(u)[0]
+        if idx.size:
+            new_lr.append((idx, u[idx], cf))
+    bp = transform(b, a)
+    return MaxwellAssembly(SpMatC(rp, ci, v, symmetric=False, lowrank=new_lr), VecC(bp))
+
+
+def calculate_sparams_periodic(mesh: Mesh, params: MaxwellParams, bc: BC, pbc: PeriodicBC,
+                               ports: Sequence[WavePort], *, mode: str = "reference",
+                               solver_options: Optional[SolveOptions] = None,
+                               _state: Optional[dict] = None) -> np.ndarray:
+    """calculate_sparams_periodic (assemble_maxwell.cpp:576-635) on the GPU: the reduced
+    system is shared by the excitations and solved as one block by BiCGSTAB (a complex
+    phase breaks the symmetry) with the port loading as a low-rank operator term; slave
+    values are recovered as x_s = phase o x_m for every pair; S as in calculate_sparams.
+    ``mode`` selects the reference's coefficients or the consistent ones."""
+    import torch
+    P = len(ports)
+    if P == 0:
+        return np.zeros((0, 0), np.complex128)
+    opts = solver_options or SolveOptions()
+    asm = assemble_maxwell_periodic(mesh, params, bc, pbc, ports, -1, mode=mode)
+    Am = asm.A
+    dmask = _dirichlet_mask(mesh, bc)
+    n = mesh.num_edges()
+    R = _solver_nrhs(P)
+    a, c, rec = _periodic_coefs(pbc, mode)
+    m = np.array([p.master_edge for p in pbc.pairs], np.int64)
+    s_ = np.array([p.slave_edge for p in pbc.pairs], np.int64)
+    act = (dmask[m] == 0) & (dmask[s_] == 0) if m.size else np.zeros(0, bool)
+    zero = np.zeros(n, bool)
+    if m.size:
+        zero[s_[dmask[s_] == 0]] = True
+    frees = [_free_port(p, dmask) for p in ports]
+    Bh = np.zeros((n, R), np.complex128)
+    for i, port in enumerate(ports):
+        if _valid_port(port):
+            e, w = frees[i]
+            col = np.zeros(n, np.complex128)
+            np.add.at(col, e, 2.0 / cmath.sqrt(complex(port.mode.Z0)) * w)
+            if m.size:
+                np.add.at(col, m[act], a[act] * col[s_[act]])
+                col[zero] = 0
+            Bh[:, i] = col
+    solver = _capi.KrylovSolver(Am.row_ptr, Am.col_idx, nrhs=R)
+    if Am.lowrank:
+        solver.set_lowrank(*_pack_lowrank(Am.lowrank))
+    solver.set_matrix(Am.values)
+    tol = 1e-14 if opts.use_direct else float(opts.tolerance)
+    X, info = solver.solve(_capi.to_dev(Bh, torch.complex128), tol=tol, max_iterations=int(opts.max_iterations),
+                           method=_capi.EFM_METHOD_BICGSTAB)
+    Xh = X.reshape(n, R).cpu().numpy().copy()
+    if m.size:
+        Xh[s_, :] = rec[:, None] * Xh[m, :]
+    S = np.full((P, P), np.nan + 1j * np.nan, np.complex128)
+    for i, port in enumerate(ports):
+        if not (info["resid"][i] <= tol or info["iters"][i] == 0):
+            sys.stderr.write(f"WARNING: Solver did not converge in calculate_sparams_periodic (active port {i}) "
+                             f"(method=BiCGSTAB+Jacobi, iters={info['iters'][i]}, residual={info['resid'][i]}). "
+                             "S-parameters may be unreliable.\n")
+            continue
+        vinc = cmath.sqrt(complex(port.mode.Z0))
+        for j in range(P):
+            e, w = frees[j]
+            V = np.sum(np.conj(w) * Xh[e, i])
+            S[j, i] = (V - vinc) / vinc if i == j else V / vinc
+    if _state is not None:
+        _state.update(dict(A=Am, info=info, X=Xh))
+    return S
+
+
 class LumpedPortWeightMode(enum.IntEnum):
     Projection = 0
     SurfaceIntegral = 1
@@ -987,9 +1308,9 @@ def build_lumped_port(mesh: Mesh, config: LumpedPortConfig) -> WavePort:
         pos = np.searchsorted(edges, mesh.tri_edges[sel])
         np.add.at(w, pos.ravel(), val.ravel())
     else:
-        nid = {int(v): k for k, v in enumerate(mesh.node_ids)}
-        p0 = mesh.xyz[[nid[int(x)] for x in mesh.edge_nodes[edges, 0]]]
-        p1 = mesh.xyz[[nid[int(x)] for x in mesh.edge_nodes[edges, 1]]]
+        nid = mesh._node_pos()
+        p0 = mesh.xyz[nid(mesh.edge_nodes[edges, 0])]
+        p1 = mesh.xyz[nid(mesh.edge_nodes[edges, 1])]
         w = ((p1 - p0) @ ed).astype(np.complex128)
     nw = np.linalg.norm(w)
     if nw > 1e-15:

@@ -238,6 +238,25 @@ int efm_sweep_solve(efm_sweep *sw, int nsys, const double *h_k0sq, const double 
                     void *stream);
 
 /* ------------------------------------------------------------------------
+ * Periodic (Floquet) elimination on the sparse matrix: replaces the dense elimination of
+ * assemble_maxwell_periodic (src/assemble_maxwell.cpp:496-574).  For every pair
+ * k = (h_master[k], h_slave[k]) with neither edge PEC: row m += h_coef_row[k] * row s,
+ * column m += h_coef_col[k] * column s (c128 coefficients: the reference uses
+ * phase*orient and conj(phase)*orient); then the rows and columns of the non-PEC
+ * slaves become unit rows/columns.  Inputs: CSR (row_ptr[n+1], col[nnz], val c128) on
+ * the device; pairs, coefficients and h_dirichlet[n] (may be NULL) on the host.
+ * Output: row_ptr_out[n+1] (device, always written) and *h_nnz; when col_out and
+ * val_out are given (capacity cap >= *h_nnz) the reduced columns (sorted per row,
+ * duplicates summed) and values are written too -- call once to size, once to fill.
+ * Exact zeros stay in the pattern (the reference drops them).  Synchronises the stream.
+ * ---------------------------------------------------------------------- */
+int efm_periodic_reduce(int64_t n, const int32_t *row_ptr, const int32_t *col, const double *val,
+                        int64_t n_pairs, const int32_t *h_master, const int32_t *h_slave,
+                        const double *h_coef_row, const double *h_coef_col,
+                        const uint8_t *h_dirichlet, int32_t *row_ptr_out, int32_t *col_out,
+                        double *val_out, int64_t cap, int64_t *h_nnz, void *stream);
+
+/* ------------------------------------------------------------------------
  * Adjoint gradient with respect to the per-tet permittivity (no reference counterpart;
  * it differentiates calculate_sparams_eigenmode, src/assemble_maxwell.cpp:637-787).
  * For every design tet e = design[d] (design NULL: all tets, n_design == n_tets):

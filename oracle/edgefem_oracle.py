@@ -18,6 +18,12 @@ build_wave_port_2d          src/ports/wave_port.cpp:516-546
 solve_te10_mode             src/ports/port_eigensolve.cpp:45-63
 sparams_eigenmode           src/assemble_maxwell.cpp:637-787
 calculate_sparams           src/assemble_maxwell.cpp:352-394 (port loading 209-242)
+frequency_sweep             src/sweep.cpp:174-329 (K/M reuse branch)
+build_periodic_pairs        src/periodic.cpp:48-164
+floquet_phase_from_angle    src/periodic.cpp:204-225 (and set_floquet_phase)
+assemble_maxwell_periodic   src/assemble_maxwell.cpp:496-574 (sparse restatement;
+                            periodic_eliminate_dense is the literal dense one)
+calculate_sparams_periodic  src/assemble_maxwell.cpp:576-635
 port_abc_coefficient        src/assemble_maxwell.cpp:274-320
 build_lumped_port           src/ports/lumped_port.cpp:21-160
 normalize_port_weights      src/assemble_maxwell.cpp:396-494
@@ -637,6 +643,205 @@ def calculate_sparams(mesh: OMesh, p: Params, dirichlet: np.ndarray, ports: Sequ
             keep = ~dirichlet[ports[j].edges]
             V = np.sum(np.conj(ports[j].weights[keep]) * x[ports[j].edges[keep]])
             S[j, i] = (V - vinc) / vinc if i == j else V / vinc
+    return S
+
+
+def frequency_sweep(mesh: OMesh, p: Params, dirichlet: np.ndarray, ports: Sequence[OPort], freqs):
+    """K/M branch of frequency_sweep (sweep.cpp:174-329): K and M assembled once; per
+    frequency A = K - k0^2 M + Y_port (+ j k0 on the ABC edges), Y_port = sum w w^H / Z0;
+    b = 2 w / sqrt(Z0) per active port; S as in calculate_sparams; FP64 LU per frequency
+    (port ABC diagonals are not part of this branch, as in the reference)."""
+    K, M = assemble_maxwell_km(mesh, p, dirichlet)
+    n = mesh.n_edges
+    R, C, V = [], [], []
+    for port in ports:
+        if port.Z0 == 0 or len(port.weights) != len(port.edges):
+            continue
+        e = np.asarray(port.edges)
+        w = np.asarray(port.weights, np.complex128)
+        keep = ~dirichlet[e]
+        e, w = e[keep], w[keep]
+        R.append(np.repeat(e, e.size)); C.append(np.tile(e, e.size))
+        V.append((w[:, None] * np.conj(w)[None, :] / port.Z0).ravel())
+    Y = sp.coo_matrix((np.concatenate(V) if V else np.zeros(0), (np.concatenate(R) if R else np.zeros(0, int),
+                       np.concatenate(C) if C else np.zeros(0, int))), shape=(n, n)).tocsr()
+    abc = np.zeros(0, int)
+    if p.use_abc:
+        sel = np.ones(mesh.tri_ids.shape[0], bool) if not p.abc_surface_tags else np.isin(mesh.tri_phys, list(p.abc_surface_tags))
+        abc = np.unique(mesh.tri_edges[sel].ravel())
+        abc = abc[~dirichlet[abc]]
+    out = []
+    P = len(ports)
+    for f in np.asarray(freqs, float):
+        k0 = 2 * math.pi * f / C0
+        A = (K - (k0 * k0) * M + Y).tocsc()
+        if abc.size:
+            A = (A + sp.csc_matrix((np.full(abc.size, 1j * k0), (abc, abc)), shape=(n, n))).tocsc()
+        lu = spla.splu(A, permc_spec="MMD_AT_PLUS_A")
+        S = np.empty((P, P), np.complex128)
+        for i, port in enumerate(ports):
+            b = np.zeros(n, np.complex128)
+            e = np.asarray(port.edges)
+            keep = ~dirichlet[e]
+            b[e[keep]] += 2.0 / np.sqrt(complex(port.Z0)) * np.asarray(port.weights)[keep]
+            x = lu.solve(b)
+            vinc = np.sqrt(complex(port.Z0))
+            for j in range(P):
+                kj = ~dirichlet[ports[j].edges]
+                Vj = np.sum(np.conj(np.asarray(ports[j].weights)[kj]) * x[np.asarray(ports[j].edges)[kj]])
+                S[j, i] = (Vj - vinc) / vinc if i == j else Vj / vinc
+        out.append(S)
+    return np.array(out)
+
+
+@dataclasses.dataclass
+class OPeriodic:
+    master: np.ndarray           # edge indices
+    slave: np.ndarray
+    orient: np.ndarray           # master_orient * slave_orient
+    period: np.ndarray
+    phase: complex = 1.0 + 0j
+
+
+def _edge_centroids(mesh: OMesh, edges: np.ndarray) -> np.ndarray:
+    i0 = np.array([mesh.node_index[int(v)] for v in mesh.edges[edges, 0]], np.int64)
+    i1 = np.array([mesh.node_index[int(v)] for v in mesh.edges[edges, 1]], np.int64)
+    return 0.5 * (mesh.xyz[i0] + mesh.xyz[i1])
+
+
+def build_periodic_pairs(mesh: OMesh, master_tag: int, slave_tag: int, period, tol: float = 1e-9) -> OPeriodic:
+    """Pairs of edges on two faces whose centroids differ by the period (periodic.cpp:48-164):
+    lookup by rounded centroid, brute force when that misses, error on an unmatched master
+    or a slave matched twice.  Orientation of an edge on a face = its local sign in the
+    LAST tri of that face (mesh order) containing it, +1 if none (periodic.cpp:135-146).
+    Pairs are listed in ascending master edge (the reference lists them in hash-set
+    order; the elimination does not depend on the order)."""
+    period = np.asarray(period, float)
+    me = np.unique(mesh.tri_edges[mesh.tri_phys == master_tag].ravel())
+    se = np.unique(mesh.tri_edges[mesh.tri_phys == slave_tag].ravel())
+    if me.size == 0 or se.size == 0:
+        raise ValueError("no edges on the master or slave surface")
+    cm = _edge_centroids(mesh, me) + period
+    cs = _edge_centroids(mesh, se)
+    key = lambda c: tuple(np.round(c / tol).astype(np.int64))
+    table = {key(c): k for k, c in enumerate(cs)}
+    match = np.empty(me.size, np.int64)
+    used = set()
+    for i, c in enumerate(cm):
+        k = table.get(key(c), -1)
+        if k < 0 or np.linalg.norm(cs[k] - c) >= tol:
+            d = np.linalg.norm(cs - c, axis=1)
+            hit = np.nonzero(d < tol)[0]
+            k = int(hit[0]) if hit.size else -1
+        if k < 0:
+            raise ValueError(f"no slave edge for master edge {me[i]}")
+        if k in used:
+            raise ValueError(f"slave edge {se[k]} matched twice")
+        used.add(k)
+        match[i] = k
+    slave = se[match]
+
+    def last_orient(tag, edges):
+        o = np.ones(edges.size, np.int64)
+        pos = {int(e): i for i, e in enumerate(edges)}
+        for f in np.nonzero(mesh.tri_phys == tag)[0]:
+            for le in range(3):
+                i = pos.get(int(mesh.tri_edges[f, le]), -1)
+                if i >= 0:
+                    o[i] = mesh.tri_sign[f, le]
+        return o
+    orient = last_orient(master_tag, me) * last_orient(slave_tag, slave)
+    return OPeriodic(master=me, slave=slave, orient=orient, period=period)
+
+
+def floquet_phase_from_angle(period, theta: float, phi: float, k0: float) -> complex:
+    """exp(j k_inc . L), k_inc = k0 (sin t cos p, sin t sin p, -cos t) (periodic.cpp:213-225)."""
+    k = k0 * np.array([math.sin(theta) * math.cos(phi), math.sin(theta) * math.sin(phi), -math.cos(theta)])
+    return complex(np.exp(1j * float(k @ np.asarray(period, float))))
+
+
+def set_floquet_phase(pbc: OPeriodic, k_t) -> None:
+    """phase = exp(j (kx Lx + ky Ly)) (periodic.cpp:204-211)."""
+    pbc.phase = complex(np.exp(1j * (k_t[0] * pbc.period[0] + k_t[1] * pbc.period[1])))
+
+
+def _periodic_coefs(pbc: OPeriodic, mode: str):
+    """(row, column, recovery) coefficient per pair.  "reference": rows += phi o R_s, cols
+    += conj(phi) o C_s, x_s = phi o x_m (assemble_maxwell.cpp:536-546, 606-613).
+    "consistent": rows += conj(phi) o, cols += phi o, x_s = phi o x_m, so the column
+    operation matches the recovery ansatz."""
+    o = pbc.orient.astype(float)
+    ph = complex(pbc.phase)
+    if mode == "reference":
+        return ph * o, np.conj(ph) * o, ph * o
+    if mode == "consistent":
+        return np.conj(ph) * o, ph * o, ph * o
+    raise ValueError(f"unknown periodic mode {mode!r}")
+
+
+def periodic_eliminate(A: sp.csr_matrix, b: np.ndarray, pbc: OPeriodic, dirichlet: np.ndarray,
+                       mode: str = "reference"):
+    """Sparse restatement of the elimination (assemble_maxwell.cpp:496-574):
+    A' = Z (E A F) Z + I_S, b' = Z E b, E = I + sum a_k e_m e_s^T, F = I + sum c_k e_s e_m^T
+    over pairs with neither edge PEC, Z zeroing the non-PEC slaves S.  Exact zeros stay
+    in the pattern (the reference's sparseView drops them)."""
+    n = A.shape[0]
+    a, c, _ = _periodic_coefs(pbc, mode)
+    m, s = pbc.master, pbc.slave
+    act = ~(dirichlet[m] | dirichlet[s])
+    E = sp.identity(n, complex, format="csr") + sp.csr_matrix((a[act], (m[act], s[act])), shape=(n, n))
+    F = sp.identity(n, complex, format="csr") + sp.csr_matrix((c[act], (s[act], m[act])), shape=(n, n))
+    S = s[~dirichlet[s]]
+    keep = np.ones(n)
+    keep[S] = 0.0
+    Z = sp.diags(keep)
+    Ap = (Z @ (E @ A @ F) @ Z).tocsr() + sp.csr_matrix((np.ones(S.size), (S, S)), shape=(n, n))
+    Ap.sort_indices()
+    bp = keep * (E @ b)
+    return Ap.tocsr(), bp
+
+
+def periodic_eliminate_dense(A: sp.csr_matrix, b: np.ndarray, pbc: OPeriodic, dirichlet: np.ndarray,
+                             mode: str = "reference"):
+    """Literal dense restatement: sequential row/column updates per pair, then zero slave
+    rows/columns with unit diagonal (assemble_maxwell.cpp:526-571).  Small cells only."""
+    D = A.toarray().astype(np.complex128)
+    bb = np.array(b, np.complex128)
+    a, c, _ = _periodic_coefs(pbc, mode)
+    for k in range(pbc.master.size):
+        mm, ss = int(pbc.master[k]), int(pbc.slave[k])
+        if dirichlet[mm] or dirichlet[ss]:
+            continue
+        D[mm, :] += a[k] * D[ss, :]
+        D[:, mm] += c[k] * D[:, ss]
+        bb[mm] += a[k] * bb[ss]
+    for ss in pbc.slave:
+        if dirichlet[ss]:
+            continue
+        D[ss, :] = 0
+        D[:, ss] = 0
+        D[ss, ss] = 1
+        bb[ss] = 0
+    return D, bb
+
+
+def calculate_sparams_periodic(mesh: OMesh, p: Params, dirichlet: np.ndarray, pbc: OPeriodic,
+                               ports: Sequence[OPort], mode: str = "reference"):
+    """assemble_maxwell_periodic + solve + slave recovery x_s = r_k x_m + the rank-1 S
+    extraction (assemble_maxwell.cpp:576-635), FP64 LU."""
+    P = len(ports)
+    S = np.empty((P, P), np.complex128)
+    _, _, rec = _periodic_coefs(pbc, mode)
+    for i in range(P):
+        A, b = assemble_maxwell(mesh, p, dirichlet, ports, i)
+        Ap, bp = periodic_eliminate(A, b, pbc, dirichlet, mode)
+        x = spla.splu(Ap.tocsc(), permc_spec="MMD_AT_PLUS_A").solve(bp)
+        x[pbc.slave] = rec * x[pbc.master]
+        vinc = np.sqrt(complex(ports[i].Z0))
+        for j in range(P):
+            kj = ~dirichlet[ports[j].edges]
+            Vj = np.sum(np.conj(np.asarray(ports[j].weights)[kj]) * x[np.asarray(ports[j].edges)[kj]])
+            S[j, i] = (Vj - vinc) / vinc if i == j else Vj / vinc
     return S
 
 

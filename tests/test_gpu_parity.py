@@ -590,3 +590,85 @@ def test_assemble_maxwell_with_ports_entries(torch_dev):
     np.testing.assert_allclose(asm.b.to_numpy(), bo, rtol=0, atol=1e-14)
     e = int(oports[1].edges[~pec[oports[1].edges]][0])
     assert abs(asm.A.coeff(e, e) - Ao[e, e]) <= 1e-12 * abs(Ao[e, e])
+
+
+def test_frequency_sweep_rank1_vs_oracle(torch_dev):
+    """frequency_sweep (K/M reuse, sweep.cpp:174-329) on the GPU with lumped ports and the
+    j k0 ABC on the port faces, against the oracle's K/M sweep (SuperLU per frequency)."""
+    from edgefem_amd import pyedgefem as em
+    from oracle import edgefem_oracle as O
+    _p("[gpu] frequency_sweep rank-1")
+    pm = em.load_gmsh(golden("rect_waveguide.msh"))
+    om = _oracle_mesh_from_file("rect_waveguide.msh")
+    bc = em.build_edge_pec(pm, 1)
+    pec = O.build_edge_pec(om, 1)
+    gports, oports = _lumped_pair(pm, om, pec, bc)
+    params = em.MaxwellParams()
+    params.use_abc = True
+    params.abc_surface_tags = {2, 3}
+    freqs = [8e9, 10e9, 12e9]
+    res = em.frequency_sweep(pm, params, bc, gports, freqs)
+    So = O.frequency_sweep(om, O.Params(use_abc=True, abc_surface_tags={2, 3}), pec, oports, freqs)
+    assert res.frequencies == freqs
+    err = np.abs(np.array(res.S_matrices) - So).max()
+    _p(f"    max |dS| {err:.2e}")
+    assert err <= 1e-6
+
+
+def _cell_setup(n=4, nz=6, seed=3):
+    from edgefem_amd import meshgen as mg
+    from edgefem_amd import pyedgefem as em
+    from oracle import edgefem_oracle as O
+    g = mg.unit_cell(n=n, nz=nz, patch=(2e-3, 3e-3), seed=seed)
+    pm = em.mesh_from_arrays(g.node_ids, g.xyz, g.tets, g.tet_phys, g.tris, g.tri_phys)
+    om = O.mesh_from_arrays(g.node_ids, g.xyz, g.tet_ids, g.tet_phys, g.tri_ids, g.tri_phys)
+    return g, pm, om
+
+
+@pytest.mark.parametrize("mode", ["reference", "consistent"])
+def test_periodic_cell_sparams_vs_oracle(torch_dev, mode):
+    """A unit cell (substrate eps 3.5(1 - 0.002j), PEC patch, periodic along x with a 25
+    degree Floquet phase, lumped ports top and bottom): pairs and orientations, the
+    reduced matrix from efm_periodic_reduce (every entry, against the oracle's sparse
+    elimination) and the S matrix from BiCGSTAB with the port loading as a low-rank term,
+    against the oracle (SuperLU), in both coefficient conventions."""
+    from edgefem_amd import pyedgefem as em
+    from oracle import edgefem_oracle as O
+    _p(f"[gpu] periodic unit cell, mode={mode}")
+    g, pm, om = _cell_setup()
+    bc = em.build_edge_pec(pm, 1)
+    pec = O.build_edge_pec(om, 1)
+    pbc = em.build_periodic_pairs(pm, 10, 11, np.array([5e-3, 0.0, 0.0]))
+    opbc = O.build_periodic_pairs(om, 10, 11, (5e-3, 0, 0))
+    np.testing.assert_array_equal([p.master_edge for p in pbc.pairs], opbc.master)
+    np.testing.assert_array_equal([p.slave_edge for p in pbc.pairs], opbc.slave)
+    np.testing.assert_array_equal([p.master_orient * p.slave_orient for p in pbc.pairs], opbc.orient)
+    f = 10e9
+    k0 = 2 * math.pi * f / em.C0
+    em.set_floquet_phase(pbc, np.array([k0 * math.sin(math.radians(25)), 0.0]))
+    opbc.phase = pbc.phase_shift
+    ports, oports = [], []
+    for tag in (2, 3):
+        cfg = em.LumpedPortConfig()
+        cfg.surface_tag = tag
+        cfg.z0 = 376.73
+        cfg.e_direction = np.array([0.0, 1.0, 0.0])
+        ports.append(em.build_lumped_port(pm, cfg))
+        oports.append(O.build_lumped_port(om, tag, 376.73, (0, 1, 0)))
+    params = em.MaxwellParams()
+    params.omega = 2 * math.pi * f
+    params.eps_r_regions = {101: 3.5 * (1 - 0.002j)}
+    op = O.Params(omega=params.omega, eps_r_regions={101: 3.5 * (1 - 0.002j)})
+    asm = em.assemble_maxwell_periodic(pm, params, bc, pbc, ports, 0, mode=mode)
+    A0, b0 = O.assemble_maxwell(om, op, pec, oports, 0)
+    Ao, bo = O.periodic_eliminate(A0, b0, opbc, pec, mode)
+    Ag = asm.A.to_dense()
+    Ad = Ao.toarray()
+    scale = np.sqrt(np.abs(np.diag(Ad))[:, None] * np.abs(np.diag(Ad))[None, :]) + 1e-300
+    assert (np.abs(Ag - Ad) / scale).max() <= 1e-12
+    np.testing.assert_allclose(asm.b.to_numpy(), bo, rtol=0, atol=1e-14)
+    st = {}
+    S = em.calculate_sparams_periodic(pm, params, bc, pbc, ports, mode=mode, _state=st)
+    So = O.calculate_sparams_periodic(om, op, pec, opbc, oports, mode=mode)
+    _p(f"    S {S.ravel()} iterations {st['info']['iters']}")
+    assert np.abs(S - So).max() <= 1e-6

@@ -522,3 +522,109 @@ def test_port_abc_diagonal_terms(kind):
     # below cutoff: no term
     p.omega = p0.omega = 2 * math.pi * 5e9
     assert abs(O.assemble_maxwell(m, p, pec, ports, 0)[0] - O.assemble_maxwell(m, p0, pec, ports, 0)[0]).max() == 0
+
+
+def test_frequency_sweep_km_equals_per_frequency_assembly():
+    """The K/M branch of frequency_sweep (sweep.cpp:174-329) against calculate_sparams with
+    a full assembly at each frequency (assemble_maxwell.cpp:352-394): A(omega) = K - k0^2 M
+    + Y_port + j k0 ABC must give the same S, with the ABC on."""
+    m = O.load_gmsh_v2(golden("rect_waveguide.msh"))
+    pec = O.build_edge_pec(m, 1)
+    ports = [O.build_lumped_port(m, t, 50.0, (0, 1, 0)) for t in (2, 3)]
+    p = O.Params(omega=0.0, use_abc=True, abc_surface_tags={2, 3})
+    freqs = [8e9, 10e9, 12e9]
+    Ssw = O.frequency_sweep(m, p, pec, ports, freqs)
+    for f, S in zip(freqs, Ssw):
+        pf = O.Params(omega=2 * math.pi * f, use_abc=True, abc_surface_tags={2, 3})
+        np.testing.assert_allclose(S, O.calculate_sparams(m, pf, pec, ports), rtol=1e-9, atol=1e-12)
+
+
+def _cell(n=3, nz=4, patch=(2e-3, 3e-3), seed=3):
+    g = mg.unit_cell(n=n, nz=nz, patch=patch, seed=seed)
+    m = O.mesh_from_arrays(g.node_ids, g.xyz, g.tet_ids, g.tet_phys, g.tri_ids, g.tri_phys)
+    return g, m
+
+
+def test_floquet_phase_reference_values():
+    """tests/test_periodic.cpp:10-158: broadside, 30 degree scan, set_floquet_phase at 45
+    degrees and the grating-lobe onset (-1) at half-wavelength spacing."""
+    k0 = 2 * math.pi * 10e9 / 299792458.0
+    assert abs(O.floquet_phase_from_angle((0.015, 0, 0), 0.0, 0.0, k0) - 1) < 1e-10
+    kx = k0 * math.sin(math.pi / 6)
+    assert abs(O.floquet_phase_from_angle((0.015, 0, 0), math.pi / 6, 0.0, k0)
+               - complex(math.cos(kx * 0.015), math.sin(kx * 0.015))) < 1e-10
+    pbc = O.OPeriodic(master=np.zeros(0, int), slave=np.zeros(0, int), orient=np.zeros(0, int),
+                      period=np.array([0.015, 0.0, 0.0]))
+    kt = (k0 * math.sin(math.pi / 4), 0.0)
+    O.set_floquet_phase(pbc, kt)
+    assert abs(pbc.phase - complex(math.cos(kt[0] * 0.015), math.sin(kt[0] * 0.015))) < 1e-10
+    lam = 299792458.0 / 10e9
+    assert abs(O.floquet_phase_from_angle((lam / 2, 0, 0), math.pi / 2, 0.0, 2 * math.pi / lam) + 1) < 1e-6
+
+
+def test_periodic_pairs_count_translation_orientation():
+    """build_periodic_pairs on a generated cell: one pair per edge of the x = 0 face
+    (ny(nz+1) + (ny+1)nz + ny nz edges), slave centroid = master centroid + period to the
+    tolerance, and the tri-local orientation product equals the geometric one (edge
+    directions parallel under the translation) on this structured mesh."""
+    n, nz = 3, 4
+    g, m = _cell(n=n, nz=nz)
+    per = np.array([5e-3, 0.0, 0.0])
+    pbc = O.build_periodic_pairs(m, 10, 11, per)
+    assert pbc.master.size == n * (nz + 1) + (n + 1) * nz + n * nz
+    cm = O._edge_centroids(m, pbc.master)
+    cs = O._edge_centroids(m, pbc.slave)
+    assert np.abs(cs - cm - per).max() < 1e-12
+    pos = {int(v): k for k, v in enumerate(m.node_ids)}
+    d = lambda e: m.xyz[pos[int(m.edges[e, 1])]] - m.xyz[pos[int(m.edges[e, 0])]]
+    geo = np.array([np.sign(d(a) @ d(b)) for a, b in zip(pbc.master, pbc.slave)])
+    np.testing.assert_array_equal(pbc.orient, geo)
+
+
+@pytest.mark.parametrize("mode", ["reference", "consistent"])
+def test_periodic_elimination_dense_vs_sparse(mode):
+    """The sparse elimination Z (E A F) Z + I_S against the literal dense sequence of row
+    and column updates of assemble_maxwell.cpp:526-571 on a small cell with a PEC patch
+    and a complex Floquet phase, in both coefficient conventions."""
+    g, m = _cell()
+    pec = O.build_edge_pec(m, 1)
+    pbc = O.build_periodic_pairs(m, 10, 11, (5e-3, 0, 0))
+    pbc.phase = np.exp(0.6j)
+    p = O.Params(omega=2 * math.pi * 10e9, eps_r_regions={101: 3.5 * (1 - 0.002j)})
+    ports = [O.build_lumped_port(m, t, 376.73, (0, 1, 0)) for t in (2, 3)]
+    A, b = O.assemble_maxwell(m, p, pec, ports, 0)
+    Ap, bp = O.periodic_eliminate(A, b, pbc, pec, mode)
+    Ad, bd = O.periodic_eliminate_dense(A, b, pbc, pec, mode)
+    np.testing.assert_allclose(Ap.toarray(), Ad, rtol=0, atol=1e-12 * np.abs(Ad).max())
+    np.testing.assert_allclose(bp, bd, rtol=0, atol=1e-15)
+
+
+def test_periodic_consistent_mode_is_galerkin_and_reference_mode_is_not():
+    """For a complex phase, the consistent convention (rows conj(phi) o, columns phi o,
+    recovery x_s = phi o x_m) leaves a residual b - A x of the full system that is
+    orthogonal to the Bloch test space (Z E (b - A x) = 0); the reference convention
+    (rows phi o, columns conj(phi) o, same recovery) does not, which is the ansatz
+    mismatch between assemble_maxwell.cpp:543 and :612.  At phi = 1 both agree."""
+    g, m = _cell()
+    pec = O.build_edge_pec(m, 1)
+    pbc = O.build_periodic_pairs(m, 10, 11, (5e-3, 0, 0))
+    p = O.Params(omega=2 * math.pi * 10e9, eps_r_regions={101: 3.5 * (1 - 0.002j)})
+    ports = [O.build_lumped_port(m, t, 376.73, (0, 1, 0)) for t in (2, 3)]
+    A, b = O.assemble_maxwell(m, p, pec, ports, 0)
+    n = A.shape[0]
+
+    def residual_in_test_space(mode, phase):
+        pbc.phase = phase
+        Ap, bp = O.periodic_eliminate(A, b, pbc, pec, mode)
+        x = spla.spsolve(Ap.tocsc(), bp)
+        a, c, rec = O._periodic_coefs(pbc, mode)
+        x[pbc.slave] = rec * x[pbc.master]
+        act = ~(pec[pbc.master] | pec[pbc.slave])
+        E = sp.identity(n, complex, format="csr") + sp.csr_matrix((a[act], (pbc.master[act], pbc.slave[act])), shape=(n, n))
+        keep = np.ones(n)
+        keep[pbc.slave[~pec[pbc.slave]]] = 0
+        return np.linalg.norm(keep * (E @ (b - A @ x))) / np.linalg.norm(b)
+
+    assert residual_in_test_space("consistent", np.exp(0.6j)) < 1e-10
+    assert residual_in_test_space("reference", np.exp(0.6j)) > 1e-3
+    assert residual_in_test_space("reference", 1.0 + 0j) < 1e-10
