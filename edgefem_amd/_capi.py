@@ -31,7 +31,7 @@ EXPORTS = [
     "efm_solver_spmv", "efm_solver_profile", "efm_solver_profile_read",
     "efm_port_overlap", "efm_solver_set_spmv_variant", "efm_solver_get_spmv_variant",
     "efm_sweep_create", "efm_sweep_destroy", "efm_sweep_solve", "efm_eps_gradient",
-    "efm_solver_set_lowrank", "efm_add_diagonal", "efm_periodic_reduce",
+    "efm_solver_set_lowrank", "efm_add_diagonal", "efm_periodic_reduce", "efm_solver_set_gradient_complex",
 ]
 
 
@@ -99,6 +99,7 @@ def load_library(path: str = LIB_PATH) -> C.CDLL:
     lib.efm_sweep_create.argtypes = [vp, vp, vp, i32, vp, vp, i32, vp, C.POINTER(vp)]
     lib.efm_solver_set_lowrank.argtypes = [vp, i32, C.c_int64, vp, vp, vp, vp]
     lib.efm_add_diagonal.argtypes = [vp, C.c_int64, vp, vp, vp, vp]
+    lib.efm_solver_set_gradient_complex.argtypes = [vp, C.c_int64, vp, vp, vp]
     lib.efm_periodic_reduce.argtypes = [C.c_int64, vp, vp, vp, C.c_int64, vp, vp, vp, vp, vp, vp, vp, vp,
                                         C.c_int64, C.POINTER(C.c_int64), vp]
     lib.efm_eps_gradient.argtypes = [vp, vp, C.c_int64, vp, i32, vp, i32, vp, i32, vp, vp, vp, vp, vp]
@@ -306,6 +307,17 @@ class KrylovSolver:
         dm = self._aux_keep[1]
         _check(self._lib.efm_solver_set_gradient(self._h, topo.n_nodes, dptr(topo.edge_nodes), dptr(topo.n2e_ptr),
                                                  dptr(topo.n2e), dptr(dm), stream_ptr()), "efm_solver_set_gradient")
+
+    def set_gradient_complex(self, n_nodes: int, edge_nodes, edge_coefs):
+        """Auxiliary preconditioner with a complex incidence (call before set_matrix):
+        edge_nodes (n, 2) int32 node indices (-1 = none), edge_coefs (n, 2) complex128."""
+        en = np.ascontiguousarray(np.asarray(edge_nodes, np.int32).reshape(-1))
+        ec = np.ascontiguousarray(np.asarray(edge_coefs, np.complex128).reshape(-1))
+        if en.size != 2 * self.n or ec.size != 2 * self.n:
+            raise EdgeFemError("gradient arrays must have two entries per edge")
+        _check(self._lib.efm_solver_set_gradient_complex(self._h, int(n_nodes), en.ctypes.data_as(C.c_void_p),
+                                                         ec.ctypes.data_as(C.c_void_p), stream_ptr()),
+               "efm_solver_set_gradient_complex")
 
     def set_lowrank(self, gidx, gval, coefs):
         """Operator term A_eff = A + sum_p coefs[p] w_p w_p^H (call before set_matrix).

@@ -861,6 +861,98 @@ __global__ void __launch_bounds__(kBlock)
 
 
 // ---------------------------------------------------------------------------
+// Auxiliary preconditioner with a complex incidence (periodic systems): row e of the
+// gradient G has up to two entries, ga[e] at node en2[e].x and gb[e] at node en2[e].y
+// (-1 = none).  Slave nodes of a Floquet pair are folded onto their master with the
+// phase, so G maps Bloch-periodic node potentials to reduced edge fields.
+//   P r = D^{-1} r + G Dn^{-1} G^T r,  Dn = diag(G^T A G)   (bilinear transposes)
+// ---------------------------------------------------------------------------
+__device__ inline double2 caux_coef(const int2 *__restrict__ en2, const double2 *__restrict__ ga,
+                                    const double2 *__restrict__ gb, int32_t f, int32_t node) {
+  const int2 nn = en2[f];
+  double2 c = make_double2(0, 0);
+  if (nn.x == node)
+    c = cadd(c, ga[f]);
+  if (nn.y == node)
+    c = cadd(c, gb[f]);
+  return c;
+}
+
+template <typename VT>
+__global__ void k_caux_diag(int64_t N, const int32_t *__restrict__ gt_ptr, const int32_t *__restrict__ gt_edge,
+                            const double2 *__restrict__ gt_coef, const int2 *__restrict__ en2,
+                            const double2 *__restrict__ ga, const double2 *__restrict__ gb,
+                            const int32_t *__restrict__ row_ptr, const int32_t *__restrict__ col,
+                            const VT *__restrict__ val, double2 *__restrict__ dninv) {
+  const int64_t nd = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (nd >= N)
+    return;
+  double2 sum = make_double2(0, 0);
+  for (int32_t q = gt_ptr[nd]; q < gt_ptr[nd + 1]; ++q) {
+    const int32_t e = gt_edge[q];
+    double2 inner = make_double2(0, 0);
+    for (int32_t k = row_ptr[e]; k < row_ptr[e + 1]; ++k) {
+      const double2 c = caux_coef(en2, ga, gb, col[k], (int32_t)nd);
+      if (c.x != 0.0 || c.y != 0.0)
+        inner = cfma(to_d2(val[k]), c, inner);
+    }
+    sum = cfma(gt_coef[q], inner, sum);
+  }
+  dninv[nd] = (sum.x == 0.0 && sum.y == 0.0) ? make_double2(0, 0) : cdiv(make_double2(1, 0), sum);
+}
+
+// y = Dn^{-1} G^T in
+template <int R>
+__global__ void __launch_bounds__(kBlock)
+    k_caux_node(int64_t N, const int32_t *__restrict__ gt_ptr, const int32_t *__restrict__ gt_edge,
+                const double2 *__restrict__ gt_coef, const double2 *__restrict__ dninv,
+                const double2 *__restrict__ in, double2 *__restrict__ y, const SysState *__restrict__ st) {
+  const int64_t nd = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (nd >= N)
+    return;
+  double2 acc[R];
+#pragma unroll
+  for (int s = 0; s < R; ++s)
+    acc[s] = make_double2(0, 0);
+  for (int32_t q = gt_ptr[nd]; q < gt_ptr[nd + 1]; ++q) {
+    const int64_t e = gt_edge[q];
+    const double2 g = gt_coef[q];
+#pragma unroll
+    for (int s = 0; s < R; ++s)
+      acc[s] = cfma(g, in[e * R + s], acc[s]);
+  }
+  const double2 di = dninv[nd];
+#pragma unroll
+  for (int s = 0; s < R; ++s)
+    if (st == nullptr || st[s].active)
+      y[nd * R + s] = cmul(di, acc[s]);
+}
+
+// out = D^{-1} in + G y
+template <int R>
+__global__ void __launch_bounds__(kBlock)
+    k_caux_edge(int64_t n, const int2 *__restrict__ en2, const double2 *__restrict__ ga,
+                const double2 *__restrict__ gb, const double2 *__restrict__ dinv, const double2 *__restrict__ y,
+                const double2 *__restrict__ in, double2 *__restrict__ out, const SysState *__restrict__ st) {
+  const int64_t e = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (e >= n)
+    return;
+  const int2 nn = en2[e];
+  const double2 di = dinv[e];
+#pragma unroll
+  for (int s = 0; s < R; ++s) {
+    if (st != nullptr && !st[s].active)
+      continue;
+    double2 v = cmul(di, in[e * R + s]);
+    if (nn.x >= 0)
+      v = cfma(ga[e], y[(int64_t)nn.x * R + s], v);
+    if (nn.y >= 0)
+      v = cfma(gb[e], y[(int64_t)nn.y * R + s], v);
+    out[e * R + s] = v;
+  }
+}
+
+// ---------------------------------------------------------------------------
 // Low-rank term of the operator: A_eff = A + sum_p c_p w_p w_p^H, with w_p given on a
 // compact list of edges (the rank-1 port loading of src/assemble_maxwell.cpp:209-242,
 // applied as an operator instead of dense blocks in the sparsity pattern).  The ports
@@ -1224,6 +1316,10 @@ struct efm_solver {
   const uint8_t *dir = nullptr;
   efm::DevBuf dninv, ynode, gsign, en_eff;
   int grid_node_groups = 1;
+  // complex-incidence auxiliary preconditioner (periodic systems)
+  bool caux = false;
+  int64_t caux_N = 0;
+  efm::DevBuf caux_en2, caux_ga, caux_gb, caux_gt_ptr, caux_gt_edge, caux_gt_coef, caux_dninv, caux_y;
   // low-rank term (ports): P lists of len entries
   int lr_P = 0;
   int64_t lr_len = 0;
@@ -1253,6 +1349,17 @@ template <int R> struct Dispatch {
   template <bool PARTIALS>
   static int aux_apply(efm_solver *s, const double2 *in, double2 *out, int ostride, hipStream_t st,
                        const SysState *S) {
+    if (s->caux) {
+      hipLaunchKernelGGL((k_caux_node<R>), dim3(grid_for(s->caux_N, kBlock)), dim3(kBlock), 0, st, s->caux_N,
+                         s->caux_gt_ptr.as<int32_t>(), s->caux_gt_edge.as<int32_t>(), s->caux_gt_coef.as<double2>(),
+                         s->caux_dninv.as<double2>(), in, s->caux_y.as<double2>(), S);
+      EFM_LAUNCH_CHECK();
+      hipLaunchKernelGGL((k_caux_edge<R>), dim3(grid_for(s->n, kBlock)), dim3(kBlock), 0, st, s->n,
+                         s->caux_en2.as<int2>(), s->caux_ga.as<double2>(), s->caux_gb.as<double2>(),
+                         s->dinv.as<double2>(), s->caux_y.as<double2>(), in, out, S);
+      EFM_LAUNCH_CHECK();
+      return EFM_OK;
+    }
     hipLaunchKernelGGL((k_aux_node<R, false>), dim3(s->grid_node_groups), dim3(kBlock), 0, st,
                        s->n_nodes, s->n2e_ptr, s->n2e, s->gsign.as<int8_t>(), s->dninv.as<double2>(),
                        in, (const double2 *)nullptr, s->ynode.as<double2>(), S);
@@ -1452,7 +1559,8 @@ template <int R> struct Dispatch {
                        s->r.as<double2>(), s->v.as<double2>(), s->p0.as<double2>(),
                        s->y.as<double2>(), S);
     EFM_LAUNCH_CHECK();
-    if (s->aux && (rc = aux_apply<false>(s, s->p0.as<double2>(), s->y.as<double2>(), R, st, S)) != EFM_OK)
+    if ((s->aux || s->caux) &&
+        (rc = aux_apply<false>(s, s->p0.as<double2>(), s->y.as<double2>(), R, st, S)) != EFM_OK)
       return rc;
     if (e0)
       EFM_HIP_CHECK(hipEventRecord(e0, st));
@@ -1471,7 +1579,8 @@ template <int R> struct Dispatch {
     hipLaunchKernelGGL(k_bs_s<R>, dim3(s->grid), dim3(kBlock), 0, st, s->n, s->dinv.as<double2>(),
                        s->v.as<double2>(), s->r.as<double2>(), s->zz.as<double2>(), S);
     EFM_LAUNCH_CHECK();
-    if (s->aux && (rc = aux_apply<false>(s, s->r.as<double2>(), s->zz.as<double2>(), R, st, S)) != EFM_OK)
+    if ((s->aux || s->caux) &&
+        (rc = aux_apply<false>(s, s->r.as<double2>(), s->zz.as<double2>(), R, st, S)) != EFM_OK)
       return rc;
     hipLaunchKernelGGL((k_bs_spmv<VT, R, 1>), dim3(s->grid_spmv), dim3(kBlock), 0, st, s->n,
                        s->row_ptr, s->col, s->val.as<VT>(), s->zz.as<double2>(), s->t.as<double2>(),
@@ -1626,6 +1735,19 @@ extern "C" int efm_solver_set_matrix(efm_solver *s, const void *values, void *st
                          s->col, s->val.as<float2>(), s->dinv.as<double2>());
     EFM_LAUNCH_CHECK();
   }
+  if (s->caux) {
+    if (s->vtype == EFM_C128)
+      hipLaunchKernelGGL(k_caux_diag<double2>, dim3(grid_for(s->caux_N, 256)), dim3(256), 0, st, s->caux_N,
+                         s->caux_gt_ptr.as<int32_t>(), s->caux_gt_edge.as<int32_t>(), s->caux_gt_coef.as<double2>(),
+                         s->caux_en2.as<int2>(), s->caux_ga.as<double2>(), s->caux_gb.as<double2>(), s->row_ptr,
+                         s->col, s->val.as<double2>(), s->caux_dninv.as<double2>());
+    else
+      hipLaunchKernelGGL(k_caux_diag<float2>, dim3(grid_for(s->caux_N, 256)), dim3(256), 0, st, s->caux_N,
+                         s->caux_gt_ptr.as<int32_t>(), s->caux_gt_edge.as<int32_t>(), s->caux_gt_coef.as<double2>(),
+                         s->caux_en2.as<int2>(), s->caux_ga.as<double2>(), s->caux_gb.as<double2>(), s->row_ptr,
+                         s->col, s->val.as<float2>(), s->caux_dninv.as<double2>());
+    EFM_LAUNCH_CHECK();
+  }
   if (s->aux) {
     if (s->vtype == EFM_C128)
       hipLaunchKernelGGL(k_aux_diag<double2>, dim3(grid_for(s->n_nodes, 256)), dim3(256), 0, st,
@@ -1676,6 +1798,68 @@ extern "C" int efm_solver_set_lowrank(efm_solver *s, int n_terms, int64_t len, c
   return EFM_OK;
 }
 
+extern "C" int efm_solver_set_gradient_complex(efm_solver *s, int64_t n_nodes, const int32_t *h_edge_nodes,
+                                               const double *h_edge_coefs, void *stream) {
+  EFM_REQUIRE(s, "efm_solver_set_gradient_complex: NULL solver");
+  if (n_nodes <= 0) {
+    s->caux = false;
+    return EFM_OK;
+  }
+  EFM_REQUIRE(h_edge_nodes && h_edge_coefs, "efm_solver_set_gradient_complex: NULL argument");
+  EFM_REQUIRE(n_nodes < INT32_MAX, "efm_solver_set_gradient_complex: too many nodes");
+  hipStream_t st = (hipStream_t)stream;
+  const int64_t n = s->n;
+  std::vector<int2> en2(n);
+  std::vector<double2> ga(n), gb(n);
+  std::vector<int32_t> cnt(n_nodes + 1, 0);
+  for (int64_t e = 0; e < n; ++e) {
+    const int32_t a = h_edge_nodes[2 * e], b = h_edge_nodes[2 * e + 1];
+    EFM_REQUIRE(a >= -1 && a < n_nodes && b >= -1 && b < n_nodes, "efm_solver_set_gradient_complex: node index");
+    en2[e] = make_int2(a, b);
+    ga[e] = make_double2(h_edge_coefs[4 * e], h_edge_coefs[4 * e + 1]);
+    gb[e] = make_double2(h_edge_coefs[4 * e + 2], h_edge_coefs[4 * e + 3]);
+    if (a >= 0)
+      ++cnt[a + 1];
+    if (b >= 0 && b != a)
+      ++cnt[b + 1];
+  }
+  for (int64_t k = 0; k < n_nodes; ++k)
+    cnt[k + 1] += cnt[k];
+  std::vector<int32_t> gt_edge(std::max<int32_t>(cnt[n_nodes], 1));
+  std::vector<double2> gt_coef(std::max<int32_t>(cnt[n_nodes], 1));
+  std::vector<int32_t> fill(cnt.begin(), cnt.end() - 1);
+  for (int64_t e = 0; e < n; ++e) {  // ascending edge order per node
+    const int2 nn = en2[e];
+    if (nn.x >= 0) {
+      const double2 c = (nn.y == nn.x) ? cadd(ga[e], gb[e]) : ga[e];
+      gt_edge[fill[nn.x]] = (int32_t)e;
+      gt_coef[fill[nn.x]++] = c;
+    }
+    if (nn.y >= 0 && nn.y != nn.x) {
+      gt_edge[fill[nn.y]] = (int32_t)e;
+      gt_coef[fill[nn.y]++] = gb[e];
+    }
+  }
+  int rc;
+  if ((rc = s->caux_en2.alloc(n * sizeof(int2))) || (rc = s->caux_ga.alloc(n * sizeof(double2))) ||
+      (rc = s->caux_gb.alloc(n * sizeof(double2))) || (rc = s->caux_gt_ptr.alloc((n_nodes + 1) * 4)) ||
+      (rc = s->caux_gt_edge.alloc(gt_edge.size() * 4)) || (rc = s->caux_gt_coef.alloc(gt_coef.size() * 16)) ||
+      (rc = s->caux_dninv.alloc(n_nodes * sizeof(double2))) ||
+      (rc = s->caux_y.alloc((size_t)n_nodes * s->R * sizeof(double2))))
+    return rc;
+  EFM_HIP_CHECK(hipMemcpyAsync(s->caux_en2.p, en2.data(), n * sizeof(int2), hipMemcpyHostToDevice, st));
+  EFM_HIP_CHECK(hipMemcpyAsync(s->caux_ga.p, ga.data(), n * sizeof(double2), hipMemcpyHostToDevice, st));
+  EFM_HIP_CHECK(hipMemcpyAsync(s->caux_gb.p, gb.data(), n * sizeof(double2), hipMemcpyHostToDevice, st));
+  EFM_HIP_CHECK(hipMemcpyAsync(s->caux_gt_ptr.p, cnt.data(), (n_nodes + 1) * 4, hipMemcpyHostToDevice, st));
+  EFM_HIP_CHECK(hipMemcpyAsync(s->caux_gt_edge.p, gt_edge.data(), gt_edge.size() * 4, hipMemcpyHostToDevice, st));
+  EFM_HIP_CHECK(hipMemcpyAsync(s->caux_gt_coef.p, gt_coef.data(), gt_coef.size() * 16, hipMemcpyHostToDevice, st));
+  EFM_HIP_CHECK(hipStreamSynchronize(st));
+  s->caux_N = n_nodes;
+  s->caux = true;
+  s->aux = false;
+  return EFM_OK;
+}
+
 extern "C" int efm_solver_set_gradient(efm_solver *s, int64_t n_nodes, const int32_t *edge_nodes,
                                        const int32_t *n2e_ptr, const int32_t *n2e,
                                        const uint8_t *dirichlet, void *stream) {
@@ -1713,6 +1897,7 @@ extern "C" int efm_solver_set_gradient(efm_solver *s, int64_t n_nodes, const int
                      dirichlet, s->en_eff.as<int2>());
   EFM_LAUNCH_CHECK();
   s->aux = true;
+  s->caux = false;
   return EFM_OK;
 }
 
@@ -1776,6 +1961,7 @@ extern "C" int efm_solver_solve(efm_solver *s, const double *B, double *X,
   const int max_it = h_opts->max_iterations;
   const int check = h_opts->check_every > 0 ? h_opts->check_every : 32;
   const bool cocg = h_opts->method == EFM_METHOD_COCG;
+  EFM_REQUIRE(!(cocg && s->caux), "efm_solver_solve: the complex-incidence preconditioner needs BiCGSTAB");
   int rc;
   if (!cocg) {
     const size_t vb = (size_t)s->n * s->R * sizeof(double2);

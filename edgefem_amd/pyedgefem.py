@@ -1203,13 +1203,54 @@ def assemble_maxwell_periodic(mesh: Mesh, params: MaxwellParams, bc: BC, pbc: Pe
     return MaxwellAssembly(SpMatC(rp, ci, v, symmetric=False, lowrank=new_lr), VecC(bp))
 
 
+def _periodic_gradient(mesh: Mesh, pbc: PeriodicBC, dmask: np.ndarray, zero: np.ndarray, mode: str):
+    """Discrete gradient of the reduced periodic edge space: the nodes of the slave face
+    are folded onto the master face nodes (matched by the period translation) with the
+    column phase (conj(phase) in the reference convention, phase in the consistent one),
+    so G maps Bloch-periodic node potentials to reduced edge fields; PEC and eliminated
+    slave edges have empty rows.  Returns (edge_nodes (n,2), edge_coefs (n,2), n_nodes)."""
+    N = mesh.node_ids.size
+    nmap = np.arange(N)
+    ncoef = np.ones(N, np.complex128)
+    if pbc.pairs:
+        ph = complex(pbc.phase_shift)
+        node_phase = np.conj(ph) if mode == "reference" else ph
+        me = np.array([p.master_edge for p in pbc.pairs])
+        se = np.array([p.slave_edge for p in pbc.pairs])
+        nid = mesh._node_pos()
+        mn = np.unique(nid(mesh.edge_nodes[me].ravel()))
+        sn = np.unique(nid(mesh.edge_nodes[se].ravel()))
+        tol = 1e-9
+        table = {tuple(k): i for i, k in zip(sn, np.round(mesh.xyz[sn] / tol).astype(np.int64))}
+        shifted = np.round((mesh.xyz[mn] + pbc.period_vector) / tol).astype(np.int64)
+        for i, k in zip(mn, shifted):
+            j = table.get(tuple(k), -1)
+            if j < 0:
+                d = np.linalg.norm(mesh.xyz[sn] - (mesh.xyz[i] + pbc.period_vector), axis=1)
+                hit = np.nonzero(d < tol)[0]
+                j = int(sn[hit[0]]) if hit.size else -1
+            if j >= 0:
+                nmap[j] = i
+                ncoef[j] = node_phase
+    nid = mesh._node_pos()
+    a = nid(mesh.edge_nodes[:, 0])
+    b = nid(mesh.edge_nodes[:, 1])
+    en = np.stack([nmap[a], nmap[b]], axis=1).astype(np.int32)
+    ec = np.stack([-ncoef[a], ncoef[b]], axis=1)
+    off = (dmask != 0) | zero
+    en[off] = -1
+    ec[off] = 0
+    return en, ec, N
+
+
 def calculate_sparams_periodic(mesh: Mesh, params: MaxwellParams, bc: BC, pbc: PeriodicBC,
                                ports: Sequence[WavePort], *, mode: str = "reference",
                                solver_options: Optional[SolveOptions] = None,
                                _state: Optional[dict] = None) -> np.ndarray:
     """calculate_sparams_periodic (assemble_maxwell.cpp:576-635) on the GPU: the reduced
     system is shared by the excitations and solved as one block by BiCGSTAB (a complex
-    phase breaks the symmetry) with the port loading as a low-rank operator term; slave
+    phase breaks the symmetry), preconditioned by edge Jacobi plus the nodal correction of
+    the periodic discrete gradient, with the port loading as a low-rank operator term; slave
     values are recovered as x_s = phase o x_m for every pair; S as in calculate_sparams.
     ``mode`` selects the reference's coefficients or the consistent ones."""
     import torch
@@ -1241,6 +1282,8 @@ def calculate_sparams_periodic(mesh: Mesh, params: MaxwellParams, bc: BC, pbc: P
                 col[zero] = 0
             Bh[:, i] = col
     solver = _capi.KrylovSolver(Am.row_ptr, Am.col_idx, nrhs=R)
+    en, ec, N = _periodic_gradient(mesh, pbc, dmask, zero, mode)
+    solver.set_gradient_complex(N, en, ec)
     if Am.lowrank:
         solver.set_lowrank(*_pack_lowrank(Am.lowrank))
     solver.set_matrix(Am.values)
@@ -1254,7 +1297,7 @@ def calculate_sparams_periodic(mesh: Mesh, params: MaxwellParams, bc: BC, pbc: P
     for i, port in enumerate(ports):
         if not (info["resid"][i] <= tol or info["iters"][i] == 0):
             sys.stderr.write(f"WARNING: Solver did not converge in calculate_sparams_periodic (active port {i}) "
-                             f"(method=BiCGSTAB+Jacobi, iters={info['iters'][i]}, residual={info['resid'][i]}). "
+                             f"(method=BiCGSTAB+periodic-aux, iters={info['iters'][i]}, residual={info['resid'][i]}). "
                              "S-parameters may be unreliable.\n")
             continue
         vinc = cmath.sqrt(complex(port.mode.Z0))

@@ -142,6 +142,16 @@ int efm_solver_set_gradient(efm_solver *s, int64_t n_nodes, const int32_t *edge_
                             const int32_t *n2e_ptr, const int32_t *n2e, const uint8_t *dirichlet,
                             void *stream);
 
+/* Auxiliary preconditioner with a complex gradient incidence, for systems whose edge
+ * space has been reduced (periodic elimination): row e of G is
+ * h_edge_coefs[2e] at node h_edge_nodes[2e] plus h_edge_coefs[2e+1] at node
+ * h_edge_nodes[2e+1] (host arrays [2*n]; c128 coefficients; node -1 = no entry).  The
+ * nodal diagonal diag(G^T A G) is formed at efm_solver_set_matrix.  BiCGSTAB only.
+ * n_nodes <= 0 switches it off; it replaces efm_solver_set_gradient.  No reference
+ * counterpart (the reference uses ILUT). */
+int efm_solver_set_gradient_complex(efm_solver *s, int64_t n_nodes, const int32_t *h_edge_nodes,
+                                    const double *h_edge_coefs, void *stream);
+
 /* Low-rank operator term: the solver iterates on A_eff = A + sum_p h_coef[p] w_p w_p^H, with
  * w_p given on a compact edge list (gidx[p*len + i] >= 0, -1 = padding; gval the values;
  * device arrays, copied).  This is the rank-1 port loading w w^H / Z0 of
