@@ -1310,6 +1310,183 @@ def calculate_sparams_periodic(mesh: Mesh, params: MaxwellParams, bc: BC, pbc: P
     return S
 
 
+class SurfaceMesh2D:
+    """2D port mesh (extract_surface_mesh): nodes projected to z = 0, tris by gmsh IDs."""
+
+    def __init__(self, node_ids, xyz, tri_ids, boundary_lines):
+        self.node_ids = np.asarray(node_ids, np.int64)
+        self.xyz = np.asarray(xyz, np.float64)
+        self.tri_ids = np.asarray(tri_ids, np.int64).reshape(-1, 3)
+        self.boundary_lines = boundary_lines            # list of (n0, n1, phys)
+        order = np.argsort(self.node_ids, kind="stable")
+        self._pos = (order, self.node_ids[order])
+
+    def node_pos(self, ids):
+        order, sid = self._pos
+        return order[np.searchsorted(sid, np.asarray(ids, np.int64))]
+
+
+class PortSurfaceMesh:
+    def __init__(self, mesh: SurfaceMesh2D, volume_tri_indices):
+        self.mesh = mesh
+        self.volume_tri_indices = np.asarray(volume_tri_indices, np.int64)
+
+
+def extract_surface_mesh(volume_mesh: Mesh, surface_tag: int) -> PortSurfaceMesh:
+    """The tris of ``surface_tag`` as a 2D mesh (wave_port.cpp:54-111): nodes in first-use
+    order with z dropped, boundary lines (edges used once) tagged 1."""
+    sel = np.nonzero(volume_mesh.tri_phys == surface_tag)[0]
+    tri_ids = volume_mesh.node_ids[volume_mesh.tri_nodes[sel]]
+    flat = tri_ids.ravel()
+    _, first = np.unique(flat, return_index=True)
+    ids = flat[np.sort(first)]
+    xyz = volume_mesh.xyz[volume_mesh._node_pos()(ids)].copy()
+    xyz[:, 2] = 0.0
+    e = np.concatenate([tri_ids[:, [0, 1]], tri_ids[:, [1, 2]], tri_ids[:, [2, 0]]])
+    e = np.sort(e, axis=1)
+    u, cnt = np.unique(e, axis=0, return_counts=True)
+    lines = [(int(a), int(b), 1) for a, b in u[cnt == 1]]
+    return PortSurfaceMesh(SurfaceMesh2D(ids, xyz, tri_ids, lines), sel)
+
+
+def _p1_gradients(xy: np.ndarray):
+    """Per tri: area and the (3, 2) gradients of the P1 shape functions (rows of
+    inverse([1 x y]) without its first row)."""
+    C = np.concatenate([np.ones(xy.shape[:2] + (1,)), xy], axis=2)          # (F,3,3)
+    area = 0.5 * np.abs(np.linalg.det(C))
+    G = np.linalg.inv(C)[:, 1:, :].transpose(0, 2, 1)                        # (F,3,2)
+    return area, G
+
+
+def solve_port_eigens(mesh: SurfaceMesh2D, num_modes: int, omega: float, eps_r=1.0, mu_r=1.0,
+                      pol: "ModePolarization" = ModePolarization.TE) -> List[PortMode]:
+    """2D nodal P1 port modes (port_eigensolve.cpp:97-275): K_s psi = kc^2 M_s psi with
+    Neumann (TE, Hz) or Dirichlet on the tag-1 boundary lines (TM, Ez); dense generalized
+    symmetric eigensolve on the host (small); kc^2 below max(1e-12, 1e-10 kc^2_max)
+    skipped (the reference's absolute 1e-12 admits the roundoff-level Neumann constant
+    mode as a spurious first TE mode on millimetre meshes); beta = sqrt(k^2 -
+    kc^2); Z0 = omega mu / beta (TE) or beta / (omega eps) (TM); field scaled to unit
+    real power.  The eigenvector sign is not the reference's (Eigen picks it; it cannot
+    be reproduced): each mode is made real with its largest entry positive."""
+    import scipy.linalg
+    if mesh.tri_ids.shape[0] == 0:
+        raise RuntimeError("Port eigensolver requires a 2D mesh.")
+    if omega == 0.0:
+        raise RuntimeError("Port eigensolver requires non-zero frequency.")
+    N = mesh.node_ids.size
+    pec = np.zeros(N, bool)
+    if ModePolarization(int(pol)) == ModePolarization.TM:
+        for a, b, tag in mesh.boundary_lines:
+            if tag == 1:
+                pec[mesh.node_pos([a, b])] = True
+    dof = -np.ones(N, np.int64)
+    dof[~pec] = np.arange(int((~pec).sum()))
+    nd = int((~pec).sum())
+    tp = mesh.node_pos(mesh.tri_ids)                                          # (F,3)
+    area, G = _p1_gradients(mesh.xyz[tp][:, :, :2])
+    ke = area[:, None, None] * np.einsum("fik,fjk->fij", G, G)
+    me = (area / 12.0)[:, None, None] * np.array([[2, 1, 1], [1, 2, 1], [1, 1, 2]], float)
+    di = dof[tp]
+    ok = (di[:, :, None] >= 0) & (di[:, None, :] >= 0)
+    r = np.broadcast_to(di[:, :, None], ke.shape)[ok]
+    c = np.broadcast_to(di[:, None, :], ke.shape)[ok]
+    A = np.zeros((nd, nd))
+    B = np.zeros((nd, nd))
+    np.add.at(A, (r, c), ke[ok])
+    np.add.at(B, (r, c), me[ok])
+    w, V = scipy.linalg.eigh(A, B)
+    eps = EPS0 * complex(eps_r)
+    mu = MU0 * complex(mu_r)
+    k = omega * cmath.sqrt(mu * eps)
+    modes = []
+    for i in range(w.size):
+        kc2 = float(w[i])
+        if not np.isfinite(kc2) or kc2 < max(1e-12, 1e-10 * abs(float(w[-1]))):
+            continue
+        kc = math.sqrt(kc2)
+        mode = PortMode()
+        mode.pol = ModePolarization(int(pol))
+        mode.fc = kc * C0 / (2 * math.pi)
+        mode.kc = kc
+        mode.omega = omega
+        mode.eps = eps
+        mode.mu = mu
+        beta = cmath.sqrt(k * k - kc2)
+        mode.beta = beta
+        if mode.pol == ModePolarization.TE:
+            mode.Z0 = 0j if beta == 0 else omega * mu / beta
+        else:
+            mode.Z0 = 0j if beta == 0 else beta / (omega * eps)
+        v = V[:, i].astype(np.complex128)
+        j = int(np.argmax(np.abs(v)))
+        v = v * (1.0 if v[j].real >= 0 else -1.0)
+        field = np.zeros(N, np.complex128)
+        field[~pec] = v
+        grad = np.einsum("fik,fi->fk", G, field[tp])                          # (F,2)
+        if mode.pol == ModePolarization.TE:
+            fe = 1j * omega * mu / kc2
+            fh = beta / kc2
+            Et = np.stack([-fe * grad[:, 1], fe * grad[:, 0]], 1)
+            Ht = np.stack([fh * grad[:, 0], fh * grad[:, 1]], 1)
+        else:
+            fe = -beta / kc2
+            fh = 1.0 / (1j * omega * mu)
+            Et = np.stack([fe * grad[:, 0], fe * grad[:, 1]], 1)
+            Ht = np.stack([-fh * grad[:, 1], fh * grad[:, 0]], 1)
+        pw = np.sum(0.5 * area * (Et[:, 0] * np.conj(Ht[:, 1]) - Et[:, 1] * np.conj(Ht[:, 0])))
+        pr = pw.real if pw.real > 0 else abs(pw)
+        if pr <= 0:
+            continue
+        mode.field = field / math.sqrt(pr)
+        modes.append(mode)
+        if len(modes) >= num_modes:
+            break
+    modes.sort(key=lambda md: md.fc)
+    return modes[:num_modes]
+
+
+def build_wave_port(volume_mesh: Mesh, surface: PortSurfaceMesh, mode: PortMode) -> WavePort:
+    """Edge weights from a nodal port mode (wave_port.cpp:113-205): per tri the tangential
+    E of the mode (TE: Et = j omega mu / kc^2 (dHz/dy, -dHz/dx); TM: -beta/kc^2 grad Ez)
+    dotted with each edge vector (min ID -> max ID), averaged over the tris sharing the
+    edge, times the sign of the summed z normal of the volume tris; edges sorted."""
+    sm = surface.mesh
+    if np.asarray(mode.field).size != sm.node_ids.size:
+        raise RuntimeError("Port mode field size does not match surface mesh nodes")
+    if mode.kc == 0.0:
+        raise RuntimeError("Port mode has zero cutoff wavenumber")
+    tp = sm.node_pos(sm.tri_ids)
+    _, G = _p1_gradients(sm.xyz[tp][:, :, :2])
+    grad = np.einsum("fik,fi->fk", G, np.asarray(mode.field, np.complex128)[tp])
+    kc2 = mode.kc * mode.kc
+    if ModePolarization(int(mode.pol)) == ModePolarization.TE:
+        fe = 1j * mode.omega * mode.mu / kc2
+        Et = np.stack([fe * grad[:, 1], -fe * grad[:, 0]], 1)
+    else:
+        fe = -mode.beta / kc2
+        Et = np.stack([fe * grad[:, 0], fe * grad[:, 1]], 1)
+    vt = surface.volume_tri_indices
+    nid = volume_mesh._node_pos()
+    v = volume_mesh.xyz[volume_mesh.tri_nodes[vt]]
+    normal_z = np.cross(v[:, 1] - v[:, 0], v[:, 2] - v[:, 0])[:, 2].sum()
+    sign = 1.0 if normal_z >= 0 else -1.0
+    edges = volume_mesh.tri_edges[vt]                                          # (F,3)
+    en = volume_mesh.edge_nodes[edges]                                         # (F,3,2)
+    ev = volume_mesh.xyz[nid(en[..., 1])] - volume_mesh.xyz[nid(en[..., 0])]  # (F,3,3)
+    integ = Et[:, None, 0] * ev[..., 0] + Et[:, None, 1] * ev[..., 1]
+    ue, inv = np.unique(edges.ravel(), return_inverse=True)
+    acc = np.zeros(ue.size, np.complex128)
+    cnt = np.zeros(ue.size)
+    np.add.at(acc, inv, integ.ravel())
+    np.add.at(cnt, inv, 1.0)
+    port = WavePort()
+    port.surface_tag = int(volume_mesh.tri_phys[vt[0]]) if vt.size else 0
+    port.mode = mode
+    port.edges = [int(e) for e in ue]
+    port.weights = sign * acc / cnt
+    return port
+
+
 class LumpedPortWeightMode(enum.IntEnum):
     Projection = 0
     SurfaceIntegral = 1

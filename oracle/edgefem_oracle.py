@@ -19,6 +19,9 @@ solve_te10_mode             src/ports/port_eigensolve.cpp:45-63
 sparams_eigenmode           src/assemble_maxwell.cpp:637-787
 calculate_sparams           src/assemble_maxwell.cpp:352-394 (port loading 209-242)
 frequency_sweep             src/sweep.cpp:174-329 (K/M reuse branch)
+extract_surface_mesh        src/ports/wave_port.cpp:54-111
+solve_port_eigens           src/ports/port_eigensolve.cpp:97-275
+build_wave_port             src/ports/wave_port.cpp:113-205
 build_periodic_pairs        src/periodic.cpp:48-164
 floquet_phase_from_angle    src/periodic.cpp:204-225 (and set_floquet_phase)
 assemble_maxwell_periodic   src/assemble_maxwell.cpp:496-574 (sparse restatement;
@@ -692,6 +695,137 @@ def frequency_sweep(mesh: OMesh, p: Params, dirichlet: np.ndarray, ports: Sequen
                 S[j, i] = (Vj - vinc) / vinc if i == j else Vj / vinc
         out.append(S)
     return np.array(out)
+
+
+def extract_surface_mesh(mesh: OMesh, tag: int):
+    """(node ids in first-use order, xy, tri node ids, boundary edges used once, volume tri
+    indices) of the tris of ``tag`` (wave_port.cpp:54-111)."""
+    sel = np.nonzero(mesh.tri_phys == tag)[0]
+    tri = mesh.tri_ids[sel]
+    ids = []
+    seen = set()
+    for t in tri:
+        for v in t:
+            if int(v) not in seen:
+                seen.add(int(v))
+                ids.append(int(v))
+    ids = np.array(ids, np.int64)
+    xy = mesh.xyz[[mesh.node_index[int(v)] for v in ids]][:, :2]
+    cnt = {}
+    for t in tri:
+        for a, b in ((t[0], t[1]), (t[1], t[2]), (t[2], t[0])):
+            k = (min(int(a), int(b)), max(int(a), int(b)))
+            cnt[k] = cnt.get(k, 0) + 1
+    lines = sorted(k for k, c in cnt.items() if c == 1)
+    return ids, xy, tri, lines, sel
+
+
+def solve_port_eigens(surf, num_modes: int, omega: float, eps_r=1.0, mu_r=1.0, pol: str = "TE"):
+    """Nodal P1 port modes (port_eigensolve.cpp:97-275), element loop restated with
+    explicit 2x2 algebra; returns a list of dicts (kc, fc, beta, Z0, field).  Eigenvector
+    sign: made real with its largest entry positive (the reference's is Eigen's).  The
+    zero-mode cut is kc^2 < max(1e-12, 1e-10 kc^2_max): with the reference's absolute
+    1e-12 the Neumann constant mode, whose computed kc^2 is roundoff of order 1e-9 on
+    millimetre meshes, would come back as a spurious first TE mode."""
+    ids, xy, tri, lines, _ = surf
+    N = ids.size
+    pos = {int(v): k for k, v in enumerate(ids)}
+    pec = np.zeros(N, bool)
+    if pol == "TM":
+        for a, b in lines:
+            pec[pos[a]] = pec[pos[b]] = True
+    dof = -np.ones(N, int)
+    dof[~pec] = np.arange((~pec).sum())
+    nd = int((~pec).sum())
+    A = np.zeros((nd, nd)); Bm = np.zeros((nd, nd))
+    tris = []
+    for t in tri:
+        ii = [pos[int(v)] for v in t]
+        p = xy[ii]
+        d = (p[1, 0] - p[0, 0]) * (p[2, 1] - p[0, 1]) - (p[2, 0] - p[0, 0]) * (p[1, 1] - p[0, 1])
+        area = abs(d) / 2.0
+        # grad l_i = perpendicular of the opposite edge / (2 * signed area)
+        g = np.array([[p[1, 1] - p[2, 1], p[2, 0] - p[1, 0]],
+                      [p[2, 1] - p[0, 1], p[0, 0] - p[2, 0]],
+                      [p[0, 1] - p[1, 1], p[1, 0] - p[0, 0]]]) / d
+        tris.append((ii, area, g))
+        for a in range(3):
+            for b in range(3):
+                if dof[ii[a]] >= 0 and dof[ii[b]] >= 0:
+                    A[dof[ii[a]], dof[ii[b]]] += area * g[a] @ g[b]
+                    Bm[dof[ii[a]], dof[ii[b]]] += area / 12.0 * (2.0 if a == b else 1.0)
+    w, V = scipy.linalg.eigh(A, Bm)
+    eps = EPS0 * complex(eps_r); mu = MU0 * complex(mu_r)
+    k = omega * np.sqrt(mu * eps)
+    out = []
+    for i in range(w.size):
+        kc2 = w[i]
+        if not np.isfinite(kc2) or kc2 < max(1e-12, 1e-10 * abs(w[-1])):
+            continue
+        kc = math.sqrt(kc2)
+        beta = np.sqrt(complex(k * k - kc2))
+        Z0 = (omega * mu / beta if pol == "TE" else beta / (omega * eps)) if beta != 0 else 0j
+        v = V[:, i].astype(complex)
+        j = int(np.argmax(np.abs(v)))
+        v = v if v[j].real >= 0 else -v
+        f = np.zeros(N, complex); f[~pec] = v
+        pw = 0.0
+        for ii, area, g in tris:
+            gr = g.T @ f[ii]
+            if pol == "TE":
+                fe = 1j * omega * mu / kc2; fh = beta / kc2
+                Et = (-fe * gr[1], fe * gr[0]); Ht = (fh * gr[0], fh * gr[1])
+            else:
+                fe = -beta / kc2; fh = 1.0 / (1j * omega * mu)
+                Et = (fe * gr[0], fe * gr[1]); Ht = (-fh * gr[1], fh * gr[0])
+            pw += 0.5 * area * (Et[0] * np.conj(Ht[1]) - Et[1] * np.conj(Ht[0]))
+        pr = pw.real if pw.real > 0 else abs(pw)
+        if pr <= 0:
+            continue
+        out.append(dict(kc=kc, fc=kc * C0 / (2 * math.pi), beta=beta, Z0=Z0, field=f / math.sqrt(pr),
+                        omega=omega, mu=mu, eps=eps, pol=pol))
+        if len(out) >= num_modes:
+            break
+    out.sort(key=lambda md: md["fc"])
+    return out[:num_modes]
+
+
+def build_wave_port(mesh: OMesh, surf, mode) -> OPort:
+    """Edge weights from a nodal mode (wave_port.cpp:113-205): tangential E per tri dotted
+    with the edge vector, averaged over the tris of each edge, times the sign of the
+    summed z normal; edges sorted."""
+    ids, xy, tri, _, sel = surf
+    pos = {int(v): k for k, v in enumerate(ids)}
+    acc, cnt = {}, {}
+    nz = 0.0
+    kc2 = mode["kc"] ** 2
+    for t3, f in zip(tri, sel):
+        ii = [pos[int(v)] for v in t3]
+        p = xy[ii]
+        d = (p[1, 0] - p[0, 0]) * (p[2, 1] - p[0, 1]) - (p[2, 0] - p[0, 0]) * (p[1, 1] - p[0, 1])
+        g = np.array([[p[1, 1] - p[2, 1], p[2, 0] - p[1, 0]],
+                      [p[2, 1] - p[0, 1], p[0, 0] - p[2, 0]],
+                      [p[0, 1] - p[1, 1], p[1, 0] - p[0, 0]]]) / d
+        gr = g.T @ mode["field"][ii]
+        if mode["pol"] == "TE":
+            fe = 1j * mode["omega"] * mode["mu"] / kc2
+            Et = np.array([fe * gr[1], -fe * gr[0], 0.0])
+        else:
+            fe = -mode["beta"] / kc2
+            Et = np.array([fe * gr[0], fe * gr[1], 0.0])
+        v = mesh.xyz[[mesh.node_index[int(x)] for x in mesh.tri_ids[f]]]
+        nz += np.cross(v[1] - v[0], v[2] - v[0])[2]
+        for le in range(3):
+            e = int(mesh.tri_edges[f, le])
+            pa = mesh.xyz[mesh.node_index[int(mesh.edges[e, 0])]]
+            pb = mesh.xyz[mesh.node_index[int(mesh.edges[e, 1])]]
+            acc[e] = acc.get(e, 0) + Et @ (pb - pa)
+            cnt[e] = cnt.get(e, 0) + 1
+    sign = 1.0 if nz >= 0 else -1.0
+    edges = np.array(sorted(acc), np.int64)
+    w = np.array([sign * acc[e] / cnt[e] for e in edges], np.complex128)
+    return OPort(surface_tag=int(mesh.tri_phys[sel[0]]), kc=mode["kc"], Z0=mode["Z0"], beta=mode["beta"],
+                 edges=edges, weights=w)
 
 
 @dataclasses.dataclass

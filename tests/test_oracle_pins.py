@@ -628,3 +628,39 @@ def test_periodic_consistent_mode_is_galerkin_and_reference_mode_is_not():
     assert residual_in_test_space("consistent", np.exp(0.6j)) < 1e-10
     assert residual_in_test_space("reference", np.exp(0.6j)) > 1e-3
     assert residual_in_test_space("reference", 1.0 + 0j) < 1e-10
+
+
+def test_nodal_port_eigens_te10_closed_forms():
+    """solve_port_eigens (port_eigensolve.cpp:97-275) on the z = 0 face of a generated WR-90
+    guide: the first TE mode has kc^2 -> (pi/a)^2 (closed form, within discretisation),
+    Hz proportional to cos(pi x / a), Z0 = omega mu0 / beta with beta from the discrete kc,
+    unit real power; build_wave_port's weights on the y-directed edges follow
+    sin(pi x / a) (the TE10 E-field); x-directed edges carry only the P1 discretisation
+    error of the y-derivative (a few per cent of the largest weight)."""
+    g = mg.box_mesh(mg.WR90_A, mg.WR90_B, 10e-3, 24, 10, 2, seed=1)
+    m = O.mesh_from_arrays(g.node_ids, g.xyz, g.tet_ids, g.tet_phys, g.tri_ids, g.tri_phys)
+    surf = O.extract_surface_mesh(m, 2)
+    om = 2 * math.pi * 10e9
+    modes = O.solve_port_eigens(surf, 2, om, 1.0, 1.0, "TE")
+    md = modes[0]
+    a = mg.WR90_A
+    assert abs(md["kc"] ** 2 / (math.pi / a) ** 2 - 1) < 5e-3
+    x = surf[1][:, 0]
+    ref = np.cos(math.pi * x / a)
+    f = md["field"].real
+    assert abs(f @ ref) / (np.linalg.norm(f) * np.linalg.norm(ref)) > 0.9999
+    k0 = om / O.C0
+    beta = math.sqrt(k0 ** 2 - md["kc"] ** 2)
+    assert abs(md["Z0"] - om * O.MU0 / beta) < 1e-9 * abs(md["Z0"])
+    port = O.build_wave_port(m, surf, md)
+    ev = np.array([m.xyz[m.node_index[int(m.edges[e, 1])]] - m.xyz[m.node_index[int(m.edges[e, 0])]]
+                   for e in port.edges])
+    ym = np.abs(ev[:, 1]) > 1e-12
+    xm = np.abs(ev[:, 0]) > 1e-12
+    assert np.abs(port.weights[xm & ~ym]).max() < 0.05 * np.abs(port.weights).max()
+    cx = np.array([0.5 * (m.xyz[m.node_index[int(m.edges[e, 1])]] + m.xyz[m.node_index[int(m.edges[e, 0])]])[0]
+                   for e in port.edges])
+    sel = ym & ~xm
+    s = np.sin(math.pi * cx[sel] / a) * ev[sel, 1]
+    wv = port.weights[sel]
+    assert abs(np.vdot(wv, s)) / (np.linalg.norm(wv) * np.linalg.norm(s)) > 0.999
