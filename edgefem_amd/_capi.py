@@ -32,6 +32,7 @@ EXPORTS = [
     "efm_port_overlap", "efm_solver_set_spmv_variant", "efm_solver_get_spmv_variant",
     "efm_sweep_create", "efm_sweep_destroy", "efm_sweep_solve", "efm_eps_gradient",
     "efm_solver_set_lowrank", "efm_add_diagonal", "efm_periodic_reduce", "efm_solver_set_gradient_complex",
+    "efm_cells_create", "efm_cells_destroy", "efm_cells_solve",
 ]
 
 
@@ -100,6 +101,10 @@ def load_library(path: str = LIB_PATH) -> C.CDLL:
     lib.efm_solver_set_lowrank.argtypes = [vp, i32, C.c_int64, vp, vp, vp, vp]
     lib.efm_add_diagonal.argtypes = [vp, C.c_int64, vp, vp, vp, vp]
     lib.efm_solver_set_gradient_complex.argtypes = [vp, C.c_int64, vp, vp, vp]
+    lib.efm_cells_create.argtypes = [C.c_int64, vp, vp, vp, vp, vp, vp, vp, i32, vp, vp, vp, vp, vp, C.POINTER(vp)]
+    lib.efm_cells_destroy.argtypes = [vp]
+    lib.efm_cells_destroy.restype = None
+    lib.efm_cells_solve.argtypes = [vp, vp, vp, C.POINTER(SolveOpts), vp, vp, vp]
     lib.efm_periodic_reduce.argtypes = [C.c_int64, vp, vp, vp, C.c_int64, vp, vp, vp, vp, vp, vp, vp, vp,
                                         C.c_int64, C.POINTER(C.c_int64), vp]
     lib.efm_eps_gradient.argtypes = [vp, vp, C.c_int64, vp, i32, vp, i32, vp, i32, vp, vp, vp, vp, vp]
@@ -540,3 +545,57 @@ def periodic_reduce(row_ptr, col_idx, values, master, slave, coef_row, coef_col,
     _check(lib.efm_periodic_reduce(n, dptr(rp), dptr(ci), dptr(v), int(m.size), *args, dptr(rp_out), dptr(col_out),
                                    dptr(val_out), int(nnz.value), C.byref(nnz), stream_ptr()), "efm_periodic_reduce")
     return rp_out, col_out[:nnz.value], val_out[:nnz.value]
+
+
+class CellBatch:
+    """Block-diagonal batch of independent BiCGSTAB systems (efm_cells_*); all arrays are
+    host numpy arrays in global (concatenated) numbering."""
+
+    def __init__(self, row_off, row_ptr, col, val, node_off, edge_nodes, edge_coefs, nrhs, port_off, port_idx,
+                 port_w, port_coef):
+        lib = load_library()
+        self._lib = lib
+        arrs = dict(row_off=np.ascontiguousarray(row_off, np.int64), row_ptr=np.ascontiguousarray(row_ptr, np.int32),
+                    col=np.ascontiguousarray(col, np.int32), val=np.ascontiguousarray(val, np.complex128),
+                    node_off=np.ascontiguousarray(node_off, np.int64),
+                    edge_nodes=np.ascontiguousarray(edge_nodes, np.int32).reshape(-1),
+                    edge_coefs=np.ascontiguousarray(edge_coefs, np.complex128).reshape(-1),
+                    port_off=np.ascontiguousarray(port_off, np.int64), port_idx=np.ascontiguousarray(port_idx, np.int32),
+                    port_w=np.ascontiguousarray(port_w, np.complex128),
+                    port_coef=np.ascontiguousarray(port_coef, np.complex128))
+        self.ncells = arrs["row_off"].size - 1
+        self.n = int(arrs["row_off"][-1])
+        self.nrhs = int(nrhs)
+        ptr = lambda a: a.ctypes.data_as(C.c_void_p)
+        h = C.c_void_p()
+        _check(lib.efm_cells_create(self.ncells, ptr(arrs["row_off"]), ptr(arrs["row_ptr"]), ptr(arrs["col"]),
+                                    ptr(arrs["val"]), ptr(arrs["node_off"]), ptr(arrs["edge_nodes"]),
+                                    ptr(arrs["edge_coefs"]), self.nrhs, ptr(arrs["port_off"]), ptr(arrs["port_idx"]),
+                                    ptr(arrs["port_w"]), ptr(arrs["port_coef"]), stream_ptr(), C.byref(h)),
+               "efm_cells_create")
+        self._h = h
+
+    def solve(self, B, tol=1e-10, max_iterations=10000, check_every=32):
+        """B: (n, nrhs) complex128 device tensor.  Returns (X, iters, resid)."""
+        torch = _torch()
+        Bd = to_dev(B, torch.complex128)
+        if Bd.numel() != self.n * self.nrhs:
+            raise EdgeFemError("right-hand side has the wrong size")
+        X = torch.empty_like(Bd)
+        iters = np.zeros(self.ncells * self.nrhs, np.int32)
+        resid = np.zeros(self.ncells * self.nrhs, np.float64)
+        opts = SolveOpts(EFM_METHOD_BICGSTAB, float(tol), int(max_iterations), int(check_every))
+        _check(self._lib.efm_cells_solve(self._h, dptr(Bd), dptr(X), C.byref(opts), iters.ctypes.data_as(C.c_void_p),
+                                         resid.ctypes.data_as(C.c_void_p), stream_ptr()), "efm_cells_solve")
+        return X, iters.reshape(self.ncells, self.nrhs), resid.reshape(self.ncells, self.nrhs)
+
+    def close(self):
+        if getattr(self, "_h", None):
+            self._lib.efm_cells_destroy(self._h)
+            self._h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass

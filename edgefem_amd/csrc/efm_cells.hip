@@ -1,0 +1,805 @@
+// Batched BiCGSTAB over many independent systems of different sizes (periodic unit cells):
+// the reduced matrices of all cells are concatenated into one block-diagonal CSR, and
+// every cell carries R excitations.  Each (cell, excitation) pair is its own Krylov
+// system with its own alpha, beta, omega, rho and convergence flag.
+//
+// Row tiling: every block owns a contiguous row range inside one cell, so the partial
+// sums of a block belong to one cell and the finalisation of cell c reduces the
+// contiguous block range of c (plus one slot per port term of c).  Nodes of the
+// auxiliary preconditioner are tiled the same way.
+//
+// Iteration (van der Vorst 1992, right-preconditioned with P = D^{-1} + G Dn^{-1} G^T,
+// the periodic auxiliary preconditioner with a complex incidence; the port loading
+// sum_p c_p w_p w_p^H of every cell is applied after each SpMV):
+//   rho_i = r0^H r; beta = (rho_i/rho_{i-1})(alpha/omega); p = r + beta (p - omega v)
+//   y = P p; v = A y; alpha = rho_i/(r0^H v); s = r - alpha v; zz = P s; t = A zz;
+//   omega = (t^H s)/(t^H t); x += alpha y + omega zz; r = s - omega t
+// This is the many-cell form of calculate_sparams_periodic (src/assemble_maxwell.cpp:
+// 576-635) called per unit cell by UnitCellDesign (python/edgefem/designs/
+// unit_cell.py:700-777); the single-cell path is efm_solver_* with the same kernels'
+// formulas.
+#include "efm_internal.h"
+
+#include <algorithm>
+#include <vector>
+
+using namespace efm;
+
+namespace {
+
+constexpr int kBlock = 256;
+constexpr int kRowsPerTile = 1024;
+constexpr int kLanes = 8;  // SpMV lanes per row
+constexpr int kMaxR = 2;
+
+struct CellSys {
+  double2 rho, alpha, beta, omega;
+  double bnorm2, rnorm2;
+  int active, iters, breakdown, pad;
+};
+
+struct Tile {
+  int32_t cell;
+  int32_t pad;
+  int64_t r0, r1;
+};
+
+__device__ inline double2 bil(double2 a, double2 b) { return cmul(a, b); }
+
+// block sum of V complex values; result in lds[0..V) for all threads
+template <int V>
+__device__ inline void bsum(double2 (&v)[V], double2 *lds) {
+#pragma unroll
+  for (int s = 0; s < V; ++s) {
+#pragma unroll
+    for (int off = 32; off > 0; off >>= 1) {
+      v[s].x += __shfl_xor(v[s].x, off, 64);
+      v[s].y += __shfl_xor(v[s].y, off, 64);
+    }
+  }
+  const int w = threadIdx.x >> 6, lane = threadIdx.x & 63;
+  if (lane == 0)
+#pragma unroll
+    for (int s = 0; s < V; ++s)
+      lds[w * V + s] = v[s];
+  __syncthreads();
+  if (threadIdx.x == 0)
+#pragma unroll
+    for (int s = 0; s < V; ++s) {
+      double2 a = lds[s];
+      for (int k = 1; k < kBlock / 64; ++k)
+        a = cadd(a, lds[k * V + s]);
+      lds[s] = a;
+    }
+  __syncthreads();
+}
+
+template <int V>
+__device__ inline void put_partials(double2 (&v)[V], double2 *lds, double2 *partial, int64_t slot) {
+  bsum<V>(v, lds);
+  if (threadIdx.x < V)
+    partial[slot * V + threadIdx.x] = lds[threadIdx.x];
+}
+
+enum : int { kInit = 0, kRho0 = 1, kAlpha = 2, kOmega = 3, kX = 4 };
+
+// One block per cell: reduce the cell's partial slots (row tiles [b0, b1) and port slots
+// [pb0, pb1)) and advance the R systems of the cell.
+template <int R, int STAGE>
+__global__ void __launch_bounds__(kBlock)
+    k_cells_finalize(int64_t ncells, const int32_t *__restrict__ cell_b0, int npt, int64_t port_base,
+                     const double2 *__restrict__ partial, CellSys *__restrict__ st, int *__restrict__ flags,
+                     double tol2, int max_it) {
+  constexpr int V = (STAGE == kOmega || STAGE == kX) ? 2 * R : R;
+  __shared__ double2 lds[(kBlock / 64) * 2 * kMaxR];
+  const int64_t c = blockIdx.x;
+  if (c >= ncells)
+    return;
+  double2 v[V];
+#pragma unroll
+  for (int s = 0; s < V; ++s)
+    v[s] = make_double2(0, 0);
+  for (int32_t b = cell_b0[c] + threadIdx.x; b < cell_b0[c + 1]; b += kBlock)
+#pragma unroll
+    for (int s = 0; s < V; ++s)
+      v[s] = cadd(v[s], partial[(int64_t)b * V + s]);
+  if (STAGE == kAlpha || STAGE == kOmega)
+    for (int p = threadIdx.x; p < npt; p += kBlock)
+#pragma unroll
+      for (int s = 0; s < V; ++s)
+        v[s] = cadd(v[s], partial[(port_base + c * npt + p) * V + s]);
+  bsum<V>(v, lds);
+  if (threadIdx.x != 0)
+    return;
+#pragma unroll
+  for (int s = 0; s < R; ++s) {
+    CellSys &S = st[c * R + s];
+    const double2 tot = lds[s];
+    if (STAGE == kInit) {
+      S.bnorm2 = tot.x;
+      S.rnorm2 = tot.x;
+      S.rho = make_double2(1, 0);
+      S.alpha = make_double2(1, 0);
+      S.omega = make_double2(1, 0);
+      S.beta = make_double2(0, 0);
+      S.iters = 0;
+      S.breakdown = 0;
+      S.active = tot.x > 0.0 ? 1 : 0;
+    } else if (!S.active) {
+    } else if (STAGE == kRho0) {
+      if (tot.x == 0.0 && tot.y == 0.0) {
+        S.breakdown = 1;
+        S.active = 0;
+      } else {
+        S.beta = cmul(cdiv(tot, S.rho), cdiv(S.alpha, S.omega));
+        S.rho = tot;
+      }
+    } else if (STAGE == kAlpha) {
+      if (tot.x == 0.0 && tot.y == 0.0) {
+        S.breakdown = 1;
+        S.active = 0;
+      } else {
+        S.alpha = cdiv(S.rho, tot);
+      }
+    } else if (STAGE == kOmega) {
+      const double tt = lds[R + s].x;
+      S.omega = (tt > 0.0) ? cscale(tot, 1.0 / tt) : make_double2(0, 0);
+    } else if (STAGE == kX) {
+      S.iters += 1;
+      S.rnorm2 = lds[R + s].x;
+      if (S.rnorm2 <= tol2 * S.bnorm2) {
+        S.active = 0;
+      } else if (S.iters >= max_it || (S.omega.x == 0.0 && S.omega.y == 0.0)) {
+        S.active = 0;
+      } else if (tot.x == 0.0 && tot.y == 0.0) {
+        S.breakdown = 1;
+        S.active = 0;
+      } else {
+        S.beta = cmul(cdiv(tot, S.rho), cdiv(S.alpha, S.omega));
+        S.rho = tot;
+      }
+    }
+    flags[c * R + s] = S.active;
+  }
+}
+
+// x = 0, r = r0 = b, p = v = 0; partials |b|^2
+template <int R>
+__global__ void __launch_bounds__(kBlock)
+    k_cells_init(const Tile *__restrict__ tiles, const double2 *__restrict__ b, double2 *__restrict__ x,
+                 double2 *__restrict__ r, double2 *__restrict__ r0, double2 *__restrict__ p,
+                 double2 *__restrict__ v, double2 *__restrict__ partial) {
+  __shared__ double2 lds[(kBlock / 64) * 2 * kMaxR];
+  const Tile t = tiles[blockIdx.x];
+  double2 d[R];
+#pragma unroll
+  for (int s = 0; s < R; ++s)
+    d[s] = make_double2(0, 0);
+  for (int64_t i = t.r0 + threadIdx.x; i < t.r1; i += kBlock)
+#pragma unroll
+    for (int s = 0; s < R; ++s) {
+      const int64_t k = i * R + s;
+      const double2 bv = b[k];
+      x[k] = make_double2(0, 0);
+      r[k] = bv;
+      r0[k] = bv;
+      p[k] = make_double2(0, 0);
+      v[k] = make_double2(0, 0);
+      d[s].x += cabs2(bv);
+    }
+  put_partials<R>(d, lds, partial, blockIdx.x);
+}
+
+template <int R>
+__global__ void __launch_bounds__(kBlock)
+    k_cells_dot_r0r(const Tile *__restrict__ tiles, const double2 *__restrict__ r0, const double2 *__restrict__ r,
+                    double2 *__restrict__ partial) {
+  __shared__ double2 lds[(kBlock / 64) * 2 * kMaxR];
+  const Tile t = tiles[blockIdx.x];
+  double2 d[R];
+#pragma unroll
+  for (int s = 0; s < R; ++s)
+    d[s] = make_double2(0, 0);
+  for (int64_t i = t.r0 + threadIdx.x; i < t.r1; i += kBlock)
+#pragma unroll
+    for (int s = 0; s < R; ++s)
+      d[s] = cfma(cconj(r0[i * R + s]), r[i * R + s], d[s]);
+  put_partials<R>(d, lds, partial, blockIdx.x);
+}
+
+// p = r + beta (p - omega v)
+template <int R>
+__global__ void __launch_bounds__(kBlock)
+    k_cells_p(const Tile *__restrict__ tiles, const double2 *__restrict__ r, const double2 *__restrict__ v,
+              double2 *__restrict__ p, const CellSys *__restrict__ st) {
+  const Tile t = tiles[blockIdx.x];
+  bool act[R];
+  double2 beta[R], omega[R];
+#pragma unroll
+  for (int s = 0; s < R; ++s) {
+    const CellSys &S = st[(int64_t)t.cell * R + s];
+    act[s] = S.active != 0;
+    beta[s] = S.beta;
+    omega[s] = S.omega;
+  }
+  for (int64_t i = t.r0 + threadIdx.x; i < t.r1; i += kBlock)
+#pragma unroll
+    for (int s = 0; s < R; ++s) {
+      if (!act[s])
+        continue;
+      const int64_t k = i * R + s;
+      p[k] = cadd(r[k], bil(beta[s], csub(p[k], bil(omega[s], v[k]))));
+    }
+}
+
+// nodal part of P: y = Dn^{-1} G^T in  (node tiles)
+template <int R>
+__global__ void __launch_bounds__(kBlock)
+    k_cells_aux_node(const Tile *__restrict__ ntiles, const int32_t *__restrict__ gt_ptr,
+                     const int32_t *__restrict__ gt_edge, const double2 *__restrict__ gt_coef,
+                     const double2 *__restrict__ dninv, const double2 *__restrict__ in, double2 *__restrict__ y,
+                     const CellSys *__restrict__ st) {
+  const Tile t = ntiles[blockIdx.x];
+  bool act[R];
+#pragma unroll
+  for (int s = 0; s < R; ++s)
+    act[s] = st[(int64_t)t.cell * R + s].active != 0;
+  for (int64_t nd = t.r0 + threadIdx.x; nd < t.r1; nd += kBlock) {
+    double2 acc[R];
+#pragma unroll
+    for (int s = 0; s < R; ++s)
+      acc[s] = make_double2(0, 0);
+    for (int32_t q = gt_ptr[nd]; q < gt_ptr[nd + 1]; ++q) {
+      const int64_t e = gt_edge[q];
+      const double2 g = gt_coef[q];
+#pragma unroll
+      for (int s = 0; s < R; ++s)
+        acc[s] = cfma(g, in[e * R + s], acc[s]);
+    }
+    const double2 di = dninv[nd];
+#pragma unroll
+    for (int s = 0; s < R; ++s)
+      if (act[s])
+        y[nd * R + s] = cmul(di, acc[s]);
+  }
+}
+
+// out = D^{-1} in + G y  (row tiles)
+template <int R>
+__global__ void __launch_bounds__(kBlock)
+    k_cells_aux_edge(const Tile *__restrict__ tiles, const int2 *__restrict__ en2, const double2 *__restrict__ ga,
+                     const double2 *__restrict__ gb, const double2 *__restrict__ dinv, const double2 *__restrict__ y,
+                     const double2 *__restrict__ in, double2 *__restrict__ out, const CellSys *__restrict__ st) {
+  const Tile t = tiles[blockIdx.x];
+  bool act[R];
+#pragma unroll
+  for (int s = 0; s < R; ++s)
+    act[s] = st[(int64_t)t.cell * R + s].active != 0;
+  for (int64_t e = t.r0 + threadIdx.x; e < t.r1; e += kBlock) {
+    const int2 nn = en2[e];
+    const double2 di = dinv[e];
+#pragma unroll
+    for (int s = 0; s < R; ++s) {
+      if (!act[s])
+        continue;
+      double2 v = cmul(di, in[e * R + s]);
+      if (nn.x >= 0)
+        v = cfma(ga[e], y[(int64_t)nn.x * R + s], v);
+      if (nn.y >= 0)
+        v = cfma(gb[e], y[(int64_t)nn.y * R + s], v);
+      out[e * R + s] = v;
+    }
+  }
+}
+
+// out = A in (block-diagonal CSR, kLanes lanes per row); MODE 0: partials r0^H out;
+// MODE 1: partials out^H other, |out|^2
+template <int R, int MODE>
+__global__ void __launch_bounds__(kBlock)
+    k_cells_spmv(const Tile *__restrict__ tiles, const int32_t *__restrict__ row_ptr, const int32_t *__restrict__ col,
+                 const double2 *__restrict__ val, const double2 *__restrict__ in, double2 *__restrict__ out,
+                 const double2 *__restrict__ other, double2 *__restrict__ partial, const CellSys *__restrict__ st) {
+  constexpr int V = MODE == 0 ? R : 2 * R;
+  __shared__ double2 lds[(kBlock / 64) * 2 * kMaxR];
+  const Tile t = tiles[blockIdx.x];
+  bool act[R];
+#pragma unroll
+  for (int s = 0; s < R; ++s)
+    act[s] = st[(int64_t)t.cell * R + s].active != 0;
+  const int g = threadIdx.x / kLanes, lane = threadIdx.x % kLanes;
+  double2 d[V];
+#pragma unroll
+  for (int s = 0; s < V; ++s)
+    d[s] = make_double2(0, 0);
+  for (int64_t row0 = t.r0; row0 < t.r1; row0 += kBlock / kLanes) {
+    const int64_t row = row0 + g;
+    double2 acc[R];
+#pragma unroll
+    for (int s = 0; s < R; ++s)
+      acc[s] = make_double2(0, 0);
+    if (row < t.r1)
+      for (int32_t k = row_ptr[row] + lane; k < row_ptr[row + 1]; k += kLanes) {
+        const double2 a = val[k];
+        const int64_t c = col[k];
+#pragma unroll
+        for (int s = 0; s < R; ++s)
+          acc[s] = cfma(a, in[c * R + s], acc[s]);
+      }
+#pragma unroll
+    for (int s = 0; s < R; ++s)
+#pragma unroll
+      for (int off = kLanes / 2; off > 0; off >>= 1) {
+        acc[s].x += __shfl_xor(acc[s].x, off, kLanes);
+        acc[s].y += __shfl_xor(acc[s].y, off, kLanes);
+      }
+    if (row < t.r1 && lane == 0)
+#pragma unroll
+      for (int s = 0; s < R; ++s) {
+        if (!act[s])
+          continue;
+        const int64_t k = row * R + s;
+        out[k] = acc[s];
+        const double2 o = other[k];
+        if (MODE == 0) {
+          d[s] = cfma(cconj(o), acc[s], d[s]);
+        } else {
+          d[s] = cfma(cconj(acc[s]), o, d[s]);
+          d[R + s].x += cabs2(acc[s]);
+        }
+      }
+  }
+  put_partials<V>(d, lds, partial, blockIdx.x);
+}
+
+// Port terms, one block per term q (cell pcell[q], excitation-independent):
+//   t_s = w_q^H in_s;  out_s += c_q w_q t_s on the term's edges; the change of the SpMV's
+// dot products (MODE 0: r0^H dv; MODE 1: dt^H s and |t|^2) into the term's partial slot.
+template <int R, int MODE>
+__global__ void __launch_bounds__(kBlock)
+    k_cells_ports(const int64_t *__restrict__ poff, const int32_t *__restrict__ pidx, const double2 *__restrict__ pw,
+                  const double2 *__restrict__ pcoef, const int32_t *__restrict__ pcell, const double2 *__restrict__ in,
+                  double2 *__restrict__ out, const double2 *__restrict__ other, double2 *__restrict__ partial,
+                  int64_t slot_base, const CellSys *__restrict__ st) {
+  constexpr int V = MODE == 0 ? R : 2 * R;
+  __shared__ double2 lds[(kBlock / 64) * 2 * kMaxR];
+  const int64_t q = blockIdx.x;
+  const int64_t e0 = poff[q], e1 = poff[q + 1];
+  const int64_t cell = pcell[q];
+  bool act[R];
+#pragma unroll
+  for (int s = 0; s < R; ++s)
+    act[s] = st[cell * R + s].active != 0;
+  double2 tv[R];
+#pragma unroll
+  for (int s = 0; s < R; ++s)
+    tv[s] = make_double2(0, 0);
+  for (int64_t i = e0 + threadIdx.x; i < e1; i += kBlock) {
+    const double2 w = cconj(pw[i]);
+    const int64_t r = pidx[i];
+#pragma unroll
+    for (int s = 0; s < R; ++s)
+      tv[s] = cfma(w, in[r * R + s], tv[s]);
+  }
+  bsum<R>(tv, lds);
+  double2 t[R];
+#pragma unroll
+  for (int s = 0; s < R; ++s)
+    t[s] = lds[s];
+  __syncthreads();
+  double2 c[V];
+#pragma unroll
+  for (int s = 0; s < V; ++s)
+    c[s] = make_double2(0, 0);
+  const double2 cq = pcoef[q];
+  for (int64_t i = e0 + threadIdx.x; i < e1; i += kBlock) {
+    const int64_t r = pidx[i];
+    const double2 cw = cmul(cq, pw[i]);
+#pragma unroll
+    for (int s = 0; s < R; ++s) {
+      if (!act[s])
+        continue;
+      const int64_t k = r * R + s;
+      const double2 d = cmul(cw, t[s]);
+      const double2 o = out[k];
+      const double2 nw = cadd(o, d);
+      out[k] = nw;
+      if (MODE == 0) {
+        c[s] = cfma(cconj(other[k]), d, c[s]);
+      } else {
+        c[s] = cfma(cconj(d), other[k], c[s]);
+        c[R + s].x += cabs2(nw) - cabs2(o);
+      }
+    }
+  }
+  put_partials<V>(c, lds, partial, slot_base + q);
+}
+
+// s = r - alpha v (in r)
+template <int R>
+__global__ void __launch_bounds__(kBlock)
+    k_cells_s(const Tile *__restrict__ tiles, const double2 *__restrict__ v, double2 *__restrict__ r,
+              const CellSys *__restrict__ st) {
+  const Tile t = tiles[blockIdx.x];
+  bool act[R];
+  double2 alpha[R];
+#pragma unroll
+  for (int s = 0; s < R; ++s) {
+    const CellSys &S = st[(int64_t)t.cell * R + s];
+    act[s] = S.active != 0;
+    alpha[s] = S.alpha;
+  }
+  for (int64_t i = t.r0 + threadIdx.x; i < t.r1; i += kBlock)
+#pragma unroll
+    for (int s = 0; s < R; ++s)
+      if (act[s]) {
+        const int64_t k = i * R + s;
+        r[k] = csub(r[k], bil(alpha[s], v[k]));
+      }
+}
+
+// x += alpha y + omega zz; r = s - omega t; partials r0^H r, |r|^2
+template <int R>
+__global__ void __launch_bounds__(kBlock)
+    k_cells_x(const Tile *__restrict__ tiles, const double2 *__restrict__ y, const double2 *__restrict__ zz,
+              const double2 *__restrict__ tvec, const double2 *__restrict__ r0, double2 *__restrict__ x,
+              double2 *__restrict__ r, double2 *__restrict__ partial, const CellSys *__restrict__ st) {
+  __shared__ double2 lds[(kBlock / 64) * 2 * kMaxR];
+  const Tile t = tiles[blockIdx.x];
+  bool act[R];
+  double2 alpha[R], omega[R];
+#pragma unroll
+  for (int s = 0; s < R; ++s) {
+    const CellSys &S = st[(int64_t)t.cell * R + s];
+    act[s] = S.active != 0;
+    alpha[s] = S.alpha;
+    omega[s] = S.omega;
+  }
+  double2 d[2 * R];
+#pragma unroll
+  for (int s = 0; s < 2 * R; ++s)
+    d[s] = make_double2(0, 0);
+  for (int64_t i = t.r0 + threadIdx.x; i < t.r1; i += kBlock)
+#pragma unroll
+    for (int s = 0; s < R; ++s) {
+      if (!act[s])
+        continue;
+      const int64_t k = i * R + s;
+      x[k] = cadd(x[k], cadd(bil(alpha[s], y[k]), bil(omega[s], zz[k])));
+      const double2 rv = csub(r[k], bil(omega[s], tvec[k]));
+      r[k] = rv;
+      d[s] = cfma(cconj(r0[k]), rv, d[s]);
+      d[R + s].x += cabs2(rv);
+    }
+  put_partials<2 * R>(d, lds, partial, blockIdx.x);
+}
+
+// Jacobi diagonal with the port terms (ports disjoint): a_rr + c_q |w|^2 on port rows
+__global__ void k_cells_dinv(int64_t n, const int32_t *__restrict__ row_ptr, const int32_t *__restrict__ col,
+                             const double2 *__restrict__ val, const double2 *__restrict__ extra,
+                             double2 *__restrict__ dinv) {
+  const int64_t r = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (r >= n)
+    return;
+  double2 d = extra[r];
+  for (int32_t k = row_ptr[r]; k < row_ptr[r + 1]; ++k)
+    if (col[k] == r)
+      d = cadd(d, val[k]);
+  dinv[r] = (d.x == 0.0 && d.y == 0.0) ? make_double2(1, 0) : cdiv(make_double2(1, 0), d);
+}
+
+// diag(G^T A G) per node (the complex incidence of the periodic auxiliary space)
+__global__ void k_cells_dn(int64_t N, const int32_t *__restrict__ gt_ptr, const int32_t *__restrict__ gt_edge,
+                           const double2 *__restrict__ gt_coef, const int2 *__restrict__ en2,
+                           const double2 *__restrict__ ga, const double2 *__restrict__ gb,
+                           const int32_t *__restrict__ row_ptr, const int32_t *__restrict__ col,
+                           const double2 *__restrict__ val, double2 *__restrict__ dninv) {
+  const int64_t nd = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (nd >= N)
+    return;
+  double2 sum = make_double2(0, 0);
+  for (int32_t q = gt_ptr[nd]; q < gt_ptr[nd + 1]; ++q) {
+    const int32_t e = gt_edge[q];
+    double2 inner = make_double2(0, 0);
+    for (int32_t k = row_ptr[e]; k < row_ptr[e + 1]; ++k) {
+      const int32_t f = col[k];
+      const int2 nn = en2[f];
+      double2 cf = make_double2(0, 0);
+      if (nn.x == nd)
+        cf = cadd(cf, ga[f]);
+      if (nn.y == nd)
+        cf = cadd(cf, gb[f]);
+      if (cf.x != 0.0 || cf.y != 0.0)
+        inner = cfma(val[k], cf, inner);
+    }
+    sum = cfma(gt_coef[q], inner, sum);
+  }
+  dninv[nd] = (sum.x == 0.0 && sum.y == 0.0) ? make_double2(0, 0) : cdiv(make_double2(1, 0), sum);
+}
+
+} // namespace
+
+struct efm_cells {
+  int64_t ncells = 0, n = 0, nnz = 0, N = 0;
+  int R = 2;
+  int npt = 0;  // port terms per cell
+  int64_t ntiles = 0, nntiles = 0, nterms = 0;
+  efm::DevBuf row_ptr, col, val, dinv, tiles, ntiles_buf, cell_b0;
+  efm::DevBuf en2, ga, gb, gt_ptr, gt_edge, gt_coef, dninv, ynode;
+  efm::DevBuf poff, pidx, pw, pcoef, pcell;
+  efm::DevBuf x, r, r0, p, v, y, zz, t, partial, state, flags;
+};
+
+namespace {
+template <typename T> int upload(efm::DevBuf &d, const T *h, size_t count, hipStream_t st) {
+  int rc = d.alloc(std::max<size_t>(count, 1) * sizeof(T));
+  if (rc)
+    return rc;
+  if (count)
+    EFM_HIP_CHECK(hipMemcpyAsync(d.p, h, count * sizeof(T), hipMemcpyHostToDevice, st));
+  return EFM_OK;
+}
+
+std::vector<Tile> make_tiles(const int64_t *off, int64_t ncells, std::vector<int32_t> *first) {
+  std::vector<Tile> t;
+  if (first)
+    first->assign(ncells + 1, 0);
+  for (int64_t c = 0; c < ncells; ++c) {
+    if (first)
+      (*first)[c] = (int32_t)t.size();
+    for (int64_t r = off[c]; r < off[c + 1]; r += kRowsPerTile)
+      t.push_back(Tile{(int32_t)c, 0, r, std::min<int64_t>(off[c + 1], r + kRowsPerTile)});
+  }
+  if (first)
+    (*first)[ncells] = (int32_t)t.size();
+  return t;
+}
+} // namespace
+
+extern "C" int efm_cells_create(int64_t ncells, const int64_t *h_row_off, const int32_t *h_row_ptr,
+                                const int32_t *h_col, const double *h_val, const int64_t *h_node_off,
+                                const int32_t *h_edge_nodes, const double *h_edge_coefs, int nrhs,
+                                const int64_t *h_port_off, const int32_t *h_port_idx, const double *h_port_w,
+                                const double *h_port_coef, void *stream, efm_cells **out) {
+  EFM_REQUIRE(out, "efm_cells_create: out is NULL");
+  *out = nullptr;
+  EFM_REQUIRE(ncells >= 1 && h_row_off && h_row_ptr && h_col && h_val && h_node_off && h_edge_nodes &&
+                  h_edge_coefs && h_port_off,
+              "efm_cells_create: NULL argument");
+  EFM_REQUIRE(nrhs == 1 || nrhs == 2, "efm_cells_create: 1 or 2 excitations per cell");
+  hipStream_t st = (hipStream_t)stream;
+  auto s = std::make_unique<efm_cells>();
+  s->ncells = ncells;
+  s->R = nrhs;
+  s->npt = nrhs;
+  s->n = h_row_off[ncells];
+  s->nnz = h_row_ptr[s->n];
+  s->N = h_node_off[ncells];
+  const int64_t n = s->n, N = s->N;
+  s->nterms = ncells * s->npt;
+  // validation of the block-diagonal structure (every column inside its cell)
+  for (int64_t c = 0; c < ncells; ++c) {
+    EFM_REQUIRE(h_row_off[c + 1] >= h_row_off[c] && h_node_off[c + 1] >= h_node_off[c],
+                "efm_cells_create: offsets must be nondecreasing");
+    for (int64_t r = h_row_off[c]; r < h_row_off[c + 1]; ++r)
+      for (int32_t k = h_row_ptr[r]; k < h_row_ptr[r + 1]; ++k)
+        EFM_REQUIRE(h_col[k] >= h_row_off[c] && h_col[k] < h_row_off[c + 1],
+                    "efm_cells_create: a column leaves its cell");
+  }
+  // gradient incidence (global node indices) and its transpose
+  std::vector<int2> en2(n);
+  std::vector<double2> ga(n), gb(n);
+  std::vector<int32_t> cnt(N + 1, 0);
+  for (int64_t e = 0; e < n; ++e) {
+    const int32_t a = h_edge_nodes[2 * e], b = h_edge_nodes[2 * e + 1];
+    EFM_REQUIRE(a >= -1 && a < N && b >= -1 && b < N, "efm_cells_create: node index out of range");
+    en2[e] = make_int2(a, b);
+    ga[e] = make_double2(h_edge_coefs[4 * e], h_edge_coefs[4 * e + 1]);
+    gb[e] = make_double2(h_edge_coefs[4 * e + 2], h_edge_coefs[4 * e + 3]);
+    if (a >= 0)
+      ++cnt[a + 1];
+    if (b >= 0 && b != a)
+      ++cnt[b + 1];
+  }
+  for (int64_t k = 0; k < N; ++k)
+    cnt[k + 1] += cnt[k];
+  std::vector<int32_t> gte(std::max<int32_t>(cnt[N], 1));
+  std::vector<double2> gtc(std::max<int32_t>(cnt[N], 1));
+  std::vector<int32_t> fill(cnt.begin(), cnt.end() - 1);
+  for (int64_t e = 0; e < n; ++e) {
+    const int2 nn = en2[e];
+    if (nn.x >= 0) {
+      gte[fill[nn.x]] = (int32_t)e;
+      gtc[fill[nn.x]++] = (nn.y == nn.x) ? cadd(ga[e], gb[e]) : ga[e];
+    }
+    if (nn.y >= 0 && nn.y != nn.x) {
+      gte[fill[nn.y]] = (int32_t)e;
+      gtc[fill[nn.y]++] = gb[e];
+    }
+  }
+  // port terms: q = c * npt + p, entries [poff[q], poff[q+1]); disjoint rows
+  const int64_t npe = h_port_off[s->nterms];
+  std::vector<double2> extra(n, make_double2(0, 0));
+  std::vector<int32_t> pcell(s->nterms);
+  std::vector<uint8_t> seen(n, 0);
+  for (int64_t q = 0; q < s->nterms; ++q) {
+    pcell[q] = (int32_t)(q / s->npt);
+    const double2 cq = make_double2(h_port_coef[2 * q], h_port_coef[2 * q + 1]);
+    for (int64_t i = h_port_off[q]; i < h_port_off[q + 1]; ++i) {
+      const int32_t r = h_port_idx[i];
+      EFM_REQUIRE(r >= h_row_off[pcell[q]] && r < h_row_off[pcell[q] + 1], "efm_cells_create: port edge outside its cell");
+      EFM_REQUIRE(!seen[r], "efm_cells_create: port terms must be disjoint");
+      seen[r] = 1;
+      const double2 w = make_double2(h_port_w[2 * i], h_port_w[2 * i + 1]);
+      extra[r] = cscale(cq, w.x * w.x + w.y * w.y);
+    }
+  }
+  std::vector<int32_t> first;
+  std::vector<Tile> tiles = make_tiles(h_row_off, ncells, &first);
+  std::vector<Tile> ntl = make_tiles(h_node_off, ncells, nullptr);
+  s->ntiles = (int64_t)tiles.size();
+  s->nntiles = (int64_t)ntl.size();
+  int rc;
+  const size_t vb = (size_t)n * s->R * sizeof(double2);
+  if ((rc = upload(s->row_ptr, h_row_ptr, n + 1, st)) || (rc = upload(s->col, h_col, s->nnz, st)) ||
+      (rc = upload(s->val, (const double2 *)h_val, s->nnz, st)) || (rc = upload(s->en2, en2.data(), n, st)) ||
+      (rc = upload(s->ga, ga.data(), n, st)) || (rc = upload(s->gb, gb.data(), n, st)) ||
+      (rc = upload(s->gt_ptr, cnt.data(), N + 1, st)) || (rc = upload(s->gt_edge, gte.data(), gte.size(), st)) ||
+      (rc = upload(s->gt_coef, gtc.data(), gtc.size(), st)) ||
+      (rc = upload(s->tiles, tiles.data(), tiles.size(), st)) ||
+      (rc = upload(s->ntiles_buf, ntl.data(), ntl.size(), st)) ||
+      (rc = upload(s->cell_b0, first.data(), first.size(), st)) ||
+      (rc = upload(s->poff, h_port_off, s->nterms + 1, st)) || (rc = upload(s->pidx, h_port_idx, npe, st)) ||
+      (rc = upload(s->pw, (const double2 *)h_port_w, npe, st)) ||
+      (rc = upload(s->pcoef, (const double2 *)h_port_coef, s->nterms, st)) ||
+      (rc = upload(s->pcell, pcell.data(), pcell.size(), st)))
+    return rc;
+  DevBuf ext;
+  if ((rc = upload(ext, extra.data(), n, st)) || (rc = s->dinv.alloc(n * sizeof(double2))) ||
+      (rc = s->dninv.alloc(std::max<int64_t>(N, 1) * sizeof(double2))) ||
+      (rc = s->ynode.alloc(std::max<int64_t>(N, 1) * s->R * sizeof(double2))) || (rc = s->x.alloc(vb)) ||
+      (rc = s->r.alloc(vb)) || (rc = s->r0.alloc(vb)) || (rc = s->p.alloc(vb)) || (rc = s->v.alloc(vb)) ||
+      (rc = s->y.alloc(vb)) || (rc = s->zz.alloc(vb)) || (rc = s->t.alloc(vb)) ||
+      (rc = s->partial.alloc((size_t)(s->ntiles + s->nterms) * 2 * kMaxR * sizeof(double2))) ||
+      (rc = s->state.alloc((size_t)ncells * s->R * sizeof(CellSys))) ||
+      (rc = s->flags.alloc((size_t)ncells * s->R * sizeof(int))))
+    return rc;
+  hipLaunchKernelGGL(k_cells_dinv, dim3(grid_for(n, 256)), dim3(256), 0, st, n, s->row_ptr.as<int32_t>(),
+                     s->col.as<int32_t>(), s->val.as<double2>(), ext.as<double2>(), s->dinv.as<double2>());
+  EFM_LAUNCH_CHECK();
+  if (N > 0) {
+    hipLaunchKernelGGL(k_cells_dn, dim3(grid_for(N, 256)), dim3(256), 0, st, N, s->gt_ptr.as<int32_t>(),
+                       s->gt_edge.as<int32_t>(), s->gt_coef.as<double2>(), s->en2.as<int2>(), s->ga.as<double2>(),
+                       s->gb.as<double2>(), s->row_ptr.as<int32_t>(), s->col.as<int32_t>(), s->val.as<double2>(),
+                       s->dninv.as<double2>());
+    EFM_LAUNCH_CHECK();
+  }
+  EFM_HIP_CHECK(hipStreamSynchronize(st));
+  *out = s.release();
+  return EFM_OK;
+}
+
+extern "C" void efm_cells_destroy(efm_cells *s) { delete s; }
+
+namespace {
+template <int R> int cells_solve(efm_cells *s, const double2 *B, double2 *X, const efm_solve_opts *o,
+                                 int32_t *h_iters, double *h_resid, hipStream_t st) {
+  const Tile *T = s->tiles.as<Tile>();
+  const Tile *NT = s->ntiles_buf.as<Tile>();
+  CellSys *S = s->state.as<CellSys>();
+  double2 *part = s->partial.as<double2>();
+  const int nt = (int)s->ntiles, nnt = (int)s->nntiles, nc = (int)s->ncells;
+  const double tol2 = o->tolerance * o->tolerance;
+  const int max_it = o->max_iterations;
+  const int check = o->check_every > 0 ? o->check_every : 32;
+  double2 *x = s->x.as<double2>(), *r = s->r.as<double2>(), *r0 = s->r0.as<double2>(), *p = s->p.as<double2>();
+  double2 *v = s->v.as<double2>(), *y = s->y.as<double2>(), *zz = s->zz.as<double2>(), *t = s->t.as<double2>();
+  auto fin = [&](auto stage_tag) -> int {
+    constexpr int STG = decltype(stage_tag)::value;
+    hipLaunchKernelGGL((k_cells_finalize<R, STG>), dim3(nc), dim3(kBlock), 0, st, (int64_t)nc,
+                       s->cell_b0.as<int32_t>(), s->npt, (int64_t)s->ntiles, part, S, s->flags.as<int>(), tol2,
+                       max_it);
+    EFM_LAUNCH_CHECK();
+    return EFM_OK;
+  };
+  auto precond = [&](const double2 *in, double2 *outv) -> int {
+    if (nnt > 0) {
+      hipLaunchKernelGGL((k_cells_aux_node<R>), dim3(nnt), dim3(kBlock), 0, st, NT, s->gt_ptr.as<int32_t>(),
+                         s->gt_edge.as<int32_t>(), s->gt_coef.as<double2>(), s->dninv.as<double2>(), in,
+                         s->ynode.as<double2>(), S);
+      EFM_LAUNCH_CHECK();
+    }
+    hipLaunchKernelGGL((k_cells_aux_edge<R>), dim3(nt), dim3(kBlock), 0, st, T, s->en2.as<int2>(),
+                       s->ga.as<double2>(), s->gb.as<double2>(), s->dinv.as<double2>(), s->ynode.as<double2>(), in,
+                       outv, S);
+    EFM_LAUNCH_CHECK();
+    return EFM_OK;
+  };
+  auto ports = [&](auto mode_tag, const double2 *in, double2 *outv, const double2 *other) -> int {
+    constexpr int MODE = decltype(mode_tag)::value;
+    if (s->nterms == 0)
+      return EFM_OK;
+    hipLaunchKernelGGL((k_cells_ports<R, MODE>), dim3((int)s->nterms), dim3(kBlock), 0, st, s->poff.as<int64_t>(),
+                       s->pidx.as<int32_t>(), s->pw.as<double2>(), s->pcoef.as<double2>(), s->pcell.as<int32_t>(),
+                       in, outv, other, part, (int64_t)s->ntiles, S);
+    EFM_LAUNCH_CHECK();
+    return EFM_OK;
+  };
+  using I0 = std::integral_constant<int, 0>;
+  using I1 = std::integral_constant<int, 1>;
+  int rc;
+  hipLaunchKernelGGL((k_cells_init<R>), dim3(nt), dim3(kBlock), 0, st, T, B, x, r, r0, p, v, part);
+  EFM_LAUNCH_CHECK();
+  if ((rc = fin(std::integral_constant<int, kInit>{})))
+    return rc;
+  std::vector<int> hf((size_t)nc * R);
+  auto n_active = [&](int &cnt) -> int {
+    EFM_HIP_CHECK(hipMemcpyAsync(hf.data(), s->flags.p, hf.size() * sizeof(int), hipMemcpyDeviceToHost, st));
+    EFM_HIP_CHECK(hipStreamSynchronize(st));
+    cnt = 0;
+    for (int f : hf)
+      cnt += f != 0;
+    return EFM_OK;
+  };
+  int active = 0;
+  if ((rc = n_active(active)))
+    return rc;
+  int it = 0;
+  while (active > 0 && it < max_it) {
+    const int chunk = std::min(check, max_it - it);
+    for (int c = 0; c < chunk; ++c, ++it) {
+      if (it == 0) {
+        hipLaunchKernelGGL((k_cells_dot_r0r<R>), dim3(nt), dim3(kBlock), 0, st, T, r0, r, part);
+        EFM_LAUNCH_CHECK();
+        if ((rc = fin(std::integral_constant<int, kRho0>{})))
+          return rc;
+      }
+      hipLaunchKernelGGL((k_cells_p<R>), dim3(nt), dim3(kBlock), 0, st, T, r, v, p, S);
+      EFM_LAUNCH_CHECK();
+      if ((rc = precond(p, y)))
+        return rc;
+      hipLaunchKernelGGL((k_cells_spmv<R, 0>), dim3(nt), dim3(kBlock), 0, st, T, s->row_ptr.as<int32_t>(),
+                         s->col.as<int32_t>(), s->val.as<double2>(), y, v, r0, part, S);
+      EFM_LAUNCH_CHECK();
+      if ((rc = ports(I0{}, y, v, r0)) || (rc = fin(std::integral_constant<int, kAlpha>{})))
+        return rc;
+      hipLaunchKernelGGL((k_cells_s<R>), dim3(nt), dim3(kBlock), 0, st, T, v, r, S);
+      EFM_LAUNCH_CHECK();
+      if ((rc = precond(r, zz)))
+        return rc;
+      hipLaunchKernelGGL((k_cells_spmv<R, 1>), dim3(nt), dim3(kBlock), 0, st, T, s->row_ptr.as<int32_t>(),
+                         s->col.as<int32_t>(), s->val.as<double2>(), zz, t, r, part, S);
+      EFM_LAUNCH_CHECK();
+      if ((rc = ports(I1{}, zz, t, r)) || (rc = fin(std::integral_constant<int, kOmega>{})))
+        return rc;
+      hipLaunchKernelGGL((k_cells_x<R>), dim3(nt), dim3(kBlock), 0, st, T, y, zz, t, r0, x, r, part, S);
+      EFM_LAUNCH_CHECK();
+      if ((rc = fin(std::integral_constant<int, kX>{})))
+        return rc;
+    }
+    if ((rc = n_active(active)))
+      return rc;
+  }
+  EFM_HIP_CHECK(hipMemcpyAsync(X, x, (size_t)s->n * R * sizeof(double2), hipMemcpyDeviceToDevice, st));
+  std::vector<CellSys> hs((size_t)nc * R);
+  EFM_HIP_CHECK(hipMemcpyAsync(hs.data(), S, hs.size() * sizeof(CellSys), hipMemcpyDeviceToHost, st));
+  EFM_HIP_CHECK(hipStreamSynchronize(st));
+  for (size_t k = 0; k < hs.size(); ++k) {
+    if (h_iters)
+      h_iters[k] = hs[k].iters;
+    if (h_resid)
+      h_resid[k] = hs[k].bnorm2 > 0 ? sqrt(hs[k].rnorm2 / hs[k].bnorm2) : 0.0;
+  }
+  return EFM_OK;
+}
+} // namespace
+
+extern "C" int efm_cells_solve(efm_cells *s, const double *B, double *X, const efm_solve_opts *h_opts,
+                               int32_t *h_iters, double *h_resid, void *stream) {
+  EFM_REQUIRE(s && B && X && h_opts, "efm_cells_solve: NULL argument");
+  EFM_REQUIRE(h_opts->method == EFM_METHOD_BICGSTAB, "efm_cells_solve: the cell batch uses BiCGSTAB");
+  EFM_REQUIRE(h_opts->max_iterations >= 1, "efm_cells_solve: max_iterations < 1");
+  hipStream_t st = (hipStream_t)stream;
+  if (s->R == 1)
+    return cells_solve<1>(s, (const double2 *)B, (double2 *)X, h_opts, h_iters, h_resid, st);
+  return cells_solve<2>(s, (const double2 *)B, (double2 *)X, h_opts, h_iters, h_resid, st);
+}

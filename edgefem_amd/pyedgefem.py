@@ -1487,6 +1487,108 @@ def build_wave_port(volume_mesh: Mesh, surface: PortSurfaceMesh, mode: PortMode)
     return port
 
 
+def _periodic_cell_system(mesh: Mesh, params: MaxwellParams, bc: BC, pbc: PeriodicBC,
+                          ports: Sequence[WavePort], mode: str):
+    """Reduced periodic system of one cell on the GPU plus its host-side descriptors:
+    (SpMatC, right-hand sides (n, P), gradient (en, ec, N), port frees, recovery data)."""
+    asm = assemble_maxwell_periodic(mesh, params, bc, pbc, ports, -1, mode=mode)
+    dmask = _dirichlet_mask(mesh, bc)
+    n = mesh.num_edges()
+    P = len(ports)
+    a, c, rec = _periodic_coefs(pbc, mode)
+    m = np.array([p.master_edge for p in pbc.pairs], np.int64)
+    s_ = np.array([p.slave_edge for p in pbc.pairs], np.int64)
+    act = (dmask[m] == 0) & (dmask[s_] == 0) if m.size else np.zeros(0, bool)
+    zero = np.zeros(n, bool)
+    if m.size:
+        zero[s_[dmask[s_] == 0]] = True
+    frees = [_free_port(p, dmask) for p in ports]
+    B = np.zeros((n, P), np.complex128)
+    for i, port in enumerate(ports):
+        if _valid_port(port):
+            e, w = frees[i]
+            col = np.zeros(n, np.complex128)
+            np.add.at(col, e, 2.0 / cmath.sqrt(complex(port.mode.Z0)) * w)
+            if m.size:
+                np.add.at(col, m[act], a[act] * col[s_[act]])
+                col[zero] = 0
+            B[:, i] = col
+    en, ec, N = _periodic_gradient(mesh, pbc, dmask, zero, mode)
+    return asm.A, B, (en, ec, N), frees, (m, s_, rec)
+
+
+def calculate_sparams_periodic_batch(cells, *, mode: str = "reference",
+                                     solver_options: Optional[SolveOptions] = None,
+                                     _state: Optional[dict] = None) -> List[np.ndarray]:
+    """calculate_sparams_periodic for many unit cells of different sizes in one batched
+    solve (efm_cells_*): every cell is reduced on the GPU (efm_periodic_reduce), the
+    reduced systems are stacked into one block-diagonal matrix, and every (cell,
+    excitation) pair runs its own BiCGSTAB (own scalars and convergence) in the same
+    launches.  ``cells`` is a sequence of (mesh, params, bc, pbc, ports) with the same
+    number of ports (1 or 2) per cell.  Returns one S matrix per cell (NaN columns where
+    a system did not converge)."""
+    import torch
+    opts = solver_options or SolveOptions()
+    cells = list(cells)
+    if not cells:
+        return []
+    P = len(cells[0][4])
+    if P not in (1, 2) or any(len(c[4]) != P for c in cells):
+        raise EdgeFemError("every cell needs the same number of ports (1 or 2)")
+    row_off, node_off, port_off = [0], [0], [0]
+    rps, cols, vals, ens, ecs, pidx, pw, pc, Bs, meta = [], [], [], [], [], [], [], [], [], []
+    nnz_off = 0
+    for mesh, params, bc, pbc, ports in cells:
+        params._check_supported()
+        A, B, (en, ec, N), frees, rec = _periodic_cell_system(mesh, params, bc, pbc, ports, mode)
+        r0, q0 = row_off[-1], node_off[-1]
+        rp = A.row_ptr.cpu().numpy().astype(np.int64)
+        rps.append(rp[:-1] + nnz_off)
+        nnz_off += int(rp[-1])
+        cols.append(A.col_idx.cpu().numpy().astype(np.int64) + r0)
+        vals.append(A.values.cpu().numpy())
+        ens.append(np.where(en >= 0, en + q0, -1))
+        ecs.append(ec)
+        lr = {i: (e, w, cf) for i, (e, w, cf) in enumerate(A.lowrank)}
+        if len(A.lowrank) != P:
+            raise EdgeFemError("every port of every cell needs a loading term")
+        for e, w, cf in A.lowrank:
+            pidx.append(np.asarray(e, np.int64) + r0)
+            pw.append(np.asarray(w, np.complex128))
+            pc.append(complex(cf))
+            port_off.append(port_off[-1] + len(e))
+        Bs.append(B)
+        meta.append((frees, rec, ports, mesh.num_edges()))
+        row_off.append(r0 + mesh.num_edges())
+        node_off.append(q0 + N)
+    rp_all = np.concatenate(rps + [np.array([nnz_off])]).astype(np.int32)
+    batch = _capi.CellBatch(np.array(row_off), rp_all, np.concatenate(cols), np.concatenate(vals),
+                            np.array(node_off), np.concatenate(ens), np.concatenate(ecs), P, np.array(port_off),
+                            np.concatenate(pidx), np.concatenate(pw), np.array(pc))
+    tol = 1e-14 if opts.use_direct else float(opts.tolerance)
+    Bd = _capi.to_dev(np.concatenate(Bs), torch.complex128)
+    X, iters, resid = batch.solve(Bd, tol=tol, max_iterations=int(opts.max_iterations))
+    Xh = X.reshape(-1, P).cpu().numpy()
+    out = []
+    for ci, (frees, (m, s_, rec), ports, n) in enumerate(meta):
+        x = Xh[row_off[ci]:row_off[ci] + n].copy()
+        if m.size:
+            x[s_, :] = rec[:, None] * x[m, :]
+        S = np.full((P, P), np.nan + 1j * np.nan, np.complex128)
+        for i, port in enumerate(ports):
+            if not (resid[ci, i] <= tol or iters[ci, i] == 0):
+                continue
+            vinc = cmath.sqrt(complex(port.mode.Z0))
+            for j in range(P):
+                e, w = frees[j]
+                V = np.sum(np.conj(w) * x[e, i])
+                S[j, i] = (V - vinc) / vinc if i == j else V / vinc
+        out.append(S)
+    if _state is not None:
+        _state.update(dict(iters=iters, resid=resid, n=row_off[-1], nnz=nnz_off, batch=batch))
+    return out
+
+
 class LumpedPortWeightMode(enum.IntEnum):
     Projection = 0
     SurfaceIntegral = 1

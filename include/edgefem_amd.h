@@ -267,6 +267,33 @@ int efm_periodic_reduce(int64_t n, const int32_t *row_ptr, const int32_t *col, c
                         double *val_out, int64_t cap, int64_t *h_nnz, void *stream);
 
 /* ------------------------------------------------------------------------
+ * Many independent systems of different sizes in one batch (periodic unit cells): the
+ * many-cell form of calculate_sparams_periodic (src/assemble_maxwell.cpp:576-635) as
+ * UnitCellDesign calls it per cell (python/edgefem/designs/unit_cell.py:700-777).  The
+ * reduced matrices of all cells form one block-diagonal CSR (host arrays: h_row_off
+ * [ncells+1] row offsets, h_row_ptr [n+1], h_col [nnz] global columns inside the
+ * cell, h_val c128).  Each cell has nrhs (1 or 2) excitations, and every (cell,
+ * excitation) pair is its own BiCGSTAB system (own scalars and convergence).  The
+ * preconditioner is the complex-incidence auxiliary one over the concatenated node
+ * space (h_node_off [ncells+1], h_edge_nodes [2n] global node or -1, h_edge_coefs [2n]
+ * c128).  Port loading: nrhs terms per cell, term q = cell * nrhs + p with entries
+ * [h_port_off[q], h_port_off[q+1]) of h_port_idx (global edges, disjoint) / h_port_w
+ * (c128) and coefficient h_port_coef[q] (c128): A_cell + sum c_q w_q w_q^H.
+ * ---------------------------------------------------------------------- */
+typedef struct efm_cells efm_cells;
+
+int efm_cells_create(int64_t ncells, const int64_t *h_row_off, const int32_t *h_row_ptr,
+                     const int32_t *h_col, const double *h_val, const int64_t *h_node_off,
+                     const int32_t *h_edge_nodes, const double *h_edge_coefs, int nrhs,
+                     const int64_t *h_port_off, const int32_t *h_port_idx, const double *h_port_w,
+                     const double *h_port_coef, void *stream, efm_cells **out);
+void efm_cells_destroy(efm_cells *cells);
+/* B, X: [n][nrhs] c128 (device); h_iters / h_resid: [ncells*nrhs] (may be NULL);
+ * h_opts->method must be EFM_METHOD_BICGSTAB. */
+int efm_cells_solve(efm_cells *cells, const double *B, double *X, const efm_solve_opts *h_opts,
+                    int32_t *h_iters, double *h_resid, void *stream);
+
+/* ------------------------------------------------------------------------
  * Adjoint gradient with respect to the per-tet permittivity (no reference counterpart;
  * it differentiates calculate_sparams_eigenmode, src/assemble_maxwell.cpp:637-787).
  * For every design tet e = design[d] (design NULL: all tets, n_design == n_tets):

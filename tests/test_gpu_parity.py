@@ -711,3 +711,51 @@ def test_nodal_port_modes_and_wave_port_vs_oracle(torch_dev):
     So = O.calculate_sparams(om, O.Params(omega=w), pec, oports)
     _p(f"    S {S.ravel()}")
     assert np.abs(S - So).max() <= 1e-6
+
+
+def test_periodic_cell_batch_vs_single_and_oracle(torch_dev):
+    """Six unit cells of different sizes, patch sizes and scan angles in one batched
+    BiCGSTAB (efm_cells_*): every cell's S against the oracle (SuperLU) and against the
+    single-cell GPU path."""
+    from edgefem_amd import meshgen as mg
+    from edgefem_amd import pyedgefem as em
+    from oracle import edgefem_oracle as O
+    _p("[gpu] batched unit cells")
+    rng = np.random.default_rng(7)
+    cells, ocells = [], []
+    f = 10e9
+    k0 = 2 * math.pi * f / em.C0
+    for c in range(6):
+        n = int(rng.integers(3, 7))
+        nz = int(rng.integers(5, 9))
+        patch = (float(rng.uniform(1e-3, 4.5e-3)), float(rng.uniform(1e-3, 4.5e-3)))
+        g = mg.unit_cell(n=n, nz=nz, patch=patch, seed=100 + c)
+        pm = em.mesh_from_arrays(g.node_ids, g.xyz, g.tets, g.tet_phys, g.tris, g.tri_phys)
+        om = O.mesh_from_arrays(g.node_ids, g.xyz, g.tet_ids, g.tet_phys, g.tri_ids, g.tri_phys)
+        bc = em.build_edge_pec(pm, 1)
+        pbc = em.build_periodic_pairs(pm, 10, 11, np.array([5e-3, 0.0, 0.0]))
+        em.set_floquet_phase(pbc, np.array([k0 * math.sin(math.radians(rng.uniform(0, 40))), 0.0]))
+        ports = []
+        for tag in (2, 3):
+            cfg = em.LumpedPortConfig()
+            cfg.surface_tag = tag
+            cfg.z0 = 376.73
+            cfg.e_direction = np.array([0.0, 1.0, 0.0])
+            ports.append(em.build_lumped_port(pm, cfg))
+        params = em.MaxwellParams()
+        params.omega = 2 * math.pi * f
+        params.eps_r_regions = {101: 3.5 * (1 - 0.002j)}
+        cells.append((pm, params, bc, pbc, ports))
+        opbc = O.build_periodic_pairs(om, 10, 11, (5e-3, 0, 0))
+        opbc.phase = pbc.phase_shift
+        oports = [O.build_lumped_port(om, t, 376.73, (0, 1, 0)) for t in (2, 3)]
+        ocells.append((om, O.Params(omega=params.omega, eps_r_regions={101: 3.5 * (1 - 0.002j)}),
+                       O.build_edge_pec(om, 1), opbc, oports))
+    st = {}
+    Sb = em.calculate_sparams_periodic_batch(cells, _state=st)
+    _p(f"    {st['n']} rows, {st['nnz']} nnz; iterations {st['iters'].ravel().tolist()}")
+    for (pm, params, bc, pbc, ports), (om, op, opec, opbc, oports), S in zip(cells, ocells, Sb):
+        So = O.calculate_sparams_periodic(om, op, opec, opbc, oports)
+        Ss = em.calculate_sparams_periodic(pm, params, bc, pbc, ports)
+        assert np.abs(S - So).max() <= 1e-6
+        assert np.abs(S - Ss).max() <= 1e-6
