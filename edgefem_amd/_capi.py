@@ -101,7 +101,8 @@ def load_library(path: str = LIB_PATH) -> C.CDLL:
     lib.efm_solver_set_lowrank.argtypes = [vp, i32, C.c_int64, vp, vp, vp, vp]
     lib.efm_add_diagonal.argtypes = [vp, C.c_int64, vp, vp, vp, vp]
     lib.efm_solver_set_gradient_complex.argtypes = [vp, C.c_int64, vp, vp, vp]
-    lib.efm_cells_create.argtypes = [C.c_int64, vp, vp, vp, vp, vp, vp, vp, i32, vp, vp, vp, vp, vp, C.POINTER(vp)]
+    lib.efm_cells_create.argtypes = [C.c_int64, vp, vp, vp, vp, C.c_int64, vp, vp, vp, i32, vp, vp, vp, vp, vp,
+                                     C.POINTER(vp)]
     lib.efm_cells_destroy.argtypes = [vp]
     lib.efm_cells_destroy.restype = None
     lib.efm_cells_solve.argtypes = [vp, vp, vp, C.POINTER(SolveOpts), vp, vp, vp]
@@ -548,15 +549,16 @@ def periodic_reduce(row_ptr, col_idx, values, master, slave, coef_row, coef_col,
 
 
 class CellBatch:
-    """Block-diagonal batch of independent BiCGSTAB systems (efm_cells_*); all arrays are
-    host numpy arrays in global (concatenated) numbering."""
+    """Block-diagonal batch of independent BiCGSTAB systems (efm_cells_*) in global
+    (concatenated) numbering: row_ptr, col, val device tensors; the rest host arrays."""
 
     def __init__(self, row_off, row_ptr, col, val, node_off, edge_nodes, edge_coefs, nrhs, port_off, port_idx,
                  port_w, port_coef):
+        torch = _torch()
         lib = load_library()
         self._lib = lib
-        arrs = dict(row_off=np.ascontiguousarray(row_off, np.int64), row_ptr=np.ascontiguousarray(row_ptr, np.int32),
-                    col=np.ascontiguousarray(col, np.int32), val=np.ascontiguousarray(val, np.complex128),
+        self._dev = (to_dev(row_ptr, torch.int32), to_dev(col, torch.int32), to_dev(val, torch.complex128))
+        arrs = dict(row_off=np.ascontiguousarray(row_off, np.int64),
                     node_off=np.ascontiguousarray(node_off, np.int64),
                     edge_nodes=np.ascontiguousarray(edge_nodes, np.int32).reshape(-1),
                     edge_coefs=np.ascontiguousarray(edge_coefs, np.complex128).reshape(-1),
@@ -568,8 +570,9 @@ class CellBatch:
         self.nrhs = int(nrhs)
         ptr = lambda a: a.ctypes.data_as(C.c_void_p)
         h = C.c_void_p()
-        _check(lib.efm_cells_create(self.ncells, ptr(arrs["row_off"]), ptr(arrs["row_ptr"]), ptr(arrs["col"]),
-                                    ptr(arrs["val"]), ptr(arrs["node_off"]), ptr(arrs["edge_nodes"]),
+        rp, cl, vl = self._dev
+        _check(lib.efm_cells_create(self.ncells, ptr(arrs["row_off"]), dptr(rp), dptr(cl), dptr(vl), int(cl.numel()),
+                                    ptr(arrs["node_off"]), ptr(arrs["edge_nodes"]),
                                     ptr(arrs["edge_coefs"]), self.nrhs, ptr(arrs["port_off"]), ptr(arrs["port_idx"]),
                                     ptr(arrs["port_w"]), ptr(arrs["port_coef"]), stream_ptr(), C.byref(h)),
                "efm_cells_create")

@@ -35,8 +35,20 @@ constexpr int kMaxR = 2;
 struct CellSys {
   double2 rho, alpha, beta, omega;
   double bnorm2, rnorm2;
-  int active, iters, breakdown, pad;
+  int active, iters, breakdown, restart;  // restart: take r0 = r, p = r at the next p step
+  int nrestart, pad[3];
 };
+
+constexpr int kMaxRestarts = 20;
+
+// true if any of the R systems of `cell` is active (block-uniform)
+template <int R> __device__ inline bool cell_active(const CellSys *st, int64_t cell) {
+  bool a = false;
+#pragma unroll
+  for (int s = 0; s < R; ++s)
+    a = a || st[cell * R + s].active != 0;
+  return a;
+}
 
 struct Tile {
   int32_t cell;
@@ -124,6 +136,8 @@ __global__ void __launch_bounds__(kBlock)
       S.beta = make_double2(0, 0);
       S.iters = 0;
       S.breakdown = 0;
+      S.restart = 0;
+      S.nrestart = 0;
       S.active = tot.x > 0.0 ? 1 : 0;
     } else if (!S.active) {
     } else if (STAGE == kRho0) {
@@ -135,6 +149,7 @@ __global__ void __launch_bounds__(kBlock)
         S.rho = tot;
       }
     } else if (STAGE == kAlpha) {
+      S.restart = 0;
       if (tot.x == 0.0 && tot.y == 0.0) {
         S.breakdown = 1;
         S.active = 0;
@@ -149,11 +164,23 @@ __global__ void __launch_bounds__(kBlock)
       S.rnorm2 = lds[R + s].x;
       if (S.rnorm2 <= tol2 * S.bnorm2) {
         S.active = 0;
-      } else if (S.iters >= max_it || (S.omega.x == 0.0 && S.omega.y == 0.0)) {
+      } else if (S.iters >= max_it) {
         S.active = 0;
-      } else if (tot.x == 0.0 && tot.y == 0.0) {
-        S.breakdown = 1;
-        S.active = 0;
+      } else if ((S.omega.x == 0.0 && S.omega.y == 0.0) || (tot.x == 0.0 && tot.y == 0.0) ||
+                 cabs2(tot) < 1e-30 * S.rnorm2 * S.rnorm2) {
+        // breakdown (omega = 0 or r0^H r ~ 0): restart from the current iterate with the
+        // shadow residual r0 = r, so rho = |r|^2 and p = r
+        if (S.nrestart >= kMaxRestarts) {
+          S.breakdown = 1;
+          S.active = 0;
+        } else {
+          S.nrestart += 1;
+          S.restart = 1;
+          S.rho = make_double2(S.rnorm2, 0.0);
+          S.beta = make_double2(0, 0);
+          S.omega = make_double2(1, 0);
+          S.alpha = make_double2(1, 0);
+        }
       } else {
         S.beta = cmul(cdiv(tot, S.rho), cdiv(S.alpha, S.omega));
         S.rho = tot;
@@ -208,17 +235,21 @@ __global__ void __launch_bounds__(kBlock)
 }
 
 // p = r + beta (p - omega v)
+// (restarting systems: r0 = r, v = 0, p = r)
 template <int R>
 __global__ void __launch_bounds__(kBlock)
-    k_cells_p(const Tile *__restrict__ tiles, const double2 *__restrict__ r, const double2 *__restrict__ v,
-              double2 *__restrict__ p, const CellSys *__restrict__ st) {
+    k_cells_p(const Tile *__restrict__ tiles, const double2 *__restrict__ r, double2 *__restrict__ v,
+              double2 *__restrict__ p, double2 *__restrict__ r0, const CellSys *__restrict__ st) {
   const Tile t = tiles[blockIdx.x];
-  bool act[R];
+  if (!cell_active<R>(st, t.cell))
+    return;
+  bool act[R], rs[R];
   double2 beta[R], omega[R];
 #pragma unroll
   for (int s = 0; s < R; ++s) {
     const CellSys &S = st[(int64_t)t.cell * R + s];
     act[s] = S.active != 0;
+    rs[s] = S.restart != 0;
     beta[s] = S.beta;
     omega[s] = S.omega;
   }
@@ -228,7 +259,13 @@ __global__ void __launch_bounds__(kBlock)
       if (!act[s])
         continue;
       const int64_t k = i * R + s;
-      p[k] = cadd(r[k], bil(beta[s], csub(p[k], bil(omega[s], v[k]))));
+      if (rs[s]) {
+        r0[k] = r[k];
+        v[k] = make_double2(0, 0);
+        p[k] = r[k];
+      } else {
+        p[k] = cadd(r[k], bil(beta[s], csub(p[k], bil(omega[s], v[k]))));
+      }
     }
 }
 
@@ -240,6 +277,8 @@ __global__ void __launch_bounds__(kBlock)
                      const double2 *__restrict__ dninv, const double2 *__restrict__ in, double2 *__restrict__ y,
                      const CellSys *__restrict__ st) {
   const Tile t = ntiles[blockIdx.x];
+  if (!cell_active<R>(st, t.cell))
+    return;
   bool act[R];
 #pragma unroll
   for (int s = 0; s < R; ++s)
@@ -271,6 +310,8 @@ __global__ void __launch_bounds__(kBlock)
                      const double2 *__restrict__ gb, const double2 *__restrict__ dinv, const double2 *__restrict__ y,
                      const double2 *__restrict__ in, double2 *__restrict__ out, const CellSys *__restrict__ st) {
   const Tile t = tiles[blockIdx.x];
+  if (!cell_active<R>(st, t.cell))
+    return;
   bool act[R];
 #pragma unroll
   for (int s = 0; s < R; ++s)
@@ -302,6 +343,8 @@ __global__ void __launch_bounds__(kBlock)
   constexpr int V = MODE == 0 ? R : 2 * R;
   __shared__ double2 lds[(kBlock / 64) * 2 * kMaxR];
   const Tile t = tiles[blockIdx.x];
+  if (!cell_active<R>(st, t.cell))  // the finalisation ignores the partials of inactive systems
+    return;
   bool act[R];
 #pragma unroll
   for (int s = 0; s < R; ++s)
@@ -369,6 +412,8 @@ __global__ void __launch_bounds__(kBlock)
 #pragma unroll
   for (int s = 0; s < R; ++s)
     act[s] = st[cell * R + s].active != 0;
+  if (!cell_active<R>(st, cell))
+    return;
   double2 tv[R];
 #pragma unroll
   for (int s = 0; s < R; ++s)
@@ -420,6 +465,8 @@ __global__ void __launch_bounds__(kBlock)
     k_cells_s(const Tile *__restrict__ tiles, const double2 *__restrict__ v, double2 *__restrict__ r,
               const CellSys *__restrict__ st) {
   const Tile t = tiles[blockIdx.x];
+  if (!cell_active<R>(st, t.cell))
+    return;
   bool act[R];
   double2 alpha[R];
 #pragma unroll
@@ -445,6 +492,8 @@ __global__ void __launch_bounds__(kBlock)
               double2 *__restrict__ r, double2 *__restrict__ partial, const CellSys *__restrict__ st) {
   __shared__ double2 lds[(kBlock / 64) * 2 * kMaxR];
   const Tile t = tiles[blockIdx.x];
+  if (!cell_active<R>(st, t.cell))  // the finalisation ignores the partials of inactive systems
+    return;
   bool act[R];
   double2 alpha[R], omega[R];
 #pragma unroll
@@ -516,6 +565,18 @@ __global__ void k_cells_dn(int64_t N, const int32_t *__restrict__ gt_ptr, const 
   dninv[nd] = (sum.x == 0.0 && sum.y == 0.0) ? make_double2(0, 0) : cdiv(make_double2(1, 0), sum);
 }
 
+// every column of a row must lie inside the row's cell; err = 1 otherwise
+__global__ void k_cells_check(const Tile *__restrict__ tiles, const int64_t *__restrict__ row_off,
+                              const int32_t *__restrict__ row_ptr, const int32_t *__restrict__ col,
+                              int *__restrict__ err) {
+  const Tile t = tiles[blockIdx.x];
+  const int64_t lo = row_off[t.cell], hi = row_off[t.cell + 1];
+  for (int64_t r = t.r0 + threadIdx.x; r < t.r1; r += kBlock)
+    for (int32_t k = row_ptr[r]; k < row_ptr[r + 1]; ++k)
+      if (col[k] < lo || col[k] >= hi)
+        err[0] = 1;
+}
+
 } // namespace
 
 struct efm_cells {
@@ -555,14 +616,14 @@ std::vector<Tile> make_tiles(const int64_t *off, int64_t ncells, std::vector<int
 }
 } // namespace
 
-extern "C" int efm_cells_create(int64_t ncells, const int64_t *h_row_off, const int32_t *h_row_ptr,
-                                const int32_t *h_col, const double *h_val, const int64_t *h_node_off,
+extern "C" int efm_cells_create(int64_t ncells, const int64_t *h_row_off, const int32_t *row_ptr,
+                                const int32_t *col, const double *val, int64_t nnz, const int64_t *h_node_off,
                                 const int32_t *h_edge_nodes, const double *h_edge_coefs, int nrhs,
                                 const int64_t *h_port_off, const int32_t *h_port_idx, const double *h_port_w,
                                 const double *h_port_coef, void *stream, efm_cells **out) {
   EFM_REQUIRE(out, "efm_cells_create: out is NULL");
   *out = nullptr;
-  EFM_REQUIRE(ncells >= 1 && h_row_off && h_row_ptr && h_col && h_val && h_node_off && h_edge_nodes &&
+  EFM_REQUIRE(ncells >= 1 && h_row_off && row_ptr && col && val && h_node_off && h_edge_nodes &&
                   h_edge_coefs && h_port_off,
               "efm_cells_create: NULL argument");
   EFM_REQUIRE(nrhs == 1 || nrhs == 2, "efm_cells_create: 1 or 2 excitations per cell");
@@ -572,19 +633,13 @@ extern "C" int efm_cells_create(int64_t ncells, const int64_t *h_row_off, const 
   s->R = nrhs;
   s->npt = nrhs;
   s->n = h_row_off[ncells];
-  s->nnz = h_row_ptr[s->n];
+  s->nnz = nnz;
   s->N = h_node_off[ncells];
   const int64_t n = s->n, N = s->N;
   s->nterms = ncells * s->npt;
-  // validation of the block-diagonal structure (every column inside its cell)
-  for (int64_t c = 0; c < ncells; ++c) {
+  for (int64_t c = 0; c < ncells; ++c)
     EFM_REQUIRE(h_row_off[c + 1] >= h_row_off[c] && h_node_off[c + 1] >= h_node_off[c],
                 "efm_cells_create: offsets must be nondecreasing");
-    for (int64_t r = h_row_off[c]; r < h_row_off[c + 1]; ++r)
-      for (int32_t k = h_row_ptr[r]; k < h_row_ptr[r + 1]; ++k)
-        EFM_REQUIRE(h_col[k] >= h_row_off[c] && h_col[k] < h_row_off[c + 1],
-                    "efm_cells_create: a column leaves its cell");
-  }
   // gradient incidence (global node indices) and its transpose
   std::vector<int2> en2(n);
   std::vector<double2> ga(n), gb(n);
@@ -640,8 +695,19 @@ extern "C" int efm_cells_create(int64_t ncells, const int64_t *h_row_off, const 
   s->nntiles = (int64_t)ntl.size();
   int rc;
   const size_t vb = (size_t)n * s->R * sizeof(double2);
-  if ((rc = upload(s->row_ptr, h_row_ptr, n + 1, st)) || (rc = upload(s->col, h_col, s->nnz, st)) ||
-      (rc = upload(s->val, (const double2 *)h_val, s->nnz, st)) || (rc = upload(s->en2, en2.data(), n, st)) ||
+  if ((rc = s->row_ptr.alloc((n + 1) * sizeof(int32_t))) || (rc = s->col.alloc(std::max<int64_t>(nnz, 1) * 4)) ||
+      (rc = s->val.alloc(std::max<int64_t>(nnz, 1) * sizeof(double2))))
+    return rc;
+  EFM_HIP_CHECK(hipMemcpyAsync(s->row_ptr.p, row_ptr, (n + 1) * sizeof(int32_t), hipMemcpyDeviceToDevice, st));
+  EFM_HIP_CHECK(hipMemcpyAsync(s->col.p, col, nnz * sizeof(int32_t), hipMemcpyDeviceToDevice, st));
+  EFM_HIP_CHECK(hipMemcpyAsync(s->val.p, val, nnz * sizeof(double2), hipMemcpyDeviceToDevice, st));
+  {  // the row pointer must end at nnz and every column must stay inside its cell
+    int32_t last = -1;
+    EFM_HIP_CHECK(hipMemcpyAsync(&last, s->row_ptr.as<int32_t>() + n, sizeof(int32_t), hipMemcpyDeviceToHost, st));
+    EFM_HIP_CHECK(hipStreamSynchronize(st));
+    EFM_REQUIRE(last == nnz, "efm_cells_create: row_ptr[n] != nnz");
+  }
+  if ((rc = upload(s->en2, en2.data(), n, st)) ||
       (rc = upload(s->ga, ga.data(), n, st)) || (rc = upload(s->gb, gb.data(), n, st)) ||
       (rc = upload(s->gt_ptr, cnt.data(), N + 1, st)) || (rc = upload(s->gt_edge, gte.data(), gte.size(), st)) ||
       (rc = upload(s->gt_coef, gtc.data(), gtc.size(), st)) ||
@@ -653,6 +719,21 @@ extern "C" int efm_cells_create(int64_t ncells, const int64_t *h_row_off, const 
       (rc = upload(s->pcoef, (const double2 *)h_port_coef, s->nterms, st)) ||
       (rc = upload(s->pcell, pcell.data(), pcell.size(), st)))
     return rc;
+  {
+    DevBuf roff, err;
+    if ((rc = upload(roff, h_row_off, ncells + 1, st)) || (rc = err.alloc(sizeof(int))))
+      return rc;
+    EFM_HIP_CHECK(hipMemsetAsync(err.p, 0, sizeof(int), st));
+    if (s->ntiles > 0) {
+      hipLaunchKernelGGL(k_cells_check, dim3((int)s->ntiles), dim3(kBlock), 0, st, s->tiles.as<Tile>(),
+                         roff.as<int64_t>(), s->row_ptr.as<int32_t>(), s->col.as<int32_t>(), err.as<int>());
+      EFM_LAUNCH_CHECK();
+    }
+    int h_err = 0;
+    EFM_HIP_CHECK(hipMemcpyAsync(&h_err, err.p, sizeof(int), hipMemcpyDeviceToHost, st));
+    EFM_HIP_CHECK(hipStreamSynchronize(st));
+    EFM_REQUIRE(h_err == 0, "efm_cells_create: a column leaves its cell");
+  }
   DevBuf ext;
   if ((rc = upload(ext, extra.data(), n, st)) || (rc = s->dinv.alloc(n * sizeof(double2))) ||
       (rc = s->dninv.alloc(std::max<int64_t>(N, 1) * sizeof(double2))) ||
@@ -753,7 +834,7 @@ template <int R> int cells_solve(efm_cells *s, const double2 *B, double2 *X, con
         if ((rc = fin(std::integral_constant<int, kRho0>{})))
           return rc;
       }
-      hipLaunchKernelGGL((k_cells_p<R>), dim3(nt), dim3(kBlock), 0, st, T, r, v, p, S);
+      hipLaunchKernelGGL((k_cells_p<R>), dim3(nt), dim3(kBlock), 0, st, T, r, v, p, r0, S);
       EFM_LAUNCH_CHECK();
       if ((rc = precond(p, y)))
         return rc;

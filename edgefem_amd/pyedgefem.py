@@ -27,6 +27,7 @@ import cmath
 import enum
 import math
 import sys
+import time
 from typing import Dict, Iterable, List, Optional, Sequence
 
 import numpy as np
@@ -1542,11 +1543,11 @@ def calculate_sparams_periodic_batch(cells, *, mode: str = "reference",
         params._check_supported()
         A, B, (en, ec, N), frees, rec = _periodic_cell_system(mesh, params, bc, pbc, ports, mode)
         r0, q0 = row_off[-1], node_off[-1]
-        rp = A.row_ptr.cpu().numpy().astype(np.int64)
-        rps.append(rp[:-1] + nnz_off)
-        nnz_off += int(rp[-1])
-        cols.append(A.col_idx.cpu().numpy().astype(np.int64) + r0)
-        vals.append(A.values.cpu().numpy())
+        nnz_c = int(A.col_idx.numel())
+        rps.append(A.row_ptr[:-1] + nnz_off)
+        nnz_off += nnz_c
+        cols.append(A.col_idx + r0)
+        vals.append(A.values)
         ens.append(np.where(en >= 0, en + q0, -1))
         ecs.append(ec)
         lr = {i: (e, w, cf) for i, (e, w, cf) in enumerate(A.lowrank)}
@@ -1561,13 +1562,18 @@ def calculate_sparams_periodic_batch(cells, *, mode: str = "reference",
         meta.append((frees, rec, ports, mesh.num_edges()))
         row_off.append(r0 + mesh.num_edges())
         node_off.append(q0 + N)
-    rp_all = np.concatenate(rps + [np.array([nnz_off])]).astype(np.int32)
-    batch = _capi.CellBatch(np.array(row_off), rp_all, np.concatenate(cols), np.concatenate(vals),
+    if nnz_off >= 2 ** 31 or row_off[-1] >= 2 ** 31:
+        raise EdgeFemError("the cell batch exceeds 32-bit indices; split it")
+    dev = rps[0].device
+    rp_all = torch.cat(rps + [torch.tensor([nnz_off], dtype=torch.int32, device=dev)]).to(torch.int32)
+    batch = _capi.CellBatch(np.array(row_off), rp_all, torch.cat(cols).to(torch.int32), torch.cat(vals),
                             np.array(node_off), np.concatenate(ens), np.concatenate(ecs), P, np.array(port_off),
                             np.concatenate(pidx), np.concatenate(pw), np.array(pc))
     tol = 1e-14 if opts.use_direct else float(opts.tolerance)
     Bd = _capi.to_dev(np.concatenate(Bs), torch.complex128)
+    t_solve = time.perf_counter()
     X, iters, resid = batch.solve(Bd, tol=tol, max_iterations=int(opts.max_iterations))
+    t_solve = time.perf_counter() - t_solve
     Xh = X.reshape(-1, P).cpu().numpy()
     out = []
     for ci, (frees, (m, s_, rec), ports, n) in enumerate(meta):
@@ -1585,7 +1591,7 @@ def calculate_sparams_periodic_batch(cells, *, mode: str = "reference",
                 S[j, i] = (V - vinc) / vinc if i == j else V / vinc
         out.append(S)
     if _state is not None:
-        _state.update(dict(iters=iters, resid=resid, n=row_off[-1], nnz=nnz_off, batch=batch))
+        _state.update(dict(iters=iters, resid=resid, n=row_off[-1], nnz=nnz_off, batch=batch, solve_s=t_solve))
     return out
 
 

@@ -270,9 +270,10 @@ int efm_periodic_reduce(int64_t n, const int32_t *row_ptr, const int32_t *col, c
  * Many independent systems of different sizes in one batch (periodic unit cells): the
  * many-cell form of calculate_sparams_periodic (src/assemble_maxwell.cpp:576-635) as
  * UnitCellDesign calls it per cell (python/edgefem/designs/unit_cell.py:700-777).  The
- * reduced matrices of all cells form one block-diagonal CSR (host arrays: h_row_off
- * [ncells+1] row offsets, h_row_ptr [n+1], h_col [nnz] global columns inside the
- * cell, h_val c128).  Each cell has nrhs (1 or 2) excitations, and every (cell,
+ * reduced matrices of all cells form one block-diagonal CSR (h_row_off [ncells+1] row
+ * offsets on the host; row_ptr [n+1], col [nnz] global columns inside the cell, val
+ * c128 on the device, copied; a column outside its cell is rejected).  Each cell has
+ * nrhs (1 or 2) excitations, and every (cell,
  * excitation) pair is its own BiCGSTAB system (own scalars and convergence).  The
  * preconditioner is the complex-incidence auxiliary one over the concatenated node
  * space (h_node_off [ncells+1], h_edge_nodes [2n] global node or -1, h_edge_coefs [2n]
@@ -282,8 +283,8 @@ int efm_periodic_reduce(int64_t n, const int32_t *row_ptr, const int32_t *col, c
  * ---------------------------------------------------------------------- */
 typedef struct efm_cells efm_cells;
 
-int efm_cells_create(int64_t ncells, const int64_t *h_row_off, const int32_t *h_row_ptr,
-                     const int32_t *h_col, const double *h_val, const int64_t *h_node_off,
+int efm_cells_create(int64_t ncells, const int64_t *h_row_off, const int32_t *row_ptr,
+                     const int32_t *col, const double *val, int64_t nnz, const int64_t *h_node_off,
                      const int32_t *h_edge_nodes, const double *h_edge_coefs, int nrhs,
                      const int64_t *h_port_off, const int32_t *h_port_idx, const double *h_port_w,
                      const double *h_port_coef, void *stream, efm_cells **out);
