@@ -315,14 +315,23 @@ __global__ void __launch_bounds__(kBlock)
     if (row < n && lane == 0) {
       const double2 *w = Wa + row * 2 * R;
       double2 *wb = Wb + row * 2 * R;
+      // the whole record is written (z = 0; the update overwrites it) so that no partial
+      // 64-byte line reaches HBM, which would need a read-modify-write
+      double2 rec[2 * R];
+#pragma unroll
+      for (int s = 0; s < R; ++s) {
+        rec[s] = make_double2(0, 0);
+        rec[R + s] = cfma(beta[s], w[R + s], w[s]);
+      }
+#pragma unroll
+      for (int s = 0; s < 2 * R; ++s)
+        wb[s] = rec[s];
 #pragma unroll
       for (int s = 0; s < R; ++s) {
         if (!act[s])
           continue;
-        const double2 pn = cfma(beta[s], w[R + s], w[s]);
-        wb[R + s] = pn;
         q[row * R + s] = acc[s];
-        pq[s] = cadd(pq[s], bil(pn, acc[s]));
+        pq[s] = cadd(pq[s], bil(rec[R + s], acc[s]));
       }
     }
   }
@@ -420,14 +429,159 @@ __global__ void __launch_bounds__(kBlock)
     if (row < r1 && l == 0) {
       const double2 *w = Wa + (int64_t)row * 2 * R;
       double2 *wb = Wb + (int64_t)row * 2 * R;
+      // the whole record is written (z = 0; the update overwrites it) so that no partial
+      // 64-byte line reaches HBM, which would need a read-modify-write
+      double2 rec[2 * R];
+#pragma unroll
+      for (int s = 0; s < R; ++s) {
+        rec[s] = make_double2(0, 0);
+        rec[R + s] = cfma(beta[s], w[R + s], w[s]);
+      }
+#pragma unroll
+      for (int s = 0; s < 2 * R; ++s)
+        wb[s] = rec[s];
 #pragma unroll
       for (int s = 0; s < R; ++s) {
         if (!act[s])
           continue;
-        const double2 pn = cfma(beta[s], w[R + s], w[s]);
-        wb[R + s] = pn;
         q[(int64_t)row * R + s] = acc[s];
-        pq[s] = cadd(pq[s], bil(pn, acc[s]));
+        pq[s] = cadd(pq[s], bil(rec[R + s], acc[s]));
+      }
+    }
+    __syncthreads();  // prod is reused by the next tile
+  }
+  write_partials<R>(pq, lds, partial);
+}
+
+// The tiled SpMV with the next tile's descriptor, values, columns and row bounds loaded
+// while the current tile's column gathers are in flight (one dependent round trip per
+// tile instead of four).  desc[t] = (first row, end row, first nnz, end nnz).
+template <typename VT, int R, int TNNZ, int RL>
+__global__ void __launch_bounds__(kBlock)
+    k_cocg_spmv_pipe(int ntiles, int xcd_span, const int4 *__restrict__ desc,
+                     const int32_t *__restrict__ row_ptr, const int32_t *__restrict__ col,
+                     const VT *__restrict__ val, const double2 *__restrict__ Wa,
+                     double2 *__restrict__ Wb, double2 *__restrict__ q,
+                     double2 *__restrict__ partial, const SysState *__restrict__ st) {
+  __shared__ double2 prod[TNNZ * R];
+  __shared__ double2 lds[(kBlock / 64) * R];
+  constexpr int J = TNNZ / kBlock;
+  bool act[R];
+  double2 beta[R];
+#pragma unroll
+  for (int s = 0; s < R; ++s) {
+    act[s] = st[s].active != 0;
+    beta[s] = st[s].beta;
+  }
+  double2 pq[R];
+#pragma unroll
+  for (int s = 0; s < R; ++s)
+    pq[s] = make_double2(0, 0);
+  const int xcd = blockIdx.x & 7;
+  const int per_xcd = gridDim.x >> 3;
+  const int lb = blockIdx.x >> 3;
+  const int t_begin = xcd * xcd_span;
+  const int t_end = min(ntiles, t_begin + xcd_span);
+  const int g = threadIdx.x / RL, l = threadIdx.x % RL;
+  // staged state of the next tile
+  int4 dn = make_int4(0, 0, 0, 0);
+  double2 an[J];
+  int32_t cn[J];
+  int32_t k0n = 0, k1n = 0;
+  auto stage = [&](int tile) {
+    dn = desc[tile];
+#pragma unroll
+    for (int j = 0; j < J; ++j) {
+      const int32_t k = dn.z + threadIdx.x + j * kBlock;
+      if (k < dn.w) {
+        an[j] = to_d2(val[k]);
+        cn[j] = col[k];
+      } else {
+        an[j] = make_double2(0, 0);
+        cn[j] = -1;
+      }
+    }
+    const int32_t row = dn.x + g;
+    if (row < dn.y) {
+      k0n = row_ptr[row] - dn.z;
+      k1n = row_ptr[row + 1] - dn.z;
+    } else {
+      k0n = k1n = 0;
+    }
+  };
+  int tile = t_begin + lb;
+  if (tile < t_end)
+    stage(tile);
+  for (; tile < t_end; tile += per_xcd) {
+    const int4 d = dn;
+    double2 a[J];
+    int32_t c[J];
+#pragma unroll
+    for (int j = 0; j < J; ++j) {
+      a[j] = an[j];
+      c[j] = cn[j];
+    }
+    const int32_t k0 = k0n, k1 = k1n;
+    // gathers of the current tile
+    double2 wz[J][R], wp[J][R];
+#pragma unroll
+    for (int j = 0; j < J; ++j)
+      if (c[j] >= 0) {
+        const double2 *w = Wa + (int64_t)c[j] * 2 * R;
+#pragma unroll
+        for (int s = 0; s < R; ++s) {
+          wz[j][s] = w[s];
+          wp[j][s] = w[R + s];
+        }
+      }
+    // next tile's loads, issued behind the gathers
+    if (tile + per_xcd < t_end)
+      stage(tile + per_xcd);
+#pragma unroll
+    for (int j = 0; j < J; ++j) {
+      const int32_t lk = threadIdx.x + j * kBlock;
+      if (c[j] >= 0) {
+#pragma unroll
+        for (int s = 0; s < R; ++s)
+          prod[lk * R + s] = bil(a[j], cfma(beta[s], wp[j][s], wz[j][s]));
+      }
+    }
+    __syncthreads();
+    const int32_t row = d.x + g;
+    double2 acc[R];
+#pragma unroll
+    for (int s = 0; s < R; ++s)
+      acc[s] = make_double2(0, 0);
+    if (row < d.y)
+      for (int32_t k = k0 + l; k < k1; k += RL)
+#pragma unroll
+        for (int s = 0; s < R; ++s)
+          acc[s] = cadd(acc[s], prod[k * R + s]);
+#pragma unroll
+    for (int s = 0; s < R; ++s)
+#pragma unroll
+      for (int off = RL / 2; off > 0; off >>= 1) {
+        acc[s].x += __shfl_xor(acc[s].x, off, RL);
+        acc[s].y += __shfl_xor(acc[s].y, off, RL);
+      }
+    if (row < d.y && l == 0) {
+      const double2 *w = Wa + (int64_t)row * 2 * R;
+      double2 *wb = Wb + (int64_t)row * 2 * R;
+      double2 rec[2 * R];
+#pragma unroll
+      for (int s = 0; s < R; ++s) {
+        rec[s] = make_double2(0, 0);
+        rec[R + s] = cfma(beta[s], w[R + s], w[s]);
+      }
+#pragma unroll
+      for (int s = 0; s < 2 * R; ++s)
+        wb[s] = rec[s];
+#pragma unroll
+      for (int s = 0; s < R; ++s) {
+        if (!act[s])
+          continue;
+        q[(int64_t)row * R + s] = acc[s];
+        pq[s] = cadd(pq[s], bil(rec[R + s], acc[s]));
       }
     }
     __syncthreads();  // prod is reused by the next tile
@@ -547,14 +701,23 @@ __global__ void __launch_bounds__(kBlock)
     if (row < r1 && l == 0) {
       const double2 *w = Wa + (int64_t)row * 2 * R;
       double2 *wb = Wb + (int64_t)row * 2 * R;
+      // the whole record is written (z = 0; the update overwrites it) so that no partial
+      // 64-byte line reaches HBM, which would need a read-modify-write
+      double2 rec[2 * R];
+#pragma unroll
+      for (int s = 0; s < R; ++s) {
+        rec[s] = make_double2(0, 0);
+        rec[R + s] = cfma(beta[s], w[R + s], w[s]);
+      }
+#pragma unroll
+      for (int s = 0; s < 2 * R; ++s)
+        wb[s] = rec[s];
 #pragma unroll
       for (int s = 0; s < R; ++s) {
         if (!act[s])
           continue;
-        const double2 pn = cfma(beta[s], w[R + s], w[s]);
-        wb[R + s] = pn;
         q[(int64_t)row * R + s] = acc[s];
-        pq[s] = cadd(pq[s], bil(pn, acc[s]));
+        pq[s] = cadd(pq[s], bil(rec[R + s], acc[s]));
       }
     }
     __syncthreads();
@@ -583,20 +746,29 @@ __global__ void __launch_bounds__(kBlock)
   for (int64_t i = (int64_t)blockIdx.x * kBlock + threadIdx.x; i < n; i += (int64_t)gridDim.x * kBlock) {
     const double2 di = dinv[i];
     double2 *w = Wb + i * 2 * R;
+    double2 rec[2 * R];
+#pragma unroll
+    for (int s = 0; s < R; ++s) {
+      rec[s] = make_double2(0, 0);
+      rec[R + s] = w[R + s];
+    }
 #pragma unroll
     for (int s = 0; s < R; ++s) {
       if (!act[s])
         continue;
       const int64_t k = i * R + s;
-      const double2 xv = cfma(alpha[s], w[R + s], x[k]);
+      const double2 xv = cfma(alpha[s], rec[R + s], x[k]);
       const double2 rv = csub(r[k], bil(alpha[s], q[k]));
       const double2 zv = bil(di, rv);
       x[k] = xv;
       r[k] = rv;
-      w[s] = zv;
+      rec[s] = zv;
       v[s] = cadd(v[s], bil(rv, zv));
       v[R + s].x += cabs2(rv);
     }
+#pragma unroll
+    for (int s = 0; s < 2 * R; ++s)  // whole record: no partial line to merge in HBM
+      w[s] = rec[s];
   }
   write_partials<2 * R>(v, lds, partial);
 }
@@ -1215,22 +1387,31 @@ __global__ void __launch_bounds__(kBlock)
     const double2 di = dinv[e];
     const int2 ab = en_eff[e];
     double2 *w = Wb + e * 2 * R;
+    double2 rec[2 * R];
+#pragma unroll
+    for (int s = 0; s < R; ++s) {
+      rec[s] = make_double2(0, 0);
+      rec[R + s] = w[R + s];
+    }
 #pragma unroll
     for (int s = 0; s < R; ++s) {
       if (!act[s])
         continue;
       const int64_t k = e * R + s;
-      const double2 xv = cfma(alpha[s], w[R + s], x[k]);
+      const double2 xv = cfma(alpha[s], rec[R + s], x[k]);
       const double2 rv = csub(r[k], bil(alpha[s], q[k]));
       double2 zv = bil(di, rv);
       if (ab.x >= 0)
         zv = cadd(zv, csub(y[(int64_t)ab.y * R + s], y[(int64_t)ab.x * R + s]));
       x[k] = xv;
       r[k] = rv;
-      w[s] = zv;
+      rec[s] = zv;
       v[s] = cadd(v[s], bil(rv, zv));
       v[R + s].x += cabs2(rv);
     }
+#pragma unroll
+    for (int s = 0; s < 2 * R; ++s)  // whole record: no partial line to merge in HBM
+      w[s] = rec[s];
   }
   write_partials<2 * R>(v, lds, partial);
 }
@@ -1291,7 +1472,7 @@ __global__ void __launch_bounds__(kBlock)
 
 } // namespace
 
-constexpr int kNumSpmvVariants = 6;  // rows<4>, rows<8>, rows<16>, tiled<T,4>, tiled<T/2,8>, coop<T,4>
+constexpr int kNumSpmvVariants = 7;  // rows<4>, rows<8>, rows<16>, tiled<T,4>, tiled<T/2,8>, coop<T,4>, pipe<T,4>
 
 struct efm_solver {
   int64_t n = 0, nnz = 0;
@@ -1304,6 +1485,7 @@ struct efm_solver {
   efm::DevBuf val, dinv, x, r, q, W0, W1, p0, r0, v, y, zz, t, partial, state, control;
   // row tiles of the tiled SpMV variants (ntiles = 0: a row is longer than a tile)
   efm::DevBuf tile_row[2];
+  efm::DevBuf tile_desc;  // int4 (row0, row1, nnz0, nnz1) per tile of the first tiling
   int ntiles[2] = {0, 0}, xcd_span[2] = {0, 0}, grid_tiles[2] = {8, 8};
   int grid_rows[3] = {8, 8, 8};
   int spmv_variant = -1;  // -1: choose by timing at the first solve
@@ -1456,6 +1638,12 @@ template <int R> struct Dispatch {
       }
       set_error("SpMV variant 5 needs an even number of right-hand sides");
       return EFM_ERR_ARG;
+    case 6:
+      *nb = s->grid_tiles[0];
+      hipLaunchKernelGGL((k_cocg_spmv_pipe<VT, R, TN, 4>), dim3(*nb), dim3(kBlock), 0, st, s->ntiles[0],
+                         s->xcd_span[0], s->tile_desc.as<int4>(), s->row_ptr, s->col, s->val.as<VT>(), Wa, Wb,
+                         s->q.as<double2>(), s->partial.as<double2>(), S);
+      break;
     default:
       set_error("unknown SpMV variant");
       return EFM_ERR_ARG;
@@ -1478,6 +1666,8 @@ template <int R> struct Dispatch {
       if (v >= 3 && v <= 4 && s->ntiles[v - 3] == 0)
         continue;
       if (v == 5 && (s->ntiles[0] == 0 || (R % 2) != 0))
+        continue;
+      if (v == 6 && s->ntiles[0] == 0)
         continue;
       int nb;
       int rc = launch_spmv<VT>(s, v, s->W0.as<double2>(), s->W1.as<double2>(), st, &nb);  // warm-up
@@ -1685,6 +1875,14 @@ extern "C" int efm_solver_create(int64_t n, int64_t nnz, const int32_t *row_ptr,
       return rc;
     EFM_HIP_CHECK(hipMemcpy(s->tile_row[tv].p, tiles.data(), tiles.size() * sizeof(int32_t),
                             hipMemcpyHostToDevice));
+    if (tv == 0) {
+      std::vector<int4> dsc(s->ntiles[0]);
+      for (int t = 0; t < s->ntiles[0]; ++t)
+        dsc[t] = make_int4(tiles[t], tiles[t + 1], h_rp[tiles[t]], h_rp[tiles[t + 1]]);
+      if ((rc = s->tile_desc.alloc(dsc.size() * sizeof(int4))))
+        return rc;
+      EFM_HIP_CHECK(hipMemcpy(s->tile_desc.p, dsc.data(), dsc.size() * sizeof(int4), hipMemcpyHostToDevice));
+    }
   }
   const int lprs[3] = {4, 8, 16};
   for (int v = 0; v < 3; ++v) {
@@ -1915,7 +2113,7 @@ extern "C" int efm_solver_spmv(efm_solver *s, const double *x, double *y, void *
 extern "C" int efm_solver_set_spmv_variant(efm_solver *s, int variant) {
   EFM_REQUIRE(s, "efm_solver_set_spmv_variant: NULL solver");
   EFM_REQUIRE(variant >= -1 && variant < kNumSpmvVariants, "efm_solver_set_spmv_variant: bad variant");
-  EFM_REQUIRE(variant < 3 || s->ntiles[variant == 5 ? 0 : variant - 3] > 0,
+  EFM_REQUIRE(variant < 3 || s->ntiles[(variant == 5 || variant == 6) ? 0 : variant - 3] > 0,
               "efm_solver_set_spmv_variant: matrix has a row longer than a tile");
   EFM_REQUIRE(variant != 5 || s->R % 2 == 0,
               "efm_solver_set_spmv_variant: variant 5 needs an even number of right-hand sides");
