@@ -403,6 +403,49 @@ def cells_main(args):
     print(json.dumps(result))
 
 
+def traffic_from_pmc(fetch_csv, write_csv, out_path, variant, commands):
+    """Per-kernel HBM bytes per launch from two rocprofv3 --pmc runs (FETCH_SIZE, then
+    WRITE_SIZE, KiB per dispatch), each scaled by what the same counter reads for the
+    calibration copy (the largest device-to-device copy of the run, of known size)."""
+    import csv
+    import numpy as np
+
+    def load(path, counter):
+        per = {}
+        copy_kib = None
+        for r in csv.DictReader(open(path)):
+            if r["Counter_Name"] != counter:
+                continue
+            name = r["Kernel_Name"]
+            v = float(r["Counter_Value"])
+            if name.startswith("__amd_rocclr_copyBuffer"):
+                copy_kib = v if copy_kib is None else max(copy_kib, v)   # the calibration copy is the largest
+            short = name.replace("void ", "").replace("(anonymous namespace)::", "").replace("efm::", "")
+            per.setdefault(short.split("(")[0], []).append(v)
+        return {k: float(np.mean(v)) for k, v in per.items()}, copy_kib
+
+    fetch, cf = load(fetch_csv, "FETCH_SIZE")
+    write, cw = load(write_csv, "WRITE_SIZE")
+    copy_bytes = None
+    prev = json.load(open(out_path)) if os.path.exists(out_path) else {}
+    copy_bytes = prev.get("calibration", {}).get("copy_bytes", 512 << 20)
+    ff = copy_bytes / (cf * 1024.0)
+    fw = copy_bytes / (cw * 1024.0)
+    kernels = {}
+    for k in fetch:
+        if k in write and not k.startswith(("__amd", "at::")):
+            kernels[k] = dict(fetch_KiB=fetch[k], write_KiB=write[k],
+                              hbm_bytes=fetch[k] * 1024 * ff + write[k] * 1024 * fw)
+    out = dict(workload="wr90_48x24x192_slab_10ghz", variant=variant, commands=commands,
+               calibration=dict(copy_bytes=copy_bytes, copy_fetch_KiB=cf, copy_write_KiB=cw, fetch_factor=ff,
+                                write_factor=fw,
+                                note="the calibration copy's FETCH_SIZE and WRITE_SIZE against its known size "
+                                     "give the factors that every kernel's counters are scaled by"),
+               kernels=kernels)
+    json.dump(out, open(out_path, "w"), indent=1)
+    print(json.dumps(out, indent=1))
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--workload", choices=["c2", "sweep", "adjoint", "cells"], default="c2",
@@ -428,7 +471,16 @@ def main():
                     help="fixed COCG SpMV variant (-1: autotune at the first solve)")
     ap.add_argument("--calibrate-copy", type=int, default=0, metavar="MB",
                     help="first run a plain device-to-device copy of this many MB (counter calibration)")
+    ap.add_argument("--traffic-from", nargs=2, metavar=("FETCH_CSV", "WRITE_CSV"),
+                    help="write profiles/traffic_c2.json from the counter CSVs of two rocprofv3 --pmc runs")
     args = ap.parse_args()
+    if args.traffic_from:
+        cmd = ("rocprofv3 --pmc {c} --output-format csv -d gpurun_out/pmc_{d} -o {d} -- python -u bench.py --no-cpu "
+               "--no-check --steps 1 --warmup 0 --max-it 20 --spmv-variant {v} --calibrate-copy 512")
+        v = max(args.spmv_variant, 0) if args.spmv_variant >= 0 else 3
+        traffic_from_pmc(args.traffic_from[0], args.traffic_from[1], os.path.join(ROOT, "profiles", "traffic_c2.json"),
+                         v, [cmd.format(c="FETCH_SIZE", d="fetch", v=v), cmd.format(c="WRITE_SIZE", d="write", v=v)])
+        return
     if args.gpus != 1:
         print(json.dumps({"error": "this build runs on one GPU; --gpus must be 1 "
                                    "(multi-GPU sharding is a design note only)"}))
