@@ -33,6 +33,7 @@ EXPORTS = [
     "efm_sweep_create", "efm_sweep_destroy", "efm_sweep_solve", "efm_eps_gradient",
     "efm_solver_set_lowrank", "efm_add_diagonal", "efm_periodic_reduce", "efm_solver_set_gradient_complex",
     "efm_cells_create", "efm_cells_destroy", "efm_cells_solve",
+    "efm_sweep_set_variants", "efm_sweep_get_variants",
 ]
 
 
@@ -111,6 +112,8 @@ def load_library(path: str = LIB_PATH) -> C.CDLL:
     lib.efm_eps_gradient.argtypes = [vp, vp, C.c_int64, vp, i32, vp, i32, vp, i32, vp, vp, vp, vp, vp]
     lib.efm_sweep_destroy.argtypes = [vp]
     lib.efm_sweep_destroy.restype = None
+    lib.efm_sweep_set_variants.argtypes = [vp, i32, i32]
+    lib.efm_sweep_get_variants.argtypes = [vp, C.POINTER(C.c_int), C.POINTER(C.c_int), vp]
     lib.efm_sweep_solve.argtypes = [vp, i32, vp, vp, vp, vp, C.POINTER(SolveOpts), C.POINTER(RefineOpts), vp, vp,
                                     vp]
     lib.efm_port_overlap.argtypes = [i64, i32, i64, vp, vp, vp, i32, vp, vp]
@@ -461,6 +464,20 @@ class SweepSolver:
                                          dptr(Bd), dptr(X), C.byref(opts), ref, iters.ctypes.data_as(C.c_void_p),
                                          resid.ctypes.data_as(C.c_void_p), stream_ptr()), "efm_sweep_solve")
         return X, dict(iters=iters, resid=resid)
+
+    def set_variants(self, spmv: int = -1, aux: int = -1) -> None:
+        """Kernel variants of the SpMV and of the nodal preconditioner part: 0 one block
+        slice per group of 64 systems, 1 pipelined (groups in turn), -1 the faster of the
+        two, timed on the first solve."""
+        _check(self._lib.efm_sweep_set_variants(self._h, int(spmv), int(aux)), "efm_sweep_set_variants")
+
+    def variants(self):
+        """(SpMV variant, nodal variant, tuning ms per launch [SpMV 0, SpMV 1, nodal 0, nodal 1])."""
+        v, a = C.c_int(), C.c_int()
+        ms = np.zeros(4, np.float64)
+        _check(self._lib.efm_sweep_get_variants(self._h, C.byref(v), C.byref(a), ms.ctypes.data_as(C.c_void_p)),
+               "efm_sweep_get_variants")
+        return v.value, a.value, ms
 
     def close(self):
         if getattr(self, "_h", None):

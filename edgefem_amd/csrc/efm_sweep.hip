@@ -33,6 +33,7 @@ constexpr int kMaxPorts = 4;
 
 struct SwState {
   double2 rho, alpha, beta;
+  double2 pend;           // alpha of the last step whose x and r updates are still pending
   double bnorm2, rnorm2;  // inner solve: right-hand side and recursive residual norms^2
   double bn_out, rn_out;  // refinement: |b|^2 and the last true residual |b - A x|^2
   int active, iters, breakdown, total_iters;
@@ -109,6 +110,11 @@ __device__ inline double bcast(double v, int l) {
 // (bilinear forms: COCG for complex-symmetric A).  Kernels per iteration: aux_node
 // (nodal part of P on r_{i+1}, formed on the fly), update (p, s, x, r, u and partials
 // of g, |r|^2), spmv (w and partials of d), finalize (scalars, convergence).
+// x and r are stored every other iteration only: after a step i whose stores are
+// skipped, x_i and r_i in memory lag by pend = alpha_i, and the next step applies
+//   x_{i+2} = x_i + pend p_i + alpha_{i+1} p_{i+1},   r_{i+2} = r_i - pend s_i - alpha_{i+1} s_{i+1}
+// (p_i and s_i are always stored).  A system that stops with a pending step gets
+// x += pend p once after the loop.
 // ---------------------------------------------------------------------------
 // x = p = s = 0; r = b (b == nullptr: r already holds the right-hand side)
 template <typename VT>
@@ -125,7 +131,8 @@ __global__ void __launch_bounds__(kBlock)
   }
 }
 
-// y[n][s] = Dn_s(n)^{-1} sum_k G (r - alpha (w + beta s))[e][s]   (FUSED), or G^T r.
+// y[n][s] = Dn_s(n)^{-1} sum_k G (r - alpha w - (alpha beta + pend) s)[e][s]   (FUSED),
+// or G^T r (r, s as stored: this is G^T r_{i+1}).
 // One node per wavefront; the node's edge list and gradient signs are loaded across the
 // lanes and broadcast, so the gathers of its edges are independent of each other.
 template <typename VT, bool FUSED>
@@ -145,7 +152,8 @@ __global__ void __launch_bounds__(kBlock)
   if (__ballot(act) == 0)
     return;
   const double2 alpha = FUSED ? st[s].alpha : make_double2(0, 0);
-  const double2 beta = FUSED ? st[s].beta : make_double2(0, 0);
+  // r_{i+1} = r - alpha w - (alpha beta + pend) s with r, s as stored
+  const double2 abp = FUSED ? cadd(cmul(st[s].alpha, st[s].beta), st[s].pend) : make_double2(0, 0);
   const double k2 = k0sq[s];
   for (int64_t nd = (int64_t)blockIdx.x * kWaves + wv; nd < N; nd += (int64_t)gridDim.x * kWaves) {
     double2 acc = make_double2(0, 0);
@@ -165,7 +173,7 @@ __global__ void __launch_bounds__(kBlock)
             const int64_t i = (int64_t)bcast(ej, j + t) * B + s;
             rv[t] = ldv(r + i);
             if (FUSED)
-              rv[t] = csub(rv[t], bil(alpha, cfma(beta, ldv(sv + i), ldv(w + i))));
+              rv[t] = csub(csub(rv[t], bil(alpha, ldv(w + i))), bil(abp, ldv(sv + i)));
           }
 #pragma unroll
         for (int t = 0; t < U; ++t)
@@ -189,6 +197,111 @@ __global__ void __launch_bounds__(kBlock)
   }
 }
 
+// dninv[n][s] = Dn_s(n)^{-1} (0 where Dn_s(n) = 0) for the systems of one solve.
+__global__ void __launch_bounds__(kBlock)
+    k_sw_dninv(int64_t N, int B, int P, const double2 *__restrict__ dnK, const double2 *__restrict__ dnM,
+               const double *__restrict__ dnMs, const double *__restrict__ k0sq,
+               const double2 *__restrict__ coef, double2 *__restrict__ dninv) {
+  for (int64_t i = (int64_t)blockIdx.x * kBlock + threadIdx.x; i < N * B; i += (int64_t)gridDim.x * kBlock) {
+    const int64_t nd = i / B;
+    const int s = (int)(i - nd * B);
+    const double k2 = k0sq[s];
+    double2 d = make_double2(dnK[nd].x - k2 * dnM[nd].x, dnK[nd].y - k2 * dnM[nd].y);
+    for (int p = 0; p < P; ++p) {
+      const double m = dnMs[(int64_t)p * N + nd];
+      const double2 c = coef[s * P + p];
+      d.x += c.x * m;
+      d.y += c.y * m;
+    }
+    dninv[i] = (d.x == 0.0 && d.y == 0.0) ? make_double2(0, 0) : cdiv(make_double2(1, 0), d);
+  }
+}
+
+// y[n][s] = Dn_s(n)^{-1} sum_k G (r - alpha (w + beta s))[e][s]   (FUSED), or G^T r:
+// the same map as k_sw_aux_node, with the node walk of k_sw_spmv_pre.  Groups of 64
+// systems are walked in turn, the blocks of an XCD walk a contiguous node range (so an
+// edge gathered by both of its end nodes is read from HBM once), each wave preloads the
+// edge-list bounds of its next 64 nodes, and the gathers of a batch of NE edges (three
+// vectors each when FUSED) are issued with no branch: lanes past the node's degree hold
+// a valid edge with gradient sign 0.
+template <typename VT, bool FUSED>
+__global__ void __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(FUSED ? 3 : 4, 8)))
+    k_sw_aux_pre(int64_t N, int B, int groups, const int32_t *__restrict__ n2e_ptr,
+                 const int32_t *__restrict__ n2e, const int8_t *__restrict__ gsign,
+                 const double2 *__restrict__ dninv, const VT *__restrict__ r,
+                 const VT *__restrict__ w, const VT *__restrict__ sv, VT *__restrict__ y,
+                 const SwState *__restrict__ st) {
+  constexpr int NE = FUSED ? 7 : 14;
+  const int lane = threadIdx.x & 63;
+  const int wv = threadIdx.x >> 6;
+  const int64_t nchunks = (N + kWaves - 1) / kWaves;
+  ChunkWalk cw(nchunks);
+  const int64_t nr = (cw.begin < cw.end) ? (cw.end - cw.begin + cw.stride - 1) / cw.stride : 0;
+  for (int g = blockIdx.y; g < groups; g += gridDim.y) {
+    const int s = g * kSys + lane;
+    const bool act = s < B && st[s].active != 0;
+    if (__ballot(act) == 0)
+      continue;
+    const double2 alpha = FUSED ? st[s].alpha : make_double2(0, 0);
+    const double2 ab = FUSED ? cadd(cmul(st[s].alpha, st[s].beta), st[s].pend) : make_double2(0, 0);
+    for (int64_t jb = 0; jb < nr; jb += 64) {
+      const int cnt = (int)min<int64_t>(64, nr - jb);
+      int lo = 0, hi = 0;
+      if (lane < cnt) {
+        const int64_t nd = (cw.begin + (jb + lane) * cw.stride) * kWaves + wv;
+        if (nd < N) {
+          lo = n2e_ptr[nd];
+          hi = n2e_ptr[nd + 1];
+        }
+      }
+      for (int j = 0; j < cnt; ++j) {
+        const int64_t nd = (cw.begin + (jb + j) * cw.stride) * kWaves + wv;
+        const int64_t ndc = min<int64_t>(nd, N - 1);
+        const double2 dinv = dninv[ndc * B + s];
+        const int32_t beg = bcast(lo, j), len = bcast(hi, j) - beg;
+        double2 sr = make_double2(0, 0), sw = make_double2(0, 0), ss = make_double2(0, 0);
+        for (int32_t b = 0; b < len; b += 64) {
+          const int m = min(64, len - b);
+          const bool in = lane < m;
+          const int32_t ek = n2e[beg + b + (in ? lane : 0)];
+          const int ej = ek;
+          const int gj = in ? (int)gsign[beg + b + lane] : 0;
+          for (int e0 = 0; e0 < m; e0 += NE) {
+            double2 xr[NE], xw[NE], xs[NE];
+#pragma unroll
+            for (int t = 0; t < NE; ++t) {
+              const int64_t i = (int64_t)bcast(ej, min(e0 + t, 63)) * B + s;
+              xr[t] = ldv(r + i);
+              if (FUSED) {
+                xw[t] = ldv(w + i);
+                xs[t] = ldv(sv + i);
+              }
+            }
+#pragma unroll
+            for (int t = 0; t < NE; ++t) {
+              // lanes past the batch's last edge hold sign 0 (e0 + t < 64 keeps the lane valid)
+              const double gs = (e0 + t < m) ? (double)bcast(gj, min(e0 + t, 63)) : 0.0;
+              sr.x += gs * xr[t].x;
+              sr.y += gs * xr[t].y;
+              if (FUSED) {
+                sw.x += gs * xw[t].x;
+                sw.y += gs * xw[t].y;
+                ss.x += gs * xs[t].x;
+                ss.y += gs * xs[t].y;
+              }
+            }
+          }
+        }
+        if (act && nd < N) {
+          // sum G (r - alpha w - (alpha beta + pend) s)
+          const double2 acc = FUSED ? csub(csub(sr, cmul(alpha, sw)), cmul(ab, ss)) : sr;
+          stv(y + nd * B + s, cmul(dinv, acc));
+        }
+      }
+    }
+  }
+}
+
 __device__ inline double2 edge_diag_inv(int64_t e, int s, int P, double k2, const double2 *dK,
                                         const double2 *dM, const double *dMs,
                                         const int8_t *rport, const double2 *coef) {
@@ -199,8 +312,10 @@ __device__ inline double2 edge_diag_inv(int64_t e, int s, int P, double k2, cons
 }
 
 // INIT: u = P r; partials g = u^T r, |r|^2.
-// else: p = u + beta p; s = w + beta s; x += alpha p; r -= alpha s; u = P r; partials.
-template <typename VT, bool INIT>
+// else: p = u + beta p; s = w + beta s; x += alpha p; r -= alpha s; u = P r; partials,
+// with x and r read and stored only when APPLY (the pending step of the previous
+// iteration, pend, is folded in first: x += pend p_old, r -= pend s_old).
+template <typename VT, bool INIT, bool APPLY>
 __global__ void __launch_bounds__(kBlock)
     k_sw_update(int64_t n, int B, int P, const int2 *__restrict__ en_eff,
                 const double2 *__restrict__ dK, const double2 *__restrict__ dM,
@@ -217,18 +332,23 @@ __global__ void __launch_bounds__(kBlock)
   const bool act = s < B && st[s].active != 0;
   if (act) {
     const double k2 = k0sq[s];
-    const double2 alpha = st[s].alpha, beta = st[s].beta;
+    const double2 alpha = st[s].alpha, beta = st[s].beta, pend = st[s].pend;
     for (int64_t e = (int64_t)blockIdx.x * kWaves + wv; e < n; e += (int64_t)gridDim.x * kWaves) {
       const int64_t i = e * B + s;
       double2 rv = ldv(r + i);
       if (!INIT) {
-        const double2 pv = cfma(beta, ldv(p + i), ldv(u + i));
-        const double2 svv = cfma(beta, ldv(sv + i), ldv(w + i));
-        stv(x + i, cfma(alpha, pv, ldv(x + i)));
+        const double2 po = ldv(p + i), so = ldv(sv + i);
+        const double2 pv = cfma(beta, po, ldv(u + i));
+        const double2 svv = cfma(beta, so, ldv(w + i));
+        if (APPLY) {
+          stv(x + i, cfma(alpha, pv, cfma(pend, po, ldv(x + i))));
+          rv = csub(rv, bil(pend, so));
+        }
         rv = csub(rv, bil(alpha, svv));
         stv(p + i, pv);
         stv(sv + i, svv);
-        stv(r + i, rv);
+        if (APPLY)
+          stv(r + i, rv);
       }
       const double2 di = edge_diag_inv(e, s, P, k2, dK, dM, dMs, rport, coef);
       double2 z = bil(di, rv);
@@ -410,6 +530,138 @@ __global__ void __launch_bounds__(kBlock)
   }
 }
 
+// Row metadata for the pipelined SpMV: bounds and port of 64 upcoming rows are held one
+// per lane, so staging a row's columns and values needs no dependent row_ptr load.
+// Lanes past the row's length hold the row's own index as column and zero values, so
+// a fixed number of gathers can be issued with no branch (exact wait counts).
+struct RowStage2 {
+  int32_t len;  // wave-uniform
+  int port;     // wave-uniform
+  int cj;
+  double kx, ky, mx, my, ms;
+  double2 c;
+};
+
+// One row with NU >= len gathers, all issued before the next row is staged.
+template <int NU, typename VT, typename StageFn>
+__device__ inline double2 sw_row_fixed(const RowStage2 &cur, RowStage2 &nxt, StageFn &&stage, int jn,
+                                       int64_t rowc, int s, int B, double k2, const VT *__restrict__ u,
+                                       double2 &ud) {
+  ud = ldv(u + rowc * B + s);
+  double2 xv[NU];
+#pragma unroll
+  for (int t = 0; t < NU; ++t)
+    xv[t] = ldv(u + (int64_t)bcast(cur.cj, t) * B + s);
+  stage(nxt, jn);
+  // K x, M x and Ms x accumulated apart (each product takes one scalar operand), then
+  // combined with this lane's k0^2 and port coefficient once per row
+  double2 aK = make_double2(0, 0), aM = make_double2(0, 0), aS = make_double2(0, 0);
+#pragma unroll
+  for (int t = 0; t < NU; ++t) {
+    aK = cfma(make_double2(bcast(cur.kx, t), bcast(cur.ky, t)), xv[t], aK);
+    aM = cfma(make_double2(bcast(cur.mx, t), bcast(cur.my, t)), xv[t], aM);
+  }
+  if (cur.port >= 0) {
+#pragma unroll
+    for (int t = 0; t < NU; ++t) {
+      const double m = bcast(cur.ms, t);
+      aS.x += m * xv[t].x;
+      aS.y += m * xv[t].y;
+    }
+  }
+  return cfma(cur.c, aS, make_double2(aK.x - k2 * aM.x, aK.y - k2 * aM.y));
+}
+
+// w = A_s u; partials d = u^T w.  Groups of 64 systems are walked one after the other
+// (gridDim.y == 1, so all resident blocks work on the same group and the XCD's share
+// of u for that group, rows +- the matrix bandwidth, stays in its L2).  Each wave
+// preloads the bounds of its next 64 rows across its lanes, stages the next row's
+// columns and values while the current row's gathers are in flight, and issues a fixed
+// number of gathers per row (13 or 20; longer rows take the general path).
+template <typename VT>
+__global__ void __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(3, 8)))
+    k_sw_spmv_pre(int64_t n, int B, int P, int groups, const int32_t *__restrict__ row_ptr,
+                  const int32_t *__restrict__ col, const double2 *__restrict__ K,
+                  const double2 *__restrict__ M, const double *__restrict__ Ms,
+                  const int8_t *__restrict__ rport, const double *__restrict__ k0sq,
+                  const double2 *__restrict__ coef, const VT *__restrict__ u,
+                  VT *__restrict__ w, double2 *__restrict__ partial,
+                  const SwState *__restrict__ st) {
+  __shared__ double2 lds[kWaves * 64];
+  const int lane = threadIdx.x & 63;
+  const int wv = threadIdx.x >> 6;
+  const int64_t nchunks = (n + kWaves - 1) / kWaves;
+  ChunkWalk cw(nchunks);
+  // rows of this wave: j = 0 .. nr-1, row_j = (begin + j stride) kWaves + wv (may be >= n
+  // in the last chunk: such a row has length 0, reads row n-1 and writes nothing)
+  const int64_t nr = (cw.begin < cw.end) ? (cw.end - cw.begin + cw.stride - 1) / cw.stride : 0;
+  for (int g = blockIdx.y; g < groups; g += gridDim.y) {
+    const int s = g * kSys + lane;
+    double2 v[1] = {make_double2(0, 0)};
+    const bool act = s < B && st[s].active != 0;
+    if (__ballot(act) != 0) {
+      const double k2 = k0sq[s];
+      for (int64_t jb = 0; jb < nr; jb += 64) {
+        const int cnt = (int)min<int64_t>(64, nr - jb);
+        int lo = 0, hi = 0, pt = -1;
+        if (lane < cnt) {
+          const int64_t row = (cw.begin + (jb + lane) * cw.stride) * kWaves + wv;
+          if (row < n) {
+            lo = row_ptr[row];
+            hi = row_ptr[row + 1];
+            pt = rport[row];
+          }
+        }
+        auto row_of = [&](int j) { return min<int64_t>((cw.begin + (jb + j) * cw.stride) * kWaves + wv, n - 1); };
+        // stage row j (j < cnt); every load is unconditional
+        auto stage = [&](RowStage2 &q, int j) {
+          const int32_t beg = bcast(lo, j);
+          q.len = bcast(hi, j) - beg;
+          q.port = bcast(pt, j);
+          const double2 cv = coef[s * P + max(q.port, 0)];  // coef holds >= 1 entry per system
+          q.c = (q.port >= 0) ? cv : make_double2(0, 0);
+          const bool in = lane < q.len;
+          const int32_t k = in ? beg + lane : 0;
+          const int cl = col[k];
+          const double2 kv = K[k], mv = M[k];
+          const double mw = Ms[k];
+          q.cj = in ? cl : (int)row_of(j);
+          q.kx = in ? kv.x : 0.0;
+          q.ky = in ? kv.y : 0.0;
+          q.mx = in ? mv.x : 0.0;
+          q.my = in ? mv.y : 0.0;
+          q.ms = in ? mw : 0.0;
+        };
+        RowStage2 cur, nxt;
+        stage(cur, 0);
+        for (int j = 0; j < cnt; ++j) {
+          const int64_t row = (cw.begin + (jb + j) * cw.stride) * kWaves + wv;
+          const int64_t rowc = min<int64_t>(row, n - 1);
+          const int jn = min(j + 1, cnt - 1);
+          double2 acc, ud;
+          if (cur.len <= 13) {
+            acc = sw_row_fixed<13>(cur, nxt, stage, jn, rowc, s, B, k2, u, ud);
+            cur = nxt;
+          } else if (cur.len <= 20) {
+            acc = sw_row_fixed<20>(cur, nxt, stage, jn, rowc, s, B, k2, u, ud);
+            cur = nxt;
+          } else {
+            acc = sw_row_apply(rowc, s, B, P, k2, row_ptr, col, K, M, Ms, rport, coef, u);
+            ud = ldv(u + rowc * B + s);
+            stage(cur, jn);
+          }
+          if (act && row < n) {
+            stv(w + row * B + s, acc);
+            v[0] = cadd(v[0], bil(ud, acc));
+          }
+        }
+      }
+    }
+    write_sys_partials<1>(v, lds, partial, g * gridDim.x + blockIdx.x);
+    __syncthreads();  // lds reuse by the next group
+  }
+}
+
 // Iterative refinement, outer step: r = b - A_s x in FP64 (x == nullptr: r = b), stored
 // in the inner solve's vector type; partials of |r|^2 (before rounding).  Systems whose
 // outer iteration has finished are skipped.
@@ -466,6 +718,23 @@ __global__ void __launch_bounds__(kBlock)
   }
 }
 
+// After a solve: x += pend p for the systems that stopped with a pending step.
+template <typename VT>
+__global__ void __launch_bounds__(kBlock)
+    k_sw_flush(int64_t n, int B, VT *__restrict__ x, const VT *__restrict__ p, const SwState *__restrict__ st) {
+  const int s = blockIdx.y * kSys + (threadIdx.x & 63);
+  const int wv = threadIdx.x >> 6;
+  if (s >= B)
+    return;
+  const double2 pend = st[s].pend;
+  if (pend.x == 0.0 && pend.y == 0.0)
+    return;
+  for (int64_t e = (int64_t)blockIdx.x * kWaves + wv; e < n; e += (int64_t)gridDim.x * kWaves) {
+    const int64_t i = e * B + s;
+    stv(x + i, cfma(pend, ldv(p + i), ldv(x + i)));
+  }
+}
+
 // Block-wide fixed-order sum of per-thread values.
 template <int V>
 __device__ inline void block_sum(double2 (&v)[V], double2 (*sh)[kBlock]) {
@@ -519,11 +788,12 @@ __global__ void __launch_bounds__(kBlock)
 
 // One block per system: reduce its partials of g, |r|^2 (update grid, gxu blocks) and
 // d (spmv grid, gxs blocks) in a fixed order and advance the scalars.
+// skip: this iteration's update stored neither x nor r, so its alpha becomes pending.
 template <bool INIT>
 __global__ void __launch_bounds__(kBlock)
     k_sw_finalize(int B, int gxu, int gxs, const double2 *__restrict__ pu,
                   const double2 *__restrict__ ps, SwState *__restrict__ st,
-                  int *__restrict__ active_flag, double tol2, int max_it, int max_inner) {
+                  int *__restrict__ active_flag, double tol2, int max_it, int max_inner, int skip) {
   const int s = blockIdx.x;
   if (s >= B)
     return;
@@ -549,6 +819,7 @@ __global__ void __launch_bounds__(kBlock)
     S.iters = 0;
     S.breakdown = 0;
     S.beta = make_double2(0, 0);
+    S.pend = make_double2(0, 0);
     S.rho = g;
     S.active = (S.active && rn2 > 0.0) ? 1 : 0;
     if (S.active) {
@@ -560,6 +831,7 @@ __global__ void __launch_bounds__(kBlock)
       }
     }
   } else if (S.active) {
+    S.pend = skip ? S.alpha : make_double2(0, 0);
     S.iters += 1;
     S.total_iters += 1;
     S.rnorm2 = rn2;
@@ -708,9 +980,19 @@ struct efm_sweep {
   const int32_t *row_ptr = nullptr, *col = nullptr;
   const int32_t *n2e_ptr = nullptr, *n2e = nullptr;
   int gx_vec = 8, gx_nodes = 8, gx_spmv = 8, gy_spmv = 1;
+  // SpMV kernel: 0 = one block slice per system group, 1 = pipelined group-sequential;
+  // -1 = chosen by timing both on the first solve
+  int spmv_variant = -1;
+  int gx_pre = 8;
+  // nodal kernel: 0 = one block slice per system group, 1 = pipelined, XCD-local node
+  // walk (groups in turn); -1 = chosen by timing both on the first solve
+  int aux_variant = -1;
+  int gx_aux = 8;
+  float tune_ms[4] = {0.f, 0.f, 0.f, 0.f};  // SpMV 0, 1; nodal 0, 1
   efm::DevBuf K, M, Ms, rport, dK, dM, dMs, dnK, dnM, dnMs, gsign, en_eff;
   efm::DevBuf x, r, p, sv, u, w, y, part_u, part_s, state, flags, k0sq, coef;
   efm::DevBuf x64;  // refinement: FP64 solution (allocated on first use)
+  efm::DevBuf dninv;  // [N][Bmax] inverse nodal diagonals of the systems of one solve
 };
 
 extern "C" int efm_sweep_create(const efm_topology *topo, const double *K_val, const double *M_val,
@@ -760,6 +1042,14 @@ extern "C" int efm_sweep_create(const efm_topology *topo, const double *K_val, c
     s->gy_spmv = groups;
     const int64_t g = std::min<int64_t>(want, std::max<int64_t>(need, 8));
     s->gx_spmv = (int)(((g + 7) / 8) * 8);
+    // pipelined variant: one block slice (gridDim.y = 1), as many blocks as are resident
+    // at 3 waves per SIMD (3 blocks of 4 waves per CU)
+    const int64_t gp = std::min<int64_t>((int64_t)ncu * 3, std::max<int64_t>(need, 8));
+    s->gx_pre = (int)(((gp + 7) / 8) * 8);
+    // pipelined nodal kernel: 3 waves per SIMD (the fused form's occupancy)
+    const int64_t needn = (N + kWaves - 1) / kWaves;
+    const int64_t ga = std::min<int64_t>((int64_t)ncu * 3, std::max<int64_t>(needn, 8));
+    s->gx_aux = (int)(((ga + 7) / 8) * 8);
   }
   if ((rc = s->K.alloc(nnz * 16)) || (rc = s->M.alloc(nnz * 16)) || (rc = s->Ms.alloc(nnz * 8)) ||
       (rc = s->rport.alloc(n + 16)) || (rc = s->dK.alloc(n * 16)) || (rc = s->dM.alloc(n * 16)) ||
@@ -768,8 +1058,9 @@ extern "C" int efm_sweep_create(const efm_topology *topo, const double *K_val, c
       (rc = s->en_eff.alloc(n * sizeof(int2))) || (rc = s->x.alloc(vb)) || (rc = s->r.alloc(vb)) ||
       (rc = s->p.alloc(vb)) || (rc = s->sv.alloc(vb)) || (rc = s->u.alloc(vb)) || (rc = s->w.alloc(vb)) ||
       (rc = s->y.alloc((size_t)N * s->Bmax * sizeof(double2))) ||
+      (rc = s->dninv.alloc((size_t)N * s->Bmax * sizeof(double2))) ||
       (rc = s->part_u.alloc((size_t)groups * s->gx_vec * 2 * 64 * sizeof(double2))) ||
-      (rc = s->part_s.alloc((size_t)groups * s->gx_spmv * 64 * sizeof(double2))) ||
+      (rc = s->part_s.alloc((size_t)groups * std::max(s->gx_spmv, s->gx_pre) * 64 * sizeof(double2))) ||
       (rc = s->state.alloc(s->Bmax * sizeof(SwState))) || (rc = s->flags.alloc(s->Bmax * sizeof(int))) ||
       (rc = s->k0sq.alloc(s->Bmax * sizeof(double))) ||
       (rc = s->coef.alloc((size_t)s->Bmax * std::max(1, n_ports) * sizeof(double2))))
@@ -814,6 +1105,25 @@ extern "C" int efm_sweep_create(const efm_topology *topo, const double *K_val, c
 
 extern "C" void efm_sweep_destroy(efm_sweep *s) { delete s; }
 
+extern "C" int efm_sweep_set_variants(efm_sweep *s, int spmv, int aux) {
+  EFM_REQUIRE(s, "efm_sweep_set_variants: NULL sweep");
+  EFM_REQUIRE(spmv >= -1 && spmv <= 1 && aux >= -1 && aux <= 1,
+              "efm_sweep_set_variants: variants must be -1, 0 or 1");
+  s->spmv_variant = spmv;
+  s->aux_variant = aux;
+  return EFM_OK;
+}
+
+extern "C" int efm_sweep_get_variants(const efm_sweep *s, int *h_spmv, int *h_aux, double *h_tune_ms) {
+  EFM_REQUIRE(s && h_spmv && h_aux, "efm_sweep_get_variants: NULL argument");
+  *h_spmv = s->spmv_variant;
+  *h_aux = s->aux_variant;
+  if (h_tune_ms)
+    for (int v = 0; v < 4; ++v)
+      h_tune_ms[v] = s->tune_ms[v];
+  return EFM_OK;
+}
+
 namespace {
 
 // One launch sequence over the padded system block [Bp]: everything a solve needs.
@@ -847,63 +1157,146 @@ struct SwRun {
     hipLaunchKernelGGL((k_sw_init<VT>), dim3(grid_for(n * Bp, kBlock)), dim3(kBlock), 0, st, n, Bp, b, x, r, p,
                        sv);
     EFM_LAUNCH_CHECK();
-    auto aux_node = [&](bool fused) -> int {
-      if (fused)
+    hipLaunchKernelGGL(k_sw_dninv, dim3(grid_for(s->N * Bp, kBlock)), dim3(kBlock), 0, st, s->N, Bp, P,
+                       s->dnK.as<double2>(), s->dnM.as<double2>(), s->dnMs.as<double>(), s->k0sq.as<double>(),
+                       s->coef.as<double2>(), s->dninv.as<double2>());
+    EFM_LAUNCH_CHECK();
+    auto aux_v = [&](bool fused, int variant) -> int {
+      const int8_t *gs = s->gsign.as<int8_t>();
+      const double2 *dn = s->dninv.as<double2>();
+      if (variant == 1) {
+        if (fused)
+          hipLaunchKernelGGL((k_sw_aux_pre<VT, true>), dim3(s->gx_aux, 1), dim3(kBlock), 0, st, s->N, Bp, groups,
+                             s->n2e_ptr, s->n2e, gs, dn, r, w, sv, y, S);
+        else
+          hipLaunchKernelGGL((k_sw_aux_pre<VT, false>), dim3(s->gx_aux, 1), dim3(kBlock), 0, st, s->N, Bp, groups,
+                             s->n2e_ptr, s->n2e, gs, dn, r, (const VT *)nullptr, (const VT *)nullptr, y, S);
+      } else if (fused) {
         hipLaunchKernelGGL((k_sw_aux_node<VT, true>), dim3(s->gx_nodes, groups), dim3(kBlock), 0, st, s->N, Bp,
-                           P, s->n2e_ptr, s->n2e, s->gsign.as<int8_t>(), s->dnK.as<double2>(),
-                           s->dnM.as<double2>(), s->dnMs.as<double>(), s->k0sq.as<double>(),
-                           s->coef.as<double2>(), r, w, sv, y, S);
-      else
+                           P, s->n2e_ptr, s->n2e, gs, s->dnK.as<double2>(), s->dnM.as<double2>(),
+                           s->dnMs.as<double>(), s->k0sq.as<double>(), s->coef.as<double2>(), r, w, sv, y, S);
+      } else {
         hipLaunchKernelGGL((k_sw_aux_node<VT, false>), dim3(s->gx_nodes, groups), dim3(kBlock), 0, st, s->N,
-                           Bp, P, s->n2e_ptr, s->n2e, s->gsign.as<int8_t>(), s->dnK.as<double2>(),
-                           s->dnM.as<double2>(), s->dnMs.as<double>(), s->k0sq.as<double>(),
-                           s->coef.as<double2>(), r, (const VT *)nullptr, (const VT *)nullptr, y, S);
+                           Bp, P, s->n2e_ptr, s->n2e, gs, s->dnK.as<double2>(), s->dnM.as<double2>(),
+                           s->dnMs.as<double>(), s->k0sq.as<double>(), s->coef.as<double2>(), r,
+                           (const VT *)nullptr, (const VT *)nullptr, y, S);
+      }
       EFM_LAUNCH_CHECK();
       return EFM_OK;
     };
-    auto update = [&](bool first) -> int {
+    // times `launch(v)` for v = 0, 1 (3 launches each after one untimed), keeps the faster
+    // in *variant and its times in ms[0..1]; the launches must be repeatable
+    auto time_pick = [&](int *variant, float *ms_out, auto &&launch) -> int {
+      hipEvent_t e0, e1;
+      EFM_HIP_CHECK(hipEventCreate(&e0));
+      EFM_HIP_CHECK(hipEventCreate(&e1));
+      int rc2 = EFM_OK;
+      float best = 0.f;
+      int pick = 0;
+      for (int v = 0; v < 2 && rc2 == EFM_OK; ++v) {
+        if ((rc2 = launch(v)))
+          break;
+        if (hipEventRecord(e0, st) != hipSuccess) {
+          rc2 = EFM_ERR_HIP;
+          break;
+        }
+        for (int k = 0; k < 3 && rc2 == EFM_OK; ++k)
+          rc2 = launch(v);
+        float ms = 0.f;
+        if (rc2 || hipEventRecord(e1, st) != hipSuccess || hipEventSynchronize(e1) != hipSuccess ||
+            hipEventElapsedTime(&ms, e0, e1) != hipSuccess) {
+          rc2 = rc2 ? rc2 : EFM_ERR_HIP;
+          break;
+        }
+        ms_out[v] = ms / 3.f;
+        if (v == 0 || ms / 3.f < best) {
+          best = ms / 3.f;
+          pick = v;
+        }
+      }
+      (void)hipEventDestroy(e0);
+      (void)hipEventDestroy(e1);
+      if (rc2 == EFM_OK)
+        *variant = pick;
+      return rc2;
+    };
+    auto aux_node = [&](bool fused) -> int {
+      if (fused && s->aux_variant < 0) {
+        // y is rewritten by every launch; the chosen kernel runs last
+        int rc2 = time_pick(&s->aux_variant, s->tune_ms + 2, [&](int v) { return aux_v(true, v); });
+        if (rc2)
+          return rc2;
+      }
+      return aux_v(fused, s->aux_variant == 1 ? 1 : 0);
+    };
+    auto update = [&](bool first, bool apply) -> int {
+      const double2 *dK = s->dK.as<double2>(), *dM = s->dM.as<double2>();
+      const double *dMs = s->dMs.as<double>(), *k2 = s->k0sq.as<double>();
+      const int8_t *rp = s->rport.as<int8_t>();
+      const double2 *cf = s->coef.as<double2>();
+      const dim3 grid(s->gx_vec, groups);
       if (first)
-        hipLaunchKernelGGL((k_sw_update<VT, true>), dim3(s->gx_vec, groups), dim3(kBlock), 0, st, n, Bp, P, en,
-                           s->dK.as<double2>(), s->dM.as<double2>(), s->dMs.as<double>(), s->rport.as<int8_t>(),
-                           s->k0sq.as<double>(), s->coef.as<double2>(), y, w, u, p, sv, x, r, pu, S);
+        hipLaunchKernelGGL((k_sw_update<VT, true, false>), grid, dim3(kBlock), 0, st, n, Bp, P, en, dK, dM, dMs, rp,
+                           k2, cf, y, w, u, p, sv, x, r, pu, S);
+      else if (apply)
+        hipLaunchKernelGGL((k_sw_update<VT, false, true>), grid, dim3(kBlock), 0, st, n, Bp, P, en, dK, dM, dMs, rp,
+                           k2, cf, y, w, u, p, sv, x, r, pu, S);
       else
-        hipLaunchKernelGGL((k_sw_update<VT, false>), dim3(s->gx_vec, groups), dim3(kBlock), 0, st, n, Bp, P, en,
-                           s->dK.as<double2>(), s->dM.as<double2>(), s->dMs.as<double>(), s->rport.as<int8_t>(),
-                           s->k0sq.as<double>(), s->coef.as<double2>(), y, w, u, p, sv, x, r, pu, S);
+        hipLaunchKernelGGL((k_sw_update<VT, false, false>), grid, dim3(kBlock), 0, st, n, Bp, P, en, dK, dM, dMs,
+                           rp, k2, cf, y, w, u, p, sv, x, r, pu, S);
       EFM_LAUNCH_CHECK();
       return EFM_OK;
     };
-    auto spmv = [&]() -> int {
-      hipLaunchKernelGGL((k_sw_spmv<VT>), dim3(s->gx_spmv, s->gy_spmv), dim3(kBlock), 0, st, n, Bp, P, groups,
-                         s->row_ptr, s->col, s->K.as<double2>(), s->M.as<double2>(), s->Ms.as<double>(),
-                         s->rport.as<int8_t>(), s->k0sq.as<double>(), s->coef.as<double2>(), u, w, ps, S);
+    auto spmv_v = [&](int variant) -> int {
+      if (variant == 1)
+        hipLaunchKernelGGL((k_sw_spmv_pre<VT>), dim3(s->gx_pre, 1), dim3(kBlock), 0, st, n, Bp, P, groups,
+                           s->row_ptr, s->col, s->K.as<double2>(), s->M.as<double2>(), s->Ms.as<double>(),
+                           s->rport.as<int8_t>(), s->k0sq.as<double>(), s->coef.as<double2>(), u, w, ps, S);
+      else
+        hipLaunchKernelGGL((k_sw_spmv<VT>), dim3(s->gx_spmv, s->gy_spmv), dim3(kBlock), 0, st, n, Bp, P, groups,
+                           s->row_ptr, s->col, s->K.as<double2>(), s->M.as<double2>(), s->Ms.as<double>(),
+                           s->rport.as<int8_t>(), s->k0sq.as<double>(), s->coef.as<double2>(), u, w, ps, S);
       EFM_LAUNCH_CHECK();
       return EFM_OK;
     };
-    auto finalize = [&](bool first) -> int {
+    auto spmv = [&]() -> int { return spmv_v(s->spmv_variant == 1 ? 1 : 0); };
+    // autotune: time both SpMV kernels on the real systems (w and the partials are
+    // rewritten by every launch, so the solve is unaffected); the chosen kernel runs last
+    auto tune = [&]() -> int {
+      if (s->spmv_variant >= 0)
+        return EFM_OK;
+      int rc2 = time_pick(&s->spmv_variant, s->tune_ms, [&](int v) { return spmv_v(v); });
+      return rc2 ? rc2 : spmv();
+    };
+    auto finalize = [&](bool first, bool skip) -> int {
+      const int gxs = s->spmv_variant == 1 ? s->gx_pre : s->gx_spmv;  // the SpMV grid that wrote ps
       if (first)
-        hipLaunchKernelGGL((k_sw_finalize<true>), dim3(Bp), dim3(kBlock), 0, st, Bp, s->gx_vec, s->gx_spmv, pu,
-                           ps, S, flags, inner_tol2, max_it, max_inner);
+        hipLaunchKernelGGL((k_sw_finalize<true>), dim3(Bp), dim3(kBlock), 0, st, Bp, s->gx_vec, gxs, pu, ps, S,
+                           flags, inner_tol2, max_it, max_inner, 0);
       else
-        hipLaunchKernelGGL((k_sw_finalize<false>), dim3(Bp), dim3(kBlock), 0, st, Bp, s->gx_vec, s->gx_spmv,
-                           pu, ps, S, flags, inner_tol2, max_it, max_inner);
+        hipLaunchKernelGGL((k_sw_finalize<false>), dim3(Bp), dim3(kBlock), 0, st, Bp, s->gx_vec, gxs, pu, ps, S,
+                           flags, inner_tol2, max_it, max_inner, skip ? 1 : 0);
       EFM_LAUNCH_CHECK();
       return EFM_OK;
     };
     int rc, active = 0;
-    if ((rc = aux_node(false)) || (rc = update(true)) || (rc = spmv()) || (rc = finalize(true)) ||
-        (rc = n_active(active)))
+    if ((rc = aux_node(false)) || (rc = update(true, false)) || (rc = spmv()) || (rc = tune()) ||
+        (rc = finalize(true, false)) || (rc = n_active(active)))
       return rc;
     int it = 0;
     const int cap = std::min(max_it, max_inner);
     while (active > 0 && it < cap) {
       const int chunk = std::min(check, cap - it);
-      for (int c = 0; c < chunk; ++c, ++it)
-        if ((rc = aux_node(true)) || (rc = update(false)) || (rc = spmv()) || (rc = finalize(false)))
+      for (int c = 0; c < chunk; ++c, ++it) {
+        const bool apply = (it & 1) != 0;  // x and r are stored on odd iterations
+        if ((rc = aux_node(true)) || (rc = update(false, apply)) || (rc = spmv()) || (rc = finalize(false, !apply)))
           return rc;
+      }
       if ((rc = n_active(active)))
         return rc;
     }
+    hipLaunchKernelGGL((k_sw_flush<VT>), dim3(s->gx_vec, groups), dim3(kBlock), 0, st, n, Bp, x, (const VT *)p, S);
+    EFM_LAUNCH_CHECK();
     return EFM_OK;
   }
 

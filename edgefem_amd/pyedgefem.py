@@ -1902,17 +1902,27 @@ def frequency_sweep_eigenmode(mesh: Mesh, params: MaxwellParams, bc: BC, ports: 
     nsys_all = len(sys_k2)
     batch = int(min(max_batch, nsys_all, 1024))
     sweep = _capi.SweepSolver(topo, K, M, ops.Ms, dmask, batch)
+    if _state is not None and ("spmv_variant" in _state or "aux_variant" in _state):
+        sweep.set_variants(int(_state.get("spmv_variant", -1)), int(_state.get("aux_variant", -1)))
     n = mesh.num_edges()
     tol = 1e-14 if opts.use_direct else float(opts.tolerance)
     infos = []
+    dev = topo.row_ptr.device
     for b0 in range(0, nsys_all, batch):
         b1 = min(nsys_all, b0 + batch)
         idx = range(b0, b1)
-        Bh = np.zeros((n, b1 - b0), np.complex128)
+        # right-hand sides 2 c_a g_a on the port rows, scattered on the device (only the
+        # port entries travel from the host)
+        rows_l, cols_l, vals_l = [], [], []
         for c, si in enumerate(idx):
             a = sys_a[si]
-            Bh[ops.g_idx[a], c] = 2.0 * sys_coef[si][a] * ops.g_val[a]
-        Bd = _capi.to_dev(Bh, torch.complex128)
+            rows_l.append(ops.g_idx[a].astype(np.int64))
+            cols_l.append(np.full(ops.g_idx[a].size, c, np.int64))
+            vals_l.append(2.0 * sys_coef[si][a] * ops.g_val[a])
+        Bd = torch.zeros((n, b1 - b0), dtype=torch.complex128, device=dev)
+        if rows_l:
+            flat = torch.from_numpy(np.concatenate(rows_l) * (b1 - b0) + np.concatenate(cols_l)).to(dev)
+            Bd.view(-1).index_put_((flat,), torch.from_numpy(np.concatenate(vals_l)).to(dev))
         X, info = sweep.solve([sys_k2[si] for si in idx], [sys_coef[si] for si in idx], Bd, tol=tol,
                               max_iterations=int(opts.max_iterations), precision=precision)
         infos.append(info)
@@ -1926,5 +1936,6 @@ def frequency_sweep_eigenmode(mesh: Mesh, params: MaxwellParams, bc: BC, ports: 
                 dict(X=X, freq_index=[sys_f[si] for si in idx], excitation=[sys_a[si] for si in idx],
                      k0sq=[sys_k2[si] for si in idx], coef=[sys_coef[si] for si in idx]))
     if _state is not None:
-        _state.update(dict(infos=infos, sweep=sweep, ops=ops))
+        v, a, tune_ms = sweep.variants()
+        _state.update(dict(infos=infos, sweep=sweep, ops=ops, spmv_chosen=v, aux_chosen=a, tune_ms=tune_ms))
     return out

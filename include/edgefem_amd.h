@@ -247,6 +247,16 @@ int efm_sweep_solve(efm_sweep *sw, int nsys, const double *h_k0sq, const double 
                     const double *B, double *X, const efm_solve_opts *h_opts,
                     const efm_refine_opts *h_refine, int32_t *h_iters, double *h_resid,
                     void *stream);
+/* Kernel variants of the sweep, for the SpMV and for the nodal part of the
+ * preconditioner: 0 = one block slice per group of 64 systems; 1 = pipelined (bounds of
+ * the next rows/nodes preloaded per wave, fixed-count gathers, groups walked in turn so
+ * that each XCD's share of the gathered vectors stays in its L2); -1 (default) = the
+ * faster of the two, timed on the first solve.  get: the variants in use (-1 before they
+ * are chosen) and, when h_tune_ms is given, the tuning times in ms per launch
+ * [SpMV 0, SpMV 1, nodal 0, nodal 1].  Autotuner of this project; no reference
+ * counterpart. */
+int efm_sweep_set_variants(efm_sweep *sw, int spmv, int aux);
+int efm_sweep_get_variants(const efm_sweep *sw, int *h_spmv, int *h_aux, double *h_tune_ms);
 
 /* ------------------------------------------------------------------------
  * Periodic (Floquet) elimination on the sparse matrix: replaces the dense elimination of

@@ -381,20 +381,23 @@ def test_full_size_c2_topology_properties(torch_dev):
     assert np.all(np.diff(first) > 0)
 
 
-@pytest.mark.parametrize("precision,max_batch", [("fp64", 256), ("fp64", 100), ("fp64-refined", 256)])
-def test_batched_sweep_matches_lu_golden(torch_dev, precision, max_batch):
+@pytest.mark.parametrize("precision,max_batch,variant", [("fp64", 256, -1), ("fp64", 256, 0), ("fp64", 256, 1),
+                                                        ("fp64", 100, 1), ("fp64-refined", 256, -1)])
+def test_batched_sweep_matches_lu_golden(torch_dev, precision, max_batch, variant):
     """128 frequencies x 2 excitations = 256 systems on the 16x8x32 guide, batched, against
-    the FP64 SuperLU golden of every point: plain FP64 COCG, also with batches that are
-    not a multiple of the 64 systems of a wavefront (100, 100, 56), and the iterative
-    refinement loop (true FP64 residual b - A x per outer step) around FP64 inner solves.
-    Complex-FP32 inner vectors are not tested here: on these indefinite systems their
-    COCG iterations stall near a 1e-4..1e-3 relative residual for some frequencies."""
+    the FP64 SuperLU golden of every point: plain FP64 COCG with each kernel variant of the
+    SpMV and the nodal preconditioner part (0: one block slice per system group, 1:
+    pipelined, groups in turn, -1: autotuned), also with batches that are not a multiple
+    of the 64 systems of a wavefront (100, 100, 56), and the iterative refinement loop
+    (true FP64 residual b - A x per outer step) around FP64 inner solves.  Complex-FP32
+    inner vectors are not tested here: on these indefinite systems their COCG iterations
+    stall near a 1e-4..1e-3 relative residual for some frequencies."""
     from edgefem_amd import pyedgefem as em
-    _p(f"[gpu] batched sweep 256 systems precision={precision} max_batch={max_batch}")
+    _p(f"[gpu] batched sweep 256 systems precision={precision} max_batch={max_batch} variant={variant}")
     pm, bc, ports = _c1_em_setup()
     gold = np.load(golden("c3_sweep.npz"))
     params = em.MaxwellParams()
-    st = {}
+    st = {"spmv_variant": variant, "aux_variant": variant}
     opts = em.SolveOptions()
     opts.tolerance = tol = 1e-10
     S = em.frequency_sweep_eigenmode(pm, params, bc, ports, gold["freqs"], precision=precision,
@@ -402,10 +405,13 @@ def test_batched_sweep_matches_lu_golden(torch_dev, precision, max_batch):
     err = np.abs(S - gold["S"]).max()
     its = np.concatenate([i["iters"] for i in st["infos"]])
     res = np.concatenate([i["resid"] for i in st["infos"]])
-    _p(f"    max |dS| {err:.2e}; iterations min {its.min()} max {its.max()}; max resid {res.max():.1e}")
+    _p(f"    max |dS| {err:.2e}; iterations min {its.min()} max {its.max()}; max resid {res.max():.1e}; "
+       f"spmv {st['spmv_chosen']} aux {st['aux_chosen']} tune ms {st['tune_ms']}")
     assert np.all(np.isfinite(S))
     assert res.max() <= tol
     assert err <= 1e-6
+    assert st["spmv_chosen"] in ((0, 1) if variant < 0 else (variant,))
+    assert st["aux_chosen"] in ((0, 1) if variant < 0 else (variant,))
 
 
 def _design_problem(freqs, seed=5):
