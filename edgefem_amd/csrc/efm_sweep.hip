@@ -20,6 +20,7 @@
 // result of a system does not depend on which other systems share the launch.
 #include "efm_internal.h"
 
+#include <type_traits>
 #include <vector>
 
 using namespace efm;
@@ -315,9 +316,10 @@ __device__ inline double2 edge_diag_inv(int64_t e, int s, int P, double k2, cons
 // else: p = u + beta p; s = w + beta s; x += alpha p; r -= alpha s; u = P r; partials,
 // with x and r read and stored only when APPLY (the pending step of the previous
 // iteration, pend, is folded in first: x += pend p_old, r -= pend s_old).
+// Persistent: one block slice (gridDim.y == 1) walks the groups of 64 systems in turn.
 template <typename VT, bool INIT, bool APPLY>
 __global__ void __launch_bounds__(kBlock)
-    k_sw_update(int64_t n, int B, int P, const int2 *__restrict__ en_eff,
+    k_sw_update(int64_t n, int B, int groups, int P, const int2 *__restrict__ en_eff,
                 const double2 *__restrict__ dK, const double2 *__restrict__ dM,
                 const double *__restrict__ dMs, const int8_t *__restrict__ rport,
                 const double *__restrict__ k0sq, const double2 *__restrict__ coef,
@@ -326,8 +328,9 @@ __global__ void __launch_bounds__(kBlock)
                 VT *__restrict__ x, VT *__restrict__ r, double2 *__restrict__ partial,
                 const SwState *__restrict__ st) {
   __shared__ double2 lds[kWaves * 2 * 64];
-  const int s = blockIdx.y * kSys + (threadIdx.x & 63);
   const int wv = threadIdx.x >> 6;
+  for (int g = blockIdx.y; g < groups; g += gridDim.y) {
+  const int s = g * kSys + (threadIdx.x & 63);
   double2 v[2] = {make_double2(0, 0), make_double2(0, 0)};
   const bool act = s < B && st[s].active != 0;
   if (act) {
@@ -360,7 +363,9 @@ __global__ void __launch_bounds__(kBlock)
       v[1].x += cabs2(rv);
     }
   }
-  write_sys_partials<2>(v, lds, partial, blockIdx.y * gridDim.x + blockIdx.x);
+  write_sys_partials<2>(v, lds, partial, g * gridDim.x + blockIdx.x);
+  __syncthreads();  // lds reuse by the next group
+  }
 }
 
 // One row of A_s applied to a vector of the layout [row][system], called by a whole
@@ -542,8 +547,10 @@ struct RowStage2 {
   double2 c;
 };
 
-// One row with NU >= len gathers, all issued before the next row is staged.
-template <int NU, typename VT, typename StageFn>
+// One row with NU >= len gathers, all issued before the next row is staged.  KC / MC:
+// K / M have nonzero imaginary parts (otherwise only their real parts are broadcast and
+// multiplied).
+template <int NU, bool KC, bool MC, typename VT, typename StageFn>
 __device__ inline double2 sw_row_fixed(const RowStage2 &cur, RowStage2 &nxt, StageFn &&stage, int jn,
                                        int64_t rowc, int s, int B, double k2, const VT *__restrict__ u,
                                        double2 &ud) {
@@ -558,8 +565,19 @@ __device__ inline double2 sw_row_fixed(const RowStage2 &cur, RowStage2 &nxt, Sta
   double2 aK = make_double2(0, 0), aM = make_double2(0, 0), aS = make_double2(0, 0);
 #pragma unroll
   for (int t = 0; t < NU; ++t) {
-    aK = cfma(make_double2(bcast(cur.kx, t), bcast(cur.ky, t)), xv[t], aK);
-    aM = cfma(make_double2(bcast(cur.mx, t), bcast(cur.my, t)), xv[t], aM);
+    const double kr = bcast(cur.kx, t), mr = bcast(cur.mx, t);
+    if (KC) {
+      aK = cfma(make_double2(kr, bcast(cur.ky, t)), xv[t], aK);
+    } else {
+      aK.x += kr * xv[t].x;
+      aK.y += kr * xv[t].y;
+    }
+    if (MC) {
+      aM = cfma(make_double2(mr, bcast(cur.my, t)), xv[t], aM);
+    } else {
+      aM.x += mr * xv[t].x;
+      aM.y += mr * xv[t].y;
+    }
   }
   if (cur.port >= 0) {
 #pragma unroll
@@ -578,7 +596,7 @@ __device__ inline double2 sw_row_fixed(const RowStage2 &cur, RowStage2 &nxt, Sta
 // preloads the bounds of its next 64 rows across its lanes, stages the next row's
 // columns and values while the current row's gathers are in flight, and issues a fixed
 // number of gathers per row (13 or 20; longer rows take the general path).
-template <typename VT>
+template <typename VT, bool KC, bool MC>
 __global__ void __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(3, 8)))
     k_sw_spmv_pre(int64_t n, int B, int P, int groups, const int32_t *__restrict__ row_ptr,
                   const int32_t *__restrict__ col, const double2 *__restrict__ K,
@@ -595,6 +613,8 @@ __global__ void __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(3, 
   // rows of this wave: j = 0 .. nr-1, row_j = (begin + j stride) kWaves + wv (may be >= n
   // in the last chunk: such a row has length 0, reads row n-1 and writes nothing)
   const int64_t nr = (cw.begin < cw.end) ? (cw.end - cw.begin + cw.stride - 1) / cw.stride : 0;
+  const double *Kd = reinterpret_cast<const double *>(K);
+  const double *Md = reinterpret_cast<const double *>(M);
   for (int g = blockIdx.y; g < groups; g += gridDim.y) {
     const int s = g * kSys + lane;
     double2 v[1] = {make_double2(0, 0)};
@@ -623,13 +643,27 @@ __global__ void __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(3, 
           const bool in = lane < q.len;
           const int32_t k = in ? beg + lane : 0;
           const int cl = col[k];
-          const double2 kv = K[k], mv = M[k];
+          double kx, ky = 0.0, mx, my = 0.0;
+          if (KC) {
+            const double2 kv = K[k];
+            kx = kv.x;
+            ky = kv.y;
+          } else {
+            kx = Kd[2 * (int64_t)k];
+          }
+          if (MC) {
+            const double2 mv = M[k];
+            mx = mv.x;
+            my = mv.y;
+          } else {
+            mx = Md[2 * (int64_t)k];
+          }
           const double mw = Ms[k];
           q.cj = in ? cl : (int)row_of(j);
-          q.kx = in ? kv.x : 0.0;
-          q.ky = in ? kv.y : 0.0;
-          q.mx = in ? mv.x : 0.0;
-          q.my = in ? mv.y : 0.0;
+          q.kx = in ? kx : 0.0;
+          q.ky = in ? ky : 0.0;
+          q.mx = in ? mx : 0.0;
+          q.my = in ? my : 0.0;
           q.ms = in ? mw : 0.0;
         };
         RowStage2 cur, nxt;
@@ -640,10 +674,10 @@ __global__ void __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(3, 
           const int jn = min(j + 1, cnt - 1);
           double2 acc, ud;
           if (cur.len <= 13) {
-            acc = sw_row_fixed<13>(cur, nxt, stage, jn, rowc, s, B, k2, u, ud);
+            acc = sw_row_fixed<13, KC, MC>(cur, nxt, stage, jn, rowc, s, B, k2, u, ud);
             cur = nxt;
           } else if (cur.len <= 20) {
-            acc = sw_row_fixed<20>(cur, nxt, stage, jn, rowc, s, B, k2, u, ud);
+            acc = sw_row_fixed<20, KC, MC>(cur, nxt, stage, jn, rowc, s, B, k2, u, ud);
             cur = nxt;
           } else {
             acc = sw_row_apply(rowc, s, B, P, k2, row_ptr, col, K, M, Ms, rport, coef, u);
@@ -660,6 +694,20 @@ __global__ void __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(3, 
     write_sys_partials<1>(v, lds, partial, g * gridDim.x + blockIdx.x);
     __syncthreads();  // lds reuse by the next group
   }
+}
+
+// Bit 1: some K value has a nonzero imaginary part; bit 2: some M value has.
+__global__ void k_sw_imag_flags(int64_t nnz, const double2 *__restrict__ K, const double2 *__restrict__ M,
+                                int *__restrict__ flags) {
+  int f = 0;
+  for (int64_t k = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; k < nnz; k += (int64_t)gridDim.x * blockDim.x) {
+    if (K[k].y != 0.0)
+      f |= 1;
+    if (M[k].y != 0.0)
+      f |= 2;
+  }
+  if (f)
+    atomicOr(flags, f);
 }
 
 // Iterative refinement, outer step: r = b - A_s x in FP64 (x == nullptr: r = b), stored
@@ -988,12 +1036,30 @@ struct efm_sweep {
   // walk (groups in turn); -1 = chosen by timing both on the first solve
   int aux_variant = -1;
   int gx_aux = 8;
+  int cplx = 3;  // bit 1: K complex, bit 2: M complex (selects the SpMV instantiation)
+  // update grids (one block slice, as many blocks as are resident): [VT c128, c64][INIT, skip, apply]
+  int gx_upd[2][3] = {{8, 8, 8}, {8, 8, 8}};
   float tune_ms[4] = {0.f, 0.f, 0.f, 0.f};  // SpMV 0, 1; nodal 0, 1
   efm::DevBuf K, M, Ms, rport, dK, dM, dMs, dnK, dnM, dnMs, gsign, en_eff;
   efm::DevBuf x, r, p, sv, u, w, y, part_u, part_s, state, flags, k0sq, coef;
   efm::DevBuf x64;  // refinement: FP64 solution (allocated on first use)
   efm::DevBuf dninv;  // [N][Bmax] inverse nodal diagonals of the systems of one solve
 };
+
+namespace {
+// Grid of a persistent kernel: as many blocks of kBlock threads as are resident on the
+// device at the kernel's occupancy (a multiple of 8 for the XCD-aware walks), capped by
+// the work (items handled one per wave).
+int resident_grid(const void *kernel, int ncu, int64_t items) {
+  int nb = 0;
+  if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&nb, kernel, kBlock, 0) != hipSuccess || nb < 1)
+    nb = 1;
+  const int64_t need = (items + kWaves - 1) / kWaves;
+  const int64_t cap = std::max<int64_t>(8, ((int64_t)ncu * nb / 8) * 8);
+  const int64_t g = std::min<int64_t>(cap, std::max<int64_t>(need, 8));
+  return (int)(((g + 7) / 8) * 8);
+}
+}  // namespace
 
 extern "C" int efm_sweep_create(const efm_topology *topo, const double *K_val, const double *M_val,
                                 int n_ports, const double *const *h_Ms_vals,
@@ -1042,15 +1108,25 @@ extern "C" int efm_sweep_create(const efm_topology *topo, const double *K_val, c
     s->gy_spmv = groups;
     const int64_t g = std::min<int64_t>(want, std::max<int64_t>(need, 8));
     s->gx_spmv = (int)(((g + 7) / 8) * 8);
-    // pipelined variant: one block slice (gridDim.y = 1), as many blocks as are resident
-    // at 3 waves per SIMD (3 blocks of 4 waves per CU)
-    const int64_t gp = std::min<int64_t>((int64_t)ncu * 3, std::max<int64_t>(need, 8));
-    s->gx_pre = (int)(((gp + 7) / 8) * 8);
-    // pipelined nodal kernel: 3 waves per SIMD (the fused form's occupancy)
-    const int64_t needn = (N + kWaves - 1) / kWaves;
-    const int64_t ga = std::min<int64_t>((int64_t)ncu * 3, std::max<int64_t>(needn, 8));
-    s->gx_aux = (int)(((ga + 7) / 8) * 8);
+    // pipelined kernels and the update: one block slice (gridDim.y = 1), as many blocks
+    // as are resident
+    s->gx_pre = resident_grid(reinterpret_cast<const void *>(&k_sw_spmv_pre<double2, true, true>), ncu, n);
+    s->gx_aux = resident_grid(reinterpret_cast<const void *>(&k_sw_aux_pre<double2, true>), ncu, N);
+    const void *upd[2][3] = {
+        {reinterpret_cast<const void *>(&k_sw_update<double2, true, false>),
+         reinterpret_cast<const void *>(&k_sw_update<double2, false, false>),
+         reinterpret_cast<const void *>(&k_sw_update<double2, false, true>)},
+        {reinterpret_cast<const void *>(&k_sw_update<float2, true, false>),
+         reinterpret_cast<const void *>(&k_sw_update<float2, false, false>),
+         reinterpret_cast<const void *>(&k_sw_update<float2, false, true>)}};
+    for (int a = 0; a < 2; ++a)
+      for (int b = 0; b < 3; ++b)
+        s->gx_upd[a][b] = resident_grid(upd[a][b], ncu, n);
   }
+  int gx_upd_max = 8;
+  for (int a = 0; a < 2; ++a)
+    for (int b = 0; b < 3; ++b)
+      gx_upd_max = std::max(gx_upd_max, s->gx_upd[a][b]);
   if ((rc = s->K.alloc(nnz * 16)) || (rc = s->M.alloc(nnz * 16)) || (rc = s->Ms.alloc(nnz * 8)) ||
       (rc = s->rport.alloc(n + 16)) || (rc = s->dK.alloc(n * 16)) || (rc = s->dM.alloc(n * 16)) ||
       (rc = s->dMs.alloc(n * 8)) || (rc = s->dnK.alloc(N * 16)) || (rc = s->dnM.alloc(N * 16)) ||
@@ -1059,7 +1135,7 @@ extern "C" int efm_sweep_create(const efm_topology *topo, const double *K_val, c
       (rc = s->p.alloc(vb)) || (rc = s->sv.alloc(vb)) || (rc = s->u.alloc(vb)) || (rc = s->w.alloc(vb)) ||
       (rc = s->y.alloc((size_t)N * s->Bmax * sizeof(double2))) ||
       (rc = s->dninv.alloc((size_t)N * s->Bmax * sizeof(double2))) ||
-      (rc = s->part_u.alloc((size_t)groups * s->gx_vec * 2 * 64 * sizeof(double2))) ||
+      (rc = s->part_u.alloc((size_t)groups * gx_upd_max * 2 * 64 * sizeof(double2))) ||
       (rc = s->part_s.alloc((size_t)groups * std::max(s->gx_spmv, s->gx_pre) * 64 * sizeof(double2))) ||
       (rc = s->state.alloc(s->Bmax * sizeof(SwState))) || (rc = s->flags.alloc(s->Bmax * sizeof(int))) ||
       (rc = s->k0sq.alloc(s->Bmax * sizeof(double))) ||
@@ -1095,6 +1171,14 @@ extern "C" int efm_sweep_create(const efm_topology *topo, const double *K_val, c
   hipLaunchKernelGGL(k_sw_edge_prep, dim3(grid_for(n, 256)), dim3(256), 0, st, n, en, dirichlet,
                      s->en_eff.as<int2>());
   EFM_LAUNCH_CHECK();
+  DevBuf cflag;
+  if ((rc = cflag.alloc(sizeof(int))))
+    return rc;
+  EFM_HIP_CHECK(hipMemsetAsync(cflag.p, 0, sizeof(int), st));
+  hipLaunchKernelGGL(k_sw_imag_flags, dim3(std::max<int64_t>(1, std::min<int64_t>(grid_for(nnz, 256), 4096))),
+                     dim3(256), 0, st, nnz, s->K.as<double2>(), s->M.as<double2>(), cflag.as<int>());
+  EFM_LAUNCH_CHECK();
+  EFM_HIP_CHECK(hipMemcpyAsync(&s->cplx, cflag.p, sizeof(int), hipMemcpyDeviceToHost, st));
   int h_err = 0;
   EFM_HIP_CHECK(hipMemcpyAsync(&h_err, err.p, sizeof(int), hipMemcpyDeviceToHost, st));
   EFM_HIP_CHECK(hipStreamSynchronize(st));
@@ -1229,30 +1313,50 @@ struct SwRun {
       }
       return aux_v(fused, s->aux_variant == 1 ? 1 : 0);
     };
+    constexpr int vti = std::is_same<VT, float2>::value ? 1 : 0;
     auto update = [&](bool first, bool apply) -> int {
       const double2 *dK = s->dK.as<double2>(), *dM = s->dM.as<double2>();
       const double *dMs = s->dMs.as<double>(), *k2 = s->k0sq.as<double>();
       const int8_t *rp = s->rport.as<int8_t>();
       const double2 *cf = s->coef.as<double2>();
-      const dim3 grid(s->gx_vec, groups);
+      const int vi = first ? 0 : (apply ? 2 : 1);
+      const dim3 grid(s->gx_upd[vti][vi], 1);
       if (first)
-        hipLaunchKernelGGL((k_sw_update<VT, true, false>), grid, dim3(kBlock), 0, st, n, Bp, P, en, dK, dM, dMs, rp,
-                           k2, cf, y, w, u, p, sv, x, r, pu, S);
+        hipLaunchKernelGGL((k_sw_update<VT, true, false>), grid, dim3(kBlock), 0, st, n, Bp, groups, P, en, dK, dM,
+                           dMs, rp, k2, cf, y, w, u, p, sv, x, r, pu, S);
       else if (apply)
-        hipLaunchKernelGGL((k_sw_update<VT, false, true>), grid, dim3(kBlock), 0, st, n, Bp, P, en, dK, dM, dMs, rp,
-                           k2, cf, y, w, u, p, sv, x, r, pu, S);
+        hipLaunchKernelGGL((k_sw_update<VT, false, true>), grid, dim3(kBlock), 0, st, n, Bp, groups, P, en, dK, dM,
+                           dMs, rp, k2, cf, y, w, u, p, sv, x, r, pu, S);
       else
-        hipLaunchKernelGGL((k_sw_update<VT, false, false>), grid, dim3(kBlock), 0, st, n, Bp, P, en, dK, dM, dMs,
-                           rp, k2, cf, y, w, u, p, sv, x, r, pu, S);
+        hipLaunchKernelGGL((k_sw_update<VT, false, false>), grid, dim3(kBlock), 0, st, n, Bp, groups, P, en, dK, dM,
+                           dMs, rp, k2, cf, y, w, u, p, sv, x, r, pu, S);
       EFM_LAUNCH_CHECK();
       return EFM_OK;
     };
     auto spmv_v = [&](int variant) -> int {
-      if (variant == 1)
-        hipLaunchKernelGGL((k_sw_spmv_pre<VT>), dim3(s->gx_pre, 1), dim3(kBlock), 0, st, n, Bp, P, groups,
-                           s->row_ptr, s->col, s->K.as<double2>(), s->M.as<double2>(), s->Ms.as<double>(),
-                           s->rport.as<int8_t>(), s->k0sq.as<double>(), s->coef.as<double2>(), u, w, ps, S);
-      else
+      if (variant == 1) {
+        const dim3 grid(s->gx_pre, 1);
+        const double2 *Kv = s->K.as<double2>(), *Mv = s->M.as<double2>(), *cf = s->coef.as<double2>();
+        const double *Msv = s->Ms.as<double>(), *k2 = s->k0sq.as<double>();
+        const int8_t *rp = s->rport.as<int8_t>();
+        switch (s->cplx & 3) {
+        case 0:
+          hipLaunchKernelGGL((k_sw_spmv_pre<VT, false, false>), grid, dim3(kBlock), 0, st, n, Bp, P, groups,
+                             s->row_ptr, s->col, Kv, Mv, Msv, rp, k2, cf, u, w, ps, S);
+          break;
+        case 1:
+          hipLaunchKernelGGL((k_sw_spmv_pre<VT, true, false>), grid, dim3(kBlock), 0, st, n, Bp, P, groups,
+                             s->row_ptr, s->col, Kv, Mv, Msv, rp, k2, cf, u, w, ps, S);
+          break;
+        case 2:
+          hipLaunchKernelGGL((k_sw_spmv_pre<VT, false, true>), grid, dim3(kBlock), 0, st, n, Bp, P, groups,
+                             s->row_ptr, s->col, Kv, Mv, Msv, rp, k2, cf, u, w, ps, S);
+          break;
+        default:
+          hipLaunchKernelGGL((k_sw_spmv_pre<VT, true, true>), grid, dim3(kBlock), 0, st, n, Bp, P, groups,
+                             s->row_ptr, s->col, Kv, Mv, Msv, rp, k2, cf, u, w, ps, S);
+        }
+      } else
         hipLaunchKernelGGL((k_sw_spmv<VT>), dim3(s->gx_spmv, s->gy_spmv), dim3(kBlock), 0, st, n, Bp, P, groups,
                            s->row_ptr, s->col, s->K.as<double2>(), s->M.as<double2>(), s->Ms.as<double>(),
                            s->rport.as<int8_t>(), s->k0sq.as<double>(), s->coef.as<double2>(), u, w, ps, S);
@@ -1269,13 +1373,14 @@ struct SwRun {
       return rc2 ? rc2 : spmv();
     };
     auto finalize = [&](bool first, bool skip) -> int {
-      const int gxs = s->spmv_variant == 1 ? s->gx_pre : s->gx_spmv;  // the SpMV grid that wrote ps
+      const int gxs = s->spmv_variant == 1 ? s->gx_pre : s->gx_spmv;           // the SpMV grid that wrote ps
+      const int gxu = s->gx_upd[vti][first ? 0 : (skip ? 1 : 2)];             // the update grid that wrote pu
       if (first)
-        hipLaunchKernelGGL((k_sw_finalize<true>), dim3(Bp), dim3(kBlock), 0, st, Bp, s->gx_vec, gxs, pu, ps, S,
-                           flags, inner_tol2, max_it, max_inner, 0);
+        hipLaunchKernelGGL((k_sw_finalize<true>), dim3(Bp), dim3(kBlock), 0, st, Bp, gxu, gxs, pu, ps, S, flags,
+                           inner_tol2, max_it, max_inner, 0);
       else
-        hipLaunchKernelGGL((k_sw_finalize<false>), dim3(Bp), dim3(kBlock), 0, st, Bp, s->gx_vec, gxs, pu, ps, S,
-                           flags, inner_tol2, max_it, max_inner, skip ? 1 : 0);
+        hipLaunchKernelGGL((k_sw_finalize<false>), dim3(Bp), dim3(kBlock), 0, st, Bp, gxu, gxs, pu, ps, S, flags,
+                           inner_tol2, max_it, max_inner, skip ? 1 : 0);
       EFM_LAUNCH_CHECK();
       return EFM_OK;
     };

@@ -414,6 +414,44 @@ def test_batched_sweep_matches_lu_golden(torch_dev, precision, max_batch, varian
     assert st["aux_chosen"] in ((0, 1) if variant < 0 else (variant,))
 
 
+def test_batched_sweep_lossy_complex_mass_vs_oracle(torch_dev):
+    """Batched sweep with a lossy permittivity (complex M, so the SpMV takes its complex-M
+    instantiation) on a 6x3x12 WR-90 section: both kernel variants against the oracle
+    (SuperLU) at three frequencies."""
+    from edgefem_amd import meshgen as mg
+    from edgefem_amd import pyedgefem as em
+    from oracle import edgefem_oracle as O
+    _p("[gpu] batched sweep, lossy fill (complex M), variants 0 and 1")
+    L = 30e-3
+    g = mg.box_mesh(mg.WR90_A, mg.WR90_B, L, 6, 3, 12, seed=7)
+    pm = em.mesh_from_arrays(g.node_ids, g.xyz, g.tets, g.tet_phys, g.tris, g.tri_phys)
+    bc = em.build_edge_pec(pm, 1)
+    mode = em.solve_te10_mode(em.RectWaveguidePort(mg.WR90_A, mg.WR90_B), 10e9)
+    kc2 = (math.pi / mg.WR90_A) ** 2
+    ports = [em.build_wave_port_2d(pm, t, mode, bc.dirichlet_edges, kc2) for t in (2, 3)]
+    zc = g.centroids()[:, 2]
+    eps = np.ones(g.tets.shape[0], np.complex128)
+    slab = (zc > 0.3 * L) & (zc < 0.7 * L)
+    eps[slab] = (1.0 + 3.0 * mg.uniform01(9, np.nonzero(slab)[0].astype(np.int64))) * (1 - 0.02j)
+    params = em.MaxwellParams()
+    params.eps_r_tet = eps
+    freqs = np.array([9e9, 10e9, 11e9])
+    om = O.mesh_from_arrays(g.node_ids, g.xyz, g.tet_ids, g.tet_phys, g.tri_ids, g.tri_phys)
+    opec = O.build_edge_pec(om, 1)
+    kc, beta, Z0 = O.solve_te10_mode(mg.WR90_A, 10e9)
+    oports = [O.build_wave_port_2d(om, t, kc, Z0, beta, opec, kc2) for t in (2, 3)]
+    Sref = np.array([O.sparams_eigenmode(om, O.Params(omega=2 * math.pi * f, eps_tet=eps), opec, oports)
+                     for f in freqs])
+    for variant in (0, 1):
+        st = {"spmv_variant": variant, "aux_variant": variant}
+        S = em.frequency_sweep_eigenmode(pm, params, bc, ports, freqs, _state=st)
+        err = np.abs(S - Sref).max()
+        _p(f"    variant {variant}: max |dS| {err:.2e}; |S21| {np.abs(S[:, 1, 0]).round(6)}")
+        assert st["spmv_chosen"] == variant and st["aux_chosen"] == variant
+        assert err <= 1e-6
+        assert np.all(np.abs(S[:, 1, 0]) < 1.0)   # lossy
+
+
 def _design_problem(freqs, seed=5):
     """Small WR-90 section (6x3x12, 30 mm) with a per-tet permittivity design region in
     its middle half, on the GPU and in the oracle."""
