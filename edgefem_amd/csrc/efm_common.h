@@ -73,6 +73,26 @@ __host__ __device__ inline double cabs2(double2 a) { return a.x * a.x + a.y * a.
 __device__ inline double2 to_d2(double2 v) { return v; }
 __device__ inline double2 to_d2(float2 v) { return make_double2((double)v.x, (double)v.y); }
 
+// Streaming (non-temporal) accesses for data used once per kernel, so that they do not
+// evict reused data (gathered vectors) from the caches.
+typedef double efm_v2d __attribute__((ext_vector_type(2)));
+typedef float efm_v2f __attribute__((ext_vector_type(2)));
+__device__ inline double2 ld_stream(const double2 *p) {
+  const efm_v2d v = __builtin_nontemporal_load(reinterpret_cast<const efm_v2d *>(p));
+  return make_double2(v.x, v.y);
+}
+__device__ inline double2 ld_stream(const float2 *p) {
+  const efm_v2f v = __builtin_nontemporal_load(reinterpret_cast<const efm_v2f *>(p));
+  return make_double2((double)v.x, (double)v.y);
+}
+__device__ inline int32_t ld_stream(const int32_t *p) { return __builtin_nontemporal_load(p); }
+__device__ inline void st_stream(double2 *p, double2 v) {
+  efm_v2d w;
+  w.x = v.x;
+  w.y = v.y;
+  __builtin_nontemporal_store(w, reinterpret_cast<efm_v2d *>(p));
+}
+
 // --------------------------------------------------------------------------
 // Fixed tables of the reference
 // --------------------------------------------------------------------------

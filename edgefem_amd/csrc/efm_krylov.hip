@@ -453,9 +453,12 @@ __global__ void __launch_bounds__(kBlock)
   write_partials<R>(pq, lds, partial);
 }
 
-// The tiled SpMV with the next tile's descriptor, values, columns and row bounds loaded
-// while the current tile's column gathers are in flight (one dependent round trip per
-// tile instead of four).  desc[t] = (first row, end row, first nnz, end nnz).
+// The tiled SpMV with the next tile's values, columns and row bounds loaded while the
+// current tile's column gathers are in flight.  Every load of a tile is issued with no
+// branch (entries past the tile's end read its first entry and contribute zero; rows past
+// its end read its last row and write nothing), so the wait before the products covers
+// the gathers only and the next tile's loads stay in flight across it.  desc[t] = (first
+// row, end row, first nnz, end nnz) is read with uniform (scalar) loads.
 template <typename VT, int R, int TNNZ, int RL>
 __global__ void __launch_bounds__(kBlock)
     k_cocg_spmv_pipe(int ntiles, int xcd_span, const int4 *__restrict__ desc,
@@ -483,80 +486,79 @@ __global__ void __launch_bounds__(kBlock)
   const int t_begin = xcd * xcd_span;
   const int t_end = min(ntiles, t_begin + xcd_span);
   const int g = threadIdx.x / RL, l = threadIdx.x % RL;
-  // staged state of the next tile
-  int4 dn = make_int4(0, 0, 0, 0);
-  double2 an[J];
-  int32_t cn[J];
-  int32_t k0n = 0, k1n = 0;
-  auto stage = [&](int tile) {
-    dn = desc[tile];
-#pragma unroll
-    for (int j = 0; j < J; ++j) {
-      const int32_t k = dn.z + threadIdx.x + j * kBlock;
-      if (k < dn.w) {
-        an[j] = to_d2(val[k]);
-        cn[j] = col[k];
-      } else {
-        an[j] = make_double2(0, 0);
-        cn[j] = -1;
-      }
-    }
-    const int32_t row = dn.x + g;
-    if (row < dn.y) {
-      k0n = row_ptr[row] - dn.z;
-      k1n = row_ptr[row + 1] - dn.z;
-    } else {
-      k0n = k1n = 0;
-    }
-  };
-  int tile = t_begin + lb;
-  if (tile < t_end)
-    stage(tile);
-  for (; tile < t_end; tile += per_xcd) {
-    const int4 d = dn;
+  // staged state of a tile: entries, columns, this thread's row bounds (relative)
+  struct Stage {
+    int4 d;
     double2 a[J];
     int32_t c[J];
+    int32_t k0, k1;
+  };
+  auto stage = [&](Stage &x, int tile) {
+    x.d = desc[tile];
+    const int32_t kz = x.d.z < x.d.w ? x.d.z : 0;  // a valid entry either way
 #pragma unroll
     for (int j = 0; j < J; ++j) {
-      a[j] = an[j];
-      c[j] = cn[j];
+      const int32_t k = x.d.z + threadIdx.x + j * kBlock;
+      const bool in = k < x.d.w;
+      const int32_t kk = in ? k : kz;
+      const double2 av = ld_stream(val + kk);
+      x.c[j] = ld_stream(col + kk);
+      x.a[j] = in ? av : make_double2(0, 0);
     }
-    const int32_t k0 = k0n, k1 = k1n;
-    // gathers of the current tile
+    const int32_t row = x.d.x + g;
+    const int32_t rc = row < x.d.y ? row : x.d.y - 1;
+    const int32_t rb = row_ptr[rc], re = row_ptr[rc + 1];
+    x.k0 = rb - x.d.z;
+    x.k1 = row < x.d.y ? re - x.d.z : x.k0;
+  };
+  int tile = t_begin + lb;
+  if (tile >= t_end) {
+    write_partials<R>(pq, lds, partial);
+    return;
+  }
+  Stage cur, nxt;
+  stage(cur, tile);
+  for (; tile < t_end; tile += per_xcd) {
+    // gathers of this tile and the records of its rows (lanes of a row read the same one)
     double2 wz[J][R], wp[J][R];
 #pragma unroll
-    for (int j = 0; j < J; ++j)
-      if (c[j] >= 0) {
-        const double2 *w = Wa + (int64_t)c[j] * 2 * R;
+    for (int j = 0; j < J; ++j) {
+      const double2 *w = Wa + (int64_t)cur.c[j] * 2 * R;
 #pragma unroll
-        for (int s = 0; s < R; ++s) {
-          wz[j][s] = w[s];
-          wp[j][s] = w[R + s];
-        }
+      for (int s = 0; s < R; ++s) {
+        wz[j][s] = w[s];
+        wp[j][s] = w[R + s];
       }
-    // next tile's loads, issued behind the gathers
-    if (tile + per_xcd < t_end)
-      stage(tile + per_xcd);
+    }
+    const int32_t row = cur.d.x + g;
+    const int32_t rc = row < cur.d.y ? row : cur.d.y - 1;
+    double2 rz[R], rp[R];
+    {
+      const double2 *w = Wa + (int64_t)rc * 2 * R;
+#pragma unroll
+      for (int s = 0; s < R; ++s) {
+        rz[s] = w[s];
+        rp[s] = w[R + s];
+      }
+    }
+    // the next tile's loads, behind the gathers
+    stage(nxt, min(tile + per_xcd, t_end - 1));
 #pragma unroll
     for (int j = 0; j < J; ++j) {
       const int32_t lk = threadIdx.x + j * kBlock;
-      if (c[j] >= 0) {
 #pragma unroll
-        for (int s = 0; s < R; ++s)
-          prod[lk * R + s] = bil(a[j], cfma(beta[s], wp[j][s], wz[j][s]));
-      }
+      for (int s = 0; s < R; ++s)
+        prod[lk * R + s] = bil(cur.a[j], cfma(beta[s], wp[j][s], wz[j][s]));
     }
     __syncthreads();
-    const int32_t row = d.x + g;
     double2 acc[R];
 #pragma unroll
     for (int s = 0; s < R; ++s)
       acc[s] = make_double2(0, 0);
-    if (row < d.y)
-      for (int32_t k = k0 + l; k < k1; k += RL)
+    for (int32_t k = cur.k0 + l; k < cur.k1; k += RL)
 #pragma unroll
-        for (int s = 0; s < R; ++s)
-          acc[s] = cadd(acc[s], prod[k * R + s]);
+      for (int s = 0; s < R; ++s)
+        acc[s] = cadd(acc[s], prod[k * R + s]);
 #pragma unroll
     for (int s = 0; s < R; ++s)
 #pragma unroll
@@ -564,9 +566,124 @@ __global__ void __launch_bounds__(kBlock)
         acc[s].x += __shfl_xor(acc[s].x, off, RL);
         acc[s].y += __shfl_xor(acc[s].y, off, RL);
       }
-    if (row < d.y && l == 0) {
+    if (row < cur.d.y && l == 0) {
+      double2 *wb = Wb + (int64_t)row * 2 * R;
+      // the whole record is written (z = 0; the update overwrites it) so that no partial
+      // 64-byte line reaches HBM
+      double2 rec[2 * R];
+#pragma unroll
+      for (int s = 0; s < R; ++s) {
+        rec[s] = make_double2(0, 0);
+        rec[R + s] = cfma(beta[s], rp[s], rz[s]);
+      }
+#pragma unroll
+      for (int s = 0; s < 2 * R; ++s)
+        st_stream(wb + s, rec[s]);
+#pragma unroll
+      for (int s = 0; s < R; ++s) {
+        if (!act[s])
+          continue;
+        st_stream(q + (int64_t)row * R + s, acc[s]);
+        pq[s] = cadd(pq[s], bil(rec[R + s], acc[s]));
+      }
+    }
+    __syncthreads();  // prod is reused by the next tile
+    cur = nxt;
+  }
+  write_partials<R>(pq, lds, partial);
+}
+
+// Tiled variant whose record gather touches one cache line per nonzero.  A column's
+// record {z[R], p[R]} is 2R 16-byte pieces; 2R adjacent lanes load the pieces of one
+// nonzero in one instruction (so the L1 looks up the record's line once, where a lane
+// loading the whole record takes 2R lookups), lane q < R forms z_q + beta_q p_q with the
+// piece of lane q + R, and the products go to LDS for the row reduction of the tiled
+// kernel.  All loads of a tile step are issued with no branch (entries past the tile's
+// end read the tile's first entry and contribute zero).
+template <typename VT, int R, int TNNZ, int RL>
+__global__ void __launch_bounds__(kBlock)
+    k_cocg_spmv_lines(int ntiles, int xcd_span, const int32_t *__restrict__ tile_row,
+                      const int32_t *__restrict__ row_ptr, const int32_t *__restrict__ col,
+                      const VT *__restrict__ val, const double2 *__restrict__ Wa,
+                      double2 *__restrict__ Wb, double2 *__restrict__ q,
+                      double2 *__restrict__ partial, const SysState *__restrict__ st) {
+  constexpr int LPN = 2 * R;            // lanes per nonzero
+  constexpr int NPS = kBlock / LPN;     // nonzeros per block step
+  constexpr int J = TNNZ / NPS;         // steps per tile
+  static_assert(64 % LPN == 0 && J >= 1, "record pieces must tile a wavefront");
+  __shared__ double2 prod[TNNZ * R];
+  __shared__ double2 lds[(kBlock / 64) * R];
+  const int piece = threadIdx.x % LPN, slot = threadIdx.x / LPN;
+  // beta of the system whose z / p this lane combines (lanes piece < R)
+  const double2 beta_l = st[piece % R].beta;
+  const bool act_l = piece < R && st[piece % R].active != 0;
+  bool act[R];
+  double2 beta[R];
+#pragma unroll
+  for (int s = 0; s < R; ++s) {
+    act[s] = st[s].active != 0;
+    beta[s] = st[s].beta;
+  }
+  double2 pq[R];
+#pragma unroll
+  for (int s = 0; s < R; ++s)
+    pq[s] = make_double2(0, 0);
+  const int xcd = blockIdx.x & 7;
+  const int per_xcd = gridDim.x >> 3;
+  const int lb = blockIdx.x >> 3;
+  const int t_begin = xcd * xcd_span;
+  const int t_end = min(ntiles, t_begin + xcd_span);
+  for (int tile = t_begin + lb; tile < t_end; tile += per_xcd) {
+    const int32_t r0 = tile_row[tile], r1 = tile_row[tile + 1];
+    const int32_t kb = row_ptr[r0], ke = row_ptr[r1];
+    int32_t c[J];
+    double2 a[J], g[J];
+#pragma unroll
+    for (int j = 0; j < J; ++j) {
+      const int32_t k = kb + slot + j * NPS;
+      const int32_t kk = (k < ke) ? k : (ke > kb ? kb : 0);  // a valid entry either way
+      c[j] = ld_stream(col + kk);
+      const double2 av = ld_stream(val + kk);
+      a[j] = (k < ke) ? av : make_double2(0, 0);
+    }
+#pragma unroll
+    for (int j = 0; j < J; ++j)
+      g[j] = Wa[(int64_t)c[j] * LPN + piece];
+#pragma unroll
+    for (int j = 0; j < J; ++j) {
+      const double2 other = make_double2(__shfl_xor(g[j].x, R, LPN), __shfl_xor(g[j].y, R, LPN));
+      const int32_t lk = slot + j * NPS;
+      if (piece < R && kb + lk < ke)
+        prod[lk * R + piece] = bil(a[j], cfma(beta_l, other, g[j]));
+    }
+    __syncthreads();
+    const int gr = threadIdx.x / RL, l = threadIdx.x % RL;
+    const int32_t row = r0 + gr;
+    double2 acc[R];
+#pragma unroll
+    for (int s = 0; s < R; ++s)
+      acc[s] = make_double2(0, 0);
+    if (row < r1) {
+      const int32_t k0 = row_ptr[row] - kb, k1 = row_ptr[row + 1] - kb;
+      for (int32_t k = k0 + l; k < k1; k += RL) {
+#pragma unroll
+        for (int s = 0; s < R; ++s)
+          acc[s] = cadd(acc[s], prod[k * R + s]);
+      }
+    }
+#pragma unroll
+    for (int s = 0; s < R; ++s) {
+#pragma unroll
+      for (int off = RL / 2; off > 0; off >>= 1) {
+        acc[s].x += __shfl_xor(acc[s].x, off, RL);
+        acc[s].y += __shfl_xor(acc[s].y, off, RL);
+      }
+    }
+    if (row < r1 && l == 0) {
       const double2 *w = Wa + (int64_t)row * 2 * R;
       double2 *wb = Wb + (int64_t)row * 2 * R;
+      // the whole record is written (z = 0; the update overwrites it) so that no partial
+      // 64-byte line reaches HBM
       double2 rec[2 * R];
 #pragma unroll
       for (int s = 0; s < R; ++s) {
@@ -585,6 +702,237 @@ __global__ void __launch_bounds__(kBlock)
       }
     }
     __syncthreads();  // prod is reused by the next tile
+  }
+  (void)act_l;
+  write_partials<R>(pq, lds, partial);
+}
+
+// Tile-local column format for the first tiling (<= 1024 nonzeros per tile): the
+// tile's distinct columns, sorted, at ucol[kb .. kb + nu[t]) (kb = the tile's first
+// entry, so no offsets are needed), and for every entry its position in that list
+// (lidx, 16 bits).  One block per tile: bitonic sort of the tile's columns in LDS,
+// unique by a block scan, then a binary search per entry.
+constexpr int kLocalMax = 1024;
+
+__global__ void __launch_bounds__(kBlock)
+    k_build_local(int ntiles, const int4 *__restrict__ desc, const int32_t *__restrict__ col,
+                  int32_t *__restrict__ ucol, int32_t *__restrict__ nu, uint16_t *__restrict__ lidx) {
+  __shared__ int32_t key[kLocalMax];
+  __shared__ int32_t scan[kBlock];
+  const int t = blockIdx.x;
+  if (t >= ntiles)
+    return;
+  const int4 d = desc[t];
+  const int32_t kb = d.z, m = d.w - d.z;  // m <= kLocalMax
+  for (int i = threadIdx.x; i < kLocalMax; i += kBlock)
+    key[i] = i < m ? col[kb + i] : INT32_MAX;
+  __syncthreads();
+  for (int size = 2; size <= kLocalMax; size <<= 1) {
+    for (int stride = size >> 1; stride > 0; stride >>= 1) {
+      for (int i = threadIdx.x; i < kLocalMax; i += kBlock) {
+        const int j = i ^ stride;
+        if (j > i) {
+          const bool up = (i & size) == 0;
+          const int32_t a = key[i], b = key[j];
+          if ((a > b) == up) {
+            key[i] = b;
+            key[j] = a;
+          }
+        }
+      }
+      __syncthreads();
+    }
+  }
+  // unique: thread k handles entries 4k .. 4k+3
+  constexpr int PER = kLocalMax / kBlock;
+  int flag[PER], cnt = 0;
+#pragma unroll
+  for (int e = 0; e < PER; ++e) {
+    const int i = threadIdx.x * PER + e;
+    flag[e] = (key[i] != INT32_MAX && (i == 0 || key[i] != key[i - 1])) ? 1 : 0;
+    cnt += flag[e];
+  }
+  scan[threadIdx.x] = cnt;
+  __syncthreads();
+  for (int off = 1; off < kBlock; off <<= 1) {  // inclusive Hillis-Steele scan
+    const int v = threadIdx.x >= off ? scan[threadIdx.x - off] : 0;
+    __syncthreads();
+    scan[threadIdx.x] += v;
+    __syncthreads();
+  }
+  int pos = scan[threadIdx.x] - cnt;
+  const int total = scan[kBlock - 1];
+#pragma unroll
+  for (int e = 0; e < PER; ++e)
+    if (flag[e])
+      ucol[kb + pos++] = key[threadIdx.x * PER + e];
+  if (threadIdx.x == 0)
+    nu[t] = total;
+  __syncthreads();
+  // compact the distinct keys to the front of key[] for the searches
+  pos = scan[threadIdx.x] - cnt;
+  int32_t mine[PER];
+#pragma unroll
+  for (int e = 0; e < PER; ++e)
+    mine[e] = key[threadIdx.x * PER + e];
+  __syncthreads();
+#pragma unroll
+  for (int e = 0; e < PER; ++e)
+    if (flag[e])
+      key[pos++] = mine[e];
+  __syncthreads();
+  for (int i = threadIdx.x; i < m; i += kBlock) {
+    const int32_t c = col[kb + i];
+    int lo = 0, hi = total - 1;
+    while (lo < hi) {
+      const int mid = (lo + hi) >> 1;
+      if (key[mid] < c)
+        lo = mid + 1;
+      else
+        hi = mid;
+    }
+    lidx[kb + i] = (uint16_t)lo;
+  }
+}
+
+// SpMV on the tile-local format: phase 1 loads each distinct column's record of the tile
+// once (2R lanes per record, one cache line) and keeps p_new = z + beta p in LDS; phase 2
+// runs RL lanes per row over the row's entries (value and 16-bit local index streamed)
+// and reads p_new from LDS.  The gathered bytes drop by the tile's column reuse (about
+// 4x on tet meshes).
+template <typename VT, int R, int RL>
+__global__ void __launch_bounds__(kBlock)
+    k_cocg_spmv_local(int ntiles, int xcd_span, const int4 *__restrict__ desc,
+                      const int32_t *__restrict__ nu, const int32_t *__restrict__ ucol,
+                      const uint16_t *__restrict__ lidx, const int32_t *__restrict__ row_ptr,
+                      const VT *__restrict__ val, const double2 *__restrict__ Wa,
+                      double2 *__restrict__ Wb, double2 *__restrict__ q,
+                      double2 *__restrict__ partial, const SysState *__restrict__ st) {
+  constexpr int LPN = 2 * R;          // lanes per record
+  constexpr int NPS = kBlock / LPN;   // records per block step
+  static_assert(64 % LPN == 0, "record pieces must tile a wavefront");
+  __shared__ double2 pn[tile_nnz(R) * R];  // a tile has at most tile_nnz(R) distinct columns
+  __shared__ double2 lds[(kBlock / 64) * R];
+  const int piece = threadIdx.x % LPN, slot = threadIdx.x / LPN;
+  const double2 beta_l = st[piece % R].beta;
+  bool act[R];
+  double2 beta[R];
+#pragma unroll
+  for (int s = 0; s < R; ++s) {
+    act[s] = st[s].active != 0;
+    beta[s] = st[s].beta;
+  }
+  double2 pq[R];
+#pragma unroll
+  for (int s = 0; s < R; ++s)
+    pq[s] = make_double2(0, 0);
+  const int xcd = blockIdx.x & 7;
+  const int per_xcd = gridDim.x >> 3;
+  const int lb = blockIdx.x >> 3;
+  const int t_begin = xcd * xcd_span;
+  const int t_end = min(ntiles, t_begin + xcd_span);
+  const int g = threadIdx.x / RL, l = threadIdx.x % RL;
+  constexpr int EPL = 6;  // entries per lane held in registers (rows up to RL * EPL)
+  for (int tile = t_begin + lb; tile < t_end; tile += per_xcd) {
+    const int4 d = desc[tile];
+    const int u = nu[tile];
+    const int32_t kb = d.z;
+    // phase 2 inputs first (independent of phase 1): row bounds, values, local indices
+    const int32_t row = d.x + g;
+    const int32_t rc = row < d.y ? row : d.y - 1;
+    const int32_t k0 = row_ptr[rc], k1 = row < d.y ? row_ptr[rc + 1] : k0;
+    double2 a[EPL];
+    int li[EPL];
+    const int32_t kz = kb < d.w ? kb : 0;  // a valid entry either way
+#pragma unroll
+    for (int e = 0; e < EPL; ++e) {
+      const int32_t k = k0 + l + e * RL;
+      const int32_t kk = k < k1 ? k : kz;
+      a[e] = ld_stream(val + kk);
+      li[e] = (int)__builtin_nontemporal_load(lidx + kk);
+    }
+    double2 rz[R], rp[R];
+    {
+      const double2 *w = Wa + (int64_t)rc * 2 * R;
+#pragma unroll
+      for (int s = 0; s < R; ++s) {
+        rz[s] = w[s];
+        rp[s] = w[R + s];
+      }
+    }
+    // phase 1: the tile's distinct records
+    for (int i0 = 0; i0 < u; i0 += 4 * NPS) {
+      int32_t c[4];
+      double2 gv[4];
+#pragma unroll
+      for (int j = 0; j < 4; ++j) {
+        const int i = i0 + slot + j * NPS;
+        c[j] = ucol[kb + (i < u ? i : 0)];
+      }
+#pragma unroll
+      for (int j = 0; j < 4; ++j)
+        gv[j] = Wa[(int64_t)c[j] * LPN + piece];
+#pragma unroll
+      for (int j = 0; j < 4; ++j) {
+        const double2 other = make_double2(__shfl_xor(gv[j].x, R, LPN), __shfl_xor(gv[j].y, R, LPN));
+        const int i = i0 + slot + j * NPS;
+        if (piece < R && i < u)
+          pn[i * R + piece] = cfma(beta_l, other, gv[j]);
+      }
+    }
+    __syncthreads();
+    double2 acc[R];
+#pragma unroll
+    for (int s = 0; s < R; ++s)
+      acc[s] = make_double2(0, 0);
+#pragma unroll
+    for (int e = 0; e < EPL; ++e) {
+      // entries past the row's end are skipped by a select (their LDS slot may hold
+      // anything, so no multiplication by zero)
+      const bool in = k0 + l + e * RL < k1;
+      const int lk = in ? li[e] : 0;
+#pragma unroll
+      for (int s = 0; s < R; ++s) {
+        const double2 t = cfma(a[e], pn[lk * R + s], acc[s]);
+        acc[s].x = in ? t.x : acc[s].x;  // component selects (a struct select goes through memory)
+        acc[s].y = in ? t.y : acc[s].y;
+      }
+    }
+    // rows longer than RL * EPL entries
+    for (int32_t k = k0 + l + EPL * RL; k < k1; k += RL) {
+      const double2 av = to_d2(val[k]);
+      const int lk = (int)lidx[k];
+#pragma unroll
+      for (int s = 0; s < R; ++s)
+        acc[s] = cfma(av, pn[lk * R + s], acc[s]);
+    }
+#pragma unroll
+    for (int s = 0; s < R; ++s)
+#pragma unroll
+      for (int off = RL / 2; off > 0; off >>= 1) {
+        acc[s].x += __shfl_xor(acc[s].x, off, RL);
+        acc[s].y += __shfl_xor(acc[s].y, off, RL);
+      }
+    if (row < d.y && l == 0) {
+      double2 *wb = Wb + (int64_t)row * 2 * R;
+      double2 rec[2 * R];
+#pragma unroll
+      for (int s = 0; s < R; ++s) {
+        rec[s] = make_double2(0, 0);
+        rec[R + s] = cfma(beta[s], rp[s], rz[s]);
+      }
+#pragma unroll
+      for (int s = 0; s < 2 * R; ++s)
+        st_stream(wb + s, rec[s]);
+#pragma unroll
+      for (int s = 0; s < R; ++s) {
+        if (!act[s])
+          continue;
+        st_stream(q + (int64_t)row * R + s, acc[s]);
+        pq[s] = cadd(pq[s], bil(rec[R + s], acc[s]));
+      }
+    }
+    __syncthreads();  // pn is reused by the next tile
   }
   write_partials<R>(pq, lds, partial);
 }
@@ -1472,7 +1820,7 @@ __global__ void __launch_bounds__(kBlock)
 
 } // namespace
 
-constexpr int kNumSpmvVariants = 7;  // rows<4>, rows<8>, rows<16>, tiled<T,4>, tiled<T/2,8>, coop<T,4>, pipe<T,4>
+constexpr int kNumSpmvVariants = 9;  // rows<4>, rows<8>, rows<16>, tiled<T,4>, tiled<T/2,8>, coop<T,4>, pipe<T,4>, lines<T/2,8>, local<4>
 
 struct efm_solver {
   int64_t n = 0, nnz = 0;
@@ -1486,7 +1834,14 @@ struct efm_solver {
   // row tiles of the tiled SpMV variants (ntiles = 0: a row is longer than a tile)
   efm::DevBuf tile_row[2];
   efm::DevBuf tile_desc;  // int4 (row0, row1, nnz0, nnz1) per tile of the first tiling
+  // tile-local column format of the first tiling (built on first use by variant 8)
+  efm::DevBuf ucol, nu, lidx;
+  bool local_built = false;
+  int grid_local = 8;
   int ntiles[2] = {0, 0}, xcd_span[2] = {0, 0}, grid_tiles[2] = {8, 8};
+  int grid_lines = 8;  // the one-line-per-nonzero variant (second tiling, resident blocks)
+  int grid_tiles_res[2] = {8, 8};  // resident-block grids of the tiled variants
+  int grid_pipe_res = 8;
   int grid_rows[3] = {8, 8, 8};
   int spmv_variant = -1;  // -1: choose by timing at the first solve
   double tune_ms[kNumSpmvVariants] = {0};
@@ -1617,13 +1972,13 @@ template <int R> struct Dispatch {
                          s->col, s->val.as<VT>(), Wa, Wb, s->q.as<double2>(), s->partial.as<double2>(), S);
       break;
     case 3:
-      *nb = s->grid_tiles[0];
+      *nb = s->grid_tiles_res[0];
       hipLaunchKernelGGL((k_cocg_spmv_tiled<VT, R, TN, 4>), dim3(*nb), dim3(kBlock), 0, st, s->ntiles[0],
                          s->xcd_span[0], s->tile_row[0].as<int32_t>(), s->row_ptr, s->col, s->val.as<VT>(),
                          Wa, Wb, s->q.as<double2>(), s->partial.as<double2>(), S);
       break;
     case 4:
-      *nb = s->grid_tiles[1];
+      *nb = s->grid_tiles_res[1];
       hipLaunchKernelGGL((k_cocg_spmv_tiled<VT, R, tile_nnz2(R), 8>), dim3(*nb), dim3(kBlock), 0, st,
                          s->ntiles[1], s->xcd_span[1], s->tile_row[1].as<int32_t>(), s->row_ptr, s->col,
                          s->val.as<VT>(), Wa, Wb, s->q.as<double2>(), s->partial.as<double2>(), S);
@@ -1639,11 +1994,44 @@ template <int R> struct Dispatch {
       set_error("SpMV variant 5 needs an even number of right-hand sides");
       return EFM_ERR_ARG;
     case 6:
-      *nb = s->grid_tiles[0];
+      *nb = s->grid_pipe_res;
       hipLaunchKernelGGL((k_cocg_spmv_pipe<VT, R, TN, 4>), dim3(*nb), dim3(kBlock), 0, st, s->ntiles[0],
                          s->xcd_span[0], s->tile_desc.as<int4>(), s->row_ptr, s->col, s->val.as<VT>(), Wa, Wb,
                          s->q.as<double2>(), s->partial.as<double2>(), S);
       break;
+    case 7:
+      if constexpr (R == 1 || R == 2 || R == 4) {
+        *nb = s->grid_lines;
+        hipLaunchKernelGGL((k_cocg_spmv_lines<VT, R, tile_nnz2(R), 8>), dim3(*nb), dim3(kBlock), 0, st,
+                           s->ntiles[1], s->xcd_span[1], s->tile_row[1].as<int32_t>(), s->row_ptr, s->col,
+                           s->val.as<VT>(), Wa, Wb, s->q.as<double2>(), s->partial.as<double2>(), S);
+        break;
+      }
+      set_error("SpMV variant 7 needs 1, 2 or 4 right-hand sides");
+      return EFM_ERR_ARG;
+    case 8:
+      if constexpr (R == 1 || R == 2 || R == 4) {
+        if (!s->local_built) {
+          int rc;
+          if ((rc = s->ucol.alloc((size_t)s->nnz * sizeof(int32_t))) ||
+              (rc = s->nu.alloc((size_t)s->ntiles[0] * sizeof(int32_t))) ||
+              (rc = s->lidx.alloc((size_t)s->nnz * sizeof(uint16_t))))
+            return rc;
+          hipLaunchKernelGGL(k_build_local, dim3(s->ntiles[0]), dim3(kBlock), 0, st, s->ntiles[0],
+                             s->tile_desc.as<int4>(), s->col, s->ucol.as<int32_t>(), s->nu.as<int32_t>(),
+                             s->lidx.as<uint16_t>());
+          EFM_LAUNCH_CHECK();
+          s->local_built = true;
+        }
+        *nb = s->grid_local;
+        hipLaunchKernelGGL((k_cocg_spmv_local<VT, R, 4>), dim3(*nb), dim3(kBlock), 0, st, s->ntiles[0],
+                           s->xcd_span[0], s->tile_desc.as<int4>(), s->nu.as<int32_t>(), s->ucol.as<int32_t>(),
+                           s->lidx.as<uint16_t>(), s->row_ptr, s->val.as<VT>(), Wa, Wb, s->q.as<double2>(),
+                           s->partial.as<double2>(), S);
+        break;
+      }
+      set_error("SpMV variant 8 needs 1, 2 or 4 right-hand sides");
+      return EFM_ERR_ARG;
     default:
       set_error("unknown SpMV variant");
       return EFM_ERR_ARG;
@@ -1668,6 +2056,10 @@ template <int R> struct Dispatch {
       if (v == 5 && (s->ntiles[0] == 0 || (R % 2) != 0))
         continue;
       if (v == 6 && s->ntiles[0] == 0)
+        continue;
+      if (v == 7 && (s->ntiles[1] == 0 || !(R == 1 || R == 2 || R == 4)))
+        continue;
+      if (v == 8 && (s->ntiles[0] == 0 || !(R == 1 || R == 2 || R == 4)))
         continue;
       int nb;
       int rc = launch_spmv<VT>(s, v, s->W0.as<double2>(), s->W1.as<double2>(), st, &nb);  // warm-up
@@ -1891,11 +2283,105 @@ extern "C" int efm_solver_create(int64_t n, int64_t nnz, const int32_t *row_ptr,
     const int64_t span = (nchunks + 7) / 8;
     s->grid_rows[v] = (int)(8 * std::max<int64_t>(1, std::min<int64_t>(span, (int64_t)ncu)));
   }
+  // tiled variants: as many blocks as are resident (a multiple of 8), so that each block's
+  // tile walk runs once instead of in several rounds of blocks
+  {
+    const bool c64 = value_type == EFM_C64;
+    auto occ = [&](const void *k) {
+      int nb = 0;
+      if (!k || hipOccupancyMaxActiveBlocksPerMultiprocessor(&nb, k, kBlock, 0) != hipSuccess || nb < 1)
+        nb = 1;
+      return nb;
+    };
+    auto tiled_k = [&](int tv) -> const void * {
+#define EFM_TILED_K(RR)                                                                                         \
+  (tv == 0 ? (c64 ? reinterpret_cast<const void *>(&k_cocg_spmv_tiled<float2, RR, tile_nnz(RR), 4>)             \
+                  : reinterpret_cast<const void *>(&k_cocg_spmv_tiled<double2, RR, tile_nnz(RR), 4>))           \
+           : (c64 ? reinterpret_cast<const void *>(&k_cocg_spmv_tiled<float2, RR, tile_nnz2(RR), 8>)            \
+                  : reinterpret_cast<const void *>(&k_cocg_spmv_tiled<double2, RR, tile_nnz2(RR), 8>)))
+      switch (nrhs) {
+      case 1: return EFM_TILED_K(1);
+      case 2: return EFM_TILED_K(2);
+      case 3: return EFM_TILED_K(3);
+      case 4: return EFM_TILED_K(4);
+      default: return EFM_TILED_K(8);
+      }
+#undef EFM_TILED_K
+    };
+    for (int tv = 0; tv < 2; ++tv)
+      if (s->ntiles[tv] > 0) {
+        const int64_t per_xcd =
+            std::min<int64_t>(s->xcd_span[tv], std::max<int64_t>(1, (int64_t)ncu * occ(tiled_k(tv)) / 8));
+        s->grid_tiles_res[tv] = (int)(8 * per_xcd);
+      }
+    if (s->ntiles[0] > 0) {
+#define EFM_PIPE_K(RR)                                                                                    \
+  (c64 ? reinterpret_cast<const void *>(&k_cocg_spmv_pipe<float2, RR, tile_nnz(RR), 4>)                   \
+       : reinterpret_cast<const void *>(&k_cocg_spmv_pipe<double2, RR, tile_nnz(RR), 4>))
+      const void *kp = nrhs == 1 ? EFM_PIPE_K(1)
+                     : nrhs == 2 ? EFM_PIPE_K(2)
+                     : nrhs == 3 ? EFM_PIPE_K(3)
+                     : nrhs == 4 ? EFM_PIPE_K(4)
+                                 : EFM_PIPE_K(8);
+#undef EFM_PIPE_K
+      const int64_t per_xcd =
+          std::min<int64_t>(s->xcd_span[0], std::max<int64_t>(1, (int64_t)ncu * occ(kp) / 8));
+      s->grid_pipe_res = (int)(8 * per_xcd);
+    }
+  }
+  // one-line-per-nonzero variant: as many blocks as are resident (a multiple of 8)
+  if (s->ntiles[1] > 0) {
+    const void *kl = nullptr;
+    const bool c64 = value_type == EFM_C64;
+    switch (nrhs) {
+    case 1:
+      kl = c64 ? reinterpret_cast<const void *>(&k_cocg_spmv_lines<float2, 1, tile_nnz2(1), 8>)
+               : reinterpret_cast<const void *>(&k_cocg_spmv_lines<double2, 1, tile_nnz2(1), 8>);
+      break;
+    case 2:
+      kl = c64 ? reinterpret_cast<const void *>(&k_cocg_spmv_lines<float2, 2, tile_nnz2(2), 8>)
+               : reinterpret_cast<const void *>(&k_cocg_spmv_lines<double2, 2, tile_nnz2(2), 8>);
+      break;
+    case 4:
+      kl = c64 ? reinterpret_cast<const void *>(&k_cocg_spmv_lines<float2, 4, tile_nnz2(4), 8>)
+               : reinterpret_cast<const void *>(&k_cocg_spmv_lines<double2, 4, tile_nnz2(4), 8>);
+      break;
+    default:
+      break;
+    }
+    int nbl = 0;
+    if (!kl || hipOccupancyMaxActiveBlocksPerMultiprocessor(&nbl, kl, kBlock, 0) != hipSuccess || nbl < 1)
+      nbl = 1;
+    const int64_t per_xcd = std::min<int64_t>(s->xcd_span[1], std::max<int64_t>(1, (int64_t)ncu * nbl / 8));
+    s->grid_lines = (int)(8 * per_xcd);
+  }
+  if (s->ntiles[0] > 0 && (nrhs == 1 || nrhs == 2 || nrhs == 4)) {
+    const bool c64 = value_type == EFM_C64;
+    const void *kl = nullptr;
+    if (nrhs == 1)
+      kl = c64 ? reinterpret_cast<const void *>(&k_cocg_spmv_local<float2, 1, 4>)
+               : reinterpret_cast<const void *>(&k_cocg_spmv_local<double2, 1, 4>);
+    else if (nrhs == 2)
+      kl = c64 ? reinterpret_cast<const void *>(&k_cocg_spmv_local<float2, 2, 4>)
+               : reinterpret_cast<const void *>(&k_cocg_spmv_local<double2, 2, 4>);
+    else
+      kl = c64 ? reinterpret_cast<const void *>(&k_cocg_spmv_local<float2, 4, 4>)
+               : reinterpret_cast<const void *>(&k_cocg_spmv_local<double2, 4, 4>);
+    int nbl = 0;
+    if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&nbl, kl, kBlock, 0) != hipSuccess || nbl < 1)
+      nbl = 1;
+    const int64_t per_xcd = std::min<int64_t>(s->xcd_span[0], std::max<int64_t>(1, (int64_t)ncu * nbl / 8));
+    s->grid_local = (int)(8 * per_xcd);
+  }
   int gmax = std::max(s->grid, s->grid_spmv);
   for (int v = 0; v < 3; ++v)
     gmax = std::max(gmax, s->grid_rows[v]);
   for (int v = 0; v < 2; ++v)
     gmax = std::max(gmax, s->grid_tiles[v]);
+  gmax = std::max(gmax, s->grid_lines);
+  gmax = std::max(gmax, std::max(s->grid_tiles_res[0], s->grid_tiles_res[1]));
+  gmax = std::max(gmax, s->grid_pipe_res);
+  gmax = std::max(gmax, s->grid_local);
   if ((rc = s->val.alloc(vbytes)) || (rc = s->dinv.alloc(n * sizeof(double2))) ||
       (rc = s->x.alloc(vb)) || (rc = s->r.alloc(vb)) || (rc = s->q.alloc(vb)) ||
       (rc = s->W0.alloc(2 * vb)) || (rc = s->W1.alloc(2 * vb)) ||
@@ -2113,10 +2599,13 @@ extern "C" int efm_solver_spmv(efm_solver *s, const double *x, double *y, void *
 extern "C" int efm_solver_set_spmv_variant(efm_solver *s, int variant) {
   EFM_REQUIRE(s, "efm_solver_set_spmv_variant: NULL solver");
   EFM_REQUIRE(variant >= -1 && variant < kNumSpmvVariants, "efm_solver_set_spmv_variant: bad variant");
-  EFM_REQUIRE(variant < 3 || s->ntiles[(variant == 5 || variant == 6) ? 0 : variant - 3] > 0,
+  EFM_REQUIRE(variant < 3 ||
+                  s->ntiles[(variant == 5 || variant == 6 || variant == 8) ? 0 : (variant == 7 ? 1 : variant - 3)] > 0,
               "efm_solver_set_spmv_variant: matrix has a row longer than a tile");
   EFM_REQUIRE(variant != 5 || s->R % 2 == 0,
               "efm_solver_set_spmv_variant: variant 5 needs an even number of right-hand sides");
+  EFM_REQUIRE((variant != 7 && variant != 8) || s->R == 1 || s->R == 2 || s->R == 4,
+              "efm_solver_set_spmv_variant: variants 7 and 8 need 1, 2 or 4 right-hand sides");
   s->spmv_variant = variant;
   return EFM_OK;
 }

@@ -184,9 +184,13 @@ int efm_solver_solve(efm_solver *s, const double *B, double *X, const efm_solve_
 /* The COCG SpMV has several row-to-lane mappings (0-2: 4, 8, 16 lanes per row; 3-4:
  * LDS-tiled with 4 or 8 lanes per row; 5: LDS-tiled with a cooperative gather of each
  * column's record, even nrhs only; 6: LDS-tiled with the next tile's loads issued while
- * the current tile's gathers are in flight).  By default (-1) the first solve times every
- * variant on the installed matrix and keeps the fastest; a fixed variant makes results
- * reproducible bit for bit across runs.  h_tune_ms (7 entries, may be NULL) receives the
+ * the current tile's gathers are in flight; 7: LDS-tiled with each column's record
+ * loaded by 2*nrhs adjacent lanes, one cache line per nonzero, nrhs 1, 2 or 4; 8: a
+ * tile-local column format (each tile's distinct columns loaded once into LDS, 16-bit
+ * local indices per entry, built on the device at first use), nrhs 1, 2 or 4).  By
+ * default (-1) the first solve times every variant on the installed matrix and keeps the
+ * fastest; a fixed variant makes results reproducible bit for bit across runs.
+ * h_tune_ms (9 entries, may be NULL) receives the
  * measured average ms per launch (-1 = not available).  No reference counterpart. */
 int efm_solver_set_spmv_variant(efm_solver *s, int variant);
 int efm_solver_get_spmv_variant(const efm_solver *s, int *h_variant, double *h_tune_ms);
