@@ -483,7 +483,7 @@ def main():
     if args.traffic_from:
         cmd = ("rocprofv3 --pmc {c} --output-format csv -d gpurun_out/pmc_{d} -o {d} -- python -u bench.py --no-cpu "
                "--no-check --steps 1 --warmup 0 --max-it 20 --spmv-variant {v} --calibrate-copy 512")
-        v = max(args.spmv_variant, 0) if args.spmv_variant >= 0 else 3
+        v = args.spmv_variant if args.spmv_variant >= 0 else 6
         traffic_from_pmc(args.traffic_from[0], args.traffic_from[1], os.path.join(ROOT, "profiles", "traffic_c2.json"),
                          v, [cmd.format(c="FETCH_SIZE", d="fetch", v=v), cmd.format(c="WRITE_SIZE", d="write", v=v)])
         return
