@@ -578,12 +578,12 @@ __global__ void __launch_bounds__(kBlock)
       }
 #pragma unroll
       for (int s = 0; s < 2 * R; ++s)
-        st_stream(wb + s, rec[s]);
+        wb[s] = rec[s];
 #pragma unroll
       for (int s = 0; s < R; ++s) {
         if (!act[s])
           continue;
-        st_stream(q + (int64_t)row * R + s, acc[s]);
+        q[(int64_t)row * R + s] = acc[s];
         pq[s] = cadd(pq[s], bil(rec[R + s], acc[s]));
       }
     }
@@ -923,12 +923,12 @@ __global__ void __launch_bounds__(kBlock)
       }
 #pragma unroll
       for (int s = 0; s < 2 * R; ++s)
-        st_stream(wb + s, rec[s]);
+        wb[s] = rec[s];
 #pragma unroll
       for (int s = 0; s < R; ++s) {
         if (!act[s])
           continue;
-        st_stream(q + (int64_t)row * R + s, acc[s]);
+        q[(int64_t)row * R + s] = acc[s];
         pq[s] = cadd(pq[s], bil(rec[R + s], acc[s]));
       }
     }
