@@ -483,7 +483,7 @@ def main():
     if args.traffic_from:
         cmd = ("rocprofv3 --pmc {c} --output-format csv -d gpurun_out/pmc_{d} -o {d} -- python -u bench.py --no-cpu "
                "--no-check --steps 1 --warmup 0 --max-it 20 --spmv-variant {v} --calibrate-copy 512")
-        v = args.spmv_variant if args.spmv_variant >= 0 else 6
+        v = args.spmv_variant if args.spmv_variant >= 0 else 9
         traffic_from_pmc(args.traffic_from[0], args.traffic_from[1], os.path.join(ROOT, "profiles", "traffic_c2.json"),
                          v, [cmd.format(c="FETCH_SIZE", d="fetch", v=v), cmd.format(c="WRITE_SIZE", d="write", v=v)])
         return
@@ -545,7 +545,10 @@ def main():
     nnz = mesh.topology.nnz
     R = len(ports)
     s_val = 16 if args.mode == "fp64" else 8
-    spmv_bytes = nnz * (s_val + 4) + 4 * (n + 1) + R * n * 16 * 4
+    # algorithmic bytes of the chosen SpMV: the fused form reads z and p_old and writes p and
+    # q per row and RHS (4 passes); the z-gather form (variant 9) reads z, p_old, q_old and
+    # writes p, q (5 passes)
+    spmv_bytes = nnz * (s_val + 4) + 4 * (n + 1) + R * n * 16 * (5 if variant == 9 else 4)
     avg_ms = spmv_ms / max(spmv_n, 1)
     achieved = spmv_bytes / (avg_ms * 1e-3)
     iters = int(max(pipe.last_info["iters"]))
@@ -556,7 +559,7 @@ def main():
         tj = json.load(open(tpath))
         if tj.get("variant") == variant and args.mode == "fp64":
             for k, v in tj["kernels"].items():
-                if k.startswith("k_cocg_spmv"):
+                if k.startswith(("k_cocg_spmv", "k_cocgz_spmv")):
                     traffic = v["hbm_bytes"] / 1e9
     dS = None
     ref_rel = None
@@ -581,7 +584,7 @@ def main():
                    "mode": args.mode},
         "roofline": {"bound": "hbm", "achieved": achieved / 1e9, "peak": HBM_PEAK / 1e9, "unit": "GB/s",
                      "frac": achieved / HBM_PEAK, "traffic": traffic, "traffic_unit": "GB per launch",
-                     "kernel": "k_cocg_spmv",
+                     "kernel": "k_cocgz_spmv" if variant == 9 else "k_cocg_spmv",
                      "bytes_per_launch": spmv_bytes, "avg_launch_ms": avg_ms, "launches": spmv_n,
                      "variant": variant, "autotune_ms": [round(float(t), 4) for t in tune_ms]},
         "cpu_baseline": cpu,

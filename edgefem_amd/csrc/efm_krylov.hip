@@ -1804,6 +1804,203 @@ __global__ void __launch_bounds__(kBlock)
     write_partials<2 * R>(v, lds, partial);
 }
 
+// ---------------------------------------------------------------------------
+// COCG in the z-gather form.  The same iteration, with A p carried by the recurrence
+//   p_new = z + beta p_old,   q_new = A z + beta q_old   (= A p_new),
+// so the SpMV gathers only z (R values per column, a separate array) and reads p_old,
+// q_old at the row itself; the update writes z, x, r.  The gathered bytes per nonzero
+// are halved against the {z, p} record of the fused form.
+// ---------------------------------------------------------------------------
+// after the fused-form init: zf = W0.z, pf = 0, q = 0
+template <int R>
+__global__ void __launch_bounds__(kBlock)
+    k_cocgz_init(int64_t n, const double2 *__restrict__ W, double2 *__restrict__ zf, double2 *__restrict__ pf,
+                 double2 *__restrict__ q) {
+  for (int64_t i = (int64_t)blockIdx.x * kBlock + threadIdx.x; i < n * R; i += (int64_t)gridDim.x * kBlock) {
+    const int64_t row = i / R, sys = i - row * R;
+    zf[i] = W[row * 2 * R + sys];
+    pf[i] = make_double2(0, 0);
+    q[i] = make_double2(0, 0);
+  }
+}
+
+// Tiled, pipelined SpMV of the z-gather form (the tile walk, staging and branch-free loads
+// of k_cocg_spmv_pipe); partials p_new^T q_new.  EXACT: zf holds p (already formed), and
+// the kernel writes q = A p only (the periodic replacement of the q recurrence).
+template <typename VT, int R, int TNNZ, int RL, bool EXACT>
+__global__ void __launch_bounds__(kBlock)
+    k_cocgz_spmv(int ntiles, int xcd_span, const int4 *__restrict__ desc, const int32_t *__restrict__ row_ptr,
+                 const int32_t *__restrict__ col, const VT *__restrict__ val, const double2 *__restrict__ zf,
+                 double2 *__restrict__ pf, double2 *__restrict__ q, double2 *__restrict__ partial,
+                 const SysState *__restrict__ st) {
+  __shared__ double2 prod[TNNZ * R];
+  __shared__ double2 lds[(kBlock / 64) * R];
+  constexpr int J = TNNZ / kBlock;
+  bool act[R];
+  double2 beta[R];
+#pragma unroll
+  for (int s = 0; s < R; ++s) {
+    act[s] = st[s].active != 0;
+    beta[s] = EXACT ? make_double2(0, 0) : st[s].beta;
+  }
+  double2 pq[R];
+#pragma unroll
+  for (int s = 0; s < R; ++s)
+    pq[s] = make_double2(0, 0);
+  const int xcd = blockIdx.x & 7;
+  const int per_xcd = gridDim.x >> 3;
+  const int lb = blockIdx.x >> 3;
+  const int t_begin = xcd * xcd_span;
+  const int t_end = min(ntiles, t_begin + xcd_span);
+  const int g = threadIdx.x / RL, l = threadIdx.x % RL;
+  struct Stage {
+    int4 d;
+    double2 a[J];
+    int32_t c[J];
+    int32_t k0, k1;
+  };
+  auto stage = [&](Stage &x, int tile) {
+    x.d = desc[tile];
+    const int32_t kz = x.d.z < x.d.w ? x.d.z : 0;  // a valid entry either way
+#pragma unroll
+    for (int j = 0; j < J; ++j) {
+      const int32_t k = x.d.z + threadIdx.x + j * kBlock;
+      const bool in = k < x.d.w;
+      const int32_t kk = in ? k : kz;
+      const double2 av = ld_stream(val + kk);
+      x.c[j] = ld_stream(col + kk);
+      x.a[j].x = in ? av.x : 0.0;
+      x.a[j].y = in ? av.y : 0.0;
+    }
+    const int32_t row = x.d.x + g;
+    const int32_t rc = row < x.d.y ? row : x.d.y - 1;
+    const int32_t rb = row_ptr[rc], re = row_ptr[rc + 1];
+    x.k0 = rb - x.d.z;
+    x.k1 = row < x.d.y ? re - x.d.z : x.k0;
+  };
+  int tile = t_begin + lb;
+  if (tile >= t_end) {
+    write_partials<R>(pq, lds, partial);
+    return;
+  }
+  Stage cur, nxt;
+  stage(cur, tile);
+  for (; tile < t_end; tile += per_xcd) {
+    double2 zg[J][R];
+#pragma unroll
+    for (int j = 0; j < J; ++j) {
+      const double2 *zc = zf + (int64_t)cur.c[j] * R;
+#pragma unroll
+      for (int s = 0; s < R; ++s)
+        zg[j][s] = zc[s];
+    }
+    const int32_t row = cur.d.x + g;
+    const int32_t rc = row < cur.d.y ? row : cur.d.y - 1;
+    double2 zr[R], po[R], qo[R];
+#pragma unroll
+    for (int s = 0; s < R; ++s) {
+      zr[s] = zf[(int64_t)rc * R + s];
+      po[s] = pf[(int64_t)rc * R + s];
+      qo[s] = q[(int64_t)rc * R + s];
+    }
+    stage(nxt, min(tile + per_xcd, t_end - 1));
+#pragma unroll
+    for (int j = 0; j < J; ++j) {
+      const int32_t lk = threadIdx.x + j * kBlock;
+#pragma unroll
+      for (int s = 0; s < R; ++s)
+        prod[lk * R + s] = bil(cur.a[j], zg[j][s]);
+    }
+    __syncthreads();
+    double2 acc[R];
+#pragma unroll
+    for (int s = 0; s < R; ++s)
+      acc[s] = make_double2(0, 0);
+    for (int32_t k = cur.k0 + l; k < cur.k1; k += RL)
+#pragma unroll
+      for (int s = 0; s < R; ++s)
+        acc[s] = cadd(acc[s], prod[k * R + s]);
+#pragma unroll
+    for (int s = 0; s < R; ++s)
+#pragma unroll
+      for (int off = RL / 2; off > 0; off >>= 1) {
+        acc[s].x += __shfl_xor(acc[s].x, off, RL);
+        acc[s].y += __shfl_xor(acc[s].y, off, RL);
+      }
+    if (row < cur.d.y && l == 0) {
+#pragma unroll
+      for (int s = 0; s < R; ++s) {
+        if (!act[s])
+          continue;
+        const double2 pn = EXACT ? zr[s] : cfma(beta[s], po[s], zr[s]);
+        const double2 qn = EXACT ? acc[s] : cfma(beta[s], qo[s], acc[s]);
+        if (!EXACT)
+          pf[(int64_t)row * R + s] = pn;
+        q[(int64_t)row * R + s] = qn;
+        pq[s] = cadd(pq[s], bil(pn, qn));
+      }
+    }
+    __syncthreads();  // prod is reused by the next tile
+    cur = nxt;
+  }
+  write_partials<R>(pq, lds, partial);
+}
+
+// p = z + beta p (before an exact q = A p)
+template <int R>
+__global__ void __launch_bounds__(kBlock)
+    k_cocgz_pform(int64_t n, const double2 *__restrict__ zf, double2 *__restrict__ pf,
+                  const SysState *__restrict__ st) {
+  for (int64_t i = (int64_t)blockIdx.x * kBlock + threadIdx.x; i < n * R; i += (int64_t)gridDim.x * kBlock) {
+    const int sys = (int)(i % R);
+    if (st[sys].active)
+      pf[i] = cfma(st[sys].beta, pf[i], zf[i]);
+  }
+}
+
+// x += alpha p, r -= alpha q, z = D^{-1} r (+ G y with the auxiliary preconditioner);
+// partials r^T z, r^H r
+template <int R, bool AUX>
+__global__ void __launch_bounds__(kBlock)
+    k_cocgz_update(int64_t n, const int2 *__restrict__ en_eff, const double2 *__restrict__ dinv,
+                   const double2 *__restrict__ y, const double2 *__restrict__ pf, const double2 *__restrict__ q,
+                   double2 *__restrict__ zf, double2 *__restrict__ x, double2 *__restrict__ r,
+                   double2 *__restrict__ partial, const SysState *__restrict__ st) {
+  __shared__ double2 lds[(kBlock / 64) * 2 * R];
+  bool act[R];
+  double2 alpha[R];
+#pragma unroll
+  for (int s = 0; s < R; ++s) {
+    act[s] = st[s].active != 0;
+    alpha[s] = st[s].alpha;
+  }
+  double2 v[2 * R];
+#pragma unroll
+  for (int s = 0; s < 2 * R; ++s)
+    v[s] = make_double2(0, 0);
+  for (int64_t e = (int64_t)blockIdx.x * kBlock + threadIdx.x; e < n; e += (int64_t)gridDim.x * kBlock) {
+    const double2 di = dinv[e];
+    const int2 ab = AUX ? en_eff[e] : make_int2(-1, -1);
+#pragma unroll
+    for (int s = 0; s < R; ++s) {
+      if (!act[s])
+        continue;
+      const int64_t k = e * R + s;
+      const double2 xv = cfma(alpha[s], pf[k], x[k]);
+      const double2 rv = csub(r[k], bil(alpha[s], q[k]));
+      double2 zv = bil(di, rv);
+      if (AUX && ab.x >= 0)
+        zv = cadd(zv, csub(y[(int64_t)ab.y * R + s], y[(int64_t)ab.x * R + s]));
+      x[k] = xv;
+      r[k] = rv;
+      zf[k] = zv;
+      v[s] = cadd(v[s], bil(rv, zv));
+      v[R + s].x += cabs2(rv);
+    }
+  }
+  write_partials<2 * R>(v, lds, partial);
+}
+
 // x = 0, r = b, W0.p = 0 (W0.z comes from the aux kernels)
 template <int R>
 __global__ void __launch_bounds__(kBlock)
@@ -1820,7 +2017,9 @@ __global__ void __launch_bounds__(kBlock)
 
 } // namespace
 
-constexpr int kNumSpmvVariants = 9;  // rows<4>, rows<8>, rows<16>, tiled<T,4>, tiled<T/2,8>, coop<T,4>, pipe<T,4>, lines<T/2,8>, local<4>
+constexpr int kNumSpmvVariants = 10;  // rows<4>, rows<8>, rows<16>, tiled<T,4>, tiled<T/2,8>, coop<T,4>, pipe<T,4>, lines<T/2,8>, local<4>, z-form
+constexpr int kZForm = 9;             // the z-gather form of COCG (its own SpMV and update)
+constexpr int kZReplace = 8;          // z-form: q = A p computed exactly every kZReplace iterations
 
 struct efm_solver {
   int64_t n = 0, nnz = 0;
@@ -1842,6 +2041,9 @@ struct efm_solver {
   int grid_lines = 8;  // the one-line-per-nonzero variant (second tiling, resident blocks)
   int grid_tiles_res[2] = {8, 8};  // resident-block grids of the tiled variants
   int grid_pipe_res = 8;
+  // z-gather form of COCG: z and p arrays [n][R]; resident grid of its SpMV
+  efm::DevBuf zf, pf;
+  int grid_z = 8;
   int grid_rows[3] = {8, 8, 8};
   int spmv_variant = -1;  // -1: choose by timing at the first solve
   double tune_ms[kNumSpmvVariants] = {0};
@@ -1925,13 +2127,21 @@ template <int R> struct Dispatch {
     return EFM_OK;
   }
 
+  // the z-form arrays from the fused-form init (z = W0.z, p = q = 0)
+  static int init_zform(efm_solver *s, hipStream_t st) {
+    hipLaunchKernelGGL(k_cocgz_init<R>, dim3(s->grid), dim3(kBlock), 0, st, s->n, s->W0.as<double2>(),
+                       s->zf.as<double2>(), s->pf.as<double2>(), s->q.as<double2>());
+    EFM_LAUNCH_CHECK();
+    return EFM_OK;
+  }
+
   static int init(efm_solver *s, const double2 *b, hipStream_t st, int method, double tol2) {
     if (method == EFM_METHOD_COCG && s->aux) {
       hipLaunchKernelGGL(k_cocg_init_aux<R>, dim3(s->grid), dim3(kBlock), 0, st, s->n, b,
                          s->x.as<double2>(), s->r.as<double2>(), s->W0.as<double2>());
       EFM_LAUNCH_CHECK();
       int rc = aux_apply<true>(s, s->r.as<double2>(), s->W0.as<double2>(), 2 * R, st, nullptr);
-      if (rc != EFM_OK)
+      if (rc != EFM_OK || (rc = init_zform(s, st)) != EFM_OK)
         return rc;
       return fin<kFinInitCocg>(s, s->grid, st, tol2, 0);
     }
@@ -1940,6 +2150,9 @@ template <int R> struct Dispatch {
                          s->dinv.as<double2>(), s->x.as<double2>(), s->r.as<double2>(),
                          s->W0.as<double2>(), s->partial.as<double2>());
       EFM_LAUNCH_CHECK();
+      int rc = init_zform(s, st);
+      if (rc != EFM_OK)
+        return rc;
       return fin<kFinInitCocg>(s, s->grid, st, tol2, 0);
     }
     hipLaunchKernelGGL(k_bs_init<R>, dim3(s->grid), dim3(kBlock), 0, st, s->n, b, s->x.as<double2>(),
@@ -2032,6 +2245,13 @@ template <int R> struct Dispatch {
       }
       set_error("SpMV variant 8 needs 1, 2 or 4 right-hand sides");
       return EFM_ERR_ARG;
+    case kZForm:
+      *nb = s->grid_z;
+      hipLaunchKernelGGL((k_cocgz_spmv<VT, R, TN, 4, false>), dim3(*nb), dim3(kBlock), 0, st, s->ntiles[0],
+                         s->xcd_span[0], s->tile_desc.as<int4>(), s->row_ptr, s->col, s->val.as<VT>(),
+                         s->zf.as<double2>(), s->pf.as<double2>(), s->q.as<double2>(), s->partial.as<double2>(),
+                         S);
+      break;
     default:
       set_error("unknown SpMV variant");
       return EFM_ERR_ARG;
@@ -2060,6 +2280,8 @@ template <int R> struct Dispatch {
       if (v == 7 && (s->ntiles[1] == 0 || !(R == 1 || R == 2 || R == 4)))
         continue;
       if (v == 8 && (s->ntiles[0] == 0 || !(R == 1 || R == 2 || R == 4)))
+        continue;
+      if (v == kZForm && (s->ntiles[0] == 0 || s->lr_P > 0))
         continue;
       int nb;
       int rc = launch_spmv<VT>(s, v, s->W0.as<double2>(), s->W1.as<double2>(), st, &nb);  // warm-up
@@ -2092,6 +2314,50 @@ template <int R> struct Dispatch {
     double2 *Wa = (it & 1) ? s->W1.as<double2>() : s->W0.as<double2>();
     double2 *Wb = (it & 1) ? s->W0.as<double2>() : s->W1.as<double2>();
     SysState *S = s->state.as<SysState>();
+    if (s->spmv_variant == kZForm) {
+      int nb_spmv = 0;
+      int rc;
+      const bool exact = (it % kZReplace) == kZReplace - 1;
+      if (exact) {
+        // replace the recurrence: p = z + beta p, then q = A p exactly
+        hipLaunchKernelGGL(k_cocgz_pform<R>, dim3(s->grid), dim3(kBlock), 0, st, s->n, s->zf.as<double2>(),
+                           s->pf.as<double2>(), S);
+        EFM_LAUNCH_CHECK();
+      }
+      if (e0)
+        EFM_HIP_CHECK(hipEventRecord(e0, st));
+      if (exact) {
+        nb_spmv = s->grid_z;
+        hipLaunchKernelGGL((k_cocgz_spmv<VT, R, tile_nnz(R), 4, true>), dim3(nb_spmv), dim3(kBlock), 0, st,
+                           s->ntiles[0], s->xcd_span[0], s->tile_desc.as<int4>(), s->row_ptr, s->col,
+                           s->val.as<VT>(), s->pf.as<double2>(), s->pf.as<double2>(), s->q.as<double2>(),
+                           s->partial.as<double2>(), S);
+        EFM_LAUNCH_CHECK();
+      } else if ((rc = launch_spmv<VT>(s, kZForm, Wa, Wb, st, &nb_spmv)) != EFM_OK) {
+        return rc;
+      }
+      if (e1)
+        EFM_HIP_CHECK(hipEventRecord(e1, st));
+      if ((rc = fin<kFinAlpha>(s, nb_spmv, st, tol2, max_it)) != EFM_OK)
+        return rc;
+      if (s->aux) {
+        hipLaunchKernelGGL((k_aux_node<R, true>), dim3(s->grid_node_groups), dim3(kBlock), 0, st,
+                           s->n_nodes, s->n2e_ptr, s->n2e, s->gsign.as<int8_t>(), s->dninv.as<double2>(),
+                           s->r.as<double2>(), s->q.as<double2>(), s->ynode.as<double2>(), S);
+        EFM_LAUNCH_CHECK();
+        hipLaunchKernelGGL((k_cocgz_update<R, true>), dim3(s->grid), dim3(kBlock), 0, st, s->n,
+                           s->en_eff.as<int2>(), s->dinv.as<double2>(), s->ynode.as<double2>(),
+                           s->pf.as<double2>(), s->q.as<double2>(), s->zf.as<double2>(), s->x.as<double2>(),
+                           s->r.as<double2>(), s->partial.as<double2>(), S);
+      } else {
+        hipLaunchKernelGGL((k_cocgz_update<R, false>), dim3(s->grid), dim3(kBlock), 0, st, s->n,
+                           (const int2 *)nullptr, s->dinv.as<double2>(), (const double2 *)nullptr,
+                           s->pf.as<double2>(), s->q.as<double2>(), s->zf.as<double2>(), s->x.as<double2>(),
+                           s->r.as<double2>(), s->partial.as<double2>(), S);
+      }
+      EFM_LAUNCH_CHECK();
+      return fin<kFinBeta>(s, s->grid, st, tol2, max_it);
+    }
     if (e0)
       EFM_HIP_CHECK(hipEventRecord(e0, st));
     int nb_spmv = 0;
@@ -2327,6 +2593,18 @@ extern "C" int efm_solver_create(int64_t n, int64_t nnz, const int32_t *row_ptr,
       const int64_t per_xcd =
           std::min<int64_t>(s->xcd_span[0], std::max<int64_t>(1, (int64_t)ncu * occ(kp) / 8));
       s->grid_pipe_res = (int)(8 * per_xcd);
+#define EFM_Z_K(RR)                                                                                       \
+  (c64 ? reinterpret_cast<const void *>(&k_cocgz_spmv<float2, RR, tile_nnz(RR), 4, false>)                \
+       : reinterpret_cast<const void *>(&k_cocgz_spmv<double2, RR, tile_nnz(RR), 4, false>))
+      const void *kz = nrhs == 1 ? EFM_Z_K(1)
+                     : nrhs == 2 ? EFM_Z_K(2)
+                     : nrhs == 3 ? EFM_Z_K(3)
+                     : nrhs == 4 ? EFM_Z_K(4)
+                                 : EFM_Z_K(8);
+#undef EFM_Z_K
+      const int64_t per_xcd_z =
+          std::min<int64_t>(s->xcd_span[0], std::max<int64_t>(1, (int64_t)ncu * occ(kz) / 8));
+      s->grid_z = (int)(8 * per_xcd_z);
     }
   }
   // one-line-per-nonzero variant: as many blocks as are resident (a multiple of 8)
@@ -2381,10 +2659,12 @@ extern "C" int efm_solver_create(int64_t n, int64_t nnz, const int32_t *row_ptr,
   gmax = std::max(gmax, s->grid_lines);
   gmax = std::max(gmax, std::max(s->grid_tiles_res[0], s->grid_tiles_res[1]));
   gmax = std::max(gmax, s->grid_pipe_res);
+  gmax = std::max(gmax, s->grid_z);
   gmax = std::max(gmax, s->grid_local);
   if ((rc = s->val.alloc(vbytes)) || (rc = s->dinv.alloc(n * sizeof(double2))) ||
       (rc = s->x.alloc(vb)) || (rc = s->r.alloc(vb)) || (rc = s->q.alloc(vb)) ||
-      (rc = s->W0.alloc(2 * vb)) || (rc = s->W1.alloc(2 * vb)) ||
+      (rc = s->W0.alloc(2 * vb)) || (rc = s->W1.alloc(2 * vb)) || (rc = s->zf.alloc(vb)) ||
+      (rc = s->pf.alloc(vb)) ||
       (rc = s->partial.alloc((size_t)(gmax + 1) * 2 * kMaxRhs * sizeof(double2))) ||
       (rc = s->state.alloc(kMaxRhs * sizeof(SysState))) || (rc = s->control.alloc(sizeof(Control))))
     return rc;
@@ -2600,7 +2880,8 @@ extern "C" int efm_solver_set_spmv_variant(efm_solver *s, int variant) {
   EFM_REQUIRE(s, "efm_solver_set_spmv_variant: NULL solver");
   EFM_REQUIRE(variant >= -1 && variant < kNumSpmvVariants, "efm_solver_set_spmv_variant: bad variant");
   EFM_REQUIRE(variant < 3 ||
-                  s->ntiles[(variant == 5 || variant == 6 || variant == 8) ? 0 : (variant == 7 ? 1 : variant - 3)] > 0,
+                  s->ntiles[(variant == 5 || variant == 6 || variant == 8 || variant == kZForm) ? 0
+                                                                                         : (variant == 7 ? 1 : variant - 3)] > 0,
               "efm_solver_set_spmv_variant: matrix has a row longer than a tile");
   EFM_REQUIRE(variant != 5 || s->R % 2 == 0,
               "efm_solver_set_spmv_variant: variant 5 needs an even number of right-hand sides");
@@ -2649,6 +2930,8 @@ extern "C" int efm_solver_solve(efm_solver *s, const double *B, double *X,
   const int check = h_opts->check_every > 0 ? h_opts->check_every : 32;
   const bool cocg = h_opts->method == EFM_METHOD_COCG;
   EFM_REQUIRE(!(cocg && s->caux), "efm_solver_solve: the complex-incidence preconditioner needs BiCGSTAB");
+  EFM_REQUIRE(!(cocg && s->spmv_variant == kZForm && s->lr_P > 0),
+              "efm_solver_solve: SpMV variant 9 (z-gather form) does not take low-rank port terms");
   int rc;
   if (!cocg) {
     const size_t vb = (size_t)s->n * s->R * sizeof(double2);
