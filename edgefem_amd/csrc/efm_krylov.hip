@@ -1959,7 +1959,8 @@ __global__ void __launch_bounds__(kBlock)
 }
 
 // x += alpha p, r -= alpha q, z = D^{-1} r (+ G y with the auxiliary preconditioner);
-// partials r^T z, r^H r
+// partials r^T z, r^H r.  One (row, system) element per thread, so every load and store
+// of a wavefront is one contiguous run (no partial lines).
 template <int R, bool AUX>
 __global__ void __launch_bounds__(kBlock)
     k_cocgz_update(int64_t n, const int2 *__restrict__ en_eff, const double2 *__restrict__ dinv,
@@ -1978,25 +1979,39 @@ __global__ void __launch_bounds__(kBlock)
 #pragma unroll
   for (int s = 0; s < 2 * R; ++s)
     v[s] = make_double2(0, 0);
-  for (int64_t e = (int64_t)blockIdx.x * kBlock + threadIdx.x; e < n; e += (int64_t)gridDim.x * kBlock) {
-    const double2 di = dinv[e];
-    const int2 ab = AUX ? en_eff[e] : make_int2(-1, -1);
+  for (int64_t k = (int64_t)blockIdx.x * kBlock + threadIdx.x; k < n * R; k += (int64_t)gridDim.x * kBlock) {
+    const int64_t e = k / R;
+    const int sys = (int)(k - e * R);
+    double2 al = make_double2(0, 0);
+    bool a = false;
 #pragma unroll
-    for (int s = 0; s < R; ++s) {
-      if (!act[s])
-        continue;
-      const int64_t k = e * R + s;
-      const double2 xv = cfma(alpha[s], pf[k], x[k]);
-      const double2 rv = csub(r[k], bil(alpha[s], q[k]));
-      double2 zv = bil(di, rv);
-      if (AUX && ab.x >= 0)
-        zv = cadd(zv, csub(y[(int64_t)ab.y * R + s], y[(int64_t)ab.x * R + s]));
-      x[k] = xv;
-      r[k] = rv;
-      zf[k] = zv;
-      v[s] = cadd(v[s], bil(rv, zv));
-      v[R + s].x += cabs2(rv);
+    for (int s = 0; s < R; ++s)
+      if (s == sys) {
+        al = alpha[s];
+        a = act[s];
+      }
+    if (!a)
+      continue;
+    const double2 di = dinv[e];
+    const double2 xv = cfma(al, pf[k], x[k]);
+    const double2 rv = csub(r[k], bil(al, q[k]));
+    double2 zv = bil(di, rv);
+    if (AUX) {
+      const int2 ab = en_eff[e];
+      if (ab.x >= 0)
+        zv = cadd(zv, csub(y[(int64_t)ab.y * R + sys], y[(int64_t)ab.x * R + sys]));
     }
+    x[k] = xv;
+    r[k] = rv;
+    zf[k] = zv;
+    const double2 rz = bil(rv, zv);
+    const double rr = cabs2(rv);
+#pragma unroll
+    for (int s = 0; s < R; ++s)
+      if (s == sys) {
+        v[s] = cadd(v[s], rz);
+        v[R + s].x += rr;
+      }
   }
   write_partials<2 * R>(v, lds, partial);
 }
