@@ -1666,9 +1666,14 @@ __global__ void __launch_bounds__(kBlock)
   }
   const int g = threadIdx.x / kLanesPerNode, lane = threadIdx.x % kLanesPerNode;
   constexpr int kNodesPerBlock = kBlock / kLanesPerNode;
-  for (int64_t nd0 = (int64_t)blockIdx.x * kNodesPerBlock; nd0 < N;
-       nd0 += (int64_t)gridDim.x * kNodesPerBlock) {
-    const int64_t nd = nd0 + g;
+  // XCD-aware walk (grid a multiple of 8): block b runs on XCD b % 8 and takes chunks of
+  // that XCD's contiguous node range, so an edge gathered by both of its end nodes is
+  // usually found in the same L2 the second time
+  const int64_t nchunks = (N + kNodesPerBlock - 1) / kNodesPerBlock;
+  const int64_t span = (nchunks + 7) / 8;
+  const int64_t c_end = min(nchunks, ((int64_t)(blockIdx.x & 7) + 1) * span);
+  for (int64_t ch = (int64_t)(blockIdx.x & 7) * span + (blockIdx.x >> 3); ch < c_end; ch += gridDim.x >> 3) {
+    const int64_t nd = ch * kNodesPerBlock + g;
     double2 acc[R];
 #pragma unroll
     for (int s = 0; s < R; ++s)
@@ -2865,7 +2870,8 @@ extern "C" int efm_solver_set_gradient(efm_solver *s, int64_t n_nodes, const int
   }
   s->grid_nodes = (int)std::max<int64_t>(1, std::min<int64_t>((n_nodes + kBlock - 1) / kBlock, (int64_t)ncu * 8));
   const int64_t node_groups = (n_nodes + (kBlock / kLanesPerNode) - 1) / (kBlock / kLanesPerNode);
-  s->grid_node_groups = (int)std::max<int64_t>(1, std::min<int64_t>(node_groups, (int64_t)ncu * 8));
+  // a multiple of 8 (the nodal kernel's XCD-aware walk)
+  s->grid_node_groups = (int)(8 * std::max<int64_t>(1, std::min<int64_t>((node_groups + 7) / 8, (int64_t)ncu)));
   if ((rc = s->gsign.alloc((size_t)2 * s->n + 16)) || (rc = s->en_eff.alloc((size_t)s->n * sizeof(int2))))
     return rc;
   hipStream_t st = (hipStream_t)stream;
