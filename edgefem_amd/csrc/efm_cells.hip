@@ -253,20 +253,29 @@ __global__ void __launch_bounds__(kBlock)
     beta[s] = S.beta;
     omega[s] = S.omega;
   }
-  for (int64_t i = t.r0 + threadIdx.x; i < t.r1; i += kBlock)
+  // one (row, system) element per thread: every access of a wavefront is contiguous
+  for (int64_t k = t.r0 * R + threadIdx.x; k < t.r1 * R; k += kBlock) {
+    const int sys = (int)(k % R);
+    bool a = false, rsk = false;
+    double2 bt = make_double2(0, 0), om = make_double2(0, 0);
 #pragma unroll
-    for (int s = 0; s < R; ++s) {
-      if (!act[s])
-        continue;
-      const int64_t k = i * R + s;
-      if (rs[s]) {
-        r0[k] = r[k];
-        v[k] = make_double2(0, 0);
-        p[k] = r[k];
-      } else {
-        p[k] = cadd(r[k], bil(beta[s], csub(p[k], bil(omega[s], v[k]))));
+    for (int s = 0; s < R; ++s)
+      if (s == sys) {
+        a = act[s];
+        rsk = rs[s];
+        bt = beta[s];
+        om = omega[s];
       }
+    if (!a)
+      continue;
+    if (rsk) {
+      r0[k] = r[k];
+      v[k] = make_double2(0, 0);
+      p[k] = r[k];
+    } else {
+      p[k] = cadd(r[k], bil(bt, csub(p[k], bil(om, v[k]))));
     }
+  }
 }
 
 // nodal part of P: y = Dn^{-1} G^T in  (node tiles)
@@ -316,20 +325,23 @@ __global__ void __launch_bounds__(kBlock)
 #pragma unroll
   for (int s = 0; s < R; ++s)
     act[s] = st[(int64_t)t.cell * R + s].active != 0;
-  for (int64_t e = t.r0 + threadIdx.x; e < t.r1; e += kBlock) {
-    const int2 nn = en2[e];
-    const double2 di = dinv[e];
+  for (int64_t k = t.r0 * R + threadIdx.x; k < t.r1 * R; k += kBlock) {
+    const int64_t e = k / R;
+    const int sys = (int)(k - e * R);
+    bool a = false;
 #pragma unroll
-    for (int s = 0; s < R; ++s) {
-      if (!act[s])
-        continue;
-      double2 v = cmul(di, in[e * R + s]);
-      if (nn.x >= 0)
-        v = cfma(ga[e], y[(int64_t)nn.x * R + s], v);
-      if (nn.y >= 0)
-        v = cfma(gb[e], y[(int64_t)nn.y * R + s], v);
-      out[e * R + s] = v;
-    }
+    for (int s = 0; s < R; ++s)
+      if (s == sys)
+        a = act[s];
+    if (!a)
+      continue;
+    const int2 nn = en2[e];
+    double2 v = cmul(dinv[e], in[k]);
+    if (nn.x >= 0)
+      v = cfma(ga[e], y[(int64_t)nn.x * R + sys], v);
+    if (nn.y >= 0)
+      v = cfma(gb[e], y[(int64_t)nn.y * R + sys], v);
+    out[k] = v;
   }
 }
 
@@ -375,10 +387,11 @@ __global__ void __launch_bounds__(kBlock)
         acc[s].x += __shfl_xor(acc[s].x, off, kLanes);
         acc[s].y += __shfl_xor(acc[s].y, off, kLanes);
       }
-    if (row < t.r1 && lane == 0)
+    // lane s of the row writes system s (the row's R values in one contiguous store)
+    if (row < t.r1 && lane < R)
 #pragma unroll
       for (int s = 0; s < R; ++s) {
-        if (!act[s])
+        if (s != lane || !act[s])
           continue;
         const int64_t k = row * R + s;
         out[k] = acc[s];
@@ -475,13 +488,19 @@ __global__ void __launch_bounds__(kBlock)
     act[s] = S.active != 0;
     alpha[s] = S.alpha;
   }
-  for (int64_t i = t.r0 + threadIdx.x; i < t.r1; i += kBlock)
+  for (int64_t k = t.r0 * R + threadIdx.x; k < t.r1 * R; k += kBlock) {
+    const int sys = (int)(k % R);
+    bool a = false;
+    double2 al = make_double2(0, 0);
 #pragma unroll
     for (int s = 0; s < R; ++s)
-      if (act[s]) {
-        const int64_t k = i * R + s;
-        r[k] = csub(r[k], bil(alpha[s], v[k]));
+      if (s == sys) {
+        a = act[s];
+        al = alpha[s];
       }
+    if (a)
+      r[k] = csub(r[k], bil(al, v[k]));
+  }
 }
 
 // x += alpha y + omega zz; r = s - omega t; partials r0^H r, |r|^2
@@ -507,18 +526,31 @@ __global__ void __launch_bounds__(kBlock)
 #pragma unroll
   for (int s = 0; s < 2 * R; ++s)
     d[s] = make_double2(0, 0);
-  for (int64_t i = t.r0 + threadIdx.x; i < t.r1; i += kBlock)
+  for (int64_t k = t.r0 * R + threadIdx.x; k < t.r1 * R; k += kBlock) {
+    const int sys = (int)(k % R);
+    bool a = false;
+    double2 al = make_double2(0, 0), om = make_double2(0, 0);
 #pragma unroll
-    for (int s = 0; s < R; ++s) {
-      if (!act[s])
-        continue;
-      const int64_t k = i * R + s;
-      x[k] = cadd(x[k], cadd(bil(alpha[s], y[k]), bil(omega[s], zz[k])));
-      const double2 rv = csub(r[k], bil(omega[s], tvec[k]));
-      r[k] = rv;
-      d[s] = cfma(cconj(r0[k]), rv, d[s]);
-      d[R + s].x += cabs2(rv);
-    }
+    for (int s = 0; s < R; ++s)
+      if (s == sys) {
+        a = act[s];
+        al = alpha[s];
+        om = omega[s];
+      }
+    if (!a)
+      continue;
+    x[k] = cadd(x[k], cadd(bil(al, y[k]), bil(om, zz[k])));
+    const double2 rv = csub(r[k], bil(om, tvec[k]));
+    r[k] = rv;
+    const double2 dr = cfma(cconj(r0[k]), rv, make_double2(0, 0));
+    const double rr = cabs2(rv);
+#pragma unroll
+    for (int s = 0; s < R; ++s)
+      if (s == sys) {
+        d[s] = cadd(d[s], dr);
+        d[R + s].x += rr;
+      }
+  }
   put_partials<2 * R>(d, lds, partial, blockIdx.x);
 }
 
