@@ -278,7 +278,10 @@ __global__ void __launch_bounds__(kBlock)
   }
 }
 
-// nodal part of P: y = Dn^{-1} G^T in  (node tiles)
+// nodal part of P: y = Dn^{-1} G^T in  (node tiles).  kNodeLanes lanes per node split the
+// node's edge list, so its gathers are independent; a shuffle sums the lanes' parts.
+constexpr int kNodeLanes = 8;
+
 template <int R>
 __global__ void __launch_bounds__(kBlock)
     k_cells_aux_node(const Tile *__restrict__ ntiles, const int32_t *__restrict__ gt_ptr,
@@ -292,23 +295,37 @@ __global__ void __launch_bounds__(kBlock)
 #pragma unroll
   for (int s = 0; s < R; ++s)
     act[s] = st[(int64_t)t.cell * R + s].active != 0;
-  for (int64_t nd = t.r0 + threadIdx.x; nd < t.r1; nd += kBlock) {
+  const int g = threadIdx.x / kNodeLanes, l = threadIdx.x % kNodeLanes;
+  constexpr int kNodes = kBlock / kNodeLanes;
+  for (int64_t nd0 = t.r0; nd0 < t.r1; nd0 += kNodes) {
+    const int64_t nd = nd0 + g;
     double2 acc[R];
 #pragma unroll
     for (int s = 0; s < R; ++s)
       acc[s] = make_double2(0, 0);
-    for (int32_t q = gt_ptr[nd]; q < gt_ptr[nd + 1]; ++q) {
-      const int64_t e = gt_edge[q];
-      const double2 g = gt_coef[q];
+    if (nd < t.r1)
+      for (int32_t q = gt_ptr[nd] + l; q < gt_ptr[nd + 1]; q += kNodeLanes) {
+        const int64_t e = gt_edge[q];
+        const double2 gc = gt_coef[q];
 #pragma unroll
-      for (int s = 0; s < R; ++s)
-        acc[s] = cfma(g, in[e * R + s], acc[s]);
-    }
-    const double2 di = dninv[nd];
+        for (int s = 0; s < R; ++s)
+          acc[s] = cfma(gc, in[e * R + s], acc[s]);
+      }
 #pragma unroll
     for (int s = 0; s < R; ++s)
-      if (act[s])
-        y[nd * R + s] = cmul(di, acc[s]);
+#pragma unroll
+      for (int off = kNodeLanes / 2; off > 0; off >>= 1) {
+        acc[s].x += __shfl_xor(acc[s].x, off, kNodeLanes);
+        acc[s].y += __shfl_xor(acc[s].y, off, kNodeLanes);
+      }
+    // lane s of the node writes system s (the node's R values in one contiguous store)
+    if (nd < t.r1 && l < R) {
+      const double2 di = dninv[nd];
+#pragma unroll
+      for (int s = 0; s < R; ++s)
+        if (s == l && act[s])
+          y[nd * R + s] = cmul(di, acc[s]);
+    }
   }
 }
 
