@@ -363,14 +363,22 @@ __global__ void __launch_bounds__(kBlock)
 }
 
 // out = A in (block-diagonal CSR, kLanes lanes per row); MODE 0: partials r0^H out;
-// MODE 1: partials out^H other, |out|^2
+// MODE 1: partials out^H other, |out|^2.  The tile's row pointers are staged in LDS once;
+// each lane holds up to kEPL entries of its row (branch-free loads: lanes past the row's
+// end read its first entry and contribute nothing), and the next step's entries are loaded
+// while the current step's gathers are in flight.  Rows longer than kLanes * kEPL take an
+// extra loop.
+constexpr int kEPL = 3;
+
 template <int R, int MODE>
 __global__ void __launch_bounds__(kBlock)
     k_cells_spmv(const Tile *__restrict__ tiles, const int32_t *__restrict__ row_ptr, const int32_t *__restrict__ col,
                  const double2 *__restrict__ val, const double2 *__restrict__ in, double2 *__restrict__ out,
                  const double2 *__restrict__ other, double2 *__restrict__ partial, const CellSys *__restrict__ st) {
   constexpr int V = MODE == 0 ? R : 2 * R;
+  constexpr int kStepRows = kBlock / kLanes;
   __shared__ double2 lds[(kBlock / 64) * 2 * kMaxR];
+  __shared__ int32_t srp[kRowsPerTile + 1];
   const Tile t = tiles[blockIdx.x];
   if (!cell_active<R>(st, t.cell))  // the finalisation ignores the partials of inactive systems
     return;
@@ -378,25 +386,63 @@ __global__ void __launch_bounds__(kBlock)
 #pragma unroll
   for (int s = 0; s < R; ++s)
     act[s] = st[(int64_t)t.cell * R + s].active != 0;
+  const int nrows = (int)(t.r1 - t.r0);  // <= kRowsPerTile
+  for (int i = threadIdx.x; i <= nrows; i += kBlock)
+    srp[i] = row_ptr[t.r0 + i];
+  __syncthreads();
   const int g = threadIdx.x / kLanes, lane = threadIdx.x % kLanes;
   double2 d[V];
 #pragma unroll
   for (int s = 0; s < V; ++s)
     d[s] = make_double2(0, 0);
-  for (int64_t row0 = t.r0; row0 < t.r1; row0 += kBlock / kLanes) {
-    const int64_t row = row0 + g;
+  struct Stage {
+    double2 a[kEPL];
+    int64_t c[kEPL];
+  };
+  auto stage = [&](Stage &x, int rl) {  // rl: the thread's row in the tile (may be >= nrows)
+    const int rc = rl < nrows ? rl : nrows - 1;
+    const int32_t kb = srp[rc], ke = rl < nrows ? srp[rc + 1] : kb;
+#pragma unroll
+    for (int j = 0; j < kEPL; ++j) {
+      const int32_t k = kb + lane + j * kLanes;
+      const bool inr = k < ke;
+      const int32_t kk = inr ? k : kb;
+      const double2 av = val[kk];
+      x.c[j] = col[kk];
+      x.a[j].x = inr ? av.x : 0.0;
+      x.a[j].y = inr ? av.y : 0.0;
+    }
+  };
+  Stage cur, nxt;
+  stage(cur, g);
+  for (int row0 = 0; row0 < nrows; row0 += kStepRows) {
+    const int rl = row0 + g;
+    double2 xv[kEPL][R];
+#pragma unroll
+    for (int j = 0; j < kEPL; ++j)
+#pragma unroll
+      for (int s = 0; s < R; ++s)
+        xv[j][s] = in[cur.c[j] * R + s];
+    stage(nxt, rl + kStepRows);
     double2 acc[R];
 #pragma unroll
     for (int s = 0; s < R; ++s)
       acc[s] = make_double2(0, 0);
-    if (row < t.r1)
-      for (int32_t k = row_ptr[row] + lane; k < row_ptr[row + 1]; k += kLanes) {
+#pragma unroll
+    for (int j = 0; j < kEPL; ++j)
+#pragma unroll
+      for (int s = 0; s < R; ++s)
+        acc[s] = cfma(cur.a[j], xv[j][s], acc[s]);
+    if (rl < nrows) {  // entries past kLanes * kEPL
+      const int32_t ke = srp[rl + 1];
+      for (int32_t k = srp[rl] + lane + kEPL * kLanes; k < ke; k += kLanes) {
         const double2 a = val[k];
         const int64_t c = col[k];
 #pragma unroll
         for (int s = 0; s < R; ++s)
           acc[s] = cfma(a, in[c * R + s], acc[s]);
       }
+    }
 #pragma unroll
     for (int s = 0; s < R; ++s)
 #pragma unroll
@@ -405,12 +451,12 @@ __global__ void __launch_bounds__(kBlock)
         acc[s].y += __shfl_xor(acc[s].y, off, kLanes);
       }
     // lane s of the row writes system s (the row's R values in one contiguous store)
-    if (row < t.r1 && lane < R)
+    if (rl < nrows && lane < R)
 #pragma unroll
       for (int s = 0; s < R; ++s) {
         if (s != lane || !act[s])
           continue;
-        const int64_t k = row * R + s;
+        const int64_t k = (t.r0 + rl) * R + s;
         out[k] = acc[s];
         const double2 o = other[k];
         if (MODE == 0) {
@@ -420,6 +466,7 @@ __global__ void __launch_bounds__(kBlock)
           d[R + s].x += cabs2(acc[s]);
         }
       }
+    cur = nxt;
   }
   put_partials<V>(d, lds, partial, blockIdx.x);
 }
@@ -620,10 +667,13 @@ __global__ void k_cells_check(const Tile *__restrict__ tiles, const int64_t *__r
                               int *__restrict__ err) {
   const Tile t = tiles[blockIdx.x];
   const int64_t lo = row_off[t.cell], hi = row_off[t.cell + 1];
-  for (int64_t r = t.r0 + threadIdx.x; r < t.r1; r += kBlock)
+  for (int64_t r = t.r0 + threadIdx.x; r < t.r1; r += kBlock) {
+    if (row_ptr[r + 1] <= row_ptr[r])  // the SpMV reads a row's first entry for its idle lanes
+      err[0] = 1;
     for (int32_t k = row_ptr[r]; k < row_ptr[r + 1]; ++k)
       if (col[k] < lo || col[k] >= hi)
         err[0] = 1;
+  }
 }
 
 } // namespace
@@ -781,7 +831,7 @@ extern "C" int efm_cells_create(int64_t ncells, const int64_t *h_row_off, const 
     int h_err = 0;
     EFM_HIP_CHECK(hipMemcpyAsync(&h_err, err.p, sizeof(int), hipMemcpyDeviceToHost, st));
     EFM_HIP_CHECK(hipStreamSynchronize(st));
-    EFM_REQUIRE(h_err == 0, "efm_cells_create: a column leaves its cell");
+    EFM_REQUIRE(h_err == 0, "efm_cells_create: a column leaves its cell, or a row has no entry");
   }
   DevBuf ext;
   if ((rc = upload(ext, extra.data(), n, st)) || (rc = s->dinv.alloc(n * sizeof(double2))) ||
