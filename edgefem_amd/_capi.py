@@ -466,15 +466,18 @@ class SweepSolver:
         return X, dict(iters=iters, resid=resid)
 
     def set_variants(self, spmv: int = -1, aux: int = -1) -> None:
-        """Kernel variants of the SpMV and of the nodal preconditioner part: 0 one block
-        slice per group of 64 systems, 1 pipelined (groups in turn), -1 the faster of the
-        two, timed on the first solve."""
+        """Kernel variants of the SpMV (0 one block slice per group of 64 systems, 1
+        pipelined, groups in turn, 2 LDS-tiled: each tile of 8 rows loads its distinct
+        columns once per group; only where every tile fits) and of the nodal
+        preconditioner part (0, 1 as for the SpMV); -1: the fastest, timed on the first
+        solve."""
         _check(self._lib.efm_sweep_set_variants(self._h, int(spmv), int(aux)), "efm_sweep_set_variants")
 
     def variants(self):
-        """(SpMV variant, nodal variant, tuning ms per launch [SpMV 0, SpMV 1, nodal 0, nodal 1])."""
+        """(SpMV variant, nodal variant, mean ms per launch over the tuning iterations
+        [SpMV 0, 1, 2, nodal 0, 1]; 0 for a variant that was not timed)."""
         v, a = C.c_int(), C.c_int()
-        ms = np.zeros(4, np.float64)
+        ms = np.zeros(5, np.float64)
         _check(self._lib.efm_sweep_get_variants(self._h, C.byref(v), C.byref(a), ms.ctypes.data_as(C.c_void_p)),
                "efm_sweep_get_variants")
         return v.value, a.value, ms

@@ -20,6 +20,7 @@
 // result of a system does not depend on which other systems share the launch.
 #include "efm_internal.h"
 
+#include <algorithm>
 #include <type_traits>
 #include <vector>
 
@@ -696,6 +697,379 @@ __global__ void __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(3, 
   }
 }
 
+// ---------------------------------------------------------------------------
+// LDS-tiled SpMV.  Rows are cut into tiles of kTileRows consecutive rows.  The entries
+// of a tile reference few distinct columns (on tetrahedral meshes ~56 for 8 rows of ~15
+// entries), so the block loads each distinct column of u once per system group into LDS
+// (one 64-lane load per column) and the rows read their columns from LDS: the vector
+// memory work per row drops from one load per entry to ~7, which relieves the address
+// unit that bounds the per-entry gathers.  Format, built once per sweep by
+// k_sw_build_tiles:
+//   ucol[t kTileCols + c], c < nu[t]: the distinct columns of tile t's entries and its
+//     own row indices, sorted (slots nu[t] .. kTileCols-1 repeat the first one)
+//   lidx[k]: slot of entry k's column (8 bits; lidx[nnz] = 0 for the padding entry)
+//   dloc[r]: slot of row r's own index
+//   nu[t] = -1: the tile does not fit (more than kTileCols distinct columns, more than
+//     kTileKeys entries and rows, or a row longer than 64 entries); the tiled kernel is
+//     then not used for the matrix.
+// K, M and Ms carry one zero entry at index nnz, which lanes past a row's end read.
+// ---------------------------------------------------------------------------
+constexpr int kTileRows = 8;
+constexpr int kTileCols = 72;
+constexpr int kTileKeys = 512;
+constexpr int kTileWaves = 8;  // blocks of 8 waves, one row of the tile each
+constexpr int kTileBlock = kTileWaves * 64;
+constexpr int kRowsPerWave = kTileRows / kTileWaves;
+constexpr int kColSlots = kTileCols / kTileWaves;  // column slots of wave w: w + kTileWaves i
+
+__global__ void __launch_bounds__(kBlock)
+    k_sw_build_tiles(int64_t n, const int32_t *__restrict__ row_ptr, const int32_t *__restrict__ col,
+                     int32_t *__restrict__ ucol, int32_t *__restrict__ nu, uint8_t *__restrict__ lidx,
+                     uint8_t *__restrict__ dloc) {
+  __shared__ int32_t key[kTileKeys];
+  __shared__ int32_t scan[kBlock];
+  __shared__ int too_long;
+  const int tid = threadIdx.x;
+  const int64_t t = blockIdx.x;
+  const int64_t r0 = t * kTileRows;
+  const int nr = (int)min<int64_t>(kTileRows, n - r0);
+  const int32_t k0 = row_ptr[r0];
+  const int ne = row_ptr[r0 + nr] - k0, m = ne + nr;
+  if (tid == 0)
+    too_long = 0;
+  __syncthreads();
+  if (tid < nr && row_ptr[r0 + tid + 1] - row_ptr[r0 + tid] > 64)
+    too_long = 1;
+  __syncthreads();
+  if (too_long || m > kTileKeys) {
+    if (tid == 0)
+      nu[t] = -1;
+    return;
+  }
+  for (int i = tid; i < kTileKeys; i += kBlock)
+    key[i] = i < ne ? col[k0 + i] : (i < m ? (int32_t)(r0 + (i - ne)) : INT32_MAX);
+  __syncthreads();
+  for (int size = 2; size <= kTileKeys; size <<= 1) {  // bitonic sort
+    for (int stride = size >> 1; stride > 0; stride >>= 1) {
+      for (int i = tid; i < kTileKeys; i += kBlock) {
+        const int j = i ^ stride;
+        if (j > i) {
+          const bool up = (i & size) == 0;
+          const int32_t a = key[i], b = key[j];
+          if ((a > b) == up) {
+            key[i] = b;
+            key[j] = a;
+          }
+        }
+      }
+      __syncthreads();
+    }
+  }
+  constexpr int PER = kTileKeys / kBlock;  // thread k flags keys PER k .. PER k + PER - 1
+  int flag[PER], cnt = 0;
+  int32_t mine[PER];
+#pragma unroll
+  for (int e = 0; e < PER; ++e) {
+    const int i = tid * PER + e;
+    mine[e] = key[i];
+    flag[e] = (mine[e] != INT32_MAX && (i == 0 || mine[e] != key[i - 1])) ? 1 : 0;
+    cnt += flag[e];
+  }
+  scan[tid] = cnt;
+  __syncthreads();
+  for (int off = 1; off < kBlock; off <<= 1) {  // inclusive Hillis-Steele scan
+    const int v = tid >= off ? scan[tid - off] : 0;
+    __syncthreads();
+    scan[tid] += v;
+    __syncthreads();
+  }
+  const int total = scan[kBlock - 1];
+  if (total > kTileCols) {
+    if (tid == 0)
+      nu[t] = -1;
+    return;
+  }
+  int pos = scan[tid] - cnt;
+  const int32_t first = key[0];
+  __syncthreads();  // every thread has read key[] before it is compacted
+#pragma unroll
+  for (int e = 0; e < PER; ++e)
+    if (flag[e]) {
+      key[pos] = mine[e];
+      ucol[t * kTileCols + pos] = mine[e];
+      ++pos;
+    }
+  for (int i = total + tid; i < kTileCols; i += kBlock)
+    ucol[t * kTileCols + i] = first;
+  if (tid == 0)
+    nu[t] = total;
+  __syncthreads();
+  for (int i = tid; i < m; i += kBlock) {
+    const int32_t c = i < ne ? col[k0 + i] : (int32_t)(r0 + (i - ne));
+    int lo = 0, hi = total - 1;
+    while (lo < hi) {
+      const int mid = (lo + hi) >> 1;
+      if (key[mid] < c)
+        lo = mid + 1;
+      else
+        hi = mid;
+    }
+    if (i < ne)
+      lidx[k0 + i] = (uint8_t)lo;
+    else
+      dloc[r0 + (i - ne)] = (uint8_t)lo;
+  }
+}
+
+// Tile metadata of one step for one wave, loaded two steps ahead of its use.
+struct TileMeta {
+  int32_t uc;  // lane i < kColSlots: column of slot w + kTileWaves i
+  int32_t rp;  // lane i <= kRowsPerWave: row_ptr of the wave's row i (row clamped to n)
+  int32_t pt;  // lane i < kRowsPerWave: port of row i (row clamped to n - 1)
+  int32_t dl;  // lane i < kRowsPerWave: slot of row i's own index
+  int nu;      // distinct columns of the tile
+};
+
+// Row entries of one step, loaded one step ahead: lane e holds entry e of each of the
+// wave's rows (the zero padding entry past the row's end).  Port rows (few) load their
+// port masses and coefficient when they are computed.
+struct TileRows {
+  int li[kRowsPerWave];
+  double kx[kRowsPerWave], ky[kRowsPerWave], mx[kRowsPerWave], my[kRowsPerWave];
+  double k2;
+  int lo[kRowsPerWave], len[kRowsPerWave], port[kRowsPerWave], dl[kRowsPerWave];
+  int64_t row0;
+  int g, nu;
+};
+
+// Accumulates K x and M x over a row's entries read from LDS (NU fixed entries, the
+// lanes past the row's end holding zero coefficients; NU = 0: len entries).
+template <int NU, bool KC, bool MC, typename VT>
+__device__ inline void tile_row(const VT *su, int lane, int len, const TileRows &q, int r, double2 &aK,
+                                double2 &aM) {
+  auto entry = [&](int t) {
+    const double2 x = ldv(su + bcast(q.li[r], t) * 64 + lane);
+    const double kr = bcast(q.kx[r], t), mr = bcast(q.mx[r], t);
+    if (KC) {
+      aK = cfma(make_double2(kr, bcast(q.ky[r], t)), x, aK);
+    } else {
+      aK.x += kr * x.x;
+      aK.y += kr * x.y;
+    }
+    if (MC) {
+      aM = cfma(make_double2(mr, bcast(q.my[r], t)), x, aM);
+    } else {
+      aM.x += mr * x.x;
+      aM.y += mr * x.y;
+    }
+  };
+  if constexpr (NU > 0) {
+#pragma unroll
+    for (int t = 0; t < NU; ++t)
+      entry(t);
+  } else {
+    for (int t = 0; t < len; ++t)
+      entry(t);
+  }
+}
+
+// w = A_s u; partials d = u^T w, on the tile format.  Each block walks its tiles (an
+// XCD-contiguous range, as the other pipelined kernels) once per active system group,
+// groups in turn, as one flat sequence of steps.  Step j: the columns of step j (in
+// registers since step j-1) are written to LDS; the stores of step j-1 are issued; the
+// metadata of step j+2 and the columns and entries of step j+1 are loaded; step j's rows
+// are computed from LDS.  The stores precede the loads, so the wait for step j+1's loads
+// at the top of the next step never includes stores issued after them.
+template <typename VT, bool KC, bool MC>
+__global__ void __launch_bounds__(kTileBlock) __attribute__((amdgpu_waves_per_eu(4, 8)))
+    k_sw_spmv_tile(int64_t n, int B, int P, int groups, const int32_t *__restrict__ row_ptr,
+                   const int32_t *__restrict__ ucol, const int32_t *__restrict__ nu,
+                   const uint8_t *__restrict__ lidx, const uint8_t *__restrict__ dloc,
+                   const double2 *__restrict__ K, const double2 *__restrict__ M, const double *__restrict__ Ms,
+                   const int8_t *__restrict__ rport, const double *__restrict__ k0sq,
+                   const double2 *__restrict__ coef, const VT *__restrict__ u, VT *__restrict__ w,
+                   double2 *__restrict__ partial, const SwState *__restrict__ st) {
+  __shared__ VT su[kTileCols * 64];
+  static_assert(kTileCols * 64 * sizeof(VT) >= kTileWaves * 64 * sizeof(double2), "partials reuse su");
+  const int lane = threadIdx.x & 63;
+  const int wv = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);  // wave-uniform (scalar branches)
+  const int64_t ntl = (n + kTileRows - 1) / kTileRows;
+  ChunkWalk cw(ntl);
+  const int nt = (cw.begin < cw.end) ? (int)((cw.end - cw.begin + cw.stride - 1) / cw.stride) : 0;
+  // groups with an active system (the same mask in every wave of the block) and this
+  // lane's active flag per group
+  uint32_t gmask = 0, mine = 0;
+  for (int g = 0; g < groups; ++g) {
+    const int s = g * kSys + lane;
+    const bool a = s < B && st[s].active != 0;
+    mine |= (a ? 1u : 0u) << g;
+    if (__ballot(a) != 0)
+      gmask |= 1u << g;
+  }
+  if (threadIdx.x < 64)
+    for (int g = 0; g < groups; ++g)
+      if (nt == 0 || !((gmask >> g) & 1))
+        partial[((int64_t)g * gridDim.x + blockIdx.x) * 64 + lane] = make_double2(0, 0);
+  const int nsteps = nt * __popc(gmask);
+  if (nsteps == 0)
+    return;
+  auto group_of = [&](int j) {  // the (j / nt)-th active group
+    uint32_t mk = gmask;
+    for (int k = j / nt; k > 0; --k)
+      mk &= mk - 1;
+    return __ffs(mk) - 1;
+  };
+  auto tile_of = [&](int j) { return cw.begin + (int64_t)(j % nt) * cw.stride; };
+  const double *Kd = reinterpret_cast<const double *>(K);
+  const double *Md = reinterpret_cast<const double *>(M);
+  const int32_t pad = (int32_t)row_ptr[n];  // the zero padding entry
+  auto load_meta = [&](TileMeta &m, int j) {
+    const int64_t t = tile_of(j);
+    const int64_t row0 = t * kTileRows + wv * kRowsPerWave;
+    m.uc = ucol[t * kTileCols + min(wv + kTileWaves * lane, kTileCols - 1)];
+    m.rp = row_ptr[min<int64_t>(row0 + min(lane, kRowsPerWave), n)];
+    const int64_t rr = min<int64_t>(row0 + min(lane, kRowsPerWave - 1), n - 1);
+    m.pt = rport[rr];
+    m.dl = dloc[rr];
+    m.nu = nu[t];
+  };
+  // issues the loads of step j (no loaded value is used here).  Every column slot is
+  // loaded (slots past nu hold a valid repeated column), so the loads are branch-free and
+  // no register merge waits for them.
+  using ST = typename std::conditional<std::is_same<VT, float2>::value, float, double>::type;
+  auto load_data = [&](ST (&ux)[kColSlots], ST (&uy)[kColSlots], TileRows &q, const TileMeta &m, int j) {
+    const int g = group_of(j);
+    const int s = g * kSys + lane;
+#pragma unroll
+    for (int i = 0; i < kColSlots; ++i) {
+      const VT t = u[(int64_t)bcast(m.uc, i) * B + s];
+      ux[i] = t.x;
+      uy[i] = t.y;
+    }
+#pragma unroll
+    for (int r = 0; r < kRowsPerWave; ++r) {
+      const int32_t lo = bcast(m.rp, r);
+      const int len = bcast(m.rp, r + 1) - lo;
+      const int32_t k = lane < len ? lo + lane : pad;
+      q.li[r] = lidx[k];
+      if (KC) {
+        const double2 kv = K[k];
+        q.kx[r] = kv.x;
+        q.ky[r] = kv.y;
+      } else {
+        q.kx[r] = Kd[2 * (int64_t)k];
+        q.ky[r] = 0.0;
+      }
+      if (MC) {
+        const double2 mv = M[k];
+        q.mx[r] = mv.x;
+        q.my[r] = mv.y;
+      } else {
+        q.mx[r] = Md[2 * (int64_t)k];
+        q.my[r] = 0.0;
+      }
+      q.lo[r] = lo;
+      q.len[r] = len;
+      q.port[r] = bcast(m.pt, r);
+      q.dl[r] = bcast(m.dl, r);
+    }
+    q.k2 = k0sq[s];
+    q.row0 = tile_of(j) * kTileRows + wv * kRowsPerWave;
+    q.g = g;
+    q.nu = m.nu;
+  };
+  ST ux[kColSlots], uy[kColSlots];
+  double2 v = make_double2(0, 0);
+  double2 pacc[kRowsPerWave];
+  int64_t prow0 = -1;
+  int ps = 0;
+  bool pact = false;
+  // step j: cur holds step j's entries, nxt receives step j+1's; mnext holds step j+1's
+  // metadata, mfree receives step j+2's (the two register sets alternate, no copies)
+  auto step = [&](int j, const TileRows &cur, TileRows &nxt, const TileMeta &mnext, TileMeta &mfree) {
+    __syncthreads();  // the previous step's rows have read su
+#pragma unroll
+    for (int i = 0; i < kColSlots; ++i) {
+      VT t;
+      t.x = ux[i];
+      t.y = uy[i];
+      su[(wv + kTileWaves * i) * 64 + lane] = t;
+    }
+    __syncthreads();
+    if (prow0 >= 0 && pact) {
+#pragma unroll
+      for (int r = 0; r < kRowsPerWave; ++r)
+        if (prow0 + r < n)
+          stv(w + (prow0 + r) * B + ps, pacc[r]);
+    }
+    load_data(ux, uy, nxt, mnext, min(j + 1, nsteps - 1));
+    load_meta(mfree, min(j + 2, nsteps - 1));
+    const int s = cur.g * kSys + lane;
+    const bool act = (mine >> cur.g) & 1;
+#pragma unroll
+    for (int r = 0; r < kRowsPerWave; ++r) {
+      double2 aK = make_double2(0, 0), aM = make_double2(0, 0);
+      const int len = cur.len[r];
+      if (len <= 13)
+        tile_row<13, KC, MC>(su, lane, len, cur, r, aK, aM);
+      else if (len <= 20)
+        tile_row<20, KC, MC>(su, lane, len, cur, r, aK, aM);
+      else
+        tile_row<0, KC, MC>(su, lane, len, cur, r, aK, aM);
+      double2 acc = make_double2(aK.x - cur.k2 * aM.x, aK.y - cur.k2 * aM.y);
+      if (cur.port[r] >= 0) {  // port row: + c_p (Ms x)
+        const double msv = Ms[lane < len ? cur.lo[r] + lane : pad];
+        const double2 c = coef[s * P + cur.port[r]];
+        double2 aS = make_double2(0, 0);
+        for (int t = 0; t < len; ++t) {
+          const double2 x = ldv(su + bcast(cur.li[r], t) * 64 + lane);
+          const double m = bcast(msv, t);
+          aS.x += m * x.x;
+          aS.y += m * x.y;
+        }
+        acc = cfma(c, aS, acc);
+      }
+      pacc[r] = acc;
+      if (act && cur.row0 + r < n)
+        v = cadd(v, bil(ldv(su + cur.dl[r] * 64 + lane), acc));
+    }
+    prow0 = cur.row0;
+    ps = s;
+    pact = act;
+    if (j + 1 == nsteps || group_of(j + 1) != cur.g) {
+      // block partial of the group, fixed order; su is free once every wave is here (the
+      // next step's first barrier orders these reads before su is rewritten)
+      double2 *red = reinterpret_cast<double2 *>(su);
+      __syncthreads();
+      red[wv * 64 + lane] = v;
+      __syncthreads();
+      if (wv == 0) {
+        double2 a = red[lane];
+        for (int ww = 1; ww < kTileWaves; ++ww)
+          a = cadd(a, red[ww * 64 + lane]);
+        partial[((int64_t)cur.g * gridDim.x + blockIdx.x) * 64 + lane] = a;
+      }
+      v = make_double2(0, 0);
+    }
+  };
+  TileRows ra, rb;
+  TileMeta ma, mb;
+  load_meta(ma, 0);
+  load_meta(mb, min(1, nsteps - 1));
+  load_data(ux, uy, ra, ma, 0);
+  for (int j = 0; j < nsteps; j += 2) {
+    step(j, ra, rb, mb, ma);
+    if (j + 1 < nsteps)
+      step(j + 1, rb, ra, ma, mb);
+  }
+  if (pact) {
+#pragma unroll
+    for (int r = 0; r < kRowsPerWave; ++r)
+      if (prow0 + r < n)
+        stv(w + (prow0 + r) * B + ps, pacc[r]);
+  }
+}
+
 // Bit 1: some K value has a nonzero imaginary part; bit 2: some M value has.
 __global__ void k_sw_imag_flags(int64_t nnz, const double2 *__restrict__ K, const double2 *__restrict__ M,
                                 int *__restrict__ flags) {
@@ -1028,31 +1402,34 @@ struct efm_sweep {
   const int32_t *row_ptr = nullptr, *col = nullptr;
   const int32_t *n2e_ptr = nullptr, *n2e = nullptr;
   int gx_vec = 8, gx_nodes = 8, gx_spmv = 8, gy_spmv = 1;
-  // SpMV kernel: 0 = one block slice per system group, 1 = pipelined group-sequential;
-  // -1 = chosen by timing both on the first solve
+  // SpMV kernel: 0 = one block slice per system group, 1 = pipelined group-sequential,
+  // 2 = LDS-tiled (when every row tile fits); -1 = chosen on the first solve (in-place
+  // timing over the iterations before its first convergence check)
   int spmv_variant = -1;
-  int gx_pre = 8;
+  int gx_pre = 8, gx_tile = 8;
+  bool tiled_ok = false;
   // nodal kernel: 0 = one block slice per system group, 1 = pipelined, XCD-local node
-  // walk (groups in turn); -1 = chosen by timing both on the first solve
+  // walk (groups in turn); -1 = chosen on the first solve (in-place timing, as for the SpMV)
   int aux_variant = -1;
   int gx_aux = 8;
   int cplx = 3;  // bit 1: K complex, bit 2: M complex (selects the SpMV instantiation)
   // update grids (one block slice, as many blocks as are resident): [VT c128, c64][INIT, skip, apply]
   int gx_upd[2][3] = {{8, 8, 8}, {8, 8, 8}};
-  float tune_ms[4] = {0.f, 0.f, 0.f, 0.f};  // SpMV 0, 1; nodal 0, 1
+  float tune_ms[5] = {0.f, 0.f, 0.f, 0.f, 0.f};  // SpMV 0, 1, 2; nodal 0, 1
   efm::DevBuf K, M, Ms, rport, dK, dM, dMs, dnK, dnM, dnMs, gsign, en_eff;
   efm::DevBuf x, r, p, sv, u, w, y, part_u, part_s, state, flags, k0sq, coef;
   efm::DevBuf x64;  // refinement: FP64 solution (allocated on first use)
   efm::DevBuf dninv;  // [N][Bmax] inverse nodal diagonals of the systems of one solve
+  efm::DevBuf tucol, tnu, tlidx, tdloc;  // row-tile format of the LDS-tiled SpMV
 };
 
 namespace {
 // Grid of a persistent kernel: as many blocks of kBlock threads as are resident on the
 // device at the kernel's occupancy (a multiple of 8 for the XCD-aware walks), capped by
 // the work (items handled one per wave).
-int resident_grid(const void *kernel, int ncu, int64_t items) {
+int resident_grid(const void *kernel, int ncu, int64_t items, int block = kBlock) {
   int nb = 0;
-  if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&nb, kernel, kBlock, 0) != hipSuccess || nb < 1)
+  if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&nb, kernel, block, 0) != hipSuccess || nb < 1)
     nb = 1;
   const int64_t need = (items + kWaves - 1) / kWaves;
   const int64_t cap = std::max<int64_t>(8, ((int64_t)ncu * nb / 8) * 8);
@@ -1112,6 +1489,8 @@ extern "C" int efm_sweep_create(const efm_topology *topo, const double *K_val, c
     // as are resident
     s->gx_pre = resident_grid(reinterpret_cast<const void *>(&k_sw_spmv_pre<double2, true, true>), ncu, n);
     s->gx_aux = resident_grid(reinterpret_cast<const void *>(&k_sw_aux_pre<double2, true>), ncu, N);
+    s->gx_tile = resident_grid(reinterpret_cast<const void *>(&k_sw_spmv_tile<double2, true, true>), ncu,
+                               ((n + kTileRows - 1) / kTileRows) * kWaves, kTileBlock);
     const void *upd[2][3] = {
         {reinterpret_cast<const void *>(&k_sw_update<double2, true, false>),
          reinterpret_cast<const void *>(&k_sw_update<double2, false, false>),
@@ -1127,7 +1506,12 @@ extern "C" int efm_sweep_create(const efm_topology *topo, const double *K_val, c
   for (int a = 0; a < 2; ++a)
     for (int b = 0; b < 3; ++b)
       gx_upd_max = std::max(gx_upd_max, s->gx_upd[a][b]);
-  if ((rc = s->K.alloc(nnz * 16)) || (rc = s->M.alloc(nnz * 16)) || (rc = s->Ms.alloc(nnz * 8)) ||
+  const int64_t ntl = (n + kTileRows - 1) / kTileRows;
+  // K, M, Ms and the tile entry slots carry one zero entry at index nnz
+  if ((rc = s->K.alloc((nnz + 1) * 16)) || (rc = s->M.alloc((nnz + 1) * 16)) ||
+      (rc = s->Ms.alloc((nnz + 1) * 8)) || (rc = s->tucol.alloc(ntl * kTileCols * sizeof(int32_t))) ||
+      (rc = s->tnu.alloc(ntl * sizeof(int32_t))) || (rc = s->tlidx.alloc(nnz + 1)) ||
+      (rc = s->tdloc.alloc(n)) ||
       (rc = s->rport.alloc(n + 16)) || (rc = s->dK.alloc(n * 16)) || (rc = s->dM.alloc(n * 16)) ||
       (rc = s->dMs.alloc(n * 8)) || (rc = s->dnK.alloc(N * 16)) || (rc = s->dnM.alloc(N * 16)) ||
       (rc = s->dnMs.alloc((size_t)N * 8 * std::max(1, n_ports))) || (rc = s->gsign.alloc(2 * n + 16)) ||
@@ -1136,13 +1520,18 @@ extern "C" int efm_sweep_create(const efm_topology *topo, const double *K_val, c
       (rc = s->y.alloc((size_t)N * s->Bmax * sizeof(double2))) ||
       (rc = s->dninv.alloc((size_t)N * s->Bmax * sizeof(double2))) ||
       (rc = s->part_u.alloc((size_t)groups * gx_upd_max * 2 * 64 * sizeof(double2))) ||
-      (rc = s->part_s.alloc((size_t)groups * std::max(s->gx_spmv, s->gx_pre) * 64 * sizeof(double2))) ||
+      (rc = s->part_s.alloc((size_t)groups * std::max(std::max(s->gx_spmv, s->gx_pre), s->gx_tile) * 64 *
+                            sizeof(double2))) ||
       (rc = s->state.alloc(s->Bmax * sizeof(SwState))) || (rc = s->flags.alloc(s->Bmax * sizeof(int))) ||
       (rc = s->k0sq.alloc(s->Bmax * sizeof(double))) ||
       (rc = s->coef.alloc((size_t)s->Bmax * std::max(1, n_ports) * sizeof(double2))))
     return rc;
   EFM_HIP_CHECK(hipMemcpyAsync(s->K.p, K_val, nnz * 16, hipMemcpyDeviceToDevice, st));
   EFM_HIP_CHECK(hipMemcpyAsync(s->M.p, M_val, nnz * 16, hipMemcpyDeviceToDevice, st));
+  EFM_HIP_CHECK(hipMemsetAsync(s->K.as<double2>() + nnz, 0, 16, st));
+  EFM_HIP_CHECK(hipMemsetAsync(s->M.as<double2>() + nnz, 0, 16, st));
+  EFM_HIP_CHECK(hipMemsetAsync(s->Ms.as<double>() + nnz, 0, 8, st));
+  EFM_HIP_CHECK(hipMemsetAsync(s->tlidx.as<uint8_t>() + nnz, 0, 1, st));
   // port arrays: device pointer table
   DevBuf ptab, err;
   if ((rc = ptab.alloc(sizeof(double *) * kMaxPorts)) || (rc = err.alloc(sizeof(int))))
@@ -1179,10 +1568,17 @@ extern "C" int efm_sweep_create(const efm_topology *topo, const double *K_val, c
                      dim3(256), 0, st, nnz, s->K.as<double2>(), s->M.as<double2>(), cflag.as<int>());
   EFM_LAUNCH_CHECK();
   EFM_HIP_CHECK(hipMemcpyAsync(&s->cplx, cflag.p, sizeof(int), hipMemcpyDeviceToHost, st));
+  // row tiles of the LDS-tiled SpMV; usable when every tile fits
+  hipLaunchKernelGGL(k_sw_build_tiles, dim3((unsigned)ntl), dim3(kBlock), 0, st, n, s->row_ptr, s->col,
+                     s->tucol.as<int32_t>(), s->tnu.as<int32_t>(), s->tlidx.as<uint8_t>(), s->tdloc.as<uint8_t>());
+  EFM_LAUNCH_CHECK();
+  std::vector<int32_t> h_nu(ntl);
+  EFM_HIP_CHECK(hipMemcpyAsync(h_nu.data(), s->tnu.p, ntl * sizeof(int32_t), hipMemcpyDeviceToHost, st));
   int h_err = 0;
   EFM_HIP_CHECK(hipMemcpyAsync(&h_err, err.p, sizeof(int), hipMemcpyDeviceToHost, st));
   EFM_HIP_CHECK(hipStreamSynchronize(st));
   EFM_REQUIRE(h_err == 0, "efm_sweep_create: two ports share an edge row (not supported)");
+  s->tiled_ok = std::all_of(h_nu.begin(), h_nu.end(), [](int32_t v) { return v >= 0; });
   *out = s.release();
   return EFM_OK;
 }
@@ -1191,8 +1587,10 @@ extern "C" void efm_sweep_destroy(efm_sweep *s) { delete s; }
 
 extern "C" int efm_sweep_set_variants(efm_sweep *s, int spmv, int aux) {
   EFM_REQUIRE(s, "efm_sweep_set_variants: NULL sweep");
-  EFM_REQUIRE(spmv >= -1 && spmv <= 1 && aux >= -1 && aux <= 1,
-              "efm_sweep_set_variants: variants must be -1, 0 or 1");
+  EFM_REQUIRE(spmv >= -1 && spmv <= 2 && aux >= -1 && aux <= 1,
+              "efm_sweep_set_variants: SpMV variant must be -1..2, nodal variant -1..1");
+  EFM_REQUIRE(spmv != 2 || s->tiled_ok,
+              "efm_sweep_set_variants: the LDS-tiled SpMV (2) does not fit this matrix's row tiles");
   s->spmv_variant = spmv;
   s->aux_variant = aux;
   return EFM_OK;
@@ -1203,12 +1601,62 @@ extern "C" int efm_sweep_get_variants(const efm_sweep *s, int *h_spmv, int *h_au
   *h_spmv = s->spmv_variant;
   *h_aux = s->aux_variant;
   if (h_tune_ms)
-    for (int v = 0; v < 4; ++v)
+    for (int v = 0; v < 5; ++v)
       h_tune_ms[v] = s->tune_ms[v];
   return EFM_OK;
 }
 
 namespace {
+
+// Event pairs around the launches of kernel variants; pick() keeps the variant with the
+// smallest mean time (stream synchronized before) and writes the means (0 for a
+// variant without launches).
+struct TuneLog {
+  std::vector<hipEvent_t> e0, e1;
+  std::vector<int> var;
+  void add(hipEvent_t a, hipEvent_t b, int v) {
+    e0.push_back(a);
+    e1.push_back(b);
+    var.push_back(v);
+  }
+  bool empty() const { return var.empty(); }
+  int pick(int *variant, float *ms_out, int nv) {  // variants 0 .. nv-1 (nv <= 4)
+    float sum[4] = {0.f, 0.f, 0.f, 0.f};
+    int cnt[4] = {0, 0, 0, 0};
+    int rc = EFM_OK;
+    for (size_t i = 0; i < var.size(); ++i) {
+      float ms = 0.f;
+      if (hipEventElapsedTime(&ms, e0[i], e1[i]) != hipSuccess) {
+        rc = EFM_ERR_HIP;
+        break;
+      }
+      sum[var[i]] += ms;
+      cnt[var[i]] += 1;
+    }
+    if (rc == EFM_OK) {
+      int best = -1;
+      for (int v = 0; v < nv; ++v) {
+        ms_out[v] = cnt[v] ? sum[v] / cnt[v] : 0.f;
+        if (cnt[v] && (best < 0 || sum[v] / cnt[v] < sum[best] / cnt[best]))
+          best = v;
+      }
+      if (best >= 0)
+        *variant = best;
+    }
+    release();
+    return rc;
+  }
+  void release() {
+    for (size_t i = 0; i < var.size(); ++i) {
+      (void)hipEventDestroy(e0[i]);
+      (void)hipEventDestroy(e1[i]);
+    }
+    e0.clear();
+    e1.clear();
+    var.clear();
+  }
+  ~TuneLog() { release(); }
+};
 
 // One launch sequence over the padded system block [Bp]: everything a solve needs.
 struct SwRun {
@@ -1268,51 +1716,6 @@ struct SwRun {
       EFM_LAUNCH_CHECK();
       return EFM_OK;
     };
-    // times `launch(v)` for v = 0, 1 (3 launches each after one untimed), keeps the faster
-    // in *variant and its times in ms[0..1]; the launches must be repeatable
-    auto time_pick = [&](int *variant, float *ms_out, auto &&launch) -> int {
-      hipEvent_t e0, e1;
-      EFM_HIP_CHECK(hipEventCreate(&e0));
-      EFM_HIP_CHECK(hipEventCreate(&e1));
-      int rc2 = EFM_OK;
-      float best = 0.f;
-      int pick = 0;
-      for (int v = 0; v < 2 && rc2 == EFM_OK; ++v) {
-        if ((rc2 = launch(v)))
-          break;
-        if (hipEventRecord(e0, st) != hipSuccess) {
-          rc2 = EFM_ERR_HIP;
-          break;
-        }
-        for (int k = 0; k < 3 && rc2 == EFM_OK; ++k)
-          rc2 = launch(v);
-        float ms = 0.f;
-        if (rc2 || hipEventRecord(e1, st) != hipSuccess || hipEventSynchronize(e1) != hipSuccess ||
-            hipEventElapsedTime(&ms, e0, e1) != hipSuccess) {
-          rc2 = rc2 ? rc2 : EFM_ERR_HIP;
-          break;
-        }
-        ms_out[v] = ms / 3.f;
-        if (v == 0 || ms / 3.f < best) {
-          best = ms / 3.f;
-          pick = v;
-        }
-      }
-      (void)hipEventDestroy(e0);
-      (void)hipEventDestroy(e1);
-      if (rc2 == EFM_OK)
-        *variant = pick;
-      return rc2;
-    };
-    auto aux_node = [&](bool fused) -> int {
-      if (fused && s->aux_variant < 0) {
-        // y is rewritten by every launch; the chosen kernel runs last
-        int rc2 = time_pick(&s->aux_variant, s->tune_ms + 2, [&](int v) { return aux_v(true, v); });
-        if (rc2)
-          return rc2;
-      }
-      return aux_v(fused, s->aux_variant == 1 ? 1 : 0);
-    };
     constexpr int vti = std::is_same<VT, float2>::value ? 1 : 0;
     auto update = [&](bool first, bool apply) -> int {
       const double2 *dK = s->dK.as<double2>(), *dM = s->dM.as<double2>();
@@ -1334,7 +1737,31 @@ struct SwRun {
       return EFM_OK;
     };
     auto spmv_v = [&](int variant) -> int {
-      if (variant == 1) {
+      if (variant == 2) {
+        const dim3 grid(s->gx_tile, 1);
+        const double2 *Kv = s->K.as<double2>(), *Mv = s->M.as<double2>(), *cf = s->coef.as<double2>();
+        const double *Msv = s->Ms.as<double>(), *k2 = s->k0sq.as<double>();
+        const int8_t *rp = s->rport.as<int8_t>();
+        const int32_t *uc = s->tucol.as<int32_t>(), *tn = s->tnu.as<int32_t>();
+        const uint8_t *li = s->tlidx.as<uint8_t>(), *dl = s->tdloc.as<uint8_t>();
+        switch (s->cplx & 3) {
+        case 0:
+          hipLaunchKernelGGL((k_sw_spmv_tile<VT, false, false>), grid, dim3(kTileBlock), 0, st, n, Bp, P, groups,
+                             s->row_ptr, uc, tn, li, dl, Kv, Mv, Msv, rp, k2, cf, u, w, ps, S);
+          break;
+        case 1:
+          hipLaunchKernelGGL((k_sw_spmv_tile<VT, true, false>), grid, dim3(kTileBlock), 0, st, n, Bp, P, groups,
+                             s->row_ptr, uc, tn, li, dl, Kv, Mv, Msv, rp, k2, cf, u, w, ps, S);
+          break;
+        case 2:
+          hipLaunchKernelGGL((k_sw_spmv_tile<VT, false, true>), grid, dim3(kTileBlock), 0, st, n, Bp, P, groups,
+                             s->row_ptr, uc, tn, li, dl, Kv, Mv, Msv, rp, k2, cf, u, w, ps, S);
+          break;
+        default:
+          hipLaunchKernelGGL((k_sw_spmv_tile<VT, true, true>), grid, dim3(kTileBlock), 0, st, n, Bp, P, groups,
+                             s->row_ptr, uc, tn, li, dl, Kv, Mv, Msv, rp, k2, cf, u, w, ps, S);
+        }
+      } else if (variant == 1) {
         const dim3 grid(s->gx_pre, 1);
         const double2 *Kv = s->K.as<double2>(), *Mv = s->M.as<double2>(), *cf = s->coef.as<double2>();
         const double *Msv = s->Ms.as<double>(), *k2 = s->k0sq.as<double>();
@@ -1363,18 +1790,10 @@ struct SwRun {
       EFM_LAUNCH_CHECK();
       return EFM_OK;
     };
-    auto spmv = [&]() -> int { return spmv_v(s->spmv_variant == 1 ? 1 : 0); };
-    // autotune: time both SpMV kernels on the real systems (w and the partials are
-    // rewritten by every launch, so the solve is unaffected); the chosen kernel runs last
-    auto tune = [&]() -> int {
-      if (s->spmv_variant >= 0)
-        return EFM_OK;
-      int rc2 = time_pick(&s->spmv_variant, s->tune_ms, [&](int v) { return spmv_v(v); });
-      return rc2 ? rc2 : spmv();
-    };
-    auto finalize = [&](bool first, bool skip) -> int {
-      const int gxs = s->spmv_variant == 1 ? s->gx_pre : s->gx_spmv;           // the SpMV grid that wrote ps
-      const int gxu = s->gx_upd[vti][first ? 0 : (skip ? 1 : 2)];             // the update grid that wrote pu
+    auto finalize = [&](bool first, bool skip, int spmv_variant) -> int {
+      // the grids that wrote the partials: the SpMV variant's and the update's
+      const int gxs = spmv_variant == 2 ? s->gx_tile : spmv_variant == 1 ? s->gx_pre : s->gx_spmv;
+      const int gxu = s->gx_upd[vti][first ? 0 : (skip ? 1 : 2)];
       if (first)
         hipLaunchKernelGGL((k_sw_finalize<true>), dim3(Bp), dim3(kBlock), 0, st, Bp, gxu, gxs, pu, ps, S, flags,
                            inner_tol2, max_it, max_inner, 0);
@@ -1384,9 +1803,37 @@ struct SwRun {
       EFM_LAUNCH_CHECK();
       return EFM_OK;
     };
+    // Autotuning in place: while a kernel variant is unset (-1), the iterations of the
+    // first convergence-check chunk cycle through its candidates (SpMV it % nvs, nodal
+    // it % 2), each launch timed with events where it runs in the iteration (so with the
+    // cache state the iteration leaves); at the chunk's check the fastest mean is kept and
+    // the means are reported in tune_ms.  Every candidate computes the same w (or y), so
+    // the iterations are those of the chosen kernel up to rounding.  Setup launches use
+    // variant 1.
+    const int nvs = s->tiled_ok ? 3 : 2;
+    TuneLog log_s, log_a;
+    auto vs_of = [&](int it) { return s->spmv_variant >= 0 ? s->spmv_variant : (it >= 0 ? it % nvs : 1); };
+    auto va_of = [&](int it) { return s->aux_variant >= 0 ? s->aux_variant : (it >= 0 ? it % 2 : 1); };
+    auto run = [&](TuneLog &lg, bool timed, int v, auto &&launch) -> int {
+      if (!timed)
+        return launch();
+      hipEvent_t e0, e1;
+      EFM_HIP_CHECK(hipEventCreate(&e0));
+      if (hipEventCreate(&e1) != hipSuccess) {
+        (void)hipEventDestroy(e0);
+        return EFM_ERR_HIP;
+      }
+      lg.add(e0, e1, v);
+      EFM_HIP_CHECK(hipEventRecord(e0, st));
+      int rc2 = launch();
+      if (rc2)
+        return rc2;
+      EFM_HIP_CHECK(hipEventRecord(e1, st));
+      return EFM_OK;
+    };
     int rc, active = 0;
-    if ((rc = aux_node(false)) || (rc = update(true, false)) || (rc = spmv()) || (rc = tune()) ||
-        (rc = finalize(true, false)) || (rc = n_active(active)))
+    if ((rc = aux_v(false, va_of(-1))) || (rc = update(true, false)) || (rc = spmv_v(vs_of(-1))) ||
+        (rc = finalize(true, false, vs_of(-1))) || (rc = n_active(active)))
       return rc;
     int it = 0;
     const int cap = std::min(max_it, max_inner);
@@ -1394,10 +1841,18 @@ struct SwRun {
       const int chunk = std::min(check, cap - it);
       for (int c = 0; c < chunk; ++c, ++it) {
         const bool apply = (it & 1) != 0;  // x and r are stored on odd iterations
-        if ((rc = aux_node(true)) || (rc = update(false, apply)) || (rc = spmv()) || (rc = finalize(false, !apply)))
+        const int va = va_of(it), vs = vs_of(it);
+        if ((rc = run(log_a, s->aux_variant < 0, va, [&]() { return aux_v(true, va); })) ||
+            (rc = update(false, apply)) ||
+            (rc = run(log_s, s->spmv_variant < 0, vs, [&]() { return spmv_v(vs); })) ||
+            (rc = finalize(false, !apply, vs)))
           return rc;
       }
-      if ((rc = n_active(active)))
+      if ((rc = n_active(active)))  // synchronizes the stream: the events are complete
+        return rc;
+      if (s->spmv_variant < 0 && !log_s.empty() && (rc = log_s.pick(&s->spmv_variant, s->tune_ms, nvs)))
+        return rc;
+      if (s->aux_variant < 0 && !log_a.empty() && (rc = log_a.pick(&s->aux_variant, s->tune_ms + 3, 2)))
         return rc;
     }
     hipLaunchKernelGGL((k_sw_flush<VT>), dim3(s->gx_vec, groups), dim3(kBlock), 0, st, n, Bp, x, (const VT *)p, S);

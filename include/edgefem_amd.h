@@ -256,11 +256,15 @@ int efm_sweep_solve(efm_sweep *sw, int nsys, const double *h_k0sq, const double 
 /* Kernel variants of the sweep, for the SpMV and for the nodal part of the
  * preconditioner: 0 = one block slice per group of 64 systems; 1 = pipelined (bounds of
  * the next rows/nodes preloaded per wave, fixed-count gathers, groups walked in turn so
- * that each XCD's share of the gathered vectors stays in its L2); -1 (default) = the
- * faster of the two, timed on the first solve.  get: the variants in use (-1 before they
- * are chosen) and, when h_tune_ms is given, the tuning times in ms per launch
- * [SpMV 0, SpMV 1, nodal 0, nodal 1].  Autotuner of this project; no reference
- * counterpart. */
+ * that each XCD's share of the gathered vectors stays in its L2); 2 (SpMV only) =
+ * LDS-tiled: each tile of 8 consecutive rows loads its distinct columns (at most 72)
+ * once per group of 64 systems into LDS and its rows read them there; available when
+ * every tile fits (set returns an error otherwise); -1 (default) = the fastest, chosen on
+ * the first solve: the iterations before its first convergence check cycle through the
+ * candidates, each launch timed in place.  get: the variants in use (-1 before they are
+ * chosen) and, when h_tune_ms is given (5 entries), the mean times in ms per launch
+ * [SpMV 0, 1, 2, nodal 0, 1] (0 for a variant not timed).  Autotuner of this project; no
+ * reference counterpart. */
 int efm_sweep_set_variants(efm_sweep *sw, int spmv, int aux);
 int efm_sweep_get_variants(const efm_sweep *sw, int *h_spmv, int *h_aux, double *h_tune_ms);
 

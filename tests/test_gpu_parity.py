@@ -427,13 +427,15 @@ def test_full_size_c2_topology_properties(torch_dev):
 
 
 @pytest.mark.parametrize("precision,max_batch,variant", [("fp64", 256, -1), ("fp64", 256, 0), ("fp64", 256, 1),
-                                                        ("fp64", 100, 1), ("fp64-refined", 256, -1)])
+                                                        ("fp64", 256, 2), ("fp64", 100, 1), ("fp64", 100, 2),
+                                                        ("fp64-refined", 256, -1)])
 def test_batched_sweep_matches_lu_golden(torch_dev, precision, max_batch, variant):
     """128 frequencies x 2 excitations = 256 systems on the 16x8x32 guide, batched, against
     the FP64 SuperLU golden of every point: plain FP64 COCG with each kernel variant of the
     SpMV and the nodal preconditioner part (0: one block slice per system group, 1:
-    pipelined, groups in turn, -1: autotuned), also with batches that are not a multiple
-    of the 64 systems of a wavefront (100, 100, 56), and the iterative refinement loop
+    pipelined, groups in turn, 2: LDS-tiled SpMV with the pipelined nodal kernel, -1:
+    autotuned), also with batches that are not a multiple of the 64 systems of a
+    wavefront (100, 100, 56), and the iterative refinement loop
     (true FP64 residual b - A x per outer step) around FP64 inner solves.  Complex-FP32
     inner vectors are not tested here: on these indefinite systems their COCG iterations
     stall near a 1e-4..1e-3 relative residual for some frequencies."""
@@ -442,7 +444,7 @@ def test_batched_sweep_matches_lu_golden(torch_dev, precision, max_batch, varian
     pm, bc, ports = _c1_em_setup()
     gold = np.load(golden("c3_sweep.npz"))
     params = em.MaxwellParams()
-    st = {"spmv_variant": variant, "aux_variant": variant}
+    st = {"spmv_variant": variant, "aux_variant": min(variant, 1)}
     opts = em.SolveOptions()
     opts.tolerance = tol = 1e-10
     S = em.frequency_sweep_eigenmode(pm, params, bc, ports, gold["freqs"], precision=precision,
@@ -455,18 +457,18 @@ def test_batched_sweep_matches_lu_golden(torch_dev, precision, max_batch, varian
     assert np.all(np.isfinite(S))
     assert res.max() <= tol
     assert err <= 1e-6
-    assert st["spmv_chosen"] in ((0, 1) if variant < 0 else (variant,))
-    assert st["aux_chosen"] in ((0, 1) if variant < 0 else (variant,))
+    assert st["spmv_chosen"] in ((0, 1, 2) if variant < 0 else (variant,))
+    assert st["aux_chosen"] in ((0, 1) if variant < 0 else (min(variant, 1),))
 
 
 def test_batched_sweep_lossy_complex_mass_vs_oracle(torch_dev):
     """Batched sweep with a lossy permittivity (complex M, so the SpMV takes its complex-M
-    instantiation) on a 6x3x12 WR-90 section: both kernel variants against the oracle
+    instantiation) on a 6x3x12 WR-90 section: the three SpMV variants against the oracle
     (SuperLU) at three frequencies."""
     from edgefem_amd import meshgen as mg
     from edgefem_amd import pyedgefem as em
     from oracle import edgefem_oracle as O
-    _p("[gpu] batched sweep, lossy fill (complex M), variants 0 and 1")
+    _p("[gpu] batched sweep, lossy fill (complex M), SpMV variants 0, 1, 2")
     L = 30e-3
     g = mg.box_mesh(mg.WR90_A, mg.WR90_B, L, 6, 3, 12, seed=7)
     pm = em.mesh_from_arrays(g.node_ids, g.xyz, g.tets, g.tet_phys, g.tris, g.tri_phys)
@@ -487,12 +489,12 @@ def test_batched_sweep_lossy_complex_mass_vs_oracle(torch_dev):
     oports = [O.build_wave_port_2d(om, t, kc, Z0, beta, opec, kc2) for t in (2, 3)]
     Sref = np.array([O.sparams_eigenmode(om, O.Params(omega=2 * math.pi * f, eps_tet=eps), opec, oports)
                      for f in freqs])
-    for variant in (0, 1):
-        st = {"spmv_variant": variant, "aux_variant": variant}
+    for variant in (0, 1, 2):
+        st = {"spmv_variant": variant, "aux_variant": min(variant, 1)}
         S = em.frequency_sweep_eigenmode(pm, params, bc, ports, freqs, _state=st)
         err = np.abs(S - Sref).max()
         _p(f"    variant {variant}: max |dS| {err:.2e}; |S21| {np.abs(S[:, 1, 0]).round(6)}")
-        assert st["spmv_chosen"] == variant and st["aux_chosen"] == variant
+        assert st["spmv_chosen"] == variant and st["aux_chosen"] == min(variant, 1)
         assert err <= 1e-6
         assert np.all(np.abs(S[:, 1, 0]) < 1.0)   # lossy
 
