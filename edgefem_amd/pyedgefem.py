@@ -129,10 +129,18 @@ class Mesh:
     # reference-style accessors ------------------------------------------------
 
     def _node_pos(self):
-        """Map from gmsh node IDs to positions in the node arrays."""
+        """Map from gmsh node IDs to positions in the node arrays (a subtraction when the
+        IDs are consecutive, as in generated meshes; a sorted search otherwise)."""
         if self.__dict__.get("_npos") is None:
-            order = np.argsort(self.node_ids, kind="stable")
-            self._npos = (order, self.node_ids[order])
+            ids = np.asarray(self.node_ids, np.int64)
+            if ids.size and ids[-1] - ids[0] == ids.size - 1 and np.all(np.diff(ids) == 1):
+                self._npos = int(ids[0])
+            else:
+                order = np.argsort(ids, kind="stable")
+                self._npos = (order, ids[order])
+        if isinstance(self._npos, int):
+            base = self._npos
+            return lambda ids: np.asarray(ids, np.int64) - base
         order, sid = self._npos
         return lambda ids: order[np.searchsorted(sid, np.asarray(ids, np.int64))]
     def num_nodes(self):
@@ -1222,17 +1230,22 @@ def _periodic_gradient(mesh: Mesh, pbc: PeriodicBC, dmask: np.ndarray, zero: np.
         mn = np.unique(nid(mesh.edge_nodes[me].ravel()))
         sn = np.unique(nid(mesh.edge_nodes[se].ravel()))
         tol = 1e-9
-        table = {tuple(k): i for i, k in zip(sn, np.round(mesh.xyz[sn] / tol).astype(np.int64))}
-        shifted = np.round((mesh.xyz[mn] + pbc.period_vector) / tol).astype(np.int64)
-        for i, k in zip(mn, shifted):
-            j = table.get(tuple(k), -1)
-            if j < 0:
-                d = np.linalg.norm(mesh.xyz[sn] - (mesh.xyz[i] + pbc.period_vector), axis=1)
-                hit = np.nonzero(d < tol)[0]
-                j = int(sn[hit[0]]) if hit.size else -1
-            if j >= 0:
-                nmap[j] = i
-                ncoef[j] = node_phase
+        # exact matches of the rounded coordinates (the slave node per key: the last one
+        # listed), then a distance search for the master nodes left over
+        ks = np.round(mesh.xyz[sn] / tol).astype(np.int64)
+        km = np.round((mesh.xyz[mn] + pbc.period_vector) / tol).astype(np.int64)
+        _, inv = np.unique(np.concatenate([ks, km]), axis=0, return_inverse=True)
+        inv = inv.reshape(-1)
+        slave_of = np.full(int(inv.max()) + 1 if inv.size else 0, -1, np.int64)
+        slave_of[inv[:sn.size]] = sn
+        match = slave_of[inv[sn.size:]]
+        for t in np.nonzero(match < 0)[0]:
+            d = np.linalg.norm(mesh.xyz[sn] - (mesh.xyz[mn[t]] + pbc.period_vector), axis=1)
+            hit = np.nonzero(d < tol)[0]
+            match[t] = int(sn[hit[0]]) if hit.size else -1
+        ok = match >= 0
+        nmap[match[ok]] = mn[ok]
+        ncoef[match[ok]] = node_phase
     nid = mesh._node_pos()
     a = nid(mesh.edge_nodes[:, 0])
     b = nid(mesh.edge_nodes[:, 1])
