@@ -1552,9 +1552,26 @@ def calculate_sparams_periodic_batch(cells, *, mode: str = "reference",
     row_off, node_off, port_off = [0], [0], [0]
     rps, cols, vals, ens, ecs, pidx, pw, pc, Bs, meta = [], [], [], [], [], [], [], [], [], []
     nnz_off = 0
-    for mesh, params, bc, pbc, ports in cells:
-        params._check_supported()
-        A, B, (en, ec, N), frees, rec = _periodic_cell_system(mesh, params, bc, pbc, ports, mode)
+    # The cells' systems are built by a few host threads: each cell's GPU calls (assembly,
+    # periodic reduction) wait on the device with the GIL released, so another cell's host
+    # work runs meanwhile.  Every thread enqueues on this thread's device and stream, and the
+    # results are kept in cell order.
+    for c in cells:
+        c[1]._check_supported()
+    dev, stream = torch.cuda.current_device(), torch.cuda.current_stream()
+
+    def build(c):
+        with torch.cuda.device(dev), torch.cuda.stream(stream):
+            return _periodic_cell_system(c[0], c[1], c[2], c[3], c[4], mode)
+    workers = min(4, len(cells))
+    if workers > 1:
+        from concurrent.futures import ThreadPoolExecutor
+        with ThreadPoolExecutor(workers) as ex:
+            systems = list(ex.map(build, cells))
+    else:
+        systems = [build(c) for c in cells]
+    for (mesh, params, bc, pbc, ports), sysc in zip(cells, systems):
+        A, B, (en, ec, N), frees, rec = sysc
         r0, q0 = row_off[-1], node_off[-1]
         nnz_c = int(A.col_idx.numel())
         rps.append(A.row_ptr[:-1] + nnz_off)
