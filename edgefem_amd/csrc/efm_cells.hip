@@ -661,6 +661,86 @@ __global__ void k_cells_dn(int64_t N, const int32_t *__restrict__ gt_ptr, const 
   dninv[nd] = (sum.x == 0.0 && sum.y == 0.0) ? make_double2(0, 0) : cdiv(make_double2(1, 0), sum);
 }
 
+// Transposed gradient incidence, built on the device: per node its incident edges in
+// ascending edge order (the order a sequential fill gives, so every sum over a node's
+// list is reproducible), with the coefficient of that node's end (ga + gb when both ends
+// of the edge fold onto the node).  Counts by integer atomics, offsets by a scan, a fill
+// in any order, then a sort of each node's short list.
+__global__ void k_gt_count(int64_t n, int64_t N, const int2 *__restrict__ en2, int32_t *__restrict__ cnt,
+                           int *__restrict__ err) {
+  for (int64_t e = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; e < n; e += (int64_t)gridDim.x * blockDim.x) {
+    const int2 nn = en2[e];
+    if (nn.x < -1 || nn.x >= N || nn.y < -1 || nn.y >= N) {
+      *err = 1;
+      continue;
+    }
+    if (nn.x >= 0)
+      atomicAdd(cnt + nn.x, 1);
+    if (nn.y >= 0 && nn.y != nn.x)
+      atomicAdd(cnt + nn.y, 1);
+  }
+}
+
+__global__ void k_gt_fill(int64_t n, const int2 *__restrict__ en2, const double2 *__restrict__ ga,
+                          const double2 *__restrict__ gb, const int32_t *__restrict__ ptr,
+                          int32_t *__restrict__ cursor, int32_t *__restrict__ gte, double2 *__restrict__ gtc) {
+  for (int64_t e = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; e < n; e += (int64_t)gridDim.x * blockDim.x) {
+    const int2 nn = en2[e];
+    if (nn.x >= 0) {
+      const int32_t q = ptr[nn.x] + atomicAdd(cursor + nn.x, 1);
+      gte[q] = (int32_t)e;
+      gtc[q] = (nn.y == nn.x) ? cadd(ga[e], gb[e]) : ga[e];
+    }
+    if (nn.y >= 0 && nn.y != nn.x) {
+      const int32_t q = ptr[nn.y] + atomicAdd(cursor + nn.y, 1);
+      gte[q] = (int32_t)e;
+      gtc[q] = gb[e];
+    }
+  }
+}
+
+// insertion sort of each node's list by edge (an edge appears once per list)
+__global__ void k_gt_sort(int64_t N, const int32_t *__restrict__ ptr, int32_t *__restrict__ gte,
+                          double2 *__restrict__ gtc) {
+  for (int64_t nd = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; nd < N; nd += (int64_t)gridDim.x * blockDim.x) {
+    const int32_t lo = ptr[nd], hi = ptr[nd + 1];
+    for (int32_t i = lo + 1; i < hi; ++i) {
+      const int32_t key = gte[i];
+      const double2 c = gtc[i];
+      int32_t j = i - 1;
+      while (j >= lo && gte[j] > key) {
+        gte[j + 1] = gte[j];
+        gtc[j + 1] = gtc[j];
+        --j;
+      }
+      gte[j + 1] = key;
+      gtc[j + 1] = c;
+    }
+  }
+}
+
+// ga[e], gb[e] from the interleaved per-edge coefficients (ga, gb)
+__global__ void k_split_pairs(int64_t n, const double2 *__restrict__ ab, double2 *__restrict__ ga,
+                              double2 *__restrict__ gb) {
+  for (int64_t e = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; e < n; e += (int64_t)gridDim.x * blockDim.x) {
+    ga[e] = ab[2 * e];
+    gb[e] = ab[2 * e + 1];
+  }
+}
+
+// Jacobi diagonal of the port loading: extra[r] = c_q |w_r|^2 on the rows of term q (one
+// block per term; the terms' rows are disjoint, checked on the host)
+__global__ void k_port_extra(const int64_t *__restrict__ poff, const int32_t *__restrict__ pidx,
+                             const double2 *__restrict__ pw, const double2 *__restrict__ pcoef,
+                             double2 *__restrict__ extra) {
+  const int64_t q = blockIdx.x;
+  const double2 c = pcoef[q];
+  for (int64_t i = poff[q] + threadIdx.x; i < poff[q + 1]; i += blockDim.x) {
+    const double2 w = pw[i];
+    extra[pidx[i]] = cscale(c, w.x * w.x + w.y * w.y);
+  }
+}
+
 // every column of a row must lie inside the row's cell; err = 1 otherwise
 __global__ void k_cells_check(const Tile *__restrict__ tiles, const int64_t *__restrict__ row_off,
                               const int32_t *__restrict__ row_ptr, const int32_t *__restrict__ col,
@@ -739,53 +819,21 @@ extern "C" int efm_cells_create(int64_t ncells, const int64_t *h_row_off, const 
   for (int64_t c = 0; c < ncells; ++c)
     EFM_REQUIRE(h_row_off[c + 1] >= h_row_off[c] && h_node_off[c + 1] >= h_node_off[c],
                 "efm_cells_create: offsets must be nondecreasing");
-  // gradient incidence (global node indices) and its transpose
-  std::vector<int2> en2(n);
-  std::vector<double2> ga(n), gb(n);
-  std::vector<int32_t> cnt(N + 1, 0);
-  for (int64_t e = 0; e < n; ++e) {
-    const int32_t a = h_edge_nodes[2 * e], b = h_edge_nodes[2 * e + 1];
-    EFM_REQUIRE(a >= -1 && a < N && b >= -1 && b < N, "efm_cells_create: node index out of range");
-    en2[e] = make_int2(a, b);
-    ga[e] = make_double2(h_edge_coefs[4 * e], h_edge_coefs[4 * e + 1]);
-    gb[e] = make_double2(h_edge_coefs[4 * e + 2], h_edge_coefs[4 * e + 3]);
-    if (a >= 0)
-      ++cnt[a + 1];
-    if (b >= 0 && b != a)
-      ++cnt[b + 1];
-  }
-  for (int64_t k = 0; k < N; ++k)
-    cnt[k + 1] += cnt[k];
-  std::vector<int32_t> gte(std::max<int32_t>(cnt[N], 1));
-  std::vector<double2> gtc(std::max<int32_t>(cnt[N], 1));
-  std::vector<int32_t> fill(cnt.begin(), cnt.end() - 1);
-  for (int64_t e = 0; e < n; ++e) {
-    const int2 nn = en2[e];
-    if (nn.x >= 0) {
-      gte[fill[nn.x]] = (int32_t)e;
-      gtc[fill[nn.x]++] = (nn.y == nn.x) ? cadd(ga[e], gb[e]) : ga[e];
-    }
-    if (nn.y >= 0 && nn.y != nn.x) {
-      gte[fill[nn.y]] = (int32_t)e;
-      gtc[fill[nn.y]++] = gb[e];
-    }
-  }
-  // port terms: q = c * npt + p, entries [poff[q], poff[q+1]); disjoint rows
+  // port terms: q = c * npt + p, entries [poff[q], poff[q+1]); rows inside the term's cell
+  // and disjoint (checked on a sorted copy of the port rows)
   const int64_t npe = h_port_off[s->nterms];
-  std::vector<double2> extra(n, make_double2(0, 0));
   std::vector<int32_t> pcell(s->nterms);
-  std::vector<uint8_t> seen(n, 0);
-  for (int64_t q = 0; q < s->nterms; ++q) {
-    pcell[q] = (int32_t)(q / s->npt);
-    const double2 cq = make_double2(h_port_coef[2 * q], h_port_coef[2 * q + 1]);
-    for (int64_t i = h_port_off[q]; i < h_port_off[q + 1]; ++i) {
-      const int32_t r = h_port_idx[i];
-      EFM_REQUIRE(r >= h_row_off[pcell[q]] && r < h_row_off[pcell[q] + 1], "efm_cells_create: port edge outside its cell");
-      EFM_REQUIRE(!seen[r], "efm_cells_create: port terms must be disjoint");
-      seen[r] = 1;
-      const double2 w = make_double2(h_port_w[2 * i], h_port_w[2 * i + 1]);
-      extra[r] = cscale(cq, w.x * w.x + w.y * w.y);
+  {
+    std::vector<int32_t> rows(h_port_idx, h_port_idx + npe);
+    for (int64_t q = 0; q < s->nterms; ++q) {
+      pcell[q] = (int32_t)(q / s->npt);
+      for (int64_t i = h_port_off[q]; i < h_port_off[q + 1]; ++i)
+        EFM_REQUIRE(h_port_idx[i] >= h_row_off[pcell[q]] && h_port_idx[i] < h_row_off[pcell[q] + 1],
+                    "efm_cells_create: port edge outside its cell");
     }
+    std::sort(rows.begin(), rows.end());
+    EFM_REQUIRE(std::adjacent_find(rows.begin(), rows.end()) == rows.end(),
+                "efm_cells_create: port terms must be disjoint");
   }
   std::vector<int32_t> first;
   std::vector<Tile> tiles = make_tiles(h_row_off, ncells, &first);
@@ -806,11 +854,52 @@ extern "C" int efm_cells_create(int64_t ncells, const int64_t *h_row_off, const 
     EFM_HIP_CHECK(hipStreamSynchronize(st));
     EFM_REQUIRE(last == nnz, "efm_cells_create: row_ptr[n] != nnz");
   }
-  if ((rc = upload(s->en2, en2.data(), n, st)) ||
-      (rc = upload(s->ga, ga.data(), n, st)) || (rc = upload(s->gb, gb.data(), n, st)) ||
-      (rc = upload(s->gt_ptr, cnt.data(), N + 1, st)) || (rc = upload(s->gt_edge, gte.data(), gte.size(), st)) ||
-      (rc = upload(s->gt_coef, gtc.data(), gtc.size(), st)) ||
-      (rc = upload(s->tiles, tiles.data(), tiles.size(), st)) ||
+  // gradient incidence (global node indices) and its transpose, on the device
+  {
+    DevBuf ab, cnt, cursor, work, err;
+    if ((rc = upload(s->en2, (const int2 *)h_edge_nodes, n, st)) ||
+        (rc = upload(ab, (const double2 *)h_edge_coefs, 2 * n, st)) ||
+        (rc = s->ga.alloc(std::max<int64_t>(n, 1) * sizeof(double2))) ||
+        (rc = s->gb.alloc(std::max<int64_t>(n, 1) * sizeof(double2))) ||
+        (rc = cnt.alloc((N + 1) * sizeof(int32_t))) || (rc = cursor.alloc((N + 1) * sizeof(int32_t))) ||
+        (rc = s->gt_ptr.alloc((N + 1) * sizeof(int32_t))) || (rc = work.alloc(scan_work_bytes(N))) ||
+        (rc = err.alloc(sizeof(int))))
+      return rc;
+    EFM_HIP_CHECK(hipMemsetAsync(cnt.p, 0, (N + 1) * sizeof(int32_t), st));
+    EFM_HIP_CHECK(hipMemsetAsync(cursor.p, 0, (N + 1) * sizeof(int32_t), st));
+    EFM_HIP_CHECK(hipMemsetAsync(err.p, 0, sizeof(int), st));
+    if (n > 0) {
+      hipLaunchKernelGGL(k_split_pairs, dim3(grid_for(n, 256)), dim3(256), 0, st, n, ab.as<double2>(),
+                         s->ga.as<double2>(), s->gb.as<double2>());
+      EFM_LAUNCH_CHECK();
+      hipLaunchKernelGGL(k_gt_count, dim3(grid_for(n, 256)), dim3(256), 0, st, n, N, s->en2.as<int2>(),
+                         cnt.as<int32_t>(), err.as<int>());
+      EFM_LAUNCH_CHECK();
+    }
+    int64_t total = 0;
+    if ((rc = exclusive_scan_i32(cnt.as<int32_t>(), N, s->gt_ptr.as<int32_t>(), work.p, st, &total)) != EFM_OK)
+      return rc;
+    int h_err = 0;
+    EFM_HIP_CHECK(hipMemcpyAsync(&h_err, err.p, sizeof(int), hipMemcpyDeviceToHost, st));
+    EFM_HIP_CHECK(hipStreamSynchronize(st));
+    EFM_REQUIRE(h_err == 0, "efm_cells_create: node index out of range");
+    if ((rc = s->gt_edge.alloc(std::max<int64_t>(total, 1) * sizeof(int32_t))) ||
+        (rc = s->gt_coef.alloc(std::max<int64_t>(total, 1) * sizeof(double2))))
+      return rc;
+    if (n > 0) {
+      hipLaunchKernelGGL(k_gt_fill, dim3(grid_for(n, 256)), dim3(256), 0, st, n, s->en2.as<int2>(),
+                         s->ga.as<double2>(), s->gb.as<double2>(), s->gt_ptr.as<int32_t>(), cursor.as<int32_t>(),
+                         s->gt_edge.as<int32_t>(), s->gt_coef.as<double2>());
+      EFM_LAUNCH_CHECK();
+    }
+    if (N > 0) {
+      hipLaunchKernelGGL(k_gt_sort, dim3(grid_for(N, 256)), dim3(256), 0, st, N, s->gt_ptr.as<int32_t>(),
+                         s->gt_edge.as<int32_t>(), s->gt_coef.as<double2>());
+      EFM_LAUNCH_CHECK();
+    }
+    EFM_HIP_CHECK(hipStreamSynchronize(st));  // the temporaries are released here
+  }
+  if ((rc = upload(s->tiles, tiles.data(), tiles.size(), st)) ||
       (rc = upload(s->ntiles_buf, ntl.data(), ntl.size(), st)) ||
       (rc = upload(s->cell_b0, first.data(), first.size(), st)) ||
       (rc = upload(s->poff, h_port_off, s->nterms + 1, st)) || (rc = upload(s->pidx, h_port_idx, npe, st)) ||
@@ -834,7 +923,7 @@ extern "C" int efm_cells_create(int64_t ncells, const int64_t *h_row_off, const 
     EFM_REQUIRE(h_err == 0, "efm_cells_create: a column leaves its cell, or a row has no entry");
   }
   DevBuf ext;
-  if ((rc = upload(ext, extra.data(), n, st)) || (rc = s->dinv.alloc(n * sizeof(double2))) ||
+  if ((rc = ext.alloc(std::max<int64_t>(n, 1) * sizeof(double2))) || (rc = s->dinv.alloc(n * sizeof(double2))) ||
       (rc = s->dninv.alloc(std::max<int64_t>(N, 1) * sizeof(double2))) ||
       (rc = s->ynode.alloc(std::max<int64_t>(N, 1) * s->R * sizeof(double2))) || (rc = s->x.alloc(vb)) ||
       (rc = s->r.alloc(vb)) || (rc = s->r0.alloc(vb)) || (rc = s->p.alloc(vb)) || (rc = s->v.alloc(vb)) ||
@@ -843,6 +932,12 @@ extern "C" int efm_cells_create(int64_t ncells, const int64_t *h_row_off, const 
       (rc = s->state.alloc((size_t)ncells * s->R * sizeof(CellSys))) ||
       (rc = s->flags.alloc((size_t)ncells * s->R * sizeof(int))))
     return rc;
+  EFM_HIP_CHECK(hipMemsetAsync(ext.p, 0, std::max<int64_t>(n, 1) * sizeof(double2), st));
+  if (s->nterms > 0 && npe > 0) {
+    hipLaunchKernelGGL(k_port_extra, dim3((unsigned)s->nterms), dim3(256), 0, st, s->poff.as<int64_t>(),
+                       s->pidx.as<int32_t>(), s->pw.as<double2>(), s->pcoef.as<double2>(), ext.as<double2>());
+    EFM_LAUNCH_CHECK();
+  }
   hipLaunchKernelGGL(k_cells_dinv, dim3(grid_for(n, 256)), dim3(256), 0, st, n, s->row_ptr.as<int32_t>(),
                      s->col.as<int32_t>(), s->val.as<double2>(), ext.as<double2>(), s->dinv.as<double2>());
   EFM_LAUNCH_CHECK();
