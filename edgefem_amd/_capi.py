@@ -68,6 +68,9 @@ def load_library(path: str = LIB_PATH) -> C.CDLL:
     global _lib
     if _lib is not None:
         return _lib
+    # torch first: it carries its own HIP runtime, and the library must bind to that one
+    # (loading the library first would bring in a second runtime, whose allocations fail)
+    _torch()
     if not os.path.exists(path):
         raise EdgeFemError(
             f"HIP library not found at {path}; build it with `python -m edgefem_amd.build` "

@@ -1810,11 +1810,16 @@ __global__ void __launch_bounds__(kBlock)
 }
 
 // ---------------------------------------------------------------------------
-// COCG in the z-gather form.  The same iteration, with A p carried by the recurrence
-//   p_new = z + beta p_old,   q_new = A z + beta q_old   (= A p_new),
-// so the SpMV gathers only z (R values per column, a separate array) and reads p_old,
-// q_old at the row itself; the update writes z, x, r.  The gathered bytes per nonzero
-// are halved against the {z, p} record of the fused form.
+// COCG with the search direction formed in its own pass (variant 9): per iteration
+//   p = z + beta p           (k_cocgz_pform: a stream over z and p)
+//   q = A p, partials p^T q  (k_cocgz_spmv: gathers p, R values per column in an array of
+//                             its own, and writes q only)
+//   x += alpha p, r -= alpha q, z = P r, partials   (k_cocgz_update)
+// The fused forms build p inside the SpMV and gather a {z, p} record (twice the gathered
+// bytes per nonzero).  Carrying q by the recurrence q_new = A z + beta q_old instead
+// (gathering z) saves the p pass but adds iterations on these indefinite systems: on the
+// 1.59 M-edge guide 4,489 iterations with an exact q every 8th, 4,129 every 2nd, against
+// 4,030 here, which is the fastest of the four per solve.
 // ---------------------------------------------------------------------------
 // after the fused-form init: zf = W0.z, pf = 0, q = 0
 template <int R>
@@ -1829,25 +1834,20 @@ __global__ void __launch_bounds__(kBlock)
   }
 }
 
-// Tiled, pipelined SpMV of the z-gather form (the tile walk, staging and branch-free loads
-// of k_cocg_spmv_pipe); partials p_new^T q_new.  EXACT: zf holds p (already formed), and
-// the kernel writes q = A p only (the periodic replacement of the q recurrence).
-template <typename VT, int R, int TNNZ, int RL, bool EXACT>
+// q = A p with p gathered from its own array (R values per column), tiled and pipelined
+// (the tile walk, staging and branch-free loads of k_cocg_spmv_pipe); partials p^T q.
+template <typename VT, int R, int TNNZ, int RL>
 __global__ void __launch_bounds__(kBlock)
     k_cocgz_spmv(int ntiles, int xcd_span, const int4 *__restrict__ desc, const int32_t *__restrict__ row_ptr,
-                 const int32_t *__restrict__ col, const VT *__restrict__ val, const double2 *__restrict__ zf,
-                 double2 *__restrict__ pf, double2 *__restrict__ q, double2 *__restrict__ partial,
-                 const SysState *__restrict__ st) {
+                 const int32_t *__restrict__ col, const VT *__restrict__ val, const double2 *__restrict__ pf,
+                 double2 *__restrict__ q, double2 *__restrict__ partial, const SysState *__restrict__ st) {
   __shared__ double2 prod[TNNZ * R];
   __shared__ double2 lds[(kBlock / 64) * R];
   constexpr int J = TNNZ / kBlock;
   bool act[R];
-  double2 beta[R];
 #pragma unroll
-  for (int s = 0; s < R; ++s) {
+  for (int s = 0; s < R; ++s)
     act[s] = st[s].active != 0;
-    beta[s] = EXACT ? make_double2(0, 0) : st[s].beta;
-  }
   double2 pq[R];
 #pragma unroll
   for (int s = 0; s < R; ++s)
@@ -1894,20 +1894,17 @@ __global__ void __launch_bounds__(kBlock)
     double2 zg[J][R];
 #pragma unroll
     for (int j = 0; j < J; ++j) {
-      const double2 *zc = zf + (int64_t)cur.c[j] * R;
+      const double2 *pc = pf + (int64_t)cur.c[j] * R;
 #pragma unroll
       for (int s = 0; s < R; ++s)
-        zg[j][s] = zc[s];
+        zg[j][s] = pc[s];
     }
     const int32_t row = cur.d.x + g;
     const int32_t rc = row < cur.d.y ? row : cur.d.y - 1;
-    double2 zr[R], po[R], qo[R];
+    double2 pr[R];
 #pragma unroll
-    for (int s = 0; s < R; ++s) {
-      zr[s] = zf[(int64_t)rc * R + s];
-      po[s] = pf[(int64_t)rc * R + s];
-      qo[s] = q[(int64_t)rc * R + s];
-    }
+    for (int s = 0; s < R; ++s)
+      pr[s] = pf[(int64_t)rc * R + s];
     stage(nxt, min(tile + per_xcd, t_end - 1));
 #pragma unroll
     for (int j = 0; j < J; ++j) {
@@ -1937,12 +1934,8 @@ __global__ void __launch_bounds__(kBlock)
       for (int s = 0; s < R; ++s) {
         if (!act[s])
           continue;
-        const double2 pn = EXACT ? zr[s] : cfma(beta[s], po[s], zr[s]);
-        const double2 qn = EXACT ? acc[s] : cfma(beta[s], qo[s], acc[s]);
-        if (!EXACT)
-          pf[(int64_t)row * R + s] = pn;
-        q[(int64_t)row * R + s] = qn;
-        pq[s] = cadd(pq[s], bil(pn, qn));
+        q[(int64_t)row * R + s] = acc[s];
+        pq[s] = cadd(pq[s], bil(pr[s], acc[s]));
       }
     }
     __syncthreads();  // prod is reused by the next tile
@@ -1951,7 +1944,7 @@ __global__ void __launch_bounds__(kBlock)
   write_partials<R>(pq, lds, partial);
 }
 
-// p = z + beta p (before an exact q = A p)
+// p = z + beta p
 template <int R>
 __global__ void __launch_bounds__(kBlock)
     k_cocgz_pform(int64_t n, const double2 *__restrict__ zf, double2 *__restrict__ pf,
@@ -2037,9 +2030,8 @@ __global__ void __launch_bounds__(kBlock)
 
 } // namespace
 
-constexpr int kNumSpmvVariants = 10;  // rows<4>, rows<8>, rows<16>, tiled<T,4>, tiled<T/2,8>, coop<T,4>, pipe<T,4>, lines<T/2,8>, local<4>, z-form
-constexpr int kZForm = 9;             // the z-gather form of COCG (its own SpMV and update)
-constexpr int kZReplace = 8;          // z-form: q = A p computed exactly every kZReplace iterations
+constexpr int kNumSpmvVariants = 10;  // rows<4>, rows<8>, rows<16>, tiled<T,4>, tiled<T/2,8>, coop<T,4>, pipe<T,4>, lines<T/2,8>, local<4>, p-pass form
+constexpr int kZForm = 9;             // COCG with p formed in its own pass (its own SpMV and update)
 
 struct efm_solver {
   int64_t n = 0, nnz = 0;
@@ -2061,7 +2053,7 @@ struct efm_solver {
   int grid_lines = 8;  // the one-line-per-nonzero variant (second tiling, resident blocks)
   int grid_tiles_res[2] = {8, 8};  // resident-block grids of the tiled variants
   int grid_pipe_res = 8;
-  // z-gather form of COCG: z and p arrays [n][R]; resident grid of its SpMV
+  // variant 9 (p formed in its own pass): z and p arrays [n][R]; resident grid of its SpMV
   efm::DevBuf zf, pf;
   int grid_z = 8;
   int grid_rows[3] = {8, 8, 8};
@@ -2147,7 +2139,7 @@ template <int R> struct Dispatch {
     return EFM_OK;
   }
 
-  // the z-form arrays from the fused-form init (z = W0.z, p = q = 0)
+  // the variant-9 arrays from the fused-form init (z = W0.z, p = q = 0)
   static int init_zform(efm_solver *s, hipStream_t st) {
     hipLaunchKernelGGL(k_cocgz_init<R>, dim3(s->grid), dim3(kBlock), 0, st, s->n, s->W0.as<double2>(),
                        s->zf.as<double2>(), s->pf.as<double2>(), s->q.as<double2>());
@@ -2267,10 +2259,9 @@ template <int R> struct Dispatch {
       return EFM_ERR_ARG;
     case kZForm:
       *nb = s->grid_z;
-      hipLaunchKernelGGL((k_cocgz_spmv<VT, R, TN, 4, false>), dim3(*nb), dim3(kBlock), 0, st, s->ntiles[0],
+      hipLaunchKernelGGL((k_cocgz_spmv<VT, R, TN, 4>), dim3(*nb), dim3(kBlock), 0, st, s->ntiles[0],
                          s->xcd_span[0], s->tile_desc.as<int4>(), s->row_ptr, s->col, s->val.as<VT>(),
-                         s->zf.as<double2>(), s->pf.as<double2>(), s->q.as<double2>(), s->partial.as<double2>(),
-                         S);
+                         s->pf.as<double2>(), s->q.as<double2>(), s->partial.as<double2>(), S);
       break;
     default:
       set_error("unknown SpMV variant");
@@ -2304,13 +2295,23 @@ template <int R> struct Dispatch {
       if (v == kZForm && (s->ntiles[0] == 0 || s->lr_P > 0))
         continue;
       int nb;
-      int rc = launch_spmv<VT>(s, v, s->W0.as<double2>(), s->W1.as<double2>(), st, &nb);  // warm-up
+      // variant 9 is timed with its p pass (run on W1 as scratch: the real p must stay as
+      // init left it)
+      auto launch = [&]() -> int {
+        if (v == kZForm) {
+          hipLaunchKernelGGL(k_cocgz_pform<R>, dim3(s->grid), dim3(kBlock), 0, st, s->n, s->zf.as<double2>(),
+                             s->W1.as<double2>(), s->state.as<SysState>());
+          EFM_LAUNCH_CHECK();
+        }
+        return launch_spmv<VT>(s, v, s->W0.as<double2>(), s->W1.as<double2>(), st, &nb);
+      };
+      int rc = launch();  // warm-up
       if (rc != EFM_OK)
         return rc;
       EFM_HIP_CHECK(hipEventRecord(e0, st));
       const int reps = 5;
       for (int k = 0; k < reps; ++k)
-        if ((rc = launch_spmv<VT>(s, v, s->W0.as<double2>(), s->W1.as<double2>(), st, &nb)) != EFM_OK)
+        if ((rc = launch()) != EFM_OK)
           return rc;
       EFM_HIP_CHECK(hipEventRecord(e1, st));
       EFM_HIP_CHECK(hipEventSynchronize(e1));
@@ -2337,25 +2338,14 @@ template <int R> struct Dispatch {
     if (s->spmv_variant == kZForm) {
       int nb_spmv = 0;
       int rc;
-      const bool exact = (it % kZReplace) == kZReplace - 1;
-      if (exact) {
-        // replace the recurrence: p = z + beta p, then q = A p exactly
-        hipLaunchKernelGGL(k_cocgz_pform<R>, dim3(s->grid), dim3(kBlock), 0, st, s->n, s->zf.as<double2>(),
-                           s->pf.as<double2>(), S);
-        EFM_LAUNCH_CHECK();
-      }
+      // p = z + beta p, then q = A p
+      hipLaunchKernelGGL(k_cocgz_pform<R>, dim3(s->grid), dim3(kBlock), 0, st, s->n, s->zf.as<double2>(),
+                         s->pf.as<double2>(), S);
+      EFM_LAUNCH_CHECK();
       if (e0)
         EFM_HIP_CHECK(hipEventRecord(e0, st));
-      if (exact) {
-        nb_spmv = s->grid_z;
-        hipLaunchKernelGGL((k_cocgz_spmv<VT, R, tile_nnz(R), 4, true>), dim3(nb_spmv), dim3(kBlock), 0, st,
-                           s->ntiles[0], s->xcd_span[0], s->tile_desc.as<int4>(), s->row_ptr, s->col,
-                           s->val.as<VT>(), s->pf.as<double2>(), s->pf.as<double2>(), s->q.as<double2>(),
-                           s->partial.as<double2>(), S);
-        EFM_LAUNCH_CHECK();
-      } else if ((rc = launch_spmv<VT>(s, kZForm, Wa, Wb, st, &nb_spmv)) != EFM_OK) {
+      if ((rc = launch_spmv<VT>(s, kZForm, Wa, Wb, st, &nb_spmv)) != EFM_OK)
         return rc;
-      }
       if (e1)
         EFM_HIP_CHECK(hipEventRecord(e1, st));
       if ((rc = fin<kFinAlpha>(s, nb_spmv, st, tol2, max_it)) != EFM_OK)
@@ -2614,8 +2604,8 @@ extern "C" int efm_solver_create(int64_t n, int64_t nnz, const int32_t *row_ptr,
           std::min<int64_t>(s->xcd_span[0], std::max<int64_t>(1, (int64_t)ncu * occ(kp) / 8));
       s->grid_pipe_res = (int)(8 * per_xcd);
 #define EFM_Z_K(RR)                                                                                       \
-  (c64 ? reinterpret_cast<const void *>(&k_cocgz_spmv<float2, RR, tile_nnz(RR), 4, false>)                \
-       : reinterpret_cast<const void *>(&k_cocgz_spmv<double2, RR, tile_nnz(RR), 4, false>))
+  (c64 ? reinterpret_cast<const void *>(&k_cocgz_spmv<float2, RR, tile_nnz(RR), 4>)                \
+       : reinterpret_cast<const void *>(&k_cocgz_spmv<double2, RR, tile_nnz(RR), 4>))
       const void *kz = nrhs == 1 ? EFM_Z_K(1)
                      : nrhs == 2 ? EFM_Z_K(2)
                      : nrhs == 3 ? EFM_Z_K(3)
@@ -2952,7 +2942,7 @@ extern "C" int efm_solver_solve(efm_solver *s, const double *B, double *X,
   const bool cocg = h_opts->method == EFM_METHOD_COCG;
   EFM_REQUIRE(!(cocg && s->caux), "efm_solver_solve: the complex-incidence preconditioner needs BiCGSTAB");
   EFM_REQUIRE(!(cocg && s->spmv_variant == kZForm && s->lr_P > 0),
-              "efm_solver_solve: SpMV variant 9 (z-gather form) does not take low-rank port terms");
+              "efm_solver_solve: SpMV variant 9 (p formed in its own pass) does not take low-rank port terms");
   int rc;
   if (!cocg) {
     const size_t vb = (size_t)s->n * s->R * sizeof(double2);

@@ -187,9 +187,9 @@ int efm_solver_solve(efm_solver *s, const double *B, double *X, const efm_solve_
  * the current tile's gathers are in flight; 7: LDS-tiled with each column's record
  * loaded by 2*nrhs adjacent lanes, one cache line per nonzero, nrhs 1, 2 or 4; 8: a
  * tile-local column format (each tile's distinct columns loaded once into LDS, 16-bit
- * local indices per entry, built on the device at first use), nrhs 1, 2 or 4; 9: the
- * z-gather form of the same COCG iteration, p = z + beta p_old and A p = A z + beta (A p)_old,
- * so the SpMV gathers only z and streams p, q at the row, COCG without low-rank terms).  By
+ * local indices per entry, built on the device at first use), nrhs 1, 2 or 4; 9: the same
+ * COCG iteration with p = z + beta p formed in a streaming pass of its own, so the SpMV
+ * gathers only p (R values per column) and writes only q, COCG without low-rank terms).  By
  * default (-1) the first solve times every variant on the installed matrix and keeps the
  * fastest; a fixed variant makes results reproducible bit for bit across runs.
  * h_tune_ms (10 entries, may be NULL) receives the

@@ -357,7 +357,7 @@ def test_solver_masks_converged_systems(torch_dev):
 def test_solver_every_spmv_variant_converges_to_same_solution(torch_dev, nrhs):
     """The COCG solve with each fused-SpMV variant pinned (row mappings, LDS tiles, the
     cooperative, pipelined and one-line-per-nonzero gathers, the tile-local column
-    format, and the z-gather form of the iteration) on the C1 eigenmode matrix
+    format, and the iteration with p formed in its own pass) on the C1 eigenmode matrix
     with the auxiliary preconditioner: every variant converges, its true residual
     |b - A x| / |b| is at the tolerance, and the solutions agree."""
     import torch
