@@ -33,7 +33,7 @@ EXPORTS = [
     "efm_sweep_create", "efm_sweep_destroy", "efm_sweep_solve", "efm_eps_gradient",
     "efm_solver_set_lowrank", "efm_add_diagonal", "efm_periodic_reduce", "efm_solver_set_gradient_complex",
     "efm_cells_create", "efm_cells_destroy", "efm_cells_solve",
-    "efm_sweep_set_variants", "efm_sweep_get_variants",
+    "efm_sweep_set_variants", "efm_sweep_get_variants", "efm_sweep_set_iteration",
 ]
 
 
@@ -116,6 +116,7 @@ def load_library(path: str = LIB_PATH) -> C.CDLL:
     lib.efm_sweep_destroy.argtypes = [vp]
     lib.efm_sweep_destroy.restype = None
     lib.efm_sweep_set_variants.argtypes = [vp, i32, i32]
+    lib.efm_sweep_set_iteration.argtypes = [vp, i32]
     lib.efm_sweep_get_variants.argtypes = [vp, C.POINTER(C.c_int), C.POINTER(C.c_int), vp]
     lib.efm_sweep_solve.argtypes = [vp, i32, vp, vp, vp, vp, C.POINTER(SolveOpts), C.POINTER(RefineOpts), vp, vp,
                                     vp]
@@ -475,6 +476,13 @@ class SweepSolver:
         preconditioner part (0, 1 as for the SpMV); -1: the fastest, timed on the first
         solve."""
         _check(self._lib.efm_sweep_set_variants(self._h, int(spmv), int(aux)), "efm_sweep_set_variants")
+
+    def set_iteration(self, form: int = 0) -> None:
+        """Iteration form of the batched COCG: 0 (default) standard, with p = u + beta p in
+        a pass of its own and two reductions per iteration; 1 Chronopoulos-Gear, one
+        reduction per iteration (s = A p carried by a recurrence, which costs iterations
+        on indefinite systems)."""
+        _check(self._lib.efm_sweep_set_iteration(self._h, int(form)), "efm_sweep_set_iteration")
 
     def variants(self):
         """(SpMV variant, nodal variant, mean ms per launch over the tuning iterations

@@ -133,11 +133,13 @@ __global__ void __launch_bounds__(kBlock)
   }
 }
 
-// y[n][s] = Dn_s(n)^{-1} sum_k G (r - alpha w - (alpha beta + pend) s)[e][s]   (FUSED),
-// or G^T r (r, s as stored: this is G^T r_{i+1}).
+// y[n][s] = Dn_s(n)^{-1} (G^T v)[n][s], NV vectors gathered per edge:
+//   NV = 1: v = r;  NV = 2: v = r - alpha w (standard COCG: the new residual, w = A p);
+//   NV = 3: v = r - alpha w - (alpha beta + pend) s (Chronopoulos-Gear, r and s as stored:
+//   this is r_{i+1}).
 // One node per wavefront; the node's edge list and gradient signs are loaded across the
 // lanes and broadcast, so the gathers of its edges are independent of each other.
-template <typename VT, bool FUSED>
+template <typename VT, int NV>
 __global__ void __launch_bounds__(kBlock)
     k_sw_aux_node(int64_t N, int B, int P, const int32_t *__restrict__ n2e_ptr,
                   const int32_t *__restrict__ n2e, const int8_t *__restrict__ gsign,
@@ -153,9 +155,9 @@ __global__ void __launch_bounds__(kBlock)
   const bool act = s < B && st[s].active != 0;
   if (__ballot(act) == 0)
     return;
-  const double2 alpha = FUSED ? st[s].alpha : make_double2(0, 0);
+  const double2 alpha = NV >= 2 ? st[s].alpha : make_double2(0, 0);
   // r_{i+1} = r - alpha w - (alpha beta + pend) s with r, s as stored
-  const double2 abp = FUSED ? cadd(cmul(st[s].alpha, st[s].beta), st[s].pend) : make_double2(0, 0);
+  const double2 abp = NV == 3 ? cadd(cmul(st[s].alpha, st[s].beta), st[s].pend) : make_double2(0, 0);
   const double k2 = k0sq[s];
   for (int64_t nd = (int64_t)blockIdx.x * kWaves + wv; nd < N; nd += (int64_t)gridDim.x * kWaves) {
     double2 acc = make_double2(0, 0);
@@ -174,8 +176,10 @@ __global__ void __launch_bounds__(kBlock)
           if (j + t < len) {
             const int64_t i = (int64_t)bcast(ej, j + t) * B + s;
             rv[t] = ldv(r + i);
-            if (FUSED)
-              rv[t] = csub(csub(rv[t], bil(alpha, ldv(w + i))), bil(abp, ldv(sv + i)));
+            if (NV >= 2)
+              rv[t] = csub(rv[t], bil(alpha, ldv(w + i)));
+            if (NV == 3)
+              rv[t] = csub(rv[t], bil(abp, ldv(sv + i)));
           }
 #pragma unroll
         for (int t = 0; t < U; ++t)
@@ -219,21 +223,20 @@ __global__ void __launch_bounds__(kBlock)
   }
 }
 
-// y[n][s] = Dn_s(n)^{-1} sum_k G (r - alpha (w + beta s))[e][s]   (FUSED), or G^T r:
-// the same map as k_sw_aux_node, with the node walk of k_sw_spmv_pre.  Groups of 64
-// systems are walked in turn, the blocks of an XCD walk a contiguous node range (so an
-// edge gathered by both of its end nodes is read from HBM once), each wave preloads the
-// edge-list bounds of its next 64 nodes, and the gathers of a batch of NE edges (three
-// vectors each when FUSED) are issued with no branch: lanes past the node's degree hold
-// a valid edge with gradient sign 0.
-template <typename VT, bool FUSED>
-__global__ void __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(FUSED ? 3 : 4, 8)))
+// The same map as k_sw_aux_node (NV gathered vectors), with the node walk of
+// k_sw_spmv_pre.  Groups of 64 systems are walked in turn, the blocks of an XCD walk a
+// contiguous node range (so an edge gathered by both of its end nodes is read from HBM
+// once), each wave preloads the edge-list bounds of its next 64 nodes, and the gathers of
+// a batch of NE edges (NV vectors each) are issued with no branch: lanes past the node's
+// degree hold a valid edge with gradient sign 0.
+template <typename VT, int NV>
+__global__ void __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(NV > 1 ? 3 : 4, 8)))
     k_sw_aux_pre(int64_t N, int B, int groups, const int32_t *__restrict__ n2e_ptr,
                  const int32_t *__restrict__ n2e, const int8_t *__restrict__ gsign,
                  const double2 *__restrict__ dninv, const VT *__restrict__ r,
                  const VT *__restrict__ w, const VT *__restrict__ sv, VT *__restrict__ y,
                  const SwState *__restrict__ st) {
-  constexpr int NE = FUSED ? 7 : 14;
+  constexpr int NE = NV == 3 ? 7 : NV == 2 ? 10 : 14;
   const int lane = threadIdx.x & 63;
   const int wv = threadIdx.x >> 6;
   const int64_t nchunks = (N + kWaves - 1) / kWaves;
@@ -244,8 +247,8 @@ __global__ void __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(FUS
     const bool act = s < B && st[s].active != 0;
     if (__ballot(act) == 0)
       continue;
-    const double2 alpha = FUSED ? st[s].alpha : make_double2(0, 0);
-    const double2 ab = FUSED ? cadd(cmul(st[s].alpha, st[s].beta), st[s].pend) : make_double2(0, 0);
+    const double2 alpha = NV >= 2 ? st[s].alpha : make_double2(0, 0);
+    const double2 ab = NV == 3 ? cadd(cmul(st[s].alpha, st[s].beta), st[s].pend) : make_double2(0, 0);
     for (int64_t jb = 0; jb < nr; jb += 64) {
       const int cnt = (int)min<int64_t>(64, nr - jb);
       int lo = 0, hi = 0;
@@ -274,10 +277,10 @@ __global__ void __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(FUS
             for (int t = 0; t < NE; ++t) {
               const int64_t i = (int64_t)bcast(ej, min(e0 + t, 63)) * B + s;
               xr[t] = ldv(r + i);
-              if (FUSED) {
+              if (NV >= 2)
                 xw[t] = ldv(w + i);
+              if (NV == 3)
                 xs[t] = ldv(sv + i);
-              }
             }
 #pragma unroll
             for (int t = 0; t < NE; ++t) {
@@ -285,9 +288,11 @@ __global__ void __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(FUS
               const double gs = (e0 + t < m) ? (double)bcast(gj, min(e0 + t, 63)) : 0.0;
               sr.x += gs * xr[t].x;
               sr.y += gs * xr[t].y;
-              if (FUSED) {
+              if (NV >= 2) {
                 sw.x += gs * xw[t].x;
                 sw.y += gs * xw[t].y;
+              }
+              if (NV == 3) {
                 ss.x += gs * xs[t].x;
                 ss.y += gs * xs[t].y;
               }
@@ -295,8 +300,9 @@ __global__ void __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(FUS
           }
         }
         if (act && nd < N) {
-          // sum G (r - alpha w - (alpha beta + pend) s)
-          const double2 acc = FUSED ? csub(csub(sr, cmul(alpha, sw)), cmul(ab, ss)) : sr;
+          // sum G (r - alpha w - (alpha beta + pend) s), the terms NV asks for
+          const double2 acc = NV == 3 ? csub(csub(sr, cmul(alpha, sw)), cmul(ab, ss))
+                                      : NV == 2 ? csub(sr, cmul(alpha, sw)) : sr;
           stv(y + nd * B + s, cmul(dinv, acc));
         }
       }
@@ -366,6 +372,71 @@ __global__ void __launch_bounds__(kBlock)
   }
   write_sys_partials<2>(v, lds, partial, g * gridDim.x + blockIdx.x);
   __syncthreads();  // lds reuse by the next group
+  }
+}
+
+// ---------------------------------------------------------------------------
+// Standard COCG on the batch (the default iteration form): per iteration
+//   p = u + beta p;  w = A p, partials p^T w;  alpha = rho / p^T w
+//   y = Dn^{-1} G^T (r - alpha w);  x += alpha p;  r -= alpha w;  u = D^{-1} r + G y,
+//   partials u^T r, |r|^2;  beta = u^T r / rho
+// Two reduction points per iteration against one for Chronopoulos-Gear, whose s = w +
+// beta s recurrence drifts from A p and costs iterations on these indefinite systems
+// (394/462/584 against 394/490/623 on the WR-90 16x8x32 sweep at 8.2/10/12.4 GHz, 2,333
+// against 3,659 on a permittivity-10 slab).
+// ---------------------------------------------------------------------------
+// p = u + beta p for the active systems
+template <typename VT>
+__global__ void __launch_bounds__(kBlock)
+    k_sw_pform(int64_t n, int B, const VT *__restrict__ u, VT *__restrict__ p, const SwState *__restrict__ st) {
+  const int s = blockIdx.y * kSys + (threadIdx.x & 63);
+  const int wv = threadIdx.x >> 6;
+  if (s >= B || st[s].active == 0)
+    return;
+  const double2 beta = st[s].beta;
+  for (int64_t e = (int64_t)blockIdx.x * kWaves + wv; e < n; e += (int64_t)gridDim.x * kWaves) {
+    const int64_t i = e * B + s;
+    stv(p + i, cfma(beta, ldv(p + i), ldv(u + i)));
+  }
+}
+
+// x += alpha p; r -= alpha w; u = D^{-1} r + G y; partials u^T r, |r|^2.  Persistent: one
+// block slice walks the groups of 64 systems in turn.
+template <typename VT>
+__global__ void __launch_bounds__(kBlock)
+    k_sw_update_std(int64_t n, int B, int groups, int P, const int2 *__restrict__ en_eff,
+                    const double2 *__restrict__ dK, const double2 *__restrict__ dM,
+                    const double *__restrict__ dMs, const int8_t *__restrict__ rport,
+                    const double *__restrict__ k0sq, const double2 *__restrict__ coef,
+                    const VT *__restrict__ y, const VT *__restrict__ w, const VT *__restrict__ p,
+                    VT *__restrict__ u, VT *__restrict__ x, VT *__restrict__ r, double2 *__restrict__ partial,
+                    const SwState *__restrict__ st) {
+  __shared__ double2 lds[kWaves * 2 * 64];
+  const int wv = threadIdx.x >> 6;
+  for (int g = blockIdx.y; g < groups; g += gridDim.y) {
+    const int s = g * kSys + (threadIdx.x & 63);
+    double2 v[2] = {make_double2(0, 0), make_double2(0, 0)};
+    const bool act = s < B && st[s].active != 0;
+    if (act) {
+      const double k2 = k0sq[s];
+      const double2 alpha = st[s].alpha;
+      for (int64_t e = (int64_t)blockIdx.x * kWaves + wv; e < n; e += (int64_t)gridDim.x * kWaves) {
+        const int64_t i = e * B + s;
+        stv(x + i, cfma(alpha, ldv(p + i), ldv(x + i)));
+        const double2 rv = csub(ldv(r + i), bil(alpha, ldv(w + i)));
+        stv(r + i, rv);
+        const double2 di = edge_diag_inv(e, s, P, k2, dK, dM, dMs, rport, coef);
+        double2 z = bil(di, rv);
+        const int2 ab = en_eff[e];
+        if (ab.x >= 0)
+          z = cadd(z, csub(ldv(y + (int64_t)ab.y * B + s), ldv(y + (int64_t)ab.x * B + s)));
+        stv(u + i, z);
+        v[0] = cadd(v[0], bil(rv, z));
+        v[1].x += cabs2(rv);
+      }
+    }
+    write_sys_partials<2>(v, lds, partial, g * gridDim.x + blockIdx.x);
+    __syncthreads();  // lds reuse by the next group
   }
 }
 
@@ -1277,6 +1348,71 @@ __global__ void __launch_bounds__(kBlock)
   active_flag[s] = S.active;
 }
 
+// Scalars of the standard form, one block per system.  STAGE 0 (after the initial u = P r):
+// rho = u^T r, |b|^2; STAGE 1 (after the SpMV): alpha = rho / p^T w; STAGE 2 (after the
+// update): convergence on |r|^2, beta = u^T r / rho.
+template <int STAGE>
+__global__ void __launch_bounds__(kBlock)
+    k_sw_fin_std(int B, int gx, const double2 *__restrict__ part, SwState *__restrict__ st,
+                 int *__restrict__ active_flag, double tol2, int max_it, int max_inner) {
+  const int s = blockIdx.x;
+  if (s >= B)
+    return;
+  const int gy = s / kSys, lane = s % kSys;
+  constexpr int V = STAGE == 1 ? 1 : 2;
+  double2 v[V];
+#pragma unroll
+  for (int k = 0; k < V; ++k)
+    v[k] = make_double2(0, 0);
+  for (int bx = threadIdx.x; bx < gx; bx += kBlock) {
+    const int64_t bl = (int64_t)gy * gx + bx;
+#pragma unroll
+    for (int k = 0; k < V; ++k)
+      v[k] = cadd(v[k], part[(bl * V + k) * 64 + lane]);
+  }
+  __shared__ double2 sh[V][kBlock];
+  block_sum<V>(v, sh);
+  if (threadIdx.x != 0)
+    return;
+  SwState &S = st[s];
+  if (STAGE == 0) {
+    const double rn2 = v[1].x;
+    S.bnorm2 = rn2;
+    S.rnorm2 = rn2;
+    S.iters = 0;
+    S.breakdown = 0;
+    S.beta = make_double2(0, 0);
+    S.pend = make_double2(0, 0);
+    S.rho = v[0];
+    S.active = (S.active && rn2 > 0.0) ? 1 : 0;
+  } else if (STAGE == 1) {
+    if (S.active) {
+      const double2 d = v[0];
+      if (d.x == 0.0 && d.y == 0.0) {
+        S.breakdown = 1;
+        S.active = 0;
+      } else {
+        S.alpha = cdiv(S.rho, d);
+      }
+    }
+  } else if (S.active) {
+    const double2 g = v[0];
+    const double rn2 = v[1].x;
+    S.iters += 1;
+    S.total_iters += 1;
+    S.rnorm2 = rn2;
+    if (rn2 <= tol2 * S.bnorm2) {
+      S.active = 0;
+    } else if (S.total_iters >= max_it || S.iters >= max_inner || (S.rho.x == 0.0 && S.rho.y == 0.0)) {
+      S.active = 0;
+    } else {
+      S.beta = cdiv(g, S.rho);
+      S.rho = g;
+    }
+  }
+  active_flag[s] = S.active;
+}
+
 // Frequency-independent diagonals: edge diagonals of K, M, Ms; nodal diag(G^T X G).
 __global__ void k_sw_edge_diag(int64_t n, const int32_t *__restrict__ row_ptr,
                                const int32_t *__restrict__ col, const double2 *__restrict__ K,
@@ -1413,6 +1549,9 @@ struct efm_sweep {
   int aux_variant = -1;
   int gx_aux = 8;
   int cplx = 3;  // bit 1: K complex, bit 2: M complex (selects the SpMV instantiation)
+  // iteration: 0 = standard COCG (p in its own pass, two reductions), 1 = Chronopoulos-Gear
+  int form = 0;
+  int gx_std[2] = {8, 8};  // grid of the standard form's update [VT c128, c64]
   // update grids (one block slice, as many blocks as are resident): [VT c128, c64][INIT, skip, apply]
   int gx_upd[2][3] = {{8, 8, 8}, {8, 8, 8}};
   float tune_ms[5] = {0.f, 0.f, 0.f, 0.f, 0.f};  // SpMV 0, 1, 2; nodal 0, 1
@@ -1488,7 +1627,7 @@ extern "C" int efm_sweep_create(const efm_topology *topo, const double *K_val, c
     // pipelined kernels and the update: one block slice (gridDim.y = 1), as many blocks
     // as are resident
     s->gx_pre = resident_grid(reinterpret_cast<const void *>(&k_sw_spmv_pre<double2, true, true>), ncu, n);
-    s->gx_aux = resident_grid(reinterpret_cast<const void *>(&k_sw_aux_pre<double2, true>), ncu, N);
+    s->gx_aux = resident_grid(reinterpret_cast<const void *>(&k_sw_aux_pre<double2, 3>), ncu, N);
     s->gx_tile = resident_grid(reinterpret_cast<const void *>(&k_sw_spmv_tile<double2, true, true>), ncu,
                                ((n + kTileRows - 1) / kTileRows) * kWaves, kTileBlock);
     const void *upd[2][3] = {
@@ -1501,11 +1640,15 @@ extern "C" int efm_sweep_create(const efm_topology *topo, const double *K_val, c
     for (int a = 0; a < 2; ++a)
       for (int b = 0; b < 3; ++b)
         s->gx_upd[a][b] = resident_grid(upd[a][b], ncu, n);
+    s->gx_std[0] = resident_grid(reinterpret_cast<const void *>(&k_sw_update_std<double2>), ncu, n);
+    s->gx_std[1] = resident_grid(reinterpret_cast<const void *>(&k_sw_update_std<float2>), ncu, n);
   }
   int gx_upd_max = 8;
-  for (int a = 0; a < 2; ++a)
+  for (int a = 0; a < 2; ++a) {
     for (int b = 0; b < 3; ++b)
       gx_upd_max = std::max(gx_upd_max, s->gx_upd[a][b]);
+    gx_upd_max = std::max(gx_upd_max, s->gx_std[a]);
+  }
   const int64_t ntl = (n + kTileRows - 1) / kTileRows;
   // K, M, Ms and the tile entry slots carry one zero entry at index nnz
   if ((rc = s->K.alloc((nnz + 1) * 16)) || (rc = s->M.alloc((nnz + 1) * 16)) ||
@@ -1596,6 +1739,13 @@ extern "C" int efm_sweep_set_variants(efm_sweep *s, int spmv, int aux) {
   return EFM_OK;
 }
 
+extern "C" int efm_sweep_set_iteration(efm_sweep *s, int form) {
+  EFM_REQUIRE(s, "efm_sweep_set_iteration: NULL sweep");
+  EFM_REQUIRE(form == 0 || form == 1, "efm_sweep_set_iteration: form must be 0 (standard) or 1 (Chronopoulos-Gear)");
+  s->form = form;
+  return EFM_OK;
+}
+
 extern "C" int efm_sweep_get_variants(const efm_sweep *s, int *h_spmv, int *h_aux, double *h_tune_ms) {
   EFM_REQUIRE(s && h_spmv && h_aux, "efm_sweep_get_variants: NULL argument");
   *h_spmv = s->spmv_variant;
@@ -1679,6 +1829,77 @@ struct SwRun {
     return EFM_OK;
   }
 
+  // w = A_s u with SpMV variant `variant` (partials of u^T w in part_s)
+  template <typename VT> int spmv_launch(int variant, const VT *u, VT *w) {
+    const int64_t n = s->n;
+    if (variant == 2) {
+      const dim3 grid(s->gx_tile, 1);
+      const double2 *Kv = s->K.as<double2>(), *Mv = s->M.as<double2>(), *cf = s->coef.as<double2>();
+      const double *Msv = s->Ms.as<double>(), *k2 = s->k0sq.as<double>();
+      const int8_t *rp = s->rport.as<int8_t>();
+      const int32_t *uc = s->tucol.as<int32_t>(), *tn = s->tnu.as<int32_t>();
+      const uint8_t *li = s->tlidx.as<uint8_t>(), *dl = s->tdloc.as<uint8_t>();
+      switch (s->cplx & 3) {
+      case 0:
+        hipLaunchKernelGGL((k_sw_spmv_tile<VT, false, false>), grid, dim3(kTileBlock), 0, st, n, Bp, P, groups,
+                           s->row_ptr, uc, tn, li, dl, Kv, Mv, Msv, rp, k2, cf, u, w, ps, S);
+        break;
+      case 1:
+        hipLaunchKernelGGL((k_sw_spmv_tile<VT, true, false>), grid, dim3(kTileBlock), 0, st, n, Bp, P, groups,
+                           s->row_ptr, uc, tn, li, dl, Kv, Mv, Msv, rp, k2, cf, u, w, ps, S);
+        break;
+      case 2:
+        hipLaunchKernelGGL((k_sw_spmv_tile<VT, false, true>), grid, dim3(kTileBlock), 0, st, n, Bp, P, groups,
+                           s->row_ptr, uc, tn, li, dl, Kv, Mv, Msv, rp, k2, cf, u, w, ps, S);
+        break;
+      default:
+        hipLaunchKernelGGL((k_sw_spmv_tile<VT, true, true>), grid, dim3(kTileBlock), 0, st, n, Bp, P, groups,
+                           s->row_ptr, uc, tn, li, dl, Kv, Mv, Msv, rp, k2, cf, u, w, ps, S);
+      }
+    } else if (variant == 1) {
+      const dim3 grid(s->gx_pre, 1);
+      const double2 *Kv = s->K.as<double2>(), *Mv = s->M.as<double2>(), *cf = s->coef.as<double2>();
+      const double *Msv = s->Ms.as<double>(), *k2 = s->k0sq.as<double>();
+      const int8_t *rp = s->rport.as<int8_t>();
+      switch (s->cplx & 3) {
+      case 0:
+        hipLaunchKernelGGL((k_sw_spmv_pre<VT, false, false>), grid, dim3(kBlock), 0, st, n, Bp, P, groups,
+                           s->row_ptr, s->col, Kv, Mv, Msv, rp, k2, cf, u, w, ps, S);
+        break;
+      case 1:
+        hipLaunchKernelGGL((k_sw_spmv_pre<VT, true, false>), grid, dim3(kBlock), 0, st, n, Bp, P, groups,
+                           s->row_ptr, s->col, Kv, Mv, Msv, rp, k2, cf, u, w, ps, S);
+        break;
+      case 2:
+        hipLaunchKernelGGL((k_sw_spmv_pre<VT, false, true>), grid, dim3(kBlock), 0, st, n, Bp, P, groups,
+                           s->row_ptr, s->col, Kv, Mv, Msv, rp, k2, cf, u, w, ps, S);
+        break;
+      default:
+        hipLaunchKernelGGL((k_sw_spmv_pre<VT, true, true>), grid, dim3(kBlock), 0, st, n, Bp, P, groups,
+                           s->row_ptr, s->col, Kv, Mv, Msv, rp, k2, cf, u, w, ps, S);
+      }
+    } else
+      hipLaunchKernelGGL((k_sw_spmv<VT>), dim3(s->gx_spmv, s->gy_spmv), dim3(kBlock), 0, st, n, Bp, P, groups,
+                         s->row_ptr, s->col, s->K.as<double2>(), s->M.as<double2>(), s->Ms.as<double>(),
+                         s->rport.as<int8_t>(), s->k0sq.as<double>(), s->coef.as<double2>(), u, w, ps, S);
+    EFM_LAUNCH_CHECK();
+    return EFM_OK;
+    }
+
+  // y = Dn^{-1} G^T v with nodal variant `variant`, NV gathered vectors (see k_sw_aux_node)
+  template <typename VT, int NV> int aux_launch(int variant, const VT *r, const VT *w, const VT *sv, VT *y) {
+    const int8_t *gs = s->gsign.as<int8_t>();
+    if (variant == 1)
+      hipLaunchKernelGGL((k_sw_aux_pre<VT, NV>), dim3(s->gx_aux, 1), dim3(kBlock), 0, st, s->N, Bp, groups,
+                         s->n2e_ptr, s->n2e, gs, s->dninv.as<double2>(), r, w, sv, y, S);
+    else
+      hipLaunchKernelGGL((k_sw_aux_node<VT, NV>), dim3(s->gx_nodes, groups), dim3(kBlock), 0, st, s->N, Bp, P,
+                         s->n2e_ptr, s->n2e, gs, s->dnK.as<double2>(), s->dnM.as<double2>(), s->dnMs.as<double>(),
+                         s->k0sq.as<double>(), s->coef.as<double2>(), r, w, sv, y, S);
+    EFM_LAUNCH_CHECK();
+    return EFM_OK;
+  }
+
   // Preconditioned COCG on the systems whose state is active: x = A^{-1} r0, where r
   // holds r0 on entry (b == nullptr) or is set from b.  Vectors of type VT.
   template <typename VT> int cocg(const double2 *b, double inner_tol2, int max_inner) {
@@ -1694,27 +1915,8 @@ struct SwRun {
                        s->coef.as<double2>(), s->dninv.as<double2>());
     EFM_LAUNCH_CHECK();
     auto aux_v = [&](bool fused, int variant) -> int {
-      const int8_t *gs = s->gsign.as<int8_t>();
-      const double2 *dn = s->dninv.as<double2>();
-      if (variant == 1) {
-        if (fused)
-          hipLaunchKernelGGL((k_sw_aux_pre<VT, true>), dim3(s->gx_aux, 1), dim3(kBlock), 0, st, s->N, Bp, groups,
-                             s->n2e_ptr, s->n2e, gs, dn, r, w, sv, y, S);
-        else
-          hipLaunchKernelGGL((k_sw_aux_pre<VT, false>), dim3(s->gx_aux, 1), dim3(kBlock), 0, st, s->N, Bp, groups,
-                             s->n2e_ptr, s->n2e, gs, dn, r, (const VT *)nullptr, (const VT *)nullptr, y, S);
-      } else if (fused) {
-        hipLaunchKernelGGL((k_sw_aux_node<VT, true>), dim3(s->gx_nodes, groups), dim3(kBlock), 0, st, s->N, Bp,
-                           P, s->n2e_ptr, s->n2e, gs, s->dnK.as<double2>(), s->dnM.as<double2>(),
-                           s->dnMs.as<double>(), s->k0sq.as<double>(), s->coef.as<double2>(), r, w, sv, y, S);
-      } else {
-        hipLaunchKernelGGL((k_sw_aux_node<VT, false>), dim3(s->gx_nodes, groups), dim3(kBlock), 0, st, s->N,
-                           Bp, P, s->n2e_ptr, s->n2e, gs, s->dnK.as<double2>(), s->dnM.as<double2>(),
-                           s->dnMs.as<double>(), s->k0sq.as<double>(), s->coef.as<double2>(), r,
-                           (const VT *)nullptr, (const VT *)nullptr, y, S);
-      }
-      EFM_LAUNCH_CHECK();
-      return EFM_OK;
+      return fused ? aux_launch<VT, 3>(variant, r, w, sv, y)
+                   : aux_launch<VT, 1>(variant, r, (const VT *)nullptr, (const VT *)nullptr, y);
     };
     constexpr int vti = std::is_same<VT, float2>::value ? 1 : 0;
     auto update = [&](bool first, bool apply) -> int {
@@ -1736,60 +1938,7 @@ struct SwRun {
       EFM_LAUNCH_CHECK();
       return EFM_OK;
     };
-    auto spmv_v = [&](int variant) -> int {
-      if (variant == 2) {
-        const dim3 grid(s->gx_tile, 1);
-        const double2 *Kv = s->K.as<double2>(), *Mv = s->M.as<double2>(), *cf = s->coef.as<double2>();
-        const double *Msv = s->Ms.as<double>(), *k2 = s->k0sq.as<double>();
-        const int8_t *rp = s->rport.as<int8_t>();
-        const int32_t *uc = s->tucol.as<int32_t>(), *tn = s->tnu.as<int32_t>();
-        const uint8_t *li = s->tlidx.as<uint8_t>(), *dl = s->tdloc.as<uint8_t>();
-        switch (s->cplx & 3) {
-        case 0:
-          hipLaunchKernelGGL((k_sw_spmv_tile<VT, false, false>), grid, dim3(kTileBlock), 0, st, n, Bp, P, groups,
-                             s->row_ptr, uc, tn, li, dl, Kv, Mv, Msv, rp, k2, cf, u, w, ps, S);
-          break;
-        case 1:
-          hipLaunchKernelGGL((k_sw_spmv_tile<VT, true, false>), grid, dim3(kTileBlock), 0, st, n, Bp, P, groups,
-                             s->row_ptr, uc, tn, li, dl, Kv, Mv, Msv, rp, k2, cf, u, w, ps, S);
-          break;
-        case 2:
-          hipLaunchKernelGGL((k_sw_spmv_tile<VT, false, true>), grid, dim3(kTileBlock), 0, st, n, Bp, P, groups,
-                             s->row_ptr, uc, tn, li, dl, Kv, Mv, Msv, rp, k2, cf, u, w, ps, S);
-          break;
-        default:
-          hipLaunchKernelGGL((k_sw_spmv_tile<VT, true, true>), grid, dim3(kTileBlock), 0, st, n, Bp, P, groups,
-                             s->row_ptr, uc, tn, li, dl, Kv, Mv, Msv, rp, k2, cf, u, w, ps, S);
-        }
-      } else if (variant == 1) {
-        const dim3 grid(s->gx_pre, 1);
-        const double2 *Kv = s->K.as<double2>(), *Mv = s->M.as<double2>(), *cf = s->coef.as<double2>();
-        const double *Msv = s->Ms.as<double>(), *k2 = s->k0sq.as<double>();
-        const int8_t *rp = s->rport.as<int8_t>();
-        switch (s->cplx & 3) {
-        case 0:
-          hipLaunchKernelGGL((k_sw_spmv_pre<VT, false, false>), grid, dim3(kBlock), 0, st, n, Bp, P, groups,
-                             s->row_ptr, s->col, Kv, Mv, Msv, rp, k2, cf, u, w, ps, S);
-          break;
-        case 1:
-          hipLaunchKernelGGL((k_sw_spmv_pre<VT, true, false>), grid, dim3(kBlock), 0, st, n, Bp, P, groups,
-                             s->row_ptr, s->col, Kv, Mv, Msv, rp, k2, cf, u, w, ps, S);
-          break;
-        case 2:
-          hipLaunchKernelGGL((k_sw_spmv_pre<VT, false, true>), grid, dim3(kBlock), 0, st, n, Bp, P, groups,
-                             s->row_ptr, s->col, Kv, Mv, Msv, rp, k2, cf, u, w, ps, S);
-          break;
-        default:
-          hipLaunchKernelGGL((k_sw_spmv_pre<VT, true, true>), grid, dim3(kBlock), 0, st, n, Bp, P, groups,
-                             s->row_ptr, s->col, Kv, Mv, Msv, rp, k2, cf, u, w, ps, S);
-        }
-      } else
-        hipLaunchKernelGGL((k_sw_spmv<VT>), dim3(s->gx_spmv, s->gy_spmv), dim3(kBlock), 0, st, n, Bp, P, groups,
-                           s->row_ptr, s->col, s->K.as<double2>(), s->M.as<double2>(), s->Ms.as<double>(),
-                           s->rport.as<int8_t>(), s->k0sq.as<double>(), s->coef.as<double2>(), u, w, ps, S);
-      EFM_LAUNCH_CHECK();
-      return EFM_OK;
-    };
+    auto spmv_v = [&](int variant) -> int { return spmv_launch<VT>(variant, u, w); };
     auto finalize = [&](bool first, bool skip, int spmv_variant) -> int {
       // the grids that wrote the partials: the SpMV variant's and the update's
       const int gxs = spmv_variant == 2 ? s->gx_tile : spmv_variant == 1 ? s->gx_pre : s->gx_spmv;
@@ -1860,6 +2009,97 @@ struct SwRun {
     return EFM_OK;
   }
 
+  // Standard preconditioned COCG (see k_sw_pform): the same contract as cocg().
+  template <typename VT> int cocg_std(const double2 *b, double inner_tol2, int max_inner) {
+    const int64_t n = s->n;
+    VT *x = s->x.as<VT>(), *r = s->r.as<VT>(), *p = s->p.as<VT>();
+    VT *sv = s->sv.as<VT>(), *u = s->u.as<VT>(), *w = s->w.as<VT>(), *y = s->y.as<VT>();
+    constexpr int vti = std::is_same<VT, float2>::value ? 1 : 0;
+    const double2 *dK = s->dK.as<double2>(), *dM = s->dM.as<double2>(), *cf = s->coef.as<double2>();
+    const double *dMs = s->dMs.as<double>(), *k2 = s->k0sq.as<double>();
+    const int8_t *rp = s->rport.as<int8_t>();
+    const int2 *en = s->en_eff.as<int2>();
+    hipLaunchKernelGGL((k_sw_init<VT>), dim3(grid_for(n * Bp, kBlock)), dim3(kBlock), 0, st, n, Bp, b, x, r, p,
+                       sv);
+    EFM_LAUNCH_CHECK();
+    hipLaunchKernelGGL(k_sw_dninv, dim3(grid_for(s->N * Bp, kBlock)), dim3(kBlock), 0, st, s->N, Bp, P,
+                       s->dnK.as<double2>(), s->dnM.as<double2>(), s->dnMs.as<double>(), k2, cf,
+                       s->dninv.as<double2>());
+    EFM_LAUNCH_CHECK();
+    auto fin = [&](auto stage, int gx, const double2 *part) -> int {
+      constexpr int STAGE = decltype(stage)::value;
+      hipLaunchKernelGGL((k_sw_fin_std<STAGE>), dim3(Bp), dim3(kBlock), 0, st, Bp, gx, part, S, flags, inner_tol2,
+                         max_it, max_inner);
+      EFM_LAUNCH_CHECK();
+      return EFM_OK;
+    };
+    auto gxs_of = [&](int v) { return v == 2 ? s->gx_tile : v == 1 ? s->gx_pre : s->gx_spmv; };
+    // autotuning in place, as in cocg()
+    const int nvs = s->tiled_ok ? 3 : 2;
+    TuneLog log_s, log_a;
+    auto vs_of = [&](int it) { return s->spmv_variant >= 0 ? s->spmv_variant : it % nvs; };
+    auto va_of = [&](int it) { return s->aux_variant >= 0 ? s->aux_variant : (it >= 0 ? it % 2 : 1); };
+    auto run = [&](TuneLog &lg, bool timed, int v, auto &&launch) -> int {
+      if (!timed)
+        return launch();
+      hipEvent_t e0, e1;
+      EFM_HIP_CHECK(hipEventCreate(&e0));
+      if (hipEventCreate(&e1) != hipSuccess) {
+        (void)hipEventDestroy(e0);
+        return EFM_ERR_HIP;
+      }
+      lg.add(e0, e1, v);
+      EFM_HIP_CHECK(hipEventRecord(e0, st));
+      int rc2 = launch();
+      if (rc2)
+        return rc2;
+      EFM_HIP_CHECK(hipEventRecord(e1, st));
+      return EFM_OK;
+    };
+    // u = P r, rho = u^T r
+    int rc, active = 0;
+    if ((rc = aux_launch<VT, 1>(va_of(-1), r, (const VT *)nullptr, (const VT *)nullptr, y)))
+      return rc;
+    hipLaunchKernelGGL((k_sw_update<VT, true, false>), dim3(s->gx_upd[vti][0], 1), dim3(kBlock), 0, st, n, Bp,
+                       groups, P, en, dK, dM, dMs, rp, k2, cf, y, w, u, p, sv, x, r, pu, S);
+    EFM_LAUNCH_CHECK();
+    if ((rc = fin(std::integral_constant<int, 0>{}, s->gx_upd[vti][0], pu)) || (rc = n_active(active)))
+      return rc;
+    int it = 0;
+    const int cap = std::min(max_it, max_inner);
+    while (active > 0 && it < cap) {
+      const int chunk = std::min(check, cap - it);
+      for (int c = 0; c < chunk; ++c, ++it) {
+        const int va = va_of(it), vs = vs_of(it);
+        hipLaunchKernelGGL((k_sw_pform<VT>), dim3(s->gx_vec, groups), dim3(kBlock), 0, st, n, Bp, (const VT *)u, p,
+                           S);
+        EFM_LAUNCH_CHECK();
+        if ((rc = run(log_s, s->spmv_variant < 0, vs, [&]() { return spmv_launch<VT>(vs, p, w); })) ||
+            (rc = fin(std::integral_constant<int, 1>{}, gxs_of(vs), ps)) ||
+            (rc = run(log_a, s->aux_variant < 0, va,
+                      [&]() { return aux_launch<VT, 2>(va, r, w, (const VT *)nullptr, y); })))
+          return rc;
+        hipLaunchKernelGGL((k_sw_update_std<VT>), dim3(s->gx_std[vti], 1), dim3(kBlock), 0, st, n, Bp, groups, P,
+                           en, dK, dM, dMs, rp, k2, cf, (const VT *)y, (const VT *)w, (const VT *)p, u, x, r, pu,
+                           S);
+        EFM_LAUNCH_CHECK();
+        if ((rc = fin(std::integral_constant<int, 2>{}, s->gx_std[vti], pu)))
+          return rc;
+      }
+      if ((rc = n_active(active)))  // synchronizes the stream: the events are complete
+        return rc;
+      if (s->spmv_variant < 0 && !log_s.empty() && (rc = log_s.pick(&s->spmv_variant, s->tune_ms, nvs)))
+        return rc;
+      if (s->aux_variant < 0 && !log_a.empty() && (rc = log_a.pick(&s->aux_variant, s->tune_ms + 3, 2)))
+        return rc;
+    }
+    return EFM_OK;
+  }
+
+  template <typename VT> int solve_inner(const double2 *b, double inner_tol2, int max_inner) {
+    return s->form == 0 ? cocg_std<VT>(b, inner_tol2, max_inner) : cocg<VT>(b, inner_tol2, max_inner);
+  }
+
   // Iterative refinement: r = b - A x64 in FP64, inner COCG on r with VT vectors to a
   // relative reduction inner_tol, x64 += d; until |b - A x64| <= tol |b| per system.
   template <typename VT> int refine(const double2 *b, double inner_tol2, int max_inner, int max_outer) {
@@ -1882,7 +2122,7 @@ struct SwRun {
     if ((rc = residual(true)) || (rc = n_active(active)))
       return rc;
     for (int k = 0; k < max_outer && active > 0; ++k) {
-      if ((rc = cocg<VT>(nullptr, inner_tol2, max_inner)))
+      if ((rc = solve_inner<VT>(nullptr, inner_tol2, max_inner)))
         return rc;
       hipLaunchKernelGGL((k_sw_accumulate<VT>), dim3(s->gx_vec, groups), dim3(kBlock), 0, st, n, Bp,
                          s->x.as<VT>(), x64, S);
@@ -1960,7 +2200,7 @@ extern "C" int efm_sweep_solve(efm_sweep *s, int nsys, const double *h_k0sq, con
   int rc;
   const double2 *xsrc;
   if (!h_refine) {
-    if ((rc = run.cocg<double2>(Bd, run.tol2, run.max_it)))
+    if ((rc = run.solve_inner<double2>(Bd, run.tol2, run.max_it)))
       return rc;
     xsrc = s->x.as<double2>();
   } else {

@@ -1934,6 +1934,8 @@ def frequency_sweep_eigenmode(mesh: Mesh, params: MaxwellParams, bc: BC, ports: 
     sweep = _capi.SweepSolver(topo, K, M, ops.Ms, dmask, batch)
     if _state is not None and ("spmv_variant" in _state or "aux_variant" in _state):
         sweep.set_variants(int(_state.get("spmv_variant", -1)), int(_state.get("aux_variant", -1)))
+    if _state is not None and "iteration" in _state:
+        sweep.set_iteration(int(_state["iteration"]))
     n = mesh.num_edges()
     tol = 1e-14 if opts.use_direct else float(opts.tolerance)
     infos = []

@@ -426,25 +426,27 @@ def test_full_size_c2_topology_properties(torch_dev):
     assert np.all(np.diff(first) > 0)
 
 
-@pytest.mark.parametrize("precision,max_batch,variant", [("fp64", 256, -1), ("fp64", 256, 0), ("fp64", 256, 1),
-                                                        ("fp64", 256, 2), ("fp64", 100, 1), ("fp64", 100, 2),
-                                                        ("fp64-refined", 256, -1)])
-def test_batched_sweep_matches_lu_golden(torch_dev, precision, max_batch, variant):
+@pytest.mark.parametrize("precision,max_batch,variant,form", [
+    ("fp64", 256, -1, 0), ("fp64", 256, 0, 0), ("fp64", 256, 1, 0), ("fp64", 256, 2, 0), ("fp64", 100, 1, 0),
+    ("fp64", 100, 2, 0), ("fp64-refined", 256, -1, 0),
+    ("fp64", 256, -1, 1), ("fp64", 256, 0, 1), ("fp64", 100, 2, 1), ("fp64-refined", 256, -1, 1)])
+def test_batched_sweep_matches_lu_golden(torch_dev, precision, max_batch, variant, form):
     """128 frequencies x 2 excitations = 256 systems on the 16x8x32 guide, batched, against
     the FP64 SuperLU golden of every point: plain FP64 COCG with each kernel variant of the
     SpMV and the nodal preconditioner part (0: one block slice per system group, 1:
     pipelined, groups in turn, 2: LDS-tiled SpMV with the pipelined nodal kernel, -1:
-    autotuned), also with batches that are not a multiple of the 64 systems of a
-    wavefront (100, 100, 56), and the iterative refinement loop
+    autotuned), in both iteration forms (0 standard COCG, 1 Chronopoulos-Gear), also with
+    batches that are not a multiple of the 64 systems of a wavefront (100, 100, 56), and
+    the iterative refinement loop
     (true FP64 residual b - A x per outer step) around FP64 inner solves.  Complex-FP32
     inner vectors are not tested here: on these indefinite systems their COCG iterations
     stall near a 1e-4..1e-3 relative residual for some frequencies."""
     from edgefem_amd import pyedgefem as em
-    _p(f"[gpu] batched sweep 256 systems precision={precision} max_batch={max_batch} variant={variant}")
+    _p(f"[gpu] batched sweep 256 systems precision={precision} max_batch={max_batch} variant={variant} form={form}")
     pm, bc, ports = _c1_em_setup()
     gold = np.load(golden("c3_sweep.npz"))
     params = em.MaxwellParams()
-    st = {"spmv_variant": variant, "aux_variant": min(variant, 1)}
+    st = {"spmv_variant": variant, "aux_variant": min(variant, 1), "iteration": form}
     opts = em.SolveOptions()
     opts.tolerance = tol = 1e-10
     S = em.frequency_sweep_eigenmode(pm, params, bc, ports, gold["freqs"], precision=precision,
@@ -463,8 +465,9 @@ def test_batched_sweep_matches_lu_golden(torch_dev, precision, max_batch, varian
 
 def test_batched_sweep_lossy_complex_mass_vs_oracle(torch_dev):
     """Batched sweep with a lossy permittivity (complex M, so the SpMV takes its complex-M
-    instantiation) on a 6x3x12 WR-90 section: the three SpMV variants against the oracle
-    (SuperLU) at three frequencies."""
+    instantiation) on a 6x3x12 WR-90 section: the three SpMV variants (standard COCG) and the
+    tiled one with the Chronopoulos-Gear iteration against the oracle (SuperLU) at three
+    frequencies."""
     from edgefem_amd import meshgen as mg
     from edgefem_amd import pyedgefem as em
     from oracle import edgefem_oracle as O
@@ -489,11 +492,11 @@ def test_batched_sweep_lossy_complex_mass_vs_oracle(torch_dev):
     oports = [O.build_wave_port_2d(om, t, kc, Z0, beta, opec, kc2) for t in (2, 3)]
     Sref = np.array([O.sparams_eigenmode(om, O.Params(omega=2 * math.pi * f, eps_tet=eps), opec, oports)
                      for f in freqs])
-    for variant in (0, 1, 2):
-        st = {"spmv_variant": variant, "aux_variant": min(variant, 1)}
+    for variant, form in ((0, 0), (1, 0), (2, 0), (2, 1)):
+        st = {"spmv_variant": variant, "aux_variant": min(variant, 1), "iteration": form}
         S = em.frequency_sweep_eigenmode(pm, params, bc, ports, freqs, _state=st)
         err = np.abs(S - Sref).max()
-        _p(f"    variant {variant}: max |dS| {err:.2e}; |S21| {np.abs(S[:, 1, 0]).round(6)}")
+        _p(f"    variant {variant} form {form}: max |dS| {err:.2e}; |S21| {np.abs(S[:, 1, 0]).round(6)}")
         assert st["spmv_chosen"] == variant and st["aux_chosen"] == min(variant, 1)
         assert err <= 1e-6
         assert np.all(np.abs(S[:, 1, 0]) < 1.0)   # lossy
