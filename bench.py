@@ -486,14 +486,14 @@ def main():
                     help="write profiles/traffic_c2.json from the counter CSVs of two rocprofv3 --pmc runs")
     args = ap.parse_args()
     if args.traffic_from:
-        cmd = ("rocprofv3 --pmc {c} --output-format csv -d gpurun_out/pmc_{d} -o {d} -- python -u bench.py --no-cpu "
+        cmd = ("rocprofv3 --pmc {c} --output-format csv -d pmc_{d} -o {d} -- python -u bench.py --no-cpu "
                "--no-check --steps 1 --warmup 0 --max-it 20 --spmv-variant {v} --calibrate-copy 512")
         v = args.spmv_variant if args.spmv_variant >= 0 else 9
         traffic_from_pmc(args.traffic_from[0], args.traffic_from[1], os.path.join(ROOT, "profiles", "traffic_c2.json"),
                          v, [cmd.format(c="FETCH_SIZE", d="fetch", v=v), cmd.format(c="WRITE_SIZE", d="write", v=v)])
         return
     if args.gpus != 1:
-        print(json.dumps({"error": "this build runs on one GPU; --gpus must be 1 "
+        print(json.dumps({"error": "edgefem_amd runs on one GPU per process; --gpus must be 1 "
                                    "(multi-GPU sharding is a design note only)"}))
         sys.exit(2)
     if args.workload == "sweep":
