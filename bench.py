@@ -242,6 +242,67 @@ def sweep_main(args):
     print(json.dumps(result))
 
 
+def fields_main(args):
+    """Post-solve field evaluation on the 48x24x192 WR-90 guide (1,327,104 tets, 1,593,480
+    edges): E at the centroid and curl E of every tet for 2 solution columns (one step = one
+    efm_tet_fields launch over all tets; the solution columns are seeded random values, the
+    kernel's work does not depend on them).  Roofline on the algorithmic bytes: per tet the
+    node and edge indices and signs (46 B) and the outputs (2 x 2 x 48 B), plus the
+    coordinates and the solution gathered once (N x 24 B, n x 2 x 16 B)."""
+    import numpy as np
+    import torch
+    from edgefem_amd import pyedgefem as em
+    torch.cuda.set_device(0)
+    g, eps, mesh, bc, ports, params = build_workload(*((12, 6, 48, 25e-3) if args.small else (48, 24, 192, 100e-3)))
+    T, n, N, R = mesh.num_tets(), mesh.num_edges(), mesh.num_nodes(), 2
+    rng = np.random.default_rng(3)
+    x = torch.from_numpy(rng.normal(size=(n, R)) + 1j * rng.normal(size=(n, R))).to("cuda")
+    for _ in range(max(args.warmup, 1)):
+        em.tet_fields(mesh, x)
+    torch.cuda.synchronize()
+    e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+    ts = time.perf_counter()
+    e0.record()
+    for _ in range(args.steps):
+        E, C = em.tet_fields(mesh, x)
+    e1.record()
+    torch.cuda.synchronize()
+    el = time.perf_counter() - ts
+    launch_ms = e0.elapsed_time(e1) / args.steps
+    bytes_per = T * 46 + T * R * 96 + N * 24 + n * R * 16
+    achieved = bytes_per / (launch_ms * 1e-3) / 1e9
+    # CPU baseline: the oracle's vectorized evaluation on a bounded sample of tets
+    cpu = None
+    if not args.no_cpu:
+        from oracle import edgefem_oracle as O
+        om = O.mesh_from_arrays(g.node_ids, g.xyz, g.tet_ids, g.tet_phys, g.tri_ids, g.tri_phys)
+        xs = x.cpu().numpy()
+        t0 = time.perf_counter()
+        reps = 0
+        while time.perf_counter() - t0 < 10.0:
+            O.tet_fields(om, xs)
+            reps += 1
+        cpu_s = (time.perf_counter() - t0) / reps
+        cpu = {"value": T / cpu_s, "unit": "tets/s", "cores": 1, "kind": "port",
+               "sample": f"oracle numpy tet_fields over all {T} tets, 2 columns, {reps} run(s), 1 host core"}
+        Eo, Co = O.tet_fields(om, xs)
+        dE = float(np.abs(E.cpu().numpy() - Eo).max() / np.abs(Eo).max())
+    else:
+        dE = None
+    result = {
+        "metric": "tet field evaluations/s (E at the centroid and curl E, 2 columns)", "value": T / (launch_ms * 1e-3),
+        "unit": "tets/s", "n_gpus": 1, "steps": args.steps, "warmup": args.warmup,
+        "ms_per_step": 1e3 * el / args.steps, "higher_is_better": True, "scaling": "weak", "vs_baseline": None,
+        "dtype": "c128", "data": "synthetic (seeded structured Kuhn mesh, seeded solution)",
+        "config": {"workload": "wr90_48x24x192_tet_fields", "tets": T, "edges": n, "columns": R},
+        "roofline": {"bound": "hbm", "achieved": achieved, "peak": 8000.0, "unit": "GB/s", "frac": achieved / 8000.0,
+                     "traffic": None, "kernel": "k_tet_fields", "bytes_per_launch": bytes_per,
+                     "avg_launch_ms": launch_ms},
+        "max_rel_diff_vs_oracle": dE, "cpu_baseline": cpu,
+    }
+    print(json.dumps(result), flush=True)
+
+
 def adjoint_main(args):
     """Adjoint gradient of J = sum_f |S21(f)|^2 over 16 frequencies (9-11 GHz) with respect
     to the permittivity of every tet of a 77-cell design slab: WR-90 guide 48x24x96 hexes,
@@ -454,9 +515,10 @@ def traffic_from_pmc(fetch_csv, write_csv, out_path, variant, commands):
 
 def main():
     ap = argparse.ArgumentParser()
-    ap.add_argument("--workload", choices=["c2", "sweep", "adjoint", "cells"], default="c2",
+    ap.add_argument("--workload", choices=["c2", "sweep", "adjoint", "cells", "fields"], default="c2",
                     help="c2: one large solve (default); sweep: batched 256-system frequency sweep; "
-                         "adjoint: permittivity gradient over 16 frequencies; cells: 512 periodic unit cells")
+                         "adjoint: permittivity gradient over 16 frequencies; cells: 512 periodic unit cells; "
+                         "fields: E and curl E of every tet of the C2 guide")
     ap.add_argument("--cells", type=int, default=512, help="cells: number of unit cells")
     ap.add_argument("--sweep-precision", choices=["fp32", "fp64"], default="fp64",
                     help="sweep: complex-FP32 inner iterations with FP64 refinement, or plain FP64")
@@ -496,6 +558,9 @@ def main():
         print(json.dumps({"error": "edgefem_amd runs on one GPU per process; --gpus must be 1 "
                                    "(multi-GPU sharding is a design note only)"}))
         sys.exit(2)
+    if args.workload == "fields":
+        fields_main(args)
+        return
     if args.workload == "sweep":
         sweep_main(args)
         return

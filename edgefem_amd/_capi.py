@@ -33,7 +33,7 @@ EXPORTS = [
     "efm_sweep_create", "efm_sweep_destroy", "efm_sweep_solve", "efm_eps_gradient",
     "efm_solver_set_lowrank", "efm_add_diagonal", "efm_periodic_reduce", "efm_solver_set_gradient_complex",
     "efm_cells_create", "efm_cells_destroy", "efm_cells_solve",
-    "efm_sweep_set_variants", "efm_sweep_get_variants", "efm_sweep_set_iteration",
+    "efm_sweep_set_variants", "efm_sweep_get_variants", "efm_sweep_set_iteration", "efm_tet_fields",
 ]
 
 
@@ -117,6 +117,7 @@ def load_library(path: str = LIB_PATH) -> C.CDLL:
     lib.efm_sweep_destroy.restype = None
     lib.efm_sweep_set_variants.argtypes = [vp, i32, i32]
     lib.efm_sweep_set_iteration.argtypes = [vp, i32]
+    lib.efm_tet_fields.argtypes = [vp, vp, i32, vp, vp, vp, vp, vp]
     lib.efm_sweep_get_variants.argtypes = [vp, C.POINTER(C.c_int), C.POINTER(C.c_int), vp]
     lib.efm_sweep_solve.argtypes = [vp, i32, vp, vp, vp, vp, C.POINTER(SolveOpts), C.POINTER(RefineOpts), vp, vp,
                                     vp]
@@ -503,6 +504,30 @@ class SweepSolver:
             self.close()
         except Exception:
             pass
+
+
+def tet_fields(topo: Topology, xyz, X, lam=None, with_curl: bool = True):
+    """E at a point of every tet (barycentric weights lam (T, 4), None: the centroid) and
+    curl E (efm_tet_fields).  X: device complex128 (n,) or (n, R), R in 1, 2, 3, 4, 8.
+    Returns (E, curl) as complex128 device tensors (T, R, 3); curl is None without
+    with_curl."""
+    torch = _torch()
+    lib = load_library()
+    Xd = to_dev(X, torch.complex128)
+    R = Xd.shape[1] if Xd.dim() == 2 else 1
+    if Xd.shape[0] != topo.n_edges:
+        raise EdgeFemError("solution length differs from the number of edges")
+    T = topo.n_tets
+    ld = None
+    if lam is not None:
+        ld = to_dev(lam, torch.float64)
+        if tuple(ld.shape) != (T, 4):
+            raise EdgeFemError("barycentric weights must be (n_tets, 4)")
+    E = torch.empty((T, R, 3), dtype=torch.complex128, device=Xd.device)
+    C = torch.empty((T, R, 3), dtype=torch.complex128, device=Xd.device) if with_curl else None
+    _check(lib.efm_tet_fields(topo.handle, dptr(xyz), int(R), dptr(Xd), dptr(ld), dptr(E), dptr(C), stream_ptr()),
+           "efm_tet_fields")
+    return E, C
 
 
 def eps_gradient(topo: Topology, xyz, design, Lam, X, terms_l, terms_r, weights):

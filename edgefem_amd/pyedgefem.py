@@ -1971,3 +1971,45 @@ def frequency_sweep_eigenmode(mesh: Mesh, params: MaxwellParams, bc: BC, ports: 
         v, a, tune_ms = sweep.variants()
         _state.update(dict(infos=infos, sweep=sweep, ops=ops, spmv_chosen=v, aux_chosen=a, tune_ms=tune_ms))
     return out
+
+
+# ---------------------------------------------------------------------------
+# Post-solve field evaluation
+# ---------------------------------------------------------------------------
+_TET_LOCAL_EDGES = ((0, 1), (0, 2), (0, 3), (1, 2), (1, 3), (2, 3))
+
+
+def compute_barycentric(vertices, point) -> np.ndarray:
+    """Barycentric coordinates of a point in a tet (edge_basis.cpp:132-151): the 3x3 system
+    [v0-v3, v1-v3, v2-v3] l = p - v3, l3 = 1 - l0 - l1 - l2.  A single point on the host,
+    as the reference's Python binding offers it; whole meshes go through tet_fields."""
+    v = np.asarray(vertices, np.float64).reshape(4, 3)
+    m = np.stack([v[0] - v[3], v[1] - v[3], v[2] - v[3]], axis=1)
+    l3 = np.linalg.solve(m, np.asarray(point, np.float64) - v[3])
+    return np.append(l3, 1.0 - l3.sum())
+
+
+def evaluate_edge_field(vertices, edge_orient, edge_dofs, point) -> np.ndarray:
+    """E at one point of one tet from its six edge values (edge_basis.cpp:162-192): the sum
+    over local edges (a, b) of orient * dof * (l_a grad l_b - l_b grad l_a).  Host, one
+    point; tet_fields evaluates every tet of a mesh on the GPU."""
+    v = np.asarray(vertices, np.float64).reshape(4, 3)
+    lam = compute_barycentric(v, point)
+    m = np.stack([v[0] - v[3], v[1] - v[3], v[2] - v[3]], axis=1)
+    g = np.empty((4, 3))
+    g[:3] = np.linalg.inv(m)            # row i of m^{-1} is grad l_i
+    g[3] = -g[:3].sum(axis=0)
+    E = np.zeros(3, np.complex128)
+    dofs = np.asarray(edge_dofs, np.complex128).reshape(6)
+    for e, (a, b) in enumerate(_TET_LOCAL_EDGES):
+        E += dofs[e] * float(edge_orient[e]) * (lam[a] * g[b] - lam[b] * g[a])
+    return E
+
+
+def tet_fields(mesh: Mesh, x, lam=None, with_curl: bool = True):
+    """E at one point of every tet and the tet-constant curl E from an edge solution x (n,)
+    or (n, R), on the GPU (efm_tet_fields): the per-tet loops of the VTK export
+    (vtk_export.cpp:14-40) and the Huygens surface (huygens_surface.cpp:95-125) in one
+    launch.  lam: (T, 4) barycentric weights of the point in each tet, None for the
+    centroid.  Returns (E, curl) as complex128 device tensors (T, R, 3)."""
+    return _capi.tet_fields(mesh.topology, mesh.dev_xyz, x, lam, with_curl)

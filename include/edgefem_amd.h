@@ -334,6 +334,23 @@ int efm_cells_solve(efm_cells *cells, const double *B, double *X, const efm_solv
  * With A complex-symmetric, lam_j = A^{-1} conj(g_j); the caller folds k0^2, the loss
  * weights and the lam/x scaling into h_w.  Synchronises the stream.
  * ---------------------------------------------------------------------- */
+/* ------------------------------------------------------------------------
+ * Post-solve field evaluation.  Replaces the per-point evaluate_edge_field
+ * (src/edge_basis.cpp:162-192; compute_barycentric 132-151) in the loops over all tets of
+ * export_fields_vtk (src/io/vtk_export.cpp:14-40, E at the centroid) and
+ * extract_huygens_surface (src/post/huygens_surface.cpp:95-125, E at the centroid and
+ * curl E from whitney_edge_curls, src/edge_basis.cpp:34-45).  For every tet t and system
+ * s < nrhs (1, 2, 3, 4 or 8):
+ *   E[t][s]    = sum_e sign_e x[edge_e][s] (l_a grad l_b - l_b grad l_a) at the point whose
+ *                barycentric weights are lam[t][0..3] (lam NULL: the centroid)
+ *   curl[t][s] = sum_e sign_e x[edge_e][s] 2 grad l_a x grad l_b  (constant on the tet)
+ * xyz: device double [N][3] in the topology's node order; x: device c128 [n][nrhs];
+ * lam: device double [T][4] or NULL; E, curl: device c128 [T][nrhs][3] (curl may be NULL).
+ * Stream-ordered; no synchronisation.
+ * ---------------------------------------------------------------------- */
+int efm_tet_fields(const efm_topology *topo, const double *xyz, int nrhs, const double *x,
+                   const double *lam, double *E, double *curl, void *stream);
+
 int efm_eps_gradient(const efm_topology *topo, const double *xyz, int64_t n_design,
                      const int32_t *design, int nl, const double *Lam, int nx, const double *X,
                      int n_terms, const int32_t *h_term_l, const int32_t *h_term_r,

@@ -28,6 +28,10 @@ assemble_maxwell_periodic   src/assemble_maxwell.cpp:496-574 (sparse restatement
                             periodic_eliminate_dense is the literal dense one)
 calculate_sparams_periodic  src/assemble_maxwell.cpp:576-635
 port_abc_coefficient        src/assemble_maxwell.cpp:274-320
+compute_barycentric         src/edge_basis.cpp:132-151
+evaluate_edge_field         src/edge_basis.cpp:162-192
+tet_fields                  the per-tet loop of src/io/vtk_export.cpp:14-40 and
+                            src/post/huygens_surface.cpp:95-125 (E at a point, curl E)
 build_lumped_port           src/ports/lumped_port.cpp:21-160
 normalize_port_weights      src/assemble_maxwell.cpp:396-494
 eps_gradient_eigenmode      adjoint of sparams_eigenmode w.r.t. per-tet eps_r (no
@@ -222,6 +226,50 @@ def whitney_mass_matrix(X: np.ndarray) -> np.ndarray:
     M = (G[:, b, d] * I[:, a, c] - G[:, b, c] * I[:, a, d]
          - G[:, a, d] * I[:, b, c] + G[:, a, c] * I[:, b, d])
     return M
+
+
+def compute_barycentric(v: np.ndarray, p: np.ndarray) -> np.ndarray:
+    """Barycentric coordinates of p in the tet v (4,3): solve [v0-v3, v1-v3, v2-v3] l = p - v3,
+    lambda_3 = 1 - sum (edge_basis.cpp:132-151)."""
+    v = np.asarray(v, np.float64)
+    T = np.stack([v[0] - v[3], v[1] - v[3], v[2] - v[3]], axis=1)
+    l012 = np.linalg.solve(T, np.asarray(p, np.float64) - v[3])
+    return np.array([l012[0], l012[1], l012[2], 1.0 - l012.sum()])
+
+
+def evaluate_edge_field(v: np.ndarray, edge_orient, edge_dofs, p: np.ndarray) -> np.ndarray:
+    """E(p) = sum_e s_e x_e (l_a grad l_b - l_b grad l_a) over the local edges (a, b) in
+    TET_PAIRS order (edge_basis.cpp:162-192)."""
+    v = np.asarray(v, np.float64)
+    lam = compute_barycentric(v, p)
+    g, _ = gradients_and_volume(v[None])
+    g = g[0]
+    E = np.zeros(3, np.complex128)
+    for e, (a, b) in enumerate(TET_PAIRS):
+        W = lam[a] * g[b] - lam[b] * g[a]
+        E += complex(edge_dofs[e]) * float(edge_orient[e]) * W
+    return E
+
+
+def tet_fields(mesh: OMesh, x: np.ndarray, lam: Optional[np.ndarray] = None):
+    """E at a point of every tet (lam (T,4) barycentric weights; None: the centroid, as the
+    VTK export and the Huygens surface evaluate it) and the tet-constant curl E =
+    sum_e s_e x_e 2 grad l_a x grad l_b (whitney_edge_curls, edge_basis.cpp:34-45).
+    x (n,) or (n, R).  Returns E, curl: (T, R, 3) complex (R = 1 for a vector x)."""
+    X = mesh.xyz[np.vectorize(mesh.node_index.__getitem__)(mesh.tet_ids)]      # (T,4,3)
+    g, _ = gradients_and_volume(X)
+    xs = np.asarray(x, np.complex128)
+    if xs.ndim == 1:
+        xs = xs[:, None]
+    c = xs[mesh.tet_edges] * mesh.tet_sign[:, :, None]                          # (T,6,R)
+    if lam is None:
+        lam = np.full((X.shape[0], 4), 0.25)
+    a, b = TET_PAIRS[:, 0], TET_PAIRS[:, 1]
+    W = lam[:, a, None] * g[:, b] - lam[:, b, None] * g[:, a]                    # (T,6,3)
+    curls = 2.0 * np.cross(g[:, a], g[:, b])                                    # (T,6,3)
+    E = np.einsum("ter,tek->trk", c, W)
+    curl = np.einsum("ter,tek->trk", c, curls)
+    return E, curl
 
 
 def triangle_whitney_mass(v: np.ndarray) -> np.ndarray:

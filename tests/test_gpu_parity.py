@@ -853,3 +853,41 @@ def test_periodic_cell_batch_vs_single_and_oracle(torch_dev):
         Ss = em.calculate_sparams_periodic(pm, params, bc, pbc, ports)
         assert np.abs(S - So).max() <= 1e-6
         assert np.abs(S - Ss).max() <= 1e-6
+
+
+def test_tet_fields_vs_oracle(torch_dev):
+    """efm_tet_fields on the C1 guide with 2 RHS: E at the centroid and at a random point
+    of every tet and curl E against the oracle (1e-12 of the largest value), and an affine
+    field a + b x r (its exact edge integrals as the solution) reproduced at every centroid
+    with curl 2b."""
+    import torch
+    from edgefem_amd import meshgen as mg
+    from edgefem_amd import pyedgefem as em
+    from oracle import edgefem_oracle as O
+    _p("[gpu] tet fields c1 2 RHS")
+    g = mg.wr90_guide()
+    pm = em.mesh_from_arrays(g.node_ids, g.xyz, g.tets, g.tet_phys, g.tris, g.tri_phys)
+    om = _oracle_mesh_from_gen(g)
+    rng = np.random.default_rng(21)
+    n = pm.num_edges()
+    x = rng.normal(size=(n, 2)) + 1j * rng.normal(size=(n, 2))
+    lam = rng.dirichlet(np.ones(4), size=pm.num_tets())
+    for l in (None, lam):
+        E, C = em.tet_fields(pm, torch.from_numpy(x).to("cuda"), None if l is None else torch.from_numpy(l))
+        Eo, Co = O.tet_fields(om, x, l)
+        assert np.abs(E.cpu().numpy() - Eo).max() <= 1e-12 * np.abs(Eo).max()
+        assert np.abs(C.cpu().numpy() - Co).max() <= 1e-12 * np.abs(Co).max()
+    a = np.array([1.0 + 2j, -0.5 + 0.1j, 0.3 - 1j])
+    b = np.array([20.0 - 5j, 7.0 + 3j, -11.0 + 2j])
+    pos = np.vectorize(om.node_index.__getitem__)
+    p0, p1 = om.xyz[pos(om.edges[:, 0])], om.xyz[pos(om.edges[:, 1])]
+    mid = 0.5 * (p0 + p1)
+    Emid = a[None] + np.cross(b.real[None], mid) + 1j * np.cross(b.imag[None], mid)
+    xa = np.einsum("ek,ek->e", Emid, p1 - p0)
+    E, C = em.tet_fields(pm, torch.from_numpy(xa).to("cuda"))
+    cen = om.xyz[pos(om.tet_ids)].mean(axis=1)
+    ref = a[None] + np.cross(b.real[None], cen) + 1j * np.cross(b.imag[None], cen)
+    err = np.abs(E.cpu().numpy()[:, 0] - ref).max() / np.abs(ref).max()
+    cerr = np.abs(C.cpu().numpy()[:, 0] - 2 * b[None]).max() / np.abs(b).max()
+    _p(f"    affine field: max rel error E {err:.1e}, curl {cerr:.1e}")
+    assert err <= 1e-12 and cerr <= 1e-9
