@@ -34,6 +34,7 @@ EXPORTS = [
     "efm_solver_set_lowrank", "efm_add_diagonal", "efm_periodic_reduce", "efm_solver_set_gradient_complex",
     "efm_cells_create", "efm_cells_destroy", "efm_cells_solve",
     "efm_sweep_set_variants", "efm_sweep_get_variants", "efm_sweep_set_iteration", "efm_tet_fields",
+    "efm_gmsh_read", "efm_gmsh_sizes", "efm_gmsh_copy", "efm_gmsh_free",
 ]
 
 
@@ -118,6 +119,11 @@ def load_library(path: str = LIB_PATH) -> C.CDLL:
     lib.efm_sweep_set_variants.argtypes = [vp, i32, i32]
     lib.efm_sweep_set_iteration.argtypes = [vp, i32]
     lib.efm_tet_fields.argtypes = [vp, vp, i32, vp, vp, vp, vp, vp]
+    lib.efm_gmsh_read.argtypes = [C.c_char_p, i32, C.POINTER(vp)]
+    lib.efm_gmsh_sizes.argtypes = [vp, vp]
+    lib.efm_gmsh_copy.argtypes = [vp] * 11
+    lib.efm_gmsh_free.argtypes = [vp]
+    lib.efm_gmsh_free.restype = None
     lib.efm_sweep_get_variants.argtypes = [vp, C.POINTER(C.c_int), C.POINTER(C.c_int), vp]
     lib.efm_sweep_solve.argtypes = [vp, i32, vp, vp, vp, vp, C.POINTER(SolveOpts), C.POINTER(RefineOpts), vp, vp,
                                     vp]
@@ -504,6 +510,29 @@ class SweepSolver:
             self.close()
         except Exception:
             pass
+
+
+def gmsh_read(path: str, n_threads: int = 0) -> dict:
+    """gmsh v2.2 ASCII $Nodes/$Elements (efm_gmsh_read, host code, n_threads 0: up to 16).
+    Returns host arrays: node_ids (N,), xyz (N,3), tet_ids (T,4) / tri_ids (F,3) / line_ids
+    (L,2) as gmsh node IDs, tet_phys, tri_phys, line_phys, tet_elem, tri_elem."""
+    lib = load_library()
+    h = C.c_void_p()
+    _check(lib.efm_gmsh_read(str(path).encode(), int(n_threads), C.byref(h)), "efm_gmsh_read")
+    try:
+        sz = np.zeros(4, np.int64)
+        _check(lib.efm_gmsh_sizes(h, sz.ctypes.data_as(C.c_void_p)), "efm_gmsh_sizes")
+        N, T, F, L = (int(v) for v in sz)
+        out = dict(node_ids=np.empty(N, np.int64), xyz=np.empty((N, 3)), tet_ids=np.empty((T, 4), np.int64),
+                   tet_phys=np.empty(T, np.int32), tet_elem=np.empty(T, np.int64), tri_ids=np.empty((F, 3), np.int64),
+                   tri_phys=np.empty(F, np.int32), tri_elem=np.empty(F, np.int64),
+                   line_ids=np.empty((L, 2), np.int64), line_phys=np.empty(L, np.int32))
+        order = ("node_ids", "xyz", "tet_ids", "tet_phys", "tet_elem", "tri_ids", "tri_phys", "tri_elem",
+                 "line_ids", "line_phys")
+        _check(lib.efm_gmsh_copy(h, *[out[k].ctypes.data_as(C.c_void_p) for k in order]), "efm_gmsh_copy")
+    finally:
+        lib.efm_gmsh_free(h)
+    return out
 
 
 def tet_fields(topo: Topology, xyz, X, lam=None, with_curl: bool = True):

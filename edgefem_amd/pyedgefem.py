@@ -26,6 +26,7 @@ from __future__ import annotations
 import cmath
 import enum
 import math
+import os
 import sys
 import time
 from typing import Dict, Iterable, List, Optional, Sequence
@@ -209,46 +210,16 @@ def _ids_to_index(node_ids: np.ndarray, conn_ids: np.ndarray) -> np.ndarray:
 def load_gmsh(path: str) -> Mesh:
     """gmsh v2.2 ASCII loader (mesh_gmsh.cpp:15-101): $Nodes ``id x y z``; $Elements keep
     types 2 (Tri3) and 4 (Tet4) with the first tag as ``phys``; type 1 lines go to
-    ``boundary_lines``; other types are skipped.  Edges are numbered at load time, on the
-    GPU, with the reference's first-seen rule."""
-    try:
-        with open(path) as f:
-            text = f.read()
-    except OSError:
+    ``boundary_lines``; other types are skipped.  The file is parsed by the library's
+    threaded reader (efm_gmsh_read, arrays in file order); edges are numbered at load time,
+    on the GPU, with the reference's first-seen rule."""
+    if not os.path.isfile(path):
         raise RuntimeError("Failed to open mesh file: " + path)
-    lines = text.split("\n")
-    node_ids = np.zeros(0, np.int64)
-    xyz = np.zeros((0, 3))
-    tets, tetp, tete, tris, trip, trie, blines = [], [], [], [], [], [], []
-    i = 0
-    while i < len(lines):
-        line = lines[i].strip()
-        if line == "$Nodes":
-            n = int(lines[i + 1].split()[0])
-            block = np.array(" ".join(lines[i + 2:i + 2 + n]).split(), dtype=np.float64).reshape(n, 4)
-            node_ids = block[:, 0].astype(np.int64)
-            xyz = block[:, 1:4].copy()
-            i += 2 + n
-        elif line == "$Elements":
-            m = int(lines[i + 1].split()[0])
-            for r in range(m):
-                tok = lines[i + 2 + r].split()
-                eid, typ, ntag = int(tok[0]), int(tok[1]), int(tok[2])
-                phys = int(tok[3]) if ntag > 0 else 0
-                rest = tok[3 + ntag:]
-                if typ == 2:
-                    tris.append([int(v) for v in rest[:3]]); trip.append(phys); trie.append(eid)
-                elif typ == 4:
-                    tets.append([int(v) for v in rest[:4]]); tetp.append(phys); tete.append(eid)
-                elif typ == 1:
-                    blines.append((int(rest[0]), int(rest[1]), phys))
-            i += 2 + m
-        else:
-            i += 1
-    tet_ids = np.asarray(tets, np.int64).reshape(-1, 4)
-    tri_ids = np.asarray(tris, np.int64).reshape(-1, 3)
-    return Mesh(node_ids, xyz, _ids_to_index(node_ids, tet_ids), tetp, _ids_to_index(node_ids, tri_ids), trip,
-                tet_elem_ids=np.asarray(tete), tri_elem_ids=np.asarray(trie), boundary_lines=blines)
+    d = _capi.gmsh_read(path)
+    blines = [(int(a), int(b), int(p)) for (a, b), p in zip(d["line_ids"], d["line_phys"])]
+    return Mesh(d["node_ids"], d["xyz"], _ids_to_index(d["node_ids"], d["tet_ids"]), d["tet_phys"],
+                _ids_to_index(d["node_ids"], d["tri_ids"]), d["tri_phys"], tet_elem_ids=d["tet_elem"],
+                tri_elem_ids=d["tri_elem"], boundary_lines=blines)
 
 
 load_gmsh_v2 = load_gmsh

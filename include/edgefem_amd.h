@@ -334,6 +334,11 @@ int efm_cells_solve(efm_cells *cells, const double *B, double *X, const efm_solv
  * With A complex-symmetric, lam_j = A^{-1} conj(g_j); the caller folds k0^2, the loss
  * weights and the lam/x scaling into h_w.  Synchronises the stream.
  * ---------------------------------------------------------------------- */
+int efm_eps_gradient(const efm_topology *topo, const double *xyz, int64_t n_design,
+                     const int32_t *design, int nl, const double *Lam, int nx, const double *X,
+                     int n_terms, const int32_t *h_term_l, const int32_t *h_term_r,
+                     const double *h_w, double *grad, void *stream);
+
 /* ------------------------------------------------------------------------
  * Post-solve field evaluation.  Replaces the per-point evaluate_edge_field
  * (src/edge_basis.cpp:162-192; compute_barycentric 132-151) in the loops over all tets of
@@ -351,10 +356,29 @@ int efm_cells_solve(efm_cells *cells, const double *B, double *X, const efm_solv
 int efm_tet_fields(const efm_topology *topo, const double *xyz, int nrhs, const double *x,
                    const double *lam, double *E, double *curl, void *stream);
 
-int efm_eps_gradient(const efm_topology *topo, const double *xyz, int64_t n_design,
-                     const int32_t *design, int nl, const double *Lam, int nx, const double *X,
-                     int n_terms, const int32_t *h_term_l, const int32_t *h_term_r,
-                     const double *h_w, double *grad, void *stream);
+/* ------------------------------------------------------------------------
+ * gmsh v2.2 ASCII reader (host code, no device memory).  Replaces the token loop of
+ * load_gmsh_v2 (src/mesh_gmsh.cpp:15-101): $Nodes records `id x y z`; $Elements records
+ * `id type ntags tags... nodes...` with type 4 (Tet4), 2 (Tri3) and 1 (Line2) kept, the
+ * first tag as the physical group, other types skipped.  The sections are split at line
+ * boundaries and parsed on n_threads host threads (0: up to 16); every array is in file
+ * order.  Fails with "Failed to open mesh file: <path>" like the reference, and on a
+ * malformed record or a section count that differs from its records.
+ * Element connectivity is returned as gmsh node IDs; edges are numbered by
+ * efm_topology_create after the IDs are mapped to node positions.
+ * ---------------------------------------------------------------------- */
+typedef struct efm_gmsh efm_gmsh;
+
+int efm_gmsh_read(const char *path, int n_threads, efm_gmsh **out);
+/* h_sizes[4] = nodes, tets, tris, lines */
+int efm_gmsh_sizes(const efm_gmsh *m, int64_t *h_sizes);
+/* Host arrays sized from efm_gmsh_sizes; any pointer may be NULL to skip it.
+ * node_ids [N], xyz [N][3], tet_nodes [T][4], tet_phys [T], tet_elem [T] (element IDs),
+ * tri_nodes [F][3], tri_phys [F], tri_elem [F], line_nodes [L][2], line_phys [L]. */
+int efm_gmsh_copy(const efm_gmsh *m, int64_t *node_ids, double *xyz, int64_t *tet_nodes,
+                  int32_t *tet_phys, int64_t *tet_elem, int64_t *tri_nodes, int32_t *tri_phys,
+                  int64_t *tri_elem, int64_t *line_nodes, int32_t *line_phys);
+void efm_gmsh_free(efm_gmsh *m);
 
 #ifdef __cplusplus
 }

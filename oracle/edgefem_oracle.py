@@ -114,37 +114,57 @@ class OMesh:
 
 
 def load_gmsh_v2(path: str) -> OMesh:
-    """Token-stream parse of gmsh v2.2 ASCII (mesh_gmsh.cpp:15-101): $Nodes gives
-    ``id x y z``; $Elements keeps types 2 (Tri3) and 4 (Tet4) with the first tag as
-    ``phys``; type 1 lines are read and dropped; other types skip the rest of the line."""
-    with open(path) as f:
-        lines = f.read().split("\n")
+    """gmsh v2.2 ASCII (mesh_gmsh.cpp:15-101), as the reference's stream reads it: whole
+    lines are compared with "$Nodes" / "$Elements"; inside a section every number is the
+    next whitespace-separated token (blank lines included); $Nodes gives ``id x y z``;
+    $Elements records ``id type ntags tags... nodes...`` keep types 2 (Tri3) and 4 (Tet4)
+    with the first tag as ``phys`` (0 without tags); type 1 (Line2) reads its two nodes and
+    drops them; any other type skips the rest of its line; after each section two lines
+    are consumed (the end of the last record and the closing marker)."""
+    with open(path, newline="") as f:
+        text = f.read()
+    pos, end = 0, len(text)
+
+    def token() -> str:
+        nonlocal pos
+        while pos < end and text[pos].isspace():
+            pos += 1
+        s = pos
+        while pos < end and not text[pos].isspace():
+            pos += 1
+        return text[s:pos]
+
+    def getline() -> str:
+        nonlocal pos
+        e = text.find("\n", pos)
+        e = end if e < 0 else e
+        line, pos = text[pos:e], e + 1
+        return line
+
     node_ids, xyz, tets, tetp, tris, trip = [], [], [], [], [], []
-    i = 0
-    while i < len(lines):
-        line = lines[i]
+    while pos < end:
+        line = getline()
         if line == "$Nodes":
-            n = int(lines[i + 1].split()[0])
-            block = np.array(" ".join(lines[i + 2:i + 2 + n]).split(), dtype=np.float64).reshape(n, 4)
-            node_ids = block[:, 0].astype(np.int64)
-            xyz = block[:, 1:4].copy()
-            i += 2 + n
+            for _ in range(int(token())):
+                node_ids.append(int(token()))
+                xyz.append([float(token()), float(token()), float(token())])
+            getline(), getline()
         elif line == "$Elements":
-            m = int(lines[i + 1].split()[0])
-            for r in range(m):
-                tok = lines[i + 2 + r].split()
-                typ = int(tok[1])
-                ntag = int(tok[2])
-                phys = int(tok[3]) if ntag > 0 else 0
-                rest = tok[3 + ntag:]
+            for _ in range(int(token())):
+                token()  # element ID
+                typ, ntag = int(token()), int(token())
+                tags = [int(token()) for _ in range(ntag)]
+                phys = tags[0] if tags else 0
                 if typ == 2:
-                    tris.append([int(v) for v in rest[:3]]); trip.append(phys)
+                    tris.append([int(token()) for _ in range(3)]); trip.append(phys)
+                elif typ == 1:
+                    token(), token()
                 elif typ == 4:
-                    tets.append([int(v) for v in rest[:4]]); tetp.append(phys)
-            i += 2 + m
-        else:
-            i += 1
-    mesh = OMesh(node_ids=np.asarray(node_ids, np.int64), xyz=np.asarray(xyz, float),
+                    tets.append([int(token()) for _ in range(4)]); tetp.append(phys)
+                else:
+                    getline()
+            getline(), getline()
+    mesh = OMesh(node_ids=np.asarray(node_ids, np.int64), xyz=np.asarray(xyz, float).reshape(-1, 3),
                  tet_ids=np.asarray(tets, np.int64).reshape(-1, 4), tet_phys=np.asarray(tetp, np.int32),
                  tri_ids=np.asarray(tris, np.int64).reshape(-1, 3), tri_phys=np.asarray(trip, np.int32))
     mesh.node_index = {int(v): k for k, v in enumerate(mesh.node_ids)}
