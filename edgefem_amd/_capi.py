@@ -35,6 +35,7 @@ EXPORTS = [
     "efm_cells_create", "efm_cells_destroy", "efm_cells_solve",
     "efm_sweep_set_variants", "efm_sweep_get_variants", "efm_sweep_set_iteration", "efm_tet_fields",
     "efm_gmsh_read", "efm_gmsh_sizes", "efm_gmsh_copy", "efm_gmsh_free",
+    "efm_touchstone_write", "efm_touchstone_write_2port", "efm_touchstone_extension",
 ]
 
 
@@ -124,6 +125,9 @@ def load_library(path: str = LIB_PATH) -> C.CDLL:
     lib.efm_gmsh_copy.argtypes = [vp] * 11
     lib.efm_gmsh_free.argtypes = [vp]
     lib.efm_gmsh_free.restype = None
+    lib.efm_touchstone_write.argtypes = [C.c_char_p, i64, vp, i32, vp, i32, C.c_double]
+    lib.efm_touchstone_write_2port.argtypes = [C.c_char_p, i64, vp, vp]
+    lib.efm_touchstone_extension.argtypes = [i32, C.c_char_p, i32]
     lib.efm_sweep_get_variants.argtypes = [vp, C.POINTER(C.c_int), C.POINTER(C.c_int), vp]
     lib.efm_sweep_solve.argtypes = [vp, i32, vp, vp, vp, vp, C.POINTER(SolveOpts), C.POINTER(RefineOpts), vp, vp,
                                     vp]
@@ -533,6 +537,32 @@ def gmsh_read(path: str, n_threads: int = 0) -> dict:
     finally:
         lib.efm_gmsh_free(h)
     return out
+
+
+def touchstone_write(path: str, freq, S, fmt: int, z0: float) -> None:
+    """efm_touchstone_write: freq (F,) float64, S (F, P, P) complex128, host arrays."""
+    lib = load_library()
+    f = np.ascontiguousarray(freq, np.float64).reshape(-1)
+    s = np.ascontiguousarray(S, np.complex128)
+    P = s.shape[1] if s.ndim == 3 else 0
+    _check(lib.efm_touchstone_write(str(path).encode(), f.size, f.ctypes.data_as(C.c_void_p), P,
+                                    s.ctypes.data_as(C.c_void_p), int(fmt), float(z0)), "efm_touchstone_write")
+
+
+def touchstone_write_2port(path: str, freq, S4) -> None:
+    """efm_touchstone_write_2port: S4 (F, 4) complex128 as s11, s21, s12, s22."""
+    lib = load_library()
+    f = np.ascontiguousarray(freq, np.float64).reshape(-1)
+    s = np.ascontiguousarray(S4, np.complex128).reshape(-1, 4)
+    _check(lib.efm_touchstone_write_2port(str(path).encode(), f.size, f.ctypes.data_as(C.c_void_p),
+                                          s.ctypes.data_as(C.c_void_p)), "efm_touchstone_write_2port")
+
+
+def touchstone_extension(num_ports: int) -> str:
+    lib = load_library()
+    buf = C.create_string_buffer(16)
+    _check(lib.efm_touchstone_extension(int(num_ports), buf, 16), "efm_touchstone_extension")
+    return buf.value.decode()
 
 
 def tet_fields(topo: Topology, xyz, X, lam=None, with_curl: bool = True):

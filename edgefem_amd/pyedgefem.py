@@ -1984,3 +1984,69 @@ def tet_fields(mesh: Mesh, x, lam=None, with_curl: bool = True):
     launch.  lam: (T, 4) barycentric weights of the point in each tet, None for the
     centroid.  Returns (E, curl) as complex128 device tensors (T, R, 3)."""
     return _capi.tet_fields(mesh.topology, mesh.dev_xyz, x, lam, with_curl)
+
+
+# ---------------------------------------------------------------------------
+# Touchstone export (src/io/touchstone.cpp, bindings python/pyedgefem.cpp:623-656)
+# ---------------------------------------------------------------------------
+
+
+class TouchstoneFormat(enum.IntEnum):
+    """Touchstone data format (include/edgefem/io/touchstone.hpp:14-18)."""
+    RI = 0  # real / imaginary
+    MA = 1  # magnitude / angle (degrees)
+    DB = 2  # dB / angle (degrees)
+
+
+class TouchstoneOptions:
+    """Touchstone export options (touchstone.hpp:21-24): ``format`` and ``z0`` (ohms)."""
+
+    def __init__(self):
+        self.format = TouchstoneFormat.RI
+        self.z0 = 50.0
+
+
+class SParams2:
+    """2-port S-parameters of one frequency, for the legacy ``write_touchstone``."""
+
+    def __init__(self):
+        self.s11 = self.s21 = self.s12 = self.s22 = 0j
+
+
+def write_touchstone(path: str, freq, data) -> None:
+    """Legacy 2-port Touchstone writer (touchstone.cpp:50-62): option line ``# Hz S RI R 50``,
+    then ``freq s11 s21 s12 s22`` as real/imaginary pairs, one line per frequency."""
+    freq = np.asarray(freq, np.float64).reshape(-1)
+    if len(data) < freq.size:
+        raise RuntimeError("write_touchstone: fewer S-parameter entries than frequencies")
+    S4 = np.array([[d.s11, d.s21, d.s12, d.s22] for d in list(data)[:freq.size]], np.complex128).reshape(-1, 4)
+    _capi.touchstone_write_2port(path, freq, S4)
+
+
+def write_touchstone_nport(path: str, freq, S_matrices, opts: Optional[TouchstoneOptions] = None) -> None:
+    """N-port Touchstone writer (touchstone.cpp:68-135); the file is formatted and written by
+    the library (efm_touchstone_write).  ``S_matrices``: a sequence of (P, P) complex
+    matrices or an (F, P, P) array, e.g. the result of ``frequency_sweep_eigenmode``.
+    Raises RuntimeError with the reference's messages, checked in its order."""
+    opts = TouchstoneOptions() if opts is None else opts
+    freq = np.asarray(freq, np.float64).reshape(-1)
+    mats = [np.asarray(m, np.complex128) for m in S_matrices]
+    if freq.size == 0 or len(mats) == 0:
+        raise RuntimeError("Empty frequency or S-parameter data")
+    for m in mats:
+        if m.ndim != 2:
+            raise RuntimeError("S-parameter matrices must be 2-D")
+    P = mats[0].shape[0]
+    if P < 1 or P > 99:
+        raise RuntimeError("Invalid number of ports: " + str(P))
+    for i, m in enumerate(mats):
+        if m.shape != (P, P):
+            raise RuntimeError("S-matrix dimension mismatch at frequency " + str(i))
+    if freq.size != len(mats):
+        raise RuntimeError("Frequency and S-matrix count mismatch")
+    _capi.touchstone_write(path, freq, np.stack(mats), int(opts.format), float(opts.z0))
+
+
+def touchstone_extension(num_ports: int) -> str:
+    """``.s<N>p`` for 1 <= N <= 99 (touchstone.cpp:137-145)."""
+    return _capi.touchstone_extension(num_ports)

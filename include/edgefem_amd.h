@@ -380,6 +380,27 @@ int efm_gmsh_copy(const efm_gmsh *m, int64_t *node_ids, double *xyz, int64_t *te
                   int64_t *tri_elem, int64_t *line_nodes, int32_t *line_phys);
 void efm_gmsh_free(efm_gmsh *m);
 
+/* ------------------------------------------------------------------------
+ * Touchstone writers (host code).  efm_touchstone_write replaces write_touchstone_nport
+ * (src/io/touchstone.cpp:68-135): nfreq frequencies (Hz), S host c128 [nfreq][P][P]
+ * row-major (S(i,j) of frequency f at (f*P + i)*P + j), format EFM_TOUCHSTONE_RI/_MA/_DB,
+ * z0 the reference impedance on the option line.  The layout and the number formatting
+ * (printf %.12g) are the reference's; more than 2 ports break a record after every 4th
+ * pair.  Errors, in the reference's order and words: "Empty frequency or S-parameter
+ * data", "Invalid number of ports: P" (P outside 1..99), "Cannot open file for writing:
+ * <path>".  (Per-matrix shape and count checks belong to the caller's container.)
+ * efm_touchstone_write_2port replaces the legacy write_touchstone (:50-62): S4 host c128
+ * [nfreq][4] in the order s11, s21, s12, s22; like the reference it writes nothing and
+ * reports nothing when the file cannot be opened.
+ * efm_touchstone_extension replaces touchstone_extension (:137-145): ".s<P>p" into out.
+ * ---------------------------------------------------------------------- */
+enum { EFM_TOUCHSTONE_RI = 0, EFM_TOUCHSTONE_MA = 1, EFM_TOUCHSTONE_DB = 2 };
+
+int efm_touchstone_write(const char *path, int64_t nfreq, const double *freq, int nports,
+                         const double *S, int format, double z0);
+int efm_touchstone_write_2port(const char *path, int64_t nfreq, const double *freq, const double *S4);
+int efm_touchstone_extension(int num_ports, char *out, int cap);
+
 #ifdef __cplusplus
 }
 #endif

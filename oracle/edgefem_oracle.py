@@ -36,6 +36,9 @@ build_lumped_port           src/ports/lumped_port.cpp:21-160
 normalize_port_weights      src/assemble_maxwell.cpp:396-494
 eps_gradient_eigenmode      adjoint of sparams_eigenmode w.r.t. per-tet eps_r (no
                             reference counterpart; pinned by central differences)
+touchstone_text             src/io/touchstone.cpp:68-135 (write_touchstone_nport;
+                            iostream precision 12 = printf %.12g)
+touchstone_text_2port       src/io/touchstone.cpp:50-62 (legacy write_touchstone)
 ==========================  =========================================================
 
 Third-party routines replaced: Eigen ``setFromTriplets`` (duplicates summed) ->
@@ -1146,3 +1149,39 @@ def eps_gradient_eigenmode(mesh: OMesh, p: Params, dirichlet: np.ndarray, ports:
     dS = k0 * k0 * np.einsum("dkj,dkl,dla->dja", le, Me, xe)            # (D,P,P)
     grad = np.einsum("ja,dja->d", np.conj(G), dS).real
     return S, grad
+
+
+def touchstone_text(freq, S, fmt: int = 0, z0: float = 50.0) -> str:
+    """Text of write_touchstone_nport (touchstone.cpp:68-135).  fmt 0 RI, 1 MA, 2 DB.
+    Every number goes through iostream at precision 12 with no float-field flag, i.e.
+    printf "%.12g".  MA: |s| and arg(s)*180/pi; DB: 20 log10(max(|s|, 1e-20)) and the same
+    angle.  More than 2 ports: a newline before every 4th value after the first, counted
+    over the whole row-major matrix; each value is written with its leading space."""
+    g = lambda v: "%.12g" % v
+    S = np.asarray(S, np.complex128)
+    P = S.shape[1]
+    out = ["! Touchstone file generated by EdgeFEM\n", "! Number of ports: %d\n" % P,
+           "# Hz S %s R %s\n" % (("RI", "MA", "DB")[fmt], g(z0))]
+    for fi, f in enumerate(np.asarray(freq, np.float64)):
+        line = g(f)
+        for q, v in enumerate(S[fi].reshape(-1)):
+            if P > 2 and q > 0 and q % 4 == 0:
+                line += "\n"
+            if fmt == 0:
+                a, b = v.real, v.imag
+            else:
+                mag = abs(v)
+                a = mag if fmt == 1 else 20.0 * math.log10(max(mag, 1e-20))
+                b = math.atan2(v.imag, v.real) * 180.0 / math.pi
+            line += " " + g(a) + " " + g(b)
+        out.append(line + "\n")
+    return "".join(out)
+
+
+def touchstone_text_2port(freq, S4) -> str:
+    """Text of the legacy write_touchstone (touchstone.cpp:50-62): option line
+    "# Hz S RI R 50", then freq and s11, s21, s12, s22 as real/imaginary pairs at %.12g."""
+    out = ["# Hz S RI R 50\n"]
+    for f, row in zip(np.asarray(freq, np.float64), np.asarray(S4, np.complex128).reshape(-1, 4)):
+        out.append(" ".join(["%.12g" % f] + ["%.12g %.12g" % (v.real, v.imag) for v in row]) + "\n")
+    return "".join(out)
