@@ -230,7 +230,7 @@ def sweep_main(args):
         "config": {"workload": "wr90_16x8x32_sweep_128f_x2", "systems": int(nsol), "edges": mesh.num_edges(),
                    "freqs_hz": [float(freqs[0]), float(freqs[-1]), int(freqs.size)],
                    "solver": "batched COCG+Jacobi+nodal-gradient", "tol": 1e-10,
-                   "iteration": "standard" if args.sweep_iteration == 0 else "chronopoulos-gear",
+                   "iteration": ("standard", "chronopoulos-gear", "standard-u-rebuilt")[args.sweep_iteration],
                    "precision": prec},
         "plain_path_solves_per_s": plain, "speedup_vs_plain": value / plain,
         "delta_S_vs_fp64_lu": dS, "delta_S_plain_vs_fp64_lu": dS_plain,
@@ -525,8 +525,9 @@ def main():
     ap.add_argument("--sweep-spmv", type=int, default=-1, choices=[-1, 0, 1, 2],
                     help="sweep: SpMV and nodal kernels (0 one slice per system group, 1 pipelined, 2 LDS-tiled "
                          "SpMV with the pipelined nodal kernel, -1 autotuned)")
-    ap.add_argument("--sweep-iteration", type=int, default=0, choices=[0, 1],
-                    help="sweep: 0 standard COCG (p in its own pass), 1 Chronopoulos-Gear (one reduction)")
+    ap.add_argument("--sweep-iteration", type=int, default=2, choices=[0, 1, 2],
+                    help="sweep: 0 standard COCG (p in its own pass), 1 Chronopoulos-Gear (one reduction), "
+                         "2 standard with u rebuilt in the p pass (no u store)")
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=3)
     ap.add_argument("--warmup", type=int, default=1)

@@ -488,11 +488,12 @@ class SweepSolver:
         solve."""
         _check(self._lib.efm_sweep_set_variants(self._h, int(spmv), int(aux)), "efm_sweep_set_variants")
 
-    def set_iteration(self, form: int = 0) -> None:
-        """Iteration form of the batched COCG: 0 (default) standard, with p = u + beta p in
-        a pass of its own and two reductions per iteration; 1 Chronopoulos-Gear, one
-        reduction per iteration (s = A p carried by a recurrence, which costs iterations
-        on indefinite systems)."""
+    def set_iteration(self, form: int = 2) -> None:
+        """Iteration form of the batched COCG: 0 standard, with p = u + beta p in a pass of
+        its own and two reductions per iteration; 1 Chronopoulos-Gear, one reduction per
+        iteration (s = A p carried by a recurrence, which costs iterations on indefinite
+        systems); 2 (the library's default) standard with u = D^-1 r + G y rebuilt in the
+        p pass instead of stored."""
         _check(self._lib.efm_sweep_set_iteration(self._h, int(form)), "efm_sweep_set_iteration")
 
     def variants(self):

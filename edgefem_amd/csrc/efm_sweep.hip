@@ -400,9 +400,35 @@ __global__ void __launch_bounds__(kBlock)
   }
 }
 
-// x += alpha p; r -= alpha w; u = D^{-1} r + G y; partials u^T r, |r|^2.  Persistent: one
-// block slice walks the groups of 64 systems in turn.
+// Iteration form 2: the same recurrence with u never stored.  The update forms u only for
+// its partial u^T r, and the p pass rebuilds it from r and the nodal y (unchanged between
+// the two): p = D^{-1} r + G y + beta p.  One vector pass fewer per iteration (the u
+// store) for a gather of y (node-sized, read in 64-system rows) and the edge diagonal.
 template <typename VT>
+__global__ void __launch_bounds__(kBlock)
+    k_sw_pform_u(int64_t n, int B, int P, const int2 *__restrict__ en_eff, const double2 *__restrict__ dK,
+                 const double2 *__restrict__ dM, const double *__restrict__ dMs, const int8_t *__restrict__ rport,
+                 const double *__restrict__ k0sq, const double2 *__restrict__ coef, const VT *__restrict__ y,
+                 const VT *__restrict__ r, VT *__restrict__ p, const SwState *__restrict__ st) {
+  const int s = blockIdx.y * kSys + (threadIdx.x & 63);
+  const int wv = threadIdx.x >> 6;
+  if (s >= B || st[s].active == 0)
+    return;
+  const double2 beta = st[s].beta;
+  const double k2 = k0sq[s];
+  for (int64_t e = (int64_t)blockIdx.x * kWaves + wv; e < n; e += (int64_t)gridDim.x * kWaves) {
+    const int64_t i = e * B + s;
+    const int2 ab = en_eff[e];
+    double2 z = bil(edge_diag_inv(e, s, P, k2, dK, dM, dMs, rport, coef), ldv(r + i));
+    if (ab.x >= 0)
+      z = cadd(z, csub(ldv(y + (int64_t)ab.y * B + s), ldv(y + (int64_t)ab.x * B + s)));
+    stv(p + i, cfma(beta, ldv(p + i), z));
+  }
+}
+
+// x += alpha p; r -= alpha w; u = D^{-1} r + G y (stored when STORE_U); partials u^T r,
+// |r|^2.  Persistent: one block slice walks the groups of 64 systems in turn.
+template <typename VT, bool STORE_U>
 __global__ void __launch_bounds__(kBlock)
     k_sw_update_std(int64_t n, int B, int groups, int P, const int2 *__restrict__ en_eff,
                     const double2 *__restrict__ dK, const double2 *__restrict__ dM,
@@ -430,7 +456,8 @@ __global__ void __launch_bounds__(kBlock)
         const int2 ab = en_eff[e];
         if (ab.x >= 0)
           z = cadd(z, csub(ldv(y + (int64_t)ab.y * B + s), ldv(y + (int64_t)ab.x * B + s)));
-        stv(u + i, z);
+        if (STORE_U)
+          stv(u + i, z);
         v[0] = cadd(v[0], bil(rv, z));
         v[1].x += cabs2(rv);
       }
@@ -1279,6 +1306,15 @@ __global__ void __launch_bounds__(kBlock)
   active_flag[s] = S.out_active;
 }
 
+// The system of a finalize block (one block per system, gridDim.x = the padded batch, a
+// multiple of 64).  The partials of 8 consecutive systems share a 128-byte line, so each
+// XCD (block b runs on XCD b % 8) takes a contiguous range of systems and reads every
+// line into one L2; one system per block in launch order would fetch each line 8 times.
+__device__ inline int fin_system() {
+  const int per_xcd = gridDim.x >> 3;
+  return (blockIdx.x & 7) * per_xcd + (blockIdx.x >> 3);
+}
+
 // One block per system: reduce its partials of g, |r|^2 (update grid, gxu blocks) and
 // d (spmv grid, gxs blocks) in a fixed order and advance the scalars.
 // skip: this iteration's update stored neither x nor r, so its alpha becomes pending.
@@ -1287,7 +1323,7 @@ __global__ void __launch_bounds__(kBlock)
     k_sw_finalize(int B, int gxu, int gxs, const double2 *__restrict__ pu,
                   const double2 *__restrict__ ps, SwState *__restrict__ st,
                   int *__restrict__ active_flag, double tol2, int max_it, int max_inner, int skip) {
-  const int s = blockIdx.x;
+  const int s = fin_system();
   if (s >= B)
     return;
   const int gy = s / kSys, lane = s % kSys;
@@ -1355,7 +1391,7 @@ template <int STAGE>
 __global__ void __launch_bounds__(kBlock)
     k_sw_fin_std(int B, int gx, const double2 *__restrict__ part, SwState *__restrict__ st,
                  int *__restrict__ active_flag, double tol2, int max_it, int max_inner) {
-  const int s = blockIdx.x;
+  const int s = fin_system();
   if (s >= B)
     return;
   const int gy = s / kSys, lane = s % kSys;
@@ -1549,8 +1585,9 @@ struct efm_sweep {
   int aux_variant = -1;
   int gx_aux = 8;
   int cplx = 3;  // bit 1: K complex, bit 2: M complex (selects the SpMV instantiation)
-  // iteration: 0 = standard COCG (p in its own pass, two reductions), 1 = Chronopoulos-Gear
-  int form = 0;
+  // iteration: 0 = standard COCG (p in its own pass, two reductions), 1 = Chronopoulos-Gear,
+  // 2 (default) = standard COCG with u rebuilt in the p pass instead of stored
+  int form = 2;
   int gx_std[2] = {8, 8};  // grid of the standard form's update [VT c128, c64]
   // update grids (one block slice, as many blocks as are resident): [VT c128, c64][INIT, skip, apply]
   int gx_upd[2][3] = {{8, 8, 8}, {8, 8, 8}};
@@ -1640,8 +1677,8 @@ extern "C" int efm_sweep_create(const efm_topology *topo, const double *K_val, c
     for (int a = 0; a < 2; ++a)
       for (int b = 0; b < 3; ++b)
         s->gx_upd[a][b] = resident_grid(upd[a][b], ncu, n);
-    s->gx_std[0] = resident_grid(reinterpret_cast<const void *>(&k_sw_update_std<double2>), ncu, n);
-    s->gx_std[1] = resident_grid(reinterpret_cast<const void *>(&k_sw_update_std<float2>), ncu, n);
+    s->gx_std[0] = resident_grid(reinterpret_cast<const void *>(&k_sw_update_std<double2, true>), ncu, n);
+    s->gx_std[1] = resident_grid(reinterpret_cast<const void *>(&k_sw_update_std<float2, true>), ncu, n);
   }
   int gx_upd_max = 8;
   for (int a = 0; a < 2; ++a) {
@@ -1741,7 +1778,9 @@ extern "C" int efm_sweep_set_variants(efm_sweep *s, int spmv, int aux) {
 
 extern "C" int efm_sweep_set_iteration(efm_sweep *s, int form) {
   EFM_REQUIRE(s, "efm_sweep_set_iteration: NULL sweep");
-  EFM_REQUIRE(form == 0 || form == 1, "efm_sweep_set_iteration: form must be 0 (standard) or 1 (Chronopoulos-Gear)");
+  EFM_REQUIRE(form >= 0 && form <= 2,
+              "efm_sweep_set_iteration: form must be 0 (standard), 1 (Chronopoulos-Gear) or 2 (standard, u rebuilt "
+              "in the p pass)");
   s->form = form;
   return EFM_OK;
 }
@@ -2071,17 +2110,21 @@ struct SwRun {
       const int chunk = std::min(check, cap - it);
       for (int c = 0; c < chunk; ++c, ++it) {
         const int va = va_of(it), vs = vs_of(it);
-        hipLaunchKernelGGL((k_sw_pform<VT>), dim3(s->gx_vec, groups), dim3(kBlock), 0, st, n, Bp, (const VT *)u, p,
-                           S);
+        if (s->form == 2)
+          hipLaunchKernelGGL((k_sw_pform_u<VT>), dim3(s->gx_vec, groups), dim3(kBlock), 0, st, n, Bp, P, en, dK, dM,
+                             dMs, rp, k2, cf, (const VT *)y, (const VT *)r, p, S);
+        else
+          hipLaunchKernelGGL((k_sw_pform<VT>), dim3(s->gx_vec, groups), dim3(kBlock), 0, st, n, Bp, (const VT *)u,
+                             p, S);
         EFM_LAUNCH_CHECK();
         if ((rc = run(log_s, s->spmv_variant < 0, vs, [&]() { return spmv_launch<VT>(vs, p, w); })) ||
             (rc = fin(std::integral_constant<int, 1>{}, gxs_of(vs), ps)) ||
             (rc = run(log_a, s->aux_variant < 0, va,
                       [&]() { return aux_launch<VT, 2>(va, r, w, (const VT *)nullptr, y); })))
           return rc;
-        hipLaunchKernelGGL((k_sw_update_std<VT>), dim3(s->gx_std[vti], 1), dim3(kBlock), 0, st, n, Bp, groups, P,
-                           en, dK, dM, dMs, rp, k2, cf, (const VT *)y, (const VT *)w, (const VT *)p, u, x, r, pu,
-                           S);
+        auto upd = s->form == 2 ? k_sw_update_std<VT, false> : k_sw_update_std<VT, true>;
+        hipLaunchKernelGGL(upd, dim3(s->gx_std[vti], 1), dim3(kBlock), 0, st, n, Bp, groups, P, en, dK, dM, dMs,
+                           rp, k2, cf, (const VT *)y, (const VT *)w, (const VT *)p, u, x, r, pu, S);
         EFM_LAUNCH_CHECK();
         if ((rc = fin(std::integral_constant<int, 2>{}, s->gx_std[vti], pu)))
           return rc;
@@ -2097,7 +2140,7 @@ struct SwRun {
   }
 
   template <typename VT> int solve_inner(const double2 *b, double inner_tol2, int max_inner) {
-    return s->form == 0 ? cocg_std<VT>(b, inner_tol2, max_inner) : cocg<VT>(b, inner_tol2, max_inner);
+    return s->form == 1 ? cocg<VT>(b, inner_tol2, max_inner) : cocg_std<VT>(b, inner_tol2, max_inner);
   }
 
   // Iterative refinement: r = b - A x64 in FP64, inner COCG on r with VT vectors to a

@@ -266,12 +266,16 @@ int efm_sweep_solve(efm_sweep *sw, int nsys, const double *h_k0sq, const double 
  * [SpMV 0, 1, 2, nodal 0, 1] (0 for a variant not timed).  Autotuner of this project; no
  * reference counterpart. */
 int efm_sweep_set_variants(efm_sweep *sw, int spmv, int aux);
-/* Iteration form of the batched COCG: 0 (default) = standard COCG, p = u + beta p in a
+/* Iteration form of the batched COCG: 0 = standard COCG, p = u + beta p in a
  * streaming pass of its own and two reductions per iteration; 1 = Chronopoulos-Gear, one
  * reduction per iteration with A p carried by the recurrence s = w + beta s (fewer
  * passes per iteration, but the recurrence drifts and costs iterations on indefinite
- * Maxwell systems: 623 against 584 on the worst frequency of the WR-90 16x8x32 sweep).
- * Same solution either way up to the tolerance.  No reference counterpart. */
+ * Maxwell systems: 623 against 584 on the worst frequency of the WR-90 16x8x32 sweep);
+ * 2 (default) = standard COCG with the preconditioned residual u = D^-1 r + G y never
+ * stored: the update forms it for its reduction only and the p pass rebuilds it from r
+ * and y (one vector pass fewer per iteration; 1,027 against 993 S-solves/s on the WR-90
+ * 16x8x32 sweep).  Same solution either way up to the tolerance.  No
+ * reference counterpart. */
 int efm_sweep_set_iteration(efm_sweep *sw, int form);
 int efm_sweep_get_variants(const efm_sweep *sw, int *h_spmv, int *h_aux, double *h_tune_ms);
 

@@ -429,13 +429,15 @@ def test_full_size_c2_topology_properties(torch_dev):
 @pytest.mark.parametrize("precision,max_batch,variant,form", [
     ("fp64", 256, -1, 0), ("fp64", 256, 0, 0), ("fp64", 256, 1, 0), ("fp64", 256, 2, 0), ("fp64", 100, 1, 0),
     ("fp64", 100, 2, 0), ("fp64-refined", 256, -1, 0),
-    ("fp64", 256, -1, 1), ("fp64", 256, 0, 1), ("fp64", 100, 2, 1), ("fp64-refined", 256, -1, 1)])
+    ("fp64", 256, -1, 1), ("fp64", 256, 0, 1), ("fp64", 100, 2, 1), ("fp64-refined", 256, -1, 1),
+    ("fp64", 256, -1, 2), ("fp64", 100, 2, 2), ("fp64-refined", 256, -1, 2)])
 def test_batched_sweep_matches_lu_golden(torch_dev, precision, max_batch, variant, form):
     """128 frequencies x 2 excitations = 256 systems on the 16x8x32 guide, batched, against
     the FP64 SuperLU golden of every point: plain FP64 COCG with each kernel variant of the
     SpMV and the nodal preconditioner part (0: one block slice per system group, 1:
     pipelined, groups in turn, 2: LDS-tiled SpMV with the pipelined nodal kernel, -1:
-    autotuned), in both iteration forms (0 standard COCG, 1 Chronopoulos-Gear), also with
+    autotuned), in every iteration form (0 standard COCG, 1 Chronopoulos-Gear, 2 standard
+    with u rebuilt in the p pass), also with
     batches that are not a multiple of the 64 systems of a wavefront (100, 100, 56), and
     the iterative refinement loop
     (true FP64 residual b - A x per outer step) around FP64 inner solves.  Complex-FP32
