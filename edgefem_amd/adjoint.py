@@ -58,6 +58,7 @@ class EpsDesignProblem:
         self.solver_options = solver_options
         P = max(1, len(self.ports))
         self.max_batch = max(P, (int(max_batch) // P) * P)   # a frequency's excitations share a batch
+        params._check_supported()  # dA/d eps = -k0^2 M_e needs unstretched, non-dispersive media
         self.base_eps, _ = em.tet_materials(mesh, params)
 
     def permittivity(self, eps_design: np.ndarray) -> np.ndarray:

@@ -34,6 +34,7 @@ from typing import Dict, Iterable, List, Optional, Sequence
 import numpy as np
 
 from . import _capi
+from . import materials
 from ._capi import EdgeFemError
 
 C0 = 299792458.0
@@ -311,6 +312,27 @@ class PortABCType(enum.IntEnum):
     ModalAdmittance = 4
 
 
+class PMLRegionSpec:
+    """Tensor PML of one physical region (maxwell.hpp:24-28): per-axis sigma_max and
+    thickness (3-vectors) and the polynomial grading order."""
+
+    def __init__(self):
+        self.sigma_max = np.zeros(3)
+        self.thickness = np.zeros(3)
+        self.grading_order = 3.0
+
+
+class PMLDiagnostic:
+    """Per-region PML reflection estimate (maxwell.hpp:30-35), reflection_est[axis] =
+    exp(-2 sigma_max thickness / ((order + 1) |omega|)), 1 where the axis has no PML."""
+
+    def __init__(self):
+        self.region_tag = 0
+        self.sigma_max = np.zeros(3)
+        self.thickness = np.zeros(3)
+        self.reflection_est = np.ones(3)
+
+
 class MaxwellParams:
     def __init__(self):
         self.omega = 0.0
@@ -322,7 +344,8 @@ class MaxwellParams:
         self.mu_models: Dict[int, object] = {}
         self.pml_sigma = 0.0
         self.pml_regions = set()
-        self.pml_tensor_regions = {}
+        self.pml_tensor_regions: Dict[int, PMLRegionSpec] = {}
+        self.enforce_pml_heuristics = True
         self.use_abc = False
         self.abc_surface_tags = set()
         self.use_port_abc = False
@@ -341,37 +364,153 @@ class MaxwellParams:
     def set_mu_r_region(self, phys_tag, mu_r):
         self.mu_r_regions[int(phys_tag)] = complex(mu_r)
 
+    def set_eps_model(self, phys_tag, model):
+        self.eps_models[int(phys_tag)] = model
+
+    def set_mu_model(self, phys_tag, model):
+        self.mu_models[int(phys_tag)] = model
+
     def get_eps_r(self, phys_tag, omega=None):
+        """Static value (region, else global); with omega, a dispersive model of the region
+        first (maxwell.hpp:92-107)."""
+        m = self.eps_models.get(int(phys_tag)) if omega is not None else None
+        if m is not None:
+            return complex(m.eval_eps(float(omega)))
         return complex(self.eps_r_regions.get(int(phys_tag), self.eps_r))
 
     def get_mu_r(self, phys_tag, omega=None):
+        m = self.mu_models.get(int(phys_tag)) if omega is not None else None
+        if m is not None:
+            return complex(m.eval_mu(float(omega)))
         return complex(self.mu_r_regions.get(int(phys_tag), self.mu_r))
 
+    def _frequency_dependent(self) -> bool:
+        """PML or a dispersive model: the matrix cannot be split into K and M once for
+        all frequencies (sweep.cpp:187-190)."""
+        return bool(self.pml_regions or self.pml_tensor_regions or self.eps_models or self.mu_models)
+
     def _check_supported(self):
-        if self.eps_models or self.mu_models:
-            raise NotImplementedError("dispersive material models are not supported by this GPU path")
-        if self.pml_regions or self.pml_tensor_regions:
-            raise NotImplementedError("PML regions are not supported by this GPU path")
+        """Paths without PML and dispersive media (the permittivity gradient, whose
+        derivative dA/d eps = -k0^2 M_e holds for unstretched, frequency-independent eps)."""
+        if self._frequency_dependent():
+            raise NotImplementedError("PML or dispersive materials are not supported on this path")
 
 
-def tet_materials(mesh: Mesh, p: MaxwellParams):
-    """Per-tet eps_r and mu_r with the reference precedence region > global
-    (maxwell.hpp:100-118), overridden by the per-tet arrays when given."""
-    p._check_supported()
+def tet_materials(mesh: Mesh, p: MaxwellParams, omega: Optional[float] = None, dispersive: bool = True):
+    """Per-tet eps_r and mu_r with the reference precedence dispersive model (evaluated at
+    omega, default p.omega) > region > global (maxwell.hpp:92-143), overridden by the
+    per-tet arrays when given.  dispersive=False gives the static values (what
+    assemble_maxwell_km reads, sweep.cpp:110-118)."""
     T = mesh.num_tets()
+    w = float(p.omega if omega is None else omega)
     if p.eps_r_tet is not None:
         eps = np.asarray(p.eps_r_tet, np.complex128).reshape(T)
     else:
         eps = np.full(T, complex(p.eps_r), np.complex128)
         for tag, val in p.eps_r_regions.items():
             eps[mesh.tet_phys == tag] = complex(val)
+        if dispersive:
+            for tag, model in p.eps_models.items():
+                if model is not None:
+                    eps[mesh.tet_phys == tag] = complex(model.eval_eps(w))
     if p.mu_r_tet is not None:
         mu = np.asarray(p.mu_r_tet, np.complex128).reshape(T)
     else:
         mu = np.full(T, complex(p.mu_r), np.complex128)
         for tag, val in p.mu_r_regions.items():
             mu[mesh.tet_phys == tag] = complex(val)
+        if dispersive:
+            for tag, model in p.mu_models.items():
+                if model is not None:
+                    mu[mesh.tet_phys == tag] = complex(model.eval_mu(w))
     return eps, mu
+
+
+def pml_stretch(mesh: Mesh, p: MaxwellParams, omega: Optional[float] = None):
+    """Per-tet scalar coordinate stretch s of the PML regions and the per-region
+    diagnostics (assemble_maxwell.cpp:58-172).  A tensor region's stretch is the mean of
+    its three axis stretches 1 + j sigma_a / omega, with sigma_a = sigma_max,a xi^order and
+    xi = 1 - d/thickness for a centroid at distance d < thickness from the nearer face of
+    the region's node bounding box (0 beyond); with the heuristics on, xi > 0 is raised
+    to 1e-3 and sigma_a to at least 1e-3 sigma_max,a.  A tag in pml_regions (and not a
+    tensor region) gets 1 + j pml_sigma / omega; s = 1 elsewhere and at omega = 0.  The
+    assembly uses K_e / s and M_e s."""
+    return _pml_stretch_arrays(mesh.xyz, mesh.tet_nodes, mesh.tet_phys, p, omega)
+
+
+def _pml_stretch_arrays(xyz, tet_nodes, tet_phys, p: MaxwellParams, omega: Optional[float] = None):
+    """pml_stretch on plain arrays: node coordinates (N, 3), tet node positions (T, 4), tags."""
+    xyz = np.asarray(xyz, np.float64)
+    tet_nodes = np.asarray(tet_nodes)
+    tet_phys = np.asarray(tet_phys)
+    T = tet_nodes.shape[0]
+    w = float(p.omega if omega is None else omega)
+    s = np.ones(T, np.complex128)
+    diags = []
+    for tag, spec in p.pml_tensor_regions.items():
+        d = PMLDiagnostic()
+        d.region_tag = int(tag)
+        d.sigma_max = np.array(spec.sigma_max, float).reshape(3)
+        d.thickness = np.array(spec.thickness, float).reshape(3)
+        d.reflection_est = np.ones(3)
+        for a in range(3):
+            sm, th = d.sigma_max[a], d.thickness[a]
+            if sm <= 0.0 or th <= 0.0 or abs(w) <= 0.0:
+                continue
+            d.reflection_est[a] = math.exp(-2.0 * (sm * th / (float(spec.grading_order) + 1.0)) / abs(w))
+        diags.append(d)
+    if w == 0.0 or not (p.pml_tensor_regions or p.pml_regions):
+        return s, diags
+    X = xyz[tet_nodes]                                 # (T, 4, 3)
+    cen = (X[:, 0] + X[:, 1] + X[:, 2] + X[:, 3]) / 4.0
+    tensor_tags = set(int(t) for t in p.pml_tensor_regions)
+    kmin = 1e-3
+    for tag, spec in p.pml_tensor_regions.items():
+        sel = np.nonzero(tet_phys == int(tag))[0]
+        if sel.size == 0:
+            continue
+        nodes = X[sel].reshape(-1, 3)
+        bmin, bmax = nodes.min(axis=0), nodes.max(axis=0)
+        st = np.ones((sel.size, 3), np.complex128)
+        sm_v = np.array(spec.sigma_max, float).reshape(3)
+        th_v = np.array(spec.thickness, float).reshape(3)
+        for a in range(3):
+            sm, th = sm_v[a], th_v[a]
+            if sm <= 0.0 or th <= 0.0:
+                continue
+            c = cen[sel, a]
+            md = np.minimum(c - bmin[a], bmax[a] - c)
+            xi = np.where(md < th, 1.0 - md / th, 0.0)
+            if p.enforce_pml_heuristics:
+                xi = np.where(xi > 0.0, np.maximum(xi, kmin), xi)
+            sig = sm * np.power(xi, float(spec.grading_order))
+            if p.enforce_pml_heuristics:
+                sig = np.where(sig < sm * kmin, sm * kmin, sig)
+            st[:, a] = 1.0 + 1j * (sig / w)
+        s[sel] = (st[:, 0] + st[:, 1] + st[:, 2]) / 3.0
+    for tag in p.pml_regions:
+        if int(tag) in tensor_tags:
+            continue
+        s[tet_phys == int(tag)] = complex(1.0, float(p.pml_sigma) / w)
+    return s, diags
+
+
+def _params_at(p: MaxwellParams, omega: float) -> MaxwellParams:
+    """A shallow copy of p at another angular frequency (the region maps and models are
+    shared, as the reference's per-frequency MaxwellParams copy shares its models)."""
+    q = MaxwellParams()
+    q.__dict__.update(p.__dict__)
+    q.omega = float(omega)
+    return q
+
+
+def _assembly_materials(mesh: Mesh, p: MaxwellParams, omega: Optional[float] = None):
+    """(eps s, mu s): the material factors of the volume assembly, K_e / (mu s) and
+    eps s M_e, i.e. the reference's (K_e / s) / mu and eps (M_e s) (assemble_maxwell.cpp:
+    165-199); plus the PML diagnostics."""
+    eps, mu = tet_materials(mesh, p, omega)
+    s, diags = pml_stretch(mesh, p, omega)
+    return eps * s, mu * s, diags
 
 
 class ModePolarization(enum.IntEnum):
@@ -762,8 +901,10 @@ def _port_abc_coefficient(params: MaxwellParams, port: WavePort) -> Optional[com
 
 def _assemble_with_ports(mesh: Mesh, params: MaxwellParams, bc: BC, ports: Sequence[WavePort]):
     """Volume + ABC (+ port ABC diagonals) on the GPU, and the rank-1 loading terms of every
-    valid port as (edges, weights, 1/Z0) for the operator (assemble_maxwell.cpp:46-350)."""
-    eps, mu = tet_materials(mesh, params)
+    valid port as (edges, weights, 1/Z0) for the operator (assemble_maxwell.cpp:46-350),
+    and the PML diagnostics.  PML stretch and dispersive models enter through the per-tet
+    material factors (eps s, mu s at params.omega)."""
+    eps, mu, diags = _assembly_materials(mesh, params)
     dmask = _dirichlet_mask(mesh, bc)
     abc = None
     if params.use_abc:
@@ -793,7 +934,7 @@ def _assemble_with_ports(mesh: Mesh, params: MaxwellParams, bc: BC, ports: Seque
         e, w = _free_port(port, dmask)
         if e.size:
             lowrank.append((e, w, 1.0 / complex(port.mode.Z0)))
-    return A, lowrank, dmask
+    return A, lowrank, dmask, diags
 
 
 def assemble_maxwell(mesh: Mesh, params: MaxwellParams, bc: BC, ports: Sequence[WavePort] = (),
@@ -804,8 +945,7 @@ def assemble_maxwell(mesh: Mesh, params: MaxwellParams, bc: BC, ports: Sequence[
     dense blocks in the sparsity pattern; ``coeff``, ``to_dense`` and ``nnz`` report the
     full matrix and ``solve_linear`` applies it.  b = 2 w / sqrt(Z0) on the free edges of
     the active port."""
-    params._check_supported()
-    A, lowrank, dmask = _assemble_with_ports(mesh, params, bc, ports)
+    A, lowrank, dmask, diags = _assemble_with_ports(mesh, params, bc, ports)
     topo = mesh.topology
     sym = all(not np.iscomplexobj(w) or np.abs(np.asarray(w).imag).max(initial=0.0) == 0 for _, w, _ in lowrank)
     b = np.zeros(mesh.num_edges(), np.complex128)
@@ -813,7 +953,9 @@ def assemble_maxwell(mesh: Mesh, params: MaxwellParams, bc: BC, ports: Sequence[
         port = ports[active_port_idx]
         e, w = _free_port(port, dmask)
         np.add.at(b, e, 2.0 / cmath.sqrt(complex(port.mode.Z0)) * w)
-    return MaxwellAssembly(SpMatC(topo.row_ptr, topo.col_idx, A, symmetric=sym, lowrank=lowrank), VecC(b))
+    asm = MaxwellAssembly(SpMatC(topo.row_ptr, topo.col_idx, A, symmetric=sym, lowrank=lowrank), VecC(b))
+    asm.diagnostics = diags
+    return asm
 
 
 def _solver_nrhs(P: int) -> int:
@@ -833,7 +975,6 @@ def calculate_sparams(mesh: Mesh, params: MaxwellParams, bc: BC, ports: Sequence
     edges; S_ii = (V_i - sqrt Z0_i)/sqrt Z0_i, S_ji = V_j / sqrt Z0_i; a column whose solve
     did not converge is NaN, with the reference's warning."""
     import torch
-    params._check_supported()
     P = len(ports)
     if P == 0:
         return np.zeros((0, 0), np.complex128)
@@ -891,7 +1032,6 @@ def normalize_port_weights(mesh: Mesh, params: MaxwellParams, bc: BC, ports: Lis
     on the GPU.  Modifies the ports in place and returns them, as the Python binding of
     the reference does."""
     import torch
-    params._check_supported()
     P = len(ports)
     if P == 0:
         return list(ports)
@@ -947,9 +1087,9 @@ def frequency_sweep(mesh: Mesh, params: MaxwellParams, bc: BC, ports: Sequence[W
     loading Y = sum w w^H / Z0 applied as the solver's low-rank term (set once); all
     excitations b_i = 2 w_i / sqrt(Z0_i) solved as one block; S as in calculate_sparams,
     NaN columns where a solve did not converge.  As in the reference, this branch has no
-    port ABC diagonals."""
+    port ABC diagonals.  With PML or a dispersive model the matrix is assembled at every
+    frequency and solved by calculate_sparams (the reference's slow path, sweep.cpp:330-346)."""
     import torch
-    params._check_supported()
     res = SweepResult()
     freqs = [float(f) for f in np.asarray(frequencies, np.float64).ravel()]
     res.frequencies = freqs
@@ -957,6 +1097,11 @@ def frequency_sweep(mesh: Mesh, params: MaxwellParams, bc: BC, ports: Sequence[W
     if P == 0 or not freqs:
         return res
     opts = solver_options or SolveOptions()
+    if params._frequency_dependent():
+        for f in freqs:
+            pf = _params_at(params, 2.0 * math.pi * f)
+            res.S_matrices.append(calculate_sparams(mesh, pf, bc, ports, opts))
+        return res
     eps, mu = tet_materials(mesh, params)
     dmask = _dirichlet_mask(mesh, bc)
     topo = mesh.topology
@@ -1143,8 +1288,7 @@ def assemble_maxwell_periodic(mesh: Mesh, params: MaxwellParams, bc: BC, pbc: Pe
     copies it to a dense matrix); the port loading stays an operator term with its
     vectors transformed the same way (u = Z E w, which equals Z F^H w in both
     conventions); b' = Z E b."""
-    params._check_supported()
-    A, lowrank, dmask = _assemble_with_ports(mesh, params, bc, ports)
+    A, lowrank, dmask, _ = _assemble_with_ports(mesh, params, bc, ports)
     topo = mesh.topology
     n = mesh.num_edges()
     b = np.zeros(n, np.complex128)
@@ -1527,8 +1671,6 @@ def calculate_sparams_periodic_batch(cells, *, mode: str = "reference",
     # periodic reduction) wait on the device with the GIL released, so another cell's host
     # work runs meanwhile.  Every thread enqueues on this thread's device and stream, and the
     # results are kept in cell order.
-    for c in cells:
-        c[1]._check_supported()
     dev, stream = torch.cuda.current_device(), torch.cuda.current_stream()
 
     def build(c):
@@ -1656,8 +1798,9 @@ def build_lumped_port(mesh: Mesh, config: LumpedPortConfig) -> WavePort:
 
 
 def assemble_maxwell_km(mesh: Mesh, params: MaxwellParams, bc: BC):
-    """K and M on the GPU (sweep.cpp:90-172); returns (K, M) as SpMatC."""
-    eps, mu = tet_materials(mesh, params)
+    """K and M on the GPU (sweep.cpp:90-172); returns (K, M) as SpMatC.  Static materials
+    only, as in the reference: dispersive models and PML are not part of K and M."""
+    eps, mu = tet_materials(mesh, params, dispersive=False)
     dmask = _dirichlet_mask(mesh, bc)
     topo = mesh.topology
     K, M = _capi.assemble_km(topo, mesh.dev_xyz, eps, 1.0 / mu, dmask)
@@ -1821,7 +1964,10 @@ def calculate_sparams_eigenmode(mesh: Mesh, params: MaxwellParams, bc: BC, ports
     topo = mesh.topology
     dmask = _dirichlet_mask(mesh, bc)
     ops = _PortOps(mesh, bc, ports)
-    K, M = _capi.assemble_km(topo, mesh.dev_xyz, eps, 1.0 / mu, dmask)
+    # the volume matrix of assemble_maxwell (PML stretch and dispersion at params.omega);
+    # beta above uses the port face's unstretched eps mu, as the reference does
+    eps_a, mu_a, _ = _assembly_materials(mesh, params)
+    K, M = _capi.assemble_km(topo, mesh.dev_xyz, eps_a, 1.0 / mu_a, dmask)
     k0 = _k0(params)
     coefs = [1j * params.port_abc_scale * b for b in betas]
     A = _capi.combine(topo, K, M, k0 * k0, ops.Ms, coefs, out_dtype=matrix_dtype or torch.complex128)
@@ -1877,6 +2023,13 @@ def frequency_sweep_eigenmode(mesh: Mesh, params: MaxwellParams, bc: BC, ports: 
     if P == 0 or freqs.size == 0:
         return out
     opts = solver_options or SolveOptions()
+    if params._frequency_dependent():
+        # PML or dispersive media: no K, M shared by the frequencies; the matrix is
+        # assembled and solved per frequency, as the reference's sweep does (sweep.cpp:330-346)
+        for i, f in enumerate(freqs):
+            out[i] = calculate_sparams_eigenmode(mesh, _params_at(params, 2.0 * math.pi * f), bc, ports,
+                                                 solver_options=opts)
+        return out
     eps, mu = tet_materials(mesh, params)
     topo = mesh.topology
     dmask = _dirichlet_mask(mesh, bc)

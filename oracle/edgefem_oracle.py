@@ -10,7 +10,11 @@ whitney_curl_curl_matrix    src/edge_basis.cpp:47-63
 whitney_mass_matrix         src/edge_basis.cpp:65-86 (lambda_int 28-30)
 triangle_whitney_mass       src/edge_basis.cpp:88-130
 build_edge_pec              src/bc.cpp:47-80
-assemble_maxwell            src/assemble_maxwell.cpp:46-350 (PML stretch s = 1 branch)
+assemble_maxwell            src/assemble_maxwell.cpp:46-350 (with the PML stretch)
+pml_stretch                 src/assemble_maxwell.cpp:58-172 (per-tet scalar stretch,
+                            diagnostics)
+debye_eps, lorentz_eps,     include/edgefem/materials/dispersive.hpp:37-287
+drude_eps, drude_lorentz_eps
 assemble_maxwell_km         src/sweep.cpp:90-172
 port_surface_mass           src/ports/wave_port.cpp:548-585
 solve_port_mode_2d          src/ports/wave_port.cpp:410-514
@@ -360,19 +364,115 @@ class Params:
     port_abc_scale: float = 1.0
     use_port_abc: bool = False
     port_abc_type: str = "beta"     # none | beta | beta_norm | impedance_match | modal_admittance
+    # dispersive models: tag -> callable omega -> complex (take precedence over the regions)
+    eps_models: Dict[int, object] = dataclasses.field(default_factory=dict)
+    mu_models: Dict[int, object] = dataclasses.field(default_factory=dict)
+    pml_sigma: float = 0.0
+    pml_regions: set = dataclasses.field(default_factory=set)
+    # tag -> (sigma_max[3], thickness[3], grading_order)
+    pml_tensor: Dict[int, tuple] = dataclasses.field(default_factory=dict)
+    enforce_pml_heuristics: bool = True
+
+
+def debye_eps(omega, eps_s, eps_inf, tau):
+    """dispersive.hpp:50-54: eps_inf + (eps_s - eps_inf) / (1 + j omega tau)."""
+    return eps_inf + (eps_s - eps_inf) / complex(1.0, omega * tau)
+
+
+def lorentz_eps(omega, eps_inf, poles):
+    """dispersive.hpp:110-121: eps_inf + sum d w0^2 / (w0^2 - w^2 + j g w); poles (d, w0, g)."""
+    e = complex(eps_inf)
+    for d, w0, g in poles:
+        e += d * w0 * w0 / complex(w0 * w0 - omega * omega, g * omega)
+    return e
+
+
+def drude_eps(omega, omega_p, gamma):
+    """dispersive.hpp:166-178: 1 - wp^2 / (w^2 + j g w), -1e30 at w = 0."""
+    if omega == 0.0:
+        return complex(-1e30, 0.0)
+    return 1.0 - omega_p * omega_p / complex(omega * omega, gamma * omega)
+
+
+def drude_lorentz_eps(omega, eps_inf, omega_p, gamma_d, poles):
+    """dispersive.hpp:240-262: Drude term on eps_inf (the whole value -1e30 at w = 0), then
+    the Lorentz poles added."""
+    if omega != 0.0:
+        e = eps_inf - omega_p * omega_p / complex(omega * omega, gamma_d * omega)
+    else:
+        e = complex(-1e30, 0.0)
+    return lorentz_eps(omega, 0.0, poles) + e
 
 
 def tet_materials(mesh: OMesh, p: Params):
+    """Per-tet eps, mu at p.omega: per-tet array > dispersive model > region > global
+    (maxwell.hpp:92-143)."""
     T = mesh.tet_ids.shape[0]
+
+    def one(tag, regions, models, default):
+        m = models.get(int(tag))
+        return complex(m(p.omega)) if m is not None else complex(regions.get(int(tag), default))
     if p.eps_tet is not None:
         eps = np.asarray(p.eps_tet, np.complex128)
     else:
-        eps = np.array([p.eps_r_regions.get(int(t), p.eps_r) for t in mesh.tet_phys], np.complex128) if T else np.zeros(0, complex)
+        eps = np.array([one(t, p.eps_r_regions, p.eps_models, p.eps_r) for t in mesh.tet_phys], np.complex128) if T else np.zeros(0, complex)
     if p.mu_tet is not None:
         mu = np.asarray(p.mu_tet, np.complex128)
     else:
-        mu = np.array([p.mu_r_regions.get(int(t), p.mu_r) for t in mesh.tet_phys], np.complex128) if T else np.zeros(0, complex)
+        mu = np.array([one(t, p.mu_r_regions, p.mu_models, p.mu_r) for t in mesh.tet_phys], np.complex128) if T else np.zeros(0, complex)
     return eps, mu
+
+
+def pml_stretch(mesh: OMesh, p: Params):
+    """Per-tet scalar PML stretch and per-region reflection estimates, tet by tet as
+    assemble_maxwell.cpp:58-172 computes them.  Returns (s[T], {tag: reflection[3]})."""
+    T = mesh.tet_ids.shape[0]
+    X = mesh.tet_xyz()
+    refl = {}
+    for tag, (smax, thick, order) in p.pml_tensor.items():
+        r = []
+        for a in range(3):
+            if smax[a] <= 0.0 or thick[a] <= 0.0 or abs(p.omega) <= 0.0:
+                r.append(1.0)
+            else:
+                r.append(math.exp(-2.0 * (smax[a] * thick[a] / (order + 1.0)) / abs(p.omega)))
+        refl[int(tag)] = r
+    bounds = {}
+    for t in range(T):
+        tag = int(mesh.tet_phys[t])
+        if tag not in p.pml_tensor:
+            continue
+        lo, hi = bounds.get(tag, (None, None))
+        for i in range(4):
+            x = X[t, i]
+            lo = x.copy() if lo is None else np.minimum(lo, x)
+            hi = x.copy() if hi is None else np.maximum(hi, x)
+        bounds[tag] = (lo, hi)
+    s = np.ones(T, np.complex128)
+    if p.omega == 0.0:
+        return s, refl
+    for t in range(T):
+        tag = int(mesh.tet_phys[t])
+        if tag in p.pml_tensor:
+            smax, thick, order = p.pml_tensor[tag]
+            c = (X[t, 0] + X[t, 1] + X[t, 2] + X[t, 3]) / 4.0
+            lo, hi = bounds[tag]
+            st = [1.0 + 0j, 1.0 + 0j, 1.0 + 0j]
+            for a in range(3):
+                if smax[a] <= 0.0 or thick[a] <= 0.0:
+                    continue
+                md = min(c[a] - lo[a], hi[a] - c[a])
+                xi = 1.0 - md / thick[a] if md < thick[a] else 0.0
+                if p.enforce_pml_heuristics and xi > 0.0:
+                    xi = max(xi, 1e-3)
+                sig = smax[a] * xi ** order
+                if p.enforce_pml_heuristics and smax[a] > 0.0 and sig < smax[a] * 1e-3:
+                    sig = smax[a] * 1e-3
+                st[a] = complex(1.0, sig / p.omega)
+            s[t] = (st[0] + st[1] + st[2]) / 3.0
+        elif tag in p.pml_regions:
+            s[t] = complex(1.0, p.pml_sigma / p.omega)
+    return s, refl
 
 
 # ---------------------------------------------------------------------------
@@ -428,7 +528,8 @@ def assemble_maxwell_km(mesh: OMesh, p: Params, dirichlet: np.ndarray):
     X = mesh.tet_xyz()
     Ke = whitney_curl_curl_matrix(X)
     Me = whitney_mass_matrix(X)
-    eps, mu = tet_materials(mesh, p)
+    # static materials: the reference's K/M assembly reads the region maps only (sweep.cpp:110-118)
+    eps, mu = tet_materials(mesh, dataclasses.replace(p, eps_models={}, mu_models={}))
     rows, cols, sign = _triplets(mesh)
     n = mesh.n_edges
     kv = (Ke.astype(np.complex128) / mu[:, None, None]) * sign
@@ -449,10 +550,13 @@ def assemble_maxwell(mesh: OMesh, p: Params, dirichlet: np.ndarray, ports=(), ac
     Ke = whitney_curl_curl_matrix(X)
     Me = whitney_mass_matrix(X)
     eps, mu = tet_materials(mesh, p)
+    s, _ = pml_stretch(mesh, p)
     k0 = p.omega / C0
     rows, cols, sign = _triplets(mesh)
     n = mesh.n_edges
-    val = (Ke / mu[:, None, None] - (k0 * k0) * eps[:, None, None] * Me) * sign
+    Ks = Ke / s[:, None, None]       # Kloc = K_e / s, Mloc = M_e s (assemble_maxwell.cpp:168-171)
+    Ms = Me * s[:, None, None]
+    val = (Ks / mu[:, None, None] - (k0 * k0) * eps[:, None, None] * Ms) * sign
     R = [rows.ravel()]; C = [cols.ravel()]; V = [val.ravel()]
     b = np.zeros(n, np.complex128)
     for i, port in enumerate(ports):

@@ -893,3 +893,116 @@ def test_tet_fields_vs_oracle(torch_dev):
     cerr = np.abs(C.cpu().numpy()[:, 0] - 2 * b[None]).max() / np.abs(b).max()
     _p(f"    affine field: max rel error E {err:.1e}, curl {cerr:.1e}")
     assert err <= 1e-12 and cerr <= 1e-9
+
+
+def test_pml_and_dispersive_assembly_vs_oracle(torch_dev):
+    """assemble_maxwell with PML and dispersive media against the oracle, entry by entry
+    (1e-12 of the largest entry): (1) the reference's PML test configuration
+    (tests/test_pml.cpp: the first tet of cube_cavity.msh as a tensor region, sigma_max 5,
+    thickness 1, order 2, omega 1) with its reflection estimates exp(-10/3); (2) at 10 GHz
+    a scalar PML region on half the tets plus a Debye model on the other half; (3) a
+    tensor region over the whole cavity with the heuristics off and a Lorentz model."""
+    from edgefem_amd import pyedgefem as em
+    from edgefem_amd import materials as M
+    from oracle import edgefem_oracle as O
+    _p("[gpu] PML and dispersive assembly vs oracle (cube_cavity.msh)")
+    pm = em.load_gmsh(golden("cube_cavity.msh"))
+    om = _oracle_mesh_from_file("cube_cavity.msh")
+    bc = em.build_edge_pec(pm, 1)
+    pec = O.build_edge_pec(om, 1)
+    base = int(pm.tet_phys[0])
+    tag = base + 1
+    T = pm.num_tets()
+
+    def check(params, op, what):
+        asm = em.assemble_maxwell(pm, params, bc)
+        A = asm.A.to_dense()
+        Ao, _ = O.assemble_maxwell(om, op, pec)
+        Ao = Ao.toarray()
+        err = np.abs(A - Ao).max() / np.abs(Ao).max()
+        _p(f"    {what}: max |dA| / max |A| {err:.2e}")
+        assert err <= 1e-12
+        return asm
+
+    pm.tet_phys[0] = tag
+    om.tet_phys = om.tet_phys.copy()
+    om.tet_phys[0] = tag
+    p = em.MaxwellParams()
+    p.omega = 1.0
+    spec = em.PMLRegionSpec()
+    spec.sigma_max, spec.thickness, spec.grading_order = np.full(3, 5.0), np.full(3, 1.0), 2.0
+    p.pml_tensor_regions = {tag: spec}
+    asm = check(p, O.Params(omega=1.0, pml_tensor={tag: ((5.0,) * 3, (1.0,) * 3, 2.0)}), "reference PML test")
+    assert len(asm.diagnostics) == 1 and asm.diagnostics[0].region_tag == tag
+    np.testing.assert_allclose(asm.diagnostics[0].reflection_est, [math.exp(-10.0 / 3.0)] * 3, rtol=1e-14)
+    half = np.arange(T) % 2 == 1
+    pm.tet_phys[:] = np.where(half, tag, base)
+    om.tet_phys = pm.tet_phys.copy()
+    w = 2 * math.pi * 10e9
+    p = em.MaxwellParams()
+    p.omega = w
+    p.pml_sigma, p.pml_regions = 0.7 * w, {tag}
+    p.set_eps_model(base, M.DebyeMaterial(6.0, 2.5, 1.0 / (2 * math.pi * 8e9)))
+    op = O.Params(omega=w, pml_sigma=0.7 * w, pml_regions={tag},
+                  eps_models={base: lambda x: O.debye_eps(x, 6.0, 2.5, 1.0 / (2 * math.pi * 8e9))})
+    check(p, op, "scalar PML + Debye")
+    pm.tet_phys[:] = tag
+    om.tet_phys = pm.tet_phys.copy()
+    p = em.MaxwellParams()
+    p.omega = w
+    spec = em.PMLRegionSpec()
+    spec.sigma_max, spec.thickness, spec.grading_order = np.array([w, 0.5 * w, 2 * w]), np.full(3, 0.3), 3.0
+    p.pml_tensor_regions = {tag: spec}
+    p.enforce_pml_heuristics = False
+    L = M.LorentzMaterial(1.5)
+    L.add_pole(2.0, 2 * math.pi * 12e9, 2 * math.pi * 1e9)
+    p.set_eps_model(tag, L)
+    op = O.Params(omega=w, pml_tensor={tag: ((w, 0.5 * w, 2 * w), (0.3,) * 3, 3.0)}, enforce_pml_heuristics=False,
+                  eps_models={tag: lambda x: O.lorentz_eps(x, 1.5, [(2.0, 2 * math.pi * 12e9, 2 * math.pi * 1e9)])})
+    check(p, op, "tensor PML + Lorentz, heuristics off")
+
+
+def test_dispersive_and_pml_eigenmode_sweep_vs_oracle(torch_dev):
+    """frequency_sweep_eigenmode with a Debye slab and a thin scalar-PML slab in a short
+    WR-90 guide (6x3x12): no K, M shared by the frequencies, so the sweep assembles and
+    solves each frequency (the reference's slow path, sweep.cpp:330-346); S at 9, 10 and 11
+    GHz against the oracle's eigenmode S with the same models (1e-6), and the lossy slabs
+    make |S21| < 1."""
+    from edgefem_amd import meshgen as mg
+    from edgefem_amd import pyedgefem as em
+    from edgefem_amd import materials as M
+    from oracle import edgefem_oracle as O
+    _p("[gpu] eigenmode sweep with a Debye slab and a PML slab vs oracle")
+    L = 30e-3
+    g = mg.box_mesh(mg.WR90_A, mg.WR90_B, L, 6, 3, 12, seed=7)
+    zc = g.centroids()[:, 2]
+    phys = g.tet_phys.copy()
+    phys[(zc > 0.3 * L) & (zc < 0.5 * L)] = 101
+    phys[(zc > 0.6 * L) & (zc < 0.7 * L)] = 102
+    pm = em.mesh_from_arrays(g.node_ids, g.xyz, g.tets, phys, g.tris, g.tri_phys)
+    bc = em.build_edge_pec(pm, 1)
+    mode = em.solve_te10_mode(em.RectWaveguidePort(mg.WR90_A, mg.WR90_B), 10e9)
+    kc2 = (math.pi / mg.WR90_A) ** 2
+    ports = [em.build_wave_port_2d(pm, t, mode, bc.dirichlet_edges, kc2) for t in (2, 3)]
+    tau = 1.0 / (2 * math.pi * 10e9)
+    sig = 0.3 * 2 * math.pi * 10e9
+    params = em.MaxwellParams()
+    params.set_eps_model(101, M.DebyeMaterial(4.0, 2.0, tau))
+    params.pml_sigma, params.pml_regions = sig, {102}
+    freqs = np.array([9e9, 10e9, 11e9])
+    om = O.mesh_from_arrays(g.node_ids, g.xyz, g.tet_ids, phys, g.tri_ids, g.tri_phys)
+    opec = O.build_edge_pec(om, 1)
+    kc, beta, Z0 = O.solve_te10_mode(mg.WR90_A, 10e9)
+    oports = [O.build_wave_port_2d(om, t, kc, Z0, beta, opec, kc2) for t in (2, 3)]
+    Sref = np.array([O.sparams_eigenmode(om, O.Params(omega=2 * math.pi * f, pml_sigma=sig, pml_regions={102},
+                                                      eps_models={101: lambda x: O.debye_eps(x, 4.0, 2.0, tau)}),
+                                         opec, oports) for f in freqs])
+    opts = em.SolveOptions()
+    opts.tolerance = 1e-10
+    S = em.frequency_sweep_eigenmode(pm, params, bc, ports, freqs, solver_options=opts)
+    err = np.abs(S - Sref).max()
+    _p(f"    max |dS| {err:.2e}; |S21| {np.abs(S[:, 1, 0]).round(6)}; |S11| {np.abs(S[:, 0, 0]).round(6)}")
+    assert err <= 1e-6
+    assert np.all(np.abs(S[:, 1, 0]) < 0.999)
+    S10 = em.calculate_sparams_eigenmode(pm, em._params_at(params, 2 * math.pi * 10e9), bc, ports, opts)
+    assert np.abs(S10 - S[1]).max() <= 1e-9
