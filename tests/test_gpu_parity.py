@@ -1004,5 +1004,6 @@ def test_dispersive_and_pml_eigenmode_sweep_vs_oracle(torch_dev):
     _p(f"    max |dS| {err:.2e}; |S21| {np.abs(S[:, 1, 0]).round(6)}; |S11| {np.abs(S[:, 0, 0]).round(6)}")
     assert err <= 1e-6
     assert np.all(np.abs(S[:, 1, 0]) < 0.999)
-    S10 = em.calculate_sparams_eigenmode(pm, em._params_at(params, 2 * math.pi * 10e9), bc, ports, opts)
+    S10 = em.calculate_sparams_eigenmode(pm, em._params_at(params, 2 * math.pi * 10e9), bc, ports,
+                                         solver_options=opts)
     assert np.abs(S10 - S[1]).max() <= 1e-9
