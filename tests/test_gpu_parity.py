@@ -441,8 +441,9 @@ def test_batched_sweep_matches_lu_golden(torch_dev, precision, max_batch, varian
     batches that are not a multiple of the 64 systems of a wavefront (100, 100, 56), and
     the iterative refinement loop
     (true FP64 residual b - A x per outer step) around FP64 inner solves.  Complex-FP32
-    inner vectors are not tested here: on these indefinite systems their COCG iterations
-    stall near a 1e-4..1e-3 relative residual for some frequencies."""
+    inner vectors are tested on a smaller guide (test_batched_sweep_fp32_inner_refined_vs_oracle):
+    on these indefinite systems their COCG iterations stall near a 1e-4..1e-3 relative
+    residual for some frequencies, which makes this golden run slow."""
     from edgefem_amd import pyedgefem as em
     _p(f"[gpu] batched sweep 256 systems precision={precision} max_batch={max_batch} variant={variant} form={form}")
     pm, bc, ports = _c1_em_setup()
@@ -1007,3 +1008,45 @@ def test_dispersive_and_pml_eigenmode_sweep_vs_oracle(torch_dev):
     S10 = em.calculate_sparams_eigenmode(pm, em._params_at(params, 2 * math.pi * 10e9), bc, ports,
                                          solver_options=opts)
     assert np.abs(S10 - S[1]).max() <= 1e-9
+
+
+def test_batched_sweep_fp32_inner_refined_vs_oracle(torch_dev):
+    """precision="fp32": the batched COCG iterates on complex-FP32 vectors inside FP64
+    iterative refinement (FP64 residual b - A x, FP64 solution and reductions), on the short
+    lossy WR-90 guide (6x3x12, complex M) at 9, 10 and 11 GHz, against the oracle's FP64 S
+    (1e-6).  The refinement must reach the FP64 tolerance on the true residual."""
+    from edgefem_amd import meshgen as mg
+    from edgefem_amd import pyedgefem as em
+    from oracle import edgefem_oracle as O
+    _p("[gpu] batched sweep, complex-FP32 inner vectors with FP64 refinement, lossy guide")
+    L = 30e-3
+    g = mg.box_mesh(mg.WR90_A, mg.WR90_B, L, 6, 3, 12, seed=7)
+    pm = em.mesh_from_arrays(g.node_ids, g.xyz, g.tets, g.tet_phys, g.tris, g.tri_phys)
+    bc = em.build_edge_pec(pm, 1)
+    mode = em.solve_te10_mode(em.RectWaveguidePort(mg.WR90_A, mg.WR90_B), 10e9)
+    kc2 = (math.pi / mg.WR90_A) ** 2
+    ports = [em.build_wave_port_2d(pm, t, mode, bc.dirichlet_edges, kc2) for t in (2, 3)]
+    zc = g.centroids()[:, 2]
+    eps = np.ones(g.tets.shape[0], np.complex128)
+    slab = (zc > 0.3 * L) & (zc < 0.7 * L)
+    eps[slab] = (1.0 + 3.0 * mg.uniform01(9, np.nonzero(slab)[0].astype(np.int64))) * (1 - 0.02j)
+    params = em.MaxwellParams()
+    params.eps_r_tet = eps
+    freqs = np.array([9e9, 10e9, 11e9])
+    om = O.mesh_from_arrays(g.node_ids, g.xyz, g.tet_ids, g.tet_phys, g.tri_ids, g.tri_phys)
+    opec = O.build_edge_pec(om, 1)
+    kc, beta, Z0 = O.solve_te10_mode(mg.WR90_A, 10e9)
+    oports = [O.build_wave_port_2d(om, t, kc, Z0, beta, opec, kc2) for t in (2, 3)]
+    Sref = np.array([O.sparams_eigenmode(om, O.Params(omega=2 * math.pi * f, eps_tet=eps), opec, oports)
+                     for f in freqs])
+    opts = em.SolveOptions()
+    opts.tolerance = tol = 1e-10
+    st = {"iteration": 2}
+    S = em.frequency_sweep_eigenmode(pm, params, bc, ports, freqs, precision="fp32", solver_options=opts, _state=st)
+    err = np.abs(S - Sref).max()
+    res = np.concatenate([i["resid"] for i in st["infos"]])
+    its = np.concatenate([i["iters"] for i in st["infos"]])
+    _p(f"    max |dS| {err:.2e}; iterations {its.min()}-{its.max()}; max resid {res.max():.1e}")
+    assert np.all(np.isfinite(S))
+    assert res.max() <= tol
+    assert err <= 1e-6
