@@ -7,6 +7,7 @@ with the oracle on the same inputs:
   * S-parameters: |delta S| <= 1e-6 against FP64 SuperLU.
 """
 
+import cmath
 import math
 import sys
 
@@ -1050,3 +1051,42 @@ def test_batched_sweep_fp32_inner_refined_vs_oracle(torch_dev):
     assert np.all(np.isfinite(S))
     assert res.max() <= tol
     assert err <= 1e-6
+
+
+def test_wr42_guide_reference_criteria_and_oracle(torch_dev):
+    """tests/test_wr42_waveguide.cpp:60-170 on its own fixture (examples/wr42_waveguide.msh,
+    a 30 mm WR-42 section): eigenmode S at 20, 22, 24 and 26 GHz with the 2D port modes;
+    per frequency |S11| < 0.2, |S21| > 0.9 and |arg S21 + beta L| < 15 degrees, at least
+    3 of 4 frequencies passing all three (the test's overall criterion); and S against
+    the oracle's FP64 LU S (1e-6) at every frequency."""
+    from edgefem_amd import pyedgefem as em
+    from oracle import edgefem_oracle as O
+    _p("[gpu] WR-42 guide (reference fixture), 20-26 GHz, reference criteria and oracle")
+    a, b, L = 0.010668, 0.004318, 0.030
+    kc2 = (math.pi / a) ** 2
+    pm = em.load_gmsh(golden("wr42_waveguide.msh"))
+    bc = em.build_edge_pec(pm, 1)
+    om = _oracle_mesh_from_file("wr42_waveguide.msh")
+    opec = O.build_edge_pec(om, 1)
+    opts = em.SolveOptions()
+    opts.tolerance = 1e-10
+    passing = 0
+    for f in (20e9, 22e9, 24e9, 26e9):
+        mode = em.solve_te10_mode(em.RectWaveguidePort(a, b), f)
+        ports = [em.build_wave_port_2d(pm, t, mode, bc.dirichlet_edges, kc2) for t in (2, 3)]
+        params = em.MaxwellParams()
+        params.omega = 2 * math.pi * f
+        S = em.calculate_sparams_eigenmode(pm, params, bc, ports, solver_options=opts)
+        kc, beta, Z0 = O.solve_te10_mode(a, f)
+        oports = [O.build_wave_port_2d(om, t, kc, Z0, beta, opec, kc2) for t in (2, 3)]
+        So = O.sparams_eigenmode(om, O.Params(omega=2 * math.pi * f), opec, oports)
+        k0 = 2 * math.pi * f / O.C0
+        bt = math.sqrt(k0 * k0 - kc2)
+        dph = abs((math.degrees(cmath.phase(S[1, 0]) + bt * L) + 180) % 360 - 180)
+        ok = abs(S[0, 0]) < 0.2 and abs(S[1, 0]) > 0.9 and dph < 15
+        passing += ok
+        err = np.abs(S - So).max()
+        _p(f"    {f / 1e9:.0f} GHz: |S11| {abs(S[0, 0]):.4f} |S21| {abs(S[1, 0]):.4f} phase err {dph:.1f} deg; "
+           f"|dS| vs oracle {err:.1e}")
+        assert err <= 1e-6
+    assert passing >= 3

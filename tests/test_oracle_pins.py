@@ -322,8 +322,10 @@ def test_port_cutoff_converges_to_analytic():
 
 
 def test_straight_guide_physics():
-    """|S21| ~ 1, |S11| ~ 0, arg S21 ~ -beta L on the generated 16x8x32 guide
-    (thresholds of tests/test_eigenmode_sparams.cpp:116-117)."""
+    """|S21| ~ 1, |S11| ~ 0, arg S21 ~ -beta L on the generated 16x8x32 guide (thresholds
+    of tests/test_eigenmode_sparams.cpp:116-117, stricter than those of
+    tests/benchmark_wr90.cpp:86-92, whose passivity bound |S11|^2 + |S21|^2 <= 1.05 is
+    checked too)."""
     g = mg.wr90_guide()
     m = O.mesh_from_arrays(g.node_ids, g.xyz, g.tet_ids, g.tet_phys, g.tri_ids, g.tri_phys)
     pec = O.build_edge_pec(m, 1)
@@ -332,6 +334,7 @@ def test_straight_guide_physics():
     ports = [O.build_wave_port_2d(m, t, kc, Z0, beta, pec, (math.pi / mg.WR90_A) ** 2) for t in (2, 3)]
     S = np.load(golden("c1_sparams_10ghz.npz"))["S"]
     assert abs(S[0, 0]) < 0.1 and abs(S[1, 0]) > 0.95
+    assert abs(S[0, 0]) ** 2 + abs(S[1, 0]) ** 2 <= 1.05
     dph = (np.degrees(np.angle(S[1, 0]) + beta * 0.05) + 180) % 360 - 180
     assert abs(dph) < 10
     assert abs(S[0, 1] - S[1, 0]) < 1e-9
@@ -868,3 +871,19 @@ def test_pml_stretch_pins():
         np.testing.assert_allclose(sh, so, rtol=1e-15, atol=0)
         np.testing.assert_allclose(dh[0].reflection_est, ro[100], rtol=1e-15)
         assert np.any(sh.imag > 0) and np.all(sh[phys == 101] == complex(1.0, 1e9 / w))
+
+
+def test_port_eigens_on_reference_port_mesh():
+    """tests/test_ports.cpp:11-39 on its own fixture (examples/wr90_port.msh, a 2D WR-90
+    cross-section): one TE mode at 10 GHz whose cutoff is within 5 % of the analytic TE10
+    cutoff c0 / (2a); the mesh holds the PEC boundary lines the test asserts."""
+    from edgefem_amd import _capi
+    m = O.load_gmsh_v2(golden("wr90_port.msh"))
+    assert m.tri_ids.shape[0] > 0
+    assert _capi.gmsh_read(golden("wr90_port.msh"))["line_ids"].shape[0] > 0
+    tag = int(np.bincount(m.tri_phys).argmax())
+    surf = O.extract_surface_mesh(m, tag)
+    modes = O.solve_port_eigens(surf, 1, 2 * math.pi * 10e9, 1.0, 1.0, "TE")
+    assert len(modes) == 1
+    fc_te10 = O.C0 / (2 * 0.02286)
+    assert abs(modes[0]["fc"] - fc_te10) / fc_te10 < 0.05
