@@ -1090,3 +1090,40 @@ def test_wr42_guide_reference_criteria_and_oracle(torch_dev):
            f"|dS| vs oracle {err:.1e}")
         assert err <= 1e-6
     assert passing >= 3
+
+
+def test_lossy_fill_attenuation_reference_criteria(torch_dev):
+    """python/tests/test_validation_suite.py:221-299 on the 16x8x32 WR-90 guide (a, b, L =
+    22.86, 10.16, 50 mm): the whole guide filled with eps_r = 2.2 (1 - 0.05j) at 10 GHz;
+    |S21| against the analytic dielectric decay exp(-alpha_d L), alpha_d = k0^2 eps_r tan_d
+    / (2 beta), within the suite's 35 %; lossy |S21| below the lossless one; and S against
+    the oracle's FP64 LU S (1e-6)."""
+    from edgefem_amd import meshgen as mg
+    from edgefem_amd import pyedgefem as em
+    from oracle import edgefem_oracle as O
+    _p("[gpu] lossy dielectric fill: analytic attenuation, lossless comparison, oracle")
+    pm, bc, ports = _c1_em_setup()
+    f, er, td, L = 10e9, 2.2, 0.05, 50e-3
+    w = 2 * math.pi * f
+    k0 = w / O.C0
+    kc = math.pi / mg.WR90_A
+    beta = math.sqrt(er * k0 * k0 - kc * kc)
+    s21_ana = math.exp(-(k0 * k0 * er * td / (2 * beta)) * L)
+    params = em.MaxwellParams()
+    params.omega = w
+    S0 = em.calculate_sparams_eigenmode(pm, params, bc, ports)
+    params.eps_r = complex(er, -er * td)
+    S = em.calculate_sparams_eigenmode(pm, params, bc, ports)
+    g = mg.wr90_guide()
+    om = O.mesh_from_arrays(g.node_ids, g.xyz, g.tet_ids, g.tet_phys, g.tri_ids, g.tri_phys)
+    opec = O.build_edge_pec(om, 1)
+    okc, obeta, oZ0 = O.solve_te10_mode(mg.WR90_A, f)
+    oports = [O.build_wave_port_2d(om, t, okc, oZ0, obeta, opec, kc * kc) for t in (2, 3)]
+    So = O.sparams_eigenmode(om, O.Params(omega=w, eps_r=complex(er, -er * td)), opec, oports)
+    s21 = abs(S[1, 0])
+    _p(f"    |S21| {s21:.4f} (analytic {s21_ana:.4f}, lossless {abs(S0[1, 0]):.4f}); |dS| vs oracle "
+       f"{np.abs(S - So).max():.1e}")
+    assert 0.1 < s21 < 1.0
+    assert abs(s21 - s21_ana) / s21_ana < 0.35
+    assert s21 < abs(S0[1, 0])
+    assert np.abs(S - So).max() <= 1e-6
