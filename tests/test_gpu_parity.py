@@ -1127,3 +1127,32 @@ def test_lossy_fill_attenuation_reference_criteria(torch_dev):
     assert abs(s21 - s21_ana) / s21_ana < 0.35
     assert s21 < abs(S0[1, 0])
     assert np.abs(S - So).max() <= 1e-6
+
+
+def test_port_abc_scale_sweep_reference_criteria(torch_dev):
+    """tests/test_abc_scaling_sweep.cpp:40-186 on examples/rect_waveguide.msh at 10 GHz: the
+    port ABC operator j beta M_s scaled by the test's 17 values of port_abc_scale; the
+    scale with the smallest |1 - |S21|| lies in [0.9, 1.1], and at 1.0 |S21| > 0.98 and
+    |S11| < 0.05."""
+    from edgefem_amd import pyedgefem as em
+    _p("[gpu] port ABC scale sweep, rect_waveguide.msh, 10 GHz")
+    pm = em.load_gmsh(golden("rect_waveguide.msh"))
+    bc = em.build_edge_pec(pm, 1)
+    f = 10e9
+    mode = em.solve_te10_mode(em.RectWaveguidePort(0.02286, 0.01016), f)
+    ports = [em.build_wave_port_2d(pm, t, mode, bc.dirichlet_edges, (math.pi / 0.02286) ** 2) for t in (2, 3)]
+    alphas = [0.1, 0.2, 0.3, 0.35, 0.4, 0.45, 0.5, 0.55, 0.6, 0.65, 0.7, 0.8, 0.9, 1.0, 1.2, 1.5, 2.0]
+    err = []
+    at_one = None
+    for a in alphas:
+        params = em.MaxwellParams()
+        params.omega = 2 * math.pi * f
+        params.port_abc_scale = a
+        S = em.calculate_sparams_eigenmode(pm, params, bc, ports)
+        err.append(abs(1.0 - abs(S[1, 0])))
+        if a == 1.0:
+            at_one = S
+    best = alphas[int(np.argmin(err))]
+    _p(f"    best scale {best}; at 1.0 |S21| {abs(at_one[1, 0]):.4f} |S11| {abs(at_one[0, 0]):.4f}")
+    assert 0.9 <= best <= 1.1
+    assert abs(at_one[1, 0]) > 0.98 and abs(at_one[0, 0]) < 0.05
