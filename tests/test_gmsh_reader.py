@@ -129,3 +129,20 @@ def test_errors(tmp_path):
         f.write(HAND.replace("4 4 2 100 100 10 20 30 40", "4 4 2 100 100 10 20 x 40"))
     with pytest.raises(_capi.EdgeFemError, match="malformed"):
         _read(bad2)
+
+
+def test_dielectric_slab_fixture_regions():
+    """tests/test_dielectric_slab_fresnel.cpp:93-170 on its own fixture
+    (examples/dielectric_slab.msh): more than 100 nodes, non-empty port surfaces 1 and 2,
+    and every tet in the air (100, 102) or dielectric (101) regions, read the same by the
+    library's reader and the oracle."""
+    from oracle import edgefem_oracle as O
+    path = os.path.join(ROOT, "tests", "golden", "dielectric_slab.msh")
+    d = _read(path)
+    assert d["node_ids"].size > 100
+    assert np.all(np.isin(d["tet_phys"], [100, 101, 102]))
+    assert np.count_nonzero(d["tet_phys"] == 101) > 0 and np.count_nonzero(d["tet_phys"] != 101) > 0
+    m = O.load_gmsh_v2(path)
+    for tag in (1, 2):
+        ids, xy, tri, lines, sel = O.extract_surface_mesh(m, tag)
+        assert tri.shape[0] > 0 and sel.size == np.count_nonzero(d["tri_phys"] == tag)
