@@ -1,6 +1,7 @@
 // C ABI housekeeping (error string, version) and the port-overlap S extraction.
 #include "efm_internal.h"
 
+#include <algorithm>
 #include <string>
 
 namespace efm {
@@ -47,21 +48,58 @@ __global__ void __launch_bounds__(kBlock)
 
 } // namespace
 
+namespace {
+
+// Staging buffer of efm_port_overlap: one per device and host thread, grown on demand and
+// kept until the process ends (never freed, so no hipFree runs at thread or process exit).
+constexpr int kMaxDevices = 64;
+thread_local void *t_overlap_buf[kMaxDevices] = {};
+thread_local size_t t_overlap_bytes[kMaxDevices] = {};
+
+int overlap_staging(size_t bytes, double2 **out) {
+  int dev = 0;
+  EFM_HIP_CHECK(hipGetDevice(&dev));
+  EFM_REQUIRE(dev >= 0 && dev < kMaxDevices, "efm_port_overlap: device ordinal out of range");
+  if (t_overlap_bytes[dev] < bytes) {
+    const size_t want = std::max<size_t>(bytes, 4096);
+    void *p = nullptr;
+    if (hipMalloc(&p, want) != hipSuccess) {
+      set_error("hipMalloc of " + std::to_string(want) + " bytes failed");
+      return EFM_ERR_NOMEM;
+    }
+    if (t_overlap_buf[dev]) {
+      // the previous buffer may still be read by work queued on another stream
+      EFM_HIP_CHECK(hipDeviceSynchronize());
+      (void)hipFree(t_overlap_buf[dev]);
+    }
+    t_overlap_buf[dev] = p;
+    t_overlap_bytes[dev] = want;
+  }
+  *out = static_cast<double2 *>(t_overlap_buf[dev]);
+  return EFM_OK;
+}
+
+int overlap_run(int64_t n, int n_ports, int64_t len, const int32_t *gidx, const double *gval,
+                const double *X, int nrhs, double *h_out, double2 *dev_out, hipStream_t st) {
+  hipLaunchKernelGGL(k_port_overlap, dim3(n_ports * nrhs), dim3(kBlock), 0, st, n, len, gidx,
+                     (const double2 *)gval, (const double2 *)X, nrhs, dev_out);
+  EFM_LAUNCH_CHECK();
+  EFM_HIP_CHECK(hipMemcpyAsync(h_out, dev_out, (size_t)n_ports * nrhs * sizeof(double2),
+                               hipMemcpyDeviceToHost, st));
+  EFM_HIP_CHECK(hipStreamSynchronize(st));
+  return EFM_OK;
+}
+
+} // namespace
+
 extern "C" int efm_port_overlap(int64_t n, int n_ports, int64_t len, const int32_t *gidx,
                                 const double *gval, const double *X, int nrhs, double *h_out,
                                 void *stream) {
   EFM_REQUIRE(gidx && gval && X && h_out, "efm_port_overlap: NULL argument");
   EFM_REQUIRE(n > 0 && n_ports > 0 && len > 0 && nrhs > 0, "efm_port_overlap: bad sizes");
-  hipStream_t st = (hipStream_t)stream;
-  DevBuf out;
-  int rc = out.alloc((size_t)n_ports * nrhs * sizeof(double2));
+  double2 *buf = nullptr;
+  int rc = overlap_staging((size_t)n_ports * nrhs * sizeof(double2), &buf);
   if (rc != EFM_OK)
     return rc;
-  hipLaunchKernelGGL(k_port_overlap, dim3(n_ports * nrhs), dim3(kBlock), 0, st, n, len, gidx,
-                     (const double2 *)gval, (const double2 *)X, nrhs, out.as<double2>());
-  EFM_LAUNCH_CHECK();
-  EFM_HIP_CHECK(hipMemcpyAsync(h_out, out.p, (size_t)n_ports * nrhs * sizeof(double2),
-                               hipMemcpyDeviceToHost, st));
-  EFM_HIP_CHECK(hipStreamSynchronize(st));
-  return EFM_OK;
+  return overlap_run(n, n_ports, len, gidx, gval, X, nrhs, h_out, buf, (hipStream_t)stream);
 }

@@ -1485,7 +1485,7 @@ __global__ void __launch_bounds__(kBlock)
 //   MODE 3 (plain SpMV):    none
 // written as one extra partial block, so the finalisation sums one block more.
 // ---------------------------------------------------------------------------
-constexpr int kMaxPorts = 8;
+constexpr int kMaxPorts = 32;  // low-rank terms
 
 template <int R, int MODE>
 __global__ void __launch_bounds__(kBlock)
@@ -2739,7 +2739,7 @@ extern "C" int efm_solver_set_matrix(efm_solver *s, const void *values, void *st
 extern "C" int efm_solver_set_lowrank(efm_solver *s, int n_terms, int64_t len, const int32_t *gidx,
                                       const double *gval, const double *h_coef, void *stream) {
   EFM_REQUIRE(s, "efm_solver_set_lowrank: NULL solver");
-  EFM_REQUIRE(n_terms >= 0 && n_terms <= kMaxPorts, "efm_solver_set_lowrank: 0..8 terms");
+  EFM_REQUIRE(n_terms >= 0 && n_terms <= kMaxPorts, "efm_solver_set_lowrank: 0..32 terms");
   if (n_terms == 0 || len <= 0) {
     s->lr_P = 0;
     s->lr_len = 0;

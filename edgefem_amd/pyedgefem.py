@@ -959,10 +959,65 @@ def assemble_maxwell(mesh: Mesh, params: MaxwellParams, bc: BC, ports: Sequence[
 
 
 def _solver_nrhs(P: int) -> int:
+    """Width of the solver's right-hand-side block for P excitations: the smallest
+    supported width (1, 2, 3, 4 or 8) that holds them, 8 when there are more (they are then
+    solved in blocks of 8, see _block_solve)."""
     for r in (1, 2, 3, 4, 8):
         if P <= r:
             return r
-    raise EdgeFemError("at most 8 ports per solve")
+    return 8
+
+
+def _block_solve(solver, n: int, P: int, rhs_block, tol: float, max_iterations: int, method: int):
+    """Solve the P excitations in blocks of the solver's width R (the reference has no port
+    limit; it solves one excitation at a time).  rhs_block(idx) returns the device (n, R)
+    complex128 right-hand sides of excitations idx (len(idx) <= R; columns past len(idx) are
+    zero).  Returns X as a contiguous device (n, P) tensor and per-excitation iters/resid."""
+    import torch
+    R = solver.nrhs
+    if P <= R:
+        Xb, info = solver.solve(rhs_block(list(range(P))), tol=tol, max_iterations=max_iterations, method=method)
+        Xb = Xb.reshape(n, R)
+        X = Xb if P == R else Xb[:, :P].contiguous()
+        return X, dict(iters=info["iters"][:P].copy(), resid=info["resid"][:P].copy())
+    X = None
+    iters = np.zeros(P, np.int32)
+    resid = np.zeros(P, np.float64)
+    for b0 in range(0, P, R):
+        idx = list(range(b0, min(P, b0 + R)))
+        Xb, info = solver.solve(rhs_block(idx), tol=tol, max_iterations=max_iterations, method=method)
+        Xb = Xb.reshape(n, R)
+        if X is None:
+            X = torch.empty((n, P), dtype=Xb.dtype, device=Xb.device)
+        X[:, b0:b0 + len(idx)] = Xb[:, :len(idx)]
+        iters[idx] = info["iters"][:len(idx)]
+        resid[idx] = info["resid"][:len(idx)]
+    return X, dict(iters=iters, resid=resid)
+
+
+def _dense_rhs_block(Bh, R: int):
+    """rhs_block for _block_solve from a host (n, P) array of right-hand sides."""
+    import torch
+
+    def block(idx):
+        blk = np.zeros((Bh.shape[0], R), np.complex128)
+        blk[:, :len(idx)] = Bh[:, idx]
+        return _capi.to_dev(blk, torch.complex128)
+    return block
+
+
+def _host_rhs_block(n: int, R: int, cols):
+    """rhs_block for _block_solve from host columns: cols(i) -> (edges, values) of column i."""
+    import torch
+
+    def block(idx):
+        Bh = np.zeros((n, R), np.complex128)
+        for c, i in enumerate(idx):
+            e, v = cols(i)
+            if e is not None and len(e):
+                np.add.at(Bh[:, c], e, v)
+        return _capi.to_dev(Bh, torch.complex128)
+    return block
 
 
 def calculate_sparams(mesh: Mesh, params: MaxwellParams, bc: BC, ports: Sequence[WavePort],
@@ -985,13 +1040,9 @@ def calculate_sparams(mesh: Mesh, params: MaxwellParams, bc: BC, ports: Sequence
     topo = mesh.topology
     n = mesh.num_edges()
     R = _solver_nrhs(P)
-    Bh = np.zeros((n, R), np.complex128)
     frees = [_free_port(p, dmask) for p in ports]
-    for i, port in enumerate(ports):
-        if _valid_port(port):
-            e, w = frees[i]
-            Bh[e, i] = 2.0 / cmath.sqrt(complex(port.mode.Z0)) * w
-    B = _capi.to_dev(Bh, torch.complex128)
+    rhs = _host_rhs_block(n, R, lambda i: (frees[i][0], 2.0 / cmath.sqrt(complex(ports[i].mode.Z0)) * frees[i][1])
+                          if _valid_port(ports[i]) else (None, None))
     solver = _capi.KrylovSolver(topo.row_ptr, topo.col_idx, nrhs=R)
     if preconditioner == "aux":
         solver.set_gradient(topo, dmask)
@@ -1000,7 +1051,7 @@ def calculate_sparams(mesh: Mesh, params: MaxwellParams, bc: BC, ports: Sequence
     solver.set_matrix(Am.values)
     tol = 1e-14 if opts.use_direct else float(opts.tolerance)
     method = _capi.EFM_METHOD_COCG if Am.symmetric else _capi.EFM_METHOD_BICGSTAB
-    X, info = solver.solve(B, tol=tol, max_iterations=int(opts.max_iterations), method=method)
+    X, info = _block_solve(solver, n, P, rhs, tol, int(opts.max_iterations), method)
     L = max([f[0].size for f in frees] + [1])
     gi = np.full((P, L), -1, np.int32)
     gv = np.zeros((P, L), np.complex128)
@@ -1008,7 +1059,7 @@ def calculate_sparams(mesh: Mesh, params: MaxwellParams, bc: BC, ports: Sequence
         gi[j, :e.size] = e
         gv[j, :e.size] = w
     V = _capi.port_overlap(_capi.to_dev(gi, torch.int32), _capi.to_dev(gv, torch.complex128),
-                           X, R)[:, :P]                                # V[j, i] = w_j^H x_i
+                           X, P)                                       # V[j, i] = w_j^H x_i
     S = np.full((P, P), np.nan + 1j * np.nan, np.complex128)
     name = ("COCG" if Am.symmetric else "BiCGSTAB") + "+" + preconditioner
     for i, port in enumerate(ports):
@@ -1021,7 +1072,7 @@ def calculate_sparams(mesh: Mesh, params: MaxwellParams, bc: BC, ports: Sequence
         for j in range(P):
             S[j, i] = (V[j, i] - vinc) / vinc if i == j else V[j, i] / vinc
     if _state is not None:
-        _state.update(dict(A=Am, B=B, X=X, info=info))
+        _state.update(dict(A=Am, X=X, info=info))
     return S
 
 
@@ -1041,20 +1092,15 @@ def normalize_port_weights(mesh: Mesh, params: MaxwellParams, bc: BC, ports: Lis
     topo = mesh.topology
     n = mesh.num_edges()
     R = _solver_nrhs(P)
-    Bh = np.zeros((n, R), np.complex128)
     frees = [_free_port(p, dmask) for p in ports]
     use = [_valid_port(p) for p in ports]
-    for i in range(P):
-        if use[i]:
-            e, w = frees[i]
-            Bh[e, i] = w
+    rhs = _host_rhs_block(n, R, lambda i: frees[i] if use[i] else (None, None))
     solver = _capi.KrylovSolver(topo.row_ptr, topo.col_idx, nrhs=R)
     solver.set_gradient(topo, dmask)
     solver.set_matrix(asm.A.values)
     tol = 1e-14 if opts.use_direct else float(opts.tolerance)
-    X, info = solver.solve(_capi.to_dev(Bh, torch.complex128), tol=tol, max_iterations=int(opts.max_iterations),
-                           method=_capi.EFM_METHOD_COCG)
-    Xh = X.reshape(n, R).cpu().numpy()
+    X, info = _block_solve(solver, n, P, rhs, tol, int(opts.max_iterations), _capi.EFM_METHOD_COCG)
+    Xh = X.cpu().numpy()
     for i, port in enumerate(ports):
         if not use[i]:
             continue
@@ -1118,12 +1164,15 @@ def frequency_sweep(mesh: Mesh, params: MaxwellParams, bc: BC, ports: Sequence[W
         abc_rows = np.unique(mesh.tri_edges[sel].ravel()).astype(np.int32)
         abc_rows = abc_rows[dmask[abc_rows] == 0]
     R = _solver_nrhs(P)
-    Bh = np.zeros((n, R), np.complex128)
-    for i, port in enumerate(ports):
-        if _valid_port(port):
-            e, w = frees[i]
-            Bh[e, i] = 2.0 / cmath.sqrt(complex(port.mode.Z0)) * w
-    B = _capi.to_dev(Bh, torch.complex128)
+    blocks = {}
+    rhs_host = _host_rhs_block(n, R, lambda i: (frees[i][0], 2.0 / cmath.sqrt(complex(ports[i].mode.Z0)) * frees[i][1])
+                               if _valid_port(ports[i]) else (None, None))
+
+    def rhs(idx):   # the right-hand sides do not depend on the frequency: upload each block once
+        key = tuple(idx)
+        if key not in blocks:
+            blocks[key] = rhs_host(idx)
+        return blocks[key]
     L = max([f[0].size for f in frees] + [1])
     gi = np.full((P, L), -1, np.int32)
     gv = np.zeros((P, L), np.complex128)
@@ -1143,8 +1192,8 @@ def frequency_sweep(mesh: Mesh, params: MaxwellParams, bc: BC, ports: Sequence[W
         if abc_rows.size:
             _capi.add_diagonal(topo, abc_rows, np.full(abc_rows.size, 1j * k0), A)
         solver.set_matrix(A)
-        X, info = solver.solve(B, tol=tol, max_iterations=int(opts.max_iterations), method=method)
-        V = _capi.port_overlap(gi_d, gv_d, X, R)[:, :P]
+        X, info = _block_solve(solver, n, P, rhs, tol, int(opts.max_iterations), method)
+        V = _capi.port_overlap(gi_d, gv_d, X, P)
         S = np.full((P, P), np.nan + 1j * np.nan, np.complex128)
         for i, port in enumerate(ports):
             if not (info["resid"][i] <= tol or info["iters"][i] == 0):
@@ -1400,7 +1449,7 @@ def calculate_sparams_periodic(mesh: Mesh, params: MaxwellParams, bc: BC, pbc: P
     if m.size:
         zero[s_[dmask[s_] == 0]] = True
     frees = [_free_port(p, dmask) for p in ports]
-    Bh = np.zeros((n, R), np.complex128)
+    Bh = np.zeros((n, P), np.complex128)
     for i, port in enumerate(ports):
         if _valid_port(port):
             e, w = frees[i]
@@ -1417,9 +1466,9 @@ def calculate_sparams_periodic(mesh: Mesh, params: MaxwellParams, bc: BC, pbc: P
         solver.set_lowrank(*_pack_lowrank(Am.lowrank))
     solver.set_matrix(Am.values)
     tol = 1e-14 if opts.use_direct else float(opts.tolerance)
-    X, info = solver.solve(_capi.to_dev(Bh, torch.complex128), tol=tol, max_iterations=int(opts.max_iterations),
-                           method=_capi.EFM_METHOD_BICGSTAB)
-    Xh = X.reshape(n, R).cpu().numpy().copy()
+    X, info = _block_solve(solver, n, P, _dense_rhs_block(Bh, R), tol, int(opts.max_iterations),
+                           _capi.EFM_METHOD_BICGSTAB)
+    Xh = X.cpu().numpy().copy()
     if m.size:
         Xh[s_, :] = rec[:, None] * Xh[m, :]
     S = np.full((P, P), np.nan + 1j * np.nan, np.complex128)
@@ -1867,7 +1916,8 @@ def _port_face_eps_mu(mesh: Mesh, eps: np.ndarray, mu: np.ndarray, surface_tag: 
 
 class _PortOps:
     """Per-port data of the eigenmode formulation, computed once per (mesh, ports):
-    M_s on the device, the normalised port vector e and g = M_s e on the port edges."""
+    M_s on the device, the normalised port vector e and g = M_s e on the free edges of the
+    port surface (the rows where M_s can be nonzero)."""
 
     def __init__(self, mesh: Mesh, bc: BC, ports: Sequence[WavePort]):
         import torch
@@ -1887,12 +1937,17 @@ class _PortOps:
             w = np.asarray(port.weights, np.complex128)
             keep = ~dmask[edges]
             pe = edges[keep]
-            # rows of M_s on the free port edges (host copy of a tiny block)
-            pos = np.concatenate([np.arange(rp[r], rp[r + 1]) for r in pe]) if pe.size else np.zeros(0, np.int64)
+            # g = M_s e over every row of M_s, i.e. every free edge of the port surface, so
+            # that a port whose edge list is a strict subset of its surface still gets the
+            # reference's full M_s e (assemble_maxwell.cpp:749-752, 773-783)
+            surf = np.unique(mesh.tri_edges[mesh.tri_phys == port.surface_tag].ravel())
+            srows = surf[~dmask[surf]] if surf.size else surf
+            pos = (np.concatenate([np.arange(rp[r], rp[r + 1]) for r in srows]) if srows.size
+                   else np.zeros(0, np.int64))
             posd = torch.from_numpy(pos).to(topo.col_idx.device)
             cols = topo.col_idx[posd].cpu().numpy()
             vals = Ms[posd].cpu().numpy()
-            rows = np.repeat(pe, [rp[r + 1] - rp[r] for r in pe])
+            rows = np.repeat(srows, [rp[r + 1] - rp[r] for r in srows])
             e_full = np.zeros(n, np.complex128)
             e_full[pe] = w[keep]
             ge = np.zeros(n, np.complex128)
@@ -1902,8 +1957,8 @@ class _PortOps:
                 e_full /= math.sqrt(nsq)
                 ge /= math.sqrt(nsq)
             self.e_vecs.append(e_full)
-            self.g_idx.append(pe.astype(np.int32))
-            self.g_val.append(ge[pe])
+            self.g_idx.append(srows.astype(np.int32))
+            self.g_val.append(ge[srows])
         L = max([len(i) for i in self.g_idx] + [1])
         P = len(ports)
         gi = np.full((P, L), -1, np.int32)
@@ -1915,13 +1970,15 @@ class _PortOps:
         self.gv_dev = _capi.to_dev(gv, torch.complex128)
 
     def rhs(self, betas, scale, nrhs_order):
-        """B (n, len(nrhs_order)) with column a = 2 j beta_a scale g_a on the port edges."""
+        """B (n, len(nrhs_order)) with column a = 2 j beta_a scale g_a on the port edges
+        (a None entry gives a zero column)."""
         import torch
         n = self.mesh.num_edges()
         R = len(nrhs_order)
         B = np.zeros((n, R), np.complex128)
         for col, a in enumerate(nrhs_order):
-            B[self.g_idx[a], col] = 2.0 * (1j * scale * betas[a]) * self.g_val[a]
+            if a is not None:
+                B[self.g_idx[a], col] = 2.0 * (1j * scale * betas[a]) * self.g_val[a]
         return _capi.to_dev(B, torch.complex128)
 
 
@@ -1971,13 +2028,15 @@ def calculate_sparams_eigenmode(mesh: Mesh, params: MaxwellParams, bc: BC, ports
     k0 = _k0(params)
     coefs = [1j * params.port_abc_scale * b for b in betas]
     A = _capi.combine(topo, K, M, k0 * k0, ops.Ms, coefs, out_dtype=matrix_dtype or torch.complex128)
-    B = ops.rhs(betas, params.port_abc_scale, list(range(P)))
-    solver = _capi.KrylovSolver(topo.row_ptr, topo.col_idx, nrhs=P, dtype=A.dtype)
+    R = _solver_nrhs(P)
+    n = mesh.num_edges()
+    rhs = lambda idx: ops.rhs(betas, params.port_abc_scale, idx + [None] * (R - len(idx)))
+    solver = _capi.KrylovSolver(topo.row_ptr, topo.col_idx, nrhs=R, dtype=A.dtype)
     if preconditioner == "aux":
         solver.set_gradient(topo, dmask)
     solver.set_matrix(A)
     tol = 1e-14 if opts.use_direct else float(opts.tolerance)
-    X, info = solver.solve(B, tol=tol, max_iterations=int(opts.max_iterations), method=_capi.EFM_METHOD_COCG)
+    X, info = _block_solve(solver, n, P, rhs, tol, int(opts.max_iterations), _capi.EFM_METHOD_COCG)
     ov = _capi.port_overlap(ops.gi_dev, ops.gv_dev, X, P)   # ov[j, a] = g_j^H x_a
     S = ov - np.eye(P)
     for a in range(P):
@@ -1987,7 +2046,8 @@ def calculate_sparams_eigenmode(mesh: Mesh, params: MaxwellParams, bc: BC, ports
                              "S-parameters may be unreliable.\n")
             S[:, a] = np.nan + 1j * np.nan
     if _state is not None:
-        _state.update(dict(A=A, B=B, X=X, info=info, betas=betas, ops=ops, K=K, M=M))
+        _state.update(dict(A=A, B=rhs(list(range(P))) if P <= R else None, X=X, info=info, betas=betas,
+                           ops=ops, K=K, M=M))
     return S
 
 
@@ -1997,6 +2057,9 @@ def check_passivity(S) -> bool:
     if not ok:
         sys.stderr.write(f"WARNING: S-matrix is not passive (max singular value {s.max():.6f})\n")
     return ok
+
+
+_SWEEP_MAX_PORTS = 4   # port masses per system in the batched sweep kernels
 
 
 def frequency_sweep_eigenmode(mesh: Mesh, params: MaxwellParams, bc: BC, ports: Sequence[WavePort],
@@ -2023,9 +2086,10 @@ def frequency_sweep_eigenmode(mesh: Mesh, params: MaxwellParams, bc: BC, ports: 
     if P == 0 or freqs.size == 0:
         return out
     opts = solver_options or SolveOptions()
-    if params._frequency_dependent():
+    if params._frequency_dependent() or P > _SWEEP_MAX_PORTS:
         # PML or dispersive media: no K, M shared by the frequencies; the matrix is
-        # assembled and solved per frequency, as the reference's sweep does (sweep.cpp:330-346)
+        # assembled and solved per frequency, as the reference's sweep does (sweep.cpp:330-346).
+        # More ports than the batched kernels carry per system take the same route.
         for i, f in enumerate(freqs):
             out[i] = calculate_sparams_eigenmode(mesh, _params_at(params, 2.0 * math.pi * f), bc, ports,
                                                  solver_options=opts)

@@ -158,7 +158,7 @@ int efm_solver_set_gradient_complex(efm_solver *s, int64_t n_nodes, const int32_
  * src/assemble_maxwell.cpp:209-242, applied inside the operator instead of as dense
  * blocks added to the sparsity pattern; the Jacobi diagonal includes it.  Ports must be
  * disjoint (an edge in two lists is rejected).  n_terms = 0 removes the term.  Call before
- * efm_solver_set_matrix.  At most 8 terms. */
+ * efm_solver_set_matrix.  At most 32 terms. */
 int efm_solver_set_lowrank(efm_solver *s, int n_terms, int64_t len, const int32_t *gidx,
                            const double *gval, const double *h_coef, void *stream);
 

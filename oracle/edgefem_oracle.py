@@ -807,6 +807,65 @@ def sparams_eigenmode(mesh: OMesh, p: Params, dirichlet: np.ndarray, ports: Sequ
     return S
 
 
+def aux_preconditioner(mesh: OMesh, dirichlet: np.ndarray, A: sp.csr_matrix):
+    """P = D^-1 + G diag(G^T A G)^-1 G^T with G the signed edge-node incidence (tail -1,
+    head +1 by gmsh ID, Dirichlet rows removed): point Jacobi plus one nodal Jacobi step
+    on the gradient space, the null space of the curl-curl part.  Returned as a function
+    of an (n, k) block."""
+    n = A.shape[0]
+    order = np.argsort(mesh.node_ids)
+    pos = lambda ids: order[np.searchsorted(mesh.node_ids, ids, sorter=order)]
+    ends = np.stack([pos(mesh.edges[:, 0]), pos(mesh.edges[:, 1])], 1)
+    G = sp.csr_matrix((np.tile([-1.0, 1.0], n), (np.repeat(np.arange(n), 2), ends.ravel())),
+                      shape=(n, mesh.xyz.shape[0]))
+    G = (sp.diags((~dirichlet).astype(float)) @ G).tocsr()
+    GT = G.T.tocsr()
+    dn = (GT @ A @ G).diagonal()
+    dninv = np.where(dn != 0, 1.0 / np.where(dn != 0, dn, 1.0), 0.0)
+    dinv = 1.0 / A.diagonal()
+    return lambda r: dinv[:, None] * r + G @ (dninv[:, None] * (GT @ r))
+
+
+def cocg_solve(A: sp.csr_matrix, B: np.ndarray, P, tol: float, maxit: int, refine: int = 0,
+               log=None):
+    """FP64 stand-in for the LU solve where a sparse factor of A does not fit in memory:
+    preconditioned conjugate orthogonal CG (the bilinear form u^T v, valid because A is
+    complex symmetric) on every column of B, to ||b - A x|| <= tol ||b|| with the true
+    residual checked at the end, then `refine` steps of x += COCG(A, b - A x).  Returns X and
+    the final true relative residual of every column."""
+    def cocg(Rhs, tol_):
+        X = np.zeros_like(Rhs)
+        R = Rhs.copy()
+        Z = P(R)
+        Pd = Z.copy()
+        rho = np.einsum("ij,ij->j", R, Z)
+        nb = np.linalg.norm(Rhs, axis=0)
+        for it in range(maxit):
+            Q = A @ Pd
+            al = rho / np.einsum("ij,ij->j", Pd, Q)
+            X += al * Pd
+            R -= al * Q
+            rn = np.linalg.norm(R, axis=0) / nb
+            if log is not None and it % 500 == 0:
+                log(f"  cocg it {it}: {rn}")
+            if np.all(rn <= tol_):
+                break
+            Z = P(R)
+            rho_n = np.einsum("ij,ij->j", R, Z)
+            Pd = Z + (rho_n / rho) * Pd
+            rho = rho_n
+        return X, it + 1
+
+    X, its = cocg(B, tol)
+    iters = [its]
+    for _ in range(refine):
+        D, its = cocg(B - A @ X, 1e-6)
+        X = X + D
+        iters.append(its)
+    res = np.linalg.norm(B - A @ X, axis=0) / np.linalg.norm(B, axis=0)
+    return X, res, iters
+
+
 def calculate_sparams(mesh: OMesh, p: Params, dirichlet: np.ndarray, ports: Sequence[OPort]):
     """Rank-1 port loading path with FP64 LU in place of the Krylov solve
     (assemble_maxwell.cpp:352-394): V_j = sum conj(w_k) x_k over free edges,

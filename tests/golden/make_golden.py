@@ -6,6 +6,15 @@ c1     c1_sparams_10ghz.npz: S of the 16x8x32 WR-90 guide (seed 1) at 10 GHz, th
        port cutoffs and SHA-256 digests of the topology / CSR arrays.
 sweep  c3_sweep.npz: S at the 128 frequencies linspace(8.2e9, 12.4e9, 128) on the same
        mesh (ports built once; only beta changes with frequency).
+c2     c2_sparams_10ghz.npz: S of the 48x24x192 WR-90 guide, 100 mm, with the lossy
+       dielectric slab z in [30, 70] mm (per-tet eps' ~ U[1, 4] from seed 2, tan d 0.002;
+       mesh seed 1) at 10 GHz, plus SHA-256 digests of its topology and CSR arrays.
+       SuperLU does not fit this system (fill measured on 48x24 cross-sections 6, 12 and
+       24 cells long: 33 M, 112 M and 350 M entries in L+U, about 3.1x per doubling of
+       the length, so about 10^10 entries and 160 GB at 192 cells), so the FP64 solve
+       here is the oracle's COCG with the auxiliary-space preconditioner to a relative
+       residual of 1e-13 followed by refinement steps x += COCG(A, b - A x); the npz
+       records the method, the iteration counts and the true FP64 residual reached.
 """
 import hashlib
 import math
@@ -79,8 +88,47 @@ def make_sweep():
     print("sweep", S.shape, np.abs(S[:, 1, 0]).min())
 
 
+def c2_setup():
+    g = mg.box_mesh(mg.WR90_A, mg.WR90_B, 100e-3, 48, 24, 192, seed=1)
+    eps = mg.slab_eps(g, z0=30e-3, z1=70e-3, lo=1.0, hi=4.0, tan_d=0.002, seed=2)
+    m = O.mesh_from_arrays(g.node_ids, g.xyz, g.tet_ids, g.tet_phys, g.tri_ids, g.tri_phys)
+    pec = O.build_edge_pec(m, 1)
+    a = mg.WR90_A
+    kc, beta, Z0 = O.solve_te10_mode(a, 10e9)
+    ports = [O.build_wave_port_2d(m, t, kc, Z0, beta, pec, (math.pi / a) ** 2) for t in (2, 3)]
+    return m, pec, ports, eps
+
+
+def make_c2():
+    import time
+    t0 = time.time()
+    log = lambda s: print(f"[{time.time() - t0:7.1f}s] {s}", flush=True)
+    m, pec, ports, eps = c2_setup()
+    log(f"mesh: {m.n_edges} edges")
+    info = {}
+
+    def solve(A, B):
+        P = O.aux_preconditioner(m, pec, A.tocsr())
+        X, res, iters = O.cocg_solve(A.tocsr(), B, P, tol=1e-13, maxit=100000, refine=2, log=log)
+        info.update(res=res, iters=iters)
+        log(f"residual {res} iterations {iters}")
+        return X
+
+    S = O.sparams_eigenmode(m, O.Params(omega=2 * math.pi * 10e9, eps_tet=eps), pec, ports, solver=solve)
+    d = topo_digests(m)
+    np.savez(os.path.join(HERE, "c2_sparams_10ghz.npz"), S=S,
+             kc2=np.array([p.kc_sq_discrete for p in ports]), n_edges=m.n_edges,
+             nnz=O.volume_pattern(m).nnz, residual=info["res"], iterations=np.array(info["iters"]),
+             method="FP64 COCG + auxiliary-space preconditioner to 1e-13, then 2 refinement steps "
+                    "(SuperLU does not fit this system)",
+             **{"sha_" + k: v for k, v in d.items()})
+    log(f"c2 S = {S}")
+
+
 if __name__ == "__main__":
     what = sys.argv[1:] or ["c1", "sweep"]
+    if "c2" in what:
+        make_c2()
     if "c1" in what:
         make_c1()
     if "sweep" in what:

@@ -1156,3 +1156,108 @@ def test_port_abc_scale_sweep_reference_criteria(torch_dev):
     _p(f"    best scale {best}; at 1.0 |S21| {abs(at_one[1, 0]):.4f} |S11| {abs(at_one[0, 0]):.4f}")
     assert 0.9 <= best <= 1.1
     assert abs(at_one[1, 0]) > 0.98 and abs(at_one[0, 0]) < 0.05
+
+
+def test_many_ports_solved_in_blocks_vs_oracle(torch_dev):
+    """More excitations than one right-hand-side block holds (the reference has no port
+    limit): ten disjoint lumped strip ports (five alternate x-strips on each port face of a
+    20x4x8 guide) through normalize_port_weights and calculate_sparams (blocks of 8 + 2, ten
+    low-rank terms), and five wave ports (the two faces repeated, which adds j beta M_s once
+    per port, as the oracle does) through calculate_sparams_eigenmode (one block of 8 with
+    three zero columns), each against the oracle's dense/SuperLU restatement."""
+    from edgefem_amd import meshgen as mg
+    from edgefem_amd import pyedgefem as em
+    from oracle import edgefem_oracle as O
+    _p("[gpu] ten lumped ports and five wave ports, block solves")
+    a, b = mg.WR90_A, mg.WR90_B
+    g = mg.box_mesh(a, b, 20e-3, 20, 4, 8, seed=3)
+    phys = g.tri_phys.copy()
+    cx = g.xyz[g.tris].mean(axis=1)[:, 0]
+    strip = np.minimum((cx / (a / 10)).astype(int), 9)
+    tags = []
+    for face in (2, 3):
+        for k in range(0, 10, 2):
+            tag = 10 * face + k
+            phys[(g.tri_phys == face) & (strip == k)] = tag
+            tags.append(tag)
+    pm = em.mesh_from_arrays(g.node_ids, g.xyz, g.tets, g.tet_phys, g.tris, phys)
+    om = O.mesh_from_arrays(g.node_ids, g.xyz, g.tet_ids, g.tet_phys, g.tri_ids, phys)
+    bc = em.build_edge_pec(pm, 1)
+    pec = O.build_edge_pec(om, 1)
+    gports, oports = [], []
+    for tag in tags:
+        cfg = em.LumpedPortConfig()
+        cfg.surface_tag = tag
+        cfg.z0 = 50.0
+        cfg.e_direction = np.array([0.0, 1.0, 0.0])
+        gports.append(em.build_lumped_port(pm, cfg))
+        oports.append(O.build_lumped_port(om, tag, 50.0, (0.0, 1.0, 0.0)))
+    params = em.MaxwellParams()
+    params.omega = 2 * math.pi * 10e9
+    op = O.Params(omega=params.omega)
+    opts = em.SolveOptions()
+    opts.tolerance = 1e-12
+    em.normalize_port_weights(pm, params, bc, gports, opts)
+    oports = O.normalize_port_weights(om, op, pec, oports)
+    for gp, o in zip(gports, oports):
+        np.testing.assert_allclose(gp.weights, o.weights, rtol=1e-8, atol=1e-12)
+    st = {}
+    S = em.calculate_sparams(pm, params, bc, gports, opts, _state=st)
+    So = O.calculate_sparams(om, op, pec, oports)
+    _p(f"    10 lumped ports: |dS| {np.abs(S - So).max():.2e}, iterations {st['info']['iters']}")
+    assert S.shape == (10, 10)
+    assert np.abs(S - So).max() <= 1e-6
+    # five wave ports, eigenmode extraction
+    f = 10e9
+    mode = em.solve_te10_mode(em.RectWaveguidePort(a, b), f)
+    kc2 = (math.pi / a) ** 2
+    pm2 = em.mesh_from_arrays(g.node_ids, g.xyz, g.tets, g.tet_phys, g.tris, g.tri_phys)
+    om2 = O.mesh_from_arrays(g.node_ids, g.xyz, g.tet_ids, g.tet_phys, g.tri_ids, g.tri_phys)
+    bc2 = em.build_edge_pec(pm2, 1)
+    pec2 = O.build_edge_pec(om2, 1)
+    faces = [2, 3, 2, 3, 2]
+    wp = [em.build_wave_port_2d(pm2, t, mode, bc2.dirichlet_edges, kc2) for t in faces]
+    kc, beta, Z0 = O.solve_te10_mode(a, f)
+    owp = [O.build_wave_port_2d(om2, t, kc, Z0, beta, pec2, kc2) for t in faces]
+    S5 = em.calculate_sparams_eigenmode(pm2, params, bc2, wp)
+    So5 = O.sparams_eigenmode(om2, O.Params(omega=2 * math.pi * f), pec2, owp)
+    _p(f"    5 wave ports: |dS| {np.abs(S5 - So5).max():.2e}")
+    assert S5.shape == (5, 5)
+    assert np.abs(S5 - So5).max() <= 1e-6
+
+
+def test_partial_port_edge_list_uses_full_surface_mass(torch_dev):
+    """A wave port whose edge list is a strict subset of its surface (every third edge of
+    port 2 dropped): the right-hand side and S use M_s e over every row of the surface
+    mass, as the reference does (assemble_maxwell.cpp:749-752, 773-783); against the oracle."""
+    import copy
+    from edgefem_amd import meshgen as mg
+    from edgefem_amd import pyedgefem as em
+    from oracle import edgefem_oracle as O
+    _p("[gpu] partial port edge list")
+    a = mg.WR90_A
+    g = mg.box_mesh(a, mg.WR90_B, 20e-3, 10, 4, 8, seed=5)
+    pm = em.mesh_from_arrays(g.node_ids, g.xyz, g.tets, g.tet_phys, g.tris, g.tri_phys)
+    om = O.mesh_from_arrays(g.node_ids, g.xyz, g.tet_ids, g.tet_phys, g.tri_ids, g.tri_phys)
+    bc = em.build_edge_pec(pm, 1)
+    pec = O.build_edge_pec(om, 1)
+    f = 10e9
+    mode = em.solve_te10_mode(em.RectWaveguidePort(a, mg.WR90_B), f)
+    kc2 = (math.pi / a) ** 2
+    wp = [em.build_wave_port_2d(pm, t, mode, bc.dirichlet_edges, kc2) for t in (2, 3)]
+    kc, beta, Z0 = O.solve_te10_mode(a, f)
+    owp = [O.build_wave_port_2d(om, t, kc, Z0, beta, pec, kc2) for t in (2, 3)]
+    keep = np.ones(len(wp[0].edges), bool)
+    keep[::3] = False
+    wp0 = copy.copy(wp[0])
+    wp0.edges = list(np.asarray(wp[0].edges)[keep])
+    wp0.weights = np.asarray(wp[0].weights)[keep]
+    owp[0].edges = np.asarray(owp[0].edges)[keep]
+    owp[0].weights = np.asarray(owp[0].weights)[keep]
+    np.testing.assert_array_equal(np.asarray(wp0.edges), owp[0].edges)
+    params = em.MaxwellParams()
+    params.omega = 2 * math.pi * f
+    S = em.calculate_sparams_eigenmode(pm, params, bc, [wp0, wp[1]])
+    So = O.sparams_eigenmode(om, O.Params(omega=params.omega), pec, owp)
+    _p(f"    |dS| {np.abs(S - So).max():.2e}")
+    assert np.abs(S - So).max() <= 1e-6
