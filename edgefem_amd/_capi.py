@@ -54,8 +54,7 @@ class SolveOpts(C.Structure):
 
 
 class RefineOpts(C.Structure):
-    _fields_ = [("inner_type", C.c_int), ("inner_tolerance", C.c_double), ("max_inner", C.c_int),
-                ("max_outer", C.c_int)]
+    _fields_ = [("inner_type", C.c_int), ("replace_factor", C.c_double)]
 
 
 class SolveResultC(C.Structure):
@@ -452,11 +451,12 @@ class SweepSolver:
         self.max_systems = int(max_systems)
 
     def solve(self, k0sq, coef, B, tol=1e-10, max_iterations=10000, check_every=16, precision="fp64",
-              inner_tol=1e-3, max_inner=1500, max_outer=12):
+              replace_factor=0.01):
         """k0sq (nsys,), coef (nsys, P) complex, B (n, nsys) complex128 device tensor.
-        precision "fp64": one FP64 COCG solve; "fp32": complex-FP32 inner COCG inside FP64
-        iterative refinement (true residual <= tol); "fp64-refined": the same loop with
-        FP64 inner vectors."""
+        precision "fp64": one FP64 COCG solve; "fp32": COCG on complex-FP32 vectors with
+        FP64 reliable updates (the residual replaced by b - A x in FP64 each time it drops by
+        replace_factor; convergence on the true residual <= tol); "fp64-refined": the same
+        with FP64 vectors."""
         torch = _torch()
         k2 = np.ascontiguousarray(np.asarray(k0sq, np.float64))
         nsys = k2.size
@@ -471,8 +471,7 @@ class SweepSolver:
         if precision == "fp64":
             ref = None
         elif precision in ("fp32", "fp64-refined"):
-            ref = C.byref(RefineOpts(EFM_C64 if precision == "fp32" else EFM_C128, float(inner_tol), int(max_inner),
-                                     int(max_outer)))
+            ref = C.byref(RefineOpts(EFM_C64 if precision == "fp32" else EFM_C128, float(replace_factor)))
         else:
             raise EdgeFemError(f"unknown precision {precision!r}")
         _check(self._lib.efm_sweep_solve(self._h, nsys, k2.ctypes.data_as(C.c_void_p), cf.ctypes.data_as(C.c_void_p),

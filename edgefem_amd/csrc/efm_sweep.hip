@@ -36,10 +36,14 @@ constexpr int kMaxPorts = 4;
 struct SwState {
   double2 rho, alpha, beta;
   double2 pend;           // alpha of the last step whose x and r updates are still pending
-  double bnorm2, rnorm2;  // inner solve: right-hand side and recursive residual norms^2
-  double bn_out, rn_out;  // refinement: |b|^2 and the last true residual |b - A x|^2
+  double bnorm2, rnorm2;  // right-hand side and recursive residual norms^2
+  double bn_out, rn_out;  // reliable updates: |b|^2 and the last true residual |b - A x|^2
+  double2 rho_prev;       // reliable updates: rho at the start of the last iteration
+  double rmax2;           // reliable updates: max |r|^2 since the last replacement
   int active, iters, breakdown, total_iters;
-  int out_active, pad[3];
+  int out_active;         // mask of the replacement kernels (residual, accumulate)
+  int want;               // reliable updates: a residual replacement is due
+  int saved_active, pad;
 };
 
 __device__ inline double2 bil(double2 a, double2 b) {
@@ -1182,9 +1186,8 @@ __global__ void k_sw_imag_flags(int64_t nnz, const double2 *__restrict__ K, cons
     atomicOr(flags, f);
 }
 
-// Iterative refinement, outer step: r = b - A_s x in FP64 (x == nullptr: r = b), stored
-// in the inner solve's vector type; partials of |r|^2 (before rounding).  Systems whose
-// outer iteration has finished are skipped.
+// Residual replacement: r = b - A_s x in FP64 (x == nullptr: r = b), stored in the Krylov
+// vector type; partials of |r|^2 (before rounding).  Systems without out_active are skipped.
 template <typename VT>
 __global__ void __launch_bounds__(kBlock)
     k_sw_residual(int64_t n, int B, int P, int groups, const int32_t *__restrict__ row_ptr,
@@ -1223,10 +1226,11 @@ __global__ void __launch_bounds__(kBlock)
   }
 }
 
-// Outer step: x64 += d for the systems whose inner solve just ran.
+// Reliable update: x64 += d (the solution increment carried in the Krylov vector type
+// since the last replacement), then d = 0, for the systems with out_active set.
 template <typename VT>
 __global__ void __launch_bounds__(kBlock)
-    k_sw_accumulate(int64_t n, int B, const VT *__restrict__ d, double2 *__restrict__ x,
+    k_sw_accumulate(int64_t n, int B, VT *__restrict__ d, double2 *__restrict__ x,
                     const SwState *__restrict__ st) {
   const int s = blockIdx.y * kSys + (threadIdx.x & 63);
   const int wv = threadIdx.x >> 6;
@@ -1235,7 +1239,22 @@ __global__ void __launch_bounds__(kBlock)
   for (int64_t e = (int64_t)blockIdx.x * kWaves + wv; e < n; e += (int64_t)gridDim.x * kWaves) {
     const int64_t i = e * B + s;
     x[i] = cadd(x[i], ldv(d + i));
+    stv(d + i, make_double2(0, 0));
   }
+}
+
+// Start of a residual replacement: the systems that asked for one become the active set
+// of the replacement kernels (all = 1: every system, for the final accumulation); the
+// others keep their state in saved_active.
+__global__ void k_sw_ru_begin(int B, SwState *__restrict__ st, int all) {
+  const int s = blockIdx.x * blockDim.x + threadIdx.x;
+  if (s >= B)
+    return;
+  SwState &S = st[s];
+  S.saved_active = S.active;
+  const int sel = all ? 1 : S.want;
+  S.active = sel;
+  S.out_active = sel;
 }
 
 // After a solve: x += pend p for the systems that stopped with a pending step.
@@ -1274,11 +1293,10 @@ __device__ inline void block_sum(double2 (&v)[V], double2 (*sh)[kBlock]) {
     v[i] = sh[i][0];
 }
 
-// Outer finalize, one block per system: true residual norm, outer convergence, and the
-// inner solve's activity (inner active = outer active).
+// Final true residual of every system: rn_out = |b - A x64|^2 from the residual kernel's
+// partials (gxs blocks per group).
 __global__ void __launch_bounds__(kBlock)
-    k_sw_outer_finalize(int B, int gxs, const double2 *__restrict__ ps, SwState *__restrict__ st,
-                        int *__restrict__ active_flag, double tol2, int max_it, int first) {
+    k_sw_true_resid(int B, int gxs, const double2 *__restrict__ ps, SwState *__restrict__ st) {
   const int s = blockIdx.x;
   if (s >= B)
     return;
@@ -1288,22 +1306,8 @@ __global__ void __launch_bounds__(kBlock)
   for (int bx = threadIdx.x; bx < gxs; bx += kBlock)
     v[0] = cadd(v[0], ps[((int64_t)gy * gxs + bx) * 64 + lane]);
   block_sum<1>(v, sh);
-  if (threadIdx.x != 0)
-    return;
-  const double rn2 = v[0].x;
-  SwState &S = st[s];
-  if (first) {
-    S.bn_out = rn2;
-    S.rn_out = rn2;
-    S.total_iters = 0;
-    S.out_active = rn2 > 0.0 ? 1 : 0;
-  } else if (S.out_active) {
-    S.rn_out = rn2;
-    if (rn2 <= tol2 * S.bn_out || S.total_iters >= max_it || S.breakdown)
-      S.out_active = 0;
-  }
-  S.active = S.out_active;
-  active_flag[s] = S.out_active;
+  if (threadIdx.x == 0)
+    st[s].rn_out = v[0].x;
 }
 
 // The system of a finalize block (one block per system, gridDim.x = the padded batch, a
@@ -1387,10 +1391,17 @@ __global__ void __launch_bounds__(kBlock)
 // Scalars of the standard form, one block per system.  STAGE 0 (after the initial u = P r):
 // rho = u^T r, |b|^2; STAGE 1 (after the SpMV): alpha = rho / p^T w; STAGE 2 (after the
 // update): convergence on |r|^2, beta = u^T r / rho.
+// With reliable updates (ru_delta2 > 0) STAGE 2 also tracks max |r|^2 since the last
+// replacement and asks for a replacement (want) once |r|^2 < ru_delta2 * max |r|^2; a
+// system whose recursive residual meets the tolerance pauses (active = 0) until its true
+// residual has been formed.  active_flag = active | want << 1.
+// STAGE 3 (after a replacement: the update's u = P r partials on the true residual):
+// convergence on the true |r|^2, else beta = u^T r / rho_prev with p kept, and the
+// systems that took no part get their saved activity back.
 template <int STAGE>
 __global__ void __launch_bounds__(kBlock)
     k_sw_fin_std(int B, int gx, const double2 *__restrict__ part, SwState *__restrict__ st,
-                 int *__restrict__ active_flag, double tol2, int max_it, int max_inner) {
+                 int *__restrict__ active_flag, double tol2, int max_it, int max_inner, double ru_delta2) {
   const int s = fin_system();
   if (s >= B)
     return;
@@ -1411,15 +1422,21 @@ __global__ void __launch_bounds__(kBlock)
   if (threadIdx.x != 0)
     return;
   SwState &S = st[s];
+  const bool ru = ru_delta2 > 0.0;
   if (STAGE == 0) {
     const double rn2 = v[1].x;
     S.bnorm2 = rn2;
     S.rnorm2 = rn2;
+    S.bn_out = rn2;
+    S.rn_out = rn2;
+    S.rmax2 = rn2;
+    S.want = 0;
     S.iters = 0;
     S.breakdown = 0;
     S.beta = make_double2(0, 0);
     S.pend = make_double2(0, 0);
     S.rho = v[0];
+    S.rho_prev = v[0];
     S.active = (S.active && rn2 > 0.0) ? 1 : 0;
   } else if (STAGE == 1) {
     if (S.active) {
@@ -1431,22 +1448,55 @@ __global__ void __launch_bounds__(kBlock)
         S.alpha = cdiv(S.rho, d);
       }
     }
-  } else if (S.active) {
-    const double2 g = v[0];
-    const double rn2 = v[1].x;
-    S.iters += 1;
-    S.total_iters += 1;
-    S.rnorm2 = rn2;
-    if (rn2 <= tol2 * S.bnorm2) {
-      S.active = 0;
-    } else if (S.total_iters >= max_it || S.iters >= max_inner || (S.rho.x == 0.0 && S.rho.y == 0.0)) {
-      S.active = 0;
+  } else if (STAGE == 2) {
+    if (S.active) {
+      const double2 g = v[0];
+      const double rn2 = v[1].x;
+      S.iters += 1;
+      S.total_iters += 1;
+      S.rnorm2 = rn2;
+      if (rn2 <= tol2 * S.bnorm2) {
+        S.active = 0;
+        if (ru) {  // converged in the recursion: confirm on the true residual
+          S.want = 1;
+          S.rho_prev = S.rho;
+        }
+      } else if (S.total_iters >= max_it || S.iters >= max_inner || (S.rho.x == 0.0 && S.rho.y == 0.0)) {
+        S.active = 0;
+      } else {
+        S.rho_prev = S.rho;
+        S.beta = cdiv(g, S.rho);
+        S.rho = g;
+        if (ru) {
+          S.rmax2 = fmax(S.rmax2, rn2);
+          if (rn2 < ru_delta2 * S.rmax2)
+            S.want = 1;
+        }
+      }
+    }
+  } else {  // STAGE 3
+    if (S.want) {
+      const double2 g = v[0];
+      const double rn2 = v[1].x;
+      S.want = 0;
+      S.rnorm2 = rn2;
+      S.rn_out = rn2;
+      S.rmax2 = rn2;
+      if (rn2 <= tol2 * S.bn_out || S.total_iters >= max_it) {
+        S.active = 0;
+      } else if (S.rho_prev.x == 0.0 && S.rho_prev.y == 0.0) {
+        S.breakdown = 1;
+        S.active = 0;
+      } else {
+        S.beta = cdiv(g, S.rho_prev);
+        S.rho = g;
+        S.active = 1;
+      }
     } else {
-      S.beta = cdiv(g, S.rho);
-      S.rho = g;
+      S.active = S.saved_active;
     }
   }
-  active_flag[s] = S.active;
+  active_flag[s] = S.active | (S.want << 1);
 }
 
 // Frequency-independent diagonals: edge diagonals of K, M, Ms; nodal diag(G^T X G).
@@ -1594,7 +1644,7 @@ struct efm_sweep {
   float tune_ms[5] = {0.f, 0.f, 0.f, 0.f, 0.f};  // SpMV 0, 1, 2; nodal 0, 1
   efm::DevBuf K, M, Ms, rport, dK, dM, dMs, dnK, dnM, dnMs, gsign, en_eff;
   efm::DevBuf x, r, p, sv, u, w, y, part_u, part_s, state, flags, k0sq, coef;
-  efm::DevBuf x64;  // refinement: FP64 solution (allocated on first use)
+  efm::DevBuf x64;  // reliable updates: FP64 solution (allocated on first use)
   efm::DevBuf dninv;  // [N][Bmax] inverse nodal diagonals of the systems of one solve
   efm::DevBuf tucol, tnu, tlidx, tdloc;  // row-tile format of the LDS-tiled SpMV
 };
@@ -1860,11 +1910,19 @@ struct SwRun {
   std::vector<int> hflags;
 
   int n_active(int &count) {
+    int want = 0;
+    return n_active(count, want);
+  }
+  // flags: bit 0 active, bit 1 a residual replacement is due (reliable updates)
+  int n_active(int &count, int &want) {
     EFM_HIP_CHECK(hipMemcpyAsync(hflags.data(), flags, Bp * sizeof(int), hipMemcpyDeviceToHost, st));
     EFM_HIP_CHECK(hipStreamSynchronize(st));
     count = 0;
-    for (int v : hflags)
-      count += v != 0;
+    want = 0;
+    for (int v : hflags) {
+      count += (v & 1) != 0;
+      want += (v & 2) != 0;
+    }
     return EFM_OK;
   }
 
@@ -2048,8 +2106,53 @@ struct SwRun {
     return EFM_OK;
   }
 
-  // Standard preconditioned COCG (see k_sw_pform): the same contract as cocg().
-  template <typename VT> int cocg_std(const double2 *b, double inner_tol2, int max_inner) {
+  // Reliable update (residual replacement) of the systems with `want` set, or of every
+  // system (all = 1: the final accumulation, then the true residual of every system into
+  // rn_out): x64 += x and x = 0; r = b - A x64 in FP64, stored as VT; y and u = P r with the
+  // partials of u^T r and |r|^2; STAGE 3 finalize (convergence on the true residual, else
+  // beta = u^T r / rho_prev with the search direction p kept).
+  template <typename VT> int replace(const double2 *b, int all, int va, double ru_delta2) {
+    const int64_t n = s->n;
+    constexpr int vti = std::is_same<VT, float2>::value ? 1 : 0;
+    VT *x = s->x.as<VT>(), *r = s->r.as<VT>(), *p = s->p.as<VT>();
+    VT *sv = s->sv.as<VT>(), *u = s->u.as<VT>(), *w = s->w.as<VT>(), *y = s->y.as<VT>();
+    double2 *x64 = s->x64.as<double2>();
+    hipLaunchKernelGGL(k_sw_ru_begin, dim3((Bp + kBlock - 1) / kBlock), dim3(kBlock), 0, st, Bp, S, all);
+    EFM_LAUNCH_CHECK();
+    hipLaunchKernelGGL((k_sw_accumulate<VT>), dim3(s->gx_vec, groups), dim3(kBlock), 0, st, n, Bp, x, x64, S);
+    EFM_LAUNCH_CHECK();
+    hipLaunchKernelGGL((k_sw_residual<VT>), dim3(s->gx_spmv, s->gy_spmv), dim3(kBlock), 0, st, n, Bp, P, groups,
+                       s->row_ptr, s->col, s->K.as<double2>(), s->M.as<double2>(), s->Ms.as<double>(),
+                       s->rport.as<int8_t>(), s->k0sq.as<double>(), s->coef.as<double2>(), b,
+                       (const double2 *)x64, r, ps, S);
+    EFM_LAUNCH_CHECK();
+    if (all) {
+      hipLaunchKernelGGL(k_sw_true_resid, dim3(Bp), dim3(kBlock), 0, st, Bp, s->gx_spmv, ps, S);
+      EFM_LAUNCH_CHECK();
+      return EFM_OK;
+    }
+    int rc;
+    if ((rc = aux_launch<VT, 1>(va, r, (const VT *)nullptr, (const VT *)nullptr, y)))
+      return rc;
+    hipLaunchKernelGGL((k_sw_update<VT, true, false>), dim3(s->gx_upd[vti][0], 1), dim3(kBlock), 0, st, n, Bp,
+                       groups, P, s->en_eff.as<int2>(), s->dK.as<double2>(), s->dM.as<double2>(), s->dMs.as<double>(),
+                       s->rport.as<int8_t>(), s->k0sq.as<double>(), s->coef.as<double2>(), y, w, u, p, sv, x, r,
+                       pu, S);
+    EFM_LAUNCH_CHECK();
+    hipLaunchKernelGGL((k_sw_fin_std<3>), dim3(Bp), dim3(kBlock), 0, st, Bp, s->gx_upd[vti][0], pu, S, flags,
+                       tol2, max_it, max_it, ru_delta2);
+    EFM_LAUNCH_CHECK();
+    return EFM_OK;
+  }
+
+  // Standard preconditioned COCG (see k_sw_pform): the same contract as cocg().  With
+  // ru_delta2 > 0 (reliable updates; b is the FP64 right-hand side and x64 the FP64
+  // solution, zero on entry): the Krylov vectors are of type VT, x carries the solution
+  // increment since the last replacement, and r is replaced by the FP64 residual
+  // b - A x64 whenever a system asks for it (k_sw_fin_std STAGE 2), at the next
+  // convergence check; the search direction is kept across a replacement, so the Krylov
+  // recurrence is not restarted.  A system converges only on its true residual.
+  template <typename VT> int cocg_std(const double2 *b, double inner_tol2, int max_inner, double ru_delta2 = 0.0) {
     const int64_t n = s->n;
     VT *x = s->x.as<VT>(), *r = s->r.as<VT>(), *p = s->p.as<VT>();
     VT *sv = s->sv.as<VT>(), *u = s->u.as<VT>(), *w = s->w.as<VT>(), *y = s->y.as<VT>();
@@ -2068,7 +2171,7 @@ struct SwRun {
     auto fin = [&](auto stage, int gx, const double2 *part) -> int {
       constexpr int STAGE = decltype(stage)::value;
       hipLaunchKernelGGL((k_sw_fin_std<STAGE>), dim3(Bp), dim3(kBlock), 0, st, Bp, gx, part, S, flags, inner_tol2,
-                         max_it, max_inner);
+                         max_it, max_inner, ru_delta2);
       EFM_LAUNCH_CHECK();
       return EFM_OK;
     };
@@ -2102,11 +2205,17 @@ struct SwRun {
     hipLaunchKernelGGL((k_sw_update<VT, true, false>), dim3(s->gx_upd[vti][0], 1), dim3(kBlock), 0, st, n, Bp,
                        groups, P, en, dK, dM, dMs, rp, k2, cf, y, w, u, p, sv, x, r, pu, S);
     EFM_LAUNCH_CHECK();
-    if ((rc = fin(std::integral_constant<int, 0>{}, s->gx_upd[vti][0], pu)) || (rc = n_active(active)))
+    int want = 0;
+    if ((rc = fin(std::integral_constant<int, 0>{}, s->gx_upd[vti][0], pu)) || (rc = n_active(active, want)))
       return rc;
     int it = 0;
     const int cap = std::min(max_it, max_inner);
-    while (active > 0 && it < cap) {
+    while ((active > 0 || want > 0) && it < cap) {
+      if (active == 0) {  // every running system waits for its replacement
+        if ((rc = replace<VT>(b, 0, va_of(it), ru_delta2)) || (rc = n_active(active, want)))
+          return rc;
+        continue;
+      }
       const int chunk = std::min(check, cap - it);
       for (int c = 0; c < chunk; ++c, ++it) {
         const int va = va_of(it), vs = vs_of(it);
@@ -2129,13 +2238,17 @@ struct SwRun {
         if ((rc = fin(std::integral_constant<int, 2>{}, s->gx_std[vti], pu)))
           return rc;
       }
-      if ((rc = n_active(active)))  // synchronizes the stream: the events are complete
+      if ((rc = n_active(active, want)))  // synchronizes the stream: the events are complete
         return rc;
       if (s->spmv_variant < 0 && !log_s.empty() && (rc = log_s.pick(&s->spmv_variant, s->tune_ms, nvs)))
         return rc;
       if (s->aux_variant < 0 && !log_a.empty() && (rc = log_a.pick(&s->aux_variant, s->tune_ms + 3, 2)))
         return rc;
+      if (want > 0 && ((rc = replace<VT>(b, 0, va_of(it), ru_delta2)) || (rc = n_active(active, want))))
+        return rc;
     }
+    if (ru_delta2 > 0.0)
+      return replace<VT>(b, 1, va_of(it), ru_delta2);
     return EFM_OK;
   }
 
@@ -2143,38 +2256,6 @@ struct SwRun {
     return s->form == 1 ? cocg<VT>(b, inner_tol2, max_inner) : cocg_std<VT>(b, inner_tol2, max_inner);
   }
 
-  // Iterative refinement: r = b - A x64 in FP64, inner COCG on r with VT vectors to a
-  // relative reduction inner_tol, x64 += d; until |b - A x64| <= tol |b| per system.
-  template <typename VT> int refine(const double2 *b, double inner_tol2, int max_inner, int max_outer) {
-    const int64_t n = s->n;
-    double2 *x64 = s->x64.as<double2>();
-    VT *r = s->r.as<VT>();
-    EFM_HIP_CHECK(hipMemsetAsync(x64, 0, (size_t)n * Bp * sizeof(double2), st));
-    auto residual = [&](bool first) -> int {
-      hipLaunchKernelGGL((k_sw_residual<VT>), dim3(s->gx_spmv, s->gy_spmv), dim3(kBlock), 0, st, n, Bp, P,
-                         groups, s->row_ptr, s->col, s->K.as<double2>(), s->M.as<double2>(), s->Ms.as<double>(),
-                         s->rport.as<int8_t>(), s->k0sq.as<double>(), s->coef.as<double2>(), b,
-                         first ? (const double2 *)nullptr : (const double2 *)x64, r, ps, S);
-      EFM_LAUNCH_CHECK();
-      hipLaunchKernelGGL(k_sw_outer_finalize, dim3(Bp), dim3(kBlock), 0, st, Bp, s->gx_spmv, ps, S, flags, tol2,
-                         max_it, first ? 1 : 0);
-      EFM_LAUNCH_CHECK();
-      return EFM_OK;
-    };
-    int rc, active = 0;
-    if ((rc = residual(true)) || (rc = n_active(active)))
-      return rc;
-    for (int k = 0; k < max_outer && active > 0; ++k) {
-      if ((rc = solve_inner<VT>(nullptr, inner_tol2, max_inner)))
-        return rc;
-      hipLaunchKernelGGL((k_sw_accumulate<VT>), dim3(s->gx_vec, groups), dim3(kBlock), 0, st, n, Bp,
-                         s->x.as<VT>(), x64, S);
-      EFM_LAUNCH_CHECK();
-      if ((rc = residual(false)) || (rc = n_active(active)))
-        return rc;
-    }
-    return EFM_OK;
-  }
 };
 
 }  // namespace
@@ -2189,8 +2270,10 @@ extern "C" int efm_sweep_solve(efm_sweep *s, int nsys, const double *h_k0sq, con
   EFM_REQUIRE(h_opts->method == EFM_METHOD_COCG, "efm_sweep_solve: the sweep uses COCG");
   EFM_REQUIRE(!h_refine || h_refine->inner_type == EFM_C128 || h_refine->inner_type == EFM_C64,
               "efm_sweep_solve: inner_type must be EFM_C128 or EFM_C64");
-  EFM_REQUIRE(!h_refine || (h_refine->inner_tolerance > 0 && h_refine->inner_tolerance < 1),
-              "efm_sweep_solve: inner_tolerance must lie in (0, 1)");
+  EFM_REQUIRE(!h_refine || (h_refine->replace_factor > 0 && h_refine->replace_factor < 1),
+              "efm_sweep_solve: replace_factor must lie in (0, 1)");
+  EFM_REQUIRE(!h_refine || s->form != 1,
+              "efm_sweep_solve: reliable updates need the standard COCG form (0 or 2)");
   hipStream_t st = (hipStream_t)stream;
   const int Bp = ((nsys + kSys - 1) / kSys) * kSys;  // padded system count this call
   const int P = s->P;
@@ -2249,11 +2332,10 @@ extern "C" int efm_sweep_solve(efm_sweep *s, int nsys, const double *h_k0sq, con
   } else {
     if (!s->x64.p && (rc = s->x64.alloc((size_t)s->n * s->Bmax * sizeof(double2))))
       return rc;
-    const double it2 = h_refine->inner_tolerance * h_refine->inner_tolerance;
-    const int mo = h_refine->max_outer > 0 ? h_refine->max_outer : 8;
-    const int mi = h_refine->max_inner > 0 ? h_refine->max_inner : run.max_it;
-    rc = (h_refine->inner_type == EFM_C64) ? run.refine<float2>(Bd, it2, mi, mo)
-                                           : run.refine<double2>(Bd, it2, mi, mo);
+    EFM_HIP_CHECK(hipMemsetAsync(s->x64.p, 0, (size_t)s->n * Bp * sizeof(double2), st));
+    const double d2 = h_refine->replace_factor * h_refine->replace_factor;
+    rc = (h_refine->inner_type == EFM_C64) ? run.cocg_std<float2>(Bd, run.tol2, run.max_it, d2)
+                                           : run.cocg_std<double2>(Bd, run.tol2, run.max_it, d2);
     if (rc)
       return rc;
     xsrc = s->x64.as<double2>();

@@ -975,11 +975,16 @@ def _block_solve(solver, n: int, P: int, rhs_block, tol: float, max_iterations: 
     zero).  Returns X as a contiguous device (n, P) tensor and per-excitation iters/resid."""
     import torch
     R = solver.nrhs
+
+    def summary(iters, resid):
+        ok = (resid <= tol) | (iters == 0)
+        return dict(iters=iters, resid=resid, iterations=int(iters.max(initial=0)), converged=bool(np.all(ok)),
+                    max_rel_residual=float(resid.max(initial=0.0)))
     if P <= R:
         Xb, info = solver.solve(rhs_block(list(range(P))), tol=tol, max_iterations=max_iterations, method=method)
         Xb = Xb.reshape(n, R)
         X = Xb if P == R else Xb[:, :P].contiguous()
-        return X, dict(iters=info["iters"][:P].copy(), resid=info["resid"][:P].copy())
+        return X, summary(info["iters"][:P].copy(), info["resid"][:P].copy())
     X = None
     iters = np.zeros(P, np.int32)
     resid = np.zeros(P, np.float64)
@@ -992,7 +997,7 @@ def _block_solve(solver, n: int, P: int, rhs_block, tol: float, max_iterations: 
         X[:, b0:b0 + len(idx)] = Xb[:, :len(idx)]
         iters[idx] = info["iters"][:len(idx)]
         resid[idx] = info["resid"][:len(idx)]
-    return X, dict(iters=iters, resid=resid)
+    return X, summary(iters, resid)
 
 
 def _dense_rhs_block(Bh, R: int):
@@ -2073,11 +2078,12 @@ def frequency_sweep_eigenmode(mesh: Mesh, params: MaxwellParams, bc: BC, ports: 
     ``max_batch`` systems per launch sequence, per-system convergence).  The port modes
     do not depend on frequency (the 2D eigenproblem has no frequency in it); beta is
     recomputed per frequency.  ``precision`` "fp64" (default) runs FP64 COCG; "fp32" runs
-    the Krylov iterations on complex-FP32 vectors inside FP64 iterative refinement (FP64
-    residual b - A x, FP64 solution, FP64 reductions), so the tolerance holds for the true
-    FP64 residual.  On these indefinite Maxwell systems FP32 Krylov vectors stall near a
-    1e-4 relative residual, so the refined solve needs more iterations than the FP64 one
-    and is slower overall; it is kept for systems where that does not hold.  Returns an
+    the Krylov iterations on complex-FP32 vectors with FP64 reliable updates: the FP64
+    solution accumulates the FP32 increments, and the residual is replaced by the FP64
+    b - A x whenever it has dropped 100-fold since the last replacement (the search
+    direction is kept), so the tolerance holds for the true FP64 residual.  FP32 vectors
+    halve the bytes of every vector pass but cost iterations (finite-precision delay of the
+    Krylov recurrence); "fp64-refined" runs the same loop on FP64 vectors.  Returns an
     array (nfreq, P, P); frequencies below a port cutoff give NaN."""
     import torch
     freqs = np.asarray(frequencies, np.float64).ravel()
@@ -2144,7 +2150,8 @@ def frequency_sweep_eigenmode(mesh: Mesh, params: MaxwellParams, bc: BC, ports: 
             flat = torch.from_numpy(np.concatenate(rows_l) * (b1 - b0) + np.concatenate(cols_l)).to(dev)
             Bd.view(-1).index_put_((flat,), torch.from_numpy(np.concatenate(vals_l)).to(dev))
         X, info = sweep.solve([sys_k2[si] for si in idx], [sys_coef[si] for si in idx], Bd, tol=tol,
-                              max_iterations=int(opts.max_iterations), precision=precision)
+                              max_iterations=int(opts.max_iterations), precision=precision,
+                              replace_factor=float((_state or {}).get("replace_factor", 0.01)))
         infos.append(info)
         ov = _capi.port_overlap(ops.gi_dev, ops.gv_dev, X, b1 - b0)     # (P, nb)
         for c, si in enumerate(idx):

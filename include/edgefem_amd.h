@@ -232,23 +232,25 @@ int efm_sweep_create(const efm_topology *topo, const double *K_val, const double
                      int n_ports, const double *const *h_Ms_vals, const uint8_t *dirichlet,
                      int max_systems, void *stream, efm_sweep **out);
 void efm_sweep_destroy(efm_sweep *sw);
-/* Mixed precision by iterative refinement: the outer loop forms r = b - A x and updates
- * x in FP64; each inner solve runs COCG with vectors stored as inner_type (EFM_C64:
- * complex FP32; arithmetic and reductions stay FP64) until its residual drops by
- * inner_tolerance.  No reference counterpart (the reference solves in FP64 only). */
+/* Mixed precision by reliable updates (residual replacement): COCG runs with its vectors
+ * stored as inner_type (EFM_C64: complex FP32; arithmetic and reductions stay FP64) and
+ * carries the solution increment since the last replacement; the FP64 solution x64, the
+ * FP64 residual b - A x64 and the convergence test live in FP64.  A system's residual is
+ * replaced by b - A x64 (x64 += increment first) once its recursive residual has dropped by
+ * replace_factor since the last replacement, and when the recursive residual meets the
+ * tolerance; the search direction is kept, so the Krylov recurrence is not restarted.  A
+ * system converges only on its true FP64 residual.  Needs the standard COCG form (0 or 2).
+ * No reference counterpart (the reference solves in FP64 only). */
 typedef struct {
-  int inner_type;          /* EFM_C128 or EFM_C64 */
-  double inner_tolerance;  /* relative residual reduction per inner solve, e.g. 1e-3 */
-  int max_inner;           /* iteration cap of one inner solve; <= 0: no cap of its own */
-  int max_outer;           /* <= 0: 8 */
+  int inner_type;          /* EFM_C128 or EFM_C64: storage of the Krylov vectors */
+  double replace_factor;   /* in (0, 1), e.g. 0.01 */
 } efm_refine_opts;
 
 /* h_k0sq[nsys]; h_coef[nsys*n_ports] c128 (host); B, X: [n_edges][nsys] c128 (device);
  * h_refine NULL: one FP64 COCG solve per system to h_opts->tolerance (recursive
- * residual); otherwise refinement to h_opts->tolerance in the true FP64 residual, with
- * h_opts->max_iterations bounding the inner iterations summed over outer steps.
- * h_iters / h_resid (nsys, may be NULL): total inner iterations and final relative
- * residual per system.  h_opts->method must be EFM_METHOD_COCG. */
+ * residual); otherwise reliable updates to h_opts->tolerance in the true FP64 residual.
+ * h_iters / h_resid (nsys, may be NULL): iterations and final relative residual per
+ * system (the true FP64 residual with h_refine).  h_opts->method must be EFM_METHOD_COCG. */
 int efm_sweep_solve(efm_sweep *sw, int nsys, const double *h_k0sq, const double *h_coef,
                     const double *B, double *X, const efm_solve_opts *h_opts,
                     const efm_refine_opts *h_refine, int32_t *h_iters, double *h_resid,

@@ -63,7 +63,10 @@ Parity pins (independent of the formulas restated here), all exercised in tests/
   7-point degree-5 quadrature at 1e-12;
 * assembly: a dense brute-force element loop on small meshes; A == K - k0^2 M; A is
   Hermitian for real materials, non-Hermitian when lossy (tests/test_maxwell.cpp);
-* port mode: kc^2 against the analytic (pi/a)^2 as h -> 0, scipy eigh residual;
+* port mode: kc^2 against the analytic (pi/a)^2 as h -> 0, scipy eigh residual; the nodal
+  solve_port_eigens TE10 (kc, cos(pi x/a), Z0 = omega mu0/beta, sin(pi x/a) edge weights) and
+  TM11/TM21 (kc^2 = (m pi/a)^2 + (n pi/b)^2 at the P1 convergence rate, sin sin field zero on
+  the wall, Z0 = beta/(omega eps0), the power scaling by an independent gradient integral);
 * S-parameters: |S21| ~ 1, |S11| ~ 0, arg S21 ~ -beta L, reciprocity, and the
   reference-measured values on examples/rect_waveguide.msh (docs/validation.md:20-27).
 """

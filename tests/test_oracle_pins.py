@@ -887,3 +887,55 @@ def test_port_eigens_on_reference_port_mesh():
     assert len(modes) == 1
     fc_te10 = O.C0 / (2 * 0.02286)
     assert abs(modes[0]["fc"] - fc_te10) / fc_te10 < 0.05
+
+
+def test_nodal_port_eigens_tm_closed_forms():
+    """solve_port_eigens(pol="TM") (port_eigensolve.cpp:97-275, PEC boundary nodes removed)
+    on the z = 0 face of a generated WR-90 guide at 20 GHz, pinned by the rectangular-guide
+    closed forms: the first two TM modes are TM11 and TM21 with kc^2 = (m pi/a)^2 + (n pi/b)^2
+    (within the P1 discretisation error) and E_z proportional to sin(m pi x/a) sin(n pi y/b),
+    zero on the wall; Z0 = beta / (omega eps0) with beta = sqrt(k0^2 - kc^2) (not the TE
+    omega mu0 / beta); and the reference's power scaling, whose TM fields
+    E_t = -(beta/kc^2) grad E_z, H_t = (1/(j omega mu)) z x grad E_z give a purely imaginary
+    power, normalised by its magnitude: 1/2 beta / (kc^2 omega mu0) int |grad E_z|^2 = 1,
+    with the gradients taken here from each triangle's own affine map."""
+    a, b = mg.WR90_A, mg.WR90_B
+    om = 2 * math.pi * 20e9
+    k0 = om / O.C0
+    closed = [(math.pi / a) ** 2 + (math.pi / b) ** 2, (2 * math.pi / a) ** 2 + (math.pi / b) ** 2]
+    # kc^2 converges to the closed forms at the P1 rate: the error falls about 4x when h halves
+    errs = []
+    for nx, ny in ((24, 10), (48, 20)):
+        gc = mg.box_mesh(a, b, 10e-3, nx, ny, 1, seed=1)
+        mc = O.mesh_from_arrays(gc.node_ids, gc.xyz, gc.tet_ids, gc.tet_phys, gc.tri_ids, gc.tri_phys)
+        mdc = O.solve_port_eigens(O.extract_surface_mesh(mc, 2), 2, om, 1.0, 1.0, "TM")
+        errs.append([abs(md["kc"] ** 2 / kc2 - 1) for md, kc2 in zip(mdc, closed)])
+    errs = np.array(errs)
+    assert np.all(errs[1] < 6e-3) and np.all(errs[0] / errs[1] > 3.0), errs
+    g = mg.box_mesh(a, b, 10e-3, 24, 10, 2, seed=1)
+    m = O.mesh_from_arrays(g.node_ids, g.xyz, g.tet_ids, g.tet_phys, g.tri_ids, g.tri_phys)
+    surf = O.extract_surface_mesh(m, 2)
+    ids, xy, tri = surf[0], surf[1], surf[2]
+    modes = O.solve_port_eigens(surf, 2, om, 1.0, 1.0, "TM")
+    assert len(modes) == 2
+    pos = {int(v): k for k, v in enumerate(ids)}
+    x, y = xy[:, 0], xy[:, 1]
+    wall = (np.abs(x) < 1e-12) | (np.abs(x - a) < 1e-12) | (np.abs(y) < 1e-12) | (np.abs(y - b) < 1e-12)
+    for md, (mm, nn) in zip(modes, [(1, 1), (2, 1)]):
+        kc2 = (mm * math.pi / a) ** 2 + (nn * math.pi / b) ** 2
+        assert abs(md["kc"] ** 2 / kc2 - 1) < 2.5e-2, (mm, nn, md["kc"] ** 2 / kc2)
+        f = md["field"]
+        assert np.abs(f[wall]).max() == 0.0
+        ref = np.sin(mm * math.pi * x / a) * np.sin(nn * math.pi * y / b)
+        assert abs(np.vdot(f, ref)) / (np.linalg.norm(f) * np.linalg.norm(ref)) > 0.999
+        beta = math.sqrt(k0 ** 2 - md["kc"] ** 2)
+        assert abs(md["Z0"] - beta / (om * O.EPS0)) < 1e-9 * abs(md["Z0"])
+        assert abs(md["Z0"] - om * O.MU0 / beta) > 0.1 * abs(md["Z0"])
+        grad2 = 0.0
+        for t in tri:
+            k = [pos[int(v)] for v in t]
+            P = xy[k]
+            J = np.array([P[1] - P[0], P[2] - P[0]])
+            gr = np.linalg.solve(J, np.array([f[k[1]] - f[k[0]], f[k[2]] - f[k[0]]]))
+            grad2 += 0.5 * abs(np.linalg.det(J)) * float(np.sum(np.abs(gr) ** 2))
+        assert abs(0.5 * beta / (md["kc"] ** 2 * om * O.MU0) * grad2 - 1.0) < 1e-9
