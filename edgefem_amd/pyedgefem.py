@@ -1541,6 +1541,25 @@ def _p1_gradients(xy: np.ndarray):
     return area, G
 
 
+def _rotate_degenerate(w, V, Ky, rtol: float = 1e-4):
+    """Within each cluster of eigenvalues equal to rtol (relative: a physically degenerate
+    pair that the mesh splits slightly counts as one), replace the B-orthonormal
+    eigenvectors by the B-orthonormal basis of the same span that diagonalises Ky,
+    ascending: the first vector minimises int |d f / d y|^2 over the span."""
+    V = V.copy()
+    i = 0
+    while i < w.size:
+        j = i + 1
+        while j < w.size and abs(w[j] - w[i]) <= rtol * max(abs(w[i]), 1e-300):
+            j += 1
+        if j - i > 1:
+            Vc = V[:, i:j]
+            _, Q = np.linalg.eigh(Vc.T @ Ky @ Vc)
+            V[:, i:j] = Vc @ Q
+        i = j
+    return V
+
+
 def solve_port_eigens(mesh: SurfaceMesh2D, num_modes: int, omega: float, eps_r=1.0, mu_r=1.0,
                       pol: "ModePolarization" = ModePolarization.TE) -> List[PortMode]:
     """2D nodal P1 port modes (port_eigensolve.cpp:97-275): K_s psi = kc^2 M_s psi with
@@ -1550,7 +1569,10 @@ def solve_port_eigens(mesh: SurfaceMesh2D, num_modes: int, omega: float, eps_r=1
     mode as a spurious first TE mode on millimetre meshes); beta = sqrt(k^2 -
     kc^2); Z0 = omega mu / beta (TE) or beta / (omega eps) (TM); field scaled to unit
     real power.  The eigenvector sign is not the reference's (Eigen picks it; it cannot
-    be reproduced): each mode is made real with its largest entry positive."""
+    be reproduced): each mode is made real with its largest entry positive.  Within a
+    degenerate pair (a square port's TE10/TE01, where the reference returns whatever
+    member of the plane Eigen gives) the mode is the one whose field varies least along
+    the second surface axis, i.e. E along that axis."""
     import scipy.linalg
     if mesh.tri_ids.shape[0] == 0:
         raise RuntimeError("Port eigensolver requires a 2D mesh.")
@@ -1578,6 +1600,13 @@ def solve_port_eigens(mesh: SurfaceMesh2D, num_modes: int, omega: float, eps_r=1
     np.add.at(A, (r, c), ke[ok])
     np.add.at(B, (r, c), me[ok])
     w, V = scipy.linalg.eigh(A, B)
+    # Degenerate eigenvalues (e.g. TE10/TE01 of a square port) leave the eigenvector an
+    # arbitrary member of a plane; rotate every such cluster so that its first vector has
+    # the least variation along the second surface axis (E along that axis), a fixed,
+    # physical choice instead of whatever the dense eigensolver returns.
+    Ky = np.zeros((nd, nd))
+    np.add.at(Ky, (r, c), (area[:, None, None] * G[:, :, None, 1] * G[:, None, :, 1])[ok])
+    V = _rotate_degenerate(w, V, Ky)
     eps = EPS0 * complex(eps_r)
     mu = MU0 * complex(mu_r)
     k = omega * cmath.sqrt(mu * eps)

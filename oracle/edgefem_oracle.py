@@ -960,7 +960,8 @@ def extract_surface_mesh(mesh: OMesh, tag: int):
 def solve_port_eigens(surf, num_modes: int, omega: float, eps_r=1.0, mu_r=1.0, pol: str = "TE"):
     """Nodal P1 port modes (port_eigensolve.cpp:97-275), element loop restated with
     explicit 2x2 algebra; returns a list of dicts (kc, fc, beta, Z0, field).  Eigenvector
-    sign: made real with its largest entry positive (the reference's is Eigen's).  The
+    sign: made real with its largest entry positive (the reference's is Eigen's).  A
+    degenerate pair is resolved to the member with E along y (the least y-variation).  The
     zero-mode cut is kc^2 < max(1e-12, 1e-10 kc^2_max): with the reference's absolute
     1e-12 the Neumann constant mode, whose computed kc^2 is roundoff of order 1e-9 on
     millimetre meshes, would come back as a spurious first TE mode."""
@@ -974,7 +975,7 @@ def solve_port_eigens(surf, num_modes: int, omega: float, eps_r=1.0, mu_r=1.0, p
     dof = -np.ones(N, int)
     dof[~pec] = np.arange((~pec).sum())
     nd = int((~pec).sum())
-    A = np.zeros((nd, nd)); Bm = np.zeros((nd, nd))
+    A = np.zeros((nd, nd)); Bm = np.zeros((nd, nd)); Ky = np.zeros((nd, nd))
     tris = []
     for t in tri:
         ii = [pos[int(v)] for v in t]
@@ -991,7 +992,20 @@ def solve_port_eigens(surf, num_modes: int, omega: float, eps_r=1.0, mu_r=1.0, p
                 if dof[ii[a]] >= 0 and dof[ii[b]] >= 0:
                     A[dof[ii[a]], dof[ii[b]]] += area * g[a] @ g[b]
                     Bm[dof[ii[a]], dof[ii[b]]] += area / 12.0 * (2.0 if a == b else 1.0)
+                    Ky[dof[ii[a]], dof[ii[b]]] += area * g[a][1] * g[b][1]
     w, V = scipy.linalg.eigh(A, Bm)
+    # a degenerate cluster (square port: TE10/TE01) is rotated so that its first vector
+    # minimises int |df/dy|^2 over the cluster's span (E along y); the reference returns
+    # whatever member of the plane Eigen gives, which nothing can reproduce
+    i = 0
+    while i < w.size:
+        j = i + 1
+        while j < w.size and abs(w[j] - w[i]) <= 1e-4 * max(abs(w[i]), 1e-300):
+            j += 1
+        if j - i > 1:
+            Vc = V[:, i:j]
+            V[:, i:j] = Vc @ np.linalg.eigh(Vc.T @ Ky @ Vc)[1]
+        i = j
     eps = EPS0 * complex(eps_r); mu = MU0 * complex(mu_r)
     k = omega * np.sqrt(mu * eps)
     out = []

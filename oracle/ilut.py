@@ -42,12 +42,15 @@ def load():
         build()
     lib = C.CDLL(LIB_PATH)
     vp, i64, i32, dbl = C.c_void_p, C.c_int64, C.c_int, C.c_double
-    lib.efo_ilut_factor.argtypes = [i64, vp, vp, vp, vp, dbl, i32, C.POINTER(vp), C.POINTER(dbl)]
+    lib.efo_ilut_factor.argtypes = [i64, vp, vp, vp, vp, dbl, i32, dbl, C.POINTER(vp), C.POINTER(i64),
+                                    C.POINTER(dbl)]
     lib.efo_ilut_factor.restype = i32
     lib.efo_ilut_nnz.argtypes = [vp]
     lib.efo_ilut_nnz.restype = i64
     lib.efo_ilut_free.argtypes = [vp]
     lib.efo_ilut_free.restype = None
+    lib.efo_ilut_apply.argtypes = [vp, vp, vp]
+    lib.efo_ilut_apply.restype = i32
     lib.efo_bicgstab.argtypes = [i64, vp, vp, vp, vp, vp, vp, dbl, i32, dbl, C.POINTER(i32), C.POINTER(dbl),
                                  C.POINTER(dbl)]
     lib.efo_bicgstab.restype = i32
@@ -72,10 +75,19 @@ def rcm_order(A) -> np.ndarray:
     return np.ascontiguousarray(reverse_cuthill_mckee(S, symmetric_mode=True).astype(np.int32))
 
 
-class ILUT:
-    """Incomplete LU of P A P^T with dual thresholds (drop tolerance, fill factor)."""
+class ILUTTimeLimit(RuntimeError):
+    """The factorization passed its time limit; rows_done / seconds give its rate."""
 
-    def __init__(self, A, droptol: float = 1e-4, fillfactor: int = 10, perm="rcm"):
+    def __init__(self, rows_done, n, seconds):
+        super().__init__(f"ILUT stopped at its time limit after {rows_done} of {n} rows ({seconds:.1f} s)")
+        self.rows_done, self.n, self.seconds = rows_done, n, seconds
+
+
+class ILUT:
+    """Incomplete LU of P A P^T with dual thresholds (drop tolerance, fill factor).
+    time_limit > 0 stops the row loop after that many seconds (ILUTTimeLimit)."""
+
+    def __init__(self, A, droptol: float = 1e-4, fillfactor: int = 10, perm="rcm", time_limit: float = 0.0):
         lib = load()
         n, rp, ci, v = _csr_arrays(A)
         if isinstance(perm, str) and perm == "rcm":
@@ -83,9 +95,12 @@ class ILUT:
         pp = None if perm is None else np.ascontiguousarray(np.asarray(perm, np.int32))
         h = C.c_void_p()
         secs = C.c_double()
+        rows = C.c_int64()
         rc = lib.efo_ilut_factor(n, rp.ctypes.data, ci.ctypes.data, v.ctypes.data,
                                  None if pp is None else pp.ctypes.data, float(droptol), int(fillfactor),
-                                 C.byref(h), C.byref(secs))
+                                 float(time_limit), C.byref(h), C.byref(rows), C.byref(secs))
+        if rc == 3:
+            raise ILUTTimeLimit(rows.value, n, secs.value)
         if rc != 0:
             raise RuntimeError({1: "ILUT: matrix has an empty row", 2: "ILUT: out of memory"}.get(rc, f"ILUT rc {rc}"))
         self._h = h
@@ -103,6 +118,15 @@ class ILUT:
             self.close()
         except Exception:
             pass
+
+
+def apply(M: ILUT, v):
+    """M^-1 v = P^T U^-1 L^-1 P v."""
+    vv = np.ascontiguousarray(np.asarray(v, np.complex128))
+    y = np.zeros_like(vv)
+    if load().efo_ilut_apply(M._h, vv.ctypes.data, y.ctypes.data) != 0:
+        raise RuntimeError("ILUT apply: out of memory")
+    return y
 
 
 def bicgstab(A, b, M: ILUT = None, tol: float = 1e-10, max_iterations: int = 10000, time_limit: float = 0.0):

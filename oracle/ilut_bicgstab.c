@@ -43,6 +43,8 @@ typedef struct {
   int32_t *perm;     /* new -> old (NULL: identity) */
 } ilut_t;
 
+void efo_ilut_free(void *h);
+
 static double now_s(void) {
   struct timespec t;
   clock_gettime(CLOCK_MONOTONIC, &t);
@@ -108,9 +110,12 @@ static void select_largest(int32_t *idx, double *mag, int m, int keep) {
   }
 }
 
-/* Returns 0 on success, 1 on a zero row, 2 on allocation failure. */
+/* Returns 0 on success, 1 on a zero row, 2 on allocation failure, 3 when time_limit > 0
+ * seconds passed before the last row (then *out is NULL and *rows_done says how far the
+ * factorization got: a rate for a bounded sample). */
 int efo_ilut_factor(int64_t n, const int64_t *rp, const int32_t *ci, const cplx *v, const int32_t *perm,
-                    double droptol, int fillfactor, void **out, double *seconds) {
+                    double droptol, int fillfactor, double time_limit, void **out, int64_t *rows_done,
+                    double *seconds) {
   const double t0 = now_s();
   *out = NULL;
   ilut_t *F = (ilut_t *)calloc(1, sizeof(ilut_t));
@@ -151,7 +156,13 @@ int efo_ilut_factor(int64_t n, const int64_t *rp, const int32_t *ci, const cplx 
     mark[i] = 0;
   int64_t nl = 0, nu = 0;
   F->lp[0] = F->up[0] = 0;
-  for (int64_t i = 0; i < n; ++i) {
+  int timed_out = 0;
+  int64_t i;
+  for (i = 0; i < n; ++i) {
+    if (time_limit > 0 && (i & 1023) == 0 && now_s() - t0 > time_limit) {
+      timed_out = 1;
+      break;
+    }
     const int64_t ro = perm ? perm[i] : i;  /* row i of P A P^T is row perm[i] of A */
     int hl = 0, nul = 0, nll = 0;
     double rownorm = 0.0;
@@ -249,9 +260,15 @@ int efo_ilut_factor(int64_t n, const int64_t *rp, const int32_t *ci, const cplx 
     F->up[i + 1] = nu;
   }
   free(w); free(mark); free(ul); free(ll); free(heap); free(mag); free(inv);
-  *out = F;
+  if (rows_done)
+    *rows_done = i;
   if (seconds)
     *seconds = now_s() - t0;
+  if (timed_out) {
+    efo_ilut_free(F);
+    return 3;
+  }
+  *out = F;
   return 0;
 }
 
@@ -289,6 +306,17 @@ static void ilut_apply(const ilut_t *F, const cplx *x, cplx *y, cplx *t) {
     else
       y[i] = t[i];
   }
+}
+
+/* y = P^T U^-1 L^-1 P x, the preconditioner's action (exported for the tests). */
+int efo_ilut_apply(void *h, const cplx *x, cplx *y) {
+  const ilut_t *F = (const ilut_t *)h;
+  cplx *t = (cplx *)malloc(F->n * sizeof(cplx));
+  if (!t)
+    return 2;
+  ilut_apply(F, x, y, t);
+  free(t);
+  return 0;
 }
 
 static void spmv(int64_t n, const int64_t *rp, const int32_t *ci, const cplx *v, const cplx *x, cplx *y) {

@@ -15,6 +15,13 @@ c2     c2_sparams_10ghz.npz: S of the 48x24x192 WR-90 guide, 100 mm, with the lo
        here is the oracle's COCG with the auxiliary-space preconditioner to a relative
        residual of 1e-13 followed by refinement steps x += COCG(A, b - A x); the npz
        records the method, the iteration counts and the true FP64 residual reached.
+c4     c4_cells.npz: S of 8 of the 512 seeded unit cells of the unit-cell batch (the
+       smallest, the largest and six between them by size), with the unit-cell caller's
+       port set-up (unit_cell.py:700-763): eps regions substrate 3.5(1 - 0.002j) and air,
+       use_port_abc, the first nodal TE mode of the top and bottom faces as wave ports
+       (solve_port_eigens + build_wave_port; the square face's degenerate TE10/TE01 pair is
+       resolved to E along y), Floquet phase along x from theta at 10 GHz, S by the oracle's
+       periodic elimination and SuperLU per excitation.
 """
 import hashlib
 import math
@@ -125,8 +132,64 @@ def make_c2():
     log(f"c2 S = {S}")
 
 
+def c4_distribution(ncell=512):
+    """The seeded cell parameters of the unit-cell batch (bench.py cells workload)."""
+    u = lambda stream: mg.uniform01(stream, np.arange(ncell, dtype=np.int64))
+    nxy = 10 + np.floor(u(11) * 19).astype(int)
+    nzs = 12 + np.floor(u(12) * 13).astype(int)
+    pw, pl = 1e-3 + 3.5e-3 * u(13), 1e-3 + 3.5e-3 * u(14)
+    theta = 40.0 * u(15)
+    return nxy, nzs, pw, pl, theta
+
+
+def c4_pick(nxy, nzs, k=8):
+    """k cells spread over the size ranking: the smallest, the largest and k-2 between."""
+    order = np.argsort(nxy * nxy * nzs, kind="stable")
+    ranks = np.round(np.linspace(0, order.size - 1, k)).astype(int)
+    return order[ranks]
+
+
+def c4_oracle_cell(c, nxy, nzs, pw, pl, theta, f=10e9):
+    g = mg.unit_cell(n=int(nxy[c]), nz=int(nzs[c]), patch=(float(pw[c]), float(pl[c])), seed=1000 + int(c))
+    m = O.mesh_from_arrays(g.node_ids, g.xyz, g.tet_ids, g.tet_phys, g.tri_ids, g.tri_phys)
+    pec = O.build_edge_pec(m, mg.CELL_TAGS["patch"])
+    pbc = O.build_periodic_pairs(m, mg.CELL_TAGS["x0"], mg.CELL_TAGS["x1"], (5e-3, 0.0, 0.0))
+    k0 = 2 * math.pi * f / O.C0
+    O.set_floquet_phase(pbc, (k0 * math.sin(math.radians(float(theta[c]))), 0.0))
+    om = 2 * math.pi * f
+    ports = []
+    for tag in (mg.CELL_TAGS["port_top"], mg.CELL_TAGS["port_bottom"]):
+        surf = O.extract_surface_mesh(m, tag)
+        md = O.solve_port_eigens(surf, 1, om, 1.0, 1.0, "TE")[0]
+        ports.append(O.build_wave_port(m, surf, md))
+    p = O.Params(omega=om, eps_r_regions={mg.CELL_TAGS["substrate"]: 3.5 * (1 - 0.002j), mg.CELL_TAGS["air"]: 1.0},
+                 use_port_abc=True)
+    return m, p, pec, pbc, ports
+
+
+def _c4_point(args):
+    c, dist = args
+    m, p, pec, pbc, ports = c4_oracle_cell(c, *dist)
+    return O.calculate_sparams_periodic(m, p, pec, pbc, ports)
+
+
+def make_c4():
+    os.environ["OPENBLAS_NUM_THREADS"] = "1"
+    dist = c4_distribution()
+    pick = c4_pick(dist[0], dist[1])
+    with ProcessPoolExecutor(max_workers=4, mp_context=mp.get_context("spawn")) as ex:
+        S = np.stack(list(ex.map(_c4_point, [(int(c), dist) for c in pick])))
+    nxy, nzs, pw, pl, theta = dist
+    np.savez(os.path.join(HERE, "c4_cells.npz"), cells=pick, n=nxy[pick], nz=nzs[pick], patch_w=pw[pick],
+             patch_l=pl[pick], theta_deg=theta[pick], S=S)
+    for c, Sc in zip(pick, S):
+        print("c4 cell", c, nxy[c], nzs[c], Sc[0, 0], Sc[1, 0])
+
+
 if __name__ == "__main__":
     what = sys.argv[1:] or ["c1", "sweep"]
+    if "c4" in what:
+        make_c4()
     if "c2" in what:
         make_c2()
     if "c1" in what:

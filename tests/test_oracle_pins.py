@@ -939,3 +939,26 @@ def test_nodal_port_eigens_tm_closed_forms():
             gr = np.linalg.solve(J, np.array([f[k[1]] - f[k[0]], f[k[2]] - f[k[0]]]))
             grad2 += 0.5 * abs(np.linalg.det(J)) * float(np.sum(np.abs(gr) ** 2))
         assert abs(0.5 * beta / (md["kc"] ** 2 * om * O.MU0) * grad2 - 1.0) < 1e-9
+
+
+def test_nodal_port_eigens_degenerate_pair_rule():
+    """The unit-cell caller's port (unit_cell.py:735-763: solve_port_eigens(..., 1, ...) on
+    the top face) is a 5 x 5 mm square, where TE10 and TE01 are degenerate and the
+    reference's first mode is whatever member of the plane Eigen returns.  The restatement
+    fixes it: the member with the least variation along y, so H_z ~ cos(pi x / P) (E along
+    y), with kc^2 -> (pi/P)^2.  The second member is then cos(pi y / P); a basis rotated by
+    any angle gives the same pair back."""
+    P = 5e-3
+    g = mg.unit_cell(n=10, nz=12, seed=3)
+    m = O.mesh_from_arrays(g.node_ids, g.xyz, g.tet_ids, g.tet_phys, g.tri_ids, g.tri_phys)
+    surf = O.extract_surface_mesh(m, mg.CELL_TAGS["port_top"])
+    om = 2 * math.pi * 10e9
+    modes = O.solve_port_eigens(surf, 2, om, 1.0, 1.0, "TE")
+    x, y = surf[1][:, 0], surf[1][:, 1]
+    assert abs(modes[0]["kc"] ** 2 / (math.pi / P) ** 2 - 1) < 2e-2
+    assert abs(modes[1]["kc"] ** 2 / modes[0]["kc"] ** 2 - 1) < 1e-4      # degenerate up to the mesh
+    for md, ref in zip(modes, (np.cos(math.pi * (x - x.min()) / P), np.cos(math.pi * (y - y.min()) / P))):
+        f = md["field"].real
+        assert abs(f @ ref) / (np.linalg.norm(f) * np.linalg.norm(ref)) > 0.9999
+    # at 10 GHz the mode is evanescent on the 5 mm cell (kc = pi/P > k0): beta and Z0 imaginary
+    assert modes[0]["kc"] > om / O.C0 and abs(modes[0]["beta"].real) < 1e-12 * abs(modes[0]["beta"])
