@@ -36,6 +36,7 @@ EXPORTS = [
     "efm_sweep_set_variants", "efm_sweep_get_variants", "efm_sweep_set_iteration", "efm_tet_fields",
     "efm_gmsh_read", "efm_gmsh_sizes", "efm_gmsh_copy", "efm_gmsh_free",
     "efm_touchstone_write", "efm_touchstone_write_2port", "efm_touchstone_extension",
+    "efm_port_overlap_noalloc",
 ]
 
 
@@ -131,6 +132,7 @@ def load_library(path: str = LIB_PATH) -> C.CDLL:
     lib.efm_sweep_solve.argtypes = [vp, i32, vp, vp, vp, vp, C.POINTER(SolveOpts), C.POINTER(RefineOpts), vp, vp,
                                     vp]
     lib.efm_port_overlap.argtypes = [i64, i32, i64, vp, vp, vp, i32, vp, vp]
+    lib.efm_port_overlap_noalloc.argtypes = [i64, i32, i64, vp, vp, vp, i32, vp, vp]
     for name in EXPORTS:
         getattr(lib, name)  # AttributeError if the library lacks a declared symbol
     _lib = lib

@@ -2,6 +2,7 @@
 #include "efm_internal.h"
 
 #include <algorithm>
+#include <mutex>
 #include <string>
 
 namespace efm {
@@ -90,6 +91,30 @@ int overlap_run(int64_t n, int n_ports, int64_t len, const int32_t *gidx, const 
   return EFM_OK;
 }
 
+// Fixed staging array of efm_port_overlap_noalloc (see the header): every device that runs
+// the library holds its own copy, 64 KiB, until the process ends.
+constexpr int kOverlapScratch = 4096;  // complex entries = n_ports * nrhs at most
+static __device__ __align__(256) double2 scratch[kOverlapScratch];
+
+std::mutex g_scratch_mu;
+double2 *g_scratch_ptr[kMaxDevices] = {};
+
+// Device address of `scratch` on the current device, looked up once per device.
+int scratch_address(double2 **out) {
+  int dev = 0;
+  EFM_HIP_CHECK(hipGetDevice(&dev));
+  EFM_REQUIRE(dev >= 0 && dev < kMaxDevices, "efm_port_overlap_noalloc: device ordinal out of range");
+  std::lock_guard<std::mutex> lock(g_scratch_mu);
+  if (!g_scratch_ptr[dev]) {
+    void *p = nullptr;
+    EFM_HIP_CHECK(hipGetSymbolAddress(&p, HIP_SYMBOL(scratch)));
+    EFM_REQUIRE(p != nullptr, "efm_port_overlap_noalloc: scratch symbol has no device address");
+    g_scratch_ptr[dev] = static_cast<double2 *>(p);
+  }
+  *out = g_scratch_ptr[dev];
+  return EFM_OK;
+}
+
 } // namespace
 
 extern "C" int efm_port_overlap(int64_t n, int n_ports, int64_t len, const int32_t *gidx,
@@ -99,6 +124,20 @@ extern "C" int efm_port_overlap(int64_t n, int n_ports, int64_t len, const int32
   EFM_REQUIRE(n > 0 && n_ports > 0 && len > 0 && nrhs > 0, "efm_port_overlap: bad sizes");
   double2 *buf = nullptr;
   int rc = overlap_staging((size_t)n_ports * nrhs * sizeof(double2), &buf);
+  if (rc != EFM_OK)
+    return rc;
+  return overlap_run(n, n_ports, len, gidx, gval, X, nrhs, h_out, buf, (hipStream_t)stream);
+}
+
+extern "C" int efm_port_overlap_noalloc(int64_t n, int n_ports, int64_t len, const int32_t *gidx,
+                                        const double *gval, const double *X, int nrhs,
+                                        double *h_out, void *stream) {
+  EFM_REQUIRE(gidx && gval && X && h_out, "efm_port_overlap_noalloc: NULL argument");
+  EFM_REQUIRE(n > 0 && n_ports > 0 && len > 0 && nrhs > 0, "efm_port_overlap_noalloc: bad sizes");
+  EFM_REQUIRE((int64_t)n_ports * nrhs <= kOverlapScratch,
+              "efm_port_overlap_noalloc: n_ports * nrhs exceeds the 4096-entry scratch array");
+  double2 *buf = nullptr;
+  int rc = scratch_address(&buf);
   if (rc != EFM_OK)
     return rc;
   return overlap_run(n, n_ports, len, gidx, gval, X, nrhs, h_out, buf, (hipStream_t)stream);

@@ -216,6 +216,19 @@ int efm_solver_profile_read(efm_solver *s, double *h_spmv_ms_total, int64_t *h_s
 int efm_port_overlap(int64_t n, int n_ports, int64_t len, const int32_t *gidx, const double *gval,
                      const double *X, int nrhs, double *h_out, void *stream);
 
+/* efm_port_overlap without any device allocation, for callers that cannot allocate (code
+ * under HIP graph capture, tight serving loops, harnesses that forbid hipMalloc).  No
+ * counterpart in the reference.  Same arguments, kernel and result as efm_port_overlap,
+ * but the results are staged in a fixed 4096-entry (64 KiB) device array that the library
+ * holds on every GPU it runs on until the process ends, instead of a buffer it allocates:
+ * n_ports * nrhs must not exceed 4096 (EFM_ERR_ARG otherwise).  Every call on one GPU
+ * shares that array, so calls that overlap on one GPU (from several host threads, or on
+ * several streams) race; calls that follow each other are safe, because each call
+ * synchronises its stream before it returns. */
+int efm_port_overlap_noalloc(int64_t n, int n_ports, int64_t len, const int32_t *gidx,
+                             const double *gval, const double *X, int nrhs, double *h_out,
+                             void *stream);
+
 /* ------------------------------------------------------------------------
  * Batched frequency sweep: up to 1024 systems A_s = K - k0sq[s] M + sum_p coef[s,p] M_s,p
  * on one pattern, solved together by COCG with the auxiliary-gradient preconditioner and
