@@ -464,9 +464,11 @@ def cells_main(args):
     substrate eps 3.5(1 - 0.002j) under 10 mm of air, centred PEC patch w, l ~ U[1, 4.5] mm
     snapped to the grid on the substrate top, hex grid nx = ny ~ U{10..28}, nz ~ U{12..24}
     (seeded splitmix64), Floquet phase along x from theta ~ U[0, 40] degrees at 10 GHz,
-    lumped ports top and bottom (E along y, Z0 = 376.73 ohm).  One step = assembly and
-    periodic reduction of every cell on the GPU, the batched BiCGSTAB of all 2 x cells
-    systems, and the S extraction."""
+    ports top and bottom as the unit-cell caller sets them up (unit_cell.py:700-763): the first
+    nodal TE mode of each face (solve_port_eigens, the square face's degenerate TE10/TE01
+    pair resolved to E along y) as a wave port (build_wave_port), use_port_abc, eps regions
+    substrate and air.  One step = assembly and periodic reduction of every cell on the GPU,
+    the batched BiCGSTAB of all 2 x cells systems, and the S extraction."""
     import numpy as np
     import torch
     from edgefem_amd import meshgen as mg
@@ -489,19 +491,19 @@ def cells_main(args):
         g = mg.unit_cell(n=int(nxy[c]), nz=int(nzs[c]), patch=(float(pw_[c]), float(pl_[c])), seed=1000 + c)
         pm = em.mesh_from_arrays(g.node_ids, g.xyz, g.tets, g.tet_phys, g.tris, g.tri_phys)
         tets += g.tets.shape[0]
-        bc = em.build_edge_pec(pm, 1)
-        pbc = em.build_periodic_pairs(pm, 10, 11, np.array([5e-3, 0.0, 0.0]))
+        T = mg.CELL_TAGS
+        bc = em.build_edge_pec(pm, T["patch"])
+        pbc = em.build_periodic_pairs(pm, T["x0"], T["x1"], np.array([5e-3, 0.0, 0.0]))
         em.set_floquet_phase(pbc, np.array([k0 * math.sin(math.radians(theta[c])), 0.0]))
         ports = []
-        for tag in (2, 3):
-            cfg = em.LumpedPortConfig()
-            cfg.surface_tag = tag
-            cfg.z0 = 376.73
-            cfg.e_direction = np.array([0.0, 1.0, 0.0])
-            ports.append(em.build_lumped_port(pm, cfg))
+        for tag in (T["port_top"], T["port_bottom"]):
+            surf = em.extract_surface_mesh(pm, tag)
+            mode = em.solve_port_eigens(surf.mesh, 1, 2 * math.pi * f, 1.0, 1.0, em.ModePolarization.TE)[0]
+            ports.append(em.build_wave_port(pm, surf, mode))
         params = em.MaxwellParams()
         params.omega = 2 * math.pi * f
-        params.eps_r_regions = {101: 3.5 * (1 - 0.002j)}
+        params.eps_r_regions = {T["substrate"]: 3.5 * (1 - 0.002j), T["air"]: 1.0}
+        params.use_port_abc = True
         cells.append((pm, params, bc, pbc, ports))
     log(f"cells: {ncell} meshes, {tets} tets, setup {time.perf_counter() - t0:.1f}s")
     opts = em.SolveOptions()
@@ -529,12 +531,17 @@ def cells_main(args):
             pm, params, bc, pbc, ports = cells[c]
             g = mg.unit_cell(n=int(nxy[c]), nz=int(nzs[c]), patch=(float(pw_[c]), float(pl_[c])), seed=1000 + c)
             om = O.mesh_from_arrays(g.node_ids, g.xyz, g.tet_ids, g.tet_phys, g.tri_ids, g.tri_phys)
-            opbc = O.build_periodic_pairs(om, 10, 11, (5e-3, 0, 0))
-            opbc.phase = pbc.phase_shift
-            oports = [O.build_lumped_port(om, t, 376.73, (0, 1, 0)) for t in (2, 3)]
+            T = mg.CELL_TAGS
+            opbc = O.build_periodic_pairs(om, T["x0"], T["x1"], (5e-3, 0, 0))
+            O.set_floquet_phase(opbc, (k0 * math.sin(math.radians(theta[c])), 0.0))
+            oports = []
+            for tag in (T["port_top"], T["port_bottom"]):
+                surf = O.extract_surface_mesh(om, tag)
+                oports.append(O.build_wave_port(om, surf, O.solve_port_eigens(surf, 1, params.omega, 1.0, 1.0, "TE")[0]))
             t1 = time.perf_counter()
-            So = O.calculate_sparams_periodic(om, O.Params(omega=params.omega, eps_r_regions={101: 3.5 * (1 - 0.002j)}),
-                                              O.build_edge_pec(om, 1), opbc, oports)
+            So = O.calculate_sparams_periodic(
+                om, O.Params(omega=params.omega, eps_r_regions={T["substrate"]: 3.5 * (1 - 0.002j), T["air"]: 1.0},
+                             use_port_abc=True), O.build_edge_pec(om, T["patch"]), opbc, oports)
             tcpu += time.perf_counter() - t1
             errs.append(float(np.abs(S[c] - So).max()))
         dS = max(errs)

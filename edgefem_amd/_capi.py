@@ -36,7 +36,7 @@ EXPORTS = [
     "efm_sweep_set_variants", "efm_sweep_get_variants", "efm_sweep_set_iteration", "efm_tet_fields",
     "efm_gmsh_read", "efm_gmsh_sizes", "efm_gmsh_copy", "efm_gmsh_free",
     "efm_touchstone_write", "efm_touchstone_write_2port", "efm_touchstone_extension",
-    "efm_port_overlap_noalloc",
+    "efm_port_overlap_noalloc", "efm_sweep_set_graph",
 ]
 
 
@@ -488,6 +488,11 @@ class SweepSolver:
         preconditioner part (0, 1 as for the SpMV); -1: the fastest, timed on the first
         solve."""
         _check(self._lib.efm_sweep_set_variants(self._h, int(spmv), int(aux)), "efm_sweep_set_variants")
+
+    def set_graph(self, enable: bool = True) -> None:
+        """Replay each full convergence-check chunk of iterations from a HIP graph captured
+        once per solve (standard forms 0 and 2)."""
+        _check(self._lib.efm_sweep_set_graph(self._h, 1 if enable else 0), "efm_sweep_set_graph")
 
     def set_iteration(self, form: int = 2) -> None:
         """Iteration form of the batched COCG: 0 standard, with p = u + beta p in a pass of

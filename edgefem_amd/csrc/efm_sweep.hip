@@ -1638,6 +1638,9 @@ struct efm_sweep {
   // iteration: 0 = standard COCG (p in its own pass, two reductions), 1 = Chronopoulos-Gear,
   // 2 (default) = standard COCG with u rebuilt in the p pass instead of stored
   int form = 2;
+  // standard forms: replay each full convergence-check chunk of iterations from a HIP graph
+  // captured once per solve (after the autotuner has chosen the kernels)
+  int use_graph = 0;
   int gx_std[2] = {8, 8};  // grid of the standard form's update [VT c128, c64]
   // update grids (one block slice, as many blocks as are resident): [VT c128, c64][INIT, skip, apply]
   int gx_upd[2][3] = {{8, 8, 8}, {8, 8, 8}};
@@ -1826,6 +1829,12 @@ extern "C" int efm_sweep_set_variants(efm_sweep *s, int spmv, int aux) {
   return EFM_OK;
 }
 
+extern "C" int efm_sweep_set_graph(efm_sweep *s, int enable) {
+  EFM_REQUIRE(s, "efm_sweep_set_graph: NULL sweep");
+  s->use_graph = enable ? 1 : 0;
+  return EFM_OK;
+}
+
 extern "C" int efm_sweep_set_iteration(efm_sweep *s, int form) {
   EFM_REQUIRE(s, "efm_sweep_set_iteration: NULL sweep");
   EFM_REQUIRE(form >= 0 && form <= 2,
@@ -1895,6 +1904,32 @@ struct TuneLog {
     var.clear();
   }
   ~TuneLog() { release(); }
+};
+
+// A captured chunk of iterations: the graph and its executable, released together.
+struct ChunkGraph {
+  hipGraph_t g = nullptr;
+  hipGraphExec_t x = nullptr;
+  ~ChunkGraph() {
+    if (x)
+      (void)hipGraphExecDestroy(x);
+    if (g)
+      (void)hipGraphDestroy(g);
+  }
+};
+
+// Ends a stream capture that an error left open (the partial graph is dropped).
+struct CaptureGuard {
+  hipStream_t st = nullptr;
+  bool on = false;
+  ~CaptureGuard() {
+    if (on) {
+      hipGraph_t g = nullptr;
+      (void)hipStreamEndCapture(st, &g);
+      if (g)
+        (void)hipGraphDestroy(g);
+    }
+  }
 };
 
 // One launch sequence over the padded system block [Bp]: everything a solve needs.
@@ -2210,6 +2245,7 @@ struct SwRun {
       return rc;
     int it = 0;
     const int cap = std::min(max_it, max_inner);
+    ChunkGraph graph;
     while ((active > 0 || want > 0) && it < cap) {
       if (active == 0) {  // every running system waits for its replacement
         if ((rc = replace<VT>(b, 0, va_of(it), ru_delta2)) || (rc = n_active(active, want)))
@@ -2217,6 +2253,23 @@ struct SwRun {
         continue;
       }
       const int chunk = std::min(check, cap - it);
+      // a full chunk with the kernels chosen: replay the captured graph (capture it first)
+      const bool replay = s->use_graph && chunk == check && s->spmv_variant >= 0 && s->aux_variant >= 0;
+      if (replay && graph.x) {
+        EFM_HIP_CHECK(hipGraphLaunch(graph.x, st));
+        it += chunk;
+        if ((rc = n_active(active, want)))
+          return rc;
+        if (want > 0 && ((rc = replace<VT>(b, 0, va_of(it), ru_delta2)) || (rc = n_active(active, want))))
+          return rc;
+        continue;
+      }
+      CaptureGuard capture;
+      if (replay) {
+        EFM_HIP_CHECK(hipStreamBeginCapture(st, hipStreamCaptureModeThreadLocal));
+        capture.st = st;
+        capture.on = true;
+      }
       for (int c = 0; c < chunk; ++c, ++it) {
         const int va = va_of(it), vs = vs_of(it);
         if (s->form == 2)
@@ -2237,6 +2290,12 @@ struct SwRun {
         EFM_LAUNCH_CHECK();
         if ((rc = fin(std::integral_constant<int, 2>{}, s->gx_std[vti], pu)))
           return rc;
+      }
+      if (replay) {  // the chunk was captured, not run: instantiate and launch it
+        capture.on = false;
+        EFM_HIP_CHECK(hipStreamEndCapture(st, &graph.g));
+        EFM_HIP_CHECK(hipGraphInstantiate(&graph.x, graph.g, nullptr, nullptr, 0));
+        EFM_HIP_CHECK(hipGraphLaunch(graph.x, st));
       }
       if ((rc = n_active(active, want)))  // synchronizes the stream: the events are complete
         return rc;

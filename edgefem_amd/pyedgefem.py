@@ -2159,6 +2159,8 @@ def frequency_sweep_eigenmode(mesh: Mesh, params: MaxwellParams, bc: BC, ports: 
         sweep.set_variants(int(_state.get("spmv_variant", -1)), int(_state.get("aux_variant", -1)))
     if _state is not None and "iteration" in _state:
         sweep.set_iteration(int(_state["iteration"]))
+    if _state is not None and "graph" in _state:
+        sweep.set_graph(bool(_state["graph"]))
     n = mesh.num_edges()
     tol = 1e-14 if opts.use_direct else float(opts.tolerance)
     infos = []

@@ -293,6 +293,13 @@ int efm_sweep_set_variants(efm_sweep *sw, int spmv, int aux);
  * reference counterpart. */
 int efm_sweep_set_iteration(efm_sweep *sw, int form);
 int efm_sweep_get_variants(const efm_sweep *sw, int *h_spmv, int *h_aux, double *h_tune_ms);
+/* Standard iteration forms (0, 2): enable = 1 captures one full convergence-check chunk of
+ * iterations (check_every iterations of kernel launches, no memsets) into a HIP graph once
+ * per solve, after the autotuner has chosen the kernels, and replays it for every later full
+ * chunk; the convergence polls and residual replacements stay outside the graph.  The
+ * results are those of the plain launch sequence.  0 (default) = plain launches.  No
+ * reference counterpart. */
+int efm_sweep_set_graph(efm_sweep *sw, int enable);
 
 /* ------------------------------------------------------------------------
  * Periodic (Floquet) elimination on the sparse matrix: replaces the dense elimination of
