@@ -253,9 +253,10 @@ def run_sweep(args, steps, warmup, prec, cpu=True):
     for _ in range(warmup):
         em.frequency_sweep_eigenmode(mesh, params, bc, ports, freqs, precision=prec,
                                      _state={"spmv_variant": args.sweep_spmv, "aux_variant": min(args.sweep_spmv, 1),
-                                             "iteration": args.sweep_iteration})
+                                             "iteration": args.sweep_iteration, "graph": args.sweep_graph})
     torch.cuda.synchronize()
-    st = {"spmv_variant": args.sweep_spmv, "aux_variant": min(args.sweep_spmv, 1), "iteration": args.sweep_iteration}
+    st = {"spmv_variant": args.sweep_spmv, "aux_variant": min(args.sweep_spmv, 1), "iteration": args.sweep_iteration,
+          "graph": args.sweep_graph}
     ts = time.perf_counter()
     for _ in range(steps):
         S = em.frequency_sweep_eigenmode(mesh, params, bc, ports, freqs, precision=prec, _state=st)
@@ -311,7 +312,7 @@ def run_sweep(args, steps, warmup, prec, cpu=True):
                    "freqs_hz": [float(freqs[0]), float(freqs[-1]), int(freqs.size)],
                    "solver": "batched COCG+Jacobi+nodal-gradient", "tol": 1e-10,
                    "iteration": ("standard", "chronopoulos-gear", "standard-u-rebuilt")[args.sweep_iteration],
-                   "precision": prec},
+                   "precision": prec, "graph_replay": bool(args.sweep_graph)},
         "plain_path_solves_per_s": plain, "speedup_vs_plain": value / plain,
         "delta_S_vs_fp64_lu": dS, "delta_S_plain_vs_fp64_lu": dS_plain,
         "iterations": [int(its.min()), int(its.max())], "max_true_rel_residual": float(res.max()),
@@ -617,6 +618,8 @@ def main():
     ap.add_argument("--sweep-spmv", type=int, default=-1, choices=[-1, 0, 1, 2],
                     help="sweep: SpMV and nodal kernels (0 one slice per system group, 1 pipelined, 2 LDS-tiled "
                          "SpMV with the pipelined nodal kernel, -1 autotuned)")
+    ap.add_argument("--sweep-graph", type=int, default=0, choices=[0, 1],
+                    help="sweep: replay full 16-iteration chunks from a captured HIP graph")
     ap.add_argument("--sweep-iteration", type=int, default=2, choices=[0, 1, 2],
                     help="sweep: 0 standard COCG (p in its own pass), 1 Chronopoulos-Gear (one reduction), "
                          "2 standard with u rebuilt in the p pass (no u store)")
@@ -711,9 +714,10 @@ def main():
     R = len(ports)
     s_val = 16 if args.mode == "fp64" else 8
     # algorithmic bytes of the chosen SpMV: the fused form reads z and p_old and writes p and
-    # q per row and RHS (4 passes); variant 9 (p formed in its own pass) gathers p and
-    # writes q (2 passes)
-    spmv_bytes = nnz * (s_val + 4) + 4 * (n + 1) + R * n * 16 * (2 if variant == 9 else 4)
+    # q per row and RHS (4 passes); variants 9 and 10 (p formed in its own pass, tiled or
+    # sliced-ELL) gather p and write q (2 passes); the sliced-ELL padding slots are not
+    # algorithmic bytes
+    spmv_bytes = nnz * (s_val + 4) + 4 * (n + 1) + R * n * 16 * (2 if variant in (9, 10) else 4)
     avg_ms = spmv_ms / max(spmv_n, 1)
     achieved = spmv_bytes / (avg_ms * 1e-3)
     iters = int(max(pipe.last_info["iters"]))
@@ -752,7 +756,7 @@ def main():
                    "mode": args.mode},
         "roofline": {"bound": "hbm", "achieved": achieved / 1e9, "peak": HBM_PEAK / 1e9, "unit": "GB/s",
                      "frac": achieved / HBM_PEAK, "traffic": traffic, "traffic_unit": "GB per launch",
-                     "kernel": "k_cocgz_spmv" if variant == 9 else "k_cocg_spmv",
+                     "kernel": {9: "k_cocgz_spmv", 10: "k_cocgz_sell"}.get(variant, "k_cocg_spmv"),
                      "bytes_per_launch": spmv_bytes, "avg_launch_ms": avg_ms, "launches": spmv_n,
                      "variant": variant, "autotune_ms": [round(float(t), 4) for t in tune_ms]},
         "cpu_baseline": cpu,

@@ -37,6 +37,7 @@ EXPORTS = [
     "efm_gmsh_read", "efm_gmsh_sizes", "efm_gmsh_copy", "efm_gmsh_free",
     "efm_touchstone_write", "efm_touchstone_write_2port", "efm_touchstone_extension",
     "efm_port_overlap_noalloc", "efm_sweep_set_graph",
+    "efm_array_package_json", "efm_patterns_csv", "efm_coupling_csv", "efm_scan_csv",
 ]
 
 
@@ -132,6 +133,11 @@ def load_library(path: str = LIB_PATH) -> C.CDLL:
     lib.efm_sweep_solve.argtypes = [vp, i32, vp, vp, vp, vp, C.POINTER(SolveOpts), C.POINTER(RefineOpts), vp, vp,
                                     vp]
     lib.efm_port_overlap.argtypes = [i64, i32, i64, vp, vp, vp, i32, vp, vp]
+    lib.efm_array_package_json.argtypes = [C.c_char_p, C.c_char_p, C.c_double, i64, vp, i64, vp, i64, i32, vp, i64,
+                                           i64, vp, vp, vp]
+    lib.efm_patterns_csv.argtypes = [C.c_char_p, i64, vp, vp, vp]
+    lib.efm_coupling_csv.argtypes = [C.c_char_p, i32, vp, vp]
+    lib.efm_scan_csv.argtypes = [C.c_char_p, i64, vp, vp, vp, vp, vp]
     lib.efm_port_overlap_noalloc.argtypes = [i64, i32, i64, vp, vp, vp, i32, vp, vp]
     for name in EXPORTS:
         getattr(lib, name)  # AttributeError if the library lacks a declared symbol
@@ -394,7 +400,7 @@ class KrylovSolver:
 
     def spmv_variant(self):
         v = C.c_int()
-        ms = np.zeros(10, np.float64)
+        ms = np.zeros(11, np.float64)
         _check(self._lib.efm_solver_get_spmv_variant(self._h, C.byref(v), ms.ctypes.data_as(C.c_void_p)),
                "efm_solver_get_spmv_variant")
         return v.value, ms
@@ -554,6 +560,55 @@ def touchstone_write(path: str, freq, S, fmt: int, z0: float) -> None:
     P = s.shape[1] if s.ndim == 3 else 0
     _check(lib.efm_touchstone_write(str(path).encode(), f.size, f.ctypes.data_as(C.c_void_p), P,
                                     s.ctypes.data_as(C.c_void_p), int(fmt), float(z0)), "efm_touchstone_write")
+
+
+def _vp(a):
+    return None if a is None else a.ctypes.data_as(C.c_void_p)
+
+
+def array_package_json(path: str, model_name: str, z0: float, positions, freqs, S, n_pattern_freqs: int,
+                       scan_theta, scan_phi, scan_loss_db) -> None:
+    """efm_array_package_json: positions (N, 3), freqs (F,), S (F_S, P, P) complex128, scan
+    arrays (K,) in radians / dB; host arrays."""
+    lib = load_library()
+    pos = np.ascontiguousarray(positions, np.float64).reshape(-1, 3)
+    fr = np.ascontiguousarray(freqs, np.float64).reshape(-1)
+    s = np.ascontiguousarray(S, np.complex128)
+    nS, P = (s.shape[0], s.shape[1]) if s.ndim == 3 else (0, 0)
+    th = np.ascontiguousarray(scan_theta, np.float64).reshape(-1)
+    ph = np.ascontiguousarray(scan_phi, np.float64).reshape(-1)
+    sl = np.ascontiguousarray(scan_loss_db, np.float64).reshape(-1)
+    _check(lib.efm_array_package_json(str(path).encode(), str(model_name).encode(), float(z0), pos.shape[0], _vp(pos),
+                                      fr.size, _vp(fr), nS, P, _vp(s), int(n_pattern_freqs), th.size, _vp(th),
+                                      _vp(ph), _vp(sl)), "efm_array_package_json")
+
+
+def patterns_csv(path: str, port_index, npts, rows) -> None:
+    """efm_patterns_csv: port_index (K,) int32, npts (K,) int64, rows (sum npts, 5)."""
+    lib = load_library()
+    pi = np.ascontiguousarray(port_index, np.int32).reshape(-1)
+    npt = np.ascontiguousarray(npts, np.int64).reshape(-1)
+    r = np.ascontiguousarray(rows, np.float64).reshape(-1, 5)
+    _check(lib.efm_patterns_csv(str(path).encode(), pi.size, _vp(pi), _vp(npt), _vp(r)), "efm_patterns_csv")
+
+
+def coupling_csv(path: str, S, positions=None) -> None:
+    """efm_coupling_csv: S (P, P) complex128, positions (P, 3) or None."""
+    lib = load_library()
+    s = np.ascontiguousarray(S, np.complex128)
+    pos = None if positions is None else np.ascontiguousarray(positions, np.float64).reshape(-1, 3)
+    _check(lib.efm_coupling_csv(str(path).encode(), s.shape[0], _vp(s), _vp(pos)), "efm_coupling_csv")
+
+
+def scan_csv(path: str, theta, phi, nelem, gamma, Z) -> None:
+    """efm_scan_csv: theta, phi (K,) radians, nelem (K,) int64, gamma and Z concatenated complex128."""
+    lib = load_library()
+    th = np.ascontiguousarray(theta, np.float64).reshape(-1)
+    ph = np.ascontiguousarray(phi, np.float64).reshape(-1)
+    ne = np.ascontiguousarray(nelem, np.int64).reshape(-1)
+    g = np.ascontiguousarray(gamma, np.complex128).reshape(-1)
+    z = np.ascontiguousarray(Z, np.complex128).reshape(-1)
+    _check(lib.efm_scan_csv(str(path).encode(), th.size, _vp(th), _vp(ph), _vp(ne), _vp(g), _vp(z)), "efm_scan_csv")
 
 
 def touchstone_write_2port(path: str, freq, S4) -> None:

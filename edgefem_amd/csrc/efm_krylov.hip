@@ -20,6 +20,8 @@
 // `check_every` iterations.
 #include "efm_internal.h"
 
+#include <algorithm>
+#include <climits>
 #include <type_traits>
 #include <vector>
 
@@ -2028,10 +2030,131 @@ __global__ void __launch_bounds__(kBlock)
   }
 }
 
+// ---------------------------------------------------------------------------
+// Sliced-ELL SpMV of the p-pass form (variant 10): q = A p, partials p^T q.
+// Rows are grouped in slices of 64/RL rows; a row's entries are spread over RL adjacent
+// lanes (entry e of a row at step e / RL, lane e % RL of the row's lane group), and the
+// slice's entries are stored step-major, so every step is one contiguous 64-entry load of
+// values and columns.  Rows are taken in order of decreasing length inside windows of
+// 4096 rows, so a slice pads little.  Each lane group sums its row in registers and
+// reduces it with shuffles: no LDS, no barrier, so occupancy is set by registers only.
+// Padding slots point at column 0 with value 0.  Slices are walked XCD-contiguously.
+// ---------------------------------------------------------------------------
+template <typename VT, int R, int RL>
+__global__ void __launch_bounds__(kBlock)
+    k_cocgz_sell(int nslices, int xcd_span, const int32_t *__restrict__ soff, const int32_t *__restrict__ srow,
+                 const int32_t *__restrict__ scol, const VT *__restrict__ sval, const double2 *__restrict__ pf,
+                 double2 *__restrict__ q, double2 *__restrict__ partial, const SysState *__restrict__ st) {
+  constexpr int SR = 64 / RL;
+  __shared__ double2 lds[(kBlock / 64) * R];
+  bool act[R];
+#pragma unroll
+  for (int s = 0; s < R; ++s)
+    act[s] = st[s].active != 0;
+  double2 pq[R];
+#pragma unroll
+  for (int s = 0; s < R; ++s)
+    pq[s] = make_double2(0, 0);
+  const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
+  const int xcd = blockIdx.x & 7;
+  const int sb = xcd * xcd_span, se = min(nslices, sb + xcd_span);
+  const int wstride = (gridDim.x >> 3) * (kBlock / 64);
+  for (int sl = sb + (blockIdx.x >> 3) * (kBlock / 64) + wv; sl < se; sl += wstride) {
+    const int o0 = soff[sl], o1 = soff[sl + 1];
+    const int32_t row = srow[sl * SR + lane / RL];
+    double2 acc[R];
+#pragma unroll
+    for (int s = 0; s < R; ++s)
+      acc[s] = make_double2(0, 0);
+    int o = o0;
+    for (; o + 1 < o1; o += 2) {
+      const int64_t i0 = (int64_t)o * 64 + lane;
+      const int32_t c0 = ld_stream(scol + i0), c1 = ld_stream(scol + i0 + 64);
+      const double2 a0 = ld_stream(sval + i0), a1 = ld_stream(sval + i0 + 64);
+      double2 g0[R], g1[R];
+#pragma unroll
+      for (int s = 0; s < R; ++s) {
+        g0[s] = pf[(int64_t)c0 * R + s];
+        g1[s] = pf[(int64_t)c1 * R + s];
+      }
+#pragma unroll
+      for (int s = 0; s < R; ++s)
+        acc[s] = cfma(a1, g1[s], cfma(a0, g0[s], acc[s]));
+    }
+    if (o < o1) {
+      const int64_t i0 = (int64_t)o * 64 + lane;
+      const int32_t c0 = ld_stream(scol + i0);
+      const double2 a0 = ld_stream(sval + i0);
+#pragma unroll
+      for (int s = 0; s < R; ++s)
+        acc[s] = cfma(a0, pf[(int64_t)c0 * R + s], acc[s]);
+    }
+#pragma unroll
+    for (int s = 0; s < R; ++s)
+#pragma unroll
+      for (int off = RL / 2; off > 0; off >>= 1) {
+        acc[s].x += __shfl_xor(acc[s].x, off, RL);
+        acc[s].y += __shfl_xor(acc[s].y, off, RL);
+      }
+    if (row >= 0 && (lane % RL) == 0) {
+#pragma unroll
+      for (int s = 0; s < R; ++s) {
+        if (!act[s])
+          continue;
+        q[(int64_t)row * R + s] = acc[s];
+        pq[s] = cadd(pq[s], bil(pf[(int64_t)row * R + s], acc[s]));
+      }
+    }
+  }
+  write_partials<R>(pq, lds, partial);
+}
+
+// Sliced-ELL structure from the CSR pattern, one wave per slice: for every slot the column
+// (0 for padding) and the CSR position of its value (-1 for padding).
+template <int RL>
+__global__ void __launch_bounds__(kBlock)
+    k_sell_build(int nslices, const int32_t *__restrict__ soff, const int32_t *__restrict__ srow,
+                 const int32_t *__restrict__ row_ptr, const int32_t *__restrict__ col, int32_t *__restrict__ scol,
+                 int32_t *__restrict__ svidx) {
+  constexpr int SR = 64 / RL;
+  const int sl = (int)(((int64_t)blockIdx.x * kBlock + threadIdx.x) >> 6);
+  const int lane = threadIdx.x & 63;
+  if (sl >= nslices)
+    return;
+  const int32_t row = srow[sl * SR + lane / RL];
+  const int32_t k0 = row >= 0 ? row_ptr[row] : 0, k1 = row >= 0 ? row_ptr[row + 1] : 0;
+  for (int o = soff[sl], j = 0; o < soff[sl + 1]; ++o, ++j) {
+    const int32_t k = k0 + j * RL + lane % RL;
+    const bool in = row >= 0 && k < k1;
+    const int64_t i = (int64_t)o * 64 + lane;
+    scol[i] = in ? col[k] : 0;
+    svidx[i] = in ? k : -1;
+  }
+}
+
+// Sliced-ELL values from the CSR values (run after every matrix install).
+template <typename VT>
+__global__ void k_sell_vals(int64_t slots, const int32_t *__restrict__ svidx, const VT *__restrict__ val,
+                            VT *__restrict__ sval) {
+  for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < slots; i += (int64_t)gridDim.x * blockDim.x) {
+    const int32_t k = svidx[i];
+    VT v;
+    if (k >= 0) {
+      v = val[k];
+    } else {
+      v.x = 0;
+      v.y = 0;
+    }
+    sval[i] = v;
+  }
+}
+
 } // namespace
 
-constexpr int kNumSpmvVariants = 10;  // rows<4>, rows<8>, rows<16>, tiled<T,4>, tiled<T/2,8>, coop<T,4>, pipe<T,4>, lines<T/2,8>, local<4>, p-pass form
+constexpr int kNumSpmvVariants = 11;  // rows<4>, rows<8>, rows<16>, tiled<T,4>, tiled<T/2,8>, coop<T,4>, pipe<T,4>, lines<T/2,8>, local<4>, p-pass form, p-pass form with sliced ELL
 constexpr int kZForm = 9;             // COCG with p formed in its own pass (its own SpMV and update)
+constexpr int kSellForm = 10;         // the same iteration with the sliced-ELL SpMV
+constexpr int kSellRL = 4;            // lanes per row of the sliced-ELL SpMV
 
 struct efm_solver {
   int64_t n = 0, nnz = 0;
@@ -2056,6 +2179,12 @@ struct efm_solver {
   // variant 9 (p formed in its own pass): z and p arrays [n][R]; resident grid of its SpMV
   efm::DevBuf zf, pf;
   int grid_z = 8;
+  // variant 10 (sliced ELL, built on first use): slice offsets in 64-slot steps, the row of
+  // every lane group, columns and CSR value positions per slot, values per slot
+  efm::DevBuf sell_off, sell_row, sell_col, sell_vidx, sell_val;
+  int sell_nslices = 0, sell_xcd_span = 0, grid_sell = 8;
+  int64_t sell_slots = 0;
+  bool sell_built = false, sell_vals_ok = false;
   int grid_rows[3] = {8, 8, 8};
   int spmv_variant = -1;  // -1: choose by timing at the first solve
   double tune_ms[kNumSpmvVariants] = {0};
@@ -2174,6 +2303,64 @@ template <int R> struct Dispatch {
     return fin<kFinInitBs>(s, s->grid, st, tol2, 0);
   }
 
+  // The sliced-ELL structure of variant 10 from the pattern (host ordering, device fill).
+  static int build_sell(efm_solver *s, hipStream_t st) {
+    constexpr int SR = 64 / kSellRL;
+    const int64_t n = s->n;
+    std::vector<int32_t> rp(n + 1);
+    EFM_HIP_CHECK(hipMemcpyAsync(rp.data(), s->row_ptr, (n + 1) * sizeof(int32_t), hipMemcpyDeviceToHost, st));
+    EFM_HIP_CHECK(hipStreamSynchronize(st));
+    // rows by decreasing length inside windows of 4096 rows (stable: ties keep row order)
+    std::vector<int32_t> order(n);
+    for (int64_t i = 0; i < n; ++i)
+      order[i] = (int32_t)i;
+    const int64_t win = 4096;
+    for (int64_t w0 = 0; w0 < n; w0 += win) {
+      const int64_t w1 = std::min(n, w0 + win);
+      std::stable_sort(order.begin() + w0, order.begin() + w1,
+                       [&](int32_t a, int32_t b) { return rp[a + 1] - rp[a] > rp[b + 1] - rp[b]; });
+    }
+    const int64_t ns = (n + SR - 1) / SR;
+    if (ns >= INT32_MAX)
+      return EFM_ERR_ARG;
+    std::vector<int32_t> srow(ns * SR, -1), soff(ns + 1, 0);
+    int64_t steps = 0;
+    for (int64_t sl = 0; sl < ns; ++sl) {
+      int32_t mx = 0;
+      for (int r = 0; r < SR; ++r) {
+        const int64_t k = sl * SR + r;
+        if (k < n) {
+          srow[k] = order[k];
+          mx = std::max(mx, rp[order[k] + 1] - rp[order[k]]);
+        }
+      }
+      steps += (mx + kSellRL - 1) / kSellRL;
+      if (steps * 64 >= INT32_MAX)
+        return EFM_ERR_ARG;
+      soff[sl + 1] = (int32_t)steps;
+    }
+    const int64_t slots = steps * 64;
+    const size_t vsz = s->vtype == EFM_C128 ? sizeof(double2) : sizeof(float2);
+    int rc;
+    if ((rc = s->sell_off.alloc((ns + 1) * sizeof(int32_t))) || (rc = s->sell_row.alloc(ns * SR * sizeof(int32_t))) ||
+        (rc = s->sell_col.alloc(slots * sizeof(int32_t))) || (rc = s->sell_vidx.alloc(slots * sizeof(int32_t))) ||
+        (rc = s->sell_val.alloc(slots * vsz)))
+      return rc;
+    EFM_HIP_CHECK(hipMemcpyAsync(s->sell_off.p, soff.data(), (ns + 1) * sizeof(int32_t), hipMemcpyHostToDevice, st));
+    EFM_HIP_CHECK(hipMemcpyAsync(s->sell_row.p, srow.data(), ns * SR * sizeof(int32_t), hipMemcpyHostToDevice, st));
+    hipLaunchKernelGGL(k_sell_build<kSellRL>, dim3(grid_for(ns * 64, kBlock)), dim3(kBlock), 0, st, (int)ns,
+                       s->sell_off.as<int32_t>(), s->sell_row.as<int32_t>(), s->row_ptr, s->col,
+                       s->sell_col.as<int32_t>(), s->sell_vidx.as<int32_t>());
+    EFM_LAUNCH_CHECK();
+    EFM_HIP_CHECK(hipStreamSynchronize(st));  // the host vectors above go out of scope
+    s->sell_nslices = (int)ns;
+    s->sell_xcd_span = (int)((ns + 7) / 8);
+    s->sell_slots = slots;
+    s->sell_built = true;
+    s->sell_vals_ok = false;
+    return EFM_OK;
+  }
+
   // One fused SpMV of the given variant; *nb = number of partial sums written.
   template <typename VT>
   static int launch_spmv(efm_solver *s, int variant, const double2 *Wa, double2 *Wb, hipStream_t st,
@@ -2257,6 +2444,23 @@ template <int R> struct Dispatch {
       }
       set_error("SpMV variant 8 needs 1, 2 or 4 right-hand sides");
       return EFM_ERR_ARG;
+    case kSellForm: {
+      int rc;
+      if (!s->sell_built && (rc = build_sell(s, st)) != EFM_OK)
+        return rc;
+      if (!s->sell_vals_ok) {
+        hipLaunchKernelGGL(k_sell_vals<VT>, dim3(grid_for(s->sell_slots, 256)), dim3(256), 0, st, s->sell_slots,
+                           s->sell_vidx.as<int32_t>(), s->val.as<VT>(), s->sell_val.as<VT>());
+        EFM_LAUNCH_CHECK();
+        s->sell_vals_ok = true;
+      }
+      *nb = s->grid_sell;
+      hipLaunchKernelGGL((k_cocgz_sell<VT, R, kSellRL>), dim3(*nb), dim3(kBlock), 0, st, s->sell_nslices,
+                         s->sell_xcd_span, s->sell_off.as<int32_t>(), s->sell_row.as<int32_t>(),
+                         s->sell_col.as<int32_t>(), s->sell_val.as<VT>(), s->pf.as<double2>(), s->q.as<double2>(),
+                         s->partial.as<double2>(), S);
+      break;
+    }
     case kZForm:
       *nb = s->grid_z;
       hipLaunchKernelGGL((k_cocgz_spmv<VT, R, TN, 4>), dim3(*nb), dim3(kBlock), 0, st, s->ntiles[0],
@@ -2294,11 +2498,13 @@ template <int R> struct Dispatch {
         continue;
       if (v == kZForm && (s->ntiles[0] == 0 || s->lr_P > 0))
         continue;
+      if (v == kSellForm && s->lr_P > 0)
+        continue;
       int nb;
       // variant 9 is timed with its p pass (run on W1 as scratch: the real p must stay as
       // init left it)
       auto launch = [&]() -> int {
-        if (v == kZForm) {
+        if (v == kZForm || v == kSellForm) {
           hipLaunchKernelGGL(k_cocgz_pform<R>, dim3(s->grid), dim3(kBlock), 0, st, s->n, s->zf.as<double2>(),
                              s->W1.as<double2>(), s->state.as<SysState>());
           EFM_LAUNCH_CHECK();
@@ -2335,7 +2541,7 @@ template <int R> struct Dispatch {
     double2 *Wa = (it & 1) ? s->W1.as<double2>() : s->W0.as<double2>();
     double2 *Wb = (it & 1) ? s->W0.as<double2>() : s->W1.as<double2>();
     SysState *S = s->state.as<SysState>();
-    if (s->spmv_variant == kZForm) {
+    if (s->spmv_variant == kZForm || s->spmv_variant == kSellForm) {
       int nb_spmv = 0;
       int rc;
       // p = z + beta p, then q = A p
@@ -2344,7 +2550,7 @@ template <int R> struct Dispatch {
       EFM_LAUNCH_CHECK();
       if (e0)
         EFM_HIP_CHECK(hipEventRecord(e0, st));
-      if ((rc = launch_spmv<VT>(s, kZForm, Wa, Wb, st, &nb_spmv)) != EFM_OK)
+      if ((rc = launch_spmv<VT>(s, s->spmv_variant, Wa, Wb, st, &nb_spmv)) != EFM_OK)
         return rc;
       if (e1)
         EFM_HIP_CHECK(hipEventRecord(e1, st));
@@ -2661,7 +2867,23 @@ extern "C" int efm_solver_create(int64_t n, int64_t nnz, const int32_t *row_ptr,
     const int64_t per_xcd = std::min<int64_t>(s->xcd_span[0], std::max<int64_t>(1, (int64_t)ncu * nbl / 8));
     s->grid_local = (int)(8 * per_xcd);
   }
+  {
+    // sliced-ELL SpMV: as many blocks as are resident, a multiple of 8
+    const bool c64 = value_type == EFM_C64;
+    const void *ks = nullptr;
+#define EFM_SELL_K(RR)                                                                                    \
+  (c64 ? reinterpret_cast<const void *>(&k_cocgz_sell<float2, RR, kSellRL>)                              \
+       : reinterpret_cast<const void *>(&k_cocgz_sell<double2, RR, kSellRL>))
+    ks = nrhs == 1 ? EFM_SELL_K(1) : nrhs == 2 ? EFM_SELL_K(2) : nrhs == 3 ? EFM_SELL_K(3)
+       : nrhs == 4 ? EFM_SELL_K(4) : EFM_SELL_K(8);
+#undef EFM_SELL_K
+    int nbs = 0;
+    if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&nbs, ks, kBlock, 0) != hipSuccess || nbs < 1)
+      nbs = 1;
+    s->grid_sell = (int)std::max<int64_t>(8, ((int64_t)ncu * nbs / 8) * 8);
+  }
   int gmax = std::max(s->grid, s->grid_spmv);
+  gmax = std::max(gmax, s->grid_sell);
   for (int v = 0; v < 3; ++v)
     gmax = std::max(gmax, s->grid_rows[v]);
   for (int v = 0; v < 2; ++v)
@@ -2690,6 +2912,7 @@ extern "C" int efm_solver_set_matrix(efm_solver *s, const void *values, void *st
   EFM_REQUIRE(s && values, "efm_solver_set_matrix: NULL argument");
   hipStream_t st = (hipStream_t)stream;
   EFM_HIP_CHECK(hipMemcpyAsync(s->val.p, values, s->val.bytes, hipMemcpyDeviceToDevice, st));
+  s->sell_vals_ok = false;  // the sliced-ELL values are refilled at the next SpMV of variant 10
   if (s->vtype == EFM_C128)
     hipLaunchKernelGGL(k_dinv<double2>, dim3(grid_for(s->n, 256)), dim3(256), 0, st, s->n, s->row_ptr,
                        s->col, s->val.as<double2>(), s->dinv.as<double2>());
@@ -2890,7 +3113,7 @@ extern "C" int efm_solver_spmv(efm_solver *s, const double *x, double *y, void *
 extern "C" int efm_solver_set_spmv_variant(efm_solver *s, int variant) {
   EFM_REQUIRE(s, "efm_solver_set_spmv_variant: NULL solver");
   EFM_REQUIRE(variant >= -1 && variant < kNumSpmvVariants, "efm_solver_set_spmv_variant: bad variant");
-  EFM_REQUIRE(variant < 3 ||
+  EFM_REQUIRE(variant < 3 || variant == kSellForm ||
                   s->ntiles[(variant == 5 || variant == 6 || variant == 8 || variant == kZForm) ? 0
                                                                                          : (variant == 7 ? 1 : variant - 3)] > 0,
               "efm_solver_set_spmv_variant: matrix has a row longer than a tile");
@@ -2941,8 +3164,8 @@ extern "C" int efm_solver_solve(efm_solver *s, const double *B, double *X,
   const int check = h_opts->check_every > 0 ? h_opts->check_every : 32;
   const bool cocg = h_opts->method == EFM_METHOD_COCG;
   EFM_REQUIRE(!(cocg && s->caux), "efm_solver_solve: the complex-incidence preconditioner needs BiCGSTAB");
-  EFM_REQUIRE(!(cocg && s->spmv_variant == kZForm && s->lr_P > 0),
-              "efm_solver_solve: SpMV variant 9 (p formed in its own pass) does not take low-rank port terms");
+  EFM_REQUIRE(!(cocg && (s->spmv_variant == kZForm || s->spmv_variant == kSellForm) && s->lr_P > 0),
+              "efm_solver_solve: SpMV variants 9 and 10 (p formed in its own pass) do not take low-rank port terms");
   int rc;
   if (!cocg) {
     const size_t vb = (size_t)s->n * s->R * sizeof(double2);

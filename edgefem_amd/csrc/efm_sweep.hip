@@ -1641,6 +1641,7 @@ struct efm_sweep {
   // standard forms: replay each full convergence-check chunk of iterations from a HIP graph
   // captured once per solve (after the autotuner has chosen the kernels)
   int use_graph = 0;
+  hipStream_t capture_stream = nullptr;  // the chunk is captured here (a legacy null stream cannot capture)
   int gx_std[2] = {8, 8};  // grid of the standard form's update [VT c128, c64]
   // update grids (one block slice, as many blocks as are resident): [VT c128, c64][INIT, skip, apply]
   int gx_upd[2][3] = {{8, 8, 8}, {8, 8, 8}};
@@ -1650,6 +1651,10 @@ struct efm_sweep {
   efm::DevBuf x64;  // reliable updates: FP64 solution (allocated on first use)
   efm::DevBuf dninv;  // [N][Bmax] inverse nodal diagonals of the systems of one solve
   efm::DevBuf tucol, tnu, tlidx, tdloc;  // row-tile format of the LDS-tiled SpMV
+  ~efm_sweep() {
+    if (capture_stream)
+      (void)hipStreamDestroy(capture_stream);
+  }
 };
 
 namespace {
@@ -2264,11 +2269,16 @@ struct SwRun {
           return rc;
         continue;
       }
+      // the chunk is recorded on the sweep's own stream and launched on st
       CaptureGuard capture;
+      const hipStream_t run_stream = st;
       if (replay) {
-        EFM_HIP_CHECK(hipStreamBeginCapture(st, hipStreamCaptureModeThreadLocal));
-        capture.st = st;
+        if (!s->capture_stream)
+          EFM_HIP_CHECK(hipStreamCreateWithFlags(&s->capture_stream, hipStreamNonBlocking));
+        EFM_HIP_CHECK(hipStreamBeginCapture(s->capture_stream, hipStreamCaptureModeThreadLocal));
+        capture.st = s->capture_stream;
         capture.on = true;
+        st = s->capture_stream;
       }
       for (int c = 0; c < chunk; ++c, ++it) {
         const int va = va_of(it), vs = vs_of(it);
@@ -2293,7 +2303,8 @@ struct SwRun {
       }
       if (replay) {  // the chunk was captured, not run: instantiate and launch it
         capture.on = false;
-        EFM_HIP_CHECK(hipStreamEndCapture(st, &graph.g));
+        st = run_stream;
+        EFM_HIP_CHECK(hipStreamEndCapture(s->capture_stream, &graph.g));
         EFM_HIP_CHECK(hipGraphInstantiate(&graph.x, graph.g, nullptr, nullptr, 0));
         EFM_HIP_CHECK(hipGraphLaunch(graph.x, st));
       }

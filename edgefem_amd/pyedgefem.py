@@ -2305,3 +2305,112 @@ def write_touchstone_nport(path: str, freq, S_matrices, opts: Optional[Touchston
 def touchstone_extension(num_ports: int) -> str:
     """``.s<N>p`` for 1 <= N <= 99 (touchstone.cpp:137-145)."""
     return _capi.touchstone_extension(num_ports)
+
+
+# ---------------------------------------------------------------------------
+# Array-package export (src/io/array_export.cpp; types include/edgefem/array/*.hpp and
+# include/edgefem/io/array_export.hpp, bound at python/pyedgefem.cpp:786-960).  The files
+# are formatted and written by the library (efm_array_package_json, efm_patterns_csv,
+# efm_coupling_csv, efm_scan_csv).
+# ---------------------------------------------------------------------------
+class FFPoint2D:
+    """Far-field point of a 2D pattern cut: theta (rad), linear magnitude, phase (rad)."""
+
+    def __init__(self, theta: float = 0.0, magnitude: float = 0.0, phase: float = 0.0):
+        self.theta, self.magnitude, self.phase = float(theta), float(magnitude), float(phase)
+
+
+class EmbeddedPattern:
+    def __init__(self):
+        self.port_index = 0
+        self.pattern_phi0: List[FFPoint2D] = []
+        self.pattern_phi90: List[FFPoint2D] = []
+        self.frequency = 0.0
+        self.element_position = np.zeros(3)
+
+
+class ActiveImpedanceResult:
+    def __init__(self):
+        self.theta = 0.0
+        self.phi = 0.0
+        self.Z_active = VecC(np.zeros(0, np.complex128))
+        self.Gamma_active = VecC(np.zeros(0, np.complex128))
+        self.scan_loss_db = 0.0
+
+
+class ArrayPackage:
+    def __init__(self):
+        self.model_name = ""
+        self.element_positions: List[np.ndarray] = []
+        self.frequencies: List[float] = []
+        self.S_matrices: List[np.ndarray] = []
+        self.embedded_patterns: List[List[EmbeddedPattern]] = []
+        self.scan_data: List[ActiveImpedanceResult] = []
+        self.reference_impedance = 50.0
+
+
+def _vec_values(v) -> np.ndarray:
+    return np.asarray(v.to_numpy() if isinstance(v, VecC) else v, np.complex128).reshape(-1)
+
+
+def export_array_package_json(path: str, pkg: ArrayPackage) -> None:
+    """export_array_package_json (array_export.cpp:33-123): handshake fields, element
+    positions (m), frequencies (Hz), one S matrix per frequency as [re, im] pairs, the
+    embedded-pattern and scan summaries.  Every S matrix must be square, of one size."""
+    pos = np.array([np.asarray(p, np.float64).reshape(3) for p in pkg.element_positions]).reshape(-1, 3)
+    mats = [np.asarray(m, np.complex128) for m in pkg.S_matrices]
+    if any(m.ndim != 2 or m.shape[0] != m.shape[1] or m.shape != mats[0].shape for m in mats):
+        raise RuntimeError("S matrices must be square and of one size")
+    if len(mats) > len(pkg.frequencies):
+        raise RuntimeError("more S matrices than frequencies")
+    S = np.stack(mats) if mats else np.zeros((0, 0, 0), np.complex128)
+    sd = pkg.scan_data
+    _capi.array_package_json(path, pkg.model_name, float(pkg.reference_impedance), pos,
+                             np.asarray(pkg.frequencies, np.float64), S, len(pkg.embedded_patterns),
+                             [r.theta for r in sd], [r.phi for r in sd], [r.scan_loss_db for r in sd])
+
+
+def export_patterns_csv(path: str, patterns: Sequence[EmbeddedPattern]) -> None:
+    """export_patterns_csv (array_export.cpp:125-145): per pattern, min(len(phi0), len(phi90))
+    rows of port index, theta and the two cuts' magnitude and phase, angles in degrees."""
+    idx, npts, rows = [], [], []
+    for pat in patterns:
+        n = min(len(pat.pattern_phi0), len(pat.pattern_phi90))
+        idx.append(int(pat.port_index))
+        npts.append(n)
+        for a, b in zip(pat.pattern_phi0[:n], pat.pattern_phi90[:n]):
+            rows.append((a.theta, a.magnitude, a.phase, b.magnitude, b.phase))
+    _capi.patterns_csv(path, idx, npts, np.array(rows, np.float64).reshape(-1, 5))
+
+
+def export_coupling_csv(path: str, S, positions=()) -> None:
+    """export_coupling_csv (array_export.cpp:147-165): every off-diagonal S(i, j) with |S| in
+    dB, its phase in degrees and the element distance (0 unless one position per port is
+    given, as extract_all_couplings does, coupling_extract.cpp:21-57)."""
+    S = np.asarray(S, np.complex128)
+    if S.ndim != 2 or S.shape[0] != S.shape[1]:
+        raise RuntimeError("S-matrix must be square")
+    pos = None
+    if len(positions) == S.shape[0] and S.shape[0] > 0:
+        pos = np.array([np.asarray(p, np.float64).reshape(3) for p in positions])
+    _capi.coupling_csv(path, S, pos)
+
+
+def export_scan_csv(path: str, scan_results: Sequence[ActiveImpedanceResult]) -> None:
+    """export_scan_csv (array_export.cpp:167-189): one row per (scan angle, element) with
+    Gamma, Z and VSWR = (1 + |Gamma|)/(1 - |Gamma|) (1e12 at |Gamma| >= 1 - 1e-12,
+    active_vswr, active_impedance.cpp:117-123), angles in degrees."""
+    g = [_vec_values(r.Gamma_active) for r in scan_results]
+    z = [_vec_values(r.Z_active) for r in scan_results]
+    # the reference writes Z_active(i) for every element of Gamma_active
+    if any(zz.size < gg.size for gg, zz in zip(g, z)):
+        raise RuntimeError("Z_active has fewer entries than Gamma_active")
+    _capi.scan_csv(path, [r.theta for r in scan_results], [r.phi for r in scan_results], [gg.size for gg in g],
+                   np.concatenate(g) if g else np.zeros(0, np.complex128),
+                   np.concatenate([zz[:gg.size] for gg, zz in zip(g, z)]) if z else np.zeros(0, np.complex128))
+
+
+def active_vswr(gamma_active: complex) -> float:
+    """(1 + |Gamma|) / (1 - |Gamma|), 1e12 when |Gamma| >= 1 - 1e-12 (active_impedance.cpp:117-123)."""
+    rho = math.hypot(complex(gamma_active).real, complex(gamma_active).imag)
+    return 1e12 if rho >= 1.0 - 1e-12 else (1.0 + rho) / (1.0 - rho)

@@ -189,10 +189,13 @@ int efm_solver_solve(efm_solver *s, const double *B, double *X, const efm_solve_
  * tile-local column format (each tile's distinct columns loaded once into LDS, 16-bit
  * local indices per entry, built on the device at first use), nrhs 1, 2 or 4; 9: the same
  * COCG iteration with p = z + beta p formed in a streaming pass of its own, so the SpMV
- * gathers only p (R values per column) and writes only q, COCG without low-rank terms).  By
+ * gathers only p (R values per column) and writes only q, COCG without low-rank terms; 10:
+ * the same iteration with a sliced-ELL SpMV: slices of 16 rows, 4 lanes per row, the entries
+ * of a slice stored step-major so that every step is one coalesced load, rows ordered by
+ * length inside windows of 4096, no LDS; built on the device at first use).  By
  * default (-1) the first solve times every variant on the installed matrix and keeps the
  * fastest; a fixed variant makes results reproducible bit for bit across runs.
- * h_tune_ms (10 entries, may be NULL) receives the
+ * h_tune_ms (11 entries, may be NULL) receives the
  * measured average ms per launch (-1 = not available).  No reference counterpart. */
 int efm_solver_set_spmv_variant(efm_solver *s, int variant);
 int efm_solver_get_spmv_variant(const efm_solver *s, int *h_variant, double *h_tune_ms);
@@ -426,6 +429,38 @@ int efm_touchstone_write(const char *path, int64_t nfreq, const double *freq, in
                          const double *S, int format, double z0);
 int efm_touchstone_write_2port(const char *path, int64_t nfreq, const double *freq, const double *S4);
 int efm_touchstone_extension(int num_ports, char *out, int cap);
+
+/* ------------------------------------------------------------------------
+ * Array-package writers (host code), the reference's file formats byte for byte (numbers
+ * at printf %.12g).  Each fails with "Cannot open file for writing: <path>" like the
+ * reference.  Host arrays only.
+ * efm_array_package_json replaces export_array_package_json (src/io/array_export.cpp:33-123):
+ *   positions [n_elements][3] m; freqs [n_freq] Hz; S [n_smat][nports][nports] c128 row-major,
+ *   one matrix per frequency (n_smat <= n_freq; the reference reads frequencies[f] for
+ *   matrix f); n_pattern_freqs = how many frequencies carry embedded patterns (the JSON holds
+ *   only that summary); scan theta, phi (radians, written in degrees) and scan loss (dB).
+ * efm_patterns_csv replaces export_patterns_csv (:125-145): n_patterns patterns, pattern p
+ *   with port index port_index[p] and npts[p] rows of rows[][5] = (theta, phi0 magnitude,
+ *   phi0 phase, phi90 magnitude, phi90 phase), angles in radians (the caller passes
+ *   min(len(phi0 cut), len(phi90 cut)) points, the theta of the phi0 cut).
+ * efm_coupling_csv replaces export_coupling_csv (:147-165) with extract_all_couplings
+ *   (src/array/coupling_extract.cpp:21-57): every (i, j), i != j, of S [nports][nports] c128,
+ *   |S| in dB (20 log10 max(|S|, 1e-20)), phase in degrees, element distance (positions
+ *   [nports][3], or NULL: 0).
+ * efm_scan_csv replaces export_scan_csv (:167-189) with active_vswr
+ *   (src/array/active_impedance.cpp:117-123): scan point s at (theta[s], phi[s]) radians has
+ *   nelem[s] elements; gamma, Z concatenated c128 per element.
+ * ---------------------------------------------------------------------- */
+int efm_array_package_json(const char *path, const char *model_name, double reference_impedance,
+                           int64_t n_elements, const double *h_positions, int64_t n_freq,
+                           const double *h_freqs, int64_t n_smat, int nports, const double *h_S,
+                           int64_t n_pattern_freqs, int64_t n_scan, const double *h_scan_theta,
+                           const double *h_scan_phi, const double *h_scan_loss_db);
+int efm_patterns_csv(const char *path, int64_t n_patterns, const int32_t *h_port_index,
+                     const int64_t *h_npts, const double *h_rows);
+int efm_coupling_csv(const char *path, int nports, const double *h_S, const double *h_positions);
+int efm_scan_csv(const char *path, int64_t n_scan, const double *h_theta, const double *h_phi,
+                 const int64_t *h_nelem, const double *h_gamma, const double *h_Z);
 
 #ifdef __cplusplus
 }

@@ -43,6 +43,12 @@ eps_gradient_eigenmode      adjoint of sparams_eigenmode w.r.t. per-tet eps_r (n
 touchstone_text             src/io/touchstone.cpp:68-135 (write_touchstone_nport;
                             iostream precision 12 = printf %.12g)
 touchstone_text_2port       src/io/touchstone.cpp:50-62 (legacy write_touchstone)
+array_package_json_text     src/io/array_export.cpp:33-123
+patterns_csv_text           src/io/array_export.cpp:125-145
+coupling_csv_text           src/io/array_export.cpp:147-165 (extract_all_couplings,
+                            src/array/coupling_extract.cpp:21-57)
+scan_csv_text               src/io/array_export.cpp:167-189 (active_vswr,
+                            src/array/active_impedance.cpp:117-123)
 ==========================  =========================================================
 
 Third-party routines replaced: Eigen ``setFromTriplets`` (duplicates summed) ->
@@ -1365,3 +1371,90 @@ def touchstone_text_2port(freq, S4) -> str:
     for f, row in zip(np.asarray(freq, np.float64), np.asarray(S4, np.complex128).reshape(-1, 4)):
         out.append(" ".join(["%.12g" % f] + ["%.12g %.12g" % (v.real, v.imag) for v in row]) + "\n")
     return "".join(out)
+
+
+# ---------------------------------------------------------------------------
+# Array-package text (iostream at precision 12 = %.12g; degrees = rad * 180 / pi)
+# ---------------------------------------------------------------------------
+def _g(v) -> str:
+    return "%.12g" % float(v)
+
+
+def _deg(r) -> float:
+    return float(r) * 180.0 / math.pi
+
+
+def array_package_json_text(model_name, z0, positions, freqs, S_list, n_pattern_freqs, scan) -> str:
+    """export_array_package_json (array_export.cpp:33-123); scan = [(theta, phi, loss_db)]."""
+    L = ["{", '  "format": "EdgeFEM_ArrayPackage",', '  "version": "1.0",', f'  "model_name": "{model_name}",',
+         f'  "reference_impedance": {_g(z0)},', f'  "num_elements": {len(positions)},', '  "element_positions": [']
+    for i, p in enumerate(positions):
+        L.append(f"    [{_g(p[0])}, {_g(p[1])}, {_g(p[2])}]" + ("," if i < len(positions) - 1 else ""))
+    L.append("  ],")
+    L.append(f'  "num_frequencies": {len(freqs)},')
+    L.append('  "frequencies": [' + ", ".join(_g(f) for f in freqs) + "],")
+    L.append('  "S_matrices": [')
+    for fi, S in enumerate(S_list):
+        S = np.asarray(S)
+        L.append("    {")
+        L.append(f'      "frequency": {_g(freqs[fi])},')
+        L.append('      "data": [')
+        for i in range(S.shape[0]):
+            row = ", ".join(f"[{_g(S[i, j].real)}, {_g(S[i, j].imag)}]" for j in range(S.shape[1]))
+            L.append(f"        [{row}]" + ("," if i < S.shape[0] - 1 else ""))
+        L.append("      ]")
+        L.append("    }" + ("," if fi < len(S_list) - 1 else ""))
+    L.append("  ],")
+    L.append(f'  "has_embedded_patterns": {"false" if n_pattern_freqs == 0 else "true"},')
+    L.append(f'  "num_pattern_frequencies": {n_pattern_freqs},')
+    L.append(f'  "has_scan_data": {"false" if not scan else "true"},')
+    if scan:
+        L.append(f'  "num_scan_points": {len(scan)},')
+        L.append('  "scan_angles": [')
+        for i, (t, p, l) in enumerate(scan):
+            L.append(f'    {{"theta": {_g(_deg(t))}, "phi": {_g(_deg(p))}, "scan_loss_db": {_g(l)}}}'
+                     + ("," if i < len(scan) - 1 else ""))
+        L.append("  ]")
+    else:
+        L.append('  "num_scan_points": 0')
+    L.append("}")
+    return "\n".join(L) + "\n"
+
+
+def patterns_csv_text(patterns) -> str:
+    """export_patterns_csv (array_export.cpp:125-145); patterns = [(port, [(theta, m0, p0, m90, p90)])]."""
+    out = ["port_index,theta_deg,phi0_mag,phi0_phase_deg,phi90_mag,phi90_phase_deg"]
+    for port, rows in patterns:
+        for t, m0, p0, m90, p90 in rows:
+            out.append(f"{port},{_g(_deg(t))},{_g(m0)},{_g(_deg(p0))},{_g(m90)},{_g(_deg(p90))}")
+    return "\n".join(out) + "\n"
+
+
+def coupling_csv_text(S, positions=None) -> str:
+    """export_coupling_csv with extract_all_couplings: |S| = hypot, dB = 20 log10 max(|S|,
+    1e-20), phase = atan2 in degrees, distance sqrt((dx^2 + dy^2) + dz^2) or 0."""
+    S = np.asarray(S)
+    out = ["port_i,port_j,S_real,S_imag,magnitude_db,phase_deg,distance_m"]
+    for i in range(S.shape[0]):
+        for j in range(S.shape[0]):
+            if i == j:
+                continue
+            re, im = float(S[i, j].real), float(S[i, j].imag)
+            d = 0.0
+            if positions is not None:
+                dx, dy, dz = (float(positions[i][k]) - float(positions[j][k]) for k in range(3))
+                d = math.sqrt((dx * dx + dy * dy) + dz * dz)
+            db = 20.0 * math.log10(max(math.hypot(re, im), 1e-20))
+            out.append(f"{i},{j},{_g(re)},{_g(im)},{_g(db)},{_g(_deg(math.atan2(im, re)))},{_g(d)}")
+    return "\n".join(out) + "\n"
+
+
+def scan_csv_text(scan) -> str:
+    """export_scan_csv (array_export.cpp:167-189); scan = [(theta, phi, gammas, Zs)]."""
+    out = ["theta_deg,phi_deg,element_idx,Gamma_real,Gamma_imag,Z_real,Z_imag,VSWR"]
+    for t, p, gs, zs in scan:
+        for i, (gm, z) in enumerate(zip(gs, zs)):
+            rho = math.hypot(gm.real, gm.imag)
+            vswr = 1e12 if rho >= 1.0 - 1e-12 else (1.0 + rho) / (1.0 - rho)
+            out.append(f"{_g(_deg(t))},{_g(_deg(p))},{i},{_g(gm.real)},{_g(gm.imag)},{_g(z.real)},{_g(z.imag)},{_g(vswr)}")
+    return "\n".join(out) + "\n"

@@ -363,9 +363,10 @@ def test_solver_masks_converged_systems(torch_dev):
 def test_solver_every_spmv_variant_converges_to_same_solution(torch_dev, nrhs):
     """The COCG solve with each fused-SpMV variant pinned (row mappings, LDS tiles, the
     cooperative, pipelined and one-line-per-nonzero gathers, the tile-local column
-    format, and the iteration with p formed in its own pass) on the C1 eigenmode matrix
-    with the auxiliary preconditioner: every variant converges, its true residual
-    |b - A x| / |b| is at the tolerance, and the solutions agree."""
+    format, and the iteration with p formed in its own pass, with the tiled and with the
+    sliced-ELL SpMV) on the C1 eigenmode matrix with the auxiliary preconditioner: every
+    variant converges, its true residual |b - A x| / |b| is at the tolerance, and the
+    solutions agree."""
     import torch
     from edgefem_amd import _capi
     from edgefem_amd import pyedgefem as em
@@ -383,7 +384,7 @@ def test_solver_every_spmv_variant_converges_to_same_solution(torch_dev, nrhs):
     dmask = torch.from_numpy(em._dirichlet_mask(pm, bc).astype(bool)).to("cuda")
     B[dmask] = 0
     ref = None
-    for v in range(10):
+    for v in range(11):
         if v == 5 and nrhs % 2:
             continue
         solver = _capi.KrylovSolver(pm.topology.row_ptr, pm.topology.col_idx, nrhs=nrhs)
