@@ -324,6 +324,73 @@ def run_sweep(args, steps, warmup, prec, cpu=True):
     return result
 
 
+def media_main(args):
+    print(json.dumps(run_media(args, args.steps, args.warmup)))
+
+
+def run_media(args, steps, warmup):
+    """The sweep where K/M reuse cannot serve: the 16x8x32 WR-90 guide with a two-pole
+    Lorentz slab (z 10-20 mm) and a scalar-PML slab (z 30-36 mm, sigma = 0.3 w0), 128
+    frequencies (8.2-12.4 GHz) x 2 excitations = 256 systems in one batched solve (K, M at
+    the median frequency, a per-system correction on the two slabs' edges), against the
+    per-frequency assembly and solve of the reference's sweep (sweep.cpp:330-346) on every
+    16th frequency."""
+    import numpy as np
+    import torch
+    from edgefem_amd import materials as MT
+    from edgefem_amd import meshgen as mg
+    from edgefem_amd import pyedgefem as em
+    torch.cuda.set_device(0)
+    L = 50e-3
+    g = mg.box_mesh(mg.WR90_A, mg.WR90_B, L, 16, 8, 32, seed=1)
+    zc = g.centroids()[:, 2]
+    phys = g.tet_phys.copy()
+    phys[(zc > 10e-3) & (zc < 20e-3)] = 121
+    phys[(zc > 30e-3) & (zc < 36e-3)] = 122
+    mesh = em.mesh_from_arrays(g.node_ids, g.xyz, g.tets, phys, g.tris, g.tri_phys)
+    bc = em.build_edge_pec(mesh, 1)
+    mode = em.solve_te10_mode(em.RectWaveguidePort(mg.WR90_A, mg.WR90_B), 10e9)
+    kc2 = (math.pi / mg.WR90_A) ** 2
+    ports = [em.build_wave_port_2d(mesh, t, mode, bc.dirichlet_edges, kc2) for t in (2, 3)]
+    w0 = 2 * math.pi * 10e9
+    lor = MT.LorentzMaterial(2.0)
+    lor.add_pole(1.5, 1.3 * w0, 0.1 * w0)
+    lor.add_pole(0.5, 0.6 * w0, 0.05 * w0)
+    params = em.MaxwellParams()
+    params.set_eps_model(121, lor)
+    params.pml_sigma, params.pml_regions = 0.3 * w0, {122}
+    freqs = np.linspace(8.2e9, 12.4e9, 128)
+    st = {"graph": bool(args.sweep_graph)}
+    for _ in range(warmup):
+        em.frequency_sweep_eigenmode(mesh, params, bc, ports, freqs, _state=dict(st))
+    torch.cuda.synchronize()
+    ts = time.perf_counter()
+    for _ in range(steps):
+        st = {"graph": bool(args.sweep_graph)}
+        S = em.frequency_sweep_eigenmode(mesh, params, bc, ports, freqs, _state=st)
+    torch.cuda.synchronize()
+    el = (time.perf_counter() - ts) / steps
+    sub = freqs[::16]
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    Sp = em.frequency_sweep_eigenmode(mesh, params, bc, ports, sub, _state={"per_frequency": True})
+    torch.cuda.synchronize()
+    plain = 2 * sub.size / (time.perf_counter() - t0)
+    its = np.concatenate([i["iters"] for i in st["infos"]])
+    nsol = 2 * freqs.size
+    return {
+        "metric": METRIC, "value": nsol / el, "unit": "S-solves/s", "n_gpus": 1, "steps": steps,
+        "warmup": warmup, "ms_per_step": 1e3 * el, "higher_is_better": True, "scaling": "weak",
+        "vs_baseline": None, "dtype": "c128", "data": "synthetic (seeded structured Kuhn mesh)",
+        "config": {"workload": "wr90_16x8x32_lorentz_pml_sweep_128f_x2", "systems": int(nsol),
+                   "edges": mesh.num_edges(), "correction_entries": int(st["correction_entries"]),
+                   "freqs_hz": [float(freqs[0]), float(freqs[-1]), int(freqs.size)], "tol": 1e-10},
+        "per_frequency_solves_per_s": plain, "speedup_vs_per_frequency": nsol / el / plain,
+        "delta_S_vs_per_frequency": float(np.abs(S[::16] - Sp).max()),
+        "iterations": [int(its.min()), int(its.max())],
+    }
+
+
 def fields_main(args):
     """Post-solve field evaluation on the 48x24x192 WR-90 guide (1,327,104 tets, 1,593,480
     edges): E at the centroid and curl E of every tet for 2 solution columns (one step = one
@@ -608,8 +675,9 @@ def traffic_from_pmc(fetch_csv, write_csv, out_path, variant, commands):
 
 def main():
     ap = argparse.ArgumentParser()
-    ap.add_argument("--workload", choices=["c2", "sweep", "adjoint", "cells", "fields"], default="c2",
+    ap.add_argument("--workload", choices=["c2", "sweep", "adjoint", "cells", "fields", "media"], default="c2",
                     help="c2: one large solve (default); sweep: batched 256-system frequency sweep; "
+                         "media: the same sweep with a Lorentz slab and a PML slab (per-system correction); "
                          "adjoint: permittivity gradient over 16 frequencies; cells: 512 periodic unit cells; "
                          "fields: E and curl E of every tet of the C2 guide")
     ap.add_argument("--cells", type=int, default=512, help="cells: number of unit cells")
@@ -658,6 +726,9 @@ def main():
         sys.exit(2)
     if args.workload == "fields":
         fields_main(args)
+        return
+    if args.workload == "media":
+        media_main(args)
         return
     if args.workload == "sweep":
         sweep_main(args)
@@ -786,6 +857,13 @@ def main():
                                                 "grad_over_forward", "iterations")}
         result["adjoint"]["workload"] = ad["config"]["workload"]
         result["adjoint"]["design_tets"] = ad["config"]["design_tets"]
+        torch.cuda.empty_cache()
+        log("extra: batched sweep with frequency-dependent media")
+        md = run_media(args, 1, 1)
+        result["media_sweep"] = {k: md[k] for k in ("value", "unit", "ms_per_step", "per_frequency_solves_per_s",
+                                                    "speedup_vs_per_frequency", "delta_S_vs_per_frequency",
+                                                    "iterations")}
+        result["media_sweep"]["workload"] = md["config"]["workload"]
     print(json.dumps(result))
 
 

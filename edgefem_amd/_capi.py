@@ -36,7 +36,8 @@ EXPORTS = [
     "efm_sweep_set_variants", "efm_sweep_get_variants", "efm_sweep_set_iteration", "efm_tet_fields",
     "efm_gmsh_read", "efm_gmsh_sizes", "efm_gmsh_copy", "efm_gmsh_free",
     "efm_touchstone_write", "efm_touchstone_write_2port", "efm_touchstone_extension",
-    "efm_port_overlap_noalloc", "efm_sweep_set_graph",
+    "efm_port_overlap_noalloc", "efm_sweep_set_graph", "efm_sweep_set_correction",
+    "efm_sweep_set_correction_values",
     "efm_array_package_json", "efm_patterns_csv", "efm_coupling_csv", "efm_scan_csv",
 ]
 
@@ -120,6 +121,8 @@ def load_library(path: str = LIB_PATH) -> C.CDLL:
     lib.efm_sweep_destroy.restype = None
     lib.efm_sweep_set_variants.argtypes = [vp, i32, i32]
     lib.efm_sweep_set_iteration.argtypes = [vp, i32]
+    lib.efm_sweep_set_correction.argtypes = [vp, i64, vp, vp, vp, vp]
+    lib.efm_sweep_set_correction_values.argtypes = [vp, i32, vp]
     lib.efm_tet_fields.argtypes = [vp, vp, i32, vp, vp, vp, vp, vp]
     lib.efm_gmsh_read.argtypes = [C.c_char_p, i32, C.POINTER(vp)]
     lib.efm_gmsh_sizes.argtypes = [vp, vp]
@@ -499,6 +502,31 @@ class SweepSolver:
         """Replay each full convergence-check chunk of iterations from a HIP graph captured
         once per solve (standard forms 0 and 2)."""
         _check(self._lib.efm_sweep_set_graph(self._h, 1 if enable else 0), "efm_sweep_set_graph")
+
+    def set_correction(self, rows, ptr, cols) -> None:
+        """The sub-pattern of a per-system correction C_s (frequency-dependent media):
+        correction row m is matrix row rows[m], with entries ptr[m]..ptr[m+1]-1 in columns
+        cols.  Empty rows remove the correction."""
+        r = np.ascontiguousarray(np.asarray(rows, np.int32))
+        pp = np.ascontiguousarray(np.asarray(ptr, np.int32))
+        cc = np.ascontiguousarray(np.asarray(cols, np.int32))
+        if r.size and pp.size != r.size + 1:
+            raise EdgeFemError("correction ptr must have len(rows) + 1 entries")
+        self._corr_vals = None
+        _check(self._lib.efm_sweep_set_correction(self._h, int(r.size), r.ctypes.data_as(C.c_void_p),
+                                                  pp.ctypes.data_as(C.c_void_p), cc.ctypes.data_as(C.c_void_p),
+                                                  stream_ptr()), "efm_sweep_set_correction")
+
+    def set_correction_values(self, vals) -> None:
+        """Values of C_s: a complex128 device tensor (entries, nsys), kept alive here until
+        replaced; later solves must have nsys systems."""
+        torch = _torch()
+        v = to_dev(vals, torch.complex128)
+        if v.dim() != 2:
+            raise EdgeFemError("correction values must be (entries, nsys)")
+        self._corr_vals = v
+        _check(self._lib.efm_sweep_set_correction_values(self._h, int(v.shape[1]), dptr(v)),
+               "efm_sweep_set_correction_values")
 
     def set_iteration(self, form: int = 2) -> None:
         """Iteration form of the batched COCG: 0 standard, with p = u + beta p in a pass of

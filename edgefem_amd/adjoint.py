@@ -15,6 +15,11 @@ and for a real loss with G = dL/d conj(S) (PyTorch's convention)
 which is evaluated per design tet by ``efm_eps_gradient``.  That kernel recomputes M_e
 from the coordinates rather than storing element blocks.
 
+For a complex port mode (g_j complex) the adjoint right-hand side conj(g_j) differs from the
+excitation's g_j, so lam_j = A^-1 conj(g_j) is solved for: a second batched solve of the same
+systems with those right-hand sides, kept beside the forward solutions; then
+    dL/d eps_e = sum_{f,j,a} Re(conj(G_fja) k0_f^2 lam_fj^T M_e x_fa).
+
 beta_i depends on the permittivity of the first tet with a face on port i, and it is held
 fixed here, so such tets may not be design tets (checked).
 """
@@ -52,9 +57,12 @@ class EpsDesignProblem:
             t = em._port_face_tet(mesh, port.surface_tag)
             if t >= 0 and np.any(self.design == t):
                 raise ValueError(f"tet {t} sets beta of port {port.surface_tag}; it cannot be a design tet")
+        # complex port modes: separate adjoint solves (A lam = conj(g))
+        self.complex_modes = False
+        for port in self.ports:
             w = np.asarray(port.weights)
             if np.iscomplexobj(w) and np.abs(w.imag).max(initial=0.0) > 0:
-                raise NotImplementedError("complex port modes need separate adjoint solves")
+                self.complex_modes = True
         self.solver_options = solver_options
         P = max(1, len(self.ports))
         self.max_batch = max(P, (int(max_batch) // P) * P)   # a frequency's excitations share a batch
@@ -70,7 +78,7 @@ class EpsDesignProblem:
         """S (F, P, P) and the kept batched solutions."""
         p = copy.copy(self.params)
         p.eps_r_tet = self.permittivity(eps_design)
-        st = {"keep_solutions": True}
+        st = {"keep_solutions": True, "adjoint_conj": self.complex_modes}
         S = em.frequency_sweep_eigenmode(self.mesh, p, self.bc, self.ports, self.freqs,
                                          solver_options=self.solver_options, max_batch=self.max_batch, _state=st)
         self.last_infos = st.get("infos", [])
@@ -91,13 +99,16 @@ class EpsDesignProblem:
                 for (f2, j), cj in cols.items():
                     if f2 != f or G[f, j, a] == 0:
                         continue
-                    cjf = complex(sol["coef"][cj][j])
                     tl.append(cj)
                     tr.append(ca)
-                    w.append(np.conj(G[f, j, a]) * k0sq / (2.0 * cjf))
+                    if sol.get("L") is not None:      # lam_j solved for (complex mode)
+                        w.append(np.conj(G[f, j, a]) * k0sq)
+                    else:                              # lam_j = x_j / (2 c_j) (real mode)
+                        w.append(np.conj(G[f, j, a]) * k0sq / (2.0 * complex(sol["coef"][cj][j])))
             if not w:
                 continue
-            g = _capi.eps_gradient(topo, self.mesh.dev_xyz, self.design, sol["X"], sol["X"], tl, tr, w)
+            XL = sol["L"] if sol.get("L") is not None else sol["X"]
+            g = _capi.eps_gradient(topo, self.mesh.dev_xyz, self.design, XL, sol["X"], tl, tr, w)
             grad = g if grad is None else grad + g
         if grad is None:
             grad = torch.zeros(self.design.size, dtype=torch.float64, device=topo.row_ptr.device)

@@ -93,6 +93,38 @@ __device__ inline void st_stream(double2 *p, double2 v) {
   __builtin_nontemporal_store(w, reinterpret_cast<efm_v2d *>(p));
 }
 
+// A double moved between the lanes of each quad by a DPP quad permutation (a VALU move,
+// no LDS traffic).  CTRL 0x4E reads lane ^ 2, 0xB1 reads lane ^ 1.  Every lane of the quad
+// must be active.
+template <int CTRL>
+__device__ inline double dpp_quad(double v) {
+  const int lo = __builtin_amdgcn_update_dpp(0, __double2loint(v), CTRL, 0xF, 0xF, false);
+  const int hi = __builtin_amdgcn_update_dpp(0, __double2hiint(v), CTRL, 0xF, 0xF, false);
+  return __hiloint2double(hi, lo);
+}
+
+// Butterfly sum over aligned groups of W lanes (W a power of two <= 64), offsets W/2 down to
+// 1 in that order, so every lane of a group ends with the same sum.  The last two steps
+// (offsets 2 and 1) are DPP quad permutations; wider offsets go through __shfl_xor.  Every
+// lane of a group must be active.
+template <int W>
+__device__ inline void group_sum(double2 &v) {
+  static_assert(W >= 1 && W <= 64 && (W & (W - 1)) == 0, "group width");
+#pragma unroll
+  for (int off = W / 2; off >= 4; off >>= 1) {
+    v.x += __shfl_xor(v.x, off, W);
+    v.y += __shfl_xor(v.y, off, W);
+  }
+  if constexpr (W >= 4) {
+    v.x += dpp_quad<0x4E>(v.x);
+    v.y += dpp_quad<0x4E>(v.y);
+  }
+  if constexpr (W >= 2) {
+    v.x += dpp_quad<0xB1>(v.x);
+    v.y += dpp_quad<0xB1>(v.y);
+  }
+}
+
 // --------------------------------------------------------------------------
 // Fixed tables of the reference
 // --------------------------------------------------------------------------

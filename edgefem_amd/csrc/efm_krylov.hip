@@ -307,13 +307,8 @@ __global__ void __launch_bounds__(kBlock)
       }
     }
 #pragma unroll
-    for (int s = 0; s < R; ++s) {
-#pragma unroll
-      for (int off = LPR / 2; off > 0; off >>= 1) {
-        acc[s].x += __shfl_xor(acc[s].x, off, LPR);
-        acc[s].y += __shfl_xor(acc[s].y, off, LPR);
-      }
-    }
+    for (int s = 0; s < R; ++s)
+      group_sum<LPR>(acc[s]);
     if (row < n && lane == 0) {
       const double2 *w = Wa + row * 2 * R;
       double2 *wb = Wb + row * 2 * R;
@@ -421,13 +416,8 @@ __global__ void __launch_bounds__(kBlock)
       }
     }
 #pragma unroll
-    for (int s = 0; s < R; ++s) {
-#pragma unroll
-      for (int off = RL / 2; off > 0; off >>= 1) {
-        acc[s].x += __shfl_xor(acc[s].x, off, RL);
-        acc[s].y += __shfl_xor(acc[s].y, off, RL);
-      }
-    }
+    for (int s = 0; s < R; ++s)
+      group_sum<RL>(acc[s]);
     if (row < r1 && l == 0) {
       const double2 *w = Wa + (int64_t)row * 2 * R;
       double2 *wb = Wb + (int64_t)row * 2 * R;
@@ -563,11 +553,7 @@ __global__ void __launch_bounds__(kBlock)
         acc[s] = cadd(acc[s], prod[k * R + s]);
 #pragma unroll
     for (int s = 0; s < R; ++s)
-#pragma unroll
-      for (int off = RL / 2; off > 0; off >>= 1) {
-        acc[s].x += __shfl_xor(acc[s].x, off, RL);
-        acc[s].y += __shfl_xor(acc[s].y, off, RL);
-      }
+      group_sum<RL>(acc[s]);
     if (row < cur.d.y && l == 0) {
       double2 *wb = Wb + (int64_t)row * 2 * R;
       // the whole record is written (z = 0; the update overwrites it) so that no partial
@@ -674,13 +660,8 @@ __global__ void __launch_bounds__(kBlock)
       }
     }
 #pragma unroll
-    for (int s = 0; s < R; ++s) {
-#pragma unroll
-      for (int off = RL / 2; off > 0; off >>= 1) {
-        acc[s].x += __shfl_xor(acc[s].x, off, RL);
-        acc[s].y += __shfl_xor(acc[s].y, off, RL);
-      }
-    }
+    for (int s = 0; s < R; ++s)
+      group_sum<RL>(acc[s]);
     if (row < r1 && l == 0) {
       const double2 *w = Wa + (int64_t)row * 2 * R;
       double2 *wb = Wb + (int64_t)row * 2 * R;
@@ -910,11 +891,7 @@ __global__ void __launch_bounds__(kBlock)
     }
 #pragma unroll
     for (int s = 0; s < R; ++s)
-#pragma unroll
-      for (int off = RL / 2; off > 0; off >>= 1) {
-        acc[s].x += __shfl_xor(acc[s].x, off, RL);
-        acc[s].y += __shfl_xor(acc[s].y, off, RL);
-      }
+      group_sum<RL>(acc[s]);
     if (row < d.y && l == 0) {
       double2 *wb = Wb + (int64_t)row * 2 * R;
       double2 rec[2 * R];
@@ -1041,13 +1018,8 @@ __global__ void __launch_bounds__(kBlock)
       }
     }
 #pragma unroll
-    for (int s = 0; s < R; ++s) {
-#pragma unroll
-      for (int off = RL / 2; off > 0; off >>= 1) {
-        acc[s].x += __shfl_xor(acc[s].x, off, RL);
-        acc[s].y += __shfl_xor(acc[s].y, off, RL);
-      }
-    }
+    for (int s = 0; s < R; ++s)
+      group_sum<RL>(acc[s]);
     if (row < r1 && l == 0) {
       const double2 *w = Wa + (int64_t)row * 2 * R;
       double2 *wb = Wb + (int64_t)row * 2 * R;
@@ -1239,13 +1211,8 @@ __global__ void __launch_bounds__(kBlock)
       }
     }
 #pragma unroll
-    for (int s = 0; s < R; ++s) {
-#pragma unroll
-      for (int off = kLanesPerRow / 2; off > 0; off >>= 1) {
-        acc[s].x += __shfl_xor(acc[s].x, off, kLanesPerRow);
-        acc[s].y += __shfl_xor(acc[s].y, off, kLanesPerRow);
-      }
-    }
+    for (int s = 0; s < R; ++s)
+      group_sum<kLanesPerRow>(acc[s]);
     if (row < n && lane == 0) {
 #pragma unroll
       for (int s = 0; s < R; ++s) {
@@ -1366,13 +1333,8 @@ __global__ void __launch_bounds__(kBlock)
       }
     }
 #pragma unroll
-    for (int s = 0; s < R; ++s) {
-#pragma unroll
-      for (int off = kLanesPerRow / 2; off > 0; off >>= 1) {
-        acc[s].x += __shfl_xor(acc[s].x, off, kLanesPerRow);
-        acc[s].y += __shfl_xor(acc[s].y, off, kLanesPerRow);
-      }
-    }
+    for (int s = 0; s < R; ++s)
+      group_sum<kLanesPerRow>(acc[s]);
     if (row < n && lane == 0) {
 #pragma unroll
       for (int s = 0; s < R; ++s)
@@ -1699,13 +1661,8 @@ __global__ void __launch_bounds__(kBlock)
       }
     }
 #pragma unroll
-    for (int s = 0; s < R; ++s) {
-#pragma unroll
-      for (int off = kLanesPerNode / 2; off > 0; off >>= 1) {
-        acc[s].x += __shfl_xor(acc[s].x, off, kLanesPerNode);
-        acc[s].y += __shfl_xor(acc[s].y, off, kLanesPerNode);
-      }
-    }
+    for (int s = 0; s < R; ++s)
+      group_sum<kLanesPerNode>(acc[s]);
     if (nd < N && lane == 0) {
       const double2 di = dninv[nd];
 #pragma unroll
@@ -1843,7 +1800,9 @@ __global__ void __launch_bounds__(kBlock)
     k_cocgz_spmv(int ntiles, int xcd_span, const int4 *__restrict__ desc, const int32_t *__restrict__ row_ptr,
                  const int32_t *__restrict__ col, const VT *__restrict__ val, const double2 *__restrict__ pf,
                  double2 *__restrict__ q, double2 *__restrict__ partial, const SysState *__restrict__ st) {
-  __shared__ double2 prod[TNNZ * R];
+  // Products in one plane per system: a wavefront's writes are contiguous 16 B runs (no
+  // bank conflicts), and a row's lanes read consecutive 16 B slots.
+  __shared__ double2 prod[R * TNNZ];
   __shared__ double2 lds[(kBlock / 64) * R];
   constexpr int J = TNNZ / kBlock;
   bool act[R];
@@ -1913,7 +1872,7 @@ __global__ void __launch_bounds__(kBlock)
       const int32_t lk = threadIdx.x + j * kBlock;
 #pragma unroll
       for (int s = 0; s < R; ++s)
-        prod[lk * R + s] = bil(cur.a[j], zg[j][s]);
+        prod[s * TNNZ + lk] = bil(cur.a[j], zg[j][s]);
     }
     __syncthreads();
     double2 acc[R];
@@ -1923,14 +1882,10 @@ __global__ void __launch_bounds__(kBlock)
     for (int32_t k = cur.k0 + l; k < cur.k1; k += RL)
 #pragma unroll
       for (int s = 0; s < R; ++s)
-        acc[s] = cadd(acc[s], prod[k * R + s]);
+        acc[s] = cadd(acc[s], prod[s * TNNZ + k]);
 #pragma unroll
     for (int s = 0; s < R; ++s)
-#pragma unroll
-      for (int off = RL / 2; off > 0; off >>= 1) {
-        acc[s].x += __shfl_xor(acc[s].x, off, RL);
-        acc[s].y += __shfl_xor(acc[s].y, off, RL);
-      }
+      group_sum<RL>(acc[s]);
     if (row < cur.d.y && l == 0) {
 #pragma unroll
       for (int s = 0; s < R; ++s) {
@@ -2091,11 +2046,7 @@ __global__ void __launch_bounds__(kBlock)
     }
 #pragma unroll
     for (int s = 0; s < R; ++s)
-#pragma unroll
-      for (int off = RL / 2; off > 0; off >>= 1) {
-        acc[s].x += __shfl_xor(acc[s].x, off, RL);
-        acc[s].y += __shfl_xor(acc[s].y, off, RL);
-      }
+      group_sum<RL>(acc[s]);
     if (row >= 0 && (lane % RL) == 0) {
 #pragma unroll
       for (int s = 0; s < R; ++s) {

@@ -1186,6 +1186,51 @@ __global__ void k_sw_imag_flags(int64_t nnz, const double2 *__restrict__ K, cons
     atomicOr(flags, f);
 }
 
+// Per-system correction of frequency-dependent media (PML stretch, dispersive models),
+//   A_s = K - k0_s^2 M + sum_p c_{s,p} M_s,p + C_s,
+// with C_s on a fixed sub-pattern (correction row m is matrix row crow[m], entries
+// cptr[m]..cptr[m+1] with columns ccol) and values cval[entry][nsys] per system.
+// MODE 0 (after the SpMV, systems with active set): w += C_s u on the correction rows,
+// partials of u^T C_s u per block (added to the SpMV's partials by the finalize).
+// MODE 1 (before a residual, systems with out_active set): t[m] = C_s x for FP64 x.
+// One wavefront per correction row and group of 64 systems (lane = system).
+template <typename VT, typename IT, int MODE>
+__global__ void __launch_bounds__(kBlock)
+    k_sw_corr(int64_t nrows, int B, int nsys, const int32_t *__restrict__ crow,
+              const int32_t *__restrict__ cptr, const int32_t *__restrict__ ccol,
+              const double2 *__restrict__ cval, const IT *__restrict__ u, VT *__restrict__ w,
+              double2 *__restrict__ t, double2 *__restrict__ partial, const SwState *__restrict__ st) {
+  __shared__ double2 lds[kWaves * 64];
+  const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
+  const int g = blockIdx.y;
+  const int s = g * kSys + lane;
+  double2 v[1] = {make_double2(0, 0)};
+  const bool act = s < B && (MODE == 0 ? st[s].active : st[s].out_active) != 0;
+  const bool has = s < nsys;
+  if (__ballot(act) != 0) {
+    for (int64_t m = (int64_t)blockIdx.x * kWaves + wv; m < nrows; m += (int64_t)gridDim.x * kWaves) {
+      const int32_t k0 = cptr[m], k1 = cptr[m + 1];
+      double2 acc = make_double2(0, 0);
+#pragma unroll 4
+      for (int32_t k = k0; k < k1; ++k) {
+        const double2 a = has ? cval[(int64_t)k * nsys + s] : make_double2(0, 0);
+        acc = cfma(a, ldv(u + (int64_t)ccol[k] * B + s), acc);
+      }
+      if (act) {
+        const int64_t i = (int64_t)crow[m] * B + s;
+        if (MODE == 0) {
+          stv(w + i, cadd(ldv(w + i), acc));
+          v[0] = cadd(v[0], bil(ldv(u + i), acc));
+        } else {
+          t[m * B + s] = acc;
+        }
+      }
+    }
+  }
+  if (MODE == 0)
+    write_sys_partials<1>(v, lds, partial, g * gridDim.x + blockIdx.x);
+}
+
 // Residual replacement: r = b - A_s x in FP64 (x == nullptr: r = b), stored in the Krylov
 // vector type; partials of |r|^2 (before rounding).  Systems without out_active are skipped.
 template <typename VT>
@@ -1196,7 +1241,8 @@ __global__ void __launch_bounds__(kBlock)
                   const int8_t *__restrict__ rport, const double *__restrict__ k0sq,
                   const double2 *__restrict__ coef, const double2 *__restrict__ b,
                   const double2 *__restrict__ x, VT *__restrict__ r, double2 *__restrict__ partial,
-                  const SwState *__restrict__ st) {
+                  const SwState *__restrict__ st, const int32_t *__restrict__ cmap,
+                  const double2 *__restrict__ tc) {
   __shared__ double2 lds[kWaves * 64];
   const int lane = threadIdx.x & 63;
   const int wv = threadIdx.x >> 6;
@@ -1213,8 +1259,14 @@ __global__ void __launch_bounds__(kBlock)
         if (row >= n)
           continue;
         double2 rv = b[row * B + s];
-        if (x)
+        if (x) {
           rv = csub(rv, sw_row_apply(row, s, B, P, k2, row_ptr, col, K, M, Ms, rport, coef, x));
+          if (cmap) {  // the per-system correction's C_s x on its rows (k_sw_corr, MODE 1)
+            const int32_t m = cmap[row];
+            if (m >= 0)
+              rv = csub(rv, tc[(int64_t)m * B + s]);
+          }
+        }
         if (act) {
           stv(r + row * B + s, rv);
           v[0].x += cabs2(rv);
@@ -1400,8 +1452,9 @@ __global__ void __launch_bounds__(kBlock)
 // systems that took no part get their saved activity back.
 template <int STAGE>
 __global__ void __launch_bounds__(kBlock)
-    k_sw_fin_std(int B, int gx, const double2 *__restrict__ part, SwState *__restrict__ st,
-                 int *__restrict__ active_flag, double tol2, int max_it, int max_inner, double ru_delta2) {
+    k_sw_fin_std(int B, int gx, const double2 *__restrict__ part, int gx2, const double2 *__restrict__ part2,
+                 SwState *__restrict__ st, int *__restrict__ active_flag, double tol2, int max_it, int max_inner,
+                 double ru_delta2) {
   const int s = fin_system();
   if (s >= B)
     return;
@@ -1417,6 +1470,9 @@ __global__ void __launch_bounds__(kBlock)
     for (int k = 0; k < V; ++k)
       v[k] = cadd(v[k], part[(bl * V + k) * 64 + lane]);
   }
+  if (STAGE == 1)  // the per-system correction's partials of u^T C_s u (gx2 blocks per group)
+    for (int bx = threadIdx.x; bx < gx2; bx += kBlock)
+      v[0] = cadd(v[0], part2[((int64_t)gy * gx2 + bx) * 64 + lane]);
   __shared__ double2 sh[V][kBlock];
   block_sum<V>(v, sh);
   if (threadIdx.x != 0)
@@ -1651,6 +1707,12 @@ struct efm_sweep {
   efm::DevBuf x64;  // reliable updates: FP64 solution (allocated on first use)
   efm::DevBuf dninv;  // [N][Bmax] inverse nodal diagonals of the systems of one solve
   efm::DevBuf tucol, tnu, tlidx, tdloc;  // row-tile format of the LDS-tiled SpMV
+  // per-system correction C_s of frequency-dependent media (efm_sweep_set_correction):
+  // c_rows rows of a fixed sub-pattern, values [c_nnz][c_nsys] in caller memory
+  int64_t c_rows = 0, c_nnz = 0;
+  int c_nsys = 0, gx_corr = 8, ncu = 256;
+  const double2 *c_vals = nullptr;
+  efm::DevBuf c_row, c_ptr, c_col, c_map, part_c, tcorr;
   ~efm_sweep() {
     if (capture_stream)
       (void)hipStreamDestroy(capture_stream);
@@ -1702,6 +1764,7 @@ extern "C" int efm_sweep_create(const efm_topology *topo, const double *K_val, c
     if (hipGetDeviceProperties(&prop, dev) == hipSuccess && prop.multiProcessorCount > 0)
       ncu = prop.multiProcessorCount;
   }
+  s->ncu = ncu;
   const int groups = s->Bmax / kSys;
   // blocks per system group: enough waves to fill the chip, a multiple of 8 (XCD walk)
   auto gx_for = [&](int64_t items) {
@@ -1837,6 +1900,62 @@ extern "C" int efm_sweep_set_variants(efm_sweep *s, int spmv, int aux) {
 extern "C" int efm_sweep_set_graph(efm_sweep *s, int enable) {
   EFM_REQUIRE(s, "efm_sweep_set_graph: NULL sweep");
   s->use_graph = enable ? 1 : 0;
+  return EFM_OK;
+}
+
+extern "C" int efm_sweep_set_correction(efm_sweep *s, int64_t n_rows, const int32_t *h_rows,
+                                        const int32_t *h_ptr, const int32_t *h_cols, void *stream) {
+  EFM_REQUIRE(s, "efm_sweep_set_correction: NULL sweep");
+  EFM_REQUIRE(n_rows >= 0 && n_rows <= s->n, "efm_sweep_set_correction: n_rows out of range");
+  s->c_rows = 0;
+  s->c_nnz = 0;
+  s->c_vals = nullptr;
+  s->c_nsys = 0;
+  if (n_rows == 0)
+    return EFM_OK;
+  EFM_REQUIRE(h_rows && h_ptr && h_cols, "efm_sweep_set_correction: NULL argument");
+  EFM_REQUIRE(h_ptr[0] == 0, "efm_sweep_set_correction: ptr[0] must be 0");
+  std::vector<int32_t> map(s->n, -1);
+  for (int64_t m = 0; m < n_rows; ++m) {
+    EFM_REQUIRE(h_ptr[m + 1] >= h_ptr[m], "efm_sweep_set_correction: ptr not monotone");
+    EFM_REQUIRE(h_rows[m] >= 0 && h_rows[m] < s->n, "efm_sweep_set_correction: row out of range");
+    EFM_REQUIRE(map[h_rows[m]] < 0, "efm_sweep_set_correction: a row appears twice");
+    map[h_rows[m]] = (int32_t)m;
+  }
+  const int64_t nz = h_ptr[n_rows];
+  for (int64_t k = 0; k < nz; ++k)
+    EFM_REQUIRE(h_cols[k] >= 0 && h_cols[k] < s->n, "efm_sweep_set_correction: column out of range");
+  hipStream_t st = (hipStream_t)stream;
+  const int groups = s->Bmax / kSys;
+  const int64_t need = (n_rows + kWaves - 1) / kWaves;
+  const int64_t want = std::max<int64_t>(8, (int64_t)s->ncu * 8 / groups);
+  const int gx = (int)std::min(want, std::max<int64_t>(need, 1));
+  int rc;
+  if ((rc = s->c_row.alloc(n_rows * sizeof(int32_t))) || (rc = s->c_ptr.alloc((n_rows + 1) * sizeof(int32_t))) ||
+      (rc = s->c_col.alloc(std::max<int64_t>(nz, 1) * sizeof(int32_t))) ||
+      (rc = s->c_map.alloc(s->n * sizeof(int32_t))) ||
+      (rc = s->part_c.alloc((size_t)groups * gx * 64 * sizeof(double2))) ||
+      (rc = s->tcorr.alloc((size_t)n_rows * s->Bmax * sizeof(double2))))
+    return rc;
+  EFM_HIP_CHECK(hipMemcpyAsync(s->c_row.p, h_rows, n_rows * sizeof(int32_t), hipMemcpyHostToDevice, st));
+  EFM_HIP_CHECK(hipMemcpyAsync(s->c_ptr.p, h_ptr, (n_rows + 1) * sizeof(int32_t), hipMemcpyHostToDevice, st));
+  if (nz > 0)
+    EFM_HIP_CHECK(hipMemcpyAsync(s->c_col.p, h_cols, nz * sizeof(int32_t), hipMemcpyHostToDevice, st));
+  EFM_HIP_CHECK(hipMemcpyAsync(s->c_map.p, map.data(), s->n * sizeof(int32_t), hipMemcpyHostToDevice, st));
+  EFM_HIP_CHECK(hipStreamSynchronize(st));
+  s->c_rows = n_rows;
+  s->c_nnz = nz;
+  s->gx_corr = gx;
+  return EFM_OK;
+}
+
+extern "C" int efm_sweep_set_correction_values(efm_sweep *s, int nsys, const double *d_vals) {
+  EFM_REQUIRE(s, "efm_sweep_set_correction_values: NULL sweep");
+  EFM_REQUIRE(s->c_rows > 0, "efm_sweep_set_correction_values: no correction pattern set");
+  EFM_REQUIRE(nsys >= 1 && nsys <= s->Bmax, "efm_sweep_set_correction_values: nsys out of range");
+  EFM_REQUIRE(d_vals || s->c_nnz == 0, "efm_sweep_set_correction_values: NULL values");
+  s->c_vals = (const double2 *)d_vals;
+  s->c_nsys = nsys;
   return EFM_OK;
 }
 
@@ -2020,8 +2139,14 @@ struct SwRun {
                          s->row_ptr, s->col, s->K.as<double2>(), s->M.as<double2>(), s->Ms.as<double>(),
                          s->rport.as<int8_t>(), s->k0sq.as<double>(), s->coef.as<double2>(), u, w, ps, S);
     EFM_LAUNCH_CHECK();
-    return EFM_OK;
+    if (s->c_rows > 0) {
+      hipLaunchKernelGGL((k_sw_corr<VT, VT, 0>), dim3(s->gx_corr, groups), dim3(kBlock), 0, st, s->c_rows, Bp,
+                         s->c_nsys, s->c_row.as<int32_t>(), s->c_ptr.as<int32_t>(), s->c_col.as<int32_t>(),
+                         s->c_vals, u, w, (double2 *)nullptr, s->part_c.as<double2>(), S);
+      EFM_LAUNCH_CHECK();
     }
+    return EFM_OK;
+  }
 
   // y = Dn^{-1} G^T v with nodal variant `variant`, NV gathered vectors (see k_sw_aux_node)
   template <typename VT, int NV> int aux_launch(int variant, const VT *r, const VT *w, const VT *sv, VT *y) {
@@ -2161,10 +2286,19 @@ struct SwRun {
     EFM_LAUNCH_CHECK();
     hipLaunchKernelGGL((k_sw_accumulate<VT>), dim3(s->gx_vec, groups), dim3(kBlock), 0, st, n, Bp, x, x64, S);
     EFM_LAUNCH_CHECK();
+    const bool corr = s->c_rows > 0;
+    if (corr) {
+      hipLaunchKernelGGL((k_sw_corr<VT, double2, 1>), dim3(s->gx_corr, groups), dim3(kBlock), 0, st, s->c_rows, Bp,
+                         s->c_nsys, s->c_row.as<int32_t>(), s->c_ptr.as<int32_t>(), s->c_col.as<int32_t>(),
+                         s->c_vals, (const double2 *)x64, (VT *)nullptr, s->tcorr.as<double2>(),
+                         (double2 *)nullptr, S);
+      EFM_LAUNCH_CHECK();
+    }
     hipLaunchKernelGGL((k_sw_residual<VT>), dim3(s->gx_spmv, s->gy_spmv), dim3(kBlock), 0, st, n, Bp, P, groups,
                        s->row_ptr, s->col, s->K.as<double2>(), s->M.as<double2>(), s->Ms.as<double>(),
                        s->rport.as<int8_t>(), s->k0sq.as<double>(), s->coef.as<double2>(), b,
-                       (const double2 *)x64, r, ps, S);
+                       (const double2 *)x64, r, ps, S, corr ? s->c_map.as<int32_t>() : (const int32_t *)nullptr,
+                       corr ? (const double2 *)s->tcorr.as<double2>() : (const double2 *)nullptr);
     EFM_LAUNCH_CHECK();
     if (all) {
       hipLaunchKernelGGL(k_sw_true_resid, dim3(Bp), dim3(kBlock), 0, st, Bp, s->gx_spmv, ps, S);
@@ -2179,8 +2313,8 @@ struct SwRun {
                        s->rport.as<int8_t>(), s->k0sq.as<double>(), s->coef.as<double2>(), y, w, u, p, sv, x, r,
                        pu, S);
     EFM_LAUNCH_CHECK();
-    hipLaunchKernelGGL((k_sw_fin_std<3>), dim3(Bp), dim3(kBlock), 0, st, Bp, s->gx_upd[vti][0], pu, S, flags,
-                       tol2, max_it, max_it, ru_delta2);
+    hipLaunchKernelGGL((k_sw_fin_std<3>), dim3(Bp), dim3(kBlock), 0, st, Bp, s->gx_upd[vti][0], pu, 0,
+                       (const double2 *)nullptr, S, flags, tol2, max_it, max_it, ru_delta2);
     EFM_LAUNCH_CHECK();
     return EFM_OK;
   }
@@ -2210,8 +2344,10 @@ struct SwRun {
     EFM_LAUNCH_CHECK();
     auto fin = [&](auto stage, int gx, const double2 *part) -> int {
       constexpr int STAGE = decltype(stage)::value;
-      hipLaunchKernelGGL((k_sw_fin_std<STAGE>), dim3(Bp), dim3(kBlock), 0, st, Bp, gx, part, S, flags, inner_tol2,
-                         max_it, max_inner, ru_delta2);
+      const bool corr = STAGE == 1 && s->c_rows > 0;
+      hipLaunchKernelGGL((k_sw_fin_std<STAGE>), dim3(Bp), dim3(kBlock), 0, st, Bp, gx, part, corr ? s->gx_corr : 0,
+                         corr ? (const double2 *)s->part_c.as<double2>() : (const double2 *)nullptr, S, flags,
+                         inner_tol2, max_it, max_inner, ru_delta2);
       EFM_LAUNCH_CHECK();
       return EFM_OK;
     };
@@ -2344,6 +2480,10 @@ extern "C" int efm_sweep_solve(efm_sweep *s, int nsys, const double *h_k0sq, con
               "efm_sweep_solve: replace_factor must lie in (0, 1)");
   EFM_REQUIRE(!h_refine || s->form != 1,
               "efm_sweep_solve: reliable updates need the standard COCG form (0 or 2)");
+  EFM_REQUIRE(s->c_rows == 0 || s->form != 1,
+              "efm_sweep_solve: a per-system correction needs the standard COCG form (0 or 2)");
+  EFM_REQUIRE(s->c_rows == 0 || ((s->c_vals || s->c_nnz == 0) && s->c_nsys == nsys),
+              "efm_sweep_solve: correction values missing or set for another number of systems");
   hipStream_t st = (hipStream_t)stream;
   const int Bp = ((nsys + kSys - 1) / kSys) * kSys;  // padded system count this call
   const int P = s->P;

@@ -1569,8 +1569,9 @@ def solve_port_eigens(mesh: SurfaceMesh2D, num_modes: int, omega: float, eps_r=1
     mode as a spurious first TE mode on millimetre meshes); beta = sqrt(k^2 -
     kc^2); Z0 = omega mu / beta (TE) or beta / (omega eps) (TM); field scaled to unit
     real power.  The eigenvector sign is not the reference's (Eigen picks it; it cannot
-    be reproduced): each mode is made real with its largest entry positive.  Within a
-    degenerate pair (a square port's TE10/TE01, where the reference returns whatever
+    be reproduced): each mode is made real with its largest entry positive, entries
+    within 1e-8 of the largest magnitude counting as tied and the first in node order
+    deciding (a symmetric face has equal and opposite extremes).  Within a degenerate pair (a square port's TE10/TE01, where the reference returns whatever
     member of the plane Eigen gives) the mode is the one whose field varies least along
     the second surface axis, i.e. E along that axis."""
     import scipy.linalg
@@ -1630,7 +1631,11 @@ def solve_port_eigens(mesh: SurfaceMesh2D, num_modes: int, omega: float, eps_r=1
         else:
             mode.Z0 = 0j if beta == 0 else beta / (omega * eps)
         v = V[:, i].astype(np.complex128)
-        j = int(np.argmax(np.abs(v)))
+        # sign: the largest entry positive; entries within 1e-8 of the largest magnitude
+        # count as tied (a symmetric port face gives equal and opposite extremes, which
+        # only roundoff would otherwise order) and the first of them in node order decides
+        av = np.abs(v)
+        j = int(np.nonzero(av >= (1.0 - 1e-8) * av.max())[0][0])
         v = v * (1.0 if v[j].real >= 0 else -1.0)
         field = np.zeros(N, np.complex128)
         field[~pec] = v
@@ -2096,6 +2101,84 @@ def check_passivity(S) -> bool:
 _SWEEP_MAX_PORTS = 4   # port masses per system in the batched sweep kernels
 
 
+def _frequency_dependent_tets(mesh: Mesh, params: MaxwellParams) -> np.ndarray:
+    """Tets whose assembly factors change with the frequency: the PML regions (scalar and
+    tensor) and the tags with a dispersive model (unless a per-tet array overrides them)."""
+    tags = {int(t) for t in params.pml_regions} | {int(t) for t in params.pml_tensor_regions}
+    if params.eps_r_tet is None:
+        tags |= {int(t) for t, m in params.eps_models.items() if m is not None}
+    if params.mu_r_tet is None:
+        tags |= {int(t) for t, m in params.mu_models.items() if m is not None}
+    if not tags:
+        return np.zeros(0, np.int64)
+    return np.nonzero(np.isin(mesh.tet_phys, sorted(tags)))[0]
+
+
+class _MediaCorrection:
+    """The per-system correction of the batched sweep for PML and dispersive media.  With
+    K = sum_t K_t / (mu s)_t(w_ref) and M = sum_t (eps s)_t(w_ref) M_t at a reference
+    frequency, the system at w is A(w) = K - k0^2 M + ports + C(w) with
+      C(w) = sum over the frequency-dependent tets t of
+             (1/(mu s)_t(w) - 1/(mu s)_t(w_ref)) K_t - k0^2 ((eps s)_t(w) - (eps s)_t(w_ref)) M_t,
+    i.e. what the reference's per-frequency assembly (assemble_maxwell.cpp:165-199,
+    sweep.cpp:330-346) changes between w_ref and w.  The sub-pattern is every (row, column)
+    pair of edges of those tets, Dirichlet rows and columns left out (their rows stay the
+    identity).  Values are assembled on the GPU by the volume assembly with the material
+    differences on those tets and zero elsewhere, then gathered onto the sub-pattern."""
+
+    def __init__(self, mesh: Mesh, params: MaxwellParams, dmask, es_ref, ms_ref):
+        import torch
+        self.mesh, self.params, self.dmask = mesh, params, dmask
+        self.tets = _frequency_dependent_tets(mesh, params)
+        self.es_ref, self.ims_ref = np.asarray(es_ref), 1.0 / np.asarray(ms_ref)
+        n = mesh.num_edges()
+        te = mesh.tet_edges[self.tets].astype(np.int64)
+        r = np.repeat(te, 6, axis=1).ravel()
+        c = np.tile(te, (1, 6)).ravel()
+        dm = np.asarray(dmask)
+        keep = (dm[r] == 0) & (dm[c] == 0)
+        key = np.unique(r[keep] * n + c[keep])
+        rr, cc = key // n, key % n
+        self.rows, starts = np.unique(rr, return_index=True)
+        self.ptr = np.append(starts, key.size).astype(np.int32)
+        self.rows = self.rows.astype(np.int32)
+        self.cols = cc.astype(np.int32)
+        # positions of the sub-pattern entries in the CSR pattern (columns sorted per row)
+        topo = mesh.topology
+        rp = topo.row_ptr.cpu().numpy().astype(np.int64)
+        ci = topo.col_idx.cpu().numpy().astype(np.int64)
+        full = np.repeat(np.arange(n, dtype=np.int64), np.diff(rp)) * n + ci
+        pos = np.searchsorted(full, key)
+        if key.size and (pos.max() >= full.size or np.any(full[pos] != key)):
+            raise RuntimeError("media correction: an entry is missing from the matrix pattern")
+        self.pos = torch.from_numpy(pos).to(topo.row_ptr.device)
+        self.nnz = int(key.size)
+        self._cache = {}
+
+    def _entries(self, omega: float):
+        """C(omega) without the k0^2 factor on its M part: (K part, M part) on the sub-pattern."""
+        key = float(omega)
+        if key not in self._cache:
+            es, ms, _ = _assembly_materials(self.mesh, _params_at(self.params, omega))
+            T = self.mesh.num_tets()
+            de = np.zeros(T, np.complex128)
+            dk = np.zeros(T, np.complex128)
+            de[self.tets] = es[self.tets] - self.es_ref[self.tets]
+            dk[self.tets] = 1.0 / ms[self.tets] - self.ims_ref[self.tets]
+            Kd, Md = _capi.assemble_km(self.mesh.topology, self.mesh.dev_xyz, de, dk, self.dmask)
+            self._cache = {key: (Kd[self.pos], Md[self.pos])}   # the last frequency only
+        return self._cache[key]
+
+    def values(self, freqs, freq_index, k0sq):
+        """(entries, nsys) complex128 device values for systems at freqs[freq_index[i]]."""
+        import torch
+        out = torch.empty((self.nnz, len(freq_index)), dtype=torch.complex128, device=self.pos.device)
+        for c, (i, k2) in enumerate(zip(freq_index, k0sq)):
+            kp, mp = self._entries(2.0 * math.pi * float(freqs[i]))
+            out[:, c] = kp - k2 * mp
+        return out
+
+
 def frequency_sweep_eigenmode(mesh: Mesh, params: MaxwellParams, bc: BC, ports: Sequence[WavePort],
                               frequencies, *, solver_options: Optional[SolveOptions] = None,
                               max_batch: int = 256, precision: str = "fp64",
@@ -2121,25 +2204,36 @@ def frequency_sweep_eigenmode(mesh: Mesh, params: MaxwellParams, bc: BC, ports: 
     if P == 0 or freqs.size == 0:
         return out
     opts = solver_options or SolveOptions()
-    if params._frequency_dependent() or P > _SWEEP_MAX_PORTS:
-        # PML or dispersive media: no K, M shared by the frequencies; the matrix is
-        # assembled and solved per frequency, as the reference's sweep does (sweep.cpp:330-346).
-        # More ports than the batched kernels carry per system take the same route.
+    fdep = params._frequency_dependent()
+    if P > _SWEEP_MAX_PORTS or (fdep and (_state or {}).get("per_frequency")):
+        # More ports than the batched kernels carry per system (or, on request, PML and
+        # dispersive media): the matrix is assembled and solved per frequency, as the
+        # reference's sweep does (sweep.cpp:330-346).
         for i, f in enumerate(freqs):
             out[i] = calculate_sparams_eigenmode(mesh, _params_at(params, 2.0 * math.pi * f), bc, ports,
                                                  solver_options=opts)
         return out
-    eps, mu = tet_materials(mesh, params)
     topo = mesh.topology
     dmask = _dirichlet_mask(mesh, bc)
     ops = _PortOps(mesh, bc, ports)
-    K, M = _capi.assemble_km(topo, mesh.dev_xyz, eps, 1.0 / mu, dmask)
+    corr = None
+    if fdep:
+        # PML or dispersive media: K, M carry the media at a reference frequency (they serve
+        # the shared SpMV and the preconditioner); each system adds the difference at its
+        # own frequency on the frequency-dependent tets' sub-pattern (_MediaCorrection)
+        w_ref = 2.0 * math.pi * float(np.median(freqs))
+        es_ref, ms_ref, _ = _assembly_materials(mesh, _params_at(params, w_ref))
+        K, M = _capi.assemble_km(topo, mesh.dev_xyz, es_ref, 1.0 / ms_ref, dmask)
+        corr = _MediaCorrection(mesh, params, dmask, es_ref, ms_ref)
+    else:
+        eps, mu = tet_materials(mesh, params)
+        K, M = _capi.assemble_km(topo, mesh.dev_xyz, eps, 1.0 / mu, dmask)
     # per frequency: k0^2 and the port betas (skip frequencies below any cutoff)
     sys_k2, sys_coef, sys_f, sys_a = [], [], [], []
     for i, f in enumerate(freqs):
-        p = MaxwellParams()
-        p.__dict__.update(params.__dict__)
-        p.omega = 2 * math.pi * f
+        p = _params_at(params, 2 * math.pi * f)
+        if fdep:  # beta from the port face's unstretched media at this frequency
+            eps, mu = tet_materials(mesh, p)
         betas = _eigenmode_betas(mesh, p, ports, eps, mu)
         if betas is None:
             continue
@@ -2161,6 +2255,8 @@ def frequency_sweep_eigenmode(mesh: Mesh, params: MaxwellParams, bc: BC, ports: 
         sweep.set_iteration(int(_state["iteration"]))
     if _state is not None and "graph" in _state:
         sweep.set_graph(bool(_state["graph"]))
+    if corr is not None:
+        sweep.set_correction(corr.rows, corr.ptr, corr.cols)
     n = mesh.num_edges()
     tol = 1e-14 if opts.use_direct else float(opts.tolerance)
     infos = []
@@ -2180,10 +2276,24 @@ def frequency_sweep_eigenmode(mesh: Mesh, params: MaxwellParams, bc: BC, ports: 
         if rows_l:
             flat = torch.from_numpy(np.concatenate(rows_l) * (b1 - b0) + np.concatenate(cols_l)).to(dev)
             Bd.view(-1).index_put_((flat,), torch.from_numpy(np.concatenate(vals_l)).to(dev))
+        if corr is not None:
+            sweep.set_correction_values(corr.values(freqs, [sys_f[si] for si in idx], [sys_k2[si] for si in idx]))
         X, info = sweep.solve([sys_k2[si] for si in idx], [sys_coef[si] for si in idx], Bd, tol=tol,
                               max_iterations=int(opts.max_iterations), precision=precision,
                               replace_factor=float((_state or {}).get("replace_factor", 0.01)))
         infos.append(info)
+        L = None
+        if _state is not None and _state.get("adjoint_conj"):
+            # adjoint solutions for complex port modes: A lam = conj(g_a) per column (A is
+            # complex-symmetric, so this is the forward system with another right-hand side)
+            La = torch.zeros((n, b1 - b0), dtype=torch.complex128, device=dev)
+            if rows_l:
+                vals_c = np.concatenate([np.conj(ops.g_val[sys_a[si]]) for si in idx])
+                La.view(-1).index_put_((flat,), torch.from_numpy(vals_c).to(dev))
+            L, linfo = sweep.solve([sys_k2[si] for si in idx], [sys_coef[si] for si in idx], La, tol=tol,
+                                   max_iterations=int(opts.max_iterations), precision=precision,
+                                   replace_factor=float((_state or {}).get("replace_factor", 0.01)))
+            _state.setdefault("adjoint_infos", []).append(linfo)
         ov = _capi.port_overlap(ops.gi_dev, ops.gv_dev, X, b1 - b0)     # (P, nb)
         for c, si in enumerate(idx):
             i, a = sys_f[si], sys_a[si]
@@ -2191,11 +2301,12 @@ def frequency_sweep_eigenmode(mesh: Mesh, params: MaxwellParams, bc: BC, ports: 
             out[i, :, a] = ov[:, c] - (np.arange(P) == a) if conv else np.nan + 1j * np.nan
         if _state is not None and _state.get("keep_solutions"):
             _state.setdefault("solutions", []).append(
-                dict(X=X, freq_index=[sys_f[si] for si in idx], excitation=[sys_a[si] for si in idx],
+                dict(X=X, L=L, freq_index=[sys_f[si] for si in idx], excitation=[sys_a[si] for si in idx],
                      k0sq=[sys_k2[si] for si in idx], coef=[sys_coef[si] for si in idx]))
     if _state is not None:
         v, a, tune_ms = sweep.variants()
-        _state.update(dict(infos=infos, sweep=sweep, ops=ops, spmv_chosen=v, aux_chosen=a, tune_ms=tune_ms))
+        _state.update(dict(infos=infos, sweep=sweep, ops=ops, spmv_chosen=v, aux_chosen=a, tune_ms=tune_ms,
+                           correction_entries=0 if corr is None else corr.nnz))
     return out
 
 

@@ -303,6 +303,22 @@ int efm_sweep_get_variants(const efm_sweep *sw, int *h_spmv, int *h_aux, double 
  * results are those of the plain launch sequence.  0 (default) = plain launches.  No
  * reference counterpart. */
 int efm_sweep_set_graph(efm_sweep *sw, int enable);
+/* Frequency-dependent media (PML stretch, dispersive models) in one batched solve: every
+ * system gets a correction C_s on a fixed sub-pattern,
+ *   A_s = K - k0sq[s] M + sum_p coef[s,p] M_s,p + C_s,
+ * so that K, M (given at efm_sweep_create, typically with the media at a reference
+ * frequency) keep serving the preconditioner and the shared SpMV, and C_s carries what
+ * differs at each system's frequency.  Replaces the per-frequency assembly and solve of
+ * frequency_sweep (src/sweep.cpp:187-190, 330-346) when PML or a dispersive model is present.
+ * set_correction: correction row m is matrix row h_rows[m] (each row at most once), with
+ * entries h_ptr[m]..h_ptr[m+1]-1 (h_ptr[0] = 0) in columns h_cols (host arrays, copied);
+ * n_rows = 0 removes the correction.  set_correction_values: d_vals c128 [entry][nsys]
+ * in device memory owned by the caller, read by every later solve of nsys systems until it
+ * is set again.  Needs the standard COCG form (0 or 2).  No reference counterpart (the
+ * reference re-assembles and solves each frequency). */
+int efm_sweep_set_correction(efm_sweep *sw, int64_t n_rows, const int32_t *h_rows,
+                             const int32_t *h_ptr, const int32_t *h_cols, void *stream);
+int efm_sweep_set_correction_values(efm_sweep *sw, int nsys, const double *d_vals);
 
 /* ------------------------------------------------------------------------
  * Periodic (Floquet) elimination on the sparse matrix: replaces the dense elimination of

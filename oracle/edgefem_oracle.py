@@ -966,7 +966,9 @@ def extract_surface_mesh(mesh: OMesh, tag: int):
 def solve_port_eigens(surf, num_modes: int, omega: float, eps_r=1.0, mu_r=1.0, pol: str = "TE"):
     """Nodal P1 port modes (port_eigensolve.cpp:97-275), element loop restated with
     explicit 2x2 algebra; returns a list of dicts (kc, fc, beta, Z0, field).  Eigenvector
-    sign: made real with its largest entry positive (the reference's is Eigen's).  A
+    sign: made real with its largest entry positive, entries within 1e-8 of the largest
+    magnitude counting as tied and the first in node order deciding (the reference's sign
+    is Eigen's).  A
     degenerate pair is resolved to the member with E along y (the least y-variation).  The
     zero-mode cut is kc^2 < max(1e-12, 1e-10 kc^2_max): with the reference's absolute
     1e-12 the Neumann constant mode, whose computed kc^2 is roundoff of order 1e-9 on
@@ -1023,7 +1025,8 @@ def solve_port_eigens(surf, num_modes: int, omega: float, eps_r=1.0, mu_r=1.0, p
         beta = np.sqrt(complex(k * k - kc2))
         Z0 = (omega * mu / beta if pol == "TE" else beta / (omega * eps)) if beta != 0 else 0j
         v = V[:, i].astype(complex)
-        j = int(np.argmax(np.abs(v)))
+        av = np.abs(v)
+        j = int(np.nonzero(av >= (1.0 - 1e-8) * av.max())[0][0])   # ties within 1e-8: node order
         v = v if v[j].real >= 0 else -v
         f = np.zeros(N, complex); f[~pec] = v
         pw = 0.0

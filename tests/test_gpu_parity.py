@@ -8,6 +8,7 @@ with the oracle on the same inputs:
 """
 
 import cmath
+import dataclasses
 import math
 import sys
 
@@ -581,6 +582,64 @@ def test_adjoint_gradient_torch_autograd(torch_dev):
         assert abs(grad[k] - fd) <= 1e-5 * max(abs(fd), 1e-6 * np.abs(ref).max())
 
 
+def test_adjoint_gradient_complex_port_modes(torch_dev):
+    """Complex port modes (each port's weights turned by a complex phase, so g_j = M_s e_j
+    is complex and the adjoint right-hand side conj(g_j) is no multiple of the excitation):
+    the adjoint solves A lam_j = conj(g_j) as a second batched solve.  A phase-sensitive
+    loss L = sum_f |S(f) - T|^2 (every entry, seeded target T) through torch.autograd
+    against the oracle's adjoint (SuperLU, lam = A^-T conj(g)) and central differences."""
+    import copy
+    import torch
+    from edgefem_amd import adjoint as adj
+    from edgefem_amd import pyedgefem as em
+    from oracle import edgefem_oracle as O
+    _p("[gpu] adjoint gradient with complex port modes vs oracle and central differences")
+    freqs = np.array([9.5e9, 11e9])
+    prob, design, eps0, (om, opec, oports) = _design_problem(freqs, seed=9)
+    phases = [np.exp(0.7j), np.exp(-2.1j)]
+    ports = []
+    for port, ph in zip(prob.ports, phases):
+        q = copy.copy(port)
+        q.weights = np.asarray(port.weights, np.complex128) * ph
+        ports.append(q)
+    oports = [dataclasses.replace(o, weights=np.asarray(o.weights, np.complex128) * ph)
+              for o, ph in zip(oports, phases)]
+    opts = em.SolveOptions()
+    opts.tolerance = 1e-12
+    prob = adj.EpsDesignProblem(prob.mesh, em.MaxwellParams(), prob.bc, ports, freqs, design, solver_options=opts)
+    assert prob.complex_modes
+    rng = np.random.default_rng(4)
+    T = rng.normal(size=(2, 2)) + 1j * rng.normal(size=(2, 2))
+    e = torch.tensor(eps0, dtype=torch.float64, device="cuda", requires_grad=True)
+    S = adj.SParamsOfPermittivity.apply(e, prob)
+    Tt = torch.tensor(T, dtype=torch.complex128, device="cuda")
+    Lt = ((S - Tt).abs() ** 2).sum()
+    Lt.backward()
+    grad = e.grad.cpu().numpy()
+    eps_full = prob.permittivity(eps0)
+    ref = np.zeros(design.size)
+    Lref = 0.0
+    for f in freqs:
+        p = O.Params(omega=2 * math.pi * f, eps_tet=eps_full)
+        So = O.sparams_eigenmode(om, p, opec, oports)
+        Lref += float((np.abs(So - T) ** 2).sum())
+        ref += O.eps_gradient_eigenmode(om, p, opec, oports, So - T, design=design)[1]
+    Lv = float(Lt.detach())
+    _p(f"    L {Lv:.12f} (oracle {Lref:.12f}); max |grad| {np.abs(ref).max():.3e}")
+    assert abs(Lv - Lref) <= 1e-8
+    np.testing.assert_allclose(grad, ref, rtol=1e-6, atol=1e-8 * np.abs(ref).max())
+    for k in rng.choice(design.size, 2, replace=False):
+        h = 1e-4 * eps0[k]
+        ep, em_ = eps0.copy(), eps0.copy()
+        ep[k] += h
+        em_[k] -= h
+        Lp = float((np.abs(prob.solve(ep)[0] - T) ** 2).sum())
+        Lm = float((np.abs(prob.solve(em_)[0] - T) ** 2).sum())
+        fd = (Lp - Lm) / (2 * h)
+        _p(f"    tet {design[k]}: adjoint {grad[k]:.9e} central difference {fd:.9e}")
+        assert abs(grad[k] - fd) <= 1e-5 * max(abs(fd), 1e-6 * np.abs(ref).max())
+
+
 def _lumped_pair(pm, om, pec_o, bc, z0=50.0):
     from edgefem_amd import pyedgefem as em
     from oracle import edgefem_oracle as O
@@ -970,10 +1029,11 @@ def test_pml_and_dispersive_assembly_vs_oracle(torch_dev):
 
 def test_dispersive_and_pml_eigenmode_sweep_vs_oracle(torch_dev):
     """frequency_sweep_eigenmode with a Debye slab and a thin scalar-PML slab in a short
-    WR-90 guide (6x3x12): no K, M shared by the frequencies, so the sweep assembles and
-    solves each frequency (the reference's slow path, sweep.cpp:330-346); S at 9, 10 and 11
-    GHz against the oracle's eigenmode S with the same models (1e-6), and the lossy slabs
-    make |S21| < 1."""
+    WR-90 guide (6x3x12): no K, M shared by the frequencies, so the batched sweep carries the
+    media at the median frequency in K, M and a per-system correction on the slabs' edges
+    (the reference assembles and solves each frequency, sweep.cpp:330-346); S at 9, 10 and
+    11 GHz against the oracle's eigenmode S with the same models (1e-6), against the
+    per-frequency assembly and solve (1e-8), and the lossy slabs make |S21| < 1."""
     from edgefem_amd import meshgen as mg
     from edgefem_amd import pyedgefem as em
     from edgefem_amd import materials as M
@@ -1005,14 +1065,90 @@ def test_dispersive_and_pml_eigenmode_sweep_vs_oracle(torch_dev):
                                          opec, oports) for f in freqs])
     opts = em.SolveOptions()
     opts.tolerance = 1e-10
-    S = em.frequency_sweep_eigenmode(pm, params, bc, ports, freqs, solver_options=opts)
+    st = {}
+    S = em.frequency_sweep_eigenmode(pm, params, bc, ports, freqs, solver_options=opts, _state=st)
     err = np.abs(S - Sref).max()
-    _p(f"    max |dS| {err:.2e}; |S21| {np.abs(S[:, 1, 0]).round(6)}; |S11| {np.abs(S[:, 0, 0]).round(6)}")
+    _p(f"    max |dS| {err:.2e}; |S21| {np.abs(S[:, 1, 0]).round(6)}; |S11| {np.abs(S[:, 0, 0]).round(6)}; "
+       f"iterations {st['infos'][0]['iters']}")
     assert err <= 1e-6
     assert np.all(np.abs(S[:, 1, 0]) < 0.999)
+    Spf = em.frequency_sweep_eigenmode(pm, params, bc, ports, freqs, solver_options=opts,
+                                       _state={"per_frequency": True})
     S10 = em.calculate_sparams_eigenmode(pm, em._params_at(params, 2 * math.pi * 10e9), bc, ports,
                                          solver_options=opts)
-    assert np.abs(S10 - S[1]).max() <= 1e-9
+    assert np.abs(S10 - Spf[1]).max() == 0.0
+    dpf = np.abs(S - Spf).max()
+    _p(f"    batched vs per-frequency |dS| {dpf:.2e}")
+    assert dpf <= 1e-8
+
+
+def test_batched_media_sweep_many_systems_vs_per_frequency(torch_dev):
+    """The batched sweep with frequency-dependent media on 40 frequencies x 2 excitations
+    (80 systems in batches of 48 and 32, so two groups of 64 lanes and a batch boundary):
+    a two-pole Lorentz slab, a Drude-Lorentz slab and a graded tensor-PML slab in the short
+    WR-90 guide.  Against the per-frequency assembly and solve (the reference's path,
+    sweep.cpp:330-346) at 1e-8 in FP64, with HIP-graph replay on; at 1e-6 with
+    complex-FP32 Krylov vectors and FP64 reliable updates.  Spot frequencies against the
+    oracle's eigenmode S (1e-6)."""
+    from edgefem_amd import meshgen as mg
+    from edgefem_amd import pyedgefem as em
+    from edgefem_amd import materials as M
+    from oracle import edgefem_oracle as O
+    _p("[gpu] batched media sweep, 80 systems, Lorentz + Drude-Lorentz + tensor PML")
+    L = 30e-3
+    g = mg.box_mesh(mg.WR90_A, mg.WR90_B, L, 6, 3, 12, seed=11)
+    zc = g.centroids()[:, 2]
+    phys = g.tet_phys.copy()
+    phys[(zc > 0.2 * L) & (zc < 0.35 * L)] = 111
+    phys[(zc > 0.45 * L) & (zc < 0.55 * L)] = 112
+    phys[(zc > 0.65 * L) & (zc < 0.8 * L)] = 113
+    pm = em.mesh_from_arrays(g.node_ids, g.xyz, g.tets, phys, g.tris, g.tri_phys)
+    bc = em.build_edge_pec(pm, 1)
+    mode = em.solve_te10_mode(em.RectWaveguidePort(mg.WR90_A, mg.WR90_B), 10e9)
+    kc2 = (math.pi / mg.WR90_A) ** 2
+    ports = [em.build_wave_port_2d(pm, t, mode, bc.dirichlet_edges, kc2) for t in (2, 3)]
+    w0 = 2 * math.pi * 10e9
+    lor = M.LorentzMaterial(2.0)
+    lor.add_pole(1.5, 1.3 * w0, 0.1 * w0)
+    lor.add_pole(0.5, 0.6 * w0, 0.05 * w0)
+    dl = M.DrudeLorentzMaterial(3.0, 0.5 * w0, 0.2 * w0)
+    spec = em.PMLRegionSpec()
+    spec.sigma_max = np.array([0.0, 0.0, 0.4 * w0])
+    spec.thickness = np.array([0.0, 0.0, 0.2 * L])
+    spec.grading_order = 2.0
+    params = em.MaxwellParams()
+    params.set_eps_model(111, lor)
+    params.set_eps_model(112, dl)
+    params.pml_tensor_regions = {113: spec}
+    freqs = np.linspace(8.5e9, 12e9, 40)
+    opts = em.SolveOptions()
+    opts.tolerance = 1e-10
+    Spf = em.frequency_sweep_eigenmode(pm, params, bc, ports, freqs, solver_options=opts,
+                                       _state={"per_frequency": True})
+    st = {"graph": True}
+    S = em.frequency_sweep_eigenmode(pm, params, bc, ports, freqs, solver_options=opts, max_batch=48, _state=st)
+    d64 = np.abs(S - Spf).max()
+    it = np.concatenate([i["iters"] for i in st["infos"]])
+    _p(f"    FP64 batched vs per-frequency |dS| {d64:.2e}; iterations {it.min()}-{it.max()}; "
+       f"|S21| {np.abs(S[:, 1, 0]).min():.4f}-{np.abs(S[:, 1, 0]).max():.4f}")
+    assert d64 <= 1e-8
+    assert np.all(np.abs(S[:, 1, 0]) < 0.999)
+    S32 = em.frequency_sweep_eigenmode(pm, params, bc, ports, freqs, solver_options=opts, max_batch=48,
+                                       precision="fp32")
+    d32 = np.abs(S32 - Spf).max()
+    _p(f"    FP32 + reliable updates vs per-frequency |dS| {d32:.2e}")
+    assert d32 <= 1e-6
+    om = O.mesh_from_arrays(g.node_ids, g.xyz, g.tet_ids, phys, g.tri_ids, g.tri_phys)
+    opec = O.build_edge_pec(om, 1)
+    kc, beta, Z0 = O.solve_te10_mode(mg.WR90_A, 10e9)
+    oports = [O.build_wave_port_2d(om, t, kc, Z0, beta, opec, kc2) for t in (2, 3)]
+    models = {111: lambda x: O.lorentz_eps(x, 2.0, [(1.5, 1.3 * w0, 0.1 * w0), (0.5, 0.6 * w0, 0.05 * w0)]),
+              112: lambda x: O.drude_lorentz_eps(x, 3.0, 0.5 * w0, 0.2 * w0, [])}
+    ptensor = {113: (spec.sigma_max, spec.thickness, spec.grading_order)}
+    for i in (0, 23, 39):
+        Sref = O.sparams_eigenmode(om, O.Params(omega=2 * math.pi * freqs[i], eps_models=models, pml_tensor=ptensor),
+                                   opec, oports)
+        assert np.abs(S[i] - Sref).max() <= 1e-6
 
 
 def test_batched_sweep_fp32_inner_refined_vs_oracle(torch_dev):

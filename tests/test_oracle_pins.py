@@ -941,6 +941,29 @@ def test_nodal_port_eigens_tm_closed_forms():
         assert abs(0.5 * beta / (md["kc"] ** 2 * om * O.MU0) * grad2 - 1.0) < 1e-9
 
 
+def test_nodal_port_mode_sign_survives_roundoff():
+    """On the square unit-cell face the TE10 field's largest magnitudes sit on both x-edges
+    with opposite signs and equal up to roundoff, so "largest entry positive" alone would
+    let the last bit of the assembly pick the sign (and S21's sign with it).  The rule takes
+    entries within 1e-8 of the largest as tied and the first in node order decides: the
+    mode keeps its sign under 1e-13 relative perturbations of the node coordinates, and that
+    first tied entry is positive."""
+    g = mg.unit_cell(n=11, nz=12, seed=434 + 1000)
+    m = O.mesh_from_arrays(g.node_ids, g.xyz, g.tet_ids, g.tet_phys, g.tri_ids, g.tri_phys)
+    surf = O.extract_surface_mesh(m, mg.CELL_TAGS["port_top"])
+    om = 2 * math.pi * 10e9
+    f0 = O.solve_port_eigens(surf, 1, om, 1.0, 1.0, "TE")[0]["field"]
+    a = np.abs(f0)
+    tied = np.nonzero(a >= (1 - 1e-8) * a.max())[0]
+    assert tied.size >= 2 and np.sign(f0[tied].real).min() < 0 < np.sign(f0[tied].real).max()
+    assert f0[tied[0]].real > 0
+    rng = np.random.default_rng(2)
+    for _ in range(8):
+        xy = surf[1] * (1.0 + 1e-13 * rng.standard_normal(surf[1].shape))
+        f = O.solve_port_eigens((surf[0], xy, surf[2], surf[3], surf[4]), 1, om, 1.0, 1.0, "TE")[0]["field"]
+        assert np.abs(f - f0).max() <= 1e-9 * a.max()
+
+
 def test_nodal_port_eigens_degenerate_pair_rule():
     """The unit-cell caller's port (unit_cell.py:735-763: solve_port_eigens(..., 1, ...) on
     the top face) is a 5 x 5 mm square, where TE10 and TE01 are degenerate and the
