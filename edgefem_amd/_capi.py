@@ -403,7 +403,7 @@ class KrylovSolver:
 
     def spmv_variant(self):
         v = C.c_int()
-        ms = np.zeros(11, np.float64)
+        ms = np.zeros(12, np.float64)
         _check(self._lib.efm_solver_get_spmv_variant(self._h, C.byref(v), ms.ctypes.data_as(C.c_void_p)),
                "efm_solver_get_spmv_variant")
         return v.value, ms

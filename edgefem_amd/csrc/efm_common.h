@@ -86,6 +86,7 @@ __device__ inline double2 ld_stream(const float2 *p) {
   return make_double2((double)v.x, (double)v.y);
 }
 __device__ inline int32_t ld_stream(const int32_t *p) { return __builtin_nontemporal_load(p); }
+__device__ inline double ld_stream(const double *p) { return __builtin_nontemporal_load(p); }
 __device__ inline void st_stream(double2 *p, double2 v) {
   efm_v2d w;
   w.x = v.x;

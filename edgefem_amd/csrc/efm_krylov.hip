@@ -581,472 +581,6 @@ __global__ void __launch_bounds__(kBlock)
   write_partials<R>(pq, lds, partial);
 }
 
-// Tiled variant whose record gather touches one cache line per nonzero.  A column's
-// record {z[R], p[R]} is 2R 16-byte pieces; 2R adjacent lanes load the pieces of one
-// nonzero in one instruction (so the L1 looks up the record's line once, where a lane
-// loading the whole record takes 2R lookups), lane q < R forms z_q + beta_q p_q with the
-// piece of lane q + R, and the products go to LDS for the row reduction of the tiled
-// kernel.  All loads of a tile step are issued with no branch (entries past the tile's
-// end read the tile's first entry and contribute zero).
-template <typename VT, int R, int TNNZ, int RL>
-__global__ void __launch_bounds__(kBlock)
-    k_cocg_spmv_lines(int ntiles, int xcd_span, const int32_t *__restrict__ tile_row,
-                      const int32_t *__restrict__ row_ptr, const int32_t *__restrict__ col,
-                      const VT *__restrict__ val, const double2 *__restrict__ Wa,
-                      double2 *__restrict__ Wb, double2 *__restrict__ q,
-                      double2 *__restrict__ partial, const SysState *__restrict__ st) {
-  constexpr int LPN = 2 * R;            // lanes per nonzero
-  constexpr int NPS = kBlock / LPN;     // nonzeros per block step
-  constexpr int J = TNNZ / NPS;         // steps per tile
-  static_assert(64 % LPN == 0 && J >= 1, "record pieces must tile a wavefront");
-  __shared__ double2 prod[TNNZ * R];
-  __shared__ double2 lds[(kBlock / 64) * R];
-  const int piece = threadIdx.x % LPN, slot = threadIdx.x / LPN;
-  // beta of the system whose z / p this lane combines (lanes piece < R)
-  const double2 beta_l = st[piece % R].beta;
-  const bool act_l = piece < R && st[piece % R].active != 0;
-  bool act[R];
-  double2 beta[R];
-#pragma unroll
-  for (int s = 0; s < R; ++s) {
-    act[s] = st[s].active != 0;
-    beta[s] = st[s].beta;
-  }
-  double2 pq[R];
-#pragma unroll
-  for (int s = 0; s < R; ++s)
-    pq[s] = make_double2(0, 0);
-  const int xcd = blockIdx.x & 7;
-  const int per_xcd = gridDim.x >> 3;
-  const int lb = blockIdx.x >> 3;
-  const int t_begin = xcd * xcd_span;
-  const int t_end = min(ntiles, t_begin + xcd_span);
-  for (int tile = t_begin + lb; tile < t_end; tile += per_xcd) {
-    const int32_t r0 = tile_row[tile], r1 = tile_row[tile + 1];
-    const int32_t kb = row_ptr[r0], ke = row_ptr[r1];
-    int32_t c[J];
-    double2 a[J], g[J];
-#pragma unroll
-    for (int j = 0; j < J; ++j) {
-      const int32_t k = kb + slot + j * NPS;
-      const int32_t kk = (k < ke) ? k : (ke > kb ? kb : 0);  // a valid entry either way
-      c[j] = ld_stream(col + kk);
-      const double2 av = ld_stream(val + kk);
-      a[j] = (k < ke) ? av : make_double2(0, 0);
-    }
-#pragma unroll
-    for (int j = 0; j < J; ++j)
-      g[j] = Wa[(int64_t)c[j] * LPN + piece];
-#pragma unroll
-    for (int j = 0; j < J; ++j) {
-      const double2 other = make_double2(__shfl_xor(g[j].x, R, LPN), __shfl_xor(g[j].y, R, LPN));
-      const int32_t lk = slot + j * NPS;
-      if (piece < R && kb + lk < ke)
-        prod[lk * R + piece] = bil(a[j], cfma(beta_l, other, g[j]));
-    }
-    __syncthreads();
-    const int gr = threadIdx.x / RL, l = threadIdx.x % RL;
-    const int32_t row = r0 + gr;
-    double2 acc[R];
-#pragma unroll
-    for (int s = 0; s < R; ++s)
-      acc[s] = make_double2(0, 0);
-    if (row < r1) {
-      const int32_t k0 = row_ptr[row] - kb, k1 = row_ptr[row + 1] - kb;
-      for (int32_t k = k0 + l; k < k1; k += RL) {
-#pragma unroll
-        for (int s = 0; s < R; ++s)
-          acc[s] = cadd(acc[s], prod[k * R + s]);
-      }
-    }
-#pragma unroll
-    for (int s = 0; s < R; ++s)
-      group_sum<RL>(acc[s]);
-    if (row < r1 && l == 0) {
-      const double2 *w = Wa + (int64_t)row * 2 * R;
-      double2 *wb = Wb + (int64_t)row * 2 * R;
-      // the whole record is written (z = 0; the update overwrites it) so that no partial
-      // 64-byte line reaches HBM
-      double2 rec[2 * R];
-#pragma unroll
-      for (int s = 0; s < R; ++s) {
-        rec[s] = make_double2(0, 0);
-        rec[R + s] = cfma(beta[s], w[R + s], w[s]);
-      }
-#pragma unroll
-      for (int s = 0; s < 2 * R; ++s)
-        wb[s] = rec[s];
-#pragma unroll
-      for (int s = 0; s < R; ++s) {
-        if (!act[s])
-          continue;
-        q[(int64_t)row * R + s] = acc[s];
-        pq[s] = cadd(pq[s], bil(rec[R + s], acc[s]));
-      }
-    }
-    __syncthreads();  // prod is reused by the next tile
-  }
-  (void)act_l;
-  write_partials<R>(pq, lds, partial);
-}
-
-// Tile-local column format for the first tiling (<= 1024 nonzeros per tile): the
-// tile's distinct columns, sorted, at ucol[kb .. kb + nu[t]) (kb = the tile's first
-// entry, so no offsets are needed), and for every entry its position in that list
-// (lidx, 16 bits).  One block per tile: bitonic sort of the tile's columns in LDS,
-// unique by a block scan, then a binary search per entry.
-constexpr int kLocalMax = 1024;
-
-__global__ void __launch_bounds__(kBlock)
-    k_build_local(int ntiles, const int4 *__restrict__ desc, const int32_t *__restrict__ col,
-                  int32_t *__restrict__ ucol, int32_t *__restrict__ nu, uint16_t *__restrict__ lidx) {
-  __shared__ int32_t key[kLocalMax];
-  __shared__ int32_t scan[kBlock];
-  const int t = blockIdx.x;
-  if (t >= ntiles)
-    return;
-  const int4 d = desc[t];
-  const int32_t kb = d.z, m = d.w - d.z;  // m <= kLocalMax
-  for (int i = threadIdx.x; i < kLocalMax; i += kBlock)
-    key[i] = i < m ? col[kb + i] : INT32_MAX;
-  __syncthreads();
-  for (int size = 2; size <= kLocalMax; size <<= 1) {
-    for (int stride = size >> 1; stride > 0; stride >>= 1) {
-      for (int i = threadIdx.x; i < kLocalMax; i += kBlock) {
-        const int j = i ^ stride;
-        if (j > i) {
-          const bool up = (i & size) == 0;
-          const int32_t a = key[i], b = key[j];
-          if ((a > b) == up) {
-            key[i] = b;
-            key[j] = a;
-          }
-        }
-      }
-      __syncthreads();
-    }
-  }
-  // unique: thread k handles entries 4k .. 4k+3
-  constexpr int PER = kLocalMax / kBlock;
-  int flag[PER], cnt = 0;
-#pragma unroll
-  for (int e = 0; e < PER; ++e) {
-    const int i = threadIdx.x * PER + e;
-    flag[e] = (key[i] != INT32_MAX && (i == 0 || key[i] != key[i - 1])) ? 1 : 0;
-    cnt += flag[e];
-  }
-  scan[threadIdx.x] = cnt;
-  __syncthreads();
-  for (int off = 1; off < kBlock; off <<= 1) {  // inclusive Hillis-Steele scan
-    const int v = threadIdx.x >= off ? scan[threadIdx.x - off] : 0;
-    __syncthreads();
-    scan[threadIdx.x] += v;
-    __syncthreads();
-  }
-  int pos = scan[threadIdx.x] - cnt;
-  const int total = scan[kBlock - 1];
-#pragma unroll
-  for (int e = 0; e < PER; ++e)
-    if (flag[e])
-      ucol[kb + pos++] = key[threadIdx.x * PER + e];
-  if (threadIdx.x == 0)
-    nu[t] = total;
-  __syncthreads();
-  // compact the distinct keys to the front of key[] for the searches
-  pos = scan[threadIdx.x] - cnt;
-  int32_t mine[PER];
-#pragma unroll
-  for (int e = 0; e < PER; ++e)
-    mine[e] = key[threadIdx.x * PER + e];
-  __syncthreads();
-#pragma unroll
-  for (int e = 0; e < PER; ++e)
-    if (flag[e])
-      key[pos++] = mine[e];
-  __syncthreads();
-  for (int i = threadIdx.x; i < m; i += kBlock) {
-    const int32_t c = col[kb + i];
-    int lo = 0, hi = total - 1;
-    while (lo < hi) {
-      const int mid = (lo + hi) >> 1;
-      if (key[mid] < c)
-        lo = mid + 1;
-      else
-        hi = mid;
-    }
-    lidx[kb + i] = (uint16_t)lo;
-  }
-}
-
-// SpMV on the tile-local format: phase 1 loads each distinct column's record of the tile
-// once (2R lanes per record, one cache line) and keeps p_new = z + beta p in LDS; phase 2
-// runs RL lanes per row over the row's entries (value and 16-bit local index streamed)
-// and reads p_new from LDS.  The gathered bytes drop by the tile's column reuse (about
-// 4x on tet meshes).
-template <typename VT, int R, int RL>
-__global__ void __launch_bounds__(kBlock)
-    k_cocg_spmv_local(int ntiles, int xcd_span, const int4 *__restrict__ desc,
-                      const int32_t *__restrict__ nu, const int32_t *__restrict__ ucol,
-                      const uint16_t *__restrict__ lidx, const int32_t *__restrict__ row_ptr,
-                      const VT *__restrict__ val, const double2 *__restrict__ Wa,
-                      double2 *__restrict__ Wb, double2 *__restrict__ q,
-                      double2 *__restrict__ partial, const SysState *__restrict__ st) {
-  constexpr int LPN = 2 * R;          // lanes per record
-  constexpr int NPS = kBlock / LPN;   // records per block step
-  static_assert(64 % LPN == 0, "record pieces must tile a wavefront");
-  __shared__ double2 pn[tile_nnz(R) * R];  // a tile has at most tile_nnz(R) distinct columns
-  __shared__ double2 lds[(kBlock / 64) * R];
-  const int piece = threadIdx.x % LPN, slot = threadIdx.x / LPN;
-  const double2 beta_l = st[piece % R].beta;
-  bool act[R];
-  double2 beta[R];
-#pragma unroll
-  for (int s = 0; s < R; ++s) {
-    act[s] = st[s].active != 0;
-    beta[s] = st[s].beta;
-  }
-  double2 pq[R];
-#pragma unroll
-  for (int s = 0; s < R; ++s)
-    pq[s] = make_double2(0, 0);
-  const int xcd = blockIdx.x & 7;
-  const int per_xcd = gridDim.x >> 3;
-  const int lb = blockIdx.x >> 3;
-  const int t_begin = xcd * xcd_span;
-  const int t_end = min(ntiles, t_begin + xcd_span);
-  const int g = threadIdx.x / RL, l = threadIdx.x % RL;
-  constexpr int EPL = 6;  // entries per lane held in registers (rows up to RL * EPL)
-  for (int tile = t_begin + lb; tile < t_end; tile += per_xcd) {
-    const int4 d = desc[tile];
-    const int u = nu[tile];
-    const int32_t kb = d.z;
-    // phase 2 inputs first (independent of phase 1): row bounds, values, local indices
-    const int32_t row = d.x + g;
-    const int32_t rc = row < d.y ? row : d.y - 1;
-    const int32_t k0 = row_ptr[rc], k1 = row < d.y ? row_ptr[rc + 1] : k0;
-    double2 a[EPL];
-    int li[EPL];
-    const int32_t kz = kb < d.w ? kb : 0;  // a valid entry either way
-#pragma unroll
-    for (int e = 0; e < EPL; ++e) {
-      const int32_t k = k0 + l + e * RL;
-      const int32_t kk = k < k1 ? k : kz;
-      a[e] = ld_stream(val + kk);
-      li[e] = (int)__builtin_nontemporal_load(lidx + kk);
-    }
-    double2 rz[R], rp[R];
-    {
-      const double2 *w = Wa + (int64_t)rc * 2 * R;
-#pragma unroll
-      for (int s = 0; s < R; ++s) {
-        rz[s] = w[s];
-        rp[s] = w[R + s];
-      }
-    }
-    // phase 1: the tile's distinct records
-    for (int i0 = 0; i0 < u; i0 += 4 * NPS) {
-      int32_t c[4];
-      double2 gv[4];
-#pragma unroll
-      for (int j = 0; j < 4; ++j) {
-        const int i = i0 + slot + j * NPS;
-        c[j] = ucol[kb + (i < u ? i : 0)];
-      }
-#pragma unroll
-      for (int j = 0; j < 4; ++j)
-        gv[j] = Wa[(int64_t)c[j] * LPN + piece];
-#pragma unroll
-      for (int j = 0; j < 4; ++j) {
-        const double2 other = make_double2(__shfl_xor(gv[j].x, R, LPN), __shfl_xor(gv[j].y, R, LPN));
-        const int i = i0 + slot + j * NPS;
-        if (piece < R && i < u)
-          pn[i * R + piece] = cfma(beta_l, other, gv[j]);
-      }
-    }
-    __syncthreads();
-    double2 acc[R];
-#pragma unroll
-    for (int s = 0; s < R; ++s)
-      acc[s] = make_double2(0, 0);
-#pragma unroll
-    for (int e = 0; e < EPL; ++e) {
-      // entries past the row's end are skipped by a select (their LDS slot may hold
-      // anything, so no multiplication by zero)
-      const bool in = k0 + l + e * RL < k1;
-      const int lk = in ? li[e] : 0;
-#pragma unroll
-      for (int s = 0; s < R; ++s) {
-        const double2 t = cfma(a[e], pn[lk * R + s], acc[s]);
-        acc[s].x = in ? t.x : acc[s].x;  // component selects (a struct select goes through memory)
-        acc[s].y = in ? t.y : acc[s].y;
-      }
-    }
-    // rows longer than RL * EPL entries
-    for (int32_t k = k0 + l + EPL * RL; k < k1; k += RL) {
-      const double2 av = to_d2(val[k]);
-      const int lk = (int)lidx[k];
-#pragma unroll
-      for (int s = 0; s < R; ++s)
-        acc[s] = cfma(av, pn[lk * R + s], acc[s]);
-    }
-#pragma unroll
-    for (int s = 0; s < R; ++s)
-      group_sum<RL>(acc[s]);
-    if (row < d.y && l == 0) {
-      double2 *wb = Wb + (int64_t)row * 2 * R;
-      double2 rec[2 * R];
-#pragma unroll
-      for (int s = 0; s < R; ++s) {
-        rec[s] = make_double2(0, 0);
-        rec[R + s] = cfma(beta[s], rp[s], rz[s]);
-      }
-#pragma unroll
-      for (int s = 0; s < 2 * R; ++s)
-        wb[s] = rec[s];
-#pragma unroll
-      for (int s = 0; s < R; ++s) {
-        if (!act[s])
-          continue;
-        q[(int64_t)row * R + s] = acc[s];
-        pq[s] = cadd(pq[s], bil(rec[R + s], acc[s]));
-      }
-    }
-    __syncthreads();  // pn is reused by the next tile
-  }
-  write_partials<R>(pq, lds, partial);
-}
-
-// Tiled variant with a cooperative record gather (R even).  The L1 of a CU looks up
-// about one cache line per clock, so a gather of four separate 16-byte pieces per
-// nonzero costs four lookups.  Here: (1a) the tile's columns are staged in LDS with
-// coalesced loads; (1b) four lanes read one column's 64-byte record run together (one
-// lookup) and form z + beta p; (1c) the values are loaded coalesced and multiplied in
-// LDS; (2) RL lanes per row reduce as in the plain tiled variant.
-template <typename VT, int R, int TNNZ, int RL>
-__global__ void __launch_bounds__(kBlock)
-    k_cocg_spmv_coop(int ntiles, int xcd_span, const int32_t *__restrict__ tile_row,
-                     const int32_t *__restrict__ row_ptr, const int32_t *__restrict__ col,
-                     const VT *__restrict__ val, const double2 *__restrict__ Wa,
-                     double2 *__restrict__ Wb, double2 *__restrict__ q,
-                     double2 *__restrict__ partial, const SysState *__restrict__ st) {
-  static_assert(R % 2 == 0, "cooperative gather needs an even number of systems");
-  __shared__ double2 prod[TNNZ * R];
-  __shared__ int32_t scol[TNNZ];
-  __shared__ double2 lds[(kBlock / 64) * R];
-  bool act[R];
-  double2 beta[R];
-#pragma unroll
-  for (int s = 0; s < R; ++s) {
-    act[s] = st[s].active != 0;
-    beta[s] = st[s].beta;
-  }
-  double2 pq[R];
-#pragma unroll
-  for (int s = 0; s < R; ++s)
-    pq[s] = make_double2(0, 0);
-  const int xcd = blockIdx.x & 7;
-  const int per_xcd = gridDim.x >> 3;
-  const int lb = blockIdx.x >> 3;
-  const int t_begin = xcd * xcd_span;
-  const int t_end = min(ntiles, t_begin + xcd_span);
-  constexpr int J = TNNZ / kBlock;
-  constexpr int E = 2 * R / 4;             // record entries per lane
-  constexpr int M = TNNZ / (kBlock / 4);  // nonzeros per lane group
-  const int l4 = threadIdx.x & 3, g4 = threadIdx.x >> 2;
-  for (int tile = t_begin + lb; tile < t_end; tile += per_xcd) {
-    const int32_t r0 = tile_row[tile], r1 = tile_row[tile + 1];
-    const int32_t kb = row_ptr[r0], ke = row_ptr[r1];
-    // (1a) columns -> LDS
-#pragma unroll
-    for (int j = 0; j < J; ++j) {
-      const int32_t lk = threadIdx.x + j * kBlock;
-      scol[lk] = (kb + lk < ke) ? col[kb + lk] : -1;
-    }
-    __syncthreads();
-    // (1b) cooperative record gather
-#pragma unroll 4
-    for (int m = 0; m < M; ++m) {
-      const int32_t lk = g4 + m * (kBlock / 4);
-      const int32_t c = scol[lk];
-      if (c >= 0) {
-        const double2 *w = Wa + (int64_t)c * 2 * R;
-        double2 ent[E];
-#pragma unroll
-        for (int j = 0; j < E; ++j)
-          ent[j] = w[l4 + 4 * j];
-        if constexpr (R == 2) {
-          double2 pv;
-          pv.x = __shfl_down(ent[0].x, 2, 4);
-          pv.y = __shfl_down(ent[0].y, 2, 4);
-          if (l4 < 2)
-            prod[lk * R + l4] = cfma(beta[l4], pv, ent[0]);
-        } else {
-#pragma unroll
-          for (int j = 0; j < E / 2; ++j) {
-            const int sidx = l4 + 4 * j;
-            prod[lk * R + sidx] = cfma(beta[sidx], ent[j + E / 2], ent[j]);
-          }
-        }
-      }
-    }
-    __syncthreads();
-    // (1c) values, coalesced
-#pragma unroll
-    for (int j = 0; j < J; ++j) {
-      const int32_t lk = threadIdx.x + j * kBlock;
-      if (kb + lk < ke) {
-        const double2 a = to_d2(val[kb + lk]);
-#pragma unroll
-        for (int s = 0; s < R; ++s)
-          prod[lk * R + s] = bil(a, prod[lk * R + s]);
-      }
-    }
-    __syncthreads();
-    // (2) row sums
-    const int g = threadIdx.x / RL, l = threadIdx.x % RL;
-    const int32_t row = r0 + g;
-    double2 acc[R];
-#pragma unroll
-    for (int s = 0; s < R; ++s)
-      acc[s] = make_double2(0, 0);
-    if (row < r1) {
-      const int32_t k0 = row_ptr[row] - kb, k1 = row_ptr[row + 1] - kb;
-      for (int32_t k = k0 + l; k < k1; k += RL) {
-#pragma unroll
-        for (int s = 0; s < R; ++s)
-          acc[s] = cadd(acc[s], prod[k * R + s]);
-      }
-    }
-#pragma unroll
-    for (int s = 0; s < R; ++s)
-      group_sum<RL>(acc[s]);
-    if (row < r1 && l == 0) {
-      const double2 *w = Wa + (int64_t)row * 2 * R;
-      double2 *wb = Wb + (int64_t)row * 2 * R;
-      // the whole record is written (z = 0; the update overwrites it) so that no partial
-      // 64-byte line reaches HBM, which would need a read-modify-write
-      double2 rec[2 * R];
-#pragma unroll
-      for (int s = 0; s < R; ++s) {
-        rec[s] = make_double2(0, 0);
-        rec[R + s] = cfma(beta[s], w[R + s], w[s]);
-      }
-#pragma unroll
-      for (int s = 0; s < 2 * R; ++s)
-        wb[s] = rec[s];
-#pragma unroll
-      for (int s = 0; s < R; ++s) {
-        if (!act[s])
-          continue;
-        q[(int64_t)row * R + s] = acc[s];
-        pq[s] = cadd(pq[s], bil(rec[R + s], acc[s]));
-      }
-    }
-    __syncthreads();
-  }
-  write_partials<R>(pq, lds, partial);
-}
-
 // x += alpha p, r -= alpha q, W_b.z = D^{-1} r; partials r^T z, r^H r
 template <int R>
 __global__ void __launch_bounds__(kBlock)
@@ -1795,11 +1329,17 @@ __global__ void __launch_bounds__(kBlock)
 
 // q = A p with p gathered from its own array (R values per column), tiled and pipelined
 // (the tile walk, staging and branch-free loads of k_cocg_spmv_pipe); partials p^T q.
-template <typename VT, int R, int TNNZ, int RL>
+// SPLIT (variant 11): the values are read as a real part per entry (vre) and, for the
+// tiles with an imaginary part anywhere (imoff[tile] >= 0), an imaginary part per entry of
+// that tile (vim + imoff[tile]); a tile whose values are all real streams 8 B per value
+// instead of 16.  val is unused then.
+template <typename VT, int R, int TNNZ, int RL, bool SPLIT = false>
 __global__ void __launch_bounds__(kBlock)
     k_cocgz_spmv(int ntiles, int xcd_span, const int4 *__restrict__ desc, const int32_t *__restrict__ row_ptr,
                  const int32_t *__restrict__ col, const VT *__restrict__ val, const double2 *__restrict__ pf,
-                 double2 *__restrict__ q, double2 *__restrict__ partial, const SysState *__restrict__ st) {
+                 double2 *__restrict__ q, double2 *__restrict__ partial, const SysState *__restrict__ st,
+                 const double *__restrict__ vre = nullptr, const double *__restrict__ vim = nullptr,
+                 const int32_t *__restrict__ imoff = nullptr) {
   // Products in one plane per system: a wavefront's writes are contiguous 16 B runs (no
   // bank conflicts), and a row's lanes read consecutive 16 B slots.
   __shared__ double2 prod[R * TNNZ];
@@ -1828,15 +1368,43 @@ __global__ void __launch_bounds__(kBlock)
   auto stage = [&](Stage &x, int tile) {
     x.d = desc[tile];
     const int32_t kz = x.d.z < x.d.w ? x.d.z : 0;  // a valid entry either way
+    if constexpr (SPLIT) {
+      const int32_t io = imoff[tile];  // uniform over the block
+      if (io >= 0) {
 #pragma unroll
-    for (int j = 0; j < J; ++j) {
-      const int32_t k = x.d.z + threadIdx.x + j * kBlock;
-      const bool in = k < x.d.w;
-      const int32_t kk = in ? k : kz;
-      const double2 av = ld_stream(val + kk);
-      x.c[j] = ld_stream(col + kk);
-      x.a[j].x = in ? av.x : 0.0;
-      x.a[j].y = in ? av.y : 0.0;
+        for (int j = 0; j < J; ++j) {
+          const int32_t k = x.d.z + threadIdx.x + j * kBlock;
+          const bool in = k < x.d.w;
+          const int32_t kk = in ? k : kz;
+          const double ar = ld_stream(vre + kk);
+          const double ai = ld_stream(vim + io + (kk - kz));
+          x.c[j] = ld_stream(col + kk);
+          x.a[j].x = in ? ar : 0.0;
+          x.a[j].y = in ? ai : 0.0;
+        }
+      } else {
+#pragma unroll
+        for (int j = 0; j < J; ++j) {
+          const int32_t k = x.d.z + threadIdx.x + j * kBlock;
+          const bool in = k < x.d.w;
+          const int32_t kk = in ? k : kz;
+          const double ar = ld_stream(vre + kk);
+          x.c[j] = ld_stream(col + kk);
+          x.a[j].x = in ? ar : 0.0;
+          x.a[j].y = 0.0;
+        }
+      }
+    } else {
+#pragma unroll
+      for (int j = 0; j < J; ++j) {
+        const int32_t k = x.d.z + threadIdx.x + j * kBlock;
+        const bool in = k < x.d.w;
+        const int32_t kk = in ? k : kz;
+        const double2 av = ld_stream(val + kk);
+        x.c[j] = ld_stream(col + kk);
+        x.a[j].x = in ? av.x : 0.0;
+        x.a[j].y = in ? av.y : 0.0;
+      }
     }
     const int32_t row = x.d.x + g;
     const int32_t rc = row < x.d.y ? row : x.d.y - 1;
@@ -2102,9 +1670,49 @@ __global__ void k_sell_vals(int64_t slots, const int32_t *__restrict__ svidx, co
 
 } // namespace
 
-constexpr int kNumSpmvVariants = 11;  // rows<4>, rows<8>, rows<16>, tiled<T,4>, tiled<T/2,8>, coop<T,4>, pipe<T,4>, lines<T/2,8>, local<4>, p-pass form, p-pass form with sliced ELL
+// Split storage of variant 11: flag[t] = 1 if an entry of tile t (first tiling) has a
+// nonzero imaginary part; one block per tile.
+__global__ void __launch_bounds__(kBlock)
+    k_split_flags(int ntiles, const int4 *__restrict__ desc, const double2 *__restrict__ val,
+                  int32_t *__restrict__ flag) {
+  const int t = blockIdx.x;
+  if (t >= ntiles)
+    return;
+  const int4 d = desc[t];
+  int any = 0;
+  for (int32_t k = d.z + threadIdx.x; k < d.w; k += kBlock)
+    any |= val[k].y != 0.0;
+  any = __syncthreads_or(any);
+  if (threadIdx.x == 0)
+    flag[t] = any;
+}
+
+// vre[k] = Re val[k] for every entry; vim[imoff[t] + k - d.z] = Im val[k] for the tiles with
+// imoff[t] >= 0.  One block per tile.
+__global__ void __launch_bounds__(kBlock)
+    k_split_fill(int ntiles, const int4 *__restrict__ desc, const int32_t *__restrict__ imoff,
+                 const double2 *__restrict__ val, double *__restrict__ vre, double *__restrict__ vim) {
+  const int t = blockIdx.x;
+  if (t >= ntiles)
+    return;
+  const int4 d = desc[t];
+  const int32_t io = imoff[t];
+  for (int32_t k = d.z + threadIdx.x; k < d.w; k += kBlock) {
+    const double2 v = val[k];
+    vre[k] = v.x;
+    if (io >= 0)
+      vim[io + (k - d.z)] = v.y;
+  }
+}
+
+constexpr int kNumSpmvVariants = 12;  // rows<4>, rows<8>, rows<16>, tiled<T,4>, tiled<T/2,8>, (5 removed), pipe<T,4>, (7, 8 removed), p-pass form, p-pass form with sliced ELL, p-pass form with split real/imaginary values
+// Numbers 5 (cooperative record gather), 7 (one cache line per nonzero) and 8 (tile-local
+// column format) belonged to variants that were slower on every matrix measured; they were
+// removed and their numbers are not reused.
+constexpr bool kRetired(int v) { return v == 5 || v == 7 || v == 8; }
 constexpr int kZForm = 9;             // COCG with p formed in its own pass (its own SpMV and update)
 constexpr int kSellForm = 10;         // the same iteration with the sliced-ELL SpMV
+constexpr int kSplitForm = 11;        // the same iteration, values stored as real parts plus the imaginary parts of complex tiles
 constexpr int kSellRL = 4;            // lanes per row of the sliced-ELL SpMV
 
 struct efm_solver {
@@ -2119,12 +1727,7 @@ struct efm_solver {
   // row tiles of the tiled SpMV variants (ntiles = 0: a row is longer than a tile)
   efm::DevBuf tile_row[2];
   efm::DevBuf tile_desc;  // int4 (row0, row1, nnz0, nnz1) per tile of the first tiling
-  // tile-local column format of the first tiling (built on first use by variant 8)
-  efm::DevBuf ucol, nu, lidx;
-  bool local_built = false;
-  int grid_local = 8;
   int ntiles[2] = {0, 0}, xcd_span[2] = {0, 0}, grid_tiles[2] = {8, 8};
-  int grid_lines = 8;  // the one-line-per-nonzero variant (second tiling, resident blocks)
   int grid_tiles_res[2] = {8, 8};  // resident-block grids of the tiled variants
   int grid_pipe_res = 8;
   // variant 9 (p formed in its own pass): z and p arrays [n][R]; resident grid of its SpMV
@@ -2136,6 +1739,12 @@ struct efm_solver {
   int sell_nslices = 0, sell_xcd_span = 0, grid_sell = 8;
   int64_t sell_slots = 0;
   bool sell_built = false, sell_vals_ok = false;
+  // variant 11 (split values, built on first use and after every set_matrix): real parts of
+  // every entry, imaginary parts of the tiles that have any, the offset of each tile's
+  // imaginary parts (-1: a real tile)
+  efm::DevBuf sp_re, sp_im, sp_off;
+  int64_t sp_im_n = 0;
+  bool sp_ok = false;
   int grid_rows[3] = {8, 8, 8};
   int spmv_variant = -1;  // -1: choose by timing at the first solve
   double tune_ms[kNumSpmvVariants] = {0};
@@ -2312,6 +1921,44 @@ template <int R> struct Dispatch {
     return EFM_OK;
   }
 
+  // The split values of variant 11 from the installed complex-double matrix.
+  static int build_split(efm_solver *s, hipStream_t st) {
+    const int nt = s->ntiles[0];
+    int rc;
+    DevBuf flag;
+    if ((rc = flag.alloc((size_t)nt * sizeof(int32_t))) || (rc = s->sp_off.alloc((size_t)nt * sizeof(int32_t))) ||
+        (!s->sp_re.p && (rc = s->sp_re.alloc((size_t)s->nnz * sizeof(double)))))
+      return rc;
+    hipLaunchKernelGGL(k_split_flags, dim3(nt), dim3(kBlock), 0, st, nt, s->tile_desc.as<int4>(),
+                       s->val.as<double2>(), flag.as<int32_t>());
+    EFM_LAUNCH_CHECK();
+    std::vector<int32_t> hf(nt), off(nt);
+    std::vector<int4> hd(nt);
+    EFM_HIP_CHECK(hipMemcpyAsync(hf.data(), flag.p, nt * sizeof(int32_t), hipMemcpyDeviceToHost, st));
+    EFM_HIP_CHECK(hipMemcpyAsync(hd.data(), s->tile_desc.p, nt * sizeof(int4), hipMemcpyDeviceToHost, st));
+    EFM_HIP_CHECK(hipStreamSynchronize(st));
+    int64_t m = 0;
+    for (int t = 0; t < nt; ++t) {
+      off[t] = hf[t] ? (int32_t)m : -1;
+      if (hf[t])
+        m += hd[t].w - hd[t].z;
+    }
+    if (m >= INT32_MAX)
+      return EFM_ERR_ARG;
+    if (m > s->sp_im_n || !s->sp_im.p) {
+      if ((rc = s->sp_im.alloc((size_t)std::max<int64_t>(m, 1) * sizeof(double))))
+        return rc;
+      s->sp_im_n = m;
+    }
+    EFM_HIP_CHECK(hipMemcpyAsync(s->sp_off.p, off.data(), nt * sizeof(int32_t), hipMemcpyHostToDevice, st));
+    hipLaunchKernelGGL(k_split_fill, dim3(nt), dim3(kBlock), 0, st, nt, s->tile_desc.as<int4>(),
+                       s->sp_off.as<int32_t>(), s->val.as<double2>(), s->sp_re.as<double>(), s->sp_im.as<double>());
+    EFM_LAUNCH_CHECK();
+    EFM_HIP_CHECK(hipStreamSynchronize(st));  // the host vectors above go out of scope
+    s->sp_ok = true;
+    return EFM_OK;
+  }
+
   // One fused SpMV of the given variant; *nb = number of partial sums written.
   template <typename VT>
   static int launch_spmv(efm_solver *s, int variant, const double2 *Wa, double2 *Wb, hipStream_t st,
@@ -2346,55 +1993,12 @@ template <int R> struct Dispatch {
                          s->ntiles[1], s->xcd_span[1], s->tile_row[1].as<int32_t>(), s->row_ptr, s->col,
                          s->val.as<VT>(), Wa, Wb, s->q.as<double2>(), s->partial.as<double2>(), S);
       break;
-    case 5:
-      if constexpr (R % 2 == 0) {
-        *nb = s->grid_tiles[0];
-        hipLaunchKernelGGL((k_cocg_spmv_coop<VT, R, TN, 4>), dim3(*nb), dim3(kBlock), 0, st, s->ntiles[0],
-                           s->xcd_span[0], s->tile_row[0].as<int32_t>(), s->row_ptr, s->col,
-                           s->val.as<VT>(), Wa, Wb, s->q.as<double2>(), s->partial.as<double2>(), S);
-        break;
-      }
-      set_error("SpMV variant 5 needs an even number of right-hand sides");
-      return EFM_ERR_ARG;
     case 6:
       *nb = s->grid_pipe_res;
       hipLaunchKernelGGL((k_cocg_spmv_pipe<VT, R, TN, 4>), dim3(*nb), dim3(kBlock), 0, st, s->ntiles[0],
                          s->xcd_span[0], s->tile_desc.as<int4>(), s->row_ptr, s->col, s->val.as<VT>(), Wa, Wb,
                          s->q.as<double2>(), s->partial.as<double2>(), S);
       break;
-    case 7:
-      if constexpr (R == 1 || R == 2 || R == 4) {
-        *nb = s->grid_lines;
-        hipLaunchKernelGGL((k_cocg_spmv_lines<VT, R, tile_nnz2(R), 8>), dim3(*nb), dim3(kBlock), 0, st,
-                           s->ntiles[1], s->xcd_span[1], s->tile_row[1].as<int32_t>(), s->row_ptr, s->col,
-                           s->val.as<VT>(), Wa, Wb, s->q.as<double2>(), s->partial.as<double2>(), S);
-        break;
-      }
-      set_error("SpMV variant 7 needs 1, 2 or 4 right-hand sides");
-      return EFM_ERR_ARG;
-    case 8:
-      if constexpr (R == 1 || R == 2 || R == 4) {
-        if (!s->local_built) {
-          int rc;
-          if ((rc = s->ucol.alloc((size_t)s->nnz * sizeof(int32_t))) ||
-              (rc = s->nu.alloc((size_t)s->ntiles[0] * sizeof(int32_t))) ||
-              (rc = s->lidx.alloc((size_t)s->nnz * sizeof(uint16_t))))
-            return rc;
-          hipLaunchKernelGGL(k_build_local, dim3(s->ntiles[0]), dim3(kBlock), 0, st, s->ntiles[0],
-                             s->tile_desc.as<int4>(), s->col, s->ucol.as<int32_t>(), s->nu.as<int32_t>(),
-                             s->lidx.as<uint16_t>());
-          EFM_LAUNCH_CHECK();
-          s->local_built = true;
-        }
-        *nb = s->grid_local;
-        hipLaunchKernelGGL((k_cocg_spmv_local<VT, R, 4>), dim3(*nb), dim3(kBlock), 0, st, s->ntiles[0],
-                           s->xcd_span[0], s->tile_desc.as<int4>(), s->nu.as<int32_t>(), s->ucol.as<int32_t>(),
-                           s->lidx.as<uint16_t>(), s->row_ptr, s->val.as<VT>(), Wa, Wb, s->q.as<double2>(),
-                           s->partial.as<double2>(), S);
-        break;
-      }
-      set_error("SpMV variant 8 needs 1, 2 or 4 right-hand sides");
-      return EFM_ERR_ARG;
     case kSellForm: {
       int rc;
       if (!s->sell_built && (rc = build_sell(s, st)) != EFM_OK)
@@ -2418,6 +2022,21 @@ template <int R> struct Dispatch {
                          s->xcd_span[0], s->tile_desc.as<int4>(), s->row_ptr, s->col, s->val.as<VT>(),
                          s->pf.as<double2>(), s->q.as<double2>(), s->partial.as<double2>(), S);
       break;
+    case kSplitForm:
+      if constexpr (std::is_same<VT, double2>::value) {
+        int rc;
+        if (!s->sp_ok && (rc = build_split(s, st)) != EFM_OK)
+          return rc;
+        *nb = s->grid_z;
+        hipLaunchKernelGGL((k_cocgz_spmv<VT, R, TN, 4, true>), dim3(*nb), dim3(kBlock), 0, st, s->ntiles[0],
+                           s->xcd_span[0], s->tile_desc.as<int4>(), s->row_ptr, s->col, s->val.as<VT>(),
+                           s->pf.as<double2>(), s->q.as<double2>(), s->partial.as<double2>(), S,
+                           (const double *)s->sp_re.as<double>(), (const double *)s->sp_im.as<double>(),
+                           (const int32_t *)s->sp_off.as<int32_t>());
+        break;
+      }
+      set_error("SpMV variant 11 needs complex-double matrix values");
+      return EFM_ERR_ARG;
     default:
       set_error("unknown SpMV variant");
       return EFM_ERR_ARG;
@@ -2439,23 +2058,21 @@ template <int R> struct Dispatch {
       s->tune_ms[v] = -1.0;
       if (v >= 3 && v <= 4 && s->ntiles[v - 3] == 0)
         continue;
-      if (v == 5 && (s->ntiles[0] == 0 || (R % 2) != 0))
+      if (kRetired(v))
         continue;
       if (v == 6 && s->ntiles[0] == 0)
-        continue;
-      if (v == 7 && (s->ntiles[1] == 0 || !(R == 1 || R == 2 || R == 4)))
-        continue;
-      if (v == 8 && (s->ntiles[0] == 0 || !(R == 1 || R == 2 || R == 4)))
         continue;
       if (v == kZForm && (s->ntiles[0] == 0 || s->lr_P > 0))
         continue;
       if (v == kSellForm && s->lr_P > 0)
         continue;
+      if (v == kSplitForm && (s->ntiles[0] == 0 || s->lr_P > 0 || !std::is_same<VT, double2>::value))
+        continue;
       int nb;
       // variant 9 is timed with its p pass (run on W1 as scratch: the real p must stay as
       // init left it)
       auto launch = [&]() -> int {
-        if (v == kZForm || v == kSellForm) {
+        if (v == kZForm || v == kSellForm || v == kSplitForm) {
           hipLaunchKernelGGL(k_cocgz_pform<R>, dim3(s->grid), dim3(kBlock), 0, st, s->n, s->zf.as<double2>(),
                              s->W1.as<double2>(), s->state.as<SysState>());
           EFM_LAUNCH_CHECK();
@@ -2492,7 +2109,7 @@ template <int R> struct Dispatch {
     double2 *Wa = (it & 1) ? s->W1.as<double2>() : s->W0.as<double2>();
     double2 *Wb = (it & 1) ? s->W0.as<double2>() : s->W1.as<double2>();
     SysState *S = s->state.as<SysState>();
-    if (s->spmv_variant == kZForm || s->spmv_variant == kSellForm) {
+    if (s->spmv_variant == kZForm || s->spmv_variant == kSellForm || s->spmv_variant == kSplitForm) {
       int nb_spmv = 0;
       int rc;
       // p = z + beta p, then q = A p
@@ -2774,50 +2391,6 @@ extern "C" int efm_solver_create(int64_t n, int64_t nnz, const int32_t *row_ptr,
       s->grid_z = (int)(8 * per_xcd_z);
     }
   }
-  // one-line-per-nonzero variant: as many blocks as are resident (a multiple of 8)
-  if (s->ntiles[1] > 0) {
-    const void *kl = nullptr;
-    const bool c64 = value_type == EFM_C64;
-    switch (nrhs) {
-    case 1:
-      kl = c64 ? reinterpret_cast<const void *>(&k_cocg_spmv_lines<float2, 1, tile_nnz2(1), 8>)
-               : reinterpret_cast<const void *>(&k_cocg_spmv_lines<double2, 1, tile_nnz2(1), 8>);
-      break;
-    case 2:
-      kl = c64 ? reinterpret_cast<const void *>(&k_cocg_spmv_lines<float2, 2, tile_nnz2(2), 8>)
-               : reinterpret_cast<const void *>(&k_cocg_spmv_lines<double2, 2, tile_nnz2(2), 8>);
-      break;
-    case 4:
-      kl = c64 ? reinterpret_cast<const void *>(&k_cocg_spmv_lines<float2, 4, tile_nnz2(4), 8>)
-               : reinterpret_cast<const void *>(&k_cocg_spmv_lines<double2, 4, tile_nnz2(4), 8>);
-      break;
-    default:
-      break;
-    }
-    int nbl = 0;
-    if (!kl || hipOccupancyMaxActiveBlocksPerMultiprocessor(&nbl, kl, kBlock, 0) != hipSuccess || nbl < 1)
-      nbl = 1;
-    const int64_t per_xcd = std::min<int64_t>(s->xcd_span[1], std::max<int64_t>(1, (int64_t)ncu * nbl / 8));
-    s->grid_lines = (int)(8 * per_xcd);
-  }
-  if (s->ntiles[0] > 0 && (nrhs == 1 || nrhs == 2 || nrhs == 4)) {
-    const bool c64 = value_type == EFM_C64;
-    const void *kl = nullptr;
-    if (nrhs == 1)
-      kl = c64 ? reinterpret_cast<const void *>(&k_cocg_spmv_local<float2, 1, 4>)
-               : reinterpret_cast<const void *>(&k_cocg_spmv_local<double2, 1, 4>);
-    else if (nrhs == 2)
-      kl = c64 ? reinterpret_cast<const void *>(&k_cocg_spmv_local<float2, 2, 4>)
-               : reinterpret_cast<const void *>(&k_cocg_spmv_local<double2, 2, 4>);
-    else
-      kl = c64 ? reinterpret_cast<const void *>(&k_cocg_spmv_local<float2, 4, 4>)
-               : reinterpret_cast<const void *>(&k_cocg_spmv_local<double2, 4, 4>);
-    int nbl = 0;
-    if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&nbl, kl, kBlock, 0) != hipSuccess || nbl < 1)
-      nbl = 1;
-    const int64_t per_xcd = std::min<int64_t>(s->xcd_span[0], std::max<int64_t>(1, (int64_t)ncu * nbl / 8));
-    s->grid_local = (int)(8 * per_xcd);
-  }
   {
     // sliced-ELL SpMV: as many blocks as are resident, a multiple of 8
     const bool c64 = value_type == EFM_C64;
@@ -2839,11 +2412,9 @@ extern "C" int efm_solver_create(int64_t n, int64_t nnz, const int32_t *row_ptr,
     gmax = std::max(gmax, s->grid_rows[v]);
   for (int v = 0; v < 2; ++v)
     gmax = std::max(gmax, s->grid_tiles[v]);
-  gmax = std::max(gmax, s->grid_lines);
   gmax = std::max(gmax, std::max(s->grid_tiles_res[0], s->grid_tiles_res[1]));
   gmax = std::max(gmax, s->grid_pipe_res);
   gmax = std::max(gmax, s->grid_z);
-  gmax = std::max(gmax, s->grid_local);
   if ((rc = s->val.alloc(vbytes)) || (rc = s->dinv.alloc(n * sizeof(double2))) ||
       (rc = s->x.alloc(vb)) || (rc = s->r.alloc(vb)) || (rc = s->q.alloc(vb)) ||
       (rc = s->W0.alloc(2 * vb)) || (rc = s->W1.alloc(2 * vb)) || (rc = s->zf.alloc(vb)) ||
@@ -2864,6 +2435,7 @@ extern "C" int efm_solver_set_matrix(efm_solver *s, const void *values, void *st
   hipStream_t st = (hipStream_t)stream;
   EFM_HIP_CHECK(hipMemcpyAsync(s->val.p, values, s->val.bytes, hipMemcpyDeviceToDevice, st));
   s->sell_vals_ok = false;  // the sliced-ELL values are refilled at the next SpMV of variant 10
+  s->sp_ok = false;         // and the split values of variant 11 rebuilt
   if (s->vtype == EFM_C128)
     hipLaunchKernelGGL(k_dinv<double2>, dim3(grid_for(s->n, 256)), dim3(256), 0, st, s->n, s->row_ptr,
                        s->col, s->val.as<double2>(), s->dinv.as<double2>());
@@ -3064,14 +2636,13 @@ extern "C" int efm_solver_spmv(efm_solver *s, const double *x, double *y, void *
 extern "C" int efm_solver_set_spmv_variant(efm_solver *s, int variant) {
   EFM_REQUIRE(s, "efm_solver_set_spmv_variant: NULL solver");
   EFM_REQUIRE(variant >= -1 && variant < kNumSpmvVariants, "efm_solver_set_spmv_variant: bad variant");
+  EFM_REQUIRE(variant < 0 || !kRetired(variant),
+              "efm_solver_set_spmv_variant: variants 5, 7 and 8 were removed (slower on every matrix measured)");
   EFM_REQUIRE(variant < 3 || variant == kSellForm ||
-                  s->ntiles[(variant == 5 || variant == 6 || variant == 8 || variant == kZForm) ? 0
-                                                                                         : (variant == 7 ? 1 : variant - 3)] > 0,
+                  s->ntiles[(variant == 6 || variant == kZForm || variant == kSplitForm) ? 0 : variant - 3] > 0,
               "efm_solver_set_spmv_variant: matrix has a row longer than a tile");
-  EFM_REQUIRE(variant != 5 || s->R % 2 == 0,
-              "efm_solver_set_spmv_variant: variant 5 needs an even number of right-hand sides");
-  EFM_REQUIRE((variant != 7 && variant != 8) || s->R == 1 || s->R == 2 || s->R == 4,
-              "efm_solver_set_spmv_variant: variants 7 and 8 need 1, 2 or 4 right-hand sides");
+  EFM_REQUIRE(variant != kSplitForm || s->vtype == EFM_C128,
+              "efm_solver_set_spmv_variant: variant 11 needs complex-double matrix values");
   s->spmv_variant = variant;
   return EFM_OK;
 }
@@ -3115,8 +2686,9 @@ extern "C" int efm_solver_solve(efm_solver *s, const double *B, double *X,
   const int check = h_opts->check_every > 0 ? h_opts->check_every : 32;
   const bool cocg = h_opts->method == EFM_METHOD_COCG;
   EFM_REQUIRE(!(cocg && s->caux), "efm_solver_solve: the complex-incidence preconditioner needs BiCGSTAB");
-  EFM_REQUIRE(!(cocg && (s->spmv_variant == kZForm || s->spmv_variant == kSellForm) && s->lr_P > 0),
-              "efm_solver_solve: SpMV variants 9 and 10 (p formed in its own pass) do not take low-rank port terms");
+  EFM_REQUIRE(!(cocg && (s->spmv_variant == kZForm || s->spmv_variant == kSellForm || s->spmv_variant == kSplitForm) &&
+                s->lr_P > 0),
+              "efm_solver_solve: SpMV variants 9-11 (p formed in its own pass) do not take low-rank port terms");
   int rc;
   if (!cocg) {
     const size_t vb = (size_t)s->n * s->R * sizeof(double2);

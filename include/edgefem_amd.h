@@ -182,20 +182,20 @@ int efm_solver_solve(efm_solver *s, const double *B, double *X, const efm_solve_
                      void *stream);
 
 /* The COCG SpMV has several row-to-lane mappings (0-2: 4, 8, 16 lanes per row; 3-4:
- * LDS-tiled with 4 or 8 lanes per row; 5: LDS-tiled with a cooperative gather of each
- * column's record, even nrhs only; 6: LDS-tiled with the next tile's loads issued while
- * the current tile's gathers are in flight; 7: LDS-tiled with each column's record
- * loaded by 2*nrhs adjacent lanes, one cache line per nonzero, nrhs 1, 2 or 4; 8: a
- * tile-local column format (each tile's distinct columns loaded once into LDS, 16-bit
- * local indices per entry, built on the device at first use), nrhs 1, 2 or 4; 9: the same
+ * LDS-tiled with 4 or 8 lanes per row; 6: LDS-tiled with the next tile's loads issued
+ * while the current tile's gathers are in flight; 5, 7 and 8: removed (slower on every
+ * matrix measured; set returns EFM_ERR_ARG, the tuning time is -1); 9: the same
  * COCG iteration with p = z + beta p formed in a streaming pass of its own, so the SpMV
  * gathers only p (R values per column) and writes only q, COCG without low-rank terms; 10:
  * the same iteration with a sliced-ELL SpMV: slices of 16 rows, 4 lanes per row, the entries
  * of a slice stored step-major so that every step is one coalesced load, rows ordered by
- * length inside windows of 4096, no LDS; built on the device at first use).  By
+ * length inside windows of 4096, no LDS; built on the device at first use); 11: variant 9
+ * with the values stored as a real part per entry plus the imaginary parts of the tiles
+ * that have any (a tile of real values streams 8 B per value instead of 16; complex-double
+ * matrices; rebuilt on the device after every set_matrix).  By
  * default (-1) the first solve times every variant on the installed matrix and keeps the
  * fastest; a fixed variant makes results reproducible bit for bit across runs.
- * h_tune_ms (11 entries, may be NULL) receives the
+ * h_tune_ms (12 entries, may be NULL) receives the
  * measured average ms per launch (-1 = not available).  No reference counterpart. */
 int efm_solver_set_spmv_variant(efm_solver *s, int variant);
 int efm_solver_get_spmv_variant(const efm_solver *s, int *h_variant, double *h_tune_ms);
