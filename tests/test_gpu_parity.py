@@ -1545,6 +1545,8 @@ def test_c4_cells_at_size_vs_golden(torch_dev):
     sys.path.insert(0, os.path.dirname(golden("c4_cells.npz")))
     import make_golden as MG
     _p("[gpu] config 4 cells at size vs golden")
+    if not os.path.exists(golden("c4_cells.npz")):
+        pytest.skip("tests/golden/c4_cells.npz not generated (python tests/golden/make_golden.py c4)")
     ref = np.load(golden("c4_cells.npz"))
     dist = MG.c4_distribution()
     cells = [_c4_gpu_cell(int(c), dist) for c in ref["cells"]]
