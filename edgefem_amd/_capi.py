@@ -122,7 +122,7 @@ def load_library(path: str = LIB_PATH) -> C.CDLL:
     lib.efm_sweep_set_variants.argtypes = [vp, i32, i32]
     lib.efm_sweep_set_iteration.argtypes = [vp, i32]
     lib.efm_sweep_set_correction.argtypes = [vp, i64, vp, vp, vp, vp]
-    lib.efm_sweep_set_correction_values.argtypes = [vp, i32, vp]
+    lib.efm_sweep_set_correction_values.argtypes = [vp, i32, i32, vp, vp, vp]
     lib.efm_tet_fields.argtypes = [vp, vp, i32, vp, vp, vp, vp, vp]
     lib.efm_gmsh_read.argtypes = [C.c_char_p, i32, C.POINTER(vp)]
     lib.efm_gmsh_sizes.argtypes = [vp, vp]
@@ -181,14 +181,17 @@ def dptr(t) -> Optional[int]:
         raise EdgeFemError("expected a device tensor")
     if not t.is_contiguous():
         raise EdgeFemError("expected a contiguous tensor")
+    if t.is_conj() or t.is_neg():
+        raise EdgeFemError("expected a materialised tensor (a lazy conj/neg view keeps the raw values in memory)")
     return t.data_ptr()
 
 
 def to_dev(a, dtype=None):
-    """numpy / torch -> contiguous device tensor."""
+    """numpy / torch -> contiguous device tensor (a lazily conjugated or negated torch view
+    is materialised: the C ABI reads the raw memory)."""
     torch = _torch()
     if isinstance(a, torch.Tensor):
-        t = a
+        t = a.resolve_conj().resolve_neg()
     else:
         t = torch.from_numpy(np.ascontiguousarray(a))
     if dtype is not None:
@@ -517,15 +520,18 @@ class SweepSolver:
                                                   pp.ctypes.data_as(C.c_void_p), cc.ctypes.data_as(C.c_void_p),
                                                   stream_ptr()), "efm_sweep_set_correction")
 
-    def set_correction_values(self, vals) -> None:
-        """Values of C_s: a complex128 device tensor (entries, nsys), kept alive here until
-        replaced; later solves must have nsys systems."""
+    def set_correction_values(self, vals, col_of_system) -> None:
+        """Values of the corrections: a complex128 device tensor (entries, ncols), kept
+        alive here until replaced; system s of the later solves uses column
+        col_of_system[s] (their count is the solves' number of systems)."""
         torch = _torch()
         v = to_dev(vals, torch.complex128)
         if v.dim() != 2:
-            raise EdgeFemError("correction values must be (entries, nsys)")
+            raise EdgeFemError("correction values must be (entries, ncols)")
+        cm = np.ascontiguousarray(np.asarray(col_of_system, np.int32))
         self._corr_vals = v
-        _check(self._lib.efm_sweep_set_correction_values(self._h, int(v.shape[1]), dptr(v)),
+        _check(self._lib.efm_sweep_set_correction_values(self._h, int(cm.size), int(v.shape[1]), dptr(v),
+                                                         cm.ctypes.data_as(C.c_void_p), stream_ptr()),
                "efm_sweep_set_correction_values")
 
     def set_iteration(self, form: int = 2) -> None:

@@ -149,3 +149,136 @@ def _function():
 
         SParamsOfPermittivity._fn = _F
     return SParamsOfPermittivity._fn
+
+
+class PeriodicEpsDesignProblem:
+    """A periodic unit cell (calculate_sparams_periodic, assemble_maxwell.cpp:576-635) as a
+    function of the real relative permittivity of ``design`` tets, at the frequency of
+    ``params`` and the Floquet phase set on ``pbc``.  Same contract as EpsDesignProblem
+    (``solve`` and ``gradient``), so ``SParamsOfPermittivity.apply(eps, problem)`` returns a
+    differentiable (P, P) S matrix.
+
+    Forward: the reduced system A' = Z E A F Z + I_S (E adds the slave rows to the master
+    rows, F the slave columns to the master columns, Z drops the slaves) with the ports'
+    rank-1 loading c w w^H, solved by BiCGSTAB; slaves recovered as x = R x'; S_ji =
+    (w_j^H x_i - delta_ji sqrt(Z0_i)) / sqrt(Z0_i).
+
+    Backward: A' is not symmetric (a complex Floquet phase, and the Hermitian rank-1
+    loading), so the adjoint needs its transpose.  With lam_j = A'^-T R^T conj(w_j) and
+    dA/d eps_t = -k0^2 M_t,
+        dS_ji / d eps_t = k0^2 / sqrt(Z0_i) (E^T Z lam_j)^T M_t (F Z x'_i),
+    one transposed solve per output port.  A'^T is installed on the same pattern (the
+    elimination keeps it structurally symmetric; checked) with the values transposed, the
+    port terms' vectors conjugated (c w w^H)^T = c conj(w) conj(w)^H, and the auxiliary
+    preconditioner's node phase conjugated (the rows' phase is the columns' of A'^T).
+    beta of a port depends on the permittivity of the tets on its face, so those tets may
+    not be design tets (checked)."""
+
+    def __init__(self, mesh: em.Mesh, params: em.MaxwellParams, bc: em.BC, pbc, ports: Sequence[em.WavePort],
+                 design, mode: str = "reference", solver_options: Optional[em.SolveOptions] = None):
+        self.mesh = mesh
+        self.params = params
+        self.bc = bc
+        self.pbc = pbc
+        self.ports = list(ports)
+        self.mode = mode
+        self.design = np.asarray(design, np.int64).ravel()
+        if self.design.size and (self.design.min() < 0 or self.design.max() >= mesh.tet_nodes.shape[0]):
+            raise ValueError("design tet index out of range")
+        if np.unique(self.design).size != self.design.size:
+            raise ValueError("design tets must be distinct")
+        for port in self.ports:
+            t = em._port_face_tet(mesh, port.surface_tag)
+            if t >= 0 and np.any(self.design == t):
+                raise ValueError(f"tet {t} sets beta of port {port.surface_tag}; it cannot be a design tet")
+        self.solver_options = solver_options
+        params._check_supported()
+        self.base_eps, _ = em.tet_materials(mesh, params)
+        self._tperm = None
+
+    def permittivity(self, eps_design: np.ndarray) -> np.ndarray:
+        eps = self.base_eps.copy()
+        eps[self.design] = np.asarray(eps_design, np.float64) + 1j * self.base_eps[self.design].imag
+        return eps
+
+    def solve(self, eps_design):
+        """S (P, P) and the kept state of the forward solve."""
+        p = copy.copy(self.params)
+        p.eps_r_tet = self.permittivity(eps_design)
+        st = {}
+        S = em.calculate_sparams_periodic(self.mesh, p, self.bc, self.pbc, self.ports, mode=self.mode,
+                                          solver_options=self.solver_options, _state=st)
+        self.last_info = st.get("info")
+        return S, st
+
+    def _transpose_perm(self, A):
+        """Position of entry (j, i) for every entry (i, j) of the CSR pattern of A."""
+        import torch
+        if self._tperm is not None and self._tperm[0] == A.col_idx.numel():
+            return self._tperm[1]
+        n = A.shape[0]
+        rp = A.row_ptr.to(torch.int64)
+        ci = A.col_idx.to(torch.int64)
+        rows = torch.repeat_interleave(torch.arange(n, device=ci.device), rp[1:] - rp[:-1])
+        key = rows * n + ci
+        keyT = ci * n + rows
+        pos = torch.searchsorted(key, keyT)
+        ok = bool((pos < key.numel()).all()) and bool((key[pos.clamp(max=key.numel() - 1)] == keyT).all())
+        if not ok:
+            raise RuntimeError("the reduced periodic matrix is not structurally symmetric")
+        self._tperm = (A.col_idx.numel(), pos)
+        return pos
+
+    def gradient(self, st, G: np.ndarray):
+        """dL/d eps_design (device float64 tensor) from G = dL/d conj(S), (P, P)."""
+        import torch
+        G = np.asarray(G, np.complex128).reshape(len(self.ports), len(self.ports))
+        A = st["A"]
+        n = self.mesh.num_edges()
+        P = len(self.ports)
+        m, s_, a, c, rec, act = st["pairs"]
+        # right-hand sides R^T conj(w_j): the port rows, slave entries moved onto their masters
+        rhs = np.zeros((n, P), np.complex128)
+        for j, (e, w) in enumerate(st["frees"]):
+            np.add.at(rhs[:, j], e, np.conj(w))
+        if m.size:
+            np.add.at(rhs, m, rec[:, None] * rhs[s_])
+            rhs[s_] = 0
+        R = st["nrhs"]
+        solver = _capi.KrylovSolver(A.row_ptr, A.col_idx, nrhs=R)
+        N, en, ec = st["gradient"]
+        solver.set_gradient_complex(N, en, np.conj(ec))
+        if A.lowrank:
+            gi, gv, cf = em._pack_lowrank(A.lowrank)
+            solver.set_lowrank(gi, torch.conj_physical(gv), cf)   # a conj view would pass the raw values
+        solver.set_matrix(A.values[self._transpose_perm(A)])
+        Lam, info = em._block_solve(solver, n, P, em._dense_rhs_block(rhs, R), st["tol"], st["max_iterations"],
+                                    _capi.EFM_METHOD_BICGSTAB)
+        self.last_adjoint_info = info
+        self.last_lam = Lam
+        # expansions: E^T Z lam (slave = a lam_master) and F Z x' (slave = c x'_master)
+        U = Lam.clone()
+        V = st["X_reduced"][:, :P].clone()
+        dev = U.device
+        if act.any():
+            mi = torch.from_numpy(m[act]).to(dev)
+            si = torch.from_numpy(s_[act]).to(dev)
+            U[si] = torch.from_numpy(a[act]).to(dev)[:, None] * U[mi]
+            V[si] = torch.from_numpy(c[act]).to(dev)[:, None] * V[mi]
+        # Dirichlet rows and columns of A' do not depend on eps: a Dirichlet master can pick
+        # up a slave's port weight in R^T conj(w), which its identity row passes to lam
+        dz = torch.from_numpy(np.nonzero(np.asarray(st["dmask"]) != 0)[0]).to(dev)
+        U[dz] = 0
+        V[dz] = 0
+        k0 = float(self.params.omega) / em.C0
+        tl, tr, wts = [], [], []
+        for i in range(P):
+            for j in range(P):
+                if G[j, i] != 0:
+                    tl.append(j)
+                    tr.append(i)
+                    wts.append(np.conj(G[j, i]) * k0 * k0 / st["vinc"][i])
+        if not wts:
+            return torch.zeros(self.design.size, dtype=torch.float64, device=U.device)
+        return _capi.eps_gradient(self.mesh.topology, self.mesh.dev_xyz, self.design, U.contiguous(),
+                                  V.contiguous(), tl, tr, wts)

@@ -1198,24 +1198,37 @@ template <typename VT, typename IT, int MODE>
 __global__ void __launch_bounds__(kBlock)
     k_sw_corr(int64_t nrows, int B, int nsys, const int32_t *__restrict__ crow,
               const int32_t *__restrict__ cptr, const int32_t *__restrict__ ccol,
-              const double2 *__restrict__ cval, const IT *__restrict__ u, VT *__restrict__ w,
-              double2 *__restrict__ t, double2 *__restrict__ partial, const SwState *__restrict__ st) {
+              const double2 *__restrict__ cval, int ncols, const int32_t *__restrict__ colmap,
+              const IT *__restrict__ u, VT *__restrict__ w, double2 *__restrict__ t,
+              double2 *__restrict__ partial, const SwState *__restrict__ st) {
   __shared__ double2 lds[kWaves * 64];
   const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
   const int g = blockIdx.y;
   const int s = g * kSys + lane;
   double2 v[1] = {make_double2(0, 0)};
   const bool act = s < B && (MODE == 0 ? st[s].active : st[s].out_active) != 0;
-  const bool has = s < nsys;
+  // the value column of this system (systems of one frequency share a column); padding
+  // systems read column 0 and are masked out
+  const int cs = s < nsys ? colmap[s] : 0;
+  const double z = s < nsys ? 1.0 : 0.0;
   if (__ballot(act) != 0) {
     for (int64_t m = (int64_t)blockIdx.x * kWaves + wv; m < nrows; m += (int64_t)gridDim.x * kWaves) {
       const int32_t k0 = cptr[m], k1 = cptr[m + 1];
       double2 acc = make_double2(0, 0);
-#pragma unroll 4
-      for (int32_t k = k0; k < k1; ++k) {
-        const double2 a = has ? cval[(int64_t)k * nsys + s] : make_double2(0, 0);
-        acc = cfma(a, ldv(u + (int64_t)ccol[k] * B + s), acc);
+      int32_t k = k0;
+      for (; k + 4 <= k1; k += 4) {  // four independent value/vector loads in flight
+        double2 a[4], x[4];
+#pragma unroll
+        for (int q = 0; q < 4; ++q) {
+          a[q] = cval[(int64_t)(k + q) * ncols + cs];
+          x[q] = ldv(u + (int64_t)ccol[k + q] * B + s);
+        }
+#pragma unroll
+        for (int q = 0; q < 4; ++q)
+          acc = cfma(cscale(a[q], z), x[q], acc);
       }
+      for (; k < k1; ++k)
+        acc = cfma(cscale(cval[(int64_t)k * ncols + cs], z), ldv(u + (int64_t)ccol[k] * B + s), acc);
       if (act) {
         const int64_t i = (int64_t)crow[m] * B + s;
         if (MODE == 0) {
@@ -1710,9 +1723,9 @@ struct efm_sweep {
   // per-system correction C_s of frequency-dependent media (efm_sweep_set_correction):
   // c_rows rows of a fixed sub-pattern, values [c_nnz][c_nsys] in caller memory
   int64_t c_rows = 0, c_nnz = 0;
-  int c_nsys = 0, gx_corr = 8, ncu = 256;
+  int c_nsys = 0, c_ncols = 0, gx_corr = 8, ncu = 256;
   const double2 *c_vals = nullptr;
-  efm::DevBuf c_row, c_ptr, c_col, c_map, part_c, tcorr;
+  efm::DevBuf c_row, c_ptr, c_col, c_map, part_c, tcorr, c_colmap;
   ~efm_sweep() {
     if (capture_stream)
       (void)hipStreamDestroy(capture_stream);
@@ -1935,7 +1948,8 @@ extern "C" int efm_sweep_set_correction(efm_sweep *s, int64_t n_rows, const int3
       (rc = s->c_col.alloc(std::max<int64_t>(nz, 1) * sizeof(int32_t))) ||
       (rc = s->c_map.alloc(s->n * sizeof(int32_t))) ||
       (rc = s->part_c.alloc((size_t)groups * gx * 64 * sizeof(double2))) ||
-      (rc = s->tcorr.alloc((size_t)n_rows * s->Bmax * sizeof(double2))))
+      (rc = s->tcorr.alloc((size_t)n_rows * s->Bmax * sizeof(double2))) ||
+      (rc = s->c_colmap.alloc((size_t)s->Bmax * sizeof(int32_t))))
     return rc;
   EFM_HIP_CHECK(hipMemcpyAsync(s->c_row.p, h_rows, n_rows * sizeof(int32_t), hipMemcpyHostToDevice, st));
   EFM_HIP_CHECK(hipMemcpyAsync(s->c_ptr.p, h_ptr, (n_rows + 1) * sizeof(int32_t), hipMemcpyHostToDevice, st));
@@ -1949,13 +1963,23 @@ extern "C" int efm_sweep_set_correction(efm_sweep *s, int64_t n_rows, const int3
   return EFM_OK;
 }
 
-extern "C" int efm_sweep_set_correction_values(efm_sweep *s, int nsys, const double *d_vals) {
+extern "C" int efm_sweep_set_correction_values(efm_sweep *s, int nsys, int ncols, const double *d_vals,
+                                               const int32_t *h_col_of_system, void *stream) {
   EFM_REQUIRE(s, "efm_sweep_set_correction_values: NULL sweep");
   EFM_REQUIRE(s->c_rows > 0, "efm_sweep_set_correction_values: no correction pattern set");
   EFM_REQUIRE(nsys >= 1 && nsys <= s->Bmax, "efm_sweep_set_correction_values: nsys out of range");
+  EFM_REQUIRE(ncols >= 1, "efm_sweep_set_correction_values: ncols < 1");
   EFM_REQUIRE(d_vals || s->c_nnz == 0, "efm_sweep_set_correction_values: NULL values");
+  EFM_REQUIRE(h_col_of_system, "efm_sweep_set_correction_values: NULL column map");
+  for (int i = 0; i < nsys; ++i)
+    EFM_REQUIRE(h_col_of_system[i] >= 0 && h_col_of_system[i] < ncols,
+                "efm_sweep_set_correction_values: column of a system out of range");
+  hipStream_t st = (hipStream_t)stream;
+  EFM_HIP_CHECK(hipMemcpyAsync(s->c_colmap.p, h_col_of_system, nsys * sizeof(int32_t), hipMemcpyHostToDevice, st));
+  EFM_HIP_CHECK(hipStreamSynchronize(st));
   s->c_vals = (const double2 *)d_vals;
   s->c_nsys = nsys;
+  s->c_ncols = ncols;
   return EFM_OK;
 }
 
@@ -2142,7 +2166,8 @@ struct SwRun {
     if (s->c_rows > 0) {
       hipLaunchKernelGGL((k_sw_corr<VT, VT, 0>), dim3(s->gx_corr, groups), dim3(kBlock), 0, st, s->c_rows, Bp,
                          s->c_nsys, s->c_row.as<int32_t>(), s->c_ptr.as<int32_t>(), s->c_col.as<int32_t>(),
-                         s->c_vals, u, w, (double2 *)nullptr, s->part_c.as<double2>(), S);
+                         s->c_vals, s->c_ncols, s->c_colmap.as<int32_t>(), u, w, (double2 *)nullptr,
+                         s->part_c.as<double2>(), S);
       EFM_LAUNCH_CHECK();
     }
     return EFM_OK;
@@ -2290,8 +2315,8 @@ struct SwRun {
     if (corr) {
       hipLaunchKernelGGL((k_sw_corr<VT, double2, 1>), dim3(s->gx_corr, groups), dim3(kBlock), 0, st, s->c_rows, Bp,
                          s->c_nsys, s->c_row.as<int32_t>(), s->c_ptr.as<int32_t>(), s->c_col.as<int32_t>(),
-                         s->c_vals, (const double2 *)x64, (VT *)nullptr, s->tcorr.as<double2>(),
-                         (double2 *)nullptr, S);
+                         s->c_vals, s->c_ncols, s->c_colmap.as<int32_t>(), (const double2 *)x64, (VT *)nullptr,
+                         s->tcorr.as<double2>(), (double2 *)nullptr, S);
       EFM_LAUNCH_CHECK();
     }
     hipLaunchKernelGGL((k_sw_residual<VT>), dim3(s->gx_spmv, s->gy_spmv), dim3(kBlock), 0, st, n, Bp, P, groups,

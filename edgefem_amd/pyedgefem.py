@@ -1489,7 +1489,10 @@ def calculate_sparams_periodic(mesh: Mesh, params: MaxwellParams, bc: BC, pbc: P
             V = np.sum(np.conj(w) * Xh[e, i])
             S[j, i] = (V - vinc) / vinc if i == j else V / vinc
     if _state is not None:
-        _state.update(dict(A=Am, info=info, X=Xh))
+        _state.update(dict(A=Am, info=info, X=Xh, X_reduced=X, solver=solver, frees=frees,
+                           pairs=(m, s_, a, c, rec, act), dmask=dmask, gradient=(N, en, ec),
+                           vinc=[cmath.sqrt(complex(p.mode.Z0)) for p in ports], nrhs=R, tol=tol,
+                           max_iterations=int(opts.max_iterations)))
     return S
 
 
@@ -2170,13 +2173,17 @@ class _MediaCorrection:
         return self._cache[key]
 
     def values(self, freqs, freq_index, k0sq):
-        """(entries, nsys) complex128 device values for systems at freqs[freq_index[i]]."""
+        """(entries, ncols) complex128 device values, one column per distinct frequency of
+        the systems, and the column of every system (systems at freqs[freq_index[i]])."""
         import torch
-        out = torch.empty((self.nnz, len(freq_index)), dtype=torch.complex128, device=self.pos.device)
-        for c, (i, k2) in enumerate(zip(freq_index, k0sq)):
+        order = list(dict.fromkeys(int(i) for i in freq_index))
+        col = {f: c for c, f in enumerate(order)}
+        k2_of = {int(i): float(k2) for i, k2 in zip(freq_index, k0sq)}
+        out = torch.empty((len(order), self.nnz), dtype=torch.complex128, device=self.pos.device)
+        for c, i in enumerate(order):
             kp, mp = self._entries(2.0 * math.pi * float(freqs[i]))
-            out[:, c] = kp - k2 * mp
-        return out
+            out[c] = kp - k2_of[i] * mp
+        return out.t().contiguous(), np.array([col[int(i)] for i in freq_index], np.int32)
 
 
 def frequency_sweep_eigenmode(mesh: Mesh, params: MaxwellParams, bc: BC, ports: Sequence[WavePort],
@@ -2277,7 +2284,7 @@ def frequency_sweep_eigenmode(mesh: Mesh, params: MaxwellParams, bc: BC, ports: 
             flat = torch.from_numpy(np.concatenate(rows_l) * (b1 - b0) + np.concatenate(cols_l)).to(dev)
             Bd.view(-1).index_put_((flat,), torch.from_numpy(np.concatenate(vals_l)).to(dev))
         if corr is not None:
-            sweep.set_correction_values(corr.values(freqs, [sys_f[si] for si in idx], [sys_k2[si] for si in idx]))
+            sweep.set_correction_values(*corr.values(freqs, [sys_f[si] for si in idx], [sys_k2[si] for si in idx]))
         X, info = sweep.solve([sys_k2[si] for si in idx], [sys_coef[si] for si in idx], Bd, tol=tol,
                               max_iterations=int(opts.max_iterations), precision=precision,
                               replace_factor=float((_state or {}).get("replace_factor", 0.01)))

@@ -312,13 +312,15 @@ int efm_sweep_set_graph(efm_sweep *sw, int enable);
  * frequency_sweep (src/sweep.cpp:187-190, 330-346) when PML or a dispersive model is present.
  * set_correction: correction row m is matrix row h_rows[m] (each row at most once), with
  * entries h_ptr[m]..h_ptr[m+1]-1 (h_ptr[0] = 0) in columns h_cols (host arrays, copied);
- * n_rows = 0 removes the correction.  set_correction_values: d_vals c128 [entry][nsys]
- * in device memory owned by the caller, read by every later solve of nsys systems until it
- * is set again.  Needs the standard COCG form (0 or 2).  No reference counterpart (the
- * reference re-assembles and solves each frequency). */
+ * n_rows = 0 removes the correction.  set_correction_values: d_vals c128 [entry][ncols]
+ * in device memory owned by the caller; system s of the next solves (nsys systems) uses
+ * column h_col_of_system[s] (systems of one frequency share a column); read by every later
+ * solve until it is set again.  Needs the standard COCG form (0 or 2).  No reference
+ * counterpart (the reference re-assembles and solves each frequency). */
 int efm_sweep_set_correction(efm_sweep *sw, int64_t n_rows, const int32_t *h_rows,
                              const int32_t *h_ptr, const int32_t *h_cols, void *stream);
-int efm_sweep_set_correction_values(efm_sweep *sw, int nsys, const double *d_vals);
+int efm_sweep_set_correction_values(efm_sweep *sw, int nsys, int ncols, const double *d_vals,
+                                    const int32_t *h_col_of_system, void *stream);
 
 /* ------------------------------------------------------------------------
  * Periodic (Floquet) elimination on the sparse matrix: replaces the dense elimination of

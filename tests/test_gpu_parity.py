@@ -838,6 +838,89 @@ def test_periodic_cell_sparams_vs_oracle(torch_dev, mode):
     assert np.abs(S - So).max() <= 1e-6
 
 
+@pytest.mark.parametrize("mode", ["reference", "consistent"])
+def test_periodic_adjoint_gradient_vs_central_differences(torch_dev, mode):
+    """The permittivity gradient of a periodic unit cell (25 degree Floquet phase, lumped
+    ports, rank-1 loading; a non-symmetric reduced system): the adjoint of
+    PeriodicEpsDesignProblem (transposed reduced matrix, conjugated port vectors) for a
+    phase-sensitive loss L = sum |S - T|^2 through torch.autograd, against central
+    differences of the oracle's S (SuperLU on the oracle's elimination, h = 1e-6 relative)
+    and of the GPU forward (h = 1e-4), in both coefficient conventions."""
+    import torch
+    from edgefem_amd import adjoint as adj
+    from edgefem_amd import pyedgefem as em
+    from oracle import edgefem_oracle as O
+    _p(f"[gpu] periodic adjoint gradient, mode={mode}")
+    g, pm, om = _cell_setup()
+    bc = em.build_edge_pec(pm, 1)
+    pec = O.build_edge_pec(om, 1)
+    pbc = em.build_periodic_pairs(pm, 10, 11, np.array([5e-3, 0.0, 0.0]))
+    opbc = O.build_periodic_pairs(om, 10, 11, (5e-3, 0, 0))
+    f = 10e9
+    k0 = 2 * math.pi * f / em.C0
+    em.set_floquet_phase(pbc, np.array([k0 * math.sin(math.radians(25)), 0.0]))
+    opbc.phase = pbc.phase_shift
+    ports, oports = [], []
+    for tag in (2, 3):
+        cfg = em.LumpedPortConfig()
+        cfg.surface_tag = tag
+        cfg.z0 = 376.73
+        cfg.e_direction = np.array([0.0, 1.0, 0.0])
+        ports.append(em.build_lumped_port(pm, cfg))
+        oports.append(O.build_lumped_port(om, tag, 376.73, (0, 1, 0)))
+    params = em.MaxwellParams()
+    params.omega = 2 * math.pi * f
+    params.eps_r_regions = {101: 3.5 * (1 - 0.002j)}
+    zc = g.centroids()[:, 2]
+    H = float(g.xyz[:, 2].max())
+    design = np.nonzero((zc > 0.15 * H) & (zc < 0.8 * H))[0]
+    rng = np.random.default_rng(17)
+    eps0 = 1.0 + 2.0 * rng.random(design.size)
+    opts = em.SolveOptions()
+    opts.tolerance = 1e-13
+    prob = adj.PeriodicEpsDesignProblem(pm, params, bc, pbc, ports, design, mode=mode, solver_options=opts)
+    T = rng.normal(size=(2, 2)) + 1j * rng.normal(size=(2, 2))
+    e = torch.tensor(eps0, dtype=torch.float64, device="cuda", requires_grad=True)
+    S = adj.SParamsOfPermittivity.apply(e, prob)
+    L = ((S - torch.tensor(T, dtype=torch.complex128, device="cuda")).abs() ** 2).sum()
+    L.backward()
+    grad = e.grad.cpu().numpy()
+    _p(f"    adjoint iterations {prob.last_adjoint_info['iters']}; max |grad| {np.abs(grad).max():.3e}")
+    base = prob.permittivity(eps0)
+
+    def oracle_loss(eps_full):
+        op = O.Params(omega=params.omega, eps_tet=eps_full)
+        So = O.calculate_sparams_periodic(om, op, pec, opbc, oports, mode=mode)
+        return float((np.abs(So - T) ** 2).sum())
+    assert abs(float(L.detach()) - oracle_loss(base)) <= 1e-8 * max(1.0, float(L.detach()))
+    # along a seeded direction over every design tet (h = 1e-4 in eps_r)
+    v = rng.choice([-1.0, 1.0], size=design.size)
+    h = 1e-4
+    ep, em_ = base.copy(), base.copy()
+    ep[design] += h * v
+    em_[design] -= h * v
+    fd = (oracle_loss(ep) - oracle_loss(em_)) / (2 * h)
+    _p(f"    direction: adjoint {grad @ v:.9e} oracle central difference {fd:.9e}")
+    assert abs(grad @ v - fd) <= 1e-5 * abs(fd)
+    # per tet on the largest entries (h = 1e-4 relative)
+    for k in np.argsort(-np.abs(grad))[:3]:
+        h = 1e-4 * eps0[k]
+        ep, em_ = base.copy(), base.copy()
+        ep[design[k]] += h
+        em_[design[k]] -= h
+        fd = (oracle_loss(ep) - oracle_loss(em_)) / (2 * h)
+        _p(f"    tet {design[k]}: adjoint {grad[k]:.9e} oracle central difference {fd:.9e}")
+        assert abs(grad[k] - fd) <= 1e-5 * abs(fd)
+    k = int(np.argmax(np.abs(grad)))
+    h = 1e-4 * eps0[k]
+    ep, em_ = eps0.copy(), eps0.copy()
+    ep[k] += h
+    em_[k] -= h
+    fd = (float((np.abs(prob.solve(ep)[0] - T) ** 2).sum()) - float((np.abs(prob.solve(em_)[0] - T) ** 2).sum())) / (2 * h)
+    _p(f"    tet {design[k]} (largest): adjoint {grad[k]:.9e} GPU central difference {fd:.9e}")
+    assert abs(grad[k] - fd) <= 1e-5 * abs(fd)
+
+
 def test_nodal_port_modes_and_wave_port_vs_oracle(torch_dev):
     """extract_surface_mesh, solve_port_eigens (TE and TM) and build_wave_port on the ports
     of examples/rect_waveguide.msh against the oracle: cutoffs, Z0, fields and weights;
