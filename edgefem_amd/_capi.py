@@ -37,7 +37,7 @@ EXPORTS = [
     "efm_gmsh_read", "efm_gmsh_sizes", "efm_gmsh_copy", "efm_gmsh_free",
     "efm_touchstone_write", "efm_touchstone_write_2port", "efm_touchstone_extension",
     "efm_port_overlap_noalloc", "efm_sweep_set_graph", "efm_sweep_set_correction",
-    "efm_sweep_set_correction_values",
+    "efm_sweep_set_correction_values", "efm_sweep_set_correction_terms",
     "efm_array_package_json", "efm_patterns_csv", "efm_coupling_csv", "efm_scan_csv",
 ]
 
@@ -123,6 +123,7 @@ def load_library(path: str = LIB_PATH) -> C.CDLL:
     lib.efm_sweep_set_iteration.argtypes = [vp, i32]
     lib.efm_sweep_set_correction.argtypes = [vp, i64, vp, vp, vp, vp]
     lib.efm_sweep_set_correction_values.argtypes = [vp, i32, i32, vp, vp, vp]
+    lib.efm_sweep_set_correction_terms.argtypes = [vp, i32, vp, i32, vp, vp]
     lib.efm_tet_fields.argtypes = [vp, vp, i32, vp, vp, vp, vp, vp]
     lib.efm_gmsh_read.argtypes = [C.c_char_p, i32, C.POINTER(vp)]
     lib.efm_gmsh_sizes.argtypes = [vp, vp]
@@ -519,6 +520,19 @@ class SweepSolver:
         _check(self._lib.efm_sweep_set_correction(self._h, int(r.size), r.ctypes.data_as(C.c_void_p),
                                                   pp.ctypes.data_as(C.c_void_p), cc.ctypes.data_as(C.c_void_p),
                                                   stream_ptr()), "efm_sweep_set_correction")
+
+    def set_correction_terms(self, terms, alpha) -> None:
+        """Shared correction terms: terms a complex128 device tensor (entries, 8) (columns
+        past len(alpha[0]) zero), alpha (nsys, nterms) complex weights per system."""
+        torch = _torch()
+        t = to_dev(terms, torch.complex128)
+        al = np.ascontiguousarray(np.asarray(alpha, np.complex128))
+        if t.dim() != 2 or t.shape[1] != 8 or al.ndim != 2:
+            raise EdgeFemError("correction terms must be (entries, 8) and weights (nsys, nterms)")
+        self._corr_vals = t
+        _check(self._lib.efm_sweep_set_correction_terms(self._h, int(al.shape[1]), dptr(t), int(al.shape[0]),
+                                                        al.ctypes.data_as(C.c_void_p), stream_ptr()),
+               "efm_sweep_set_correction_terms")
 
     def set_correction_values(self, vals, col_of_system) -> None:
         """Values of the corrections: a complex128 device tensor (entries, ncols), kept

@@ -1194,23 +1194,44 @@ __global__ void k_sw_imag_flags(int64_t nnz, const double2 *__restrict__ K, cons
 // partials of u^T C_s u per block (added to the SpMV's partials by the finalize).
 // MODE 1 (before a residual, systems with out_active set): t[m] = C_s x for FP64 x.
 // One wavefront per correction row and group of 64 systems (lane = system).
-template <typename VT, typename IT, int MODE>
+// Values: TERMS = false, cval[entry][ncols] with system s reading column colmap[s] (the
+// systems of one frequency share a column); TERMS = true, C_s = sum_q alpha[s][q] C_q with
+// kMaxTerms shared matrices cterm[entry][kMaxTerms] (unused terms zero): the terms of an
+// entry are one wave-uniform 128-byte read, the per-system values are formed in registers.
+constexpr int kMaxTerms = 8;
+template <typename VT, typename IT, int MODE, bool TERMS>
 __global__ void __launch_bounds__(kBlock)
     k_sw_corr(int64_t nrows, int B, int nsys, const int32_t *__restrict__ crow,
               const int32_t *__restrict__ cptr, const int32_t *__restrict__ ccol,
               const double2 *__restrict__ cval, int ncols, const int32_t *__restrict__ colmap,
+              const double2 *__restrict__ cterm, const double2 *__restrict__ alpha,
               const IT *__restrict__ u, VT *__restrict__ w, double2 *__restrict__ t,
               double2 *__restrict__ partial, const SwState *__restrict__ st) {
   __shared__ double2 lds[kWaves * 64];
-  const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
+  const int lane = threadIdx.x & 63;
+  const int wv = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
   const int g = blockIdx.y;
   const int s = g * kSys + lane;
   double2 v[1] = {make_double2(0, 0)};
   const bool act = s < B && (MODE == 0 ? st[s].active : st[s].out_active) != 0;
-  // the value column of this system (systems of one frequency share a column); padding
-  // systems read column 0 and are masked out
-  const int cs = s < nsys ? colmap[s] : 0;
+  // the value column of this system; padding systems read column 0 and are masked out
+  const int cs = (!TERMS && s < nsys) ? colmap[s] : 0;
   const double z = s < nsys ? 1.0 : 0.0;
+  double2 al[TERMS ? kMaxTerms : 1];
+#pragma unroll
+  for (int q = 0; q < (TERMS ? kMaxTerms : 1); ++q)
+    al[q] = (TERMS && s < nsys) ? alpha[(int64_t)s * kMaxTerms + q] : make_double2(0, 0);
+  auto value = [&](int32_t k) -> double2 {
+    if constexpr (TERMS) {
+      double2 a = make_double2(0, 0);
+#pragma unroll
+      for (int q = 0; q < kMaxTerms; ++q)
+        a = cfma(al[q], cterm[(int64_t)k * kMaxTerms + q], a);
+      return a;
+    } else {
+      return cscale(cval[(int64_t)k * ncols + cs], z);
+    }
+  };
   if (__ballot(act) != 0) {
     for (int64_t m = (int64_t)blockIdx.x * kWaves + wv; m < nrows; m += (int64_t)gridDim.x * kWaves) {
       const int32_t k0 = cptr[m], k1 = cptr[m + 1];
@@ -1220,15 +1241,15 @@ __global__ void __launch_bounds__(kBlock)
         double2 a[4], x[4];
 #pragma unroll
         for (int q = 0; q < 4; ++q) {
-          a[q] = cval[(int64_t)(k + q) * ncols + cs];
+          a[q] = value(k + q);
           x[q] = ldv(u + (int64_t)ccol[k + q] * B + s);
         }
 #pragma unroll
         for (int q = 0; q < 4; ++q)
-          acc = cfma(cscale(a[q], z), x[q], acc);
+          acc = cfma(a[q], x[q], acc);
       }
       for (; k < k1; ++k)
-        acc = cfma(cscale(cval[(int64_t)k * ncols + cs], z), ldv(u + (int64_t)ccol[k] * B + s), acc);
+        acc = cfma(value(k), ldv(u + (int64_t)ccol[k] * B + s), acc);
       if (act) {
         const int64_t i = (int64_t)crow[m] * B + s;
         if (MODE == 0) {
@@ -1724,8 +1745,9 @@ struct efm_sweep {
   // c_rows rows of a fixed sub-pattern, values [c_nnz][c_nsys] in caller memory
   int64_t c_rows = 0, c_nnz = 0;
   int c_nsys = 0, c_ncols = 0, gx_corr = 8, ncu = 256;
+  int c_terms = 0;  // 1: c_vals holds shared terms [entry][8] and c_alpha [system][8] their weights
   const double2 *c_vals = nullptr;
-  efm::DevBuf c_row, c_ptr, c_col, c_map, part_c, tcorr, c_colmap;
+  efm::DevBuf c_row, c_ptr, c_col, c_map, part_c, tcorr, c_colmap, c_alpha;
   ~efm_sweep() {
     if (capture_stream)
       (void)hipStreamDestroy(capture_stream);
@@ -1949,7 +1971,8 @@ extern "C" int efm_sweep_set_correction(efm_sweep *s, int64_t n_rows, const int3
       (rc = s->c_map.alloc(s->n * sizeof(int32_t))) ||
       (rc = s->part_c.alloc((size_t)groups * gx * 64 * sizeof(double2))) ||
       (rc = s->tcorr.alloc((size_t)n_rows * s->Bmax * sizeof(double2))) ||
-      (rc = s->c_colmap.alloc((size_t)s->Bmax * sizeof(int32_t))))
+      (rc = s->c_colmap.alloc((size_t)s->Bmax * sizeof(int32_t))) ||
+      (rc = s->c_alpha.alloc((size_t)s->Bmax * kMaxTerms * sizeof(double2))))
     return rc;
   EFM_HIP_CHECK(hipMemcpyAsync(s->c_row.p, h_rows, n_rows * sizeof(int32_t), hipMemcpyHostToDevice, st));
   EFM_HIP_CHECK(hipMemcpyAsync(s->c_ptr.p, h_ptr, (n_rows + 1) * sizeof(int32_t), hipMemcpyHostToDevice, st));
@@ -1980,6 +2003,29 @@ extern "C" int efm_sweep_set_correction_values(efm_sweep *s, int nsys, int ncols
   s->c_vals = (const double2 *)d_vals;
   s->c_nsys = nsys;
   s->c_ncols = ncols;
+  s->c_terms = 0;
+  return EFM_OK;
+}
+
+extern "C" int efm_sweep_set_correction_terms(efm_sweep *s, int nterms, const double *d_terms, int nsys,
+                                              const double *h_alpha, void *stream) {
+  EFM_REQUIRE(s, "efm_sweep_set_correction_terms: NULL sweep");
+  EFM_REQUIRE(s->c_rows > 0, "efm_sweep_set_correction_terms: no correction pattern set");
+  EFM_REQUIRE(nterms >= 1 && nterms <= kMaxTerms, "efm_sweep_set_correction_terms: 1..8 terms");
+  EFM_REQUIRE(nsys >= 1 && nsys <= s->Bmax, "efm_sweep_set_correction_terms: nsys out of range");
+  EFM_REQUIRE((d_terms || s->c_nnz == 0) && h_alpha, "efm_sweep_set_correction_terms: NULL argument");
+  std::vector<double2> al((size_t)nsys * kMaxTerms, make_double2(0, 0));
+  for (int i = 0; i < nsys; ++i)
+    for (int q = 0; q < nterms; ++q)
+      al[(size_t)i * kMaxTerms + q] = make_double2(h_alpha[2 * ((size_t)i * nterms + q)],
+                                                   h_alpha[2 * ((size_t)i * nterms + q) + 1]);
+  hipStream_t st = (hipStream_t)stream;
+  EFM_HIP_CHECK(hipMemcpyAsync(s->c_alpha.p, al.data(), al.size() * sizeof(double2), hipMemcpyHostToDevice, st));
+  EFM_HIP_CHECK(hipStreamSynchronize(st));
+  s->c_vals = (const double2 *)d_terms;
+  s->c_nsys = nsys;
+  s->c_ncols = kMaxTerms;
+  s->c_terms = 1;
   return EFM_OK;
 }
 
@@ -2164,10 +2210,11 @@ struct SwRun {
                          s->rport.as<int8_t>(), s->k0sq.as<double>(), s->coef.as<double2>(), u, w, ps, S);
     EFM_LAUNCH_CHECK();
     if (s->c_rows > 0) {
-      hipLaunchKernelGGL((k_sw_corr<VT, VT, 0>), dim3(s->gx_corr, groups), dim3(kBlock), 0, st, s->c_rows, Bp,
-                         s->c_nsys, s->c_row.as<int32_t>(), s->c_ptr.as<int32_t>(), s->c_col.as<int32_t>(),
-                         s->c_vals, s->c_ncols, s->c_colmap.as<int32_t>(), u, w, (double2 *)nullptr,
-                         s->part_c.as<double2>(), S);
+      auto kc = s->c_terms ? k_sw_corr<VT, VT, 0, true> : k_sw_corr<VT, VT, 0, false>;
+      hipLaunchKernelGGL(kc, dim3(s->gx_corr, groups), dim3(kBlock), 0, st, s->c_rows, Bp, s->c_nsys,
+                         s->c_row.as<int32_t>(), s->c_ptr.as<int32_t>(), s->c_col.as<int32_t>(), s->c_vals,
+                         s->c_ncols, s->c_colmap.as<int32_t>(), s->c_vals, s->c_alpha.as<double2>(), u, w,
+                         (double2 *)nullptr, s->part_c.as<double2>(), S);
       EFM_LAUNCH_CHECK();
     }
     return EFM_OK;
@@ -2313,10 +2360,11 @@ struct SwRun {
     EFM_LAUNCH_CHECK();
     const bool corr = s->c_rows > 0;
     if (corr) {
-      hipLaunchKernelGGL((k_sw_corr<VT, double2, 1>), dim3(s->gx_corr, groups), dim3(kBlock), 0, st, s->c_rows, Bp,
-                         s->c_nsys, s->c_row.as<int32_t>(), s->c_ptr.as<int32_t>(), s->c_col.as<int32_t>(),
-                         s->c_vals, s->c_ncols, s->c_colmap.as<int32_t>(), (const double2 *)x64, (VT *)nullptr,
-                         s->tcorr.as<double2>(), (double2 *)nullptr, S);
+      auto kc = s->c_terms ? k_sw_corr<VT, double2, 1, true> : k_sw_corr<VT, double2, 1, false>;
+      hipLaunchKernelGGL(kc, dim3(s->gx_corr, groups), dim3(kBlock), 0, st, s->c_rows, Bp, s->c_nsys,
+                         s->c_row.as<int32_t>(), s->c_ptr.as<int32_t>(), s->c_col.as<int32_t>(), s->c_vals,
+                         s->c_ncols, s->c_colmap.as<int32_t>(), s->c_vals, s->c_alpha.as<double2>(),
+                         (const double2 *)x64, (VT *)nullptr, s->tcorr.as<double2>(), (double2 *)nullptr, S);
       EFM_LAUNCH_CHECK();
     }
     hipLaunchKernelGGL((k_sw_residual<VT>), dim3(s->gx_spmv, s->gy_spmv), dim3(kBlock), 0, st, n, Bp, P, groups,

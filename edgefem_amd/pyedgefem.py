@@ -2157,6 +2157,47 @@ class _MediaCorrection:
         self.pos = torch.from_numpy(pos).to(topo.row_ptr.device)
         self.nnz = int(key.size)
         self._cache = {}
+        # Groups of tets that vary with the frequency in the same way: same tag and the same
+        # (eps s, 1/(mu s)) at the reference frequency, so the same model, stretch and static
+        # values.  With at most four groups every system's correction is a combination of
+        # each group's K and M parts (two shared terms per group, _terms), read once per
+        # entry instead of once per frequency.
+        vt = self.tets
+        keys = np.stack([mesh.tet_phys[vt].astype(np.float64), self.es_ref[vt].real, self.es_ref[vt].imag,
+                         self.ims_ref[vt].real, self.ims_ref[vt].imag], axis=1) if vt.size else np.zeros((0, 5))
+        uk, first, inv = np.unique(keys, axis=0, return_index=True, return_inverse=True)
+        self.group_rep = vt[first]
+        self.group_of = inv.reshape(-1)
+        self.terms = None
+        if 0 < uk.shape[0] <= 4:
+            T = mesh.num_tets()
+            tcols = torch.zeros((self.nnz, 8), dtype=torch.complex128, device=self.pos.device)
+            for gidx in range(uk.shape[0]):
+                on = np.zeros(T, np.complex128)
+                on[vt[self.group_of == gidx]] = 1.0
+                Kg, Mg = _capi.assemble_km(topo, mesh.dev_xyz, on, on, dmask)
+                tcols[:, 2 * gidx] = Kg[self.pos]
+                tcols[:, 2 * gidx + 1] = Mg[self.pos]
+            self.terms = tcols
+
+    def weights(self, freqs, freq_index, k0sq):
+        """(nsys, 2 groups) weights of the shared terms: per group (1/(mu s)(w) - 1/(mu s)(w_ref))
+        and -k0^2 ((eps s)(w) - (eps s)(w_ref)) of its representative tet."""
+        reps = self.group_rep
+        out = np.zeros((len(freq_index), 2 * reps.size), np.complex128)
+        cache = {}
+        for c, (i, k2) in enumerate(zip(freq_index, k0sq)):
+            i = int(i)
+            if i not in cache:
+                q = _params_at(self.params, 2.0 * math.pi * float(freqs[i]))
+                es_all, ms_all, _ = _assembly_materials(self.mesh, q)   # a tensor PML needs its whole region
+                es = es_all[reps]
+                ims = 1.0 / ms_all[reps]
+                cache[i] = (ims - self.ims_ref[reps], es - self.es_ref[reps])
+            dk, de = cache[i]
+            out[c, 0::2] = dk
+            out[c, 1::2] = -float(k2) * de
+        return out
 
     def _entries(self, omega: float):
         """C(omega) without the k0^2 factor on its M part: (K part, M part) on the sub-pattern."""
@@ -2283,7 +2324,10 @@ def frequency_sweep_eigenmode(mesh: Mesh, params: MaxwellParams, bc: BC, ports: 
         if rows_l:
             flat = torch.from_numpy(np.concatenate(rows_l) * (b1 - b0) + np.concatenate(cols_l)).to(dev)
             Bd.view(-1).index_put_((flat,), torch.from_numpy(np.concatenate(vals_l)).to(dev))
-        if corr is not None:
+        if corr is not None and corr.terms is not None and not (_state or {}).get("correction_columns"):
+            sweep.set_correction_terms(corr.terms, corr.weights(freqs, [sys_f[si] for si in idx],
+                                                                [sys_k2[si] for si in idx]))
+        elif corr is not None:
             sweep.set_correction_values(*corr.values(freqs, [sys_f[si] for si in idx], [sys_k2[si] for si in idx]))
         X, info = sweep.solve([sys_k2[si] for si in idx], [sys_coef[si] for si in idx], Bd, tol=tol,
                               max_iterations=int(opts.max_iterations), precision=precision,
@@ -2313,7 +2357,9 @@ def frequency_sweep_eigenmode(mesh: Mesh, params: MaxwellParams, bc: BC, ports: 
     if _state is not None:
         v, a, tune_ms = sweep.variants()
         _state.update(dict(infos=infos, sweep=sweep, ops=ops, spmv_chosen=v, aux_chosen=a, tune_ms=tune_ms,
-                           correction_entries=0 if corr is None else corr.nnz))
+                           correction_entries=0 if corr is None else corr.nnz,
+                           correction_form=None if corr is None else
+                           ("terms" if corr.terms is not None and not _state.get("correction_columns") else "columns")))
     return out
 
 

@@ -321,6 +321,14 @@ int efm_sweep_set_correction(efm_sweep *sw, int64_t n_rows, const int32_t *h_row
                              const int32_t *h_ptr, const int32_t *h_cols, void *stream);
 int efm_sweep_set_correction_values(efm_sweep *sw, int nsys, int ncols, const double *d_vals,
                                     const int32_t *h_col_of_system, void *stream);
+/* The same correction as a few shared matrices with per-system weights,
+ *   C_s = sum_q h_alpha[s][q] T_q,   q < nterms <= 8,
+ * d_terms c128 [entry][8] in caller device memory (columns past nterms zero), h_alpha c128
+ * [nsys][nterms] on the host: where the media vary by region (a dispersive model per tag, a
+ * scalar PML stretch per region) every system's correction is a combination of each
+ * region's K and M parts, and the values read per entry drop from one per frequency to 8. */
+int efm_sweep_set_correction_terms(efm_sweep *sw, int nterms, const double *d_terms, int nsys,
+                                   const double *h_alpha, void *stream);
 
 /* ------------------------------------------------------------------------
  * Periodic (Floquet) elimination on the sparse matrix: replaces the dense elimination of

@@ -1164,8 +1164,14 @@ def test_dispersive_and_pml_eigenmode_sweep_vs_oracle(torch_dev):
                                          solver_options=opts)
     assert np.abs(S10 - Spf[1]).max() == 0.0
     dpf = np.abs(S - Spf).max()
-    _p(f"    batched vs per-frequency |dS| {dpf:.2e}")
+    _p(f"    batched vs per-frequency |dS| {dpf:.2e} (correction as {st['correction_form']})")
     assert dpf <= 1e-8
+    assert st["correction_form"] == "terms"          # two groups: the Debye slab and the PML slab
+    Sc = em.frequency_sweep_eigenmode(pm, params, bc, ports, freqs, solver_options=opts,
+                                      _state={"correction_columns": True})
+    dcol = np.abs(S - Sc).max()
+    _p(f"    shared terms vs per-frequency columns |dS| {dcol:.2e}")
+    assert dcol <= 1e-8
 
 
 def test_batched_media_sweep_many_systems_vs_per_frequency(torch_dev):
@@ -1215,7 +1221,7 @@ def test_batched_media_sweep_many_systems_vs_per_frequency(torch_dev):
     S = em.frequency_sweep_eigenmode(pm, params, bc, ports, freqs, solver_options=opts, max_batch=48, _state=st)
     d64 = np.abs(S - Spf).max()
     it = np.concatenate([i["iters"] for i in st["infos"]])
-    _p(f"    FP64 batched vs per-frequency |dS| {d64:.2e}; iterations {it.min()}-{it.max()}; "
+    _p(f"    FP64 batched ({st['correction_form']}) vs per-frequency |dS| {d64:.2e}; iterations {it.min()}-{it.max()}; "
        f"|S21| {np.abs(S[:, 1, 0]).min():.4f}-{np.abs(S[:, 1, 0]).max():.4f}")
     assert d64 <= 1e-8
     assert np.all(np.abs(S[:, 1, 0]) < 0.999)
