@@ -616,10 +616,17 @@ def cells_main(args):
         cpu = dict(value=2 * len(pick) / tcpu, unit="S-solves/s", cores=1, kind="port",
                    sample=f"oracle (assembly, sparse periodic elimination, SuperLU per excitation) on the "
                           f"{len(pick)} smallest cells, 1 host core, {tcpu:.1f} s")
+    # the committed FP64 fixture of 8 of these cells (the smallest, the largest and six between)
+    dS_gold = None
+    gpath = os.path.join(ROOT, "tests", "golden", "c4_cells.npz")
+    if os.path.exists(gpath) and ncell == 512 and not args.small:
+        gold = np.load(gpath)
+        dS_gold = max(float(np.abs(S[int(c)] - Sg).max()) for c, Sg in zip(gold["cells"], gold["S"]))
     result = {
         "metric": METRIC, "value": nsol / el, "unit": "S-solves/s", "n_gpus": 1, "steps": args.steps,
         "warmup": args.warmup, "ms_per_step": 1e3 * el, "higher_is_better": True, "scaling": "weak",
         "vs_baseline": None, "dtype": "c128", "data": "synthetic (seeded unit cells)",
+        "delta_S_vs_fp64_golden_8_cells": dS_gold,
         "config": {"workload": f"unit_cells_{ncell}_uneven_bloch_x", "cells": ncell, "tets": int(tets),
                    "rows": int(st["n"]), "nnz": int(st["nnz"]), "freq_hz": f, "tol": args.tol,
                    "solver": "batched BiCGSTAB + periodic auxiliary preconditioner, per-system masks"},
