@@ -115,7 +115,24 @@ __device__ inline void reduce_all(const double2 *__restrict__ partial, int nb, d
 #pragma unroll
   for (int s = 0; s < V; ++s)
     v[s] = make_double2(0.0, 0.0);
-  for (int b = threadIdx.x; b < nb; b += kBlock) {
+  // U blocks' partials are loaded before any is added, so their round trips overlap; the
+  // additions keep the order b, b + kBlock, ... of the plain loop (same sums, bit for bit)
+  constexpr int U = V >= 32 ? 1 : (32 / V > 8 ? 8 : 32 / V);
+  int b = threadIdx.x;
+  for (; b + (U - 1) * kBlock < nb; b += U * kBlock) {
+    double2 t[U][V];
+#pragma unroll
+    for (int u = 0; u < U; ++u)
+#pragma unroll
+      for (int s = 0; s < V; ++s)
+        t[u][s] = partial[(int64_t)(b + u * kBlock) * V + s];
+#pragma unroll
+    for (int u = 0; u < U; ++u)
+#pragma unroll
+      for (int s = 0; s < V; ++s)
+        v[s] = cadd(v[s], t[u][s]);
+  }
+  for (; b < nb; b += kBlock) {
 #pragma unroll
     for (int s = 0; s < V; ++s)
       v[s] = cadd(v[s], partial[(int64_t)b * V + s]);

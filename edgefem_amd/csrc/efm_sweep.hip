@@ -1498,7 +1498,23 @@ __global__ void __launch_bounds__(kBlock)
 #pragma unroll
   for (int k = 0; k < V; ++k)
     v[k] = make_double2(0, 0);
-  for (int bx = threadIdx.x; bx < gx; bx += kBlock) {
+  // four blocks' partials loaded before any is added (their round trips overlap), added in
+  // the order of the plain loop
+  int bx = threadIdx.x;
+  for (; bx + 3 * kBlock < gx; bx += 4 * kBlock) {
+    double2 t[4][V];
+#pragma unroll
+    for (int u = 0; u < 4; ++u)
+#pragma unroll
+      for (int k = 0; k < V; ++k)
+        t[u][k] = part[(((int64_t)gy * gx + bx + u * kBlock) * V + k) * 64 + lane];
+#pragma unroll
+    for (int u = 0; u < 4; ++u)
+#pragma unroll
+      for (int k = 0; k < V; ++k)
+        v[k] = cadd(v[k], t[u][k]);
+  }
+  for (; bx < gx; bx += kBlock) {
     const int64_t bl = (int64_t)gy * gx + bx;
 #pragma unroll
     for (int k = 0; k < V; ++k)
