@@ -838,14 +838,17 @@ def test_periodic_cell_sparams_vs_oracle(torch_dev, mode):
     assert np.abs(S - So).max() <= 1e-6
 
 
-@pytest.mark.parametrize("mode", ["reference", "consistent"])
-def test_periodic_adjoint_gradient_vs_central_differences(torch_dev, mode):
-    """The permittivity gradient of a periodic unit cell (25 degree Floquet phase, lumped
-    ports, rank-1 loading; a non-symmetric reduced system): the adjoint of
-    PeriodicEpsDesignProblem (transposed reduced matrix, conjugated port vectors) for a
-    phase-sensitive loss L = sum |S - T|^2 through torch.autograd, against central
-    differences of the oracle's S (SuperLU on the oracle's elimination, h = 1e-6 relative)
-    and of the GPU forward (h = 1e-4), in both coefficient conventions."""
+@pytest.mark.parametrize("mode,port_kind", [("reference", "lumped"), ("consistent", "lumped"),
+                                            ("reference", "wave")])
+def test_periodic_adjoint_gradient_vs_central_differences(torch_dev, mode, port_kind):
+    """The permittivity gradient of a periodic unit cell (25 degree Floquet phase, rank-1
+    port loading; a non-symmetric reduced system): the adjoint of PeriodicEpsDesignProblem
+    (transposed reduced matrix, conjugated port vectors) for a phase-sensitive loss
+    L = sum |S - T|^2 through torch.autograd, against central differences of the oracle's S
+    (SuperLU on the oracle's elimination) along a seeded direction and per tet on the
+    largest entries, and of the GPU forward, in both coefficient conventions; with lumped
+    ports, and with the unit-cell caller's ports (the first nodal TE mode of each face as a
+    wave port, use_port_abc)."""
     import torch
     from edgefem_amd import adjoint as adj
     from edgefem_amd import pyedgefem as em
@@ -862,15 +865,24 @@ def test_periodic_adjoint_gradient_vs_central_differences(torch_dev, mode):
     opbc.phase = pbc.phase_shift
     ports, oports = [], []
     for tag in (2, 3):
-        cfg = em.LumpedPortConfig()
-        cfg.surface_tag = tag
-        cfg.z0 = 376.73
-        cfg.e_direction = np.array([0.0, 1.0, 0.0])
-        ports.append(em.build_lumped_port(pm, cfg))
-        oports.append(O.build_lumped_port(om, tag, 376.73, (0, 1, 0)))
+        if port_kind == "lumped":
+            cfg = em.LumpedPortConfig()
+            cfg.surface_tag = tag
+            cfg.z0 = 376.73
+            cfg.e_direction = np.array([0.0, 1.0, 0.0])
+            ports.append(em.build_lumped_port(pm, cfg))
+            oports.append(O.build_lumped_port(om, tag, 376.73, (0, 1, 0)))
+        else:
+            surf = em.extract_surface_mesh(pm, tag)
+            ports.append(em.build_wave_port(pm, surf, em.solve_port_eigens(surf.mesh, 1, 2 * math.pi * f, 1.0, 1.0,
+                                                                           em.ModePolarization.TE)[0]))
+            osurf = O.extract_surface_mesh(om, tag)
+            oports.append(O.build_wave_port(om, osurf, O.solve_port_eigens(osurf, 1, 2 * math.pi * f, 1.0, 1.0,
+                                                                           "TE")[0]))
     params = em.MaxwellParams()
     params.omega = 2 * math.pi * f
     params.eps_r_regions = {101: 3.5 * (1 - 0.002j)}
+    params.use_port_abc = port_kind == "wave"
     zc = g.centroids()[:, 2]
     H = float(g.xyz[:, 2].max())
     design = np.nonzero((zc > 0.15 * H) & (zc < 0.8 * H))[0]
@@ -889,7 +901,7 @@ def test_periodic_adjoint_gradient_vs_central_differences(torch_dev, mode):
     base = prob.permittivity(eps0)
 
     def oracle_loss(eps_full):
-        op = O.Params(omega=params.omega, eps_tet=eps_full)
+        op = O.Params(omega=params.omega, eps_tet=eps_full, use_port_abc=params.use_port_abc)
         So = O.calculate_sparams_periodic(om, op, pec, opbc, oports, mode=mode)
         return float((np.abs(So - T) ** 2).sum())
     assert abs(float(L.detach()) - oracle_loss(base)) <= 1e-8 * max(1.0, float(L.detach()))
