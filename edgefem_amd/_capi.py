@@ -37,7 +37,7 @@ EXPORTS = [
     "efm_gmsh_read", "efm_gmsh_sizes", "efm_gmsh_copy", "efm_gmsh_free",
     "efm_touchstone_write", "efm_touchstone_write_2port", "efm_touchstone_extension",
     "efm_port_overlap_noalloc", "efm_sweep_set_graph", "efm_sweep_set_correction",
-    "efm_sweep_set_correction_values", "efm_sweep_set_correction_terms",
+    "efm_sweep_set_correction_values", "efm_sweep_set_correction_terms", "efm_direct_solve",
     "efm_array_package_json", "efm_patterns_csv", "efm_coupling_csv", "efm_scan_csv",
 ]
 
@@ -124,6 +124,7 @@ def load_library(path: str = LIB_PATH) -> C.CDLL:
     lib.efm_sweep_set_correction.argtypes = [vp, i64, vp, vp, vp, vp]
     lib.efm_sweep_set_correction_values.argtypes = [vp, i32, i32, vp, vp, vp]
     lib.efm_sweep_set_correction_terms.argtypes = [vp, i32, vp, i32, vp, vp]
+    lib.efm_direct_solve.argtypes = [i64, vp, vp, vp, i32, i64, vp, vp, vp, vp, vp, i32, C.POINTER(C.c_int), vp]
     lib.efm_tet_fields.argtypes = [vp, vp, i32, vp, vp, vp, vp, vp]
     lib.efm_gmsh_read.argtypes = [C.c_char_p, i32, C.POINTER(vp)]
     lib.efm_gmsh_sizes.argtypes = [vp, vp]
@@ -827,3 +828,33 @@ class CellBatch:
             self.close()
         except Exception:
             pass
+
+
+def direct_solve(row_ptr, col_idx, values, B, nrhs: int, lowrank=None):
+    """Dense-LU direct solve on the GPU (efm_direct_solve).  lowrank: (gidx, gval, coefs) as
+    set_lowrank takes them, or None.  Returns (X, info): X a device complex128 tensor like B,
+    info > 0 for a singular matrix."""
+    torch = _torch()
+    lib = load_library()
+    rp = to_dev(row_ptr, torch.int32)
+    ci = to_dev(col_idx, torch.int32)
+    v = to_dev(values, torch.complex128)
+    Bd = to_dev(B, torch.complex128)
+    n = rp.numel() - 1
+    if Bd.numel() != n * nrhs:
+        raise EdgeFemError("right-hand side block has the wrong size")
+    X = torch.empty_like(Bd)
+    info = C.c_int()
+    if lowrank:
+        gi = to_dev(lowrank[0], torch.int32)
+        gv = to_dev(lowrank[1], torch.complex128)
+        cf = np.ascontiguousarray(np.asarray(lowrank[2], np.complex128))
+        P = cf.size
+        L = gi.numel() // max(P, 1)
+        _check(lib.efm_direct_solve(n, dptr(rp), dptr(ci), dptr(v), P, L, dptr(gi), dptr(gv),
+                                    cf.ctypes.data_as(C.c_void_p), dptr(Bd), dptr(X), int(nrhs), C.byref(info),
+                                    stream_ptr()), "efm_direct_solve")
+    else:
+        _check(lib.efm_direct_solve(n, dptr(rp), dptr(ci), dptr(v), 0, 0, None, None, None, dptr(Bd), dptr(X),
+                                    int(nrhs), C.byref(info), stream_ptr()), "efm_direct_solve")
+    return X, int(info.value)

@@ -60,7 +60,10 @@ def build(verbose: bool = False, jobs: int = 8) -> str:
     with cf.ThreadPoolExecutor(max_workers=max(1, min(jobs, len(srcs)))) as ex:
         objs = list(ex.map(compile_one, srcs))
     tmp = LIB + ".tmp"
-    cmd = [hipcc, "-shared", f"--offload-arch={ARCH}", "-fPIC", *objs, "-o", tmp]
+    rocm_lib = os.path.join(os.path.dirname(os.path.dirname(hipcc)), "lib")
+    # rocSOLVER / rocBLAS: the dense LU of the direct solve (efm_direct.hip)
+    cmd = [hipcc, "-shared", f"--offload-arch={ARCH}", "-fPIC", *objs, "-L" + rocm_lib, "-lrocsolver", "-lrocblas",
+           "-Wl,-rpath," + rocm_lib, "-o", tmp]
     r = subprocess.run(cmd, capture_output=True, text=True)
     if r.returncode != 0:
         raise RuntimeError(f"link failed:\n{r.stderr[-6000:]}")

@@ -232,6 +232,19 @@ int efm_port_overlap_noalloc(int64_t n, int n_ports, int64_t len, const int32_t 
                              const double *gval, const double *X, int nrhs, double *h_out,
                              void *stream);
 
+/* Direct solve A X = B by a dense LU on the GPU (rocSOLVER zgetrf/zgetrs): the SparseLU
+ * path of solve_linear (src/solver.cpp:11-33), taken when use_direct is set and as the
+ * fallback after a failed iterative solve (src/solver.cpp:104-115).  A: CSR (device
+ * row_ptr, col_idx, c128 values) plus n_lowrank terms c_p w_p w_p^H (lr_idx/lr_val device
+ * [n_lowrank][lowrank_len], index -1 = padding; h_lr_coef c128 host), as
+ * efm_solver_set_lowrank takes them.  B, X: device c128 [n][nrhs] (system-minor).
+ * n <= 40000 (the dense factor: 16 n^2 bytes).  h_info: 0, or k > 0 when U(k, k) is
+ * exactly zero (singular A; X is then not a solution). */
+int efm_direct_solve(int64_t n, const int32_t *row_ptr, const int32_t *col_idx, const double *values,
+                     int n_lowrank, int64_t lowrank_len, const int32_t *lr_idx, const double *lr_val,
+                     const double *h_lr_coef, const double *B, double *X, int nrhs, int *h_info,
+                     void *stream);
+
 /* ------------------------------------------------------------------------
  * Batched frequency sweep: up to 1024 systems A_s = K - k0sq[s] M + sum_p coef[s,p] M_s,p
  * on one pattern, solved together by COCG with the auxiliary-gradient preconditioner and

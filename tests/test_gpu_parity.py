@@ -643,6 +643,50 @@ def test_adjoint_gradient_complex_port_modes(torch_dev):
         assert abs(grad[k] - fd) <= 1e-5 * max(abs(fd), 1e-6 * np.abs(ref).max())
 
 
+def test_solve_linear_direct_and_fallback_vs_superlu(torch_dev):
+    """solve_linear's direct solver (use_direct, solver.cpp:11-33) and its auto-fallback after
+    a failed iterative solve (solver.cpp:104-115): a dense LU on the GPU, on the assembled
+    system of the short WR-90 guide with rank-1 lumped-port loading (non-symmetric Hermitian
+    terms), against SciPy's SuperLU of the same matrix (1e-10 relative); the fallback is
+    forced by an iteration cap of 3."""
+    import scipy.sparse.linalg as spla
+    from edgefem_amd import meshgen as mg
+    from edgefem_amd import pyedgefem as em
+    _p("[gpu] solve_linear direct solve and fallback")
+    g = mg.box_mesh(mg.WR90_A, mg.WR90_B, 30e-3, 6, 3, 12, seed=7)
+    pm = em.mesh_from_arrays(g.node_ids, g.xyz, g.tets, g.tet_phys, g.tris, g.tri_phys)
+    bc = em.build_edge_pec(pm, 1)
+    ports = []
+    for tag in (2, 3):
+        cfg = em.LumpedPortConfig()
+        cfg.surface_tag = tag
+        cfg.z0 = 50.0
+        cfg.e_direction = np.array([0.0, 1.0, 0.0])
+        ports.append(em.build_lumped_port(pm, cfg))
+    params = em.MaxwellParams()
+    params.omega = 2 * math.pi * 10e9
+    asm = em.assemble_maxwell(pm, params, bc, ports, 0)
+    A = asm.A
+    b = asm.b.to_numpy()
+    x_ref = spla.splu(A.to_scipy().tocsc()).solve(b)
+    opts = em.SolveOptions()
+    opts.use_direct = True
+    r = em.solve_linear(A, b, opts)
+    err = np.abs(r.x.to_numpy() - x_ref).max() / np.abs(x_ref).max()
+    _p(f"    direct: method {r.method} residual {r.residual:.2e} |dx|/|x| {err:.2e}")
+    assert r.method == "DenseLU" and r.converged and r.iters == 1
+    assert err <= 1e-10 and r.residual <= 1e-12
+    opts = em.SolveOptions()
+    opts.max_iterations = 3
+    r = em.solve_linear(A, b, opts)
+    err = np.abs(r.x.to_numpy() - x_ref).max() / np.abs(x_ref).max()
+    _p(f"    fallback: method {r.method} residual {r.residual:.2e} |dx|/|x| {err:.2e}")
+    assert r.method.endswith("->DenseLU") and r.converged and err <= 1e-10
+    opts.auto_fallback = False
+    r = em.solve_linear(A, b, opts)
+    assert not r.converged and "->" not in r.method
+
+
 def _lumped_pair(pm, om, pec_o, bc, z0=50.0):
     from edgefem_amd import pyedgefem as em
     from oracle import edgefem_oracle as O
