@@ -233,8 +233,9 @@ def test_c1_eigenmode_sparams_fp32_matrix(torch_dev):
     err = float(np.abs(S - ref["S"]).max())
     _p(f"    complex-FP32 matrix, FP64 vectors, no refinement: |dS| {err:.2e}, iterations {st['info']['iters']}")
     # the solve converges for the FP32-rounded matrix (relative perturbation ~6e-8 per
-    # entry), so S carries that perturbation; this storage mode is not a 1e-6 mode
-    assert err <= 1e-5
+    # entry), so S carries that perturbation: 6.2e-7 on this guide, within the 1e-6 bar here
+    # but set by the system's sensitivity, not by the solver
+    assert err <= 1e-6
 
 
 @pytest.mark.parametrize("f_ghz,s11,s21,ph", [(8.0, 0.003, 0.9997, 80.8), (10.0, 0.013, 0.9992, -100.7),
