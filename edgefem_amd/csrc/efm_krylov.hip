@@ -1486,22 +1486,42 @@ __global__ void __launch_bounds__(kBlock)
   write_partials<R>(pq, lds, partial);
 }
 
-// p = z + beta p
-template <int R>
+// p = z + beta p.  XLAG: also the previous iteration's x += alpha p, made here where p is
+// read anyway instead of in the update (which then reads neither x nor p): one vector
+// pass fewer per iteration.  The same cfma of the same operands, so x is bit-identical;
+// the last iteration's update is made by k_cocgz_xflush after the loop.
+template <int R, bool XLAG = false>
 __global__ void __launch_bounds__(kBlock)
     k_cocgz_pform(int64_t n, const double2 *__restrict__ zf, double2 *__restrict__ pf,
-                  const SysState *__restrict__ st) {
+                  const SysState *__restrict__ st, double2 *__restrict__ x = nullptr) {
   for (int64_t i = (int64_t)blockIdx.x * kBlock + threadIdx.x; i < n * R; i += (int64_t)gridDim.x * kBlock) {
     const int sys = (int)(i % R);
-    if (st[sys].active)
-      pf[i] = cfma(st[sys].beta, pf[i], zf[i]);
+    if (st[sys].active) {
+      const double2 po = pf[i];
+      if (XLAG && st[sys].iters > 0)
+        x[i] = cfma(st[sys].alpha, po, x[i]);
+      pf[i] = cfma(st[sys].beta, po, zf[i]);
+    }
   }
 }
 
-// x += alpha p, r -= alpha q, z = D^{-1} r (+ G y with the auxiliary preconditioner);
-// partials r^T z, r^H r.  One (row, system) element per thread, so every load and store
-// of a wavefront is one contiguous run (no partial lines).
-template <int R, bool AUX>
+// The pending x += alpha p of the last iteration of every system that made one (after a
+// breakdown in the alpha step alpha is zero).
+template <int R>
+__global__ void __launch_bounds__(kBlock)
+    k_cocgz_xflush(int64_t n, const double2 *__restrict__ pf, double2 *__restrict__ x,
+                   const SysState *__restrict__ st) {
+  for (int64_t i = (int64_t)blockIdx.x * kBlock + threadIdx.x; i < n * R; i += (int64_t)gridDim.x * kBlock) {
+    const int sys = (int)(i % R);
+    if (st[sys].iters > 0)
+      x[i] = cfma(st[sys].alpha, pf[i], x[i]);
+  }
+}
+
+// x += alpha p (not with XLAG: made by the next p pass), r -= alpha q, z = D^{-1} r (+ G y
+// with the auxiliary preconditioner); partials r^T z, r^H r.  One (row, system) element per
+// thread, so every load and store of a wavefront is one contiguous run (no partial lines).
+template <int R, bool AUX, bool XLAG = false>
 __global__ void __launch_bounds__(kBlock)
     k_cocgz_update(int64_t n, const int2 *__restrict__ en_eff, const double2 *__restrict__ dinv,
                    const double2 *__restrict__ y, const double2 *__restrict__ pf, const double2 *__restrict__ q,
@@ -1533,7 +1553,6 @@ __global__ void __launch_bounds__(kBlock)
     if (!a)
       continue;
     const double2 di = dinv[e];
-    const double2 xv = cfma(al, pf[k], x[k]);
     const double2 rv = csub(r[k], bil(al, q[k]));
     double2 zv = bil(di, rv);
     if (AUX) {
@@ -1541,7 +1560,8 @@ __global__ void __launch_bounds__(kBlock)
       if (ab.x >= 0)
         zv = cadd(zv, csub(y[(int64_t)ab.y * R + sys], y[(int64_t)ab.x * R + sys]));
     }
-    x[k] = xv;
+    if (!XLAG)
+      x[k] = cfma(al, pf[k], x[k]);
     r[k] = rv;
     zf[k] = zv;
     const double2 rz = bil(rv, zv);
@@ -2129,9 +2149,9 @@ template <int R> struct Dispatch {
     if (s->spmv_variant == kZForm || s->spmv_variant == kSellForm || s->spmv_variant == kSplitForm) {
       int nb_spmv = 0;
       int rc;
-      // p = z + beta p, then q = A p
-      hipLaunchKernelGGL(k_cocgz_pform<R>, dim3(s->grid), dim3(kBlock), 0, st, s->n, s->zf.as<double2>(),
-                         s->pf.as<double2>(), S);
+      // x += alpha p (the previous iteration's), p = z + beta p, then q = A p
+      hipLaunchKernelGGL((k_cocgz_pform<R, true>), dim3(s->grid), dim3(kBlock), 0, st, s->n, s->zf.as<double2>(),
+                         s->pf.as<double2>(), S, s->x.as<double2>());
       EFM_LAUNCH_CHECK();
       if (e0)
         EFM_HIP_CHECK(hipEventRecord(e0, st));
@@ -2146,12 +2166,12 @@ template <int R> struct Dispatch {
                            s->n_nodes, s->n2e_ptr, s->n2e, s->gsign.as<int8_t>(), s->dninv.as<double2>(),
                            s->r.as<double2>(), s->q.as<double2>(), s->ynode.as<double2>(), S);
         EFM_LAUNCH_CHECK();
-        hipLaunchKernelGGL((k_cocgz_update<R, true>), dim3(s->grid), dim3(kBlock), 0, st, s->n,
+        hipLaunchKernelGGL((k_cocgz_update<R, true, true>), dim3(s->grid), dim3(kBlock), 0, st, s->n,
                            s->en_eff.as<int2>(), s->dinv.as<double2>(), s->ynode.as<double2>(),
                            s->pf.as<double2>(), s->q.as<double2>(), s->zf.as<double2>(), s->x.as<double2>(),
                            s->r.as<double2>(), s->partial.as<double2>(), S);
       } else {
-        hipLaunchKernelGGL((k_cocgz_update<R, false>), dim3(s->grid), dim3(kBlock), 0, st, s->n,
+        hipLaunchKernelGGL((k_cocgz_update<R, false, true>), dim3(s->grid), dim3(kBlock), 0, st, s->n,
                            (const int2 *)nullptr, s->dinv.as<double2>(), (const double2 *)nullptr,
                            s->pf.as<double2>(), s->q.as<double2>(), s->zf.as<double2>(), s->x.as<double2>(),
                            s->r.as<double2>(), s->partial.as<double2>(), S);
@@ -2769,6 +2789,18 @@ extern "C" int efm_solver_solve(efm_solver *s, const double *B, double *X,
         s->spmv_launches += 1;
       }
     }
+  }
+  if (cocg && (s->spmv_variant == kZForm || s->spmv_variant == kSellForm || s->spmv_variant == kSplitForm)) {
+    // the last iteration's x updates (the p pass makes them one iteration late)
+    rc = with_rhs(s->R, [&](auto RC) -> int {
+      constexpr int R = decltype(RC)::value;
+      hipLaunchKernelGGL(k_cocgz_xflush<R>, dim3(s->grid), dim3(kBlock), 0, st, s->n, s->pf.as<double2>(),
+                         s->x.as<double2>(), s->state.as<SysState>());
+      EFM_LAUNCH_CHECK();
+      return EFM_OK;
+    });
+    if (rc != EFM_OK)
+      return rc;
   }
   EFM_HIP_CHECK(hipMemcpyAsync(X, s->x.p, (size_t)s->n * s->R * sizeof(double2),
                                hipMemcpyDeviceToDevice, st));

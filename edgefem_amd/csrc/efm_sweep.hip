@@ -408,31 +408,57 @@ __global__ void __launch_bounds__(kBlock)
 // its partial u^T r, and the p pass rebuilds it from r and the nodal y (unchanged between
 // the two): p = D^{-1} r + G y + beta p.  One vector pass fewer per iteration (the u
 // store) for a gather of y (node-sized, read in 64-system rows) and the edge diagonal.
-template <typename VT>
+// XLAG: the previous iteration's x += alpha p is made here, where p is read anyway, instead
+// of in the update (which then reads neither x nor p): one vector pass fewer again.  The
+// product is the same cfma of the same operands, so x is bit-identical; the last
+// iteration's update of every system is made by k_sw_xflush after the loop.
+template <typename VT, bool XLAG>
 __global__ void __launch_bounds__(kBlock)
     k_sw_pform_u(int64_t n, int B, int P, const int2 *__restrict__ en_eff, const double2 *__restrict__ dK,
                  const double2 *__restrict__ dM, const double *__restrict__ dMs, const int8_t *__restrict__ rport,
                  const double *__restrict__ k0sq, const double2 *__restrict__ coef, const VT *__restrict__ y,
-                 const VT *__restrict__ r, VT *__restrict__ p, const SwState *__restrict__ st) {
+                 const VT *__restrict__ r, VT *__restrict__ p, VT *__restrict__ x, const SwState *__restrict__ st) {
   const int s = blockIdx.y * kSys + (threadIdx.x & 63);
   const int wv = threadIdx.x >> 6;
   if (s >= B || st[s].active == 0)
     return;
   const double2 beta = st[s].beta;
   const double k2 = k0sq[s];
+  const bool xu = XLAG && st[s].iters > 0;  // an iteration done: its x update is pending
+  const double2 alpha = st[s].alpha;
   for (int64_t e = (int64_t)blockIdx.x * kWaves + wv; e < n; e += (int64_t)gridDim.x * kWaves) {
     const int64_t i = e * B + s;
     const int2 ab = en_eff[e];
     double2 z = bil(edge_diag_inv(e, s, P, k2, dK, dM, dMs, rport, coef), ldv(r + i));
     if (ab.x >= 0)
       z = cadd(z, csub(ldv(y + (int64_t)ab.y * B + s), ldv(y + (int64_t)ab.x * B + s)));
-    stv(p + i, cfma(beta, ldv(p + i), z));
+    const double2 po = ldv(p + i);
+    if (xu)
+      stv(x + i, cfma(alpha, po, ldv(x + i)));
+    stv(p + i, cfma(beta, po, z));
   }
 }
 
-// x += alpha p; r -= alpha w; u = D^{-1} r + G y (stored when STORE_U); partials u^T r,
-// |r|^2.  Persistent: one block slice walks the groups of 64 systems in turn.
-template <typename VT, bool STORE_U>
+// The pending x += alpha p of the last iteration of every system that made one (not after
+// a breakdown in the alpha step, which stops a system before its update).
+template <typename VT>
+__global__ void __launch_bounds__(kBlock)
+    k_sw_xflush(int64_t n, int B, const VT *__restrict__ p, VT *__restrict__ x, const SwState *__restrict__ st) {
+  const int s = blockIdx.y * kSys + (threadIdx.x & 63);
+  const int wv = threadIdx.x >> 6;
+  if (s >= B || st[s].iters == 0 || st[s].breakdown != 0)
+    return;
+  const double2 alpha = st[s].alpha;
+  for (int64_t e = (int64_t)blockIdx.x * kWaves + wv; e < n; e += (int64_t)gridDim.x * kWaves) {
+    const int64_t i = e * B + s;
+    stv(x + i, cfma(alpha, ldv(p + i), ldv(x + i)));
+  }
+}
+
+// x += alpha p (not with XLAG: made by the next p pass); r -= alpha w; u = D^{-1} r + G y
+// (stored when STORE_U); partials u^T r, |r|^2.  Persistent: one block slice walks the
+// groups of 64 systems in turn.
+template <typename VT, bool STORE_U, bool XLAG = false>
 __global__ void __launch_bounds__(kBlock)
     k_sw_update_std(int64_t n, int B, int groups, int P, const int2 *__restrict__ en_eff,
                     const double2 *__restrict__ dK, const double2 *__restrict__ dM,
@@ -452,7 +478,8 @@ __global__ void __launch_bounds__(kBlock)
       const double2 alpha = st[s].alpha;
       for (int64_t e = (int64_t)blockIdx.x * kWaves + wv; e < n; e += (int64_t)gridDim.x * kWaves) {
         const int64_t i = e * B + s;
-        stv(x + i, cfma(alpha, ldv(p + i), ldv(x + i)));
+        if (!XLAG)
+          stv(x + i, cfma(alpha, ldv(p + i), ldv(x + i)));
         const double2 rv = csub(ldv(r + i), bil(alpha, ldv(w + i)));
         stv(r + i, rv);
         const double2 di = edge_diag_inv(e, s, P, k2, dK, dM, dMs, rport, coef);
@@ -2417,6 +2444,9 @@ struct SwRun {
   // recurrence is not restarted.  A system converges only on its true residual.
   template <typename VT> int cocg_std(const double2 *b, double inner_tol2, int max_inner, double ru_delta2 = 0.0) {
     const int64_t n = s->n;
+    // form 2 without reliable updates (which read x during the loop): x updated one
+    // iteration late, in the p pass
+    const bool xlag = s->form == 2 && !(ru_delta2 > 0.0);
     VT *x = s->x.as<VT>(), *r = s->r.as<VT>(), *p = s->p.as<VT>();
     VT *sv = s->sv.as<VT>(), *u = s->u.as<VT>(), *w = s->w.as<VT>(), *y = s->y.as<VT>();
     constexpr int vti = std::is_same<VT, float2>::value ? 1 : 0;
@@ -2507,9 +2537,12 @@ struct SwRun {
       }
       for (int c = 0; c < chunk; ++c, ++it) {
         const int va = va_of(it), vs = vs_of(it);
-        if (s->form == 2)
-          hipLaunchKernelGGL((k_sw_pform_u<VT>), dim3(s->gx_vec, groups), dim3(kBlock), 0, st, n, Bp, P, en, dK, dM,
-                             dMs, rp, k2, cf, (const VT *)y, (const VT *)r, p, S);
+        if (s->form == 2 && xlag)
+          hipLaunchKernelGGL((k_sw_pform_u<VT, true>), dim3(s->gx_vec, groups), dim3(kBlock), 0, st, n, Bp, P, en,
+                             dK, dM, dMs, rp, k2, cf, (const VT *)y, (const VT *)r, p, x, S);
+        else if (s->form == 2)
+          hipLaunchKernelGGL((k_sw_pform_u<VT, false>), dim3(s->gx_vec, groups), dim3(kBlock), 0, st, n, Bp, P, en,
+                             dK, dM, dMs, rp, k2, cf, (const VT *)y, (const VT *)r, p, x, S);
         else
           hipLaunchKernelGGL((k_sw_pform<VT>), dim3(s->gx_vec, groups), dim3(kBlock), 0, st, n, Bp, (const VT *)u,
                              p, S);
@@ -2519,7 +2552,8 @@ struct SwRun {
             (rc = run(log_a, s->aux_variant < 0, va,
                       [&]() { return aux_launch<VT, 2>(va, r, w, (const VT *)nullptr, y); })))
           return rc;
-        auto upd = s->form == 2 ? k_sw_update_std<VT, false> : k_sw_update_std<VT, true>;
+        auto upd = s->form != 2 ? k_sw_update_std<VT, true, false>
+                                : xlag ? k_sw_update_std<VT, false, true> : k_sw_update_std<VT, false, false>;
         hipLaunchKernelGGL(upd, dim3(s->gx_std[vti], 1), dim3(kBlock), 0, st, n, Bp, groups, P, en, dK, dM, dMs,
                            rp, k2, cf, (const VT *)y, (const VT *)w, (const VT *)p, u, x, r, pu, S);
         EFM_LAUNCH_CHECK();
@@ -2541,6 +2575,11 @@ struct SwRun {
         return rc;
       if (want > 0 && ((rc = replace<VT>(b, 0, va_of(it), ru_delta2)) || (rc = n_active(active, want))))
         return rc;
+    }
+    if (xlag) {  // the last iteration's x updates
+      hipLaunchKernelGGL((k_sw_xflush<VT>), dim3(s->gx_vec, groups), dim3(kBlock), 0, st, n, Bp, (const VT *)p, x,
+                         S);
+      EFM_LAUNCH_CHECK();
     }
     if (ru_delta2 > 0.0)
       return replace<VT>(b, 1, va_of(it), ru_delta2);

@@ -1219,7 +1219,9 @@ def test_dispersive_and_pml_eigenmode_sweep_vs_oracle(torch_dev):
                                        _state={"per_frequency": True})
     S10 = em.calculate_sparams_eigenmode(pm, em._params_at(params, 2 * math.pi * 10e9), bc, ports,
                                          solver_options=opts)
-    assert np.abs(S10 - Spf[1]).max() == 0.0
+    # the per-frequency path is calculate_sparams_eigenmode; each solver picks its SpMV
+    # variant by timing, so two solves of one system may sum in different orders
+    assert np.abs(S10 - Spf[1]).max() <= 1e-12
     dpf = np.abs(S - Spf).max()
     _p(f"    batched vs per-frequency |dS| {dpf:.2e} (correction as {st['correction_form']})")
     assert dpf <= 1e-8
