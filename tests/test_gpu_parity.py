@@ -409,6 +409,62 @@ def test_solver_every_spmv_variant_converges_to_same_solution(torch_dev, nrhs):
             assert float((X - ref).abs().max() / ref.abs().max()) <= 1e-7
 
 
+@pytest.mark.parametrize("cap", [1, 2, 37])
+def test_x_update_in_p_pass_at_iteration_cap(torch_dev, cap):
+    """The COCG forms that make x += alpha p one iteration late, in the p pass (the
+    single-system variant 9 and the batched sweep's form 2), stopped at an iteration cap
+    before convergence: the returned x carries every iteration's update exactly once (the
+    last one made after the loop), so it matches the forms that update x in the update
+    pass (variant 6; sweep form 0) and its true residual |b - A x| / |b| equals the
+    recursive one the solver reports.  A missing or repeated update would move x by
+    alpha p, far outside these bounds."""
+    import torch
+    from edgefem_amd import _capi
+    from edgefem_amd import pyedgefem as em
+    _p(f"[gpu] x update in the p pass, iteration cap {cap}")
+    pm, bc, ports = _c1_em_setup()
+    params = em.MaxwellParams()
+    params.omega = 2 * math.pi * 10e9
+    st = {}
+    em.calculate_sparams_eigenmode(pm, params, bc, ports, _state=st)
+    A = st["A"]
+    n = pm.num_edges()
+    gen = torch.Generator(device="cpu").manual_seed(5)
+    B = torch.complex(torch.randn(n, 2, generator=gen, dtype=torch.float64),
+                      torch.randn(n, 2, generator=gen, dtype=torch.float64)).to("cuda")
+    B[torch.from_numpy(em._dirichlet_mask(pm, bc).astype(bool)).to("cuda")] = 0
+    X = {}
+    for v in (9, 6):
+        solver = _capi.KrylovSolver(pm.topology.row_ptr, pm.topology.col_idx, nrhs=2)
+        solver.set_gradient(pm.topology, em._dirichlet_mask(pm, bc))
+        solver.set_matrix(A)
+        solver.set_spmv_variant(v)
+        Xv, info = solver.solve(B.reshape(-1).clone(), tol=1e-14, max_iterations=cap)
+        Xv = Xv.reshape(n, 2)
+        R = B.reshape(-1) - solver.spmv(Xv.reshape(-1))
+        true = (torch.linalg.vector_norm(R.reshape(n, 2), dim=0) / torch.linalg.vector_norm(B, dim=0)).cpu().numpy()
+        _p(f"    variant {v}: iterations {list(info['iters'])} true residual {true} recursive {info['resid']}")
+        assert list(info["iters"]) == [cap, cap]
+        assert np.allclose(true, info["resid"], rtol=1e-6, atol=0)
+        X[v] = Xv
+    assert float((X[9] - X[6]).abs().max() / X[6].abs().max()) <= 1e-10
+    # the batched sweep, 24 frequencies x 2 excitations, form 2 against form 0
+    freqs = np.linspace(8.5e9, 12e9, 24)
+    opts = em.SolveOptions()
+    opts.tolerance = 1e-14
+    opts.max_iterations = cap
+    Xs = {}
+    for form in (2, 0):
+        sst = {"spmv_variant": 2, "aux_variant": 1, "iteration": form, "keep_solutions": True}
+        em.frequency_sweep_eigenmode(pm, em.MaxwellParams(), bc, ports, freqs, solver_options=opts, _state=sst)
+        its = np.concatenate([i["iters"] for i in sst["infos"]])
+        assert np.all(its == cap)
+        Xs[form] = torch.cat([sol["X"].reshape(n, -1) for sol in sst["solutions"]], dim=1)
+    d = float((Xs[2] - Xs[0]).abs().max() / Xs[0].abs().max())
+    _p(f"    sweep form 2 vs form 0 at {cap} iterations: max |dx| / max |x| {d:.2e}")
+    assert d <= 1e-10
+
+
 def test_full_size_c2_topology_properties(torch_dev):
     """Config-2 mesh at full size (1,327,104 tets): exact counts, symmetric pattern,
     valid signs, every tet edge in its own row."""
