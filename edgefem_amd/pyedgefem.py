@@ -2398,7 +2398,8 @@ def frequency_sweep_eigenmode(mesh: Mesh, params: MaxwellParams, bc: BC, ports: 
         _state.update(dict(infos=infos, sweep=sweep, ops=ops, spmv_chosen=v, aux_chosen=a, tune_ms=tune_ms,
                            correction_entries=0 if corr is None else corr.nnz,
                            correction_form=None if corr is None else
-                           ("terms" if corr.terms is not None and not _state.get("correction_columns") else "columns")))
+                           ("terms" if corr.terms is not None and not _state.get("correction_columns") else "columns"),
+                           correction_terms=0 if corr is None or corr.terms is None else 2 * corr.group_rep.size))
     return out
 
 

@@ -1358,6 +1358,55 @@ def test_batched_media_sweep_many_systems_vs_per_frequency(torch_dev):
         assert np.abs(S[i] - Sref).max() <= 1e-6
 
 
+def test_media_correction_with_tiled_and_pipelined_spmv(torch_dev):
+    """The shared-term media correction (two slabs, so four terms: a two-pole Lorentz slab
+    and a scalar-PML slab, 24 frequencies x 2 excitations) after the LDS-tiled SpMV and
+    after the pipelined SpMV (1e-10 between the two), both against the per-frequency
+    assembly and solve (1e-8); the graph replay of the tiled form against its plain
+    launches (bit-identical S)."""
+    from edgefem_amd import meshgen as mg
+    from edgefem_amd import pyedgefem as em
+    from edgefem_amd import materials as M
+    _p("[gpu] media correction after the tiled and the pipelined SpMV")
+    L = 30e-3
+    g = mg.box_mesh(mg.WR90_A, mg.WR90_B, L, 6, 3, 12, seed=13)
+    zc = g.centroids()[:, 2]
+    phys = g.tet_phys.copy()
+    phys[(zc > 0.25 * L) & (zc < 0.45 * L)] = 121
+    phys[(zc > 0.7 * L) & (zc < 0.85 * L)] = 122
+    pm = em.mesh_from_arrays(g.node_ids, g.xyz, g.tets, phys, g.tris, g.tri_phys)
+    bc = em.build_edge_pec(pm, 1)
+    mode = em.solve_te10_mode(em.RectWaveguidePort(mg.WR90_A, mg.WR90_B), 10e9)
+    kc2 = (math.pi / mg.WR90_A) ** 2
+    ports = [em.build_wave_port_2d(pm, t, mode, bc.dirichlet_edges, kc2) for t in (2, 3)]
+    w0 = 2 * math.pi * 10e9
+    lor = M.LorentzMaterial(2.5)
+    lor.add_pole(1.2, 1.4 * w0, 0.08 * w0)
+    lor.add_pole(0.4, 0.7 * w0, 0.05 * w0)
+    params = em.MaxwellParams()
+    params.set_eps_model(121, lor)
+    params.pml_sigma, params.pml_regions = 0.3 * w0, {122}
+    freqs = np.linspace(8.5e9, 12e9, 24)
+    opts = em.SolveOptions()
+    opts.tolerance = 1e-10
+    S, st = {}, {}
+    for key, sv, graph in (("tiled", 2, False), ("tiled-graph", 2, True), ("pipelined", 1, False)):
+        st[key] = {"spmv_variant": sv, "aux_variant": 1, "graph": graph}
+        S[key] = em.frequency_sweep_eigenmode(pm, params, bc, ports, freqs, solver_options=opts, _state=st[key])
+        assert st[key]["correction_form"] == "terms" and st[key]["correction_terms"] == 4
+        assert st[key]["spmv_chosen"] == sv
+    Spf = em.frequency_sweep_eigenmode(pm, params, bc, ports, freqs, solver_options=opts,
+                                       _state={"per_frequency": True})
+    d_kernel = np.abs(S["tiled"] - S["pipelined"]).max()
+    d_pf = max(np.abs(S[k] - Spf).max() for k in S)
+    it = np.concatenate([i["iters"] for i in st["tiled"]["infos"]])
+    _p(f"    tiled vs pipelined |dS| {d_kernel:.2e}; vs per-frequency {d_pf:.2e}; iterations "
+       f"{it.min()}-{it.max()}")
+    assert d_kernel <= 1e-10
+    assert d_pf <= 1e-8
+    assert np.array_equal(S["tiled"].view(np.float64), S["tiled-graph"].view(np.float64))
+
+
 def test_batched_sweep_fp32_inner_refined_vs_oracle(torch_dev):
     """precision="fp32": the batched COCG iterates on complex-FP32 vectors with FP64
     reliable updates (FP64 residual b - A x, FP64 solution and reductions), on the short
