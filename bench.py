@@ -852,27 +852,47 @@ def main():
         # gradient (grads/s); compact summaries of their own bench lines
         del pipe
         torch.cuda.empty_cache()
+
+        def extra(key, fn):
+            # a failing summary is reported in its own key; the C2 line above still prints
+            try:
+                result[key] = fn()
+            except Exception as e:  # noqa: BLE001 - reported in the line, traceback on stderr
+                import traceback
+                traceback.print_exc()
+                result[key] = {"error": f"{type(e).__name__}: {e}"}
+            torch.cuda.empty_cache()
+
+        def sweep_summary(prec):
+            sw = run_sweep(args, 2, 1, prec, cpu=(prec == "fp64"))
+            out = {k: sw[k] for k in ("value", "unit", "ms_per_step", "speedup_vs_plain", "plain_path_solves_per_s",
+                                      "delta_S_vs_fp64_lu", "iterations", "max_true_rel_residual", "dtype",
+                                      "cpu_baseline")}
+            out["workload"] = sw["config"]["workload"]
+            return out
+
+        def adjoint_summary():
+            ad = run_adjoint(args, 1, 0)
+            out = {k: ad[k] for k in ("value", "unit", "ms_per_step", "forward_ms", "gradient_kernel_ms",
+                                      "grad_over_forward", "iterations")}
+            out["workload"] = ad["config"]["workload"]
+            out["design_tets"] = ad["config"]["design_tets"]
+            return out
+
+        def media_summary():
+            md = run_media(args, 1, 1)
+            out = {k: md[k] for k in ("value", "unit", "ms_per_step", "per_frequency_solves_per_s",
+                                      "speedup_vs_per_frequency", "delta_S_vs_per_frequency", "iterations")}
+            out["workload"] = md["config"]["workload"]
+            return out
+
         log("extra: batched sweep (fp64 and fp32 vectors)")
         for prec in ("fp64", "fp32"):
-            sw = run_sweep(args, 2, 1, prec, cpu=(prec == "fp64"))
-            result[f"sweep_{prec}"] = {k: sw[k] for k in (
-                "value", "unit", "ms_per_step", "speedup_vs_plain", "plain_path_solves_per_s", "delta_S_vs_fp64_lu",
-                "iterations", "max_true_rel_residual", "dtype", "cpu_baseline")}
-            result[f"sweep_{prec}"]["workload"] = sw["config"]["workload"]
-        torch.cuda.empty_cache()
+            extra(f"sweep_{prec}", lambda prec=prec: sweep_summary(prec))
         log("extra: adjoint gradient")
-        ad = run_adjoint(args, 1, 0)
-        result["adjoint"] = {k: ad[k] for k in ("value", "unit", "ms_per_step", "forward_ms", "gradient_kernel_ms",
-                                                "grad_over_forward", "iterations")}
-        result["adjoint"]["workload"] = ad["config"]["workload"]
-        result["adjoint"]["design_tets"] = ad["config"]["design_tets"]
-        torch.cuda.empty_cache()
+        extra("adjoint", adjoint_summary)
         log("extra: batched sweep with frequency-dependent media")
-        md = run_media(args, 1, 1)
-        result["media_sweep"] = {k: md[k] for k in ("value", "unit", "ms_per_step", "per_frequency_solves_per_s",
-                                                    "speedup_vs_per_frequency", "delta_S_vs_per_frequency",
-                                                    "iterations")}
-        result["media_sweep"]["workload"] = md["config"]["workload"]
+        extra("media_sweep", media_summary)
     print(json.dumps(result))
 
 
