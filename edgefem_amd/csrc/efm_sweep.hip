@@ -1226,7 +1226,7 @@ __global__ void k_sw_imag_flags(int64_t nnz, const double2 *__restrict__ K, cons
 // kMaxTerms shared matrices cterm[entry][kMaxTerms] (unused terms zero): the terms of an
 // entry are one wave-uniform 128-byte read, the per-system values are formed in registers.
 constexpr int kMaxTerms = 8;
-template <typename VT, typename IT, int MODE, bool TERMS>
+template <typename VT, typename IT, int MODE, bool TERMS, int NT = kMaxTerms>
 __global__ void __launch_bounds__(kBlock)
     k_sw_corr(int64_t nrows, int B, int nsys, const int32_t *__restrict__ crow,
               const int32_t *__restrict__ cptr, const int32_t *__restrict__ ccol,
@@ -1244,15 +1244,15 @@ __global__ void __launch_bounds__(kBlock)
   // the value column of this system; padding systems read column 0 and are masked out
   const int cs = (!TERMS && s < nsys) ? colmap[s] : 0;
   const double z = s < nsys ? 1.0 : 0.0;
-  double2 al[TERMS ? kMaxTerms : 1];
+  double2 al[TERMS ? NT : 1];
 #pragma unroll
-  for (int q = 0; q < (TERMS ? kMaxTerms : 1); ++q)
+  for (int q = 0; q < (TERMS ? NT : 1); ++q)
     al[q] = (TERMS && s < nsys) ? alpha[(int64_t)s * kMaxTerms + q] : make_double2(0, 0);
   auto value = [&](int32_t k) -> double2 {
-    if constexpr (TERMS) {
+    if constexpr (TERMS) {  // the first NT terms (columns past the terms in use are zero)
       double2 a = make_double2(0, 0);
 #pragma unroll
-      for (int q = 0; q < kMaxTerms; ++q)
+      for (int q = 0; q < NT; ++q)
         a = cfma(al[q], cterm[(int64_t)k * kMaxTerms + q], a);
       return a;
     } else {
@@ -1789,6 +1789,7 @@ struct efm_sweep {
   int64_t c_rows = 0, c_nnz = 0;
   int c_nsys = 0, c_ncols = 0, gx_corr = 8, ncu = 256;
   int c_terms = 0;  // 1: c_vals holds shared terms [entry][8] and c_alpha [system][8] their weights
+  int c_nterms = kMaxTerms;  // the shared terms in use (columns past them are zero)
   const double2 *c_vals = nullptr;
   efm::DevBuf c_row, c_ptr, c_col, c_map, part_c, tcorr, c_colmap, c_alpha;
   ~efm_sweep() {
@@ -2069,6 +2070,7 @@ extern "C" int efm_sweep_set_correction_terms(efm_sweep *s, int nterms, const do
   s->c_nsys = nsys;
   s->c_ncols = kMaxTerms;
   s->c_terms = 1;
+  s->c_nterms = nterms;
   return EFM_OK;
 }
 
@@ -2253,7 +2255,9 @@ struct SwRun {
                          s->rport.as<int8_t>(), s->k0sq.as<double>(), s->coef.as<double2>(), u, w, ps, S);
     EFM_LAUNCH_CHECK();
     if (s->c_rows > 0) {
-      auto kc = s->c_terms ? k_sw_corr<VT, VT, 0, true> : k_sw_corr<VT, VT, 0, false>;
+      auto kc = !s->c_terms           ? k_sw_corr<VT, VT, 0, false>
+                : s->c_nterms <= 4 ? k_sw_corr<VT, VT, 0, true, 4>
+                                   : k_sw_corr<VT, VT, 0, true>;
       hipLaunchKernelGGL(kc, dim3(s->gx_corr, groups), dim3(kBlock), 0, st, s->c_rows, Bp, s->c_nsys,
                          s->c_row.as<int32_t>(), s->c_ptr.as<int32_t>(), s->c_col.as<int32_t>(), s->c_vals,
                          s->c_ncols, s->c_colmap.as<int32_t>(), s->c_vals, s->c_alpha.as<double2>(), u, w,
@@ -2403,7 +2407,9 @@ struct SwRun {
     EFM_LAUNCH_CHECK();
     const bool corr = s->c_rows > 0;
     if (corr) {
-      auto kc = s->c_terms ? k_sw_corr<VT, double2, 1, true> : k_sw_corr<VT, double2, 1, false>;
+      auto kc = !s->c_terms           ? k_sw_corr<VT, double2, 1, false>
+                : s->c_nterms <= 4 ? k_sw_corr<VT, double2, 1, true, 4>
+                                   : k_sw_corr<VT, double2, 1, true>;
       hipLaunchKernelGGL(kc, dim3(s->gx_corr, groups), dim3(kBlock), 0, st, s->c_rows, Bp, s->c_nsys,
                          s->c_row.as<int32_t>(), s->c_ptr.as<int32_t>(), s->c_col.as<int32_t>(), s->c_vals,
                          s->c_ncols, s->c_colmap.as<int32_t>(), s->c_vals, s->c_alpha.as<double2>(),
