@@ -339,7 +339,8 @@ int efm_sweep_set_correction_values(efm_sweep *sw, int nsys, int ncols, const do
  * d_terms c128 [entry][8] in caller device memory (columns past nterms zero), h_alpha c128
  * [nsys][nterms] on the host: where the media vary by region (a dispersive model per tag, a
  * scalar PML stretch per region) every system's correction is a combination of each
- * region's K and M parts, and the values read per entry drop from one per frequency to 8. */
+ * region's K and M parts, and the values read per entry drop from one per frequency to the
+ * terms (four when nterms <= 4, else all eight columns). */
 int efm_sweep_set_correction_terms(efm_sweep *sw, int nterms, const double *d_terms, int nsys,
                                    const double *h_alpha, void *stream);
 
