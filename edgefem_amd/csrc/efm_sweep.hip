@@ -174,24 +174,24 @@ __global__ void __launch_bounds__(kBlock)
         gj = gsign[base + lane];
       }
       for (int j = 0; j < len; j += U) {
+        // the U gathers of a step are unconditional (one group of loads): past the list's
+        // end they repeat its last edge with weight zero
         double2 rv[U];
 #pragma unroll
-        for (int t = 0; t < U; ++t)
-          if (j + t < len) {
-            const int64_t i = (int64_t)bcast(ej, j + t) * B + s;
-            rv[t] = ldv(r + i);
-            if (NV >= 2)
-              rv[t] = csub(rv[t], bil(alpha, ldv(w + i)));
-            if (NV == 3)
-              rv[t] = csub(rv[t], bil(abp, ldv(sv + i)));
-          }
+        for (int t = 0; t < U; ++t) {
+          const int64_t i = (int64_t)bcast(ej, min(j + t, len - 1)) * B + s;
+          rv[t] = ldv(r + i);
+          if (NV >= 2)
+            rv[t] = csub(rv[t], bil(alpha, ldv(w + i)));
+          if (NV == 3)
+            rv[t] = csub(rv[t], bil(abp, ldv(sv + i)));
+        }
 #pragma unroll
-        for (int t = 0; t < U; ++t)
-          if (j + t < len) {
-            const double gs = (double)bcast(gj, j + t);
-            acc.x += gs * rv[t].x;
-            acc.y += gs * rv[t].y;
-          }
+        for (int t = 0; t < U; ++t) {
+          const double gs = j + t < len ? (double)bcast(gj, min(j + t, len - 1)) : 0.0;
+          acc.x += gs * rv[t].x;
+          acc.y += gs * rv[t].y;
+        }
       }
     }
     if (act) {
