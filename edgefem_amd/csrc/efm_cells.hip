@@ -303,14 +303,28 @@ __global__ void __launch_bounds__(kBlock)
 #pragma unroll
     for (int s = 0; s < R; ++s)
       acc[s] = make_double2(0, 0);
-    if (nd < t.r1)
-      for (int32_t q = gt_ptr[nd] + l; q < gt_ptr[nd + 1]; q += kNodeLanes) {
-        const int64_t e = gt_edge[q];
+    if (nd < t.r1) {
+      // two of the lane's incident edges per step, both gathers issued together (past the
+      // list's end the second repeats the last edge with a zero coefficient)
+      const int32_t qe = gt_ptr[nd + 1];
+      for (int32_t q = gt_ptr[nd] + l; q < qe; q += 2 * kNodeLanes) {
+        const int32_t q2 = min(q + kNodeLanes, qe - 1);
+        const int64_t e = gt_edge[q], e2 = gt_edge[q2];
         const double2 gc = gt_coef[q];
+        const double2 gc2 = q + kNodeLanes < qe ? gt_coef[q2] : make_double2(0, 0);
+        double2 v[R], v2[R];
 #pragma unroll
-        for (int s = 0; s < R; ++s)
-          acc[s] = cfma(gc, in[e * R + s], acc[s]);
+        for (int s = 0; s < R; ++s) {
+          v[s] = in[e * R + s];
+          v2[s] = in[e2 * R + s];
+        }
+#pragma unroll
+        for (int s = 0; s < R; ++s) {
+          acc[s] = cfma(gc, v[s], acc[s]);
+          acc[s] = cfma(gc2, v2[s], acc[s]);
+        }
       }
+    }
 #pragma unroll
     for (int s = 0; s < R; ++s)
 #pragma unroll
