@@ -368,10 +368,12 @@ __global__ void __launch_bounds__(kBlock)
       continue;
     const int2 nn = en2[e];
     double2 v = cmul(dinv[e], in[k]);
-    if (nn.x >= 0)
-      v = cfma(ga[e], y[(int64_t)nn.x * R + sys], v);
-    if (nn.y >= 0)
-      v = cfma(gb[e], y[(int64_t)nn.y * R + sys], v);
+    // both node values loaded unconditionally (node 0 where an edge has no node), so they
+    // issue with the element's other loads; each is added only where the node exists
+    const double2 ya = y[(int64_t)(nn.x >= 0 ? nn.x : 0) * R + sys];
+    const double2 yb = y[(int64_t)(nn.y >= 0 ? nn.y : 0) * R + sys];
+    v = nn.x >= 0 ? cfma(ga[e], ya, v) : v;
+    v = nn.y >= 0 ? cfma(gb[e], yb, v) : v;
     out[k] = v;
   }
 }

@@ -1557,8 +1557,11 @@ __global__ void __launch_bounds__(kBlock)
     double2 zv = bil(di, rv);
     if (AUX) {
       const int2 ab = en_eff[e];
-      if (ab.x >= 0)
-        zv = cadd(zv, csub(y[(int64_t)ab.y * R + sys], y[(int64_t)ab.x * R + sys]));
+      // both node values loaded unconditionally (node 0 for an edge without nodes), so they
+      // issue with the row's other loads; the sum is taken only for edges with nodes
+      const bool hn = ab.x >= 0;
+      const double2 yd = csub(y[(int64_t)(hn ? ab.y : 0) * R + sys], y[(int64_t)(hn ? ab.x : 0) * R + sys]);
+      zv = hn ? cadd(zv, yd) : zv;
     }
     if (!XLAG)
       x[k] = cfma(al, pf[k], x[k]);

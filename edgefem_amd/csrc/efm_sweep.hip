@@ -367,8 +367,12 @@ __global__ void __launch_bounds__(kBlock)
       const double2 di = edge_diag_inv(e, s, P, k2, dK, dM, dMs, rport, coef);
       double2 z = bil(di, rv);
       const int2 ab = en_eff[e];
-      if (ab.x >= 0)
-        z = cadd(z, csub(ldv(y + (int64_t)ab.y * B + s), ldv(y + (int64_t)ab.x * B + s)));
+      {  // both node values loaded unconditionally (node 0 for an edge without nodes)
+        const bool hn = ab.x >= 0;
+        const double2 yd =
+            csub(ldv(y + (int64_t)(hn ? ab.y : 0) * B + s), ldv(y + (int64_t)(hn ? ab.x : 0) * B + s));
+        z = hn ? cadd(z, yd) : z;
+      }
       stv(u + i, z);
       v[0] = cadd(v[0], bil(rv, z));
       v[1].x += cabs2(rv);
@@ -430,8 +434,11 @@ __global__ void __launch_bounds__(kBlock)
     const int64_t i = e * B + s;
     const int2 ab = en_eff[e];
     double2 z = bil(edge_diag_inv(e, s, P, k2, dK, dM, dMs, rport, coef), ldv(r + i));
-    if (ab.x >= 0)
-      z = cadd(z, csub(ldv(y + (int64_t)ab.y * B + s), ldv(y + (int64_t)ab.x * B + s)));
+    {  // both node values loaded unconditionally (node 0 for an edge without nodes)
+      const bool hn = ab.x >= 0;
+      const double2 yd = csub(ldv(y + (int64_t)(hn ? ab.y : 0) * B + s), ldv(y + (int64_t)(hn ? ab.x : 0) * B + s));
+      z = hn ? cadd(z, yd) : z;
+    }
     const double2 po = ldv(p + i);
     if (xu)
       stv(x + i, cfma(alpha, po, ldv(x + i)));
@@ -485,8 +492,12 @@ __global__ void __launch_bounds__(kBlock)
         const double2 di = edge_diag_inv(e, s, P, k2, dK, dM, dMs, rport, coef);
         double2 z = bil(di, rv);
         const int2 ab = en_eff[e];
-        if (ab.x >= 0)
-          z = cadd(z, csub(ldv(y + (int64_t)ab.y * B + s), ldv(y + (int64_t)ab.x * B + s)));
+        {  // both node values loaded unconditionally (node 0 for an edge without nodes)
+          const bool hn = ab.x >= 0;
+          const double2 yd =
+              csub(ldv(y + (int64_t)(hn ? ab.y : 0) * B + s), ldv(y + (int64_t)(hn ? ab.x : 0) * B + s));
+          z = hn ? cadd(z, yd) : z;
+        }
         if (STORE_U)
           stv(u + i, z);
         v[0] = cadd(v[0], bil(rv, z));
