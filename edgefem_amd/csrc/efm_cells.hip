@@ -487,6 +487,114 @@ __global__ void __launch_bounds__(kBlock)
   put_partials<V>(d, lds, partial, blockIdx.x);
 }
 
+// The same product on entry tiles (the single-system SpMV's scheme): the block of row tile t
+// walks its sub-tiles sdesc[soff[t] .. soff[t+1]) of at most kSubNnz entries and kSubRows
+// rows; each thread streams 4 entries of a sub-tile (branch-free loads, the next sub-tile's
+// in flight during the gathers), the products go to LDS and kSubRL lanes per row sum them.
+constexpr int kSubNnz = 1024;
+constexpr int kSubRL = 4;
+constexpr int kSubRows = kBlock / kSubRL;
+
+template <int R, int MODE>
+__global__ void __launch_bounds__(kBlock)
+    k_cells_spmv_t(const Tile *__restrict__ tiles, const int32_t *__restrict__ soff, const int4 *__restrict__ sdesc,
+                   const int32_t *__restrict__ row_ptr, const int32_t *__restrict__ col,
+                   const double2 *__restrict__ val, const double2 *__restrict__ in, double2 *__restrict__ out,
+                   const double2 *__restrict__ other, double2 *__restrict__ partial, const CellSys *__restrict__ st) {
+  constexpr int V = MODE == 0 ? R : 2 * R;
+  constexpr int J = kSubNnz / kBlock;
+  __shared__ double2 prod[R * kSubNnz];
+  __shared__ double2 lds[(kBlock / 64) * 2 * kMaxR];
+  const Tile t = tiles[blockIdx.x];
+  if (!cell_active<R>(st, t.cell))  // the finalisation ignores the partials of inactive systems
+    return;
+  bool act[R];
+#pragma unroll
+  for (int s = 0; s < R; ++s)
+    act[s] = st[(int64_t)t.cell * R + s].active != 0;
+  double2 d[V];
+#pragma unroll
+  for (int s = 0; s < V; ++s)
+    d[s] = make_double2(0, 0);
+  const int g = threadIdx.x / kSubRL, l = threadIdx.x % kSubRL;
+  struct Stage {
+    int4 dsc;
+    double2 a[J];
+    int32_t c[J];
+    int32_t k0, k1;
+  };
+  auto stage = [&](Stage &x, int j) {
+    x.dsc = sdesc[j];
+    const int32_t kz = x.dsc.z < x.dsc.w ? x.dsc.z : 0;  // a valid entry either way
+#pragma unroll
+    for (int q = 0; q < J; ++q) {
+      const int32_t k = x.dsc.z + threadIdx.x + q * kBlock;
+      const bool inr = k < x.dsc.w;
+      const int32_t kk = inr ? k : kz;
+      const double2 av = ld_stream(val + kk);
+      x.c[q] = ld_stream(col + kk);
+      x.a[q].x = inr ? av.x : 0.0;
+      x.a[q].y = inr ? av.y : 0.0;
+    }
+    const int32_t row = x.dsc.x + g;
+    const int32_t rc = row < x.dsc.y ? row : x.dsc.y - 1;
+    x.k0 = row_ptr[rc] - x.dsc.z;
+    x.k1 = row < x.dsc.y ? row_ptr[rc + 1] - x.dsc.z : x.k0;
+  };
+  const int j0 = soff[blockIdx.x], j1 = soff[blockIdx.x + 1];
+  Stage cur, nxt;
+  if (j0 < j1)  // every tile holds a row, so at least one sub-tile
+    stage(cur, j0);
+  for (int j = j0; j < j1; ++j) {
+    double2 zg[J][R];
+#pragma unroll
+    for (int q = 0; q < J; ++q)
+#pragma unroll
+      for (int s = 0; s < R; ++s)
+        zg[q][s] = in[(int64_t)cur.c[q] * R + s];
+    stage(nxt, min(j + 1, j1 - 1));
+#pragma unroll
+    for (int q = 0; q < J; ++q) {
+      const int lk = threadIdx.x + q * kBlock;
+#pragma unroll
+      for (int s = 0; s < R; ++s)
+        prod[s * kSubNnz + lk] = cmul(cur.a[q], zg[q][s]);
+    }
+    __syncthreads();
+    double2 acc[R];
+#pragma unroll
+    for (int s = 0; s < R; ++s)
+      acc[s] = make_double2(0, 0);
+    for (int32_t k = cur.k0 + l; k < cur.k1; k += kSubRL)
+#pragma unroll
+      for (int s = 0; s < R; ++s)
+        acc[s] = cadd(acc[s], prod[s * kSubNnz + k]);
+#pragma unroll
+    for (int s = 0; s < R; ++s)
+      group_sum<kSubRL>(acc[s]);
+    const int32_t row = cur.dsc.x + g;
+    if (row < cur.dsc.y && l == 0) {
+#pragma unroll
+      for (int s = 0; s < R; ++s) {
+        if (!act[s])
+          continue;
+        const int64_t k = (int64_t)row * R + s;
+        out[k] = acc[s];
+        const double2 o = other[k];
+        if (MODE == 0) {
+          d[s] = cfma(cconj(o), acc[s], d[s]);
+        } else {
+          d[s] = cfma(cconj(acc[s]), o, d[s]);
+          d[R + s].x += cabs2(acc[s]);
+        }
+      }
+    }
+    __syncthreads();  // prod is reused by the next sub-tile
+    cur = nxt;
+  }
+  put_partials<V>(d, lds, partial, blockIdx.x);
+}
+
 // Port terms, one block per term q (cell pcell[q], excitation-independent):
 //   t_s = w_q^H in_s;  out_s += c_q w_q t_s on the term's edges; the change of the SpMV's
 // dot products (MODE 0: r0^H dv; MODE 1: dt^H s and |t|^2) into the term's partial slot.
@@ -783,6 +891,8 @@ struct efm_cells {
   efm::DevBuf en2, ga, gb, gt_ptr, gt_edge, gt_coef, dninv, ynode;
   efm::DevBuf poff, pidx, pw, pcoef, pcell;
   efm::DevBuf x, r, r0, p, v, y, zz, t, partial, state, flags;
+  efm::DevBuf soff, sdesc;  // entry sub-tiles of the row tiles (nsub = 0: not built)
+  int64_t nsub = 0;
 };
 
 namespace {
@@ -869,6 +979,35 @@ extern "C" int efm_cells_create(int64_t ncells, const int64_t *h_row_off, const 
     EFM_HIP_CHECK(hipMemcpyAsync(&last, s->row_ptr.as<int32_t>() + n, sizeof(int32_t), hipMemcpyDeviceToHost, st));
     EFM_HIP_CHECK(hipStreamSynchronize(st));
     EFM_REQUIRE(last == nnz, "efm_cells_create: row_ptr[n] != nnz");
+  }
+  {  // entry sub-tiles of every row tile for the SpMV: <= kSubNnz entries, <= kSubRows rows
+    std::vector<int32_t> hrp(n + 1);
+    EFM_HIP_CHECK(hipMemcpy(hrp.data(), s->row_ptr.p, (n + 1) * sizeof(int32_t), hipMemcpyDeviceToHost));
+    std::vector<int4> sd;
+    std::vector<int32_t> so(tiles.size() + 1, 0);
+    bool ok = true;
+    for (size_t ti = 0; ti < tiles.size() && ok; ++ti) {
+      so[ti] = (int32_t)sd.size();
+      for (int64_t r = tiles[ti].r0; r < tiles[ti].r1;) {
+        int64_t r1 = r;
+        while (r1 < tiles[ti].r1 && r1 - r < kSubRows && hrp[r1 + 1] - hrp[r] <= kSubNnz)
+          ++r1;
+        if (r1 == r) {  // a row longer than a sub-tile: keep the row-parallel kernel
+          ok = false;
+          break;
+        }
+        sd.push_back(make_int4((int32_t)r, (int32_t)r1, hrp[r], hrp[r1]));
+        r = r1;
+      }
+    }
+    so[tiles.size()] = (int32_t)sd.size();
+    s->nsub = 0;
+    if (ok && !sd.empty()) {
+      if ((rc = upload(s->sdesc, sd.data(), sd.size(), st)) || (rc = upload(s->soff, so.data(), so.size(), st)))
+        return rc;
+      EFM_HIP_CHECK(hipStreamSynchronize(st));  // sd and so are freed at the end of this block
+      s->nsub = (int64_t)sd.size();
+    }
   }
   // gradient incidence (global node indices) and its transpose, on the device
   {
@@ -1048,8 +1187,13 @@ template <int R> int cells_solve(efm_cells *s, const double2 *B, double2 *X, con
       EFM_LAUNCH_CHECK();
       if ((rc = precond(p, y)))
         return rc;
-      hipLaunchKernelGGL((k_cells_spmv<R, 0>), dim3(nt), dim3(kBlock), 0, st, T, s->row_ptr.as<int32_t>(),
-                         s->col.as<int32_t>(), s->val.as<double2>(), y, v, r0, part, S);
+      if (s->nsub > 0)
+        hipLaunchKernelGGL((k_cells_spmv_t<R, 0>), dim3(nt), dim3(kBlock), 0, st, T, s->soff.as<int32_t>(),
+                           s->sdesc.as<int4>(), s->row_ptr.as<int32_t>(), s->col.as<int32_t>(),
+                           s->val.as<double2>(), y, v, r0, part, S);
+      else
+        hipLaunchKernelGGL((k_cells_spmv<R, 0>), dim3(nt), dim3(kBlock), 0, st, T, s->row_ptr.as<int32_t>(),
+                           s->col.as<int32_t>(), s->val.as<double2>(), y, v, r0, part, S);
       EFM_LAUNCH_CHECK();
       if ((rc = ports(I0{}, y, v, r0)) || (rc = fin(std::integral_constant<int, kAlpha>{})))
         return rc;
@@ -1057,8 +1201,13 @@ template <int R> int cells_solve(efm_cells *s, const double2 *B, double2 *X, con
       EFM_LAUNCH_CHECK();
       if ((rc = precond(r, zz)))
         return rc;
-      hipLaunchKernelGGL((k_cells_spmv<R, 1>), dim3(nt), dim3(kBlock), 0, st, T, s->row_ptr.as<int32_t>(),
-                         s->col.as<int32_t>(), s->val.as<double2>(), zz, t, r, part, S);
+      if (s->nsub > 0)
+        hipLaunchKernelGGL((k_cells_spmv_t<R, 1>), dim3(nt), dim3(kBlock), 0, st, T, s->soff.as<int32_t>(),
+                           s->sdesc.as<int4>(), s->row_ptr.as<int32_t>(), s->col.as<int32_t>(),
+                           s->val.as<double2>(), zz, t, r, part, S);
+      else
+        hipLaunchKernelGGL((k_cells_spmv<R, 1>), dim3(nt), dim3(kBlock), 0, st, T, s->row_ptr.as<int32_t>(),
+                           s->col.as<int32_t>(), s->val.as<double2>(), zz, t, r, part, S);
       EFM_LAUNCH_CHECK();
       if ((rc = ports(I1{}, zz, t, r)) || (rc = fin(std::integral_constant<int, kOmega>{})))
         return rc;
