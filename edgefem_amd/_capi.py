@@ -32,7 +32,7 @@ EXPORTS = [
     "efm_port_overlap", "efm_solver_set_spmv_variant", "efm_solver_get_spmv_variant",
     "efm_sweep_create", "efm_sweep_destroy", "efm_sweep_solve", "efm_eps_gradient",
     "efm_solver_set_lowrank", "efm_add_diagonal", "efm_periodic_reduce", "efm_solver_set_gradient_complex",
-    "efm_cells_create", "efm_cells_destroy", "efm_cells_solve",
+    "efm_cells_create", "efm_cells_destroy", "efm_cells_solve", "efm_cells_set_spmv_variant",
     "efm_sweep_set_variants", "efm_sweep_get_variants", "efm_sweep_set_iteration", "efm_tet_fields",
     "efm_gmsh_read", "efm_gmsh_sizes", "efm_gmsh_copy", "efm_gmsh_free",
     "efm_touchstone_write", "efm_touchstone_write_2port", "efm_touchstone_extension",
@@ -113,6 +113,7 @@ def load_library(path: str = LIB_PATH) -> C.CDLL:
                                      C.POINTER(vp)]
     lib.efm_cells_destroy.argtypes = [vp]
     lib.efm_cells_destroy.restype = None
+    lib.efm_cells_set_spmv_variant.argtypes = [vp, i32]
     lib.efm_cells_solve.argtypes = [vp, vp, vp, C.POINTER(SolveOpts), vp, vp, vp]
     lib.efm_periodic_reduce.argtypes = [C.c_int64, vp, vp, vp, C.c_int64, vp, vp, vp, vp, vp, vp, vp, vp,
                                         C.c_int64, C.POINTER(C.c_int64), vp]
@@ -817,6 +818,11 @@ class CellBatch:
         _check(self._lib.efm_cells_solve(self._h, dptr(Bd), dptr(X), C.byref(opts), iters.ctypes.data_as(C.c_void_p),
                                          resid.ctypes.data_as(C.c_void_p), stream_ptr()), "efm_cells_solve")
         return X, iters.reshape(self.ncells, self.nrhs), resid.reshape(self.ncells, self.nrhs)
+
+    def set_spmv_variant(self, variant):
+        """1: entry sub-tiles (default), 0: rows over 8 lanes (the only one when a row
+        holds more than 1024 entries)."""
+        _check(self._lib.efm_cells_set_spmv_variant(self._h, int(variant)), "efm_cells_set_spmv_variant")
 
     def close(self):
         if getattr(self, "_h", None):

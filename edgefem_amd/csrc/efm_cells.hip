@@ -893,6 +893,7 @@ struct efm_cells {
   efm::DevBuf x, r, r0, p, v, y, zz, t, partial, state, flags;
   efm::DevBuf soff, sdesc;  // entry sub-tiles of the row tiles (nsub = 0: not built)
   int64_t nsub = 0;
+  int spmv = 0;  // 1: k_cells_spmv_t on the sub-tiles, 0: row-parallel k_cells_spmv
 };
 
 namespace {
@@ -1007,6 +1008,7 @@ extern "C" int efm_cells_create(int64_t ncells, const int64_t *h_row_off, const 
         return rc;
       EFM_HIP_CHECK(hipStreamSynchronize(st));  // sd and so are freed at the end of this block
       s->nsub = (int64_t)sd.size();
+      s->spmv = 1;
     }
   }
   // gradient incidence (global node indices) and its transpose, on the device
@@ -1110,6 +1112,15 @@ extern "C" int efm_cells_create(int64_t ncells, const int64_t *h_row_off, const 
 
 extern "C" void efm_cells_destroy(efm_cells *s) { delete s; }
 
+extern "C" int efm_cells_set_spmv_variant(efm_cells *s, int variant) {
+  EFM_REQUIRE(s, "efm_cells_set_spmv_variant: cells is NULL");
+  EFM_REQUIRE(variant == 0 || variant == 1, "efm_cells_set_spmv_variant: variant must be 0 or 1");
+  EFM_REQUIRE(variant == 0 || s->nsub > 0,
+              "efm_cells_set_spmv_variant: no entry sub-tiles (a row holds more than 1024 entries)");
+  s->spmv = variant;
+  return EFM_OK;
+}
+
 namespace {
 template <int R> int cells_solve(efm_cells *s, const double2 *B, double2 *X, const efm_solve_opts *o,
                                  int32_t *h_iters, double *h_resid, hipStream_t st) {
@@ -1187,7 +1198,7 @@ template <int R> int cells_solve(efm_cells *s, const double2 *B, double2 *X, con
       EFM_LAUNCH_CHECK();
       if ((rc = precond(p, y)))
         return rc;
-      if (s->nsub > 0)
+      if (s->spmv == 1)
         hipLaunchKernelGGL((k_cells_spmv_t<R, 0>), dim3(nt), dim3(kBlock), 0, st, T, s->soff.as<int32_t>(),
                            s->sdesc.as<int4>(), s->row_ptr.as<int32_t>(), s->col.as<int32_t>(),
                            s->val.as<double2>(), y, v, r0, part, S);
@@ -1201,7 +1212,7 @@ template <int R> int cells_solve(efm_cells *s, const double2 *B, double2 *X, con
       EFM_LAUNCH_CHECK();
       if ((rc = precond(r, zz)))
         return rc;
-      if (s->nsub > 0)
+      if (s->spmv == 1)
         hipLaunchKernelGGL((k_cells_spmv_t<R, 1>), dim3(nt), dim3(kBlock), 0, st, T, s->soff.as<int32_t>(),
                            s->sdesc.as<int4>(), s->row_ptr.as<int32_t>(), s->col.as<int32_t>(),
                            s->val.as<double2>(), zz, t, r, part, S);

@@ -386,6 +386,11 @@ int efm_cells_create(int64_t ncells, const int64_t *h_row_off, const int32_t *ro
                      const int64_t *h_port_off, const int32_t *h_port_idx, const double *h_port_w,
                      const double *h_port_coef, void *stream, efm_cells **out);
 void efm_cells_destroy(efm_cells *cells);
+/* SpMV kernel of the batch: 1 = entry sub-tiles of at most 1024 entries and 64 rows (the
+ * default), 0 = rows over 8 lanes (the default when a row holds more than 1024 entries;
+ * asking for 1 then fails with EFM_ERR_ARG).  Both give the same products up to the
+ * summation order.  No reference counterpart. */
+int efm_cells_set_spmv_variant(efm_cells *cells, int variant);
 /* B, X: [n][nrhs] c128 (device); h_iters / h_resid: [ncells*nrhs] (may be NULL);
  * h_opts->method must be EFM_METHOD_BICGSTAB. */
 int efm_cells_solve(efm_cells *cells, const double *B, double *X, const efm_solve_opts *h_opts,

@@ -1121,6 +1121,98 @@ def test_periodic_cell_batch_vs_single_and_oracle(torch_dev):
         assert np.abs(S - Ss).max() <= 1e-6
 
 
+def _ragged_cells(sizes, long_row=None, nrhs=2, seed=0):
+    """Synthetic block-diagonal batch for efm_cells_*: per cell a diagonally dominant
+    complex matrix with 4-24 off-diagonal entries per row (long_row = (cell, row, count):
+    one row with `count` entries), an edge-node incidence with coefficients -1 / +1 and
+    three port entries per (cell, excitation).  Returns the CellBatch inputs, B and the
+    dense solve of every cell's operator A_c + sum_q c_q w_q w_q^H."""
+    rng = np.random.default_rng(seed)
+    row_off = np.concatenate([[0], np.cumsum(sizes)]).astype(np.int64)
+    nodes = [max(4, n // 6) for n in sizes]
+    node_off = np.concatenate([[0], np.cumsum(nodes)]).astype(np.int64)
+    rp, cols, vals, en, ec = [0], [], [], [], []
+    dense = []
+    for c, nc in enumerate(sizes):
+        Ad = np.zeros((nc, nc), complex)
+        for i in range(nc):
+            k = int(rng.integers(4, 25))
+            if long_row is not None and long_row[0] == c and long_row[1] == i:
+                k = long_row[2]
+            off = rng.choice(np.delete(np.arange(nc), i), size=k, replace=False)
+            v = (rng.standard_normal(k) + 1j * rng.standard_normal(k)) * 0.3 / np.sqrt(k)
+            Ad[i, off] = v
+            Ad[i, i] = 1.0 + np.abs(v).sum() + 0.2j
+            cc = np.sort(np.append(off, i))
+            cols.append(cc + row_off[c])
+            vals.append(Ad[i, cc])
+            rp.append(rp[-1] + cc.size)
+            a, b = rng.choice(nodes[c], size=2, replace=False) + node_off[c]
+            en.append((a, b))
+            ec.append((-1.0, 1.0))
+        dense.append(Ad)
+    port_off, port_idx, port_w, port_coef = [0], [], [], []
+    for c, nc in enumerate(sizes):
+        rows = rng.choice(nc, size=3 * nrhs, replace=False) + row_off[c]
+        for p in range(nrhs):
+            port_idx.extend(rows[3 * p:3 * p + 3])
+            port_w.extend(rng.standard_normal(3) + 1j * rng.standard_normal(3))
+            port_off.append(len(port_idx))
+            port_coef.append(0.05 + 0.02j)
+    port_idx, port_w = np.array(port_idx, np.int32), np.array(port_w, complex)
+    n = int(row_off[-1])
+    B = rng.standard_normal((n, nrhs)) + 1j * rng.standard_normal((n, nrhs))
+    Xref = np.zeros_like(B)
+    for c, Ad in enumerate(dense):
+        A = Ad.copy()
+        for p in range(nrhs):
+            q = c * nrhs + p
+            idx = port_idx[port_off[q]:port_off[q + 1]] - row_off[c]
+            w = np.zeros(A.shape[0], complex)
+            w[idx] = port_w[port_off[q]:port_off[q + 1]]
+            A += port_coef[q] * np.outer(w, w.conj())
+        Xref[row_off[c]:row_off[c + 1]] = np.linalg.solve(A, B[row_off[c]:row_off[c + 1]])
+    inputs = (row_off, np.array(rp, np.int32), np.concatenate(cols).astype(np.int32), np.concatenate(vals),
+              node_off, np.array(en, np.int32), np.array(ec, complex), nrhs, np.array(port_off, np.int64),
+              port_idx, port_w, np.array(port_coef, complex))
+    return inputs, B, Xref
+
+
+def test_cells_spmv_variants_on_ragged_rows_vs_dense_solve(torch_dev):
+    """Both SpMV kernels of the cell batch (entry sub-tiles, and rows over 8 lanes) on
+    ragged synthetic cells of 700, 1500 and 333 rows (row tiles of 1024 rows split inside
+    a cell) against a dense LU solve of every cell; a cell with a row of 1100 entries
+    leaves only the row-parallel kernel, which must then be the default."""
+    from edgefem_amd import _capi
+    _p("[gpu] cell batch SpMV variants on ragged rows")
+    for long_row in (None, (1, 5, 1100)):
+        inputs, B, Xref = _ragged_cells([700, 1500, 333], long_row=long_row, seed=3)
+        batch = _capi.CellBatch(*inputs)
+        variants = (1, 0) if long_row is None else (0,)
+        if long_row is not None:
+            with pytest.raises(_capi.EdgeFemError):
+                batch.set_spmv_variant(1)
+        Xs = {}
+        for v in variants:
+            batch.set_spmv_variant(v)
+            X, iters, resid = batch.solve(torch_dev.from_numpy(B).cuda(), tol=1e-12)
+            X = X.cpu().numpy().reshape(B.shape)
+            err = np.abs(X - Xref).max() / np.abs(Xref).max()
+            _p(f"    long_row={long_row} variant {v}: iterations {iters.min()}-{iters.max()}, "
+               f"max residual {resid.max():.2e}, |X - X_LU| / |X_LU| {err:.2e}")
+            assert resid.max() <= 1e-12
+            assert err <= 1e-9
+            Xs[v] = X
+        if long_row is None:
+            assert np.abs(Xs[1] - Xs[0]).max() / np.abs(Xref).max() <= 1e-9
+        if long_row is not None:  # the default after create is the row-parallel kernel
+            batch2 = _capi.CellBatch(*inputs)
+            X2, _, _ = batch2.solve(torch_dev.from_numpy(B).cuda(), tol=1e-12)
+            assert np.array_equal(X2.cpu().numpy().reshape(B.shape), Xs[0])
+            batch2.close()
+        batch.close()
+
+
 def test_tet_fields_vs_oracle(torch_dev):
     """efm_tet_fields on the C1 guide with 2 RHS: E at the centroid and at a random point
     of every tet and curl E against the oracle (1e-12 of the largest value), and an affine
